@@ -1,0 +1,182 @@
+"""Python side of the HIP engine (include/fastpart.h): one Engine = one
+fp_ctx = one hypergraph and its connectivity state resident on one GPU.
+
+The methods mirror the reference's flow around partition_gpu_wrapper
+(TV/partition.cpp:984-1184): upload both CSR views once, hand over the state
+left by the host-side initial partition, run REFI_ITER refinement passes on
+the device, read the refined state and metrics back.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+
+import numpy as np
+
+from . import _native
+from .hypergraph import Hypergraph
+
+_U32P = ctypes.POINTER(ctypes.c_uint32)
+_I32P = ctypes.POINTER(ctypes.c_int32)
+_I64P = ctypes.POINTER(ctypes.c_int64)
+
+METRICS = {"CN": 0, "TV": 1, "TM": 2, "MSM": 3, "MSV": 4, "MSRV": 5}
+SCHED_REF, SCHED_BAL = 0, 1
+
+
+class Schedule(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("round", ctypes.c_int), ("ldg_C", ctypes.c_double),
+                ("max_pw", ctypes.c_longlong), ("min_pw", ctypes.c_longlong), ("cap", ctypes.c_longlong)]
+
+
+class RefineStats(ctypes.Structure):
+    _fields_ = [("moves", ctypes.c_longlong), ("rounds", ctypes.c_longlong),
+                ("pin_visits", ctypes.c_longlong), ("device_ms", ctypes.c_double)]
+
+
+class Metrics(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_longlong) for k in ("tv", "cn", "tm", "msm", "msv", "msrv")]
+
+
+def declare(lib) -> None:
+    c = ctypes
+    P = c.c_void_p
+    lib.fp_ctx_last_error.restype = c.c_char_p
+    lib.fp_ctx_last_error.argtypes = [P]
+    lib.fp_ctx_device_bytes.restype = c.c_longlong
+    lib.fp_ctx_device_bytes.argtypes = [P]
+    lib.fp_ctx_create.argtypes = [c.POINTER(P), c.c_int]
+    lib.fp_ctx_destroy.argtypes = [P]
+    lib.fp_ctx_destroy.restype = None
+    lib.fp_device_count.argtypes = [c.POINTER(c.c_int)]
+    lib.fp_upload.argtypes = [P, c.c_int, c.c_int, _U32P, _U32P, _U32P, _U32P, _I32P, _I32P, _I32P]
+    lib.fp_set_partition.argtypes = [P, c.c_int, _I32P]
+    lib.fp_set_state.argtypes = [P, c.c_int, _I32P, _I32P, _I32P, _I32P, _I64P]
+    lib.fp_get_state.argtypes = [P, _I32P, _I32P, _I32P, _I32P, _I64P]
+    lib.fp_refine.argtypes = [P, c.c_int, c.c_int, c.POINTER(Schedule), c.POINTER(RefineStats)]
+    lib.fp_eval_moves.argtypes = [P, c.c_int, c.POINTER(Schedule), _I32P, _I32P]
+    lib.fp_eval.argtypes = [P, c.POINTER(Metrics), _I32P, _I64P, _I64P, _I64P, _I64P, _I64P]
+    lib.fp_set_kernel_timing.argtypes = [P, c.c_int]
+    lib.fp_last_kernel_timing.argtypes = [P, c.POINTER(c.c_longlong), c.POINTER(c.c_double)]
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    _native.engine().fp_device_count(ctypes.byref(n))
+    return n.value
+
+
+def schedule(kind: int, round_size: int, C: float, max_pw: int, min_pw: int, cap: int = 0) -> Schedule:
+    return Schedule(kind, round_size, C, max_pw, min_pw, cap)
+
+
+@dataclasses.dataclass
+class State:
+    part: np.ndarray
+    ep_ends: np.ndarray
+    ep_parts: np.ndarray
+    ep_cnt: np.ndarray
+    sizes: np.ndarray
+
+
+class Engine:
+    def __init__(self, device: int = 0):
+        self.lib = _native.engine()
+        self.ctx = ctypes.c_void_p()
+        rc = self.lib.fp_ctx_create(ctypes.byref(self.ctx), device)
+        _native.check(rc, "fp_ctx_create")
+        self.hg: Hypergraph | None = None
+        self.K = 0
+
+    def close(self):
+        if self.ctx:
+            self.lib.fp_ctx_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            detail = self.lib.fp_ctx_last_error(self.ctx)
+            _native.check(rc, what, detail.decode() if detail else "")
+
+    def device_bytes(self) -> int:
+        return int(self.lib.fp_ctx_device_bytes(self.ctx))
+
+    def upload(self, hg: Hypergraph, ordering=None, with_vertex_view: bool = True):
+        self.hg = hg
+        ordp = None if ordering is None else np.ascontiguousarray(ordering, dtype=np.int32)
+        self._check(self.lib.fp_upload(self.ctx, hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.net_pins, _U32P),
+                                       _p(hg.vtx_ptr, _U32P) if with_vertex_view else None,
+                                       _p(hg.vtx_nets, _U32P) if with_vertex_view else None,
+                                       _p(hg.src, _I32P), _p(hg.vwgt, _I32P), _p(ordp, _I32P)), "fp_upload")
+
+    def set_partition(self, K: int, part):
+        self.K = K
+        part = np.ascontiguousarray(part, dtype=np.int32)
+        self._check(self.lib.fp_set_partition(self.ctx, K, _p(part, _I32P)), "fp_set_partition")
+
+    def set_state(self, K: int, part, ep_ends, ep_parts, ep_cnt, sizes=None):
+        self.K = K
+        arrs = [np.ascontiguousarray(x, dtype=np.int32) for x in (part, ep_ends, ep_parts, ep_cnt)]
+        sz = None if sizes is None else np.ascontiguousarray(sizes, dtype=np.int64)
+        self._check(self.lib.fp_set_state(self.ctx, K, *(_p(x, _I32P) for x in arrs), _p(sz, _I64P)),
+                    "fp_set_state")
+
+    def get_state(self) -> State:
+        hg = self.hg
+        st = State(np.zeros(hg.nov, np.int32), np.zeros(hg.noe, np.int32), np.zeros(hg.pins, np.int32),
+                   np.zeros(hg.pins, np.int32), np.zeros(self.K, np.int64))
+        self._check(self.lib.fp_get_state(self.ctx, _p(st.part, _I32P), _p(st.ep_ends, _I32P),
+                                          _p(st.ep_parts, _I32P), _p(st.ep_cnt, _I32P), _p(st.sizes, _I64P)),
+                    "fp_get_state")
+        return st
+
+    def refine(self, metric: str, passes: int, sched: Schedule) -> RefineStats:
+        stats = RefineStats()
+        self._check(self.lib.fp_refine(self.ctx, METRICS[metric], passes, ctypes.byref(sched),
+                                       ctypes.byref(stats)), "fp_refine")
+        return stats
+
+    def eval_moves(self, metric: str, sched: Schedule):
+        dec = np.zeros(self.hg.nov, np.int32)
+        gain = np.zeros(self.hg.nov, np.int32)
+        self._check(self.lib.fp_eval_moves(self.ctx, METRICS[metric], ctypes.byref(sched), _p(dec, _I32P),
+                                           _p(gain, _I32P)), "fp_eval_moves")
+        return dec, gain
+
+    def metrics(self, vectors: bool = False):
+        m = Metrics()
+        K = self.K
+        lam = np.zeros(self.hg.noe, np.int32) if vectors else None
+        vec = {k: np.zeros(K, np.int64) for k in ("sv", "rv", "sm", "rm", "sv_msv")} if vectors else {}
+        self._check(self.lib.fp_eval(self.ctx, ctypes.byref(m), _p(lam, _I32P), *(
+            _p(vec.get(k), _I64P) for k in ("sv", "rv", "sm", "rm", "sv_msv"))), "fp_eval")
+        res = {"TV": m.tv, "CN": m.cn, "TM": m.tm, "MSM": m.msm, "MSV": m.msv, "MSRV": m.msrv}
+        if vectors:
+            res["lam"] = lam
+            res.update(vec)
+        return res
+
+    def set_kernel_timing(self, on: bool):
+        self._check(self.lib.fp_set_kernel_timing(self.ctx, int(on)), "fp_set_kernel_timing")
+
+    def kernel_timing(self):
+        n, ms = ctypes.c_longlong(), ctypes.c_double()
+        self._check(self.lib.fp_last_kernel_timing(self.ctx, ctypes.byref(n), ctypes.byref(ms)),
+                    "fp_last_kernel_timing")
+        return n.value, ms.value

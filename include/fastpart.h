@@ -1,0 +1,169 @@
+/* fastpart.h -- C-ABI of the FastPartitioner refinement and metric engine for
+ * MI355X (gfx950), libfastpart.so.
+ *
+ * The engine keeps one hypergraph per context resident in HBM (both CSR
+ * views, net sources, vertex weights) together with its connectivity state
+ * (part vector, per-net lists of (part, pin count) and their lengths lambda,
+ * part weights) and runs refinement passes and metric evaluation on it.
+ *
+ * It replaces the device side of the reference's per-metric
+ * partition_gpu_wrapper(...) entry points (TV/partition.cpp:6-26 declared,
+ * TV/gpu_partition.cu:332-553 defined; CN/partition.cpp:7-27; TM/partition.cpp:3-7;
+ * MGV/partition.cpp:4-25; MSRV/partition.cpp:4-24). A caller that already holds
+ * the reference's host arrays maps them as:
+ *   row_ptr/col_ind (vertex->nets)      -> fp_upload vtx_ptr/vtx_nets
+ *   the reader's net->pins CSR          -> fp_upload net_ptr/net_pins
+ *   src_nodes                           -> fp_upload src
+ *   part_vec, ep_ends, ep_parts, ep_cnt -> fp_set_state / fp_get_state
+ *   ep_starts                           == net_ptr (the list of net n starts at net_ptr[n])
+ *   REFI_ITER, C                        -> fp_refine passes, fp_schedule.ldg_C
+ *
+ * Conventions: every entry point returns fp_status (0 = FP_OK) and never
+ * exits the process; fp_ctx_last_error() explains the last failure. Arrays are
+ * host pointers to plain int/unsigned/long long data; the engine copies them.
+ * One context = one HIP stream; calls on one context must not overlap, calls
+ * on different contexts may run from different threads.
+ */
+#ifndef FASTPART_H
+#define FASTPART_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef FP_STATUS_DEFINED
+#define FP_STATUS_DEFINED
+typedef enum {
+  FP_OK = 0,
+  FP_ERR_ARG = 1,         /* invalid argument or malformed input */
+  FP_ERR_HIP = 2,         /* HIP runtime error */
+  FP_ERR_OOM = 3,         /* device allocation failed */
+  FP_ERR_STATE = 4,       /* call out of order (e.g. refine before upload) */
+  FP_ERR_UNSUPPORTED = 5, /* input outside the supported envelope */
+  FP_ERR_BUDGET = 6,      /* rejected by an HBM footprint budget */
+  FP_ERR_IO = 7,          /* file I/O error */
+  FP_ERR_NODEV = 8        /* no usable GPU */
+} fp_status;
+#endif
+
+typedef struct fp_ctx fp_ctx;
+
+/* The six objectives (shared/metric.h:7-21). */
+typedef enum {
+  FP_METRIC_CN = 0,   /* cut nets */
+  FP_METRIC_TV = 1,   /* total volume, sum of (lambda - 1) */
+  FP_METRIC_TM = 2,   /* total messages */
+  FP_METRIC_MSM = 3,  /* max send messages */
+  FP_METRIC_MSV = 4,  /* max send volume */
+  FP_METRIC_MSRV = 5  /* max send + receive volume */
+} fp_metric;
+
+/* Refinement schedules. Both are deterministic: results do not depend on
+ * thread timing or atomic ordering.
+ *
+ * FP_SCHED_REF: the reference's round schedule partition_par
+ *   (TV/partition.cpp:669-896) without its data race (TV/partition.cpp:806-814):
+ *   rounds of `round` consecutive positions of the ordering; every vertex of a
+ *   round decides on the state frozen at the round start; accepted moves are
+ *   applied in ordering order; as in the reference, applying a move lowers the
+ *   source part's size and leaves the destination's unchanged
+ *   (TV/partition.cpp:827-829). round = 8 * threads reproduces partition_par.
+ *
+ * FP_SCHED_BAL: the balanced round schedule. Same per-round decisions on
+ *   weighted part sizes; then for each destination part d the candidates of
+ *   the round, in ordering order, are admitted while
+ *   size[d] + (weights of earlier candidates into d) + w <= cap
+ *   (a prefix sum per part); admitted moves update both part sizes.
+ */
+typedef enum { FP_SCHED_REF = 0, FP_SCHED_BAL = 1 } fp_sched_kind;
+
+typedef struct {
+  int kind;          /* fp_sched_kind */
+  int round;         /* vertices per round, >= 1 */
+  double ldg_C;      /* LDG capacity constant C of TV/partition.cpp:103 */
+  long long max_pw;  /* a destination heavier than this is refused (TV/partition.cpp:135-138) */
+  long long min_pw;  /* a destination lighter than this is refused (TV/partition.cpp:140-143) */
+  long long cap;     /* FP_SCHED_BAL: per-part admission cap */
+} fp_schedule;
+
+typedef struct {
+  long long moves;        /* applied moves */
+  long long rounds;       /* rounds executed */
+  long long pin_visits;   /* passes * pins: (vertex, net) incidences scored */
+  double device_ms;       /* device time of the passes (HIP events on the context stream) */
+} fp_refine_stats;
+
+typedef struct {
+  long long tv, cn, tm, msm, msv, msrv;
+} fp_metrics;
+
+/* ---- context ------------------------------------------------------------ */
+fp_status fp_device_count(int* count);
+fp_status fp_ctx_create(fp_ctx** ctx, int device);
+void fp_ctx_destroy(fp_ctx* ctx);
+const char* fp_ctx_last_error(const fp_ctx* ctx);
+/* bytes of device memory the context holds now */
+long long fp_ctx_device_bytes(const fp_ctx* ctx);
+
+/* ---- hypergraph --------------------------------------------------------- */
+/* Uploads both CSR views once (shared/hypergraph_rw.cpp:97-242 and the
+ * transpose of TV/tv.cpp:76-100). vtx_ptr/vtx_nets may be NULL: the engine
+ * then builds the vertex->nets view itself (ascending nets per vertex, as the
+ * reference transpose). src (net source vertex, -1 = none) and vwgt (vertex
+ * weights, NULL = unit) are optional; ordering is the vertex visiting order
+ * of the refinement (NULL = identity; TV/tv.cpp:122-128). Replaces any
+ * previous hypergraph and state of the context. */
+fp_status fp_upload(fp_ctx* ctx, int nov, int noe, const unsigned* net_ptr, const unsigned* net_pins,
+                    const unsigned* vtx_ptr, const unsigned* vtx_nets, const int* src, const int* vwgt,
+                    const int* ordering);
+
+/* ---- connectivity state ------------------------------------------------- */
+/* State from a part vector alone. Each net's list is ordered by first
+ * appearance along the ordering, which is the state the reference's
+ * streaming initial partition leaves (TV/partition.cpp:240-263). */
+fp_status fp_set_partition(fp_ctx* ctx, int K, const int* part);
+
+/* State in the reference's layout (TV/partition.cpp:150-158): ep_ends[noe]
+ * (lambda), and per net n the slots net_ptr[n] .. net_ptr[n]+ep_ends[n]-1 of
+ * ep_parts[pins]/ep_cnt[pins]. sizes (K, nullable) are the part sizes to
+ * refine from; NULL = recomputed from part and the vertex weights. */
+fp_status fp_set_state(fp_ctx* ctx, int K, const int* part, const int* ep_ends, const int* ep_parts,
+                       const int* ep_cnt, const long long* sizes);
+
+/* Downloads the state in the reference's layout; slots past a net's lambda
+ * read part -1 / count 0, as the reference leaves them. Any pointer may be NULL. */
+fp_status fp_get_state(fp_ctx* ctx, int* part, int* ep_ends, int* ep_parts, int* ep_cnt, long long* sizes);
+
+/* ---- refinement --------------------------------------------------------- */
+/* `passes` refinement passes of `metric` under `sched` on the resident state.
+ * TV: vertex-centric LDG refinement of TV/partition.cpp:307-440 (decision and
+ * gain test), rounds per sched. stats may be NULL. */
+fp_status fp_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule* sched, fp_refine_stats* stats);
+
+/* One frozen-state evaluation of every vertex without applying anything:
+ * dec[v] = LDG decision (TV/partition.cpp:349), gain[v] = leave_gain -
+ * arrival_loss (TV/partition.cpp:355-380) or INT_MIN when dec[v] is v's own
+ * part. Either pointer may be NULL. */
+fp_status fp_eval_moves(fp_ctx* ctx, fp_metric metric, const fp_schedule* sched, int* dec, int* gain);
+
+/* ---- metrics ------------------------------------------------------------ */
+/* All six metrics of the resident part vector, from scratch on the device
+ * (shared/metric.cpp: TV :57, CN :3, TM_scratch :122, MGM_scratch :216,
+ * MGV_Scratch :301, MGAV_Scratch :353). Optional outputs: lam[noe] (lambda per
+ * net); K-long int64 vectors sv/rv (MGAV_Scratch send/receive volumes),
+ * sm/rm (distinct messages sent/received per part, CN/cn.cpp:54-58) and
+ * sv_msv (MGV_Scratch send volumes). */
+fp_status fp_eval(fp_ctx* ctx, fp_metrics* out, int* lam, long long* sv, long long* rv, long long* sm,
+                  long long* rm, long long* sv_msv);
+
+/* Event timing of the dominant refinement kernel in the last fp_refine:
+ * launches, total device ms of those launches (HIP events around each launch,
+ * on the context stream). */
+fp_status fp_last_kernel_timing(fp_ctx* ctx, long long* launches, double* total_ms);
+/* Enables (1) or disables (0) per-launch event timing of fp_refine's
+ * dominant kernel (adds events around every launch). */
+fp_status fp_set_kernel_timing(fp_ctx* ctx, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

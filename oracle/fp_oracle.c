@@ -1,0 +1,613 @@
+/* fp_oracle.c -- CPU restatement of FastPartitioner's refinement and metric
+ * path. TEST INFRASTRUCTURE ONLY (the checker): used by tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg. The product
+ * library never links, calls or falls back to this file.
+ *
+ * Plain C, scalar, single threaded. Each function cites the reference lines
+ * it restates; the header documents the contract of each entry point.
+ */
+#include "fp_oracle.h"
+
+#include <limits.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---------------------------------------------------------------------- */
+/* connectivity list edits (identical in every reference directory)        */
+/* ---------------------------------------------------------------------- */
+
+/* remove one pin of part p from net n: TV/partition.cpp:388-411 */
+static void list_remove(int n, int p, const unsigned* net_ptr, int* ep_ends, int* ep_parts, int* ep_cnt) {
+  int s = (int)net_ptr[n];
+  int end = ep_ends[n];
+  for (int j = 0; j < end; j++) {
+    if (ep_parts[s + j] == p) {
+      ep_cnt[s + j]--;
+      if (ep_cnt[s + j] == 0) {
+        ep_parts[s + j] = ep_parts[s + end - 1];
+        ep_cnt[s + j] = ep_cnt[s + end - 1];
+        ep_parts[s + end - 1] = -1;
+        ep_cnt[s + end - 1] = 0;
+        ep_ends[n]--;
+      }
+      return;
+    }
+  }
+}
+
+/* add one pin of part q to net n: TV/partition.cpp:413-435 */
+static void list_add(int n, int q, const unsigned* net_ptr, int* ep_ends, int* ep_parts, int* ep_cnt) {
+  int s = (int)net_ptr[n];
+  int end = ep_ends[n];
+  for (int j = 0; j < end; j++) {
+    if (ep_parts[s + j] == q) {
+      ep_cnt[s + j]++;
+      return;
+    }
+  }
+  ep_parts[s + end] = q;
+  ep_cnt[s + end] = 1;
+  ep_ends[n]++;
+}
+
+/* move vertex v from p to q: all removals over v's nets, then all adds
+ * (TV/partition.cpp:387-436). */
+static void move_vertex(int v, int p, int q, const unsigned* net_ptr, const unsigned* vtx_ptr,
+                        const unsigned* vtx_nets, int* ep_ends, int* ep_parts, int* ep_cnt) {
+  for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) list_remove((int)vtx_nets[i], p, net_ptr, ep_ends, ep_parts, ep_cnt);
+  for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) list_add((int)vtx_nets[i], q, net_ptr, ep_ends, ep_parts, ep_cnt);
+}
+
+/* ---------------------------------------------------------------------- */
+
+void or_transpose(int nov, int noe, const unsigned* net_ptr, const unsigned* net_pins, unsigned* vtx_ptr,
+                  unsigned* vtx_nets) {
+  /* TV/tv.cpp:79-100 */
+  memset(vtx_ptr, 0, sizeof(unsigned) * (size_t)(nov + 1));
+  for (int i = 0; i < noe; i++)
+    for (unsigned j = net_ptr[i]; j < net_ptr[i + 1]; j++) vtx_ptr[net_pins[j] + 1]++;
+  for (int i = 1; i <= nov; i++) vtx_ptr[i] += vtx_ptr[i - 1];
+  for (int i = 0; i < noe; i++)
+    for (unsigned j = net_ptr[i]; j < net_ptr[i + 1]; j++) vtx_nets[vtx_ptr[net_pins[j]]++] = (unsigned)i;
+  for (int i = nov; i > 0; i--) vtx_ptr[i] = vtx_ptr[i - 1];
+  vtx_ptr[0] = 0;
+}
+
+int or_build_state(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                   const int* ordering, int K, const int* part, const int* vwgt, int* ep_ends, int* ep_parts,
+                   int* ep_cnt, long long* sizes) {
+  long long pins = net_ptr[noe];
+  for (long long i = 0; i < pins; i++) { ep_parts[i] = -1; ep_cnt[i] = 0; }
+  for (int n = 0; n < noe; n++) ep_ends[n] = 0;
+  for (int k = 0; k < K; k++) sizes[k] = 0;
+  for (int z = 0; z < nov; z++) {
+    int v = ordering ? ordering[z] : z;
+    int q = part[v];
+    if (q < 0 || q >= K) return 1;
+    sizes[q] += vwgt ? vwgt[v] : 1;
+    for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) list_add((int)vtx_nets[i], q, net_ptr, ep_ends, ep_parts, ep_cnt);
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* metrics                                                                 */
+/* ---------------------------------------------------------------------- */
+
+int or_metrics(int nov, int noe, const unsigned* net_ptr, const unsigned* net_pins, const int* src, int K,
+               const int* part, long long* out, int* lam, long long* sv, long long* rv, long long* sm,
+               long long* rm, long long* sv_msv) {
+  int* mark = (int*)malloc(sizeof(int) * (size_t)K);
+  unsigned char* p2p = (unsigned char*)calloc((size_t)K * (size_t)K, 1);
+  long long* s_send = (long long*)calloc((size_t)K, sizeof(long long));
+  long long* s_recv = (long long*)calloc((size_t)K, sizeof(long long));
+  long long* s_sm = (long long*)calloc((size_t)K, sizeof(long long));
+  long long* s_rm = (long long*)calloc((size_t)K, sizeof(long long));
+  long long* s_msv = (long long*)calloc((size_t)K, sizeof(long long));
+  if (!mark || !p2p || !s_send || !s_recv || !s_sm || !s_rm || !s_msv) return 2;
+  for (int k = 0; k < K; k++) mark[k] = -1;
+  long long tv = 0, cn = 0;
+  for (int n = 0; n < noe; n++) {
+    /* lambda(n): distinct parts among the pins (CN/cn.cpp:42-50) */
+    int l = 0;
+    for (unsigned j = net_ptr[n]; j < net_ptr[n + 1]; j++) {
+      int q = part[net_pins[j]];
+      if (mark[q] != n) { mark[q] = n; l++; }
+    }
+    lam[n] = l;
+    if (l != 0) tv += l - 1;    /* metric.cpp:62-65 */
+    if (l > 1) cn++;            /* metric.cpp:8-11 */
+    int s = src[n];
+    if (s < 0 || s >= nov) { free(mark); free(p2p); free(s_send); free(s_recv); free(s_sm); free(s_rm); free(s_msv); return 1; }
+    int sp = part[s];
+    /* MGAV_Scratch send: (metric.cpp:365-366) */
+    s_send[sp] += l - 1;
+    /* MGV_Scratch: adds lambda-1 once per occurrence of src among the pins (metric.cpp:311-318) */
+    for (unsigned j = net_ptr[n]; j < net_ptr[n + 1]; j++)
+      if ((int)net_pins[j] == s) s_msv[sp] += l - 1;
+    /* receive volumes and messages: each distinct part != sp (metric.cpp:369-375, TM_scratch :169-176) */
+    for (unsigned j = net_ptr[n]; j < net_ptr[n + 1]; j++) {
+      int q = part[net_pins[j]];
+      if (mark[q] == -2 - n) continue;  /* already counted for this net */
+      mark[q] = -2 - n;
+      if (q == sp) continue;
+      s_recv[q]++;
+      unsigned char* e = &p2p[(size_t)sp * (size_t)K + (size_t)q];
+      if (!*e) { *e = 1; s_sm[sp]++; s_rm[q]++; }
+    }
+  }
+  long long tm = 0, msm = -1, msv = -1, msrv = -1;
+  for (int k = 0; k < K; k++) {
+    tm += s_sm[k];
+    if (s_sm[k] > msm) msm = s_sm[k];             /* MGM_scratch max, metric.cpp:273-283 */
+    if (s_msv[k] > msv) msv = s_msv[k];           /* MGV_Scratch max, metric.cpp:321-333 */
+    if (s_send[k] + s_recv[k] > msrv) msrv = s_send[k] + s_recv[k]; /* metric.cpp:379-389 */
+  }
+  out[0] = tv; out[1] = cn; out[2] = tm; out[3] = msm; out[4] = msv; out[5] = msrv;
+  for (int k = 0; k < K; k++) {
+    if (sv) sv[k] = s_send[k];
+    if (rv) rv[k] = s_recv[k];
+    if (sm) sm[k] = s_sm[k];
+    if (rm) rm[k] = s_rm[k];
+    if (sv_msv) sv_msv[k] = s_msv[k];
+  }
+  free(mark); free(p2p); free(s_send); free(s_recv); free(s_sm); free(s_rm); free(s_msv);
+  return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* TV                                                                      */
+/* ---------------------------------------------------------------------- */
+
+/* LDG2 decision (TV/partition.cpp:89-145). Returns the decision, or -2 when
+ * the reference would index part_sizes[-1] (max band hit while the vertex is
+ * still unassigned: undefined in the reference). */
+static int ldg2(int K, int apc, const int* active, const double* scores, const int* sizes, double C, int max_pw,
+                int min_pw, int own) {
+  double max_val = 0;
+  int max_index = -1;
+  for (int i = 0; i < apc; i++) {
+    int q = active[i];
+    double penalty = 1 - (1.0 * sizes[q] / C);
+    double score = scores[q] * penalty;
+    if (score > 0) {
+      if (max_index == -1 || score > max_val || (score == max_val && sizes[q] < sizes[max_index])) {
+        max_val = score;
+        max_index = q;
+      }
+    }
+  }
+  if (max_index == -1) {
+    int min_load = INT_MAX;
+    for (int i = 0; i < K; i++)
+      if (sizes[i] < min_load) { min_load = sizes[i]; max_index = i; }
+  }
+  int d = max_index;
+  if (sizes[d] > max_pw) {
+    d = own;
+    if (d < 0) return -2;
+  }
+  if (sizes[d] < min_pw && own != -1) d = own;
+  return d;
+}
+
+int or_tv_ldg(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+              const int* ordering, int K, double C, int max_pw, int min_pw, int par_skip, int* part, int* ep_ends,
+              int* ep_parts, int* ep_cnt, int* sizes, int* marker) {
+  long long pins = net_ptr[noe];
+  int* active = (int*)malloc(sizeof(int) * (size_t)K);
+  double* scores = (double*)malloc(sizeof(double) * (size_t)K);
+  if (!active || !scores) return 2;
+  /* TV/partition.cpp:166-174 */
+  for (long long i = 0; i < pins; i++) { ep_cnt[i] = 0; ep_parts[i] = -1; }
+  for (int k = 0; k < K; k++) { marker[k] = -1; sizes[k] = 0; scores[k] = 0.0; active[k] = -1; }
+  for (int v = 0; v < nov; v++) part[v] = -1;
+  for (int n = 0; n < noe; n++) ep_ends[n] = 0;
+  int rc = 0;
+  for (int z = 0; z < nov; z++) {
+    int v = ordering[z];
+    int apc = 0;
+    /* TV/partition.cpp:214-233 */
+    for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
+      int n = (int)vtx_nets[i];
+      int s = (int)net_ptr[n];
+      for (int j = 0; j < ep_ends[n]; j++) {
+        int q = ep_parts[s + j];
+        if (marker[q] != v) { scores[q] = 1; marker[q] = v; active[apc++] = q; }
+        else scores[q]++;
+      }
+    }
+    int d = ldg2(K, apc, active, scores, sizes, C, max_pw, min_pw, part[v]);
+    if (d == -2) { rc = 3; break; }
+    if (par_skip && sizes[d] > max_pw) continue;   /* TV/partition.cpp:598-602 */
+    sizes[d]++;
+    part[v] = d;
+    for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) list_add((int)vtx_nets[i], d, net_ptr, ep_ends, ep_parts, ep_cnt);
+  }
+  free(active); free(scores);
+  return rc;
+}
+
+int or_tv_refine_serial(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr,
+                        const unsigned* vtx_nets, const int* ordering, int K, double C, int max_pw, int min_pw,
+                        int passes, int* part, int* ep_ends, int* ep_parts, int* ep_cnt, int* sizes, int* marker) {
+  (void)noe;
+  int* active = (int*)malloc(sizeof(int) * (size_t)K);
+  double* scores = (double*)calloc((size_t)K, sizeof(double));
+  if (!active || !scores) return 2;
+  for (int it = 0; it < passes; it++) {
+    for (int z = 0; z < nov; z++) {
+      int v = ordering[z];
+      int own = part[v];
+      int apc = 0;
+      /* TV/partition.cpp:318-338: marker is NOT reset between vertices or passes */
+      for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
+        int n = (int)vtx_nets[i];
+        int s = (int)net_ptr[n];
+        int end = ep_ends[n];
+        for (int j = 0; j < end; j++) {
+          int q = ep_parts[s + j];
+          if (marker[q] != v) { scores[q] = 1; marker[q] = v; active[apc++] = q; }
+          else scores[q]++;
+        }
+      }
+      scores[own]--;   /* TV/partition.cpp:343-346 */
+      int d = ldg2(K, apc, active, scores, sizes, C, max_pw, min_pw, own);
+      if (d == -2) { free(active); free(scores); return 3; }
+      if (d == own) continue;
+      /* TV/partition.cpp:355-378 */
+      int leave_gain = 0, arrival_loss = 0;
+      for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
+        int n = (int)vtx_nets[i];
+        int s = (int)net_ptr[n];
+        int end = ep_ends[n];
+        int found = 0;
+        for (int j = 0; j < end; j++) {
+          if (ep_parts[s + j] == own && ep_cnt[s + j] == 1) leave_gain++;
+          else if (ep_parts[s + j] == d) found = 1;
+        }
+        if (!found) arrival_loss++;
+      }
+      if (leave_gain >= arrival_loss) {   /* TV/partition.cpp:380-386 */
+        part[v] = d;
+        sizes[own]--;
+        sizes[d]++;
+        move_vertex(v, own, d, net_ptr, vtx_ptr, vtx_nets, ep_ends, ep_parts, ep_cnt);
+      }
+    }
+  }
+  free(active); free(scores);
+  return 0;
+}
+
+/* One frozen-state TV evaluation of vertex v (TV/partition.cpp:732-805).
+ * Scores start fresh for every vertex (the per-thread marker of partition_par
+ * is reset each pass and each vertex is visited once per pass,
+ * TV/partition.cpp:714). Returns the LDG2 decision and writes the gain test
+ * terms. sizes_i is the int view used by LDG2. */
+static int tv_eval(int v, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets, int K,
+                   double C, int max_pw, int min_pw, const int* part, const int* ep_ends, const int* ep_parts,
+                   const int* ep_cnt, const int* sizes_i, int* marker, int* active, double* scores, int* lg,
+                   int* al) {
+  int own = part[v];
+  int apc = 0;
+  for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
+    int n = (int)vtx_nets[i];
+    int s = (int)net_ptr[n];
+    for (int j = 0; j < ep_ends[n]; j++) {
+      int q = ep_parts[s + j];
+      if (marker[q] != v) { scores[q] = 1; marker[q] = v; active[apc++] = q; }
+      else scores[q]++;
+    }
+  }
+  scores[own]--;
+  int d = ldg2(K, apc, active, scores, sizes_i, C, max_pw, min_pw, own);
+  /* reset this vertex's marks so the next vertex starts fresh */
+  for (int i = 0; i < apc; i++) marker[active[i]] = -1;
+  *lg = 0; *al = 0;
+  if (d < 0 || d == own) return d;
+  for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
+    int n = (int)vtx_nets[i];
+    int s = (int)net_ptr[n];
+    int found = 0;
+    for (int j = 0; j < ep_ends[n]; j++) {
+      int t = ep_parts[s + j];
+      if (t == own && ep_cnt[s + j] == 1) (*lg)++;
+      else if (t == d) found = 1;
+    }
+    if (!found) (*al)++;
+  }
+  return d;
+}
+
+int or_tv_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                     const int* ordering, int K, double C, int max_pw, int min_pw, int passes, int R, int chunk,
+                     int* part, int* ep_ends, int* ep_parts, int* ep_cnt, int* sizes, int* gains) {
+  (void)noe;
+  if (R < 1 || chunk < 1) return 1;
+  int* marker = (int*)malloc(sizeof(int) * (size_t)K);
+  int* active = (int*)malloc(sizeof(int) * (size_t)K);
+  double* scores = (double*)calloc((size_t)K, sizeof(double));
+  int* dec = (int*)malloc(sizeof(int) * (size_t)R);
+  if (!marker || !active || !scores || !dec) return 2;
+  for (int k = 0; k < K; k++) marker[k] = -1;
+  for (int it = 0; it < passes; it++) {
+    for (int r0 = 0; r0 < nov; r0 += R) {
+      int r1 = r0 + R < nov ? r0 + R : nov;
+      /* compute phase on the frozen state (TV/partition.cpp:725-805) */
+      for (int z = r0; z < r1; z++) {
+        int v = ordering[z];
+        int lg, al;
+        int d = tv_eval(v, net_ptr, vtx_ptr, vtx_nets, K, C, max_pw, min_pw, part, ep_ends, ep_parts, ep_cnt,
+                        sizes, marker, active, scores, &lg, &al);
+        if (d == -2) { free(marker); free(active); free(scores); free(dec); return 3; }
+        if (gains) gains[v] = (d == part[v]) ? INT_MIN : lg - al;
+        dec[z - r0] = (d != part[v] && lg >= al) ? d : -1;
+      }
+      /* apply in ordering order (TV/partition.cpp:810-892) */
+      for (int c0 = r0; c0 < r1; c0 += chunk) {
+        int c1 = c0 + chunk < r1 ? c0 + chunk : r1;
+        for (int z = c0; z < c1; z++) {
+          int d = dec[z - r0];
+          if (d < 0) continue;
+          int v = ordering[z];
+          int own = part[v];
+          if (sizes[d] > max_pw) break;   /* "Weird": ends this thread's list */
+          part[v] = d;
+          sizes[own]--;
+          move_vertex(v, own, d, net_ptr, vtx_ptr, vtx_nets, ep_ends, ep_parts, ep_cnt);
+        }
+      }
+    }
+  }
+  free(marker); free(active); free(scores); free(dec);
+  return 0;
+}
+
+/* LDG2 on int64 weighted sizes (same double expression, TV/partition.cpp:103-104) */
+static int ldg2_w(int K, int apc, const int* active, const double* scores, const long long* sizes, double C,
+                  long long max_pw, long long min_pw, int own) {
+  double max_val = 0;
+  int max_index = -1;
+  for (int i = 0; i < apc; i++) {
+    int q = active[i];
+    double penalty = 1 - (1.0 * (double)sizes[q] / C);
+    double score = scores[q] * penalty;
+    if (score > 0) {
+      if (max_index == -1 || score > max_val || (score == max_val && sizes[q] < sizes[max_index])) {
+        max_val = score;
+        max_index = q;
+      }
+    }
+  }
+  if (max_index == -1) {
+    long long min_load = LLONG_MAX;
+    for (int i = 0; i < K; i++)
+      if (sizes[i] < min_load) { min_load = sizes[i]; max_index = i; }
+  }
+  int d = max_index;
+  if (sizes[d] > max_pw) d = own;
+  if (sizes[d] < min_pw) d = own;
+  return d;
+}
+
+int or_tv_refine_bal(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                     const int* ordering, const int* vwgt, int K, double C, long long max_pw, long long min_pw,
+                     long long cap, int passes, int R, int* part, int* ep_ends, int* ep_parts, int* ep_cnt,
+                     long long* sizes, int* gains) {
+  (void)noe;
+  if (R < 1) return 1;
+  int* marker = (int*)malloc(sizeof(int) * (size_t)K);
+  int* active = (int*)malloc(sizeof(int) * (size_t)K);
+  double* scores = (double*)calloc((size_t)K, sizeof(double));
+  int* dec = (int*)malloc(sizeof(int) * (size_t)R);
+  long long* inflow = (long long*)calloc((size_t)K, sizeof(long long));
+  if (!marker || !active || !scores || !dec || !inflow) return 2;
+  for (int k = 0; k < K; k++) marker[k] = -1;
+  for (int it = 0; it < passes; it++) {
+    for (int r0 = 0; r0 < nov; r0 += R) {
+      int r1 = r0 + R < nov ? r0 + R : nov;
+      for (int z = r0; z < r1; z++) {
+        int v = ordering[z];
+        int own = part[v];
+        int apc = 0;
+        for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
+          int n = (int)vtx_nets[i];
+          int s = (int)net_ptr[n];
+          for (int j = 0; j < ep_ends[n]; j++) {
+            int q = ep_parts[s + j];
+            if (marker[q] != v) { scores[q] = 1; marker[q] = v; active[apc++] = q; }
+            else scores[q]++;
+          }
+        }
+        scores[own]--;
+        int d = ldg2_w(K, apc, active, scores, sizes, C, max_pw, min_pw, own);
+        for (int i = 0; i < apc; i++) marker[active[i]] = -1;
+        dec[z - r0] = -1;
+        if (gains) gains[v] = INT_MIN;
+        if (d == own) continue;
+        int lg = 0, al = 0;
+        for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
+          int n = (int)vtx_nets[i];
+          int s = (int)net_ptr[n];
+          int found = 0;
+          for (int j = 0; j < ep_ends[n]; j++) {
+            int t = ep_parts[s + j];
+            if (t == own && ep_cnt[s + j] == 1) lg++;
+            else if (t == d) found = 1;
+          }
+          if (!found) al++;
+        }
+        if (gains) gains[v] = lg - al;
+        if (lg >= al) dec[z - r0] = d;
+      }
+      /* per-destination prefix admission in ordering order */
+      for (int k = 0; k < K; k++) inflow[k] = 0;
+      for (int z = r0; z < r1; z++) {
+        int d = dec[z - r0];
+        if (d < 0) continue;
+        long long w = vwgt ? vwgt[ordering[z]] : 1;
+        long long before = inflow[d];
+        inflow[d] += w;
+        if (sizes[d] + before + w > cap) dec[z - r0] = -1;
+      }
+      for (int z = r0; z < r1; z++) {
+        int d = dec[z - r0];
+        if (d < 0) continue;
+        int v = ordering[z];
+        int own = part[v];
+        long long w = vwgt ? vwgt[v] : 1;
+        part[v] = d;
+        sizes[own] -= w;
+        sizes[d] += w;
+        move_vertex(v, own, d, net_ptr, vtx_ptr, vtx_nets, ep_ends, ep_parts, ep_cnt);
+      }
+    }
+  }
+  free(marker); free(active); free(scores); free(dec); free(inflow);
+  return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* CN                                                                      */
+/* ---------------------------------------------------------------------- */
+
+int or_cn_ldg(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+              const int* ordering, int K, double C, int* part, int* ep_ends, int* ep_parts, int* ep_cnt,
+              int* sizes) {
+  long long pins = net_ptr[noe];
+  int* marker = (int*)malloc(sizeof(int) * (size_t)K);
+  int* active = (int*)malloc(sizeof(int) * (size_t)K);
+  double* scores = (double*)calloc((size_t)K, sizeof(double));
+  if (!marker || !active || !scores) return 2;
+  for (long long i = 0; i < pins; i++) { ep_cnt[i] = 0; ep_parts[i] = -1; }
+  for (int k = 0; k < K; k++) { marker[k] = -1; sizes[k] = 0; }
+  for (int v = 0; v < nov; v++) part[v] = -1;
+  for (int n = 0; n < noe; n++) ep_ends[n] = 0;
+  for (int z = 0; z < nov; z++) {
+    int v = ordering[z];
+    int apc = 0;
+    /* CN/partition.cpp:138-156: only nets that are not cut vote */
+    for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
+      int n = (int)vtx_nets[i];
+      if (ep_ends[n] == 1) {
+        int q = ep_parts[net_ptr[n]];
+        if (marker[q] != v) { scores[q] = 1; marker[q] = v; active[apc++] = q; }
+        else scores[q]++;
+      }
+    }
+    /* CN/partition.cpp:159-191 */
+    double max_val = 0;
+    int max_index = -1;
+    for (int i = 0; i < apc; i++) {
+      int q = active[i];
+      double penalty = 1 - (1.0 * sizes[q] / C);
+      double score = scores[q] * penalty;
+      if (score > 0 && (max_index == -1 || score > max_val || (score == max_val && sizes[q] < sizes[max_index]))) {
+        max_val = score;
+        max_index = q;
+      }
+    }
+    if (max_index == -1) {
+      int min_load = INT_MAX;
+      for (int i = 0; i < K; i++)
+        if (sizes[i] < min_load) { min_load = sizes[i]; max_index = i; }
+    }
+    int d = max_index;
+    sizes[d]++;
+    part[v] = d;
+    for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) list_add((int)vtx_nets[i], d, net_ptr, ep_ends, ep_parts, ep_cnt);
+  }
+  free(marker); free(active); free(scores);
+  return 0;
+}
+
+/* lambda=2 push decision for net n on the current state (CN/partition.cpp:254-310).
+ * Returns 1 and fills push[] (count *np) with the old-side pins in net->pins
+ * order if the push is allowed, else 0. */
+static int cn_net_decision(int n, const unsigned* net_ptr, const unsigned* net_pins, const unsigned* vtx_ptr,
+                           const unsigned* vtx_nets, const int* part, const int* ep_ends, const int* ep_parts,
+                           const int* ep_cnt, int* push, int* np, int* newd) {
+  int s = (int)net_ptr[n];
+  int p1 = ep_parts[s], c1 = ep_cnt[s];
+  int p2 = ep_parts[s + 1], c2 = ep_cnt[s + 1];
+  int oldd, nd;
+  if (c1 < c2) { oldd = p1; nd = p2; }
+  else { oldd = p2; nd = p1; }
+  *np = 0;
+  for (unsigned k = net_ptr[n]; k < net_ptr[n + 1]; k++) {
+    int v = (int)net_pins[k];
+    if (part[v] != oldd) continue;
+    push[(*np)++] = v;
+    for (unsigned j = vtx_ptr[v]; j < vtx_ptr[v + 1]; j++)
+      if (ep_ends[vtx_nets[j]] == 1) return 0;
+  }
+  *newd = nd;
+  return 1;
+}
+
+int or_cn_refine_serial(int nov, int noe, const unsigned* net_ptr, const unsigned* net_pins,
+                        const unsigned* vtx_ptr, const unsigned* vtx_nets, int K, int passes, int* part,
+                        int* ep_ends, int* ep_parts, int* ep_cnt, int* sizes) {
+  (void)nov; (void)K;
+  int maxdeg = 0;
+  for (int n = 0; n < noe; n++)
+    if ((int)(net_ptr[n + 1] - net_ptr[n]) > maxdeg) maxdeg = (int)(net_ptr[n + 1] - net_ptr[n]);
+  int* push = (int*)malloc(sizeof(int) * (size_t)(maxdeg + 1));
+  if (!push) return 2;
+  for (int it = 0; it < passes; it++) {
+    for (int n = 0; n < noe; n++) {
+      if (ep_ends[n] != 2) continue;
+      int np, nd;
+      if (!cn_net_decision(n, net_ptr, net_pins, vtx_ptr, vtx_nets, part, ep_ends, ep_parts, ep_cnt, push, &np, &nd))
+        continue;
+      /* CN/partition.cpp:313-371: move immediately, no balance check */
+      for (int i = 0; i < np; i++) {
+        int v = push[i];
+        int own = part[v];
+        part[v] = nd;
+        sizes[own]--;
+        sizes[nd]++;
+        move_vertex(v, own, nd, net_ptr, vtx_ptr, vtx_nets, ep_ends, ep_parts, ep_cnt);
+      }
+    }
+  }
+  free(push);
+  return 0;
+}
+
+int or_cn_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* net_pins, const unsigned* vtx_ptr,
+                     const unsigned* vtx_nets, int K, int passes, int* part, int* ep_ends, int* ep_parts,
+                     int* ep_cnt, int* sizes) {
+  (void)nov; (void)K;
+  long long pins = net_ptr[noe];
+  int maxdeg = 0;
+  for (int n = 0; n < noe; n++)
+    if ((int)(net_ptr[n + 1] - net_ptr[n]) > maxdeg) maxdeg = (int)(net_ptr[n + 1] - net_ptr[n]);
+  int* push = (int*)malloc(sizeof(int) * (size_t)(maxdeg + 1));
+  int* mv_v = (int*)malloc(sizeof(int) * (size_t)(pins + 1));
+  int* mv_d = (int*)malloc(sizeof(int) * (size_t)(pins + 1));
+  if (!push || !mv_v || !mv_d) return 2;
+  for (int it = 0; it < passes; it++) {
+    /* evaluation on the frozen state (CN/partition.cpp:629-772) */
+    long long m = 0;
+    for (int n = 0; n < noe; n++) {
+      if (ep_ends[n] != 2) continue;
+      int np, nd;
+      if (!cn_net_decision(n, net_ptr, net_pins, vtx_ptr, vtx_nets, part, ep_ends, ep_parts, ep_cnt, push, &np, &nd))
+        continue;
+      for (int i = 0; i < np; i++) { mv_v[m] = push[i]; mv_d[m] = nd; m++; }
+    }
+    /* apply in push order; the source part is read at apply time (CN/partition.cpp:778-840) */
+    for (long long i = 0; i < m; i++) {
+      int v = mv_v[i], nd = mv_d[i];
+      int own = part[v];
+      part[v] = nd;
+      sizes[own]--;
+      sizes[nd]++;
+      move_vertex(v, own, nd, net_ptr, vtx_ptr, vtx_nets, ep_ends, ep_parts, ep_cnt);
+    }
+  }
+  free(push); free(mv_v); free(mv_d);
+  return 0;
+}

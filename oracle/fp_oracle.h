@@ -1,0 +1,106 @@
+/* fp_oracle.h -- CPU restatement of FastPartitioner's refinement and metric
+ * path. TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py use it as the checker. The product library
+ * never links, calls or falls back to it.
+ *
+ * Every function cites the reference lines it restates (paths relative to the
+ * reference tree). Arrays are plain C arrays; ep_starts is the net->pins CSR
+ * offset array (net_ptr), as in the reference (TV/partition.cpp:177-197).
+ * Unless stated, a connectivity list slot past ep_ends[n] holds part -1 and
+ * count 0, as the reference leaves it (TV/partition.cpp:174,403-404).
+ *
+ * Status returns: 0 ok, nonzero = input outside what the reference defines.
+ */
+#ifndef FP_ORACLE_H
+#define FP_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* vertex->nets CSR from net->pins CSR by counting sort (TV/tv.cpp:76-100). */
+void or_transpose(int nov, int noe, const unsigned* net_ptr, const unsigned* net_pins, unsigned* vtx_ptr,
+                  unsigned* vtx_nets);
+
+/* Connectivity lists for a given part vector, with each net's list in the
+ * order parts first appear when the vertices are visited in `ordering` order
+ * and each vertex's nets in vertex->nets order. This is exactly the state the
+ * reference's streaming initial partition leaves (append on first appearance,
+ * TV/partition.cpp:240-263), so it reproduces that state from the part vector
+ * alone. sizes[k] = sum of vwgt (NULL = unit) over part k. */
+int or_build_state(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                   const int* ordering, int K, const int* part, const int* vwgt, int* ep_ends, int* ep_parts,
+                   int* ep_cnt, long long* sizes);
+
+/* Metrics of shared/metric.cpp for one part vector.
+ * out[0]=TV (metric.cpp:57), out[1]=CN (:3), out[2]=TM (TM_scratch :122),
+ * out[3]=MSM (MGM_scratch :216), out[4]=MSV (MGV_Scratch :301),
+ * out[5]=MSRV (MGAV_Scratch :353). lam[noe] receives lambda(n).
+ * Per-part int64 vectors (each K long, any may be NULL):
+ *   sv  = MGAV_Scratch send  (metric.cpp:365-366)
+ *   rv  = MGAV_Scratch receive (:369-375)
+ *   sm, rm = distinct (sender part, receiver part) pair counts (CN/cn.cpp:54-58)
+ *   sv_msv = MGV_Scratch send volumes (metric.cpp:310-319). */
+int or_metrics(int nov, int noe, const unsigned* net_ptr, const unsigned* net_pins, const int* src, int K,
+               const int* part, long long* out, int* lam, long long* sv, long long* rv, long long* sm,
+               long long* rm, long long* sv_msv);
+
+/* ---- TV ---------------------------------------------------------------- */
+/* Streaming LDG2 initial partition (TV/partition.cpp:89-145, 207-264).
+ * par_skip=1 applies partition_par's extra "size > max_pw -> skip" rule
+ * (TV/partition.cpp:598-602). marker[K] receives the scoring marker array
+ * the serial refinement continues from (TV/partition.cpp:151,167,221-226). */
+int or_tv_ldg(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+              const int* ordering, int K, double C, int max_pw, int min_pw, int par_skip, int* part, int* ep_ends,
+              int* ep_parts, int* ep_cnt, int* sizes, int* marker);
+
+/* Serial refinement, partition() (TV/partition.cpp:307-440), continuing with
+ * the marker array left by the initial partition (stale-marker semantics). */
+int or_tv_refine_serial(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr,
+                        const unsigned* vtx_nets, const int* ordering, int K, double C, int max_pw, int min_pw,
+                        int passes, int* part, int* ep_ends, int* ep_parts, int* ep_cnt, int* sizes, int* marker);
+
+/* Round schedule of partition_par (TV/partition.cpp:669-896) with its data
+ * race (TV/partition.cpp:806-814) removed: rounds of R consecutive positions
+ * of `ordering`; every vertex of a round decides on the state frozen at the
+ * round start; accepted moves apply in ordering order; apply decrements the
+ * source part size only (the destination increment is commented out at
+ * TV/partition.cpp:827-829). chunk = vertices per OpenMP thread per round
+ * (8, TV/partition.cpp:717), used only by the apply's per-thread "Weird"
+ * break (TV/partition.cpp:832-835). R = 8*threads reproduces partition_par.
+ * gains (nullable, nov) receives leave_gain - arrival_loss of each vertex's
+ * last evaluation (INT_MIN when its decision was its own part). */
+int or_tv_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                     const int* ordering, int K, double C, int max_pw, int min_pw, int passes, int R, int chunk,
+                     int* part, int* ep_ends, int* ep_parts, int* ep_cnt, int* sizes, int* gains);
+
+/* Balanced deterministic round schedule (this build's production schedule;
+ * no reference counterpart): same per-round decisions as or_tv_refine_par on
+ * weighted part sizes, then per destination part d the candidates in
+ * ordering order are admitted while sizes[d] + (weights of earlier
+ * candidates into d this round) + w <= cap; admitted moves apply in ordering
+ * order and update both part sizes. vwgt NULL = unit weights. */
+int or_tv_refine_bal(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                     const int* ordering, const int* vwgt, int K, double C, long long max_pw, long long min_pw,
+                     long long cap, int passes, int R, int* part, int* ep_ends, int* ep_parts, int* ep_cnt,
+                     long long* sizes, int* gains);
+
+/* ---- CN ---------------------------------------------------------------- */
+/* Initial partition of CN/partition.cpp:132-221 (LDG over uncut nets). */
+int or_cn_ldg(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+              const int* ordering, int K, double C, int* part, int* ep_ends, int* ep_parts, int* ep_cnt,
+              int* sizes);
+/* Serial lambda=2 push refinement (CN/partition.cpp:245-377). */
+int or_cn_refine_serial(int nov, int noe, const unsigned* net_ptr, const unsigned* net_pins,
+                        const unsigned* vtx_ptr, const unsigned* vtx_nets, int K, int passes, int* part,
+                        int* ep_ends, int* ep_parts, int* ep_cnt, int* sizes);
+/* partition_par's CN refinement (CN/partition.cpp:618-779) with one thread's
+ * order: every lambda=2 net is evaluated on the state frozen at pass start,
+ * pushes are applied in net order, then pin order. */
+int or_cn_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* net_pins, const unsigned* vtx_ptr,
+                     const unsigned* vtx_nets, int K, int passes, int* part, int* ep_ends, int* ep_parts,
+                     int* ep_cnt, int* sizes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,211 @@
+"""ctypes binding of liboracle.so (fp_oracle.c) and of the reference harness
+binaries in oracle/_ref. TEST INFRASTRUCTURE ONLY: imported by tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg, never by the product.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+_U32P = ctypes.POINTER(ctypes.c_uint32)
+_I32P = ctypes.POINTER(ctypes.c_int32)
+_I64P = ctypes.POINTER(ctypes.c_int64)
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} missing: run `make -C oracle` or __graft_entry__.build()")
+        _lib = ctypes.CDLL(LIB)
+    return _lib
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _ord(hg, ordering):
+    if ordering is None:
+        ordering = np.arange(hg.nov, dtype=np.int32)
+    return np.ascontiguousarray(ordering, dtype=np.int32)
+
+
+class State:
+    """Connectivity state in the reference's layout (TV/partition.cpp:150-158)."""
+
+    def __init__(self, hg, K, sizes_dtype=np.int32):
+        self.K = K
+        self.part = np.full(hg.nov, -1, dtype=np.int32)
+        self.ep_ends = np.zeros(hg.noe, dtype=np.int32)
+        self.ep_parts = np.full(hg.pins, -1, dtype=np.int32)
+        self.ep_cnt = np.zeros(hg.pins, dtype=np.int32)
+        self.sizes = np.zeros(K, dtype=sizes_dtype)
+
+    def copy(self):
+        s = State.__new__(State)
+        s.K = self.K
+        for k in ("part", "ep_ends", "ep_parts", "ep_cnt", "sizes"):
+            setattr(s, k, getattr(self, k).copy())
+        return s
+
+
+def build_state(hg, K, part, ordering=None):
+    st = State(hg, K, np.int64)
+    st.part[:] = part
+    rc = lib().or_build_state(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
+                              _p(hg.vtx_nets, _U32P), _p(_ord(hg, ordering), _I32P), K, _p(st.part, _I32P),
+                              _p(hg.vwgt, _I32P), _p(st.ep_ends, _I32P), _p(st.ep_parts, _I32P),
+                              _p(st.ep_cnt, _I32P), _p(st.sizes, _I64P))
+    if rc:
+        raise ValueError(f"or_build_state rc={rc}")
+    return st
+
+
+METRIC_NAMES = ("TV", "CN", "TM", "MSM", "MSV", "MSRV")
+
+
+def metrics(hg, K, part):
+    out = np.zeros(6, dtype=np.int64)
+    lam = np.zeros(hg.noe, dtype=np.int32)
+    vec = {k: np.zeros(K, dtype=np.int64) for k in ("sv", "rv", "sm", "rm", "sv_msv")}
+    rc = lib().or_metrics(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.net_pins, _U32P), _p(hg.src, _I32P),
+                          K, _p(np.ascontiguousarray(part, dtype=np.int32), _I32P), _p(out, _I64P),
+                          _p(lam, _I32P), _p(vec["sv"], _I64P), _p(vec["rv"], _I64P), _p(vec["sm"], _I64P),
+                          _p(vec["rm"], _I64P), _p(vec["sv_msv"], _I64P))
+    if rc:
+        raise ValueError(f"or_metrics rc={rc}")
+    res = dict(zip(METRIC_NAMES, (int(x) for x in out)))
+    res["lam"] = lam
+    res.update(vec)
+    return res
+
+
+def tv_ldg(hg, K, C, max_pw, min_pw, par_skip=False, ordering=None):
+    st = State(hg, K)
+    marker = np.zeros(K, dtype=np.int32)
+    rc = lib().or_tv_ldg(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P), _p(hg.vtx_nets, _U32P),
+                         _p(_ord(hg, ordering), _I32P), K, ctypes.c_double(C), max_pw, min_pw, int(par_skip),
+                         _p(st.part, _I32P), _p(st.ep_ends, _I32P), _p(st.ep_parts, _I32P), _p(st.ep_cnt, _I32P),
+                         _p(st.sizes, _I32P), _p(marker, _I32P))
+    if rc:
+        raise ValueError(f"or_tv_ldg rc={rc}")
+    return st, marker
+
+
+def tv_refine_serial(hg, st, marker, C, max_pw, min_pw, passes, ordering=None):
+    rc = lib().or_tv_refine_serial(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
+                                   _p(hg.vtx_nets, _U32P), _p(_ord(hg, ordering), _I32P), st.K,
+                                   ctypes.c_double(C), max_pw, min_pw, passes, _p(st.part, _I32P),
+                                   _p(st.ep_ends, _I32P), _p(st.ep_parts, _I32P), _p(st.ep_cnt, _I32P),
+                                   _p(st.sizes, _I32P), _p(marker, _I32P))
+    if rc:
+        raise ValueError(f"or_tv_refine_serial rc={rc}")
+    return st
+
+
+def tv_refine_par(hg, st, C, max_pw, min_pw, passes, R, chunk=8, ordering=None, gains=None):
+    assert st.sizes.dtype == np.int32
+    rc = lib().or_tv_refine_par(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
+                                _p(hg.vtx_nets, _U32P), _p(_ord(hg, ordering), _I32P), st.K, ctypes.c_double(C),
+                                max_pw, min_pw, passes, R, chunk, _p(st.part, _I32P), _p(st.ep_ends, _I32P),
+                                _p(st.ep_parts, _I32P), _p(st.ep_cnt, _I32P), _p(st.sizes, _I32P),
+                                _p(gains, _I32P))
+    if rc:
+        raise ValueError(f"or_tv_refine_par rc={rc}")
+    return st
+
+
+def tv_refine_bal(hg, st, C, max_pw, min_pw, cap, passes, R, ordering=None, gains=None):
+    assert st.sizes.dtype == np.int64
+    L = ctypes.c_longlong
+    rc = lib().or_tv_refine_bal(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
+                                _p(hg.vtx_nets, _U32P), _p(_ord(hg, ordering), _I32P), _p(hg.vwgt, _I32P),
+                                st.K, ctypes.c_double(C), L(max_pw), L(min_pw), L(cap), passes, R,
+                                _p(st.part, _I32P), _p(st.ep_ends, _I32P), _p(st.ep_parts, _I32P),
+                                _p(st.ep_cnt, _I32P), _p(st.sizes, _I64P), _p(gains, _I32P))
+    if rc:
+        raise ValueError(f"or_tv_refine_bal rc={rc}")
+    return st
+
+
+def cn_ldg(hg, K, C, ordering=None):
+    st = State(hg, K)
+    rc = lib().or_cn_ldg(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P), _p(hg.vtx_nets, _U32P),
+                         _p(_ord(hg, ordering), _I32P), K, ctypes.c_double(C), _p(st.part, _I32P),
+                         _p(st.ep_ends, _I32P), _p(st.ep_parts, _I32P), _p(st.ep_cnt, _I32P),
+                         _p(st.sizes, _I32P))
+    if rc:
+        raise ValueError(f"or_cn_ldg rc={rc}")
+    return st
+
+
+def cn_refine(hg, st, passes, serial):
+    f = lib().or_cn_refine_serial if serial else lib().or_cn_refine_par
+    rc = f(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.net_pins, _U32P), _p(hg.vtx_ptr, _U32P),
+           _p(hg.vtx_nets, _U32P), st.K, passes, _p(st.part, _I32P), _p(st.ep_ends, _I32P),
+           _p(st.ep_parts, _I32P), _p(st.ep_cnt, _I32P), _p(st.sizes, _I32P))
+    if rc:
+        raise ValueError(f"or_cn_refine rc={rc}")
+    return st
+
+
+# ---- reference harness (oracle/_ref/ref_<m>) -----------------------------
+
+def ref_available(metric: str) -> bool:
+    return os.path.exists(os.path.join(REF_DIR, f"ref_{metric}"))
+
+
+def run_reference(metric: str, hg, K: int, passes: int, threads: int, timeout: float = 600.0,
+                  env_extra=None):
+    """Run the reference's own partition()/partition_par() on `hg` and return
+    the states it reached: dict with init_part/init_ends/init_parts and
+    part/ends/parts/cnt (after refinement), the wall time and the
+    'Refinement took' seconds it printed."""
+    exe = os.path.join(REF_DIR, f"ref_{metric}")
+    with tempfile.TemporaryDirectory() as d:
+        inst = os.path.join(d, "g.fphg")
+        out = os.path.join(d, "g.fprs")
+        hg.write_fphg(inst)
+        env = dict(os.environ)
+        if env_extra:
+            env.update(env_extra)
+        p = subprocess.run([exe, inst, str(K), str(passes), str(threads), out], capture_output=True, text=True,
+                           timeout=timeout, env=env)
+        if p.returncode != 0:
+            raise RuntimeError(f"{exe} failed rc={p.returncode}: {p.stderr[-2000:]}")
+        with open(out, "rb") as f:
+            raw = f.read()
+    assert raw[:4] == b"FPRS"
+    nov, noe, K_ = struct.unpack_from("<iii", raw, 4)
+    pins, wall = struct.unpack_from("<qd", raw, 16)
+    off = 32
+
+    def take(n):
+        nonlocal off
+        a = np.frombuffer(raw, dtype="<i4", count=n, offset=off).copy()
+        off += 4 * n
+        return a
+
+    res = {}
+    for tag in ("init_", ""):
+        res[tag + "part"] = take(nov)
+        res[tag + "ends"] = take(noe)
+        res[tag + "parts"] = take(pins)
+    res["cnt"] = take(pins)
+    res["wall"] = wall
+    m = re.findall(r"Refinement took : ([0-9.eE+-]+) seconds", p.stdout)
+    res["refine_seconds"] = float(m[-1]) if m else None
+    res["stdout"] = p.stdout
+    return res
