@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark: TV refinement throughput on one MI355X.
+
+Workload (one "step" = one refinement pass over the whole hypergraph):
+  column-net hypergraph of a 1,000,000 x 1,000,000 sparse matrix, total row
+  degree ~ Zipf(alpha=2.1) on [2, 5000] with the diagonal always present (net =
+  row, source = diagonal vertex, vertex = column), unit weights, K = 256,
+  eps = 0.03, balanced deterministic round schedule; initial partition = the
+  streaming LDG (host). Inputs are resident in HBM before the timed region.
+
+Metric: pin-visits/s = steps * pins / seconds (a pin-visit is one (vertex,
+incident net) incidence scored in a pass).
+
+Also reported on the same JSON line:
+  roofline      the evaluation kernel (k_tv_eval), timed live with HIP events
+                on the engine's stream, against the 8 TB/s HBM peak
+  cpu_baseline  the reference's own OpenMP refinement (partition_par,
+                TV/partition.cpp:512-981, built from its sources in oracle/_ref)
+                on the host cores, a bounded sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def lambda_visits(hg, lam) -> int:
+    sizes = np.diff(hg.net_ptr.astype(np.int64))
+    return int(np.dot(sizes, lam.astype(np.int64)))
+
+
+def cpu_baseline(hg, K, passes, threads):
+    from oracle import oracle as O
+
+    if not O.ref_available("tv"):
+        return None
+    # partition_par reads its per-thread move counts before every thread has
+    # written them (TV/partition.cpp:806-814; the array is uninitialized,
+    # :671), which can crash the reference; a crashed run is retried, then
+    # the thread count is halved.
+    crashes = []
+    t = threads
+    while t >= 1:
+        for _ in range(2):
+            env = {"OMP_PROC_BIND": "close", "OMP_PLACES": "cores", "OMP_NUM_THREADS": str(t)}
+            try:
+                r = O.run_reference("tv", hg, K, passes, t, timeout=900, env_extra=env)
+            except RuntimeError as e:
+                crashes.append(f"{t} threads: {str(e)[-60:]}")
+                continue
+            secs = r["refine_seconds"]
+            after = [ln for ln in r["stdout"].splitlines() if ln.startswith("After Refinement TV")]
+            note = f"; crashed runs before this one: {crashes}" if crashes else ""
+            return {
+                "value": passes * hg.pins / secs,
+                "unit": "pin-visits/s",
+                "cores": t,
+                "kind": "reference",
+                "sample": (f"reference partition_par (TV/partition.cpp:512-981, built from its sources) on the "
+                           f"same 1M-vertex instance, K={K}, its hard-coded eps=0.05 constants, {passes} passes, "
+                           f"{t} OpenMP threads; 'Refinement took' = {secs:.3f} s; "
+                           f"{after[-1] if after else ''}{note}"),
+            }
+        t //= 2
+    return {"error": f"reference crashed at every thread count: {crashes}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--round", type=int, default=262144, help="vertices per round")
+    ap.add_argument("--sched", choices=("bal", "ref"), default="bal")
+    ap.add_argument("--scale", type=float, default=1.0, help="fraction of the 1M-vertex workload")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU baseline")
+    ap.add_argument("--cpu-passes", type=int, default=3)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES three times (HBM counter calibration)")
+    args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+
+        src = torch.ones(args.copy_calibration // 4, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": args.copy_calibration, "ok": bool((dst[-1] == 1).item())}))
+        return
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from fastpartitioner_amd import hypergraph as H
+    from fastpartitioner_amd.engine import SCHED_BAL, SCHED_REF, Engine, schedule
+
+    K, eps = 256, 0.03
+    t = time.time()
+    hg = H.workload("c2", seed=1, scale=args.scale)
+    W = hg.total_weight()
+    C, max_pw, min_pw = H.ref_constants(W, K, eps)
+    cap = H.balance_cap(W, K, eps)
+    part0 = H.initial_partition(hg, K, C, max_pw, min_pw)
+    print(f"# workload: nov={hg.nov} pins={hg.pins} K={K} prep {time.time() - t:.1f}s", file=sys.stderr, flush=True)
+
+    if args.sched == "bal":
+        sched = schedule(SCHED_BAL, args.round, C, cap, 0, cap)
+    else:
+        sched = schedule(SCHED_REF, args.round, C, max_pw, min_pw)
+
+    with Engine(0) as eng:
+        eng.upload(hg)
+        eng.set_partition(K, part0)
+        if args.warmup:
+            eng.refine("TV", args.warmup, sched)
+            eng.set_partition(K, part0)
+        m0 = eng.metrics(vectors=True)
+        lam0 = lambda_visits(hg, m0["lam"])
+        eng.set_kernel_timing(True)
+        t0 = time.perf_counter()
+        st = eng.refine("TV", args.steps, sched)
+        t1 = time.perf_counter()
+        launches, eval_ms = eng.kernel_timing()
+        eng.set_kernel_timing(False)
+        m1 = eng.metrics(vectors=True)
+        lam1 = lambda_visits(hg, m1["lam"])
+        sizes = eng.get_state().sizes
+    secs = t1 - t0
+    value = args.steps * hg.pins / secs
+
+    # algorithmic bytes of the evaluation kernel per pass (packed u32 list entries):
+    # 16 B per vertex (vtx_ptr, part, slot_own/slot_dec), 12 B per incidence
+    # (vtx_nets, net_ptr, lambda), 4 B per list entry visited (Lambda)
+    lam_avg = 0.5 * (lam0 + lam1)
+    bytes_pass = 16 * hg.nov + 12 * hg.pins + 4 * lam_avg
+    avg_launch_ms = eval_ms / max(launches, 1)
+    bytes_launch = bytes_pass * args.steps / max(launches, 1)
+    achieved = bytes_launch / (avg_launch_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_tv_eval", "launches": launches, "avg_launch_us": round(avg_launch_ms * 1e3, 2),
+            "bytes_per_launch": int(bytes_launch)}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        try:
+            cpu = cpu_baseline(hg, K, args.cpu_passes, threads)
+        except Exception as e:  # reported, never fatal for the GPU number
+            cpu = {"error": str(e)[:300]}
+
+    out = {
+        "metric": "refine pin-visits/s (1M-vtx, K=256, TV)",
+        "value": round(value, 1),
+        "unit": "pin-visits/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(secs * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "colnet-zipf2.1-1M-K256-TV", "nov": hg.nov, "pins": hg.pins, "K": K,
+                   "eps": eps, "schedule": args.sched, "round": args.round, "passes": args.steps,
+                   "lambda_visits_per_pass": int(lam_avg),
+                   "tv_initial": int(m0["TV"]), "tv_refined": int(m1["TV"]),
+                   "max_part": int(sizes.max()), "cap": int(cap),
+                   "moves": int(st.moves), "rounds": int(st.rounds),
+                   "device_ms": round(st.device_ms, 3)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if cpu and "value" in cpu:
+        out["gpu_over_cpu"] = round(value / cpu["value"], 2)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -13,7 +13,7 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 HOST_LIB = os.path.join(_HERE, "libfastpart_host.so")
-ENGINE_LIB = os.path.join(_HERE, "libfastpart.so")
+ENGINE_LIB = os.environ.get("FP_ENGINE_LIB") or os.path.join(_HERE, "libfastpart.so")
 
 FP_OK = 0
 STATUS_NAMES = {0: "FP_OK", 1: "FP_ERR_ARG", 2: "FP_ERR_HIP", 3: "FP_ERR_OOM", 4: "FP_ERR_STATE",

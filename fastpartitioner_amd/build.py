@@ -1,7 +1,7 @@
 """In-tree build of the package's native libraries.
 
   libfastpart_host.so  g++    csrc/fp_host.cpp         (host helpers)
-  libfastpart.so       hipcc  csrc/fp_engine.hip       (HIP engine, gfx950)
+  libfastpart.so       hipcc  csrc/fp_{context,tv,metrics}.hip  (HIP engine, gfx950)
 
 `python -m fastpartitioner_amd.build` builds both.
 """
@@ -25,6 +25,9 @@ HIP_FLAGS = ["-O3", "-fPIC", "-shared", "-std=c++17", "-fopenmp", "-ffp-contract
              f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
+ENGINE_SOURCES = ["fp_context.hip", "fp_tv.hip", "fp_metrics.hip"]
+
+
 def _run(cmd):
     print("+", " ".join(cmd), flush=True)
     p = subprocess.run(cmd, capture_output=True, text=True)
@@ -42,7 +45,8 @@ def build_host():
 
 def build_engine():
     out = os.path.join(HERE, "libfastpart.so")
-    _run([HIPCC, *HIP_FLAGS, "-o", out, os.path.join(CSRC, "fp_engine.hip")])
+    srcs = [os.path.join(CSRC, f) for f in ENGINE_SOURCES]
+    _run([HIPCC, *HIP_FLAGS, "-o", out, *srcs])
     return out
 
 

@@ -1,0 +1,414 @@
+// fp_context.hip -- context lifetime, hypergraph upload and connectivity state
+// transfer of the HIP engine (include/fastpart.h).
+#include "fp_internal.h"
+
+namespace fpe {
+
+fp_status fail(fp_ctx* c, fp_status st, const char* fmt, ...) {
+  if (c) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(c->err, sizeof c->err, fmt, ap);
+    va_end(ap);
+  }
+  return st;
+}
+
+fp_status dalloc_bytes(fp_ctx* ctx, void** p, size_t bytes) {
+  *p = nullptr;
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(ctx, FP_ERR_OOM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+  }
+  ctx->owned.push_back(q);
+  ctx->dev_bytes += (long long)bytes;
+  *p = q;
+  return FP_OK;
+}
+
+void dfree_all(fp_ctx* ctx) {
+  for (void* p : ctx->owned) (void)hipFree(p);
+  ctx->owned.clear();
+  ctx->dev_bytes = 0;
+}
+
+hipEvent_t pooled_event(fp_ctx* ctx, size_t i) {
+  while (ctx->ev_pool.size() <= i) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    ctx->ev_pool.push_back(e);
+  }
+  return ctx->ev_pool[i];
+}
+
+namespace {
+
+int bits_for(int K) {
+  int b = 1;
+  while ((1 << b) < K) b++;
+  return b;
+}
+
+// K-sized arrays and the packed-entry layout for K parts
+fp_status ensure_k(fp_ctx* ctx, int K) {
+  if (K < 1) return fail(ctx, FP_ERR_ARG, "K must be >= 1");
+  if (K > 16384) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d above 16384", K);  // parts fit the 15-bit view
+  const int pb = bits_for(K);
+  if ((unsigned long long)ctx->max_net >= (1ull << (32 - pb)))
+    return fail(ctx, FP_ERR_UNSUPPORTED, "a net of %u pins is too large for K = %d in the packed list layout",
+                ctx->max_net, K);
+  if (ctx->K != K) {
+    // K-sized arrays are (re)allocated; earlier ones stay owned until the next upload
+    FP_TRY(dalloc(ctx, &ctx->sizes, (size_t)K));
+    FP_TRY(dalloc(ctx, &ctx->sizes_frozen, (size_t)K));
+    FP_TRY(dalloc(ctx, &ctx->cand_tot, (size_t)K));
+    FP_TRY(dalloc(ctx, &ctx->over_idx, (size_t)K));
+    FP_TRY(dalloc(ctx, &ctx->p2p, ((size_t)K * (size_t)K + 31) / 32));
+    FP_TRY(dalloc(ctx, &ctx->mvec, (size_t)5 * (size_t)K + 2));
+    fill_async<long long>(ctx, ctx->cand_tot, (size_t)K, 0);
+    ctx->K = K;
+    ctx->hist_cap = 0;  // hist is K x tiles
+  }
+  ctx->pb = pb;
+  ctx->pmask = (1u << pb) - 1u;
+  return FP_OK;
+}
+
+// ent: the segment array (lambda + list per net), host copy of the device layout
+fp_status upload_state(fp_ctx* ctx, const int* part, const std::vector<unsigned>& ent, const long long* sizes_in) {
+  std::vector<long long> sz((size_t)ctx->K, 0);
+  if (sizes_in) {
+    std::copy(sizes_in, sizes_in + ctx->K, sz.begin());
+  } else {
+    for (int v = 0; v < ctx->nov; v++) sz[(size_t)part[v]] += ctx->h_vwgt.empty() ? 1 : ctx->h_vwgt[(size_t)v];
+  }
+  FP_TRY(h2d(ctx, ctx->part, part, (size_t)ctx->nov));
+  FP_TRY(h2d(ctx, ctx->ent, ent.data(), ent.size()));
+  // evaluation view of every segment
+  std::vector<unsigned short> ev(ent.size(), 0);
+  for (int n = 0; n < ctx->noe; n++) {
+    const size_t seg = (size_t)ctx->h_net_ptr[(size_t)n] + (size_t)n;
+    const unsigned L = ent[seg];
+    ev[seg] = (unsigned short)L;
+    for (unsigned t = 0; t < L; t++) ev[seg + 1 + t] = ev_word(ent[seg + 1 + t], ctx->pb, ctx->pmask);
+  }
+  FP_TRY(h2d(ctx, ctx->ev, ev.data(), ev.size()));
+  FP_TRY(h2d(ctx, ctx->sizes, sz.data(), (size_t)ctx->K));
+  FP_HIP_OK(hipStreamSynchronize(ctx->stream));
+  ctx->has_state = true;
+  return FP_OK;
+}
+
+}  // namespace
+}  // namespace fpe
+
+using namespace fpe;
+
+extern "C" {
+
+fp_status fp_device_count(int* count) {
+  if (!count) return FP_ERR_ARG;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    *count = 0;
+    return FP_ERR_NODEV;
+  }
+  *count = n;
+  return n > 0 ? FP_OK : FP_ERR_NODEV;
+}
+
+fp_status fp_ctx_create(fp_ctx** out, int device) {
+  if (!out) return FP_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    return FP_ERR_NODEV;
+  }
+  if (device < 0 || device >= n) return FP_ERR_ARG;
+  fp_ctx* ctx = new fp_ctx();
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    delete ctx;
+    return FP_ERR_HIP;
+  }
+  *out = ctx;
+  return FP_OK;
+}
+
+void fp_ctx_destroy(fp_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  dfree_all(ctx);
+  for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* fp_ctx_last_error(const fp_ctx* ctx) { return ctx ? ctx->err : "null context"; }
+
+long long fp_ctx_device_bytes(const fp_ctx* ctx) { return ctx ? ctx->dev_bytes : 0; }
+
+fp_status fp_upload(fp_ctx* ctx, int nov, int noe, const unsigned* net_ptr, const unsigned* net_pins,
+                    const unsigned* vtx_ptr, const unsigned* vtx_nets, const int* src, const int* vwgt,
+                    const int* ordering) {
+  if (!ctx) return FP_ERR_ARG;
+  if (nov < 1 || noe < 0 || !net_ptr) return fail(ctx, FP_ERR_ARG, "fp_upload: bad sizes or null arrays");
+  const long long pins = net_ptr[noe];
+  if (!net_pins && pins > 0) return fail(ctx, FP_ERR_ARG, "fp_upload: null net_pins");
+  FP_HIP_OK(hipSetDevice(ctx->device));
+  FP_HIP_OK(hipStreamSynchronize(ctx->stream));
+  dfree_all(ctx);
+  ctx->has_graph = ctx->has_state = false;
+  ctx->K = 0;
+  ctx->rmax = 0;
+  ctx->hist_cap = 0;
+  if (net_ptr[0] != 0) return fail(ctx, FP_ERR_ARG, "fp_upload: net_ptr[0] != 0");
+  unsigned max_net = 0;
+  for (int n = 0; n < noe; n++) {
+    if (net_ptr[n + 1] < net_ptr[n]) return fail(ctx, FP_ERR_ARG, "fp_upload: net_ptr not monotone at %d", n);
+    max_net = std::max(max_net, net_ptr[n + 1] - net_ptr[n]);
+  }
+  for (long long k = 0; k < pins; k++)
+    if (net_pins[k] >= (unsigned)nov) return fail(ctx, FP_ERR_ARG, "fp_upload: pin %lld out of range", k);
+  if (src)
+    for (int n = 0; n < noe; n++)
+      if (src[n] < -1 || src[n] >= nov) return fail(ctx, FP_ERR_ARG, "fp_upload: src[%d] out of range", n);
+  long long tw = 0;
+  ctx->h_vwgt.clear();
+  if (vwgt) {
+    for (int v = 0; v < nov; v++) {
+      if (vwgt[v] < 0) return fail(ctx, FP_ERR_ARG, "fp_upload: negative weight at %d", v);
+      tw += vwgt[v];
+    }
+    ctx->h_vwgt.assign(vwgt, vwgt + nov);
+  } else {
+    tw = nov;
+  }
+  std::vector<int> rank((size_t)nov);
+  ctx->h_order.resize((size_t)nov);
+  bool ident = true;
+  if (ordering) {
+    std::vector<char> seen((size_t)nov, 0);
+    for (int z = 0; z < nov; z++) {
+      const int v = ordering[z];
+      if (v < 0 || v >= nov || seen[(size_t)v]) return fail(ctx, FP_ERR_ARG, "fp_upload: ordering is not a permutation");
+      seen[(size_t)v] = 1;
+      rank[(size_t)v] = z;
+      ctx->h_order[(size_t)z] = v;
+      ident = ident && v == z;
+    }
+  } else {
+    std::iota(rank.begin(), rank.end(), 0);
+    std::iota(ctx->h_order.begin(), ctx->h_order.end(), 0);
+  }
+  ctx->nov = nov;
+  ctx->noe = noe;
+  ctx->pins = pins;
+  ctx->total_weight = tw;
+  ctx->max_net = max_net;
+  ctx->h_net_ptr.assign(net_ptr, net_ptr + noe + 1);
+  // vertex->nets view
+  std::vector<unsigned> h_vptr, h_vnets;
+  if (vtx_ptr && vtx_nets) {
+    if (vtx_ptr[0] != 0 || (long long)vtx_ptr[nov] != pins) return fail(ctx, FP_ERR_ARG, "fp_upload: vtx_ptr inconsistent");
+    for (int v = 0; v < nov; v++)
+      if (vtx_ptr[v + 1] < vtx_ptr[v]) return fail(ctx, FP_ERR_ARG, "fp_upload: vtx_ptr not monotone");
+    for (long long k = 0; k < pins; k++)
+      if (vtx_nets[k] >= (unsigned)noe) return fail(ctx, FP_ERR_ARG, "fp_upload: vtx_nets out of range");
+    h_vptr.assign(vtx_ptr, vtx_ptr + nov + 1);
+    h_vnets.assign(vtx_nets, vtx_nets + pins);
+  } else {
+    // counting-sort transpose: ascending nets per vertex (TV/tv.cpp:79-100)
+    h_vptr.assign((size_t)nov + 1, 0);
+    h_vnets.assign((size_t)pins, 0);
+    for (long long k = 0; k < pins; k++) h_vptr[net_pins[k] + 1]++;
+    for (int v = 0; v < nov; v++) h_vptr[(size_t)v + 1] += h_vptr[(size_t)v];
+    std::vector<unsigned> fill(h_vptr.begin(), h_vptr.end() - 1);
+    for (int n = 0; n < noe; n++)
+      for (unsigned k = net_ptr[n]; k < net_ptr[n + 1]; k++) h_vnets[fill[net_pins[k]]++] = (unsigned)n;
+  }
+  unsigned max_vdeg = 0;
+  for (int v = 0; v < nov; v++) max_vdeg = std::max(max_vdeg, h_vptr[(size_t)v + 1] - h_vptr[(size_t)v]);
+  ctx->max_vdeg = max_vdeg;
+  // per net: ordering ranks of its pins, ascending (the apply replays moves in this order)
+  ctx->h_rank_pins.resize((size_t)pins);
+  bool dup = false;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(|| : dup)
+  for (int n = 0; n < noe; n++) {
+    int* r = ctx->h_rank_pins.data() + net_ptr[n];
+    const int len = (int)(net_ptr[n + 1] - net_ptr[n]);
+    bool sorted = true;
+    for (int k = 0; k < len; k++) {
+      r[k] = rank[net_pins[net_ptr[n] + (unsigned)k]];
+      if (k && r[k] <= r[k - 1]) sorted = false;
+    }
+    if (!sorted) {
+      std::sort(r, r + len);
+      for (int k = 1; k < len; k++)
+        if (r[k] == r[k - 1]) dup = true;
+    }
+  }
+  ctx->dup_pins = dup;
+
+  FP_TRY(dalloc(ctx, &ctx->net_ptr, (size_t)noe + 1));
+  FP_TRY(dalloc(ctx, &ctx->net_pins, (size_t)pins));
+  FP_TRY(dalloc(ctx, &ctx->vtx_ptr, (size_t)nov + 1));
+  FP_TRY(dalloc(ctx, &ctx->vtx_nets, (size_t)pins));
+  FP_TRY(dalloc(ctx, &ctx->apply_pins, (size_t)pins));
+  FP_TRY(dalloc(ctx, &ctx->src, (size_t)std::max(noe, 1)));
+  FP_TRY(h2d(ctx, ctx->net_ptr, net_ptr, (size_t)noe + 1));
+  FP_TRY(h2d(ctx, ctx->net_pins, net_pins, (size_t)pins));
+  FP_TRY(h2d(ctx, ctx->vtx_ptr, h_vptr.data(), (size_t)nov + 1));
+  FP_TRY(h2d(ctx, ctx->vtx_nets, h_vnets.data(), (size_t)pins));
+  FP_TRY(h2d(ctx, ctx->apply_pins, ctx->h_rank_pins.data(), (size_t)pins));
+  if (src) {
+    FP_TRY(h2d(ctx, ctx->src, src, (size_t)noe));
+  } else {
+    fill_async<int>(ctx, ctx->src, (size_t)std::max(noe, 1), -1);
+  }
+  ctx->vwgt = nullptr;
+  if (vwgt) {
+    FP_TRY(dalloc(ctx, &ctx->vwgt, (size_t)nov));
+    FP_TRY(h2d(ctx, ctx->vwgt, vwgt, (size_t)nov));
+  }
+  ctx->order = nullptr;
+  if (!ident) {
+    FP_TRY(dalloc(ctx, &ctx->order, (size_t)nov));
+    FP_TRY(h2d(ctx, ctx->order, ordering, (size_t)nov));
+  }
+  // state and scratch that do not depend on K
+  FP_TRY(dalloc(ctx, &ctx->part, (size_t)nov));
+  FP_TRY(dalloc(ctx, &ctx->ent, (size_t)pins + (size_t)noe + ENT_PAD));
+  FP_TRY(dalloc(ctx, &ctx->ev, (size_t)pins + (size_t)noe + ENT_PAD));
+  fill_async<unsigned>(ctx, ctx->ent, (size_t)pins + (size_t)noe + ENT_PAD, 0u);
+  fill_async<unsigned short>(ctx, ctx->ev, (size_t)pins + (size_t)noe + ENT_PAD, (unsigned short)0);
+  {
+    // per incidence: the segment of its net
+    std::vector<unsigned> seg((size_t)pins);
+    for (long long k = 0; k < pins; k++) seg[(size_t)k] = net_ptr[h_vnets[(size_t)k]] + h_vnets[(size_t)k];
+    FP_TRY(dalloc(ctx, &ctx->vtx_seg, (size_t)pins));
+    FP_TRY(h2d(ctx, ctx->vtx_seg, seg.data(), (size_t)pins));
+    FP_HIP_OK(hipStreamSynchronize(ctx->stream));
+  }
+  FP_TRY(dalloc(ctx, &ctx->net_stamp, (size_t)std::max(noe, 1)));
+  FP_TRY(dalloc(ctx, &ctx->touched, (size_t)std::max(noe, 1)));
+  FP_TRY(dalloc(ctx, &ctx->counters, 4));
+  FP_TRY(dalloc(ctx, &ctx->moves, 1));
+  FP_TRY(dalloc(ctx, &ctx->over_count, 1));
+  fill_async<unsigned>(ctx, ctx->net_stamp, (size_t)std::max(noe, 1), 0u);
+  ctx->tag = 0;
+  FP_HIP_OK(hipGetLastError());
+  FP_HIP_OK(hipStreamSynchronize(ctx->stream));
+  ctx->has_graph = true;
+  return FP_OK;
+}
+
+fp_status fp_set_partition(fp_ctx* ctx, int K, const int* part) {
+  if (!ctx) return FP_ERR_ARG;
+  if (!ctx->has_graph) return fail(ctx, FP_ERR_STATE, "fp_set_partition before fp_upload");
+  if (!part) return fail(ctx, FP_ERR_ARG, "null part");
+  FP_HIP_OK(hipSetDevice(ctx->device));
+  for (int v = 0; v < ctx->nov; v++)
+    if (part[v] < 0 || part[v] >= K) return fail(ctx, FP_ERR_ARG, "part[%d] = %d out of [0, K)", v, part[v]);
+  FP_TRY(ensure_k(ctx, K));
+  const int noe = ctx->noe;
+  std::vector<unsigned> ent((size_t)ctx->pins + (size_t)noe + ENT_PAD, 0);
+  const unsigned inc = 1u << ctx->pb;
+  // list order = first appearance of each part along the ordering (the state
+  // the streaming initial partition leaves, TV/partition.cpp:240-263): walk
+  // each net's pins in rank order and append unseen parts
+#pragma omp parallel
+  {
+    std::vector<int> slot((size_t)K, -1), owner((size_t)K, -1);
+#pragma omp for schedule(dynamic, 1024)
+    for (int n = 0; n < noe; n++) {
+      const unsigned s = ctx->h_net_ptr[(size_t)n], e = ctx->h_net_ptr[(size_t)n + 1];
+      unsigned* list = ent.data() + s + (unsigned)n + 1;
+      int L = 0;
+      for (unsigned k = s; k < e; k++) {
+        const int q = part[ctx->h_order[(size_t)ctx->h_rank_pins[k]]];
+        if (owner[(size_t)q] != n) {
+          owner[(size_t)q] = n;
+          slot[(size_t)q] = L;
+          list[L] = inc | (unsigned)q;
+          L++;
+        } else {
+          list[slot[(size_t)q]] += inc;
+        }
+      }
+      list[-1] = (unsigned)L;
+    }
+  }
+  return upload_state(ctx, part, ent, nullptr);
+}
+
+fp_status fp_set_state(fp_ctx* ctx, int K, const int* part, const int* ep_ends, const int* ep_parts,
+                       const int* ep_cnt, const long long* sizes) {
+  if (!ctx) return FP_ERR_ARG;
+  if (!ctx->has_graph) return fail(ctx, FP_ERR_STATE, "fp_set_state before fp_upload");
+  if (!part || !ep_ends || !ep_parts || !ep_cnt) return fail(ctx, FP_ERR_ARG, "null state array");
+  FP_HIP_OK(hipSetDevice(ctx->device));
+  for (int v = 0; v < ctx->nov; v++)
+    if (part[v] < 0 || part[v] >= K) return fail(ctx, FP_ERR_ARG, "part[%d] = %d out of [0, K)", v, part[v]);
+  FP_TRY(ensure_k(ctx, K));
+  std::vector<unsigned> ent((size_t)ctx->pins + (size_t)ctx->noe + ENT_PAD, 0);
+  for (int n = 0; n < ctx->noe; n++) {
+    const unsigned s = ctx->h_net_ptr[(size_t)n], len = ctx->h_net_ptr[(size_t)n + 1] - s;
+    const int L = ep_ends[n];
+    if (L < 0 || (unsigned)L > len) return fail(ctx, FP_ERR_ARG, "ep_ends[%d] out of range", n);
+    unsigned* seg = ent.data() + s + (unsigned)n;
+    seg[0] = (unsigned)L;
+    for (int j = 0; j < L; j++) {
+      const int q = ep_parts[s + (unsigned)j], c = ep_cnt[s + (unsigned)j];
+      if (q < 0 || q >= K || c < 1 || (unsigned)c > len)
+        return fail(ctx, FP_ERR_ARG, "list entry %d of net %d invalid", j, n);
+      seg[1 + j] = ((unsigned)c << ctx->pb) | (unsigned)q;
+    }
+  }
+  return upload_state(ctx, part, ent, sizes);
+}
+
+fp_status fp_get_state(fp_ctx* ctx, int* part, int* ep_ends, int* ep_parts, int* ep_cnt, long long* sizes) {
+  if (!ctx) return FP_ERR_ARG;
+  if (!ctx->has_state) return fail(ctx, FP_ERR_STATE, "no state");
+  FP_HIP_OK(hipSetDevice(ctx->device));
+  std::vector<unsigned> ent((size_t)ctx->pins + (size_t)ctx->noe);
+  FP_TRY(d2h(ctx, ent.data(), ctx->ent, ent.size()));
+  if (part) FP_TRY(d2h(ctx, part, ctx->part, (size_t)ctx->nov));
+  if (sizes) FP_TRY(d2h(ctx, sizes, ctx->sizes, (size_t)ctx->K));
+  FP_HIP_OK(hipStreamSynchronize(ctx->stream));
+  for (int n = 0; n < ctx->noe; n++) {
+    const unsigned s = ctx->h_net_ptr[(size_t)n], e = ctx->h_net_ptr[(size_t)n + 1];
+    const unsigned* seg = ent.data() + s + (unsigned)n;
+    const int L = (int)seg[0];
+    if (ep_ends) ep_ends[n] = L;
+    for (unsigned k = s; k < e; k++) {
+      const int j = (int)(k - s);
+      const bool live = j < L;
+      if (ep_parts) ep_parts[k] = live ? (int)(seg[1 + j] & ctx->pmask) : -1;
+      if (ep_cnt) ep_cnt[k] = live ? (int)(seg[1 + j] >> ctx->pb) : 0;
+    }
+  }
+  return FP_OK;
+}
+
+fp_status fp_set_kernel_timing(fp_ctx* ctx, int enable) {
+  if (!ctx) return FP_ERR_ARG;
+  ctx->kernel_timing = enable != 0;
+  return FP_OK;
+}
+
+fp_status fp_last_kernel_timing(fp_ctx* ctx, long long* launches, double* total_ms) {
+  if (!ctx) return FP_ERR_ARG;
+  if (launches) *launches = ctx->timed_launches;
+  if (total_ms) *total_ms = ctx->timed_ms;
+  return FP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,278 @@
+// fp_internal.h -- shared internals of the HIP engine (not part of the C-ABI).
+//
+// Device layout of one context (all in HBM, allocated once per upload):
+//   net_ptr[noe+1], net_pins[pins]   net->pins CSR (reader order)
+//   vtx_ptr[nov+1], vtx_nets[pins]   vertex->nets CSR (ascending nets)
+//   apply_pins[pins]                 per net, the ordering ranks of its pins, ascending
+//   src[noe], vwgt[nov], order[nov]  net sources, vertex weights (nullable), visiting order (nullable)
+//   part[nov]                        part of each vertex
+//   ent[pins + noe + 64]             net n's segment starts at seg(n) = net_ptr[n] + n:
+//                                    ent[seg] = lambda(n), then its connectivity list,
+//                                    one u32 per entry: (pin count << pb) | part,
+//                                    pb = ceil(log2 K); 64 words of tail padding let a
+//                                    wavefront read a whole 64-word window of any segment
+//   ev[pins + noe + 64] (u16)        evaluation view of the same segments: ev[seg] =
+//                                    lambda(n), then per entry part | (count == 1) << 15;
+//                                    a 64-lane window of it is one 128-byte line
+//   vtx_seg[pins]                    per (vertex, net) incidence: seg(net), so one random
+//                                    read brings a net's lambda and first 63 entries
+//   sizes[K] (int64)                 part weights
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "../../include/fastpart.h"
+
+namespace fpe {
+
+constexpr int WAVE = 64;
+constexpr int LDS_BUDGET = 64 * 1024;
+constexpr unsigned NOFIRST = 0xFFFFFFFFu;
+constexpr int ENT_PAD = 64;  // tail padding of the segment arrays
+constexpr unsigned EV_SINGLE = 0x8000u;  // evaluation view: the part has one pin in the net
+
+// evaluation-view word of a packed (count << pb | part) list entry
+__host__ __device__ __forceinline__ unsigned short ev_word(unsigned x, int pb, unsigned pmask) {
+  return (unsigned short)((x & pmask) | ((x >> pb) == 1u ? EV_SINGLE : 0u));
+}
+
+// ---- wave helpers ---------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
+// index of the wavefront in its block, as a wave-uniform (scalar) value
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE)); }
+
+// orders the LDS traffic of one wavefront: everything before is visible to
+// every lane of the wave after
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x) {
+  for (int off = WAVE / 2; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  return x;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T x) {
+  const int lane = lane_id();
+  for (int off = 1; off < WAVE; off <<= 1) {
+    const T y = __shfl_up(x, off);
+    if (lane >= off) x += y;
+  }
+  return x;
+}
+
+// Exclusive scan over a whole block (blockDim.x multiple of 64, <= 1024).
+// `scratch` holds blockDim.x/64 values. Returns the exclusive prefix; *total
+// receives the block total. Must be called by every thread of the block.
+template <typename T>
+__device__ __forceinline__ T block_excl_scan(T x, T* scratch, T* total) {
+  const int lane = lane_id(), wave = (int)(threadIdx.x / WAVE), nw = (int)(blockDim.x / WAVE);
+  const T incl = wave_incl_scan(x);
+  if (lane == WAVE - 1) scratch[wave] = incl;
+  __syncthreads();
+  T base = 0, tot = 0;
+  for (int w = 0; w < nw; w++) {
+    const T s = scratch[w];
+    if (w < wave) base += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + incl - x;
+}
+
+// 64-bit lane shuffle by a DPP pattern (no LDS traffic). Rows not in ROW_MASK
+// keep their own value.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v) {
+  const int lo = (int)(unsigned)(v & 0xFFFFFFFFull), hi = (int)(unsigned)(v >> 32);
+  const int rlo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xF, false);
+  const int rhi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xF, false);
+  return ((unsigned long long)(unsigned)rhi << 32) | (unsigned)rlo;
+}
+
+__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long v, int l) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v & 0xFFFFFFFFull), l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), l);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// Wave-wide max / min of a 64-bit key through DPP: quad_perm [1,0,3,2] and
+// [2,3,0,1], row_ror 4 and 8 (row-wide), row_bcast15 into rows 1 and 3,
+// row_bcast31 into rows 2 and 3; lane 63 then holds the result.
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long x) {
+  x = max(x, dpp_u64<0xB1>(x));
+  x = max(x, dpp_u64<0x4E>(x));
+  x = max(x, dpp_u64<0x124>(x));
+  x = max(x, dpp_u64<0x128>(x));
+  x = max(x, dpp_u64<0x142, 0xA>(x));
+  x = max(x, dpp_u64<0x143, 0xC>(x));
+  return readlane_u64(x, WAVE - 1);
+}
+
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long x) {
+  x = min(x, dpp_u64<0xB1>(x));
+  x = min(x, dpp_u64<0x4E>(x));
+  x = min(x, dpp_u64<0x124>(x));
+  x = min(x, dpp_u64<0x128>(x));
+  x = min(x, dpp_u64<0x142, 0xA>(x));
+  x = min(x, dpp_u64<0x143, 0xC>(x));
+  return readlane_u64(x, WAVE - 1);
+}
+
+// first index k in [lo, hi) with arr[k] >= key (arr ascending); wave-uniform result
+__device__ __forceinline__ unsigned wave_lower_bound(const int* __restrict__ arr, unsigned lo, unsigned hi, int key) {
+  const int lane = lane_id();
+  while (hi - lo > WAVE) {
+    const unsigned step = (hi - lo + WAVE - 1) / WAVE;
+    const unsigned k = lo + (unsigned)lane * step;
+    const bool lt = k < hi && arr[k] < key;
+    const int c = __popcll(__ballot(lt));
+    if (c == 0) return lo;
+    const unsigned nlo = lo + (unsigned)(c - 1) * step + 1;
+    const unsigned nhi = min(hi, lo + (unsigned)c * step);
+    lo = nlo;
+    hi = nhi;
+  }
+  const unsigned k = lo + (unsigned)lane;
+  const bool lt = k < hi && arr[k] < key;
+  return lo + (unsigned)__popcll(__ballot(lt));
+}
+
+// index of the entry of part p in list[0..L) or -1; wave-uniform
+__device__ __forceinline__ int wave_find(const unsigned* list, int L, int p, unsigned pmask) {
+  const int lane = lane_id();
+  for (int t0 = 0; t0 < L; t0 += WAVE) {
+    const int t = t0 + lane;
+    const bool hit = t < L && (int)(list[t] & pmask) == p;
+    const unsigned long long m = __ballot(hit);
+    if (m) return t0 + __ffsll((long long)m) - 1;
+  }
+  return -1;
+}
+
+__device__ __forceinline__ void atomic_add_i64(long long* p, long long v) {
+  atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
+}
+
+template <typename T>
+__global__ void k_fill(T* p, size_t n, T v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+}  // namespace fpe
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct fp_ctx {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  char err[512] = {0};
+  long long dev_bytes = 0;
+  std::vector<void*> owned;
+
+  // hypergraph
+  int nov = 0, noe = 0;
+  long long pins = 0;
+  unsigned *net_ptr = nullptr, *net_pins = nullptr, *vtx_ptr = nullptr, *vtx_nets = nullptr;
+  int *src = nullptr, *vwgt = nullptr, *order = nullptr, *apply_pins = nullptr;
+  bool has_graph = false, dup_pins = false;
+  long long total_weight = 0;
+  unsigned max_net = 0, max_vdeg = 0;
+  std::vector<unsigned> h_net_ptr;
+  std::vector<int> h_rank_pins;  // per net, ordering ranks of its pins ascending
+  std::vector<int> h_order;      // rank -> vertex
+  std::vector<int> h_vwgt;
+
+  // state
+  int K = 0, pb = 0;
+  unsigned pmask = 0;
+  int* part = nullptr;
+  unsigned* ent = nullptr;
+  unsigned short* ev = nullptr;
+  unsigned* vtx_seg = nullptr;
+  long long* sizes = nullptr;
+  bool has_state = false;
+
+  // round scratch
+  int rmax = 0;
+  int *slot_own = nullptr, *slot_dec = nullptr, *touched = nullptr, *counters = nullptr;
+  unsigned* net_stamp = nullptr;
+  unsigned tag = 0;
+  long long *cand_tot = nullptr, *sizes_frozen = nullptr, *pref = nullptr;
+  int* hist = nullptr;
+  size_t hist_cap = 0;
+  int *over_idx = nullptr, *over_count = nullptr;
+  unsigned long long* moves = nullptr;
+
+  // metric scratch
+  unsigned* p2p = nullptr;
+  unsigned long long* mvec = nullptr;  // 5*K + 2
+
+  // timing
+  bool kernel_timing = false;
+  long long timed_launches = 0;
+  double timed_ms = 0;
+  std::vector<hipEvent_t> ev_pool;
+};
+
+namespace fpe {
+
+fp_status fail(fp_ctx* c, fp_status st, const char* fmt, ...);
+fp_status dalloc_bytes(fp_ctx* ctx, void** p, size_t bytes);
+void dfree_all(fp_ctx* ctx);
+hipEvent_t pooled_event(fp_ctx* ctx, size_t i);
+
+template <typename T>
+fp_status dalloc(fp_ctx* ctx, T** p, size_t count) {
+  void* q = nullptr;
+  fp_status s = dalloc_bytes(ctx, &q, (count ? count : 1) * sizeof(T));
+  *p = static_cast<T*>(q);
+  return s;
+}
+
+#define FP_HIP_OK(call)                                                                              \
+  do {                                                                                               \
+    hipError_t e_ = (call);                                                                          \
+    if (e_ != hipSuccess) return ::fpe::fail(ctx, FP_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+#define FP_TRY(x)               \
+  do {                          \
+    fp_status s_ = (x);         \
+    if (s_ != FP_OK) return s_; \
+  } while (0)
+
+template <typename T>
+fp_status h2d(fp_ctx* ctx, T* d, const T* h, size_t count) {
+  if (count == 0) return FP_OK;
+  FP_HIP_OK(hipMemcpyAsync(d, h, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  return FP_OK;
+}
+template <typename T>
+fp_status d2h(fp_ctx* ctx, T* h, const T* d, size_t count) {
+  if (count == 0) return FP_OK;
+  FP_HIP_OK(hipMemcpyAsync(h, d, count * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+  return FP_OK;
+}
+template <typename T>
+void fill_async(fp_ctx* ctx, T* p, size_t n, T v) {
+  if (n == 0) return;
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+  k_fill<T><<<blocks, 256, 0, ctx->stream>>>(p, n, v);
+}
+
+}  // namespace fpe

@@ -1,0 +1,711 @@
+// fp_tv.hip -- TV refinement on the device (TV/partition.cpp:307-440 decisions,
+// TV/partition.cpp:669-896 round schedule).
+//
+// One pass = rounds of R consecutive ordering positions. A round is:
+//   k_tv_eval    one wavefront per vertex (grid-stride over the round,
+//                software-pipelined over the wave's vertices) sweeps the lists
+//                of the vertex's nets in their 16-bit evaluation view -- one
+//                128-byte read per net brings its lambda and first 63
+//                entries, 8 nets in flight -- into LDS histograms
+//                (pin count + first-appearance rank per part), then picks the
+//                LDG2 decision by a lexicographic wave reduction and applies
+//                the TV gain test. Candidates stamp their nets into the
+//                round's touched-net list; BAL also accumulates candidate
+//                weights per part and per (part, 64-slot tile).
+//   k_bal_scan   BAL: parts whose candidate weight exceeds cap - size get a
+//                dense index; for each, an exclusive scan over tiles
+//   k_bal_admit  BAL: prefix admission per part in ordering order, commit of
+//                part vector and sizes
+//   k_apply      one wavefront per touched net replays the round's moves on
+//                its list in ordering order (plus, for REF, the commit of the
+//                part vector and the source-part sizes in extra blocks)
+#include "fp_internal.h"
+
+namespace fpe {
+namespace {
+
+constexpr int EVAL_BLOCK = 256;  // 4 wavefronts
+constexpr int TILE = 64;         // slots per admission tile
+#ifndef FP_NETS_IN_FLIGHT
+#define FP_NETS_IN_FLIGHT 8
+#endif
+constexpr int NETS_IN_FLIGHT = FP_NETS_IN_FLIGHT;  // segment head windows loaded together per wave
+constexpr int LONG_WINDOWS = 4;                    // windows of one long list loaded together
+constexpr int APPLY_BLOCK = 256;
+constexpr int APPLY_NET_BLOCKS = 2048;
+
+struct EvalArgs {
+  const unsigned* vtx_ptr;
+  const unsigned* vtx_nets;
+  const unsigned* vtx_seg;
+  const unsigned short* ev;
+  const int* part;
+  const int* order;  // nullable: identity
+  const int* vwgt;   // nullable: unit
+  const long long* sizes;
+  int K, pb;
+  unsigned pmask;
+  int r0, nslots, ntiles;
+  double C;
+  long long max_pw, min_pw;
+  int* slot_own;
+  int* slot_dec;
+  int* dec_out;   // nullable (fp_eval_moves)
+  int* gain_out;  // nullable
+  // touched-net list of the round (nullable: no stamping)
+  unsigned tag;
+  unsigned* net_stamp;
+  int* touched;
+  int* tcount;
+  int* tcount_next;
+  // BAL (nullable): candidate weight per part, per (part, tile); frozen sizes copy
+  long long* cand_tot;
+  int* hist;
+  long long* sizes_frozen;
+};
+
+__device__ __forceinline__ bool better(double sc, long long sz, unsigned fi, double bsc, long long bsz, unsigned bfi) {
+  return sc > bsc || (sc == bsc && (sz < bsz || (sz == bsz && fi < bfi)));
+}
+
+// Per vertex: sweep, LDG2 decision, TV gain test (TV/partition.cpp:732-805).
+__global__ void __launch_bounds__(EVAL_BLOCK) k_tv_eval(EvalArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int K = a.K;
+  double* s_pen = reinterpret_cast<double*>(lds);
+  long long* s_size = reinterpret_cast<long long*>(s_pen + K);
+  unsigned* wbase = reinterpret_cast<unsigned*>(s_size + K);
+  __shared__ long long red_sz[EVAL_BLOCK / WAVE];
+  __shared__ int red_ix[EVAL_BLOCK / WAVE];
+  const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
+  unsigned* cnt = wbase + (size_t)wave * 2 * (size_t)K;
+  unsigned* first = cnt + K;
+
+  // penalties of the frozen part sizes: penalty = 1 - (1.0 * size / C)  (TV/partition.cpp:103)
+  long long bs = LLONG_MAX;
+  int bi = INT_MAX;
+  for (int q = (int)threadIdx.x; q < K; q += (int)blockDim.x) {
+    const long long sz = a.sizes[q];
+    s_size[q] = sz;
+    s_pen[q] = 1.0 - ((double)sz / a.C);
+    if (sz < bs) { bs = sz; bi = q; }
+    if (blockIdx.x == 0 && a.sizes_frozen) a.sizes_frozen[q] = sz;
+  }
+  for (int q = lane; q < K; q += WAVE) { cnt[q] = 0; first[q] = NOFIRST; }
+  // fallback: lowest-index part of minimum size (TV/partition.cpp:118-132)
+  for (int off = WAVE / 2; off > 0; off >>= 1) {
+    const long long os = __shfl_xor(bs, off);
+    const int oi = __shfl_xor(bi, off);
+    if (os < bs || (os == bs && oi < bi)) { bs = os; bi = oi; }
+  }
+  if (lane == 0) { red_sz[wave] = bs; red_ix[wave] = bi; }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && a.tcount_next) *a.tcount_next = 0;
+  __syncthreads();
+  int fb = red_ix[0];
+  {
+    long long b = red_sz[0];
+    for (int w = 1; w < nw; w++)
+      if (red_sz[w] < b || (red_sz[w] == b && red_ix[w] < fb)) { b = red_sz[w]; fb = red_ix[w]; }
+  }
+
+  // One wavefront per vertex, grid-stride over the round's slots, software
+  // pipelined over the wave's vertices: while vertex X is swept, the segment
+  // addresses of X+1's nets and the vertex header of X+2 are in flight.
+  const int stride = (int)gridDim.x * nw;
+  const int slot_end = a.nslots;
+  int slot = (int)blockIdx.x * nw + wave;
+  int hv[3], hown[3], hdeg[3];
+  unsigned hvb[3];
+  auto load_vertex = [&](int k, int sl) {
+    if (sl < slot_end) {
+      const int vv = a.order ? a.order[a.r0 + sl] : a.r0 + sl;
+      hv[k] = vv;
+      hown[k] = a.part[vv];
+      hvb[k] = a.vtx_ptr[vv];
+      hdeg[k] = (int)(a.vtx_ptr[vv + 1] - hvb[k]);
+    } else {
+      hv[k] = 0; hown[k] = 0; hvb[k] = 0; hdeg[k] = 0;
+    }
+  };
+  auto load_segs = [&](int k) -> unsigned {
+    return lane < min(hdeg[k], WAVE) ? a.vtx_seg[hvb[k] + (unsigned)lane] : 0u;
+  };
+  load_vertex(0, slot);
+  load_vertex(1, slot + stride);
+  unsigned sg0 = load_segs(0);
+  while (slot < slot_end) {
+    load_vertex(2, slot + 2 * stride);
+    const unsigned sg1 = load_segs(1);
+    const int v = hv[0], own = hown[0], deg = hdeg[0];
+    const unsigned vb = hvb[0];
+
+    // sweep: histogram of parts over the lists of v's nets. The first
+    // appearance of a part is ranked by (net index i in v's net list, position
+    // t in that net's list), packed as i << pb | t (t < lambda <= K <= 2^pb):
+    // the order in which the reference's scan meets the parts
+    // (TV/partition.cpp:318-338). One 64-word read of a segment brings lambda
+    // (lane 0) and the first 63 entries (lanes 1..63); NETS_IN_FLIGHT such
+    // reads are issued together, longer lists are finished after.
+    int lg = 0;
+    for (int c0 = 0; c0 < deg; c0 += WAVE) {
+      const int nn = min(WAVE, deg - c0);
+      const unsigned sg = c0 == 0 ? sg0 : (lane < nn ? a.vtx_seg[vb + (unsigned)(c0 + lane)] : 0u);
+      unsigned long long longmask = 0;  // nets whose list has more than 63 entries
+      for (int j0 = 0; j0 < nn; j0 += NETS_IN_FLIGHT) {
+        unsigned xv[NETS_IN_FLIGHT];
+#pragma unroll
+        for (int u = 0; u < NETS_IN_FLIGHT; u++) {
+          const unsigned sj = (unsigned)__builtin_amdgcn_readlane((int)sg, min(j0 + u, nn - 1));
+          xv[u] = a.ev[sj + (unsigned)lane];
+        }
+#pragma unroll
+        for (int u = 0; u < NETS_IN_FLIGHT; u++) {
+          const int j = j0 + u;
+          const int Lj = __builtin_amdgcn_readlane((int)xv[u], 0) & (j < nn ? -1 : 0);
+          if (Lj > WAVE - 1) longmask |= 1ull << j;
+          const int t = lane - 1;
+          const bool ok = lane >= 1 && t < Lj;
+          const unsigned x = xv[u];
+          const int q = (int)(x & ~EV_SINGLE);
+          if (ok) {
+            atomicAdd(&cnt[q], 1u);
+            atomicMin(&first[q], ((unsigned)(c0 + j) << a.pb) | (unsigned)t);
+          }
+          // leave gain: nets where v is the only pin of its part (TV/partition.cpp:365-368)
+          lg += __popcll(__ballot(ok && q == own && (x & EV_SINGLE)));
+        }
+      }
+      // the rest of the long lists: entries 63.. in windows of 64
+      while (longmask) {
+        const int j = __ffsll((long long)longmask) - 1;
+        longmask &= longmask - 1;
+        const unsigned sj = (unsigned)__builtin_amdgcn_readlane((int)sg, j);
+        const int Lj = (int)a.ev[sj];
+        for (int t0 = WAVE - 1; t0 < Lj; t0 += WAVE * LONG_WINDOWS) {
+          unsigned xv[LONG_WINDOWS];
+#pragma unroll
+          for (int u = 0; u < LONG_WINDOWS; u++) {
+            const int t = t0 + u * WAVE + lane;
+            xv[u] = a.ev[sj + 1u + (unsigned)(t < Lj ? t : 0)];
+          }
+#pragma unroll
+          for (int u = 0; u < LONG_WINDOWS; u++) {
+            const int t = t0 + u * WAVE + lane;
+            const bool ok = t < Lj;
+            const unsigned x = xv[u];
+            const int q = (int)(x & ~EV_SINGLE);
+            if (ok) {
+              atomicAdd(&cnt[q], 1u);
+              atomicMin(&first[q], ((unsigned)(c0 + j) << a.pb) | (unsigned)t);
+            }
+            lg += __popcll(__ballot(ok && q == own && (x & EV_SINGLE)));
+          }
+        }
+      }
+    }
+    wave_lds_sync();
+    // candidates: lexicographic max of (score, -size, -first rank) over score >
+    // 0 equals the sequential LDG2 scan over the active parts
+    // (TV/partition.cpp:98-116). Scores are positive doubles, so their bit
+    // patterns order like the values: key1 = score bits (max), key2 = size <<
+    // 32 | first rank (min among key1 ties).
+    const unsigned cfb = cnt[fb];
+    unsigned long long b1 = 0, b2 = ~0ull;
+    int bq = -1;
+    unsigned bc = 0;
+    for (int q = lane; q < K; q += WAVE) {
+      const unsigned c = cnt[q];
+      if (c) {
+        const unsigned f = first[q];
+        cnt[q] = 0;
+        first[q] = NOFIRST;
+        const int adj = (int)c - (q == own ? 1 : 0);   // scores[part_id]-- (TV/partition.cpp:345)
+        const double sc = (double)adj * s_pen[q];
+        if (sc > 0) {
+          const unsigned long long k1 = (unsigned long long)__double_as_longlong(sc);
+          const unsigned long long k2 = ((unsigned long long)s_size[q] << 32) | f;
+          if (k1 > b1 || (k1 == b1 && k2 < b2)) { b1 = k1; b2 = k2; bq = q; bc = c; }
+        }
+      }
+    }
+    const unsigned long long m1 = wave_max_u64(b1);
+    const unsigned long long m2 = wave_min_u64(b1 == m1 ? b2 : ~0ull);
+    wave_lds_sync();
+    int d = fb;
+    unsigned craw = cfb;
+    if (m1 != 0) {
+      const int wl = __ffsll((long long)__ballot(b1 == m1 && b2 == m2)) - 1;
+      d = __builtin_amdgcn_readlane(bq, wl);
+      craw = (unsigned)__builtin_amdgcn_readlane((int)bc, wl);
+    }
+    if (s_size[d] > a.max_pw) d = own;   // TV/partition.cpp:135-138
+    if (s_size[d] < a.min_pw) d = own;   // TV/partition.cpp:140-143
+    const int al = deg - (int)craw;      // arrival loss: nets of v lacking d (TV/partition.cpp:369-377)
+    const bool accept = d != own && lg >= al;   // TV/partition.cpp:380
+    if (lane == 0) {
+      a.slot_own[slot] = own;
+      a.slot_dec[slot] = accept ? d : -1;
+      if (a.dec_out) a.dec_out[v] = d;
+      if (a.gain_out) a.gain_out[v] = d == own ? INT_MIN : lg - al;
+      if (accept && a.cand_tot) {
+        const int w = a.vwgt ? a.vwgt[v] : 1;
+        atomic_add_i64(&a.cand_tot[d], w);
+        atomicAdd(&a.hist[(size_t)d * (size_t)a.ntiles + (size_t)(slot / TILE)], w);
+      }
+    }
+    if (accept && a.touched) {
+      for (int i = lane; i < deg; i += WAVE) {
+        const unsigned n = a.vtx_nets[vb + (unsigned)i];
+        if (atomicExch(&a.net_stamp[n], a.tag) != a.tag) a.touched[atomicAdd(a.tcount, 1)] = (int)n;
+      }
+    }
+    slot += stride;
+    hv[0] = hv[1]; hown[0] = hown[1]; hvb[0] = hvb[1]; hdeg[0] = hdeg[1];
+    hv[1] = hv[2]; hown[1] = hown[2]; hvb[1] = hvb[2]; hdeg[1] = hdeg[2];
+    sg0 = sg1;
+  }
+}
+
+// BAL step 1: over-subscribed parts (candidate weight > cap - frozen size) get
+// a dense index; one block per such part scans its tile histogram (exclusive)
+// into pref and clears the histogram row.
+__global__ void __launch_bounds__(256) k_bal_scan(int* __restrict__ hist, long long* __restrict__ pref, int ntiles,
+                                                  int K, const long long* __restrict__ sizes_frozen,
+                                                  const long long* __restrict__ cand_tot, long long cap,
+                                                  int* __restrict__ over_idx, int* __restrict__ over_count) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  int* s_list = reinterpret_cast<int*>(lds);  // K
+  __shared__ int s_wi[16];
+  __shared__ long long s_wl[16];
+  __shared__ long long s_carry;
+  int base = 0;
+  for (int d0 = 0; d0 < K; d0 += (int)blockDim.x) {
+    const int d = d0 + (int)threadIdx.x;
+    const int f = (d < K && cand_tot[d] > cap - sizes_frozen[d]) ? 1 : 0;
+    int tot;
+    const int ex = block_excl_scan(f, s_wi, &tot);
+    if (d < K) {
+      if (f) s_list[base + ex] = d;
+      if (blockIdx.x == 0) over_idx[d] = f ? base + ex : -1;
+    }
+    base += tot;
+  }
+  const int nover = base;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *over_count = nover;
+  __syncthreads();
+  for (int o = (int)blockIdx.x; o < nover; o += (int)gridDim.x) {
+    const int d = s_list[o];
+    int* row = hist + (size_t)d * (size_t)ntiles;
+    long long* out = pref + (size_t)o * (size_t)ntiles;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (int t0 = 0; t0 < ntiles; t0 += (int)blockDim.x) {
+      const int t = t0 + (int)threadIdx.x;
+      const long long x = t < ntiles ? row[t] : 0;
+      long long tot;
+      const long long ex = block_excl_scan(x, s_wl, &tot);
+      if (t < ntiles) {
+        out[t] = s_carry + ex;
+        row[t] = 0;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) s_carry += tot;
+      __syncthreads();
+    }
+  }
+}
+
+// BAL step 2, one wavefront per tile: a candidate into an over-subscribed part
+// d is admitted iff frozen_size[d] + (weight of earlier candidates into d) + w
+// <= cap; every other candidate is admitted. Commits part vector and sizes of
+// admitted moves; clears the tile's histogram cells of non-over parts.
+__global__ void __launch_bounds__(256) k_bal_admit(int* __restrict__ slot_dec, const int* __restrict__ slot_own,
+                                                   const int* __restrict__ order, const int* __restrict__ vwgt,
+                                                   int r0, int nslots, int ntiles, int K,
+                                                   const long long* __restrict__ sizes_frozen, long long cap,
+                                                   const int* __restrict__ over_idx,
+                                                   const int* __restrict__ over_count,
+                                                   const long long* __restrict__ pref, int* __restrict__ hist,
+                                                   int* __restrict__ part, long long* __restrict__ sizes,
+                                                   long long* __restrict__ cand_tot,
+                                                   unsigned long long* __restrict__ moves) {
+  const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
+  const int nover = *over_count;
+  if (blockIdx.x == 0)
+    for (int d = (int)threadIdx.x; d < K; d += (int)blockDim.x) cand_tot[d] = 0;
+  for (int tile = (int)blockIdx.x * nw + wave; tile < ntiles; tile += (int)gridDim.x * nw) {
+    const int slot = tile * TILE + lane;
+    int d = -1, v = -1;
+    long long w = 0;
+    if (slot < nslots) {
+      d = slot_dec[slot];
+      const int z = r0 + slot;
+      v = order ? order[z] : z;
+      w = vwgt ? vwgt[v] : 1;
+    }
+    const int o = (d >= 0 && nover > 0) ? over_idx[d] : -1;
+    bool adm = d >= 0;
+    if (__ballot(o >= 0)) {
+      // weight of earlier candidates into the same part within the tile
+      long long pre = 0;
+      for (int j = 0; j < WAVE; j++) {
+        const int dj = __shfl(d, j);
+        const long long wj = __shfl(w, j);
+        if (j < lane && dj == d) pre += wj;
+      }
+      if (o >= 0) adm = sizes_frozen[d] + pref[(size_t)o * (size_t)ntiles + (size_t)tile] + pre + w <= cap;
+    }
+    if (d >= 0) {
+      if (o < 0) hist[(size_t)d * (size_t)ntiles + (size_t)tile] = 0;
+      if (!adm) slot_dec[slot] = -1;
+    }
+    if (adm) {
+      part[v] = d;
+      atomic_add_i64(&sizes[slot_own[slot]], -w);
+      atomic_add_i64(&sizes[d], w);
+    }
+    const unsigned long long am = __ballot(adm);
+    if (lane == 0 && am) atomicAdd(moves, (unsigned long long)__popcll(am));
+  }
+}
+
+struct ApplyArgs {
+  const unsigned* net_ptr;
+  const int* apply_pins;
+  const int* slot_own;
+  const int* slot_dec;
+  const int* touched;
+  const int* tcount;
+  unsigned* ent;
+  unsigned short* ev;
+  int K, pb;
+  unsigned pmask;
+  int r0, r1;
+  // REF commit in blocks >= net_blocks (nullable part: no commit)
+  int net_blocks;
+  const int* order;
+  int* part;
+  long long* sizes;
+  unsigned long long* moves;
+};
+
+// Replays the round's moves on every touched net in ordering order
+// (TV/partition.cpp:846-890: remove = decrement or swap-with-last, add =
+// increment or append); one wavefront per net, the list staged in LDS.
+// Blocks past net_blocks commit the REF part vector and source-part sizes
+// (TV/partition.cpp:818-830: the destination's size is left unchanged).
+__global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
+  if ((int)blockIdx.x >= a.net_blocks) {
+    const int slot = ((int)blockIdx.x - a.net_blocks) * (int)blockDim.x + (int)threadIdx.x;
+    bool mv = false;
+    if (slot < a.r1 - a.r0) {
+      const int d = a.slot_dec[slot];
+      if (d >= 0) {
+        const int z = a.r0 + slot;
+        a.part[a.order ? a.order[z] : z] = d;
+        atomic_add_i64(&a.sizes[a.slot_own[slot]], -1);
+        mv = true;
+      }
+    }
+    const unsigned long long m = __ballot(mv);
+    if (lane == 0 && m) atomicAdd(a.moves, (unsigned long long)__popcll(m));
+    return;
+  }
+  unsigned* list = reinterpret_cast<unsigned*>(lds) + (size_t)wave * (size_t)a.K;
+  const unsigned inc = 1u << a.pb;
+  const int ntouched = *a.tcount;
+  for (int i = (int)blockIdx.x * nw + wave; i < ntouched; i += a.net_blocks * nw) {
+    const int n = a.touched[i];
+    const unsigned s = a.net_ptr[n], e = a.net_ptr[n + 1];
+    unsigned* seg = a.ent + s + (unsigned)n;
+    int L = (int)seg[0];
+    for (int t = lane; t < L; t += WAVE) list[t] = seg[1 + t];
+    const unsigned lo = wave_lower_bound(a.apply_pins, s, e, a.r0);
+    const unsigned hi = wave_lower_bound(a.apply_pins, lo, e, a.r1);
+    wave_lds_sync();
+    bool dirty = false;
+    for (unsigned k0 = lo; k0 < hi; k0 += WAVE) {
+      const unsigned k = k0 + (unsigned)lane;
+      int dd = -1, oo = -1;
+      if (k < hi) {
+        const int slot = a.apply_pins[k] - a.r0;
+        dd = a.slot_dec[slot];
+        if (dd >= 0) oo = a.slot_own[slot];
+      }
+      unsigned long long m = __ballot(dd >= 0);
+      while (m) {
+        dirty = true;
+        const int l = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const int p = __shfl(oo, l), q = __shfl(dd, l);
+        // remove one pin of p
+        int j = wave_find(list, L, p, a.pmask);
+        if (j >= 0) {
+          const unsigned x = list[j];
+          if ((x >> a.pb) > 1u) {
+            if (lane == 0) list[j] = x - inc;
+          } else {
+            if (lane == 0) list[j] = list[L - 1];
+            L--;
+          }
+          wave_lds_sync();
+        }
+        // add one pin of q
+        j = wave_find(list, L, q, a.pmask);
+        if (j >= 0) {
+          if (lane == 0) list[j] += inc;
+        } else {
+          if (lane == 0) list[L] = inc | (unsigned)q;
+          L++;
+        }
+        wave_lds_sync();
+      }
+    }
+    if (dirty) {
+      unsigned short* vseg = a.ev + s + (unsigned)n;
+      for (int t = lane; t < L; t += WAVE) {
+        const unsigned x = list[t];
+        seg[1 + t] = x;
+        vseg[1 + t] = ev_word(x, a.pb, a.pmask);
+      }
+      if (lane == 0) {
+        seg[0] = (unsigned)L;
+        vseg[0] = (unsigned short)L;
+      }
+    }
+    wave_lds_sync();
+  }
+}
+
+int eval_waves(int K) {
+  // LDS: 16 bytes per part shared by the block + 8 bytes per part per wavefront
+  for (int w = EVAL_BLOCK / WAVE; w >= 1; w--)
+    if ((size_t)K * 16 + (size_t)w * 8 * (size_t)K <= (size_t)LDS_BUDGET) return w;
+  return 0;
+}
+
+size_t eval_lds(int K, int waves) { return (size_t)K * 16 + (size_t)waves * 8 * (size_t)K; }
+
+int eval_grid(int nslots, int waves) { return std::max(1, std::min((nslots + waves - 1) / waves, 8192)); }
+
+fp_status ensure_round_scratch(fp_ctx* ctx, int R) {
+  if (R > ctx->rmax) {
+    FP_TRY(dalloc(ctx, &ctx->slot_own, (size_t)R));
+    FP_TRY(dalloc(ctx, &ctx->slot_dec, (size_t)R));
+    ctx->rmax = R;
+  }
+  return FP_OK;
+}
+
+fp_status ensure_bal_scratch(fp_ctx* ctx, int R) {
+  const size_t ntiles = (size_t)(R + TILE - 1) / TILE;
+  const size_t need = ntiles * (size_t)ctx->K;
+  if (need > ctx->hist_cap) {
+    FP_TRY(dalloc(ctx, &ctx->hist, need));
+    FP_TRY(dalloc(ctx, &ctx->pref, need));
+    fill_async<int>(ctx, ctx->hist, need, 0);
+    ctx->hist_cap = need;
+  }
+  return FP_OK;
+}
+
+fp_status check_sched(fp_ctx* ctx, const fp_schedule* s) {
+  if (!s) return fail(ctx, FP_ERR_ARG, "null schedule");
+  if (s->kind != FP_SCHED_REF && s->kind != FP_SCHED_BAL) return fail(ctx, FP_ERR_ARG, "unknown schedule kind %d", s->kind);
+  if (s->round < 1) return fail(ctx, FP_ERR_ARG, "round size must be >= 1");
+  if (!(s->ldg_C > 0)) return fail(ctx, FP_ERR_ARG, "ldg_C must be > 0");
+  return FP_OK;
+}
+
+// what the evaluation kernel's packed keys assume
+fp_status check_envelope(fp_ctx* ctx) {
+  if (eval_waves(ctx->K) == 0) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d needs more LDS than a workgroup has", ctx->K);
+  if ((unsigned long long)ctx->max_vdeg + 1 >= (1ull << (32 - ctx->pb)))
+    return fail(ctx, FP_ERR_UNSUPPORTED, "a vertex of degree %u is too large for K = %d", ctx->max_vdeg, ctx->K);
+  if (ctx->total_weight >= (1ll << 32)) return fail(ctx, FP_ERR_UNSUPPORTED, "total vertex weight >= 2^32");
+  return FP_OK;
+}
+
+EvalArgs make_eval_args(fp_ctx* ctx, const fp_schedule* s) {
+  EvalArgs a{};
+  a.vtx_ptr = ctx->vtx_ptr;
+  a.vtx_nets = ctx->vtx_nets;
+  a.vtx_seg = ctx->vtx_seg;
+  a.ev = ctx->ev;
+  a.part = ctx->part;
+  a.order = ctx->order;
+  a.vwgt = ctx->vwgt;
+  a.sizes = ctx->sizes;
+  a.K = ctx->K;
+  a.pb = ctx->pb;
+  a.pmask = ctx->pmask;
+  a.C = s->ldg_C;
+  a.max_pw = s->max_pw;
+  a.min_pw = s->min_pw;
+  a.slot_own = ctx->slot_own;
+  a.slot_dec = ctx->slot_dec;
+  return a;
+}
+
+}  // namespace
+}  // namespace fpe
+
+using namespace fpe;
+
+extern "C" {
+
+fp_status fp_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule* sched, fp_refine_stats* stats) {
+  if (!ctx) return FP_ERR_ARG;
+  if (!ctx->has_state) return fail(ctx, FP_ERR_STATE, "fp_refine before a state was set");
+  FP_TRY(check_sched(ctx, sched));
+  if (passes < 0) return fail(ctx, FP_ERR_ARG, "passes < 0");
+  if (metric != FP_METRIC_TV) return fail(ctx, FP_ERR_UNSUPPORTED, "refinement for metric %d not available", (int)metric);
+  if (ctx->dup_pins) return fail(ctx, FP_ERR_UNSUPPORTED, "a net lists the same vertex twice; refinement needs distinct pins");
+  const int nov = ctx->nov, K = ctx->K;
+  FP_TRY(check_envelope(ctx));
+  const int waves = eval_waves(K);
+  const size_t lds_a = (size_t)(APPLY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
+  if (lds_a > (size_t)LDS_BUDGET) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d too large for the apply kernel", K);
+  FP_HIP_OK(hipSetDevice(ctx->device));
+  const int R = std::min(sched->round, nov);
+  const bool bal = sched->kind == FP_SCHED_BAL;
+  FP_TRY(ensure_round_scratch(ctx, R));
+  if (bal) FP_TRY(ensure_bal_scratch(ctx, R));
+  const size_t lds_e = eval_lds(K, waves);
+  const size_t lds_s = (size_t)K * sizeof(int);
+
+  fill_async<int>(ctx, ctx->counters, 4, 0);
+  fill_async<unsigned long long>(ctx, ctx->moves, 1, 0ull);
+  hipEvent_t e0 = pooled_event(ctx, 0), e1 = pooled_event(ctx, 1);
+  if (!e0 || !e1) return fail(ctx, FP_ERR_HIP, "hipEventCreate failed");
+  FP_HIP_OK(hipEventRecord(e0, ctx->stream));
+  ctx->timed_launches = 0;
+  ctx->timed_ms = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> timed;
+  long long rounds = 0;
+  for (int it = 0; it < passes; it++) {
+    for (int r0 = 0; r0 < nov; r0 += R) {
+      const int nslots = std::min(R, nov - r0);
+      const int ntiles = (nslots + TILE - 1) / TILE;
+      ctx->tag++;
+      const int par = (int)(ctx->tag & 1u);
+      EvalArgs a = make_eval_args(ctx, sched);
+      a.r0 = r0;
+      a.nslots = nslots;
+      a.ntiles = ntiles;
+      a.tag = ctx->tag;
+      a.net_stamp = ctx->net_stamp;
+      a.touched = ctx->touched;
+      a.tcount = ctx->counters + par;
+      a.tcount_next = ctx->counters + (par ^ 1);
+      if (bal) {
+        a.cand_tot = ctx->cand_tot;
+        a.hist = ctx->hist;
+        a.sizes_frozen = ctx->sizes_frozen;
+      }
+      hipEvent_t ta = nullptr, tb = nullptr;
+      if (ctx->kernel_timing) {
+        ta = pooled_event(ctx, 2 + 2 * timed.size());
+        tb = pooled_event(ctx, 3 + 2 * timed.size());
+        if (!ta || !tb) return fail(ctx, FP_ERR_HIP, "hipEventCreate failed");
+        FP_HIP_OK(hipEventRecord(ta, ctx->stream));
+      }
+      k_tv_eval<<<eval_grid(nslots, waves), waves * WAVE, lds_e, ctx->stream>>>(a);
+      if (ctx->kernel_timing) {
+        FP_HIP_OK(hipEventRecord(tb, ctx->stream));
+        timed.emplace_back(ta, tb);
+      }
+      if (bal) {
+        k_bal_scan<<<std::max(1, std::min(K, 512)), 256, lds_s, ctx->stream>>>(
+            ctx->hist, ctx->pref, ntiles, K, ctx->sizes_frozen, ctx->cand_tot, sched->cap, ctx->over_idx,
+            ctx->over_count);
+        k_bal_admit<<<std::max(1, std::min((ntiles + 3) / 4, 4096)), 256, 0, ctx->stream>>>(
+            ctx->slot_dec, ctx->slot_own, ctx->order, ctx->vwgt, r0, nslots, ntiles, K, ctx->sizes_frozen,
+            sched->cap, ctx->over_idx, ctx->over_count, ctx->pref, ctx->hist, ctx->part, ctx->sizes,
+            ctx->cand_tot, ctx->moves);
+      }
+      ApplyArgs ap{};
+      ap.net_ptr = ctx->net_ptr;
+      ap.apply_pins = ctx->apply_pins;
+      ap.slot_own = ctx->slot_own;
+      ap.slot_dec = ctx->slot_dec;
+      ap.touched = ctx->touched;
+      ap.tcount = ctx->counters + par;
+      ap.ent = ctx->ent;
+      ap.ev = ctx->ev;
+      ap.K = K;
+      ap.pb = ctx->pb;
+      ap.pmask = ctx->pmask;
+      ap.r0 = r0;
+      ap.r1 = r0 + nslots;
+      ap.net_blocks = APPLY_NET_BLOCKS;
+      int grid_a = APPLY_NET_BLOCKS;
+      if (!bal) {
+        ap.order = ctx->order;
+        ap.part = ctx->part;
+        ap.sizes = ctx->sizes;
+        ap.moves = ctx->moves;
+        grid_a += (nslots + APPLY_BLOCK - 1) / APPLY_BLOCK;
+      }
+      k_apply<<<grid_a, APPLY_BLOCK, lds_a, ctx->stream>>>(ap);
+      rounds++;
+    }
+  }
+  FP_HIP_OK(hipGetLastError());
+  FP_HIP_OK(hipEventRecord(e1, ctx->stream));
+  unsigned long long mv = 0;
+  FP_TRY(d2h(ctx, &mv, ctx->moves, 1));
+  FP_HIP_OK(hipStreamSynchronize(ctx->stream));
+  float ms = 0;
+  FP_HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+  for (auto& pr : timed) {
+    float t = 0;
+    FP_HIP_OK(hipEventElapsedTime(&t, pr.first, pr.second));
+    ctx->timed_ms += t;
+  }
+  ctx->timed_launches = (long long)timed.size();
+  if (stats) {
+    stats->moves = (long long)mv;
+    stats->rounds = rounds;
+    stats->pin_visits = (long long)passes * ctx->pins;
+    stats->device_ms = ms;
+  }
+  return FP_OK;
+}
+
+fp_status fp_eval_moves(fp_ctx* ctx, fp_metric metric, const fp_schedule* sched, int* dec, int* gain) {
+  if (!ctx) return FP_ERR_ARG;
+  if (!ctx->has_state) return fail(ctx, FP_ERR_STATE, "fp_eval_moves before a state was set");
+  FP_TRY(check_sched(ctx, sched));
+  if (metric != FP_METRIC_TV) return fail(ctx, FP_ERR_UNSUPPORTED, "move evaluation for metric %d not available", (int)metric);
+  const int nov = ctx->nov, K = ctx->K;
+  FP_TRY(check_envelope(ctx));
+  const int waves = eval_waves(K);
+  FP_HIP_OK(hipSetDevice(ctx->device));
+  FP_TRY(ensure_round_scratch(ctx, nov));
+  int *d_dec = nullptr, *d_gain = nullptr;
+  FP_HIP_OK(hipMalloc(&d_dec, sizeof(int) * (size_t)nov));
+  if (hipMalloc(&d_gain, sizeof(int) * (size_t)nov) != hipSuccess) {
+    (void)hipFree(d_dec);
+    return fail(ctx, FP_ERR_OOM, "hipMalloc failed");
+  }
+  EvalArgs a = make_eval_args(ctx, sched);
+  a.r0 = 0;
+  a.nslots = nov;
+  a.ntiles = (nov + TILE - 1) / TILE;
+  a.dec_out = d_dec;
+  a.gain_out = d_gain;
+  k_tv_eval<<<eval_grid(nov, waves), waves * WAVE, eval_lds(K, waves), ctx->stream>>>(a);
+  fp_status st = FP_OK;
+  if (hipGetLastError() != hipSuccess) st = fail(ctx, FP_ERR_HIP, "k_tv_eval launch failed");
+  if (st == FP_OK && dec) st = d2h(ctx, dec, d_dec, (size_t)nov);
+  if (st == FP_OK && gain) st = d2h(ctx, gain, d_gain, (size_t)nov);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess && st == FP_OK) st = fail(ctx, FP_ERR_HIP, "sync failed");
+  (void)hipFree(d_dec);
+  (void)hipFree(d_gain);
+  return st;
+}
+
+}  // extern "C"
