@@ -1,0 +1,60 @@
+"""Writes the golden fixtures of tests/golden/ by running the REFERENCE's own
+CPU partitioner (oracle/_ref/ref_<m>, compiled from the reference sources by
+oracle/Makefile) on small seeded instances. Only data is committed: the
+instance (net->pins CSR and sources) and the states the reference reached.
+
+Run from the repository root where the reference tree is available:
+    make -C oracle ref && python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import re
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from fastpartitioner_amd import hypergraph as H  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+# (name, reference binary, K, passes, threads: 0 = partition(), 1 = partition_par with one thread)
+CASES = [
+    ("tv_serial", "tv", 8, 5, 0),
+    ("tv_par1", "tv", 8, 5, 1),
+    ("cn_serial", "cn", 8, 5, 0),
+    ("cn_par1", "cn", 8, 5, 1),
+]
+PRINTED = ("TV", "CN", "TM", "MGM", "MGV", "MGAV")
+
+
+def instance():
+    # 2,000 x 2,000 column-net matrix: diagonal + 5 uniform off-diagonals per row
+    return H.colnet(2000, H.DEG_FIXED, 5, seed=7)
+
+
+def main():
+    hg = instance()
+    for name, m, K, passes, threads in CASES:
+        if not O.ref_available(m):
+            raise SystemExit(f"oracle/_ref/ref_{m} not built: run `make -C oracle ref`")
+        r = O.run_reference(m, hg, K, passes, threads)
+        printed = {}
+        for key in PRINTED:
+            mm = re.findall(rf"^After Refinement {key} : (-?\d+)$", r["stdout"], re.M)
+            printed[key] = int(mm[-1])
+        np.savez_compressed(
+            os.path.join(OUT, f"{name}.npz"),
+            nov=hg.nov, net_ptr=hg.net_ptr, net_pins=hg.net_pins, src=hg.src, K=K, passes=passes,
+            threads=threads, init_part=r["init_part"], init_ends=r["init_ends"], init_parts=r["init_parts"],
+            part=r["part"], ends=r["ends"], parts=r["parts"], cnt=r["cnt"],
+            printed=np.array([printed[k] for k in PRINTED], dtype=np.int64))
+        print(name, {k: printed[k] for k in PRINTED})
+
+
+if __name__ == "__main__":
+    main()

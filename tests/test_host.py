@@ -1,0 +1,129 @@
+"""Host side: generators, CSR transpose, PaToH text I/O, initial partition,
+balance constants, and the C-ABI libraries (load + exported symbols; no
+device work here)."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from fastpartitioner_amd import _native
+from fastpartitioner_amd import hypergraph as H
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_c1_shape_and_sources():
+    hg = H.workload("c1")
+    assert (hg.nov, hg.noe, hg.pins) == (10_000, 10_000, 80_000)
+    assert np.array_equal(hg.src, np.arange(hg.noe))
+    for n in range(0, hg.noe, 97):
+        row = hg.net_pins[hg.net_ptr[n]:hg.net_ptr[n + 1]]
+        assert n in row and np.all(np.diff(row.astype(np.int64)) > 0)
+
+
+def test_c3_shape_and_weights():
+    hg = H.workload("c3", scale=0.005)
+    assert hg.pins == 13 * hg.nov
+    assert hg.vwgt.min() >= 1 and hg.vwgt.max() <= 8
+
+
+def test_zipf_degrees_bounded():
+    hg = H.colnet(20_000, H.DEG_ZIPF, 2, 500, 2.1, seed=4)
+    d = np.diff(hg.net_ptr.astype(np.int64))
+    assert d.min() >= 2 and d.max() <= 500
+    # a heavy-tailed law: the mean sits well below the midpoint, many rows are minimal
+    assert d.mean() < 12 and (d == 2).mean() > 0.3
+
+
+def test_generator_independent_of_thread_count():
+    code = ("import sys; sys.path.insert(0, %r); from fastpartitioner_amd import hypergraph as H; "
+            "import hashlib; hg = H.workload('c2', scale=0.01); "
+            "print(hashlib.sha1(hg.net_pins.tobytes() + hg.net_ptr.tobytes()).hexdigest())") % ROOT
+    outs = set()
+    for t in ("1", "4"):
+        env = dict(os.environ, OMP_NUM_THREADS=t)
+        outs.add(subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                                check=True).stdout.strip())
+    assert len(outs) == 1
+
+
+def test_transpose_matches_numpy():
+    hg = H.colnet(3000, H.DEG_UNIFORM, 0, 6, seed=9)
+    rows = np.repeat(np.arange(hg.noe), np.diff(hg.net_ptr.astype(np.int64)))
+    order = np.lexsort((rows, hg.net_pins))
+    assert np.array_equal(hg.vtx_nets, rows[order].astype(np.uint32))
+    assert np.array_equal(hg.vtx_ptr[1:], np.cumsum(np.bincount(hg.net_pins, minlength=hg.nov)))
+
+
+def test_patoh_round_trip(tmp_path):
+    hg = H.colnet(500, H.DEG_UNIFORM, 1, 4, seed=3)
+    for base in (0, 1):
+        p = str(tmp_path / f"g{base}.hg")
+        H.write_patoh(hg, p, index_base=base)
+        back = H.read_patoh(p)
+        assert back.nov == hg.nov and back.noe == hg.noe
+        assert np.array_equal(back.net_ptr, hg.net_ptr) and np.array_equal(back.net_pins, hg.net_pins)
+
+
+def test_patoh_pin_count_mismatch_fails(tmp_path):
+    p = tmp_path / "bad.hg"
+    p.write_text("0 4 2 5\n0 1\n2 3\n")   # header says 5 pins, file has 4
+    with pytest.raises(_native.FastPartError):
+        H.read_patoh(str(p))
+
+
+def test_patoh_blank_lines_and_noe_correction(tmp_path):
+    p = tmp_path / "blank.hg"
+    p.write_text("1 4 3 4\n1 2\n\n3 4\n")   # base 1, one blank line, 2 nets instead of 3
+    hg = H.read_patoh(str(p))
+    assert hg.noe == 2 and list(hg.net_pins) == [0, 1, 2, 3]
+
+
+def test_reference_constants():
+    # TV/partition.cpp:95-96,159-160 at the 10k / K=16 instance
+    C, max_pw, min_pw = H.ref_constants(10_000, 16)
+    assert C == pytest.approx(656.25) and max_pw == 687 and min_pw == 562
+    C, max_pw, min_pw = H.ref_constants(1_000_000, 256)
+    assert C == pytest.approx(4101.3) and max_pw == 4296 and min_pw == 3515
+    assert H.balance_cap(1_000_000, 256, 0.03) == 4024
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_initial_partition_matches_oracle_ldg(seed):
+    hg = H.colnet(5000, H.DEG_UNIFORM, 2, 10, seed=seed)
+    K = 16
+    C, max_pw, min_pw = H.ref_constants(hg.nov, K)
+    st, _ = O.tv_ldg(hg, K, C, max_pw, min_pw)
+    assert np.array_equal(H.initial_partition(hg, K, C, max_pw, min_pw), st.part)
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fp_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,lib", [("fastpart.h", _native.ENGINE_LIB), ("fastpart_host.h", _native.HOST_LIB)])
+def test_library_exports_every_declared_symbol(header, lib):
+    names = _declared(header)
+    assert len(names) >= 8
+    so = ctypes.CDLL(lib)
+    missing = [n for n in names if not hasattr(so, n)]
+    assert not missing, missing
+
+
+def test_engine_reports_no_device_without_gpu():
+    # on a machine without a GPU the engine fails loudly with FP_ERR_NODEV; it never falls back
+    lib = _native.engine()
+    n = ctypes.c_int(-1)
+    rc = lib.fp_device_count(ctypes.byref(n))
+    if rc == 0 and n.value > 0:
+        pytest.skip("a GPU is present")
+    assert rc == 8
+    ctx = ctypes.c_void_p()
+    assert lib.fp_ctx_create(ctypes.byref(ctx), 0) == 8

@@ -1,0 +1,246 @@
+"""The oracle (oracle/fp_oracle.c) pinned against the reference.
+
+1. Golden fixtures written by the reference's own partitioner (tests/golden/,
+   made by tests/golden/make_golden.py from oracle/_ref binaries built from
+   the reference sources): the oracle must reproduce every state bit-exactly.
+2. Where the reference binaries are built (this container), fresh seeded
+   instances through the reference itself.
+3. Metrics against their set definitions on hand-checkable and brute-force
+   cases (the README figure; random tiny hypergraphs).
+"""
+import glob
+import itertools
+import os
+import random
+
+import numpy as np
+import pytest
+
+from fastpartitioner_amd import hypergraph as H
+from oracle import oracle as O
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+PRINTED = ("TV", "CN", "TM", "MSM", "MSV", "MSRV")  # the reference prints MGM/MGV/MGAV for MSM/MSV/MSRV
+
+
+def _hg_from(z):
+    return H.Hypergraph.from_nets(int(z["nov"]), z["net_ptr"], z["net_pins"], z["src"])
+
+
+def _run_oracle(kind, hg, K, passes, threads):
+    """Oracle state after the initial partition and after refinement, for the
+    reference driver mode (threads 0 = partition(), 1 = partition_par(1))."""
+    C, max_pw, min_pw = H.ref_constants(hg.nov, K)
+    if kind == "tv":
+        st, marker = O.tv_ldg(hg, K, C, max_pw, min_pw, par_skip=threads > 0)
+        init = st.copy()
+        if threads == 0:
+            O.tv_refine_serial(hg, st, marker, C, max_pw, min_pw, passes)
+        else:
+            O.tv_refine_par(hg, st, C, max_pw, min_pw, passes, 8 * threads)
+    else:
+        st = O.cn_ldg(hg, K, C)
+        init = st.copy()
+        O.cn_refine(hg, st, passes, serial=threads == 0)
+    return init, st
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "*.npz"))), ids=os.path.basename)
+def test_golden_fixture(path):
+    z = np.load(path)
+    hg = _hg_from(z)
+    kind = os.path.basename(path).split("_")[0]
+    init, st = _run_oracle(kind, hg, int(z["K"]), int(z["passes"]), int(z["threads"]))
+    assert np.array_equal(init.part, z["init_part"])
+    assert np.array_equal(init.ep_ends, z["init_ends"])
+    assert np.array_equal(init.ep_parts, z["init_parts"])
+    assert np.array_equal(st.part, z["part"])
+    assert np.array_equal(st.ep_ends, z["ends"])
+    assert np.array_equal(st.ep_parts, z["parts"])
+    assert np.array_equal(st.ep_cnt, z["cnt"])
+    m = O.metrics(hg, int(z["K"]), st.part)
+    assert [m[k] for k in PRINTED] == [int(x) for x in z["printed"]]
+
+
+def test_golden_fixtures_present():
+    assert len(glob.glob(os.path.join(GOLD, "*.npz"))) >= 4
+
+
+needs_ref = pytest.mark.skipif(not (O.ref_available("tv") and O.ref_available("cn")),
+                               reason="reference binaries (oracle/_ref) not built here")
+
+
+@needs_ref
+@pytest.mark.parametrize("kind,threads", [("tv", 0), ("tv", 1), ("cn", 0), ("cn", 1)])
+@pytest.mark.parametrize("seed", [3, 11])
+def test_oracle_matches_reference_binary(kind, threads, seed):
+    hg = H.colnet(1500, H.DEG_UNIFORM, 2, 9, seed=seed)
+    K, passes = 12, 4
+    r = O.run_reference(kind, hg, K, passes, threads)
+    init, st = _run_oracle(kind, hg, K, passes, threads)
+    assert np.array_equal(init.part, r["init_part"])
+    assert np.array_equal(init.ep_parts, r["init_parts"])
+    assert np.array_equal(st.part, r["part"])
+    assert np.array_equal(st.ep_ends, r["ends"])
+    assert np.array_equal(st.ep_parts, r["parts"])
+    assert np.array_equal(st.ep_cnt, r["cnt"])
+
+
+@needs_ref
+def test_build_state_reproduces_reference_initial_lists():
+    # the list order the streaming initial partition leaves is a function of
+    # the part vector and the visiting order alone
+    hg = H.colnet(1500, H.DEG_FIXED, 6, seed=5)
+    r = O.run_reference("tv", hg, 10, 1, 0)
+    st = O.build_state(hg, 10, r["init_part"])
+    assert np.array_equal(st.ep_ends, r["init_ends"])
+    assert np.array_equal(st.ep_parts, r["init_parts"])
+
+
+def test_readme_figure_metrics():
+    # README Fig. 1/2: nets N1={v1,v2,v3}, N2={v1,v6,v7,v8}, N3={v3,v4,v5}, N4={v6,v7};
+    # parts P1={v1,v2}, P2={v3,v4,v5}, P3={v6,v7,v8}: lambda=(2,2,1,1), TV=2, CN=2
+    nets = [[0, 1, 2], [0, 5, 6, 7], [2, 3, 4], [5, 6]]
+    ptr = np.cumsum([0] + [len(n) for n in nets]).astype(np.uint32)
+    pins = np.array(sum(nets, []), dtype=np.uint32)
+    src = np.array([0, 0, 2, 5], dtype=np.int32)
+    hg = H.Hypergraph.from_nets(8, ptr, pins, src)
+    part = np.array([0, 0, 1, 1, 1, 2, 2, 2], dtype=np.int32)
+    m = O.metrics(hg, 3, part)
+    assert list(m["lam"]) == [2, 2, 1, 1]
+    assert m["TV"] == 2 and m["CN"] == 2
+    # sources v1, v1, v3, v6: messages P1->P2, P1->P3; volumes P1: 1+1
+    assert m["TM"] == 2 and m["MSM"] == 2
+    assert m["MSV"] == 2 and list(m["sv"]) == [2, 0, 0]
+
+
+def _brute_metrics(nov, nets, src, K, part):
+    lam = [len({part[v] for v in n}) for n in nets]
+    tv = sum(l - 1 for l in lam if l)
+    cn = sum(1 for l in lam if l > 1)
+    pairs = set()
+    send = [0] * K
+    recv = [0] * K
+    send_msv = [0] * K
+    for n, s, l in zip(nets, src, lam):
+        sp = part[s]
+        send[sp] += l - 1
+        send_msv[sp] += (l - 1) * sum(1 for v in n if v == s)
+        for q in {part[v] for v in n} - {sp}:
+            recv[q] += 1
+            pairs.add((sp, q))
+    sm = [sum(1 for (a, _) in pairs if a == k) for k in range(K)]
+    return {"TV": tv, "CN": cn, "TM": len(pairs), "MSM": max(sm), "MSV": max(send_msv),
+            "MSRV": max(a + b for a, b in zip(send, recv)), "lam": lam}
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_metrics_brute_force(seed):
+    rnd = random.Random(seed)
+    nov, noe, K = rnd.randint(3, 14), rnd.randint(1, 12), rnd.randint(1, 5)
+    nets = [rnd.sample(range(nov), rnd.randint(1, nov)) for _ in range(noe)]
+    src = [rnd.choice(n) for n in nets]
+    part = [rnd.randrange(K) for _ in range(nov)]
+    ptr = np.cumsum([0] + [len(n) for n in nets]).astype(np.uint32)
+    hg = H.Hypergraph.from_nets(nov, ptr, np.array(sum(nets, []), dtype=np.uint32), np.array(src, dtype=np.int32))
+    m = O.metrics(hg, K, np.array(part, dtype=np.int32))
+    b = _brute_metrics(nov, nets, src, K, part)
+    for k in ("TV", "CN", "TM", "MSM", "MSV", "MSRV"):
+        assert m[k] == b[k], k
+    assert list(m["lam"]) == b["lam"]
+
+
+def _ldg2_lexicographic(K, counts, first, sizes, C, max_pw, min_pw, own):
+    """LDG2 restated as an argmax of (score, -size, -first appearance) over
+    positive scores; fallback = lowest-index lightest part; then the bands."""
+    best = None
+    for q in range(K):
+        if counts[q] == 0:
+            continue
+        sc = (counts[q] - (q == own)) * (1 - (1.0 * sizes[q] / C))
+        if sc > 0:
+            key = (sc, -sizes[q], -first[q])
+            if best is None or key > best[0]:
+                best = (key, q)
+    d = best[1] if best else min(range(K), key=lambda q: (sizes[q], q))
+    if sizes[d] > max_pw:
+        d = own
+    if sizes[d] < min_pw and own != -1:
+        d = own
+    return d
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_ldg2_scan_equals_lexicographic_argmax(seed):
+    # decisions of the oracle's sequential LDG2 scan (on one frozen round)
+    # against the lexicographic formulation, on states with many ties
+    hg = H.colnet(300, H.DEG_UNIFORM, 1, 6, seed=seed)
+    K = 6
+    rng = np.random.default_rng(seed)
+    part = rng.integers(0, K, hg.nov).astype(np.int32)
+    st = O.build_state(hg, K, part)
+    sizes = st.sizes.astype(np.int32)
+    C, max_pw, min_pw = 55.0, 60, 40
+    gains = np.zeros(hg.nov, np.int32)
+    st2 = O.State(hg, K)
+    st2.part[:] = st.part
+    st2.ep_ends[:] = st.ep_ends
+    st2.ep_parts[:] = st.ep_parts
+    st2.ep_cnt[:] = st.ep_cnt
+    st2.sizes[:] = sizes
+    O.tv_refine_par(hg, st2, C, max_pw, min_pw, 1, hg.nov, gains=gains)
+    moved_or_decided = 0
+    for v in range(hg.nov):
+        counts = [0] * K
+        first = [10 ** 9] * K
+        pos = 0
+        for i in range(hg.vtx_ptr[v], hg.vtx_ptr[v + 1]):
+            n = hg.vtx_nets[i]
+            s = hg.net_ptr[n]
+            for j in range(st.ep_ends[n]):
+                q = st.ep_parts[s + j]
+                counts[q] += 1
+                first[q] = min(first[q], pos)
+                pos += 1
+        d = _ldg2_lexicographic(K, counts, first, sizes, C, max_pw, min_pw, int(part[v]))
+        if d == part[v]:
+            assert gains[v] == np.iinfo(np.int32).min
+        else:
+            moved_or_decided += 1
+            assert gains[v] != np.iinfo(np.int32).min
+    assert moved_or_decided > 0
+
+
+def test_bal_schedule_respects_cap_and_prefix_rule():
+    hg = H.colnet(4000, H.DEG_UNIFORM, 3, 9, seed=2, weights=(1, 5))
+    K, eps = 16, 0.03
+    W = hg.total_weight()
+    C, max_pw, min_pw = H.ref_constants(W, K, eps)
+    cap = H.balance_cap(W, K, eps)
+    part = H.initial_partition(hg, K, C, max_pw, min_pw)
+    st = O.build_state(hg, K, part)
+    before = st.sizes.copy()
+    O.tv_refine_bal(hg, st, C, cap, 0, cap, 5, 512)
+    # part sizes stay consistent with the part vector and within max(cap, start)
+    sizes = np.bincount(st.part, weights=hg.vwgt, minlength=K).astype(np.int64)
+    assert np.array_equal(sizes, st.sizes)
+    assert np.all(st.sizes <= np.maximum(cap, before))
+    # state is the canonical function of the moves: lists hold exact pin counts per part
+    rebuilt = O.build_state(hg, K, st.part)
+    for n in range(0, hg.noe, 37):
+        s, L = hg.net_ptr[n], st.ep_ends[n]
+        got = dict(zip(st.ep_parts[s:s + L], st.ep_cnt[s:s + L]))
+        want = dict(zip(rebuilt.ep_parts[s:s + rebuilt.ep_ends[n]], rebuilt.ep_cnt[s:s + rebuilt.ep_ends[n]]))
+        assert got == want
+
+
+def test_tv_par_moves_never_grow_volume_bound():
+    # every accepted move passed leave_gain >= arrival_loss on the frozen
+    # state; with one vertex per round that bounds TV from above
+    hg = H.workload("c1", scale=0.2)
+    K = 8
+    C, max_pw, min_pw = H.ref_constants(hg.nov, K)
+    st, _ = O.tv_ldg(hg, K, C, max_pw, min_pw, par_skip=True)
+    tv0 = O.metrics(hg, K, st.part)["TV"]
+    O.tv_refine_par(hg, st, C, max_pw, min_pw, 3, 1)
+    assert O.metrics(hg, K, st.part)["TV"] <= tv0
