@@ -1,0 +1,406 @@
+// fp_cn.hip -- CN (cut-net) refinement on the device: the lambda = 2 push of
+// CN/partition.cpp:618-779 with one pass per round, evaluated on the state
+// frozen at the pass start and applied in (net, pin) order, i.e. the
+// reference's partition_par with one thread (CN/partition.cpp:436-836).
+//
+// A pass:
+//   k_cn_eval     one wavefront filters 64 nets at a time; every lambda = 2
+//                 net picks old = the part with the smaller pin count (ties:
+//                 the second entry) and is accepted iff none of its old-side
+//                 pins lies on an uncut net (CN/partition.cpp:650-690)
+//   scan          push offsets: exclusive scan of the accepted nets' old-side
+//                 pin counts, in net order
+//   k_cn_pushes   push list in (net, pin) order
+//   k_cn_prep     per push: its source part (the new part of the vertex's
+//                 previous pushing net, else its part at pass start -- a
+//                 vertex can be pushed by several nets) and the ops it puts
+//                 on each of the vertex's nets
+//   k_cn_alloc / k_cn_scatter   per touched net, the op list (push indices)
+//   k_cn_apply    one wavefront per touched net: sorts its ops by push index
+//                 in LDS and replays remove/add on the list
+//   k_cn_commit   part vector and part sizes (CN/partition.cpp:786-789)
+#include "fp_internal.h"
+
+namespace fpe {
+namespace {
+
+constexpr int CN_MAXOPS = 4096;  // per net and pass, sorted in LDS
+constexpr int SCAN_BLOCK = 1024;
+
+struct CnEvalArgs {
+  const unsigned* net_ptr;
+  const unsigned* net_pins;
+  const unsigned* vtx_ptr;
+  const unsigned* vtx_seg;
+  const unsigned* ent;
+  const unsigned short* ev;
+  const int* part;
+  int noe, pb;
+  unsigned pmask;
+  int* cn_old;  // accepted: old part; else -1
+  int* cn_new;
+  int* npush;
+};
+
+__global__ void __launch_bounds__(256) k_cn_eval(CnEvalArgs a) {
+  const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
+  for (int n0 = ((int)blockIdx.x * nw + wave) * WAVE; n0 < a.noe; n0 += (int)gridDim.x * nw * WAVE) {
+    const int n = n0 + lane;
+    bool two = false;
+    if (n < a.noe) {
+      two = a.ev[a.net_ptr[n] + (unsigned)n] == 2;
+      if (!two) { a.cn_old[n] = -1; a.npush[n] = 0; }
+    }
+    unsigned long long m = __ballot(two);
+    while (m) {
+      const int nn = n0 + __ffsll((long long)m) - 1;
+      m &= m - 1;
+      const unsigned s = a.net_ptr[nn], e = a.net_ptr[nn + 1];
+      const unsigned seg = s + (unsigned)nn;
+      const unsigned x1 = a.ent[seg + 1], x2 = a.ent[seg + 2];
+      const int p1 = (int)(x1 & a.pmask), c1 = (int)(x1 >> a.pb);
+      const int p2 = (int)(x2 & a.pmask), c2 = (int)(x2 >> a.pb);
+      const int oldp = c1 < c2 ? p1 : p2, newp = c1 < c2 ? p2 : p1;
+      bool bad = false;
+      int cnt = 0;
+      for (unsigned k0 = s; k0 < e && !bad; k0 += WAVE) {
+        const unsigned k = k0 + (unsigned)lane;
+        bool isold = false, b = false;
+        if (k < e) {
+          const int v = (int)a.net_pins[k];
+          isold = a.part[v] == oldp;
+          if (isold)
+            for (unsigned j = a.vtx_ptr[v]; j < a.vtx_ptr[v + 1] && !b; j++) b = a.ev[a.vtx_seg[j]] == 1;
+        }
+        cnt += __popcll(__ballot(isold));
+        bad = __ballot(b) != 0;
+      }
+      if (lane == 0) {
+        a.cn_old[nn] = bad ? -1 : oldp;
+        a.cn_new[nn] = newp;
+        a.npush[nn] = bad ? 0 : cnt;
+      }
+    }
+  }
+}
+
+// exclusive scan of an int array, three launches
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_sums(const int* __restrict__ x, int n, long long* __restrict__ sums) {
+  __shared__ long long scratch[SCAN_BLOCK / WAVE];
+  const int i = (int)(blockIdx.x * SCAN_BLOCK + threadIdx.x);
+  long long tot;
+  (void)block_excl_scan<long long>(i < n ? x[i] : 0, scratch, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_top(long long* __restrict__ sums, int nblocks,
+                                                         long long* __restrict__ total) {
+  __shared__ long long scratch[SCAN_BLOCK / WAVE];
+  __shared__ long long carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int b0 = 0; b0 < nblocks; b0 += SCAN_BLOCK) {
+    const int b = b0 + (int)threadIdx.x;
+    const long long v = b < nblocks ? sums[b] : 0;
+    long long tot;
+    const long long ex = block_excl_scan(v, scratch, &tot);
+    if (b < nblocks) sums[b] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const int* __restrict__ x, int n,
+                                                           const long long* __restrict__ sums,
+                                                           long long* __restrict__ out) {
+  __shared__ long long scratch[SCAN_BLOCK / WAVE];
+  const int i = (int)(blockIdx.x * SCAN_BLOCK + threadIdx.x);
+  long long tot;
+  const long long ex = block_excl_scan<long long>(i < n ? x[i] : 0, scratch, &tot);
+  if (i < n) out[i] = sums[blockIdx.x] + ex;
+}
+
+// the pushes of every accepted net, at its offset, in pin order
+__global__ void __launch_bounds__(256) k_cn_pushes(const unsigned* __restrict__ net_ptr,
+                                                   const unsigned* __restrict__ net_pins, const int* __restrict__ part,
+                                                   const int* __restrict__ cn_old, const long long* __restrict__ pbase,
+                                                   int noe, int* __restrict__ mv_v, int* __restrict__ mv_net) {
+  const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
+  for (int n0 = ((int)blockIdx.x * nw + wave) * WAVE; n0 < noe; n0 += (int)gridDim.x * nw * WAVE) {
+    const int n = n0 + lane;
+    unsigned long long m = __ballot(n < noe && cn_old[n] >= 0);
+    while (m) {
+      const int nn = n0 + __ffsll((long long)m) - 1;
+      m &= m - 1;
+      const int oldp = cn_old[nn];
+      long long pos = pbase[nn];
+      for (unsigned k0 = net_ptr[nn]; k0 < net_ptr[nn + 1]; k0 += WAVE) {
+        const unsigned k = k0 + (unsigned)lane;
+        int v = -1;
+        bool isold = false;
+        if (k < net_ptr[nn + 1]) {
+          v = (int)net_pins[k];
+          isold = part[v] == oldp;
+        }
+        const unsigned long long b = __ballot(isold);
+        if (isold) {
+          const long long at = pos + __popcll(b & ((1ull << lane) - 1ull));
+          mv_v[at] = v;
+          mv_net[at] = nn;
+        }
+        pos += __popcll(b);
+      }
+    }
+  }
+}
+
+struct CnPrepArgs {
+  const int* mv_v;
+  const int* mv_net;
+  long long nmoves;
+  const unsigned* vtx_ptr;
+  const unsigned* vtx_nets;
+  const int* part;
+  const int* cn_old;
+  const int* cn_new;
+  int* mv_from;
+  int* mv_last;
+  unsigned tag;
+  unsigned* net_stamp;
+  int* touched;
+  int* tcount;
+  int* opcnt;
+  unsigned long long* totals;  // [0] ops
+};
+
+// Source part of each push: a vertex v can be pushed by several nets of the
+// pass (every accepted net whose old part is v's part at pass start); pushes
+// apply in net order, so the source of the push by net n is the new part of
+// the largest such net below n, else v's part at pass start.
+__global__ void k_cn_prep(CnPrepArgs a) {
+  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= a.nmoves) return;
+  const int v = a.mv_v[m], n = a.mv_net[m];
+  const int p0 = a.part[v];
+  int prev = -1;
+  bool last = true;
+  const unsigned vb = a.vtx_ptr[v], ve = a.vtx_ptr[v + 1];
+  for (unsigned j = vb; j < ve; j++) {
+    const int k = (int)a.vtx_nets[j];
+    if (k != n && a.cn_old[k] == p0) {
+      if (k < n) prev = max(prev, k);
+      else last = false;
+    }
+  }
+  a.mv_from[m] = prev >= 0 ? a.cn_new[prev] : p0;
+  a.mv_last[m] = last ? 1 : 0;
+  for (unsigned j = vb; j < ve; j++) {
+    const unsigned t = a.vtx_nets[j];
+    if (atomicExch(&a.net_stamp[t], a.tag) != a.tag) a.touched[atomicAdd(a.tcount, 1)] = (int)t;
+    atomicAdd(&a.opcnt[t], 1);
+  }
+  atomicAdd(&a.totals[0], (unsigned long long)(ve - vb));
+}
+
+__global__ void k_cn_alloc(const int* __restrict__ touched, const int* __restrict__ tcount,
+                           const int* __restrict__ opcnt, long long* __restrict__ opoff, int* __restrict__ opfill,
+                           unsigned long long* __restrict__ totals) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= *tcount) return;
+  const int t = touched[i];
+  const int c = opcnt[t];
+  opoff[t] = (long long)atomicAdd(&totals[1], (unsigned long long)c);
+  opfill[t] = 0;
+  atomicMax(reinterpret_cast<unsigned long long*>(&totals[2]), (unsigned long long)c);
+}
+
+__global__ void k_cn_scatter(const int* __restrict__ mv_v, long long nmoves, const unsigned* __restrict__ vtx_ptr,
+                             const unsigned* __restrict__ vtx_nets, const long long* __restrict__ opoff,
+                             int* __restrict__ opfill, int* __restrict__ ops) {
+  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nmoves) return;
+  const int v = mv_v[m];
+  for (unsigned j = vtx_ptr[v]; j < vtx_ptr[v + 1]; j++) {
+    const unsigned t = vtx_nets[j];
+    ops[opoff[t] + atomicAdd(&opfill[t], 1)] = (int)m;
+  }
+}
+
+struct CnApplyArgs {
+  const unsigned* net_ptr;
+  const int* touched;
+  const int* tcount;
+  const int* opcnt;
+  const long long* opoff;
+  const int* ops;
+  const int* mv_from;
+  const int* mv_net;
+  const int* cn_new;
+  unsigned* ent;
+  unsigned short* ev;
+  int K, pb;
+  unsigned pmask;
+  int* opcnt_reset;
+};
+
+// ascending sort of keys[0..P) in LDS by one wavefront (P a power of two)
+__device__ void wave_bitonic_sort(unsigned* keys, int P) {
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = lane_id(); i < P; i += WAVE) {
+        const int l = i ^ j;
+        if (l > i) {
+          const unsigned x = keys[i], y = keys[l];
+          const bool up = (i & k) == 0;
+          if ((x > y) == up) { keys[i] = y; keys[l] = x; }
+        }
+      }
+      wave_lds_sync();
+    }
+  }
+}
+
+__global__ void __launch_bounds__(WAVE) k_cn_apply(CnApplyArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  unsigned* list = reinterpret_cast<unsigned*>(lds);
+  unsigned* keys = list + a.K;
+  const int lane = lane_id();
+  const int nt = *a.tcount;
+  for (int i = (int)blockIdx.x; i < nt; i += (int)gridDim.x) {
+    const int t = a.touched[i];
+    const int c = a.opcnt[t];
+    const long long off = a.opoff[t];
+    int P = 1;
+    while (P < c) P <<= 1;
+    for (int j = lane; j < P; j += WAVE) keys[j] = j < c ? (unsigned)a.ops[off + j] : 0xFFFFFFFFu;
+    wave_lds_sync();
+    if (c > 1) wave_bitonic_sort(keys, P);
+    const unsigned seg = a.net_ptr[t] + (unsigned)t;
+    int L = wave_list_load(a.ent, seg, list);
+    for (int j = 0; j < c; j++) {
+      const int m = (int)keys[j];
+      wave_list_move(list, L, a.mv_from[m], a.cn_new[a.mv_net[m]], a.pb, a.pmask);
+    }
+    wave_list_store(a.ent, a.ev, seg, list, L, a.pb, a.pmask);
+    if (lane == 0) a.opcnt_reset[t] = 0;
+    wave_lds_sync();
+  }
+}
+
+__global__ void k_cn_commit(const int* __restrict__ mv_v, const int* __restrict__ mv_net,
+                            const int* __restrict__ mv_from, const int* __restrict__ mv_last, long long nmoves,
+                            const int* __restrict__ cn_new, const int* __restrict__ vwgt, int* __restrict__ part,
+                            long long* __restrict__ sizes) {
+  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nmoves) return;
+  const int v = mv_v[m];
+  const int to = cn_new[mv_net[m]];
+  const long long w = vwgt ? vwgt[v] : 1;
+  atomic_add_i64(&sizes[mv_from[m]], -w);
+  atomic_add_i64(&sizes[to], w);
+  if (mv_last[m]) part[v] = to;
+}
+
+}  // namespace
+
+fp_status cn_refine(fp_ctx* ctx, int passes, fp_refine_stats* stats) {
+  const int noe = ctx->noe;
+  const long long pins = ctx->pins;
+  const int nblk = (noe + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  auto& s = ctx->cn;
+  if (!s.ready) {
+    const size_t ne = (size_t)std::max(noe, 1), np = (size_t)std::max(pins, 1ll);
+    FP_TRY(dalloc(ctx, &s.old_part, ne));
+    FP_TRY(dalloc(ctx, &s.new_part, ne));
+    FP_TRY(dalloc(ctx, &s.npush, ne));
+    FP_TRY(dalloc(ctx, &s.pbase, ne));
+    FP_TRY(dalloc(ctx, &s.opoff, ne));
+    FP_TRY(dalloc(ctx, &s.opcnt, ne));
+    FP_TRY(dalloc(ctx, &s.opfill, ne));
+    FP_TRY(dalloc(ctx, &s.sums, (size_t)std::max(nblk, 1)));
+    FP_TRY(dalloc(ctx, &s.mv_v, np));
+    FP_TRY(dalloc(ctx, &s.mv_net, np));
+    FP_TRY(dalloc(ctx, &s.mv_from, np));
+    FP_TRY(dalloc(ctx, &s.mv_last, np));
+    FP_TRY(dalloc(ctx, &s.totals, 8));
+    fill_async<int>(ctx, s.opcnt, ne, 0);
+    s.ready = true;
+  }
+  const size_t lds_apply = (size_t)ctx->K * sizeof(unsigned) + (size_t)CN_MAXOPS * sizeof(unsigned);
+  if (lds_apply > (size_t)LDS_BUDGET) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d too large for the CN apply", ctx->K);
+  long long moves = 0;
+  hipEvent_t e0 = pooled_event(ctx, 0), e1 = pooled_event(ctx, 1);
+  if (!e0 || !e1) return fail(ctx, FP_ERR_HIP, "hipEventCreate failed");
+  float total_ms = 0;
+  for (int it = 0; it < passes; it++) {
+    FP_HIP_OK(hipEventRecord(e0, ctx->stream));
+    CnEvalArgs ea{ctx->net_ptr, ctx->net_pins, ctx->vtx_ptr, ctx->vtx_seg, ctx->ent, ctx->ev, ctx->part,
+                   noe, ctx->pb, ctx->pmask, s.old_part, s.new_part, s.npush};
+    const int waves_needed = (noe + WAVE - 1) / WAVE;
+    k_cn_eval<<<std::max(1, std::min((waves_needed + 3) / 4, 8192)), 256, 0, ctx->stream>>>(ea);
+    k_scan_sums<<<std::max(nblk, 1), SCAN_BLOCK, 0, ctx->stream>>>(s.npush, noe, s.sums);
+    k_scan_top<<<1, SCAN_BLOCK, 0, ctx->stream>>>(s.sums, nblk, reinterpret_cast<long long*>(s.totals + 3));
+    k_scan_apply<<<std::max(nblk, 1), SCAN_BLOCK, 0, ctx->stream>>>(s.npush, noe, s.sums, s.pbase);
+    k_cn_pushes<<<std::max(1, std::min((waves_needed + 3) / 4, 8192)), 256, 0, ctx->stream>>>(
+        ctx->net_ptr, ctx->net_pins, ctx->part, s.old_part, s.pbase, noe, s.mv_v, s.mv_net);
+    unsigned long long nm = 0;
+    FP_TRY(d2h(ctx, &nm, s.totals + 3, 1));
+    FP_HIP_OK(hipStreamSynchronize(ctx->stream));
+    const long long M = (long long)nm;
+    if (M == 0) {
+      FP_HIP_OK(hipEventRecord(e1, ctx->stream));
+      FP_HIP_OK(hipEventSynchronize(e1));
+      float ms = 0;
+      FP_HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+      total_ms += ms;
+      continue;
+    }
+    moves += M;
+    ctx->tag++;
+    const int par = (int)(ctx->tag & 1u);
+    fill_async<int>(ctx, ctx->counters + par, 1, 0);
+    fill_async<unsigned long long>(ctx, s.totals, 3, 0ull);
+    CnPrepArgs pa{s.mv_v, s.mv_net, M, ctx->vtx_ptr, ctx->vtx_nets, ctx->part, s.old_part, s.new_part, s.mv_from,
+                  s.mv_last, ctx->tag, ctx->net_stamp, ctx->touched, ctx->counters + par, s.opcnt, s.totals};
+    const int gm = (int)((M + 255) / 256);
+    k_cn_prep<<<gm, 256, 0, ctx->stream>>>(pa);
+    k_cn_alloc<<<(noe + 255) / 256, 256, 0, ctx->stream>>>(ctx->touched, ctx->counters + par, s.opcnt, s.opoff,
+                                                           s.opfill, s.totals);
+    unsigned long long tot[3] = {0, 0, 0};
+    FP_TRY(d2h(ctx, tot, s.totals, 3));
+    FP_HIP_OK(hipStreamSynchronize(ctx->stream));
+    if ((long long)tot[2] > CN_MAXOPS) {
+      ctx->has_state = false;
+      return fail(ctx, FP_ERR_UNSUPPORTED, "a net receives %llu moves in one CN pass (limit %d); state discarded",
+                  tot[2], CN_MAXOPS);
+    }
+    if ((long long)tot[0] > s.ops_cap) {
+      s.ops_cap = std::max((long long)tot[0], 2 * s.ops_cap);
+      FP_TRY(dalloc(ctx, &s.ops, (size_t)s.ops_cap));
+    }
+    k_cn_scatter<<<gm, 256, 0, ctx->stream>>>(s.mv_v, M, ctx->vtx_ptr, ctx->vtx_nets, s.opoff, s.opfill, s.ops);
+    CnApplyArgs aa{ctx->net_ptr, ctx->touched, ctx->counters + par, s.opcnt, s.opoff, s.ops, s.mv_from, s.mv_net,
+                   s.new_part, ctx->ent, ctx->ev, ctx->K, ctx->pb, ctx->pmask, s.opcnt};
+    k_cn_apply<<<8192, WAVE, lds_apply, ctx->stream>>>(aa);
+    k_cn_commit<<<gm, 256, 0, ctx->stream>>>(s.mv_v, s.mv_net, s.mv_from, s.mv_last, M, s.new_part, ctx->vwgt,
+                                              ctx->part, ctx->sizes);
+    FP_HIP_OK(hipGetLastError());
+    FP_HIP_OK(hipEventRecord(e1, ctx->stream));
+    FP_HIP_OK(hipEventSynchronize(e1));
+    float ms = 0;
+    FP_HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    total_ms += ms;
+  }
+  FP_HIP_OK(hipStreamSynchronize(ctx->stream));
+  if (stats) {
+    stats->moves = moves;
+    stats->rounds = passes;
+    stats->pin_visits = (long long)passes * ctx->pins;
+    stats->device_ms = total_ms;
+  }
+  return FP_OK;
+}
+
+}  // namespace fpe

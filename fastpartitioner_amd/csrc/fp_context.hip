@@ -168,6 +168,8 @@ fp_status fp_upload(fp_ctx* ctx, int nov, int noe, const unsigned* net_ptr, cons
   ctx->K = 0;
   ctx->rmax = 0;
   ctx->hist_cap = 0;
+  ctx->cn.ready = false;
+  ctx->cn.ops_cap = 0;
   if (net_ptr[0] != 0) return fail(ctx, FP_ERR_ARG, "fp_upload: net_ptr[0] != 0");
   unsigned max_net = 0;
   for (int n = 0; n < noe; n++) {

@@ -163,6 +163,57 @@ __device__ __forceinline__ int wave_find(const unsigned* list, int L, int p, uns
   return -1;
 }
 
+// One move's edit of one net's connectivity list held in LDS (one wavefront):
+// remove one pin of part p (decrement, or swap the last entry into its slot
+// when its count reaches 0) then add one pin of part q (increment, or append);
+// TV/partition.cpp:388-435. L is the list length (wave-uniform).
+__device__ __forceinline__ void wave_list_move(unsigned* list, int& L, int p, int q, int pb, unsigned pmask) {
+  const int lane = lane_id();
+  const unsigned inc = 1u << pb;
+  int j = wave_find(list, L, p, pmask);
+  if (j >= 0) {
+    const unsigned x = list[j];
+    if ((x >> pb) > 1u) {
+      if (lane == 0) list[j] = x - inc;
+    } else {
+      if (lane == 0) list[j] = list[L - 1];
+      L--;
+    }
+    wave_lds_sync();
+  }
+  j = wave_find(list, L, q, pmask);
+  if (j >= 0) {
+    if (lane == 0) list[j] += inc;
+  } else {
+    if (lane == 0) list[L] = inc | (unsigned)q;
+    L++;
+  }
+  wave_lds_sync();
+}
+
+// Stages net n's list (segment at seg = net_ptr[n] + n) into LDS; returns lambda.
+__device__ __forceinline__ int wave_list_load(const unsigned* ent, unsigned seg, unsigned* list) {
+  const int L = (int)ent[seg];
+  for (int t = lane_id(); t < L; t += WAVE) list[t] = ent[seg + 1 + (unsigned)t];
+  wave_lds_sync();
+  return L;
+}
+
+// Writes a list back to both the packed segment and its evaluation view.
+__device__ __forceinline__ void wave_list_store(unsigned* ent, unsigned short* ev, unsigned seg, const unsigned* list,
+                                                int L, int pb, unsigned pmask) {
+  const int lane = lane_id();
+  for (int t = lane; t < L; t += WAVE) {
+    const unsigned x = list[t];
+    ent[seg + 1 + (unsigned)t] = x;
+    ev[seg + 1 + (unsigned)t] = ev_word(x, pb, pmask);
+  }
+  if (lane == 0) {
+    ent[seg] = (unsigned)L;
+    ev[seg] = (unsigned short)L;
+  }
+}
+
 __device__ __forceinline__ void atomic_add_i64(long long* p, long long v) {
   atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
 }
@@ -218,6 +269,17 @@ struct fp_ctx {
   int *over_idx = nullptr, *over_count = nullptr;
   unsigned long long* moves = nullptr;
 
+  // CN refinement scratch (allocated on the first CN refinement of an upload)
+  struct {
+    bool ready = false;
+    int *old_part = nullptr, *new_part = nullptr, *npush = nullptr;
+    long long *pbase = nullptr, *sums = nullptr, *opoff = nullptr;
+    int *mv_v = nullptr, *mv_net = nullptr, *mv_from = nullptr, *mv_last = nullptr;
+    int *opcnt = nullptr, *opfill = nullptr, *ops = nullptr;
+    long long ops_cap = 0;
+    unsigned long long* totals = nullptr;
+  } cn;
+
   // metric scratch
   unsigned* p2p = nullptr;
   unsigned long long* mvec = nullptr;  // 5*K + 2
@@ -232,6 +294,7 @@ struct fp_ctx {
 namespace fpe {
 
 fp_status fail(fp_ctx* c, fp_status st, const char* fmt, ...);
+fp_status cn_refine(fp_ctx* ctx, int passes, fp_refine_stats* stats);
 fp_status dalloc_bytes(fp_ctx* ctx, void** p, size_t bytes);
 void dfree_all(fp_ctx* ctx);
 hipEvent_t pooled_event(fp_ctx* ctx, size_t i);

@@ -414,17 +414,14 @@ __global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
     return;
   }
   unsigned* list = reinterpret_cast<unsigned*>(lds) + (size_t)wave * (size_t)a.K;
-  const unsigned inc = 1u << a.pb;
   const int ntouched = *a.tcount;
   for (int i = (int)blockIdx.x * nw + wave; i < ntouched; i += a.net_blocks * nw) {
     const int n = a.touched[i];
     const unsigned s = a.net_ptr[n], e = a.net_ptr[n + 1];
-    unsigned* seg = a.ent + s + (unsigned)n;
-    int L = (int)seg[0];
-    for (int t = lane; t < L; t += WAVE) list[t] = seg[1 + t];
+    const unsigned seg = s + (unsigned)n;
+    int L = wave_list_load(a.ent, seg, list);
     const unsigned lo = wave_lower_bound(a.apply_pins, s, e, a.r0);
     const unsigned hi = wave_lower_bound(a.apply_pins, lo, e, a.r1);
-    wave_lds_sync();
     bool dirty = false;
     for (unsigned k0 = lo; k0 < hi; k0 += WAVE) {
       const unsigned k = k0 + (unsigned)lane;
@@ -439,42 +436,10 @@ __global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
         dirty = true;
         const int l = __ffsll((long long)m) - 1;
         m &= m - 1;
-        const int p = __shfl(oo, l), q = __shfl(dd, l);
-        // remove one pin of p
-        int j = wave_find(list, L, p, a.pmask);
-        if (j >= 0) {
-          const unsigned x = list[j];
-          if ((x >> a.pb) > 1u) {
-            if (lane == 0) list[j] = x - inc;
-          } else {
-            if (lane == 0) list[j] = list[L - 1];
-            L--;
-          }
-          wave_lds_sync();
-        }
-        // add one pin of q
-        j = wave_find(list, L, q, a.pmask);
-        if (j >= 0) {
-          if (lane == 0) list[j] += inc;
-        } else {
-          if (lane == 0) list[L] = inc | (unsigned)q;
-          L++;
-        }
-        wave_lds_sync();
+        wave_list_move(list, L, __shfl(oo, l), __shfl(dd, l), a.pb, a.pmask);
       }
     }
-    if (dirty) {
-      unsigned short* vseg = a.ev + s + (unsigned)n;
-      for (int t = lane; t < L; t += WAVE) {
-        const unsigned x = list[t];
-        seg[1 + t] = x;
-        vseg[1 + t] = ev_word(x, a.pb, a.pmask);
-      }
-      if (lane == 0) {
-        seg[0] = (unsigned)L;
-        vseg[0] = (unsigned short)L;
-      }
-    }
+    if (dirty) wave_list_store(a.ent, a.ev, seg, list, L, a.pb, a.pmask);
     wave_lds_sync();
   }
 }
@@ -561,8 +526,12 @@ fp_status fp_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule
   if (!ctx->has_state) return fail(ctx, FP_ERR_STATE, "fp_refine before a state was set");
   FP_TRY(check_sched(ctx, sched));
   if (passes < 0) return fail(ctx, FP_ERR_ARG, "passes < 0");
-  if (metric != FP_METRIC_TV) return fail(ctx, FP_ERR_UNSUPPORTED, "refinement for metric %d not available", (int)metric);
   if (ctx->dup_pins) return fail(ctx, FP_ERR_UNSUPPORTED, "a net lists the same vertex twice; refinement needs distinct pins");
+  if (metric == FP_METRIC_CN) {
+    FP_HIP_OK(hipSetDevice(ctx->device));
+    return cn_refine(ctx, passes, stats);
+  }
+  if (metric != FP_METRIC_TV) return fail(ctx, FP_ERR_UNSUPPORTED, "refinement for metric %d not available", (int)metric);
   const int nov = ctx->nov, K = ctx->K;
   FP_TRY(check_envelope(ctx));
   const int waves = eval_waves(K);

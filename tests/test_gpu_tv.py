@@ -88,3 +88,21 @@ def test_gains_match_oracle():
         dec, gain = eng.eval_moves("TV", schedule(SCHED_REF, hg.nov, C, max_pw, min_pw))
     O.tv_refine_par(hg, st.copy(), C, max_pw, min_pw, 1, hg.nov, gains=gains)
     assert np.array_equal(gain, gains)
+
+
+@pytest.mark.parametrize("name,scale,K", [("c1", 1.0, 16), ("c2", 0.02, 64)])
+def test_cn_refinement_matches_oracle(name, scale, K):
+    hg = H.workload(name, scale=scale)
+    C, _, _ = H.ref_constants(hg.nov, K)
+    st = O.cn_ldg(hg, K, C)
+    with Engine(0) as eng:
+        eng.upload(hg)
+        eng.set_state(K, st.part, st.ep_ends, st.ep_parts, st.ep_cnt)
+        stats = eng.refine("CN", 6, schedule(SCHED_REF, 1, C, 0, 0))
+        got = eng.get_state()
+        m = eng.metrics()
+    O.cn_refine(hg, st, 6, serial=False)
+    _same_state(got, st)
+    assert np.array_equal(got.sizes, st.sizes.astype(np.int64))
+    assert m["CN"] == O.metrics(hg, K, st.part)["CN"]
+    assert stats.moves > 0
