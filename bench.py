@@ -39,6 +39,18 @@ def lambda_visits(hg, lam) -> int:
     return int(np.dot(sizes, lam.astype(np.int64)))
 
 
+def measured_traffic(args):
+    """HBM bytes per k_tv_eval launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this exact workload and round size, or None."""
+    path = os.path.join(ROOT, "profiles", "tv_eval_traffic.json")
+    key = f"colnet-zipf2.1-1M-K256-TV/{args.sched}/{args.round}"
+    if args.scale != 1.0 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f).get(key)
+    return None if rec is None else rec["traffic_bytes"]
+
+
 def cpu_baseline(hg, K, passes, threads):
     from oracle import oracle as O
 
@@ -56,7 +68,7 @@ def cpu_baseline(hg, K, passes, threads):
             try:
                 r = O.run_reference("tv", hg, K, passes, t, timeout=900, env_extra=env)
             except RuntimeError as e:
-                crashes.append(f"{t} threads: {str(e)[-60:]}")
+                crashes.append(f"{t} threads: " + ("segfault" if "rc=-11" in str(e) else str(e)[-40:]))
                 continue
             secs = r["refine_seconds"]
             after = [ln for ln in r["stdout"].splitlines() if ln.startswith("After Refinement TV")]
@@ -143,16 +155,18 @@ def main():
     secs = t1 - t0
     value = args.steps * hg.pins / secs
 
-    # algorithmic bytes of the evaluation kernel per pass (packed u32 list entries):
-    # 16 B per vertex (vtx_ptr, part, slot_own/slot_dec), 12 B per incidence
-    # (vtx_nets, net_ptr, lambda), 4 B per list entry visited (Lambda)
+    # algorithmic bytes per pass of the evaluation step, per-unit model of the
+    # refinement's data in the reference layout (int32 part + int32 count per
+    # list entry), each byte counted once per use, no cache credit:
+    # 12 B per vertex, 12 B per (vertex, net) incidence, 8 B per list entry
+    # visited (Lambda = sum over nets of |n| * lambda(n))
     lam_avg = 0.5 * (lam0 + lam1)
-    bytes_pass = 16 * hg.nov + 12 * hg.pins + 4 * lam_avg
+    bytes_pass = 12 * hg.nov + 12 * hg.pins + 8 * lam_avg
     avg_launch_ms = eval_ms / max(launches, 1)
     bytes_launch = bytes_pass * args.steps / max(launches, 1)
     achieved = bytes_launch / (avg_launch_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args),
             "kernel": "k_tv_eval", "launches": launches, "avg_launch_us": round(avg_launch_ms * 1e3, 2),
             "bytes_per_launch": int(bytes_launch)}
 
