@@ -2,8 +2,10 @@
 
   libfastpart_host.so  g++    csrc/fp_host.cpp         (host helpers)
   libfastpart.so       hipcc  csrc/fp_{context,tv,cn,metrics}.hip  (HIP engine, gfx950)
+  libfastpart_legacy_<m>.so  g++  csrc/fp_legacy.cpp  (the reference's partition_gpu_wrapper
+                       for objective <m>, over libfastpart.so)
 
-`python -m fastpartitioner_amd.build` builds both.
+`python -m fastpartitioner_amd.build` builds all of them.
 """
 from __future__ import annotations
 
@@ -50,8 +52,19 @@ def build_engine():
     return out
 
 
+LEGACY_OBJECTIVES = ["tv", "cn"]
+
+
+def build_legacy(objective):
+    out = os.path.join(HERE, f"libfastpart_legacy_{objective}.so")
+    _run([CXX, "-O2", "-fPIC", "-shared", "-fopenmp", "-std=c++17", "-Wall", f"-DFP_LEGACY_{objective.upper()}",
+          "-o", out, os.path.join(CSRC, "fp_legacy.cpp"), f"-L{HERE}", "-lfastpart", "-lfastpart_host",
+          "-Wl,-rpath,$ORIGIN"])
+    return out
+
+
 def build_all():
-    return [build_host(), build_engine()]
+    return [build_host(), build_engine()] + [build_legacy(m) for m in LEGACY_OBJECTIVES]
 
 
 if __name__ == "__main__":

@@ -108,6 +108,21 @@ using namespace fpe;
 
 extern "C" {
 
+const char* fp_status_string(fp_status st) {
+  switch (st) {
+    case FP_OK: return "ok";
+    case FP_ERR_ARG: return "invalid argument";
+    case FP_ERR_HIP: return "HIP runtime error";
+    case FP_ERR_OOM: return "device allocation failed";
+    case FP_ERR_STATE: return "call out of order";
+    case FP_ERR_UNSUPPORTED: return "input outside the supported envelope";
+    case FP_ERR_BUDGET: return "rejected by the HBM footprint budget";
+    case FP_ERR_IO: return "file I/O error";
+    case FP_ERR_NODEV: return "no usable GPU";
+  }
+  return "unknown status";
+}
+
 fp_status fp_device_count(int* count) {
   if (!count) return FP_ERR_ARG;
   int n = 0;

@@ -524,13 +524,13 @@ extern "C" {
 fp_status fp_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule* sched, fp_refine_stats* stats) {
   if (!ctx) return FP_ERR_ARG;
   if (!ctx->has_state) return fail(ctx, FP_ERR_STATE, "fp_refine before a state was set");
-  FP_TRY(check_sched(ctx, sched));
   if (passes < 0) return fail(ctx, FP_ERR_ARG, "passes < 0");
   if (ctx->dup_pins) return fail(ctx, FP_ERR_UNSUPPORTED, "a net lists the same vertex twice; refinement needs distinct pins");
-  if (metric == FP_METRIC_CN) {
+  if (metric == FP_METRIC_CN) {  // the lambda = 2 push takes no schedule
     FP_HIP_OK(hipSetDevice(ctx->device));
     return cn_refine(ctx, passes, stats);
   }
+  FP_TRY(check_sched(ctx, sched));
   if (metric != FP_METRIC_TV) return fail(ctx, FP_ERR_UNSUPPORTED, "refinement for metric %d not available", (int)metric);
   const int nov = ctx->nov, K = ctx->K;
   FP_TRY(check_envelope(ctx));

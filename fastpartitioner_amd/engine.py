@@ -146,9 +146,11 @@ class Engine:
                     "fp_get_state")
         return st
 
-    def refine(self, metric: str, passes: int, sched: Schedule) -> RefineStats:
+    def refine(self, metric: str, passes: int, sched: Schedule | None = None) -> RefineStats:
+        """`passes` passes of `metric`; sched is required except for CN."""
         stats = RefineStats()
-        self._check(self.lib.fp_refine(self.ctx, METRICS[metric], passes, ctypes.byref(sched),
+        self._check(self.lib.fp_refine(self.ctx, METRICS[metric], passes,
+                                       ctypes.byref(sched) if sched is not None else None,
                                        ctypes.byref(stats)), "fp_refine")
         return stats
 

@@ -97,6 +97,8 @@ typedef struct {
 } fp_metrics;
 
 /* ---- context ------------------------------------------------------------ */
+/* A fixed English name of a status code. */
+const char* fp_status_string(fp_status st);
 fp_status fp_device_count(int* count);
 fp_status fp_ctx_create(fp_ctx** ctx, int device);
 void fp_ctx_destroy(fp_ctx* ctx);
@@ -136,7 +138,9 @@ fp_status fp_get_state(fp_ctx* ctx, int* part, int* ep_ends, int* ep_parts, int*
 /* ---- refinement --------------------------------------------------------- */
 /* `passes` refinement passes of `metric` under `sched` on the resident state.
  * TV: vertex-centric LDG refinement of TV/partition.cpp:307-440 (decision and
- * gain test), rounds per sched. stats may be NULL. */
+ * gain test), rounds per sched.
+ * CN: the lambda = 2 push of CN/partition.cpp:436-836; sched is not used and
+ * may be NULL. stats may be NULL. */
 fp_status fp_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule* sched, fp_refine_stats* stats);
 
 /* One frozen-state evaluation of every vertex without applying anything:

@@ -163,17 +163,20 @@ def cn_refine(hg, st, passes, serial):
 
 # ---- reference harness (oracle/_ref/ref_<m>) -----------------------------
 
-def ref_available(metric: str) -> bool:
-    return os.path.exists(os.path.join(REF_DIR, f"ref_{metric}"))
+def ref_available(metric: str, binary: str = "ref") -> bool:
+    return os.path.exists(os.path.join(REF_DIR, f"{binary}_{metric}"))
 
 
 def run_reference(metric: str, hg, K: int, passes: int, threads: int, timeout: float = 600.0,
-                  env_extra=None):
+                  env_extra=None, binary: str = "ref"):
     """Run the reference's own partition()/partition_par() on `hg` and return
     the states it reached: dict with init_part/init_ends/init_parts and
     part/ends/parts/cnt (after refinement), the wall time and the
-    'Refinement took' seconds it printed."""
-    exe = os.path.join(REF_DIR, f"ref_{metric}")
+    'Refinement took' seconds it printed.
+
+    binary="dropin" with threads=-1 runs the reference's partition_gpu(),
+    whose device entry point is served by libfastpart_legacy_<metric>.so."""
+    exe = os.path.join(REF_DIR, f"{binary}_{metric}")
     with tempfile.TemporaryDirectory() as d:
         inst = os.path.join(d, "g.fphg")
         out = os.path.join(d, "g.fprs")

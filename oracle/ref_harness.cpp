@@ -22,6 +22,12 @@
 // Usage: ref_<m> <instance.fphg> <K> <passes> <threads> <out.fprs>
 //   threads == 0 -> partition()            (serial schedule)
 //   threads >= 1 -> partition_par(threads)  (round schedule, OpenMP)
+//
+// Built with -DREF_DROPIN (binaries dropin_<m>): threads == -1 runs the
+// reference's partition_gpu() (TV/partition.cpp:984, CN/partition.cpp:1599),
+// whose partition_gpu_wrapper then resolves to this repository's
+// libfastpart_legacy_<m>.so -- the reference driving the engine through its
+// own device entry point.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -40,11 +46,15 @@ void partition(unsigned int*, unsigned int*, unsigned int*, int, int, long long,
                string&, int&, int&, unsigned int*, unsigned int*);
 void partition_par(unsigned int*, unsigned int*, unsigned int*, int, int, long long, int*, int*, int, int*,
                    string, int, int, int, unsigned int*, unsigned int*);
+void partition_gpu(unsigned int*, unsigned int*, unsigned int*, int, int, long long, int*, int*, int, int*,
+                   string, int, int, unsigned int*, unsigned int*);
 #else
 void partition(unsigned int*, unsigned int*, unsigned int*, int, int, long long, int*, int*, int, int*,
                string, int, int);
 void partition_par(unsigned int*, unsigned int*, unsigned int*, int, int, long long, int*, int*, int, int*,
                    string, int, int, int);
+void partition_gpu(unsigned int*, unsigned int*, unsigned int*, int, int, long long, int*, int*, int, int*,
+                   string, int, int);
 #endif
 
 extern "C" int __real__Z10TM_scratchiPiS_S_S_iiS_(int, int*, int*, int*, int*, int, int, int*);
@@ -118,6 +128,17 @@ int main(int argc, char** argv) {
   int refi_opt = passes > 0 ? 1 : 0, refi_iter = passes;
 
   auto t0 = chrono::steady_clock::now();
+#if defined(REF_DROPIN) && defined(REF_CN)
+  if (nt == -1)
+    partition_gpu(row_ptr, col_ind, net_app, nov, noe, pins, edge_w, src_nodes, K, ordering, fname, refi_opt,
+                  refi_iter, col_ptr, row_ind);
+  else
+#elif defined(REF_DROPIN)
+  if (nt == -1)
+    partition_gpu(row_ptr, col_ind, net_app, nov, noe, pins, edge_w, src_nodes, K, ordering, fname, refi_opt,
+                  refi_iter);
+  else
+#endif
 #if defined(REF_CN)
   if (nt == 0)
     partition(row_ptr, col_ind, net_app, nov, noe, pins, edge_w, src_nodes, K, ordering, fname, refi_opt,

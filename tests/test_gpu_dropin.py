@@ -1,0 +1,73 @@
+"""The reference's own partition_gpu() driving the engine through its own
+device entry point.
+
+oracle/_ref/dropin_<m> is the reference's partitioner for objective <m>
+(compiled from its sources) whose partition_gpu_wrapper resolves to
+libfastpart_legacy_<m>.so. The reference builds its initial partition on the
+host, hands its arrays to the wrapper, and prints metrics from what comes
+back. The state it ends with must equal the oracle's refinement of the state
+it handed over.
+"""
+import numpy as np
+import pytest
+
+from fastpartitioner_amd import hypergraph as H
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _dropin(metric, hg, K, passes, env=None):
+    if not O.ref_available(metric, "dropin"):
+        pytest.skip(f"oracle/_ref/dropin_{metric} not built (needs the reference sources at build time)")
+    print(f"dropin_{metric}: nov={hg.nov} K={K} passes={passes} env={env}", flush=True)
+    return O.run_reference(metric, hg, K, passes, -1, timeout=120, env_extra=env, binary="dropin")
+
+
+def _assert_final(r, st):
+    assert np.array_equal(st.part, r["part"])
+    assert np.array_equal(st.ep_ends, r["ends"])
+    assert np.array_equal(st.ep_parts, r["parts"])
+    assert np.array_equal(st.ep_cnt, r["cnt"])
+
+
+@pytest.mark.parametrize("R", [8, 256])
+def test_reference_partition_gpu_tv(R):
+    hg = H.colnet(3000, H.DEG_UNIFORM, 2, 9, seed=5)
+    K, passes = 12, 4
+    r = _dropin("tv", hg, K, passes, {"FASTPART_ROUND": str(R)})
+    st = O.build_state(hg, K, r["init_part"])
+    # the lists the reference's stream left are the first-appearance lists
+    assert np.array_equal(st.ep_parts, r["init_parts"])
+    st.sizes = st.sizes.astype(np.int32)
+    C, max_pw, min_pw = H.ref_constants(hg.nov, K)
+    O.tv_refine_par(hg, st, C, max_pw, min_pw, passes, R)
+    _assert_final(r, st)
+    m = O.metrics(hg, K, st.part)
+    assert f"After Refinement TV : {m['TV']}\n" in r["stdout"]
+
+
+def test_reference_partition_gpu_cn():
+    hg = H.colnet(3000, H.DEG_UNIFORM, 2, 9, seed=8)
+    K, passes = 12, 4
+    r = _dropin("cn", hg, K, passes)
+    st = O.build_state(hg, K, r["init_part"])
+    st.sizes = st.sizes.astype(np.int32)
+    # CN's stream appends in its own order; take the lists it handed over
+    st.ep_ends[:] = r["init_ends"]
+    st.ep_parts[:] = r["init_parts"]
+    _recount(hg, st)
+    O.cn_refine(hg, st, passes, serial=False)
+    _assert_final(r, st)
+    m = O.metrics(hg, K, st.part)
+    assert f"After Refinement CN : {m['CN']}\n" in r["stdout"]
+
+
+def _recount(hg, st):
+    """ep_cnt of the lists in st.ep_parts: pins of each net per listed part."""
+    st.ep_cnt[:] = 0
+    for n in range(hg.noe):
+        s, L = int(hg.net_ptr[n]), int(st.ep_ends[n])
+        slot = {int(st.ep_parts[s + j]): s + j for j in range(L)}
+        for v in hg.net_pins[hg.net_ptr[n]:hg.net_ptr[n + 1]]:
+            st.ep_cnt[slot[int(st.part[v])]] += 1

@@ -127,3 +127,39 @@ def test_engine_reports_no_device_without_gpu():
     assert rc == 8
     ctx = ctypes.c_void_p()
     assert lib.fp_ctx_create(ctypes.byref(ctx), 0) == 8
+
+
+# The reference's partition_gpu_wrapper declarations, mangled (C++ linkage):
+# TV/partition.cpp:6-26 and CN/partition.cpp:7-27.
+LEGACY_SYMBOLS = {
+    "tv": "_Z21partition_gpu_wrapperPjS_S_iixPiS0_iS0_S0_S0_S0_S0_S0_S0_S0_S0_Pdid",
+    "cn": "_Z21partition_gpu_wrapperPjS_S_iixPiS0_iS0_iiS_S_S0_S0_S0_S0_S0_S0_",
+}
+
+
+@pytest.mark.parametrize("objective", sorted(LEGACY_SYMBOLS))
+def test_legacy_library_exports_reference_entry_point(objective):
+    lib = os.path.join(os.path.dirname(_native.ENGINE_LIB), f"libfastpart_legacy_{objective}.so")
+    so = ctypes.CDLL(lib)
+    assert hasattr(so, LEGACY_SYMBOLS[objective])
+    # where the reference objects are built, their undefined wrapper symbol is exactly this one
+    obj = os.path.join(ROOT, "oracle", "_ref", "obj", f"{objective}_partition.o")
+    if os.path.exists(obj):
+        und = subprocess.run(["nm", "-u", obj], capture_output=True, text=True, check=True).stdout
+        assert [ln.split()[-1] for ln in und.splitlines() if "partition_gpu_wrapper" in ln] == \
+            [LEGACY_SYMBOLS[objective]]
+
+
+@pytest.mark.parametrize("objective", sorted(LEGACY_SYMBOLS))
+def test_reference_partition_gpu_fails_loudly_without_gpu(objective, tmp_path):
+    exe = os.path.join(ROOT, "oracle", "_ref", f"dropin_{objective}")
+    if not os.path.exists(exe):
+        pytest.skip("drop-in binaries not built here")
+    n = ctypes.c_int(-1)
+    if _native.engine().fp_device_count(ctypes.byref(n)) == 0 and n.value > 0:
+        pytest.skip("a GPU is present")
+    hg = H.colnet(400, H.DEG_FIXED, 3, seed=2)
+    inst = str(tmp_path / "g.fphg")
+    hg.write_fphg(inst)
+    p = subprocess.run([exe, inst, "4", "2", "-1", str(tmp_path / "o")], capture_output=True, text=True)
+    assert p.returncode == 8 and "no usable GPU" in p.stderr
