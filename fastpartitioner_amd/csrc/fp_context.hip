@@ -185,6 +185,9 @@ fp_status fp_upload(fp_ctx* ctx, int nov, int noe, const unsigned* net_ptr, cons
   ctx->hist_cap = 0;
   ctx->cn.ready = false;
   ctx->cn.ops_cap = 0;
+  ctx->vol.rmax = 0;
+  ctx->vol.kcap = 0;
+  ctx->vol.evcap = 0;
   if (net_ptr[0] != 0) return fail(ctx, FP_ERR_ARG, "fp_upload: net_ptr[0] != 0");
   unsigned max_net = 0;
   for (int n = 0; n < noe; n++) {
@@ -272,6 +275,19 @@ fp_status fp_upload(fp_ctx* ctx, int nov, int noe, const unsigned* net_ptr, cons
     }
   }
   ctx->dup_pins = dup;
+  // the volume objectives read a net's source part through its pins
+  // (MGV/partition.cpp:158-186): every non-empty net's source must be a pin
+  bool src_ok = src != nullptr;
+  if (src_ok) {
+#pragma omp parallel for schedule(dynamic, 1024) reduction(&& : src_ok)
+    for (int n = 0; n < noe; n++) {
+      if (net_ptr[n + 1] == net_ptr[n]) continue;
+      bool hit = false;
+      for (unsigned k = net_ptr[n]; k < net_ptr[n + 1] && !hit; k++) hit = (int)net_pins[k] == src[n];
+      src_ok = src_ok && hit;
+    }
+  }
+  ctx->src_on_pins = src_ok;
 
   FP_TRY(dalloc(ctx, &ctx->net_ptr, (size_t)noe + 1));
   FP_TRY(dalloc(ctx, &ctx->net_pins, (size_t)pins));

@@ -241,6 +241,7 @@ struct fp_ctx {
   unsigned *net_ptr = nullptr, *net_pins = nullptr, *vtx_ptr = nullptr, *vtx_nets = nullptr;
   int *src = nullptr, *vwgt = nullptr, *order = nullptr, *apply_pins = nullptr;
   bool has_graph = false, dup_pins = false;
+  bool src_on_pins = false;  // every non-empty net's source is one of its pins
   long long total_weight = 0;
   unsigned max_net = 0, max_vdeg = 0;
   std::vector<unsigned> h_net_ptr;
@@ -280,6 +281,18 @@ struct fp_ctx {
     unsigned long long* totals = nullptr;
   } cn;
 
+  // MSV / MSRV refinement scratch (allocated on first use)
+  struct {
+    int* vol = nullptr;      // K: send (+ receive) volume per part
+    int* mstat = nullptr;    // [0] max, [1] parts at the max, [2] round moves, [3] moves applied
+    int* slot_mi = nullptr;  // rmax: move index of a slot or -1
+    int4* mv = nullptr;      // rmax moves: vertex, own part, new part, event offset
+    int4* evbuf = nullptr;   // per (move, incident net): source part, volume changes
+    int rmax = 0;
+    int kcap = 0;
+    long long evcap = 0;
+  } vol;
+
   // metric scratch
   unsigned* p2p = nullptr;
   unsigned long long* mvec = nullptr;  // 5*K + 2
@@ -295,6 +308,7 @@ namespace fpe {
 
 fp_status fail(fp_ctx* c, fp_status st, const char* fmt, ...);
 fp_status cn_refine(fp_ctx* ctx, int passes, fp_refine_stats* stats);
+fp_status vol_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule* sched, fp_refine_stats* stats);
 fp_status dalloc_bytes(fp_ctx* ctx, void** p, size_t bytes);
 void dfree_all(fp_ctx* ctx);
 hipEvent_t pooled_event(fp_ctx* ctx, size_t i);

@@ -531,9 +531,13 @@ fp_status fp_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule
     return cn_refine(ctx, passes, stats);
   }
   FP_TRY(check_sched(ctx, sched));
+  FP_TRY(check_envelope(ctx));
+  if (metric == FP_METRIC_MSV || metric == FP_METRIC_MSRV) {
+    FP_HIP_OK(hipSetDevice(ctx->device));
+    return vol_refine(ctx, metric, passes, sched, stats);
+  }
   if (metric != FP_METRIC_TV) return fail(ctx, FP_ERR_UNSUPPORTED, "refinement for metric %d not available", (int)metric);
   const int nov = ctx->nov, K = ctx->K;
-  FP_TRY(check_envelope(ctx));
   const int waves = eval_waves(K);
   const size_t lds_a = (size_t)(APPLY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
   if (lds_a > (size_t)LDS_BUDGET) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d too large for the apply kernel", K);

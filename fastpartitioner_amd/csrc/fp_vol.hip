@@ -1,0 +1,780 @@
+// fp_vol.hip -- MSV and MSRV refinement on the device: the round schedule of
+// partition_par in MGV/partition.cpp:706-1040 (max send volume) and
+// MSRV/partition.cpp:769-1110 (max send + receive volume).
+//
+// vol[p] is part p's send volume (MSV) or send + receive volume (MSRV); the
+// objective is max_p vol[p], and the "max set" is the set of parts holding it.
+// A round covers R consecutive visiting positions (MSV: vertex ids, the
+// reference's refinement ignores the ordering, MGV/partition.cpp:755-757):
+//   k_vol_eval     one wavefront per vertex: LDG decision on the frozen state
+//                  (the band-free LDG of MGV/partition.cpp:28-65), then the
+//                  reference's per-part gain/loss terms, accepted iff every
+//                  max part gains at least (MSV) / more than (MSRV) it loses
+//                  and no other part reaches the max
+//   k_vol_compact  accepted moves in visiting order, with event offsets
+//   k_vol_replay   (dry) one wavefront per touched net replays the round's
+//                  moves on its list in order and records, per (move, net),
+//                  the volume changes the reference's apply makes
+//   k_vol_scan     one wavefront applies the moves' volume changes in order,
+//                  tracks the max set, and stops after the first move whose
+//                  max set holds a part the previous one did not
+//                  (MGV/partition.cpp:977-1003)
+//   k_vol_replay   (commit) list edits, part vector and both part sizes of the
+//                  moves up to the stop
+#include "fp_internal.h"
+
+namespace fpe {
+namespace {
+
+constexpr int VOL_BLOCK = 256;
+constexpr int VOL_KMAX = 4096;  // k_vol_scan holds three K-int arrays in LDS
+constexpr int REPLAY_BLOCK = 256;
+constexpr int REPLAY_NET_BLOCKS = 2048;
+
+// per part: volume of every net whose source is in the part, lambda - 1 (plus,
+// for MSRV, one per other part of the net: what it receives)
+__global__ void k_vol_init(const unsigned* __restrict__ net_ptr, const unsigned* __restrict__ ent,
+                           const int* __restrict__ src, const int* __restrict__ part, int noe, unsigned pmask,
+                           int msrv, int* __restrict__ vol) {
+  for (int n = (int)(blockIdx.x * blockDim.x + threadIdx.x); n < noe; n += (int)(gridDim.x * blockDim.x)) {
+    const unsigned seg = net_ptr[n] + (unsigned)n;
+    const int L = (int)ent[seg];
+    if (L == 0) continue;
+    const int sp = part[src[n]];
+    if (L > 1) atomicAdd(&vol[sp], L - 1);
+    if (msrv)
+      for (int t = 0; t < L; t++) {
+        const int q = (int)(ent[seg + 1 + (unsigned)t] & pmask);
+        if (q != sp) atomicAdd(&vol[q], 1);
+      }
+  }
+}
+
+__device__ __forceinline__ void wave_max_count(int& m, int& c) {
+  for (int off = WAVE / 2; off > 0; off >>= 1) {
+    const int om = __shfl_xor(m, off), oc = __shfl_xor(c, off);
+    if (om > m) {
+      m = om;
+      c = oc;
+    } else if (om == m) {
+      c += oc;
+    }
+  }
+}
+
+// mstat[0] = max volume, mstat[1] = number of parts holding it (MGV/partition.cpp:712-729)
+__global__ void __launch_bounds__(1024) k_vol_maxset(const int* __restrict__ vol, int K, int* __restrict__ mstat) {
+  __shared__ int s_m[16], s_c[16];
+  int m = INT_MIN, c = 0;
+  for (int q = (int)threadIdx.x; q < K; q += (int)blockDim.x) {
+    const int x = vol[q];
+    if (x > m) {
+      m = x;
+      c = 1;
+    } else if (x == m) {
+      c++;
+    }
+  }
+  wave_max_count(m, c);
+  const int lane = lane_id(), w = (int)(threadIdx.x / WAVE);
+  if (lane == 0) {
+    s_m[w] = m;
+    s_c[w] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int bm = INT_MIN, bc = 0;
+    for (int i = 0; i < (int)(blockDim.x / WAVE); i++) {
+      if (s_m[i] > bm) {
+        bm = s_m[i];
+        bc = s_c[i];
+      } else if (s_m[i] == bm) {
+        bc += s_c[i];
+      }
+    }
+    mstat[0] = bm;
+    mstat[1] = bc;
+  }
+}
+
+struct VolEvalArgs {
+  const unsigned* vtx_ptr;
+  const unsigned* vtx_nets;
+  const unsigned* vtx_seg;
+  const unsigned short* ev;
+  const int* src;
+  const int* part;
+  const int* order;  // nullable: identity
+  const long long* sizes;
+  const int* vol;
+  const int* mstat;
+  int K, pb, msrv;
+  int r0, nslots;
+  double C;
+  int* slot_own;
+  int* slot_dec;
+  unsigned tag;
+  unsigned* net_stamp;
+  int* touched;
+  int* tcount;
+  int* tcount_next;
+};
+
+__global__ void __launch_bounds__(VOL_BLOCK) k_vol_eval(VolEvalArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int K = a.K;
+  double* s_pen = reinterpret_cast<double*>(lds);
+  long long* s_size = reinterpret_cast<long long*>(s_pen + K);
+  unsigned* wbase = reinterpret_cast<unsigned*>(s_size + K);
+  __shared__ long long red_sz[VOL_BLOCK / WAVE];
+  __shared__ int red_ix[VOL_BLOCK / WAVE];
+  const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
+  unsigned* cnt = wbase + (size_t)wave * 2 * (size_t)K;
+  unsigned* first = cnt + K;
+  int* delta = reinterpret_cast<int*>(cnt);  // reused once the decision is made (cnt is all 0 again)
+
+  // penalty = 1 - (1.0 * size / C) (MGV/partition.cpp:36); fallback = lowest-index lightest part (:48-60)
+  long long bs = LLONG_MAX;
+  int bi = INT_MAX;
+  for (int q = (int)threadIdx.x; q < K; q += (int)blockDim.x) {
+    const long long sz = a.sizes[q];
+    s_size[q] = sz;
+    s_pen[q] = 1.0 - ((double)sz / a.C);
+    if (sz < bs) {
+      bs = sz;
+      bi = q;
+    }
+  }
+  for (int q = lane; q < K; q += WAVE) {
+    cnt[q] = 0;
+    first[q] = NOFIRST;
+  }
+  for (int off = WAVE / 2; off > 0; off >>= 1) {
+    const long long os = __shfl_xor(bs, off);
+    const int oi = __shfl_xor(bi, off);
+    if (os < bs || (os == bs && oi < bi)) {
+      bs = os;
+      bi = oi;
+    }
+  }
+  if (lane == 0) {
+    red_sz[wave] = bs;
+    red_ix[wave] = bi;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && a.tcount_next) *a.tcount_next = 0;
+  __syncthreads();
+  int fb = red_ix[0];
+  {
+    long long b = red_sz[0];
+    for (int w = 1; w < nw; w++)
+      if (red_sz[w] < b || (red_sz[w] == b && red_ix[w] < fb)) {
+        b = red_sz[w];
+        fb = red_ix[w];
+      }
+  }
+  const int M = a.mstat[0], cntM = a.mstat[1];
+
+  for (int slot = (int)blockIdx.x * nw + wave; slot < a.nslots; slot += (int)gridDim.x * nw) {
+    const int z = a.r0 + slot;
+    const int v = a.order ? a.order[z] : z;
+    const int own = a.part[v];
+    const unsigned vb = a.vtx_ptr[v];
+    const int deg = (int)(a.vtx_ptr[v + 1] - vb);
+
+    // scores: pin counts per part over v's nets, first appearance ranked by
+    // (net index, list position) as the reference's scan meets them
+    // (MGV/partition.cpp:765-790)
+    for (int i = 0; i < deg; i++) {
+      const unsigned sg = a.vtx_seg[vb + (unsigned)i];
+      const int L = (int)a.ev[sg];
+      for (int t = lane; t < L; t += WAVE) {
+        const int q = (int)(a.ev[sg + 1u + (unsigned)t] & ~EV_SINGLE);
+        atomicAdd(&cnt[q], 1u);
+        atomicMin(&first[q], ((unsigned)i << a.pb) | (unsigned)t);
+      }
+    }
+    wave_lds_sync();
+    // LDG: lexicographic max of (score, -size, -first rank) over score > 0
+    unsigned long long b1 = 0, b2 = ~0ull;
+    int bq = -1;
+    for (int q = lane; q < K; q += WAVE) {
+      const unsigned c = cnt[q];
+      if (c) {
+        const unsigned f = first[q];
+        cnt[q] = 0;
+        first[q] = NOFIRST;
+        const int adj = (int)c - (q == own ? 1 : 0);  // scores[part_id]-- (MGV/partition.cpp:792-796)
+        const double sc = (double)adj * s_pen[q];
+        if (sc > 0) {
+          const unsigned long long k1 = (unsigned long long)__double_as_longlong(sc);
+          const unsigned long long k2 = ((unsigned long long)s_size[q] << 32) | f;
+          if (k1 > b1 || (k1 == b1 && k2 < b2)) {
+            b1 = k1;
+            b2 = k2;
+            bq = q;
+          }
+        }
+      }
+    }
+    const unsigned long long m1 = wave_max_u64(b1);
+    const unsigned long long m2 = wave_min_u64(b1 == m1 ? b2 : ~0ull);
+    wave_lds_sync();
+    int d = fb;
+    if (m1 != 0) {
+      const int wl = __ffsll((long long)__ballot(b1 == m1 && b2 == m2)) - 1;
+      d = __builtin_amdgcn_readlane(bq, wl);
+    }
+
+    int dec = -1;
+    if (d != own) {
+      // gains / losses (MGV/partition.cpp:800-840; MSRV/partition.cpp:852-897),
+      // accumulated as delta[p] = losses[p] - gains[p]
+      int d_own = 0, d_dec = 0;
+      for (int i = 0; i < deg; i++) {
+        const unsigned sg = a.vtx_seg[vb + (unsigned)i];
+        const int n = (int)a.vtx_nets[vb + (unsigned)i];
+        const int L = (int)a.ev[sg];
+        bool single = false, found = false;
+        for (int t0 = 0; t0 < L; t0 += WAVE) {
+          const int t = t0 + lane;
+          const bool ok = t < L;
+          const unsigned x = ok ? a.ev[sg + 1u + (unsigned)t] : 0u;
+          const int q = (int)(x & ~EV_SINGLE);
+          single = single || __ballot(ok && q == own && (x & EV_SINGLE)) != 0;
+          found = found || __ballot(ok && q == d) != 0;
+        }
+        const int s = a.src[n];
+        if (s == v) {
+          d_own -= L - 1;  // gains[part_id] += end_cnt - 1
+          d_dec += L - 1;  // losses[decision] += end_cnt - 1
+          if (single) d_dec -= 1;
+          if (!found) d_dec += 1;
+        } else {
+          const int sp = a.part[s];
+          if (single) {
+            if (lane == 0) atomicAdd(&delta[sp], -1);
+            if (a.msrv) d_own -= 1;
+          }
+          if (!found) {
+            if (lane == 0) atomicAdd(&delta[sp], 1);
+            if (a.msrv) d_dec += 1;
+          }
+        }
+      }
+      if (lane == 0) {
+        atomicAdd(&delta[own], d_own);
+        atomicAdd(&delta[d], d_dec);
+      }
+      wave_lds_sync();
+      // the affected parts: item 0 = own, 1 = d, 2 + i = source part of net i;
+      // every other part has gains = losses = 0
+      bool bad = false;
+      for (int b = 0; b < deg + 2; b += WAVE) {
+        const int it = b + lane;
+        int p = -1;
+        if (it == 0) p = own;
+        else if (it == 1) p = d;
+        else if (it < deg + 2) p = a.part[a.src[a.vtx_nets[vb + (unsigned)(it - 2)]]];
+        if (p >= 0) {
+          const int dl = delta[p];
+          const int vv = a.vol[p];
+          if (vv == M) {
+            if (a.msrv ? dl >= 0 : dl > 0) bad = true;  // a max part must gain (MSRV: strictly)
+            atomicMin(&first[p], (unsigned)it);
+          } else if (vv + dl >= M) {
+            bad = true;  // no other part may reach the max
+          }
+        }
+      }
+      bad = __ballot(bad) != 0;
+      wave_lds_sync();
+      // MSRV: max parts untouched by the move have gains = losses = 0 and fail
+      // the strict test, so every max part must be affected
+      int maxhit = 0;
+      for (int b = 0; b < deg + 2; b += WAVE) {
+        const int it = b + lane;
+        int p = -1;
+        if (it == 0) p = own;
+        else if (it == 1) p = d;
+        else if (it < deg + 2) p = a.part[a.src[a.vtx_nets[vb + (unsigned)(it - 2)]]];
+        if (p >= 0) maxhit += first[p] == (unsigned)it ? 1 : 0;
+      }
+      maxhit = wave_sum(maxhit);
+      wave_lds_sync();
+      for (int b = 0; b < deg + 2; b += WAVE) {
+        const int it = b + lane;
+        int p = -1;
+        if (it == 0) p = own;
+        else if (it == 1) p = d;
+        else if (it < deg + 2) p = a.part[a.src[a.vtx_nets[vb + (unsigned)(it - 2)]]];
+        if (p >= 0) {
+          delta[p] = 0;
+          first[p] = NOFIRST;
+        }
+      }
+      wave_lds_sync();
+      if (!bad && (!a.msrv || maxhit == cntM)) dec = d;
+    }
+    if (lane == 0) {
+      a.slot_own[slot] = own;
+      a.slot_dec[slot] = dec;
+    }
+    if (dec >= 0) {
+      for (int i = lane; i < deg; i += WAVE) {
+        const unsigned n = a.vtx_nets[vb + (unsigned)i];
+        if (atomicExch(&a.net_stamp[n], a.tag) != a.tag) a.touched[atomicAdd(a.tcount, 1)] = (int)n;
+      }
+    }
+  }
+}
+
+// Accepted moves of the round in visiting order: slot_mi[slot] = move index
+// (or -1), mv[i] = (vertex, own part, new part, first event slot).
+__global__ void __launch_bounds__(1024) k_vol_compact(const int* __restrict__ slot_dec, const int* __restrict__ slot_own,
+                                                      const int* __restrict__ order,
+                                                      const unsigned* __restrict__ vtx_ptr, int r0, int nslots,
+                                                      int* __restrict__ slot_mi, int4* __restrict__ mv,
+                                                      int* __restrict__ mstat) {
+  __shared__ int s_w[16];
+  int base = 0, ebase = 0;
+  for (int s0 = 0; s0 < nslots; s0 += (int)blockDim.x) {
+    const int s = s0 + (int)threadIdx.x;
+    int f = 0, deg = 0, v = 0;
+    if (s < nslots && slot_dec[s] >= 0) {
+      f = 1;
+      v = order ? order[r0 + s] : r0 + s;
+      deg = (int)(vtx_ptr[v + 1] - vtx_ptr[v]);
+    }
+    int tot, totd;
+    const int ex = block_excl_scan(f, s_w, &tot);
+    const int exd = block_excl_scan(deg, s_w, &totd);
+    if (s < nslots) slot_mi[s] = f ? base + ex : -1;
+    if (f) mv[base + ex] = make_int4(v, slot_own[s], slot_dec[s], ebase + exd);
+    base += tot;
+    ebase += totd;
+  }
+  if (threadIdx.x == 0) {
+    mstat[2] = base;
+    mstat[3] = base;
+  }
+}
+
+struct VolReplayArgs {
+  const unsigned* net_ptr;
+  const int* apply_pins;
+  const int* slot_mi;
+  const int4* mv;
+  const int* touched;
+  const int* tcount;
+  const int* src;
+  const unsigned* vtx_ptr;
+  const unsigned* vtx_nets;
+  const int* mstat;
+  const int* vwgt;
+  unsigned* ent;
+  unsigned short* ev;
+  int* part;
+  long long* sizes;
+  unsigned long long* moves;
+  int4* evbuf;
+  int K, pb;
+  unsigned pmask;
+  int r0, r1, msrv, commit, net_blocks;
+};
+
+// dry (commit = 0): per (move, net) volume changes of the reference's apply
+//   (MGV/partition.cpp:897-975; MSRV/partition.cpp:952-1052), recorded as
+//   (source part or -1, change of own part, change of new part, change of
+//   source part) at event slot mv.evoff + (position of the net in the move's
+//   vertex->nets list); the list edits stay in LDS
+// commit (commit = 1): list edits of the moves before the stop; blocks past
+//   net_blocks commit part vector and both part sizes
+__global__ void __launch_bounds__(REPLAY_BLOCK) k_vol_replay(VolReplayArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
+  const int nstop = a.mstat[3];
+  if ((int)blockIdx.x >= a.net_blocks) {
+    const int mi = ((int)blockIdx.x - a.net_blocks) * (int)blockDim.x + (int)threadIdx.x;
+    const bool mvd = mi < nstop;
+    if (mvd) {
+      const int4 m = a.mv[mi];
+      const long long w = a.vwgt ? a.vwgt[m.x] : 1;
+      a.part[m.x] = m.z;
+      atomic_add_i64(&a.sizes[m.y], -w);
+      atomic_add_i64(&a.sizes[m.z], w);
+    }
+    const unsigned long long b = __ballot(mvd);
+    if (lane == 0 && b) atomicAdd(a.moves, (unsigned long long)__popcll(b));
+    return;
+  }
+  unsigned* list = reinterpret_cast<unsigned*>(lds) + (size_t)wave * (size_t)a.K;
+  const int ntouched = *a.tcount;
+  for (int i = (int)blockIdx.x * nw + wave; i < ntouched; i += a.net_blocks * nw) {
+    const int n = a.touched[i];
+    const unsigned s = a.net_ptr[n], e = a.net_ptr[n + 1];
+    const unsigned seg = s + (unsigned)n;
+    int L = wave_list_load(a.ent, seg, list);
+    const unsigned lo = wave_lower_bound(a.apply_pins, s, e, a.r0);
+    const unsigned hi = wave_lower_bound(a.apply_pins, lo, e, a.r1);
+    const int srcv = a.src[n];
+    int sp_cur = a.commit ? 0 : a.part[srcv];  // the source's part as the replay goes
+    bool dirty = false;
+    for (unsigned k0 = lo; k0 < hi; k0 += WAVE) {
+      const unsigned k = k0 + (unsigned)lane;
+      int mi = -1;
+      if (k < hi) mi = a.slot_mi[a.apply_pins[k] - a.r0];
+      if (a.commit && mi >= nstop) mi = -1;
+      unsigned long long bm = __ballot(mi >= 0);
+      while (bm) {
+        const int l = __ffsll((long long)bm) - 1;
+        bm &= bm - 1;
+        const int mj = __shfl(mi, l);
+        const int4 m = a.mv[mj];  // (v, own, d, evoff)
+        const int v = m.x, own = m.y, d = m.z;
+        if (!a.commit) {
+          const bool isrc = srcv == v;
+          const int jo = wave_find(list, L, own, a.pmask);
+          const unsigned co = jo >= 0 ? (list[jo] >> a.pb) : 0u;
+          const bool found = wave_find(list, L, d, a.pmask) >= 0;
+          const int lam_b = L;
+          const int lam_mid = co == 1u ? L - 1 : L;
+          int e_own = 0, e_dec = 0, e_sp = 0, spx = -1;
+          if (isrc) e_own -= lam_b - 1;                       // send_volumes[part_id] -= end_cnt - 1
+          if (co == 1u) {
+            if (!isrc) {
+              spx = sp_cur;
+              e_sp -= 1;                                      // send_volumes[net_src_part]--
+              if (a.msrv) e_own -= 1;                         // received_volumes[part_id]--
+            }
+          } else if (isrc && a.msrv) {
+            e_own += 1;                                       // received_volumes[part_id]++
+          }
+          if (isrc) e_dec += lam_mid - 1;                     // send_volumes[net_src_part] += end_cnt - 1
+          if (found) {
+            if (isrc && a.msrv) e_dec -= 1;                   // received_volumes[decision]--
+          } else if (isrc) {
+            e_dec += 1;                                       // send_volumes[net_src_part]++ (source part = d)
+          } else {
+            spx = sp_cur;
+            e_sp += 1;                                        // send_volumes[net_src_part]++
+            if (a.msrv) e_dec += 1;                           // received_volumes[decision]++
+          }
+          const unsigned vb = a.vtx_ptr[v], ve = a.vtx_ptr[v + 1];
+          const unsigned pos = wave_lower_bound(reinterpret_cast<const int*>(a.vtx_nets), vb, ve, n) - vb;
+          if (lane == 0) a.evbuf[(unsigned)m.w + pos] = make_int4(spx, e_own, e_dec, e_sp);
+          if (isrc) sp_cur = d;
+        }
+        wave_list_move(list, L, own, d, a.pb, a.pmask);
+        dirty = true;
+      }
+    }
+    if (a.commit && dirty) wave_list_store(a.ent, a.ev, seg, list, L, a.pb, a.pmask);
+    wave_lds_sync();
+  }
+}
+
+// One wavefront: the round's moves in order, each followed by the max-set
+// update of MGV/partition.cpp:977-1003; stops after the first move whose max
+// set holds a part the previous max set did not. Writes back vol, the max and
+// its multiplicity, and the number of moves to commit (mstat[3]).
+__global__ void __launch_bounds__(WAVE) k_vol_scan(int* __restrict__ vol_g, int K, const int4* __restrict__ mv,
+                                                   const int4* __restrict__ evbuf,
+                                                   const unsigned* __restrict__ vtx_ptr, int* __restrict__ mstat) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  int* vol = reinterpret_cast<int*>(lds);
+  int* delta = vol + K;
+  int* claim = delta + K;
+  const int lane = lane_id();
+  for (int q = lane; q < K; q += WAVE) {
+    vol[q] = vol_g[q];
+    delta[q] = 0;
+    claim[q] = INT_MAX;
+  }
+  wave_lds_sync();
+  int M = mstat[0], cntM = mstat[1];
+  const int nmoves = mstat[2];
+  int nstop = nmoves;
+  for (int mi = 0; mi < nmoves; mi++) {
+    const int4 m = mv[mi];
+    const int own = m.y, d = m.z;
+    const unsigned eb = (unsigned)m.w;
+    const int deg = (int)(vtx_ptr[m.x + 1] - vtx_ptr[m.x]);
+    const int nit = deg + 2;
+    // item it: 0 -> own, 1 -> d, 2 + i -> source part recorded for net i (or -1)
+    auto item = [&](int it) -> int {
+      if (it == 0) return own;
+      if (it == 1) return d;
+      return it < nit ? evbuf[eb + (unsigned)(it - 2)].x : -1;
+    };
+    int s_own = 0, s_dec = 0;
+    for (int b = 0; b < deg; b += WAVE) {
+      const int i = b + lane;
+      if (i < deg) {
+        const int4 e = evbuf[eb + (unsigned)i];
+        s_own += e.y;
+        s_dec += e.z;
+        if (e.x >= 0 && e.w != 0) atomicAdd(&delta[e.x], e.w);
+      }
+    }
+    s_own = wave_sum(s_own);
+    s_dec = wave_sum(s_dec);
+    if (lane == 0) {
+      atomicAdd(&delta[own], s_own);
+      atomicAdd(&delta[d], s_dec);
+    }
+    wave_lds_sync();
+    for (int b = 0; b < nit; b += WAVE) {
+      const int p = item(b + lane);
+      if (p >= 0) atomicMin(&claim[p], b + lane);
+    }
+    wave_lds_sync();
+    // one representative item per affected part: old and new volumes
+    int up = INT_MIN, inS = 0, newM = 0;
+    bool enterM = false;
+    for (int b = 0; b < nit; b += WAVE) {
+      const int it = b + lane;
+      const int p = item(it);
+      if (p >= 0 && claim[p] == it) {
+        const int old = vol[p], nw = old + delta[p];
+        up = max(up, nw);
+        inS += old == M;
+        newM += nw == M;
+        enterM = enterM || (nw == M && old != M);
+      }
+    }
+    {
+      int c0 = 0;
+      wave_max_count(up, c0);
+    }
+    inS = wave_sum(inS);
+    newM = wave_sum(newM);
+    enterM = __ballot(enterM) != 0;
+    wave_lds_sync();
+    // commit the new volumes; delta still holds each part's change
+    for (int b = 0; b < nit; b += WAVE) {
+      const int it = b + lane;
+      const int p = item(it);
+      if (p >= 0 && claim[p] == it) vol[p] += delta[p];
+    }
+    wave_lds_sync();
+    bool stop;
+    int nM, nC;
+    if (up > M) {
+      // the max rose: the new max set is the affected parts at `up`
+      int cu = 0;
+      bool enter = false;
+      for (int b = 0; b < nit; b += WAVE) {
+        const int it = b + lane;
+        const int p = item(it);
+        if (p >= 0 && claim[p] == it && vol[p] == up) {
+          cu++;
+          enter = enter || (vol[p] - delta[p] != M);
+        }
+      }
+      stop = __ballot(enter) != 0;
+      nM = up;
+      nC = wave_sum(cu);
+    } else if (cntM - inS > 0 || newM > 0) {
+      // the max stays: untouched parts at M plus affected parts that land on it
+      stop = enterM;
+      nM = M;
+      nC = cntM - inS + newM;
+    } else {
+      // the max fell: full reduction; parts at the new max that were not at the old one enter
+      int mm = INT_MIN, cc = 0;
+      for (int q = lane; q < K; q += WAVE) {
+        const int x = vol[q];
+        if (x > mm) {
+          mm = x;
+          cc = 1;
+        } else if (x == mm) {
+          cc++;
+        }
+      }
+      wave_max_count(mm, cc);
+      int stay = 0;
+      for (int b = 0; b < nit; b += WAVE) {
+        const int it = b + lane;
+        const int p = item(it);
+        if (p >= 0 && claim[p] == it && vol[p] == mm && vol[p] - delta[p] == M) stay++;
+      }
+      stay = wave_sum(stay);
+      stop = cc > stay;
+      nM = mm;
+      nC = cc;
+    }
+    M = nM;
+    cntM = nC;
+    wave_lds_sync();
+    for (int b = 0; b < nit; b += WAVE) {
+      const int p = item(b + lane);
+      if (p >= 0) {
+        delta[p] = 0;
+        claim[p] = INT_MAX;
+      }
+    }
+    wave_lds_sync();
+    if (stop) {
+      nstop = mi + 1;
+      break;
+    }
+  }
+  for (int q = lane; q < K; q += WAVE) vol_g[q] = vol[q];
+  if (lane == 0) {
+    mstat[0] = M;
+    mstat[1] = cntM;
+    mstat[3] = nstop;
+  }
+}
+
+int vol_eval_waves(int K) {
+  for (int w = VOL_BLOCK / WAVE; w >= 1; w--)
+    if ((size_t)K * 16 + (size_t)w * 8 * (size_t)K <= (size_t)LDS_BUDGET) return w;
+  return 0;
+}
+
+fp_status ensure_vol_scratch(fp_ctx* ctx, int R) {
+  // capacities are reset by every upload, which frees the old buffers
+  auto& s = ctx->vol;
+  if (ctx->K > s.kcap) {
+    FP_TRY(dalloc(ctx, &s.vol, (size_t)ctx->K));
+    FP_TRY(dalloc(ctx, &s.mstat, 8));
+    s.kcap = ctx->K;
+  }
+  if (R > s.rmax) {
+    FP_TRY(dalloc(ctx, &s.slot_mi, (size_t)R));
+    FP_TRY(dalloc(ctx, &s.mv, (size_t)R));
+    s.rmax = R;
+  }
+  const long long need = std::max(ctx->pins, 1ll);
+  if (need > s.evcap) {
+    FP_TRY(dalloc(ctx, &s.evbuf, (size_t)need));
+    s.evcap = need;
+  }
+  return FP_OK;
+}
+
+}  // namespace
+
+fp_status vol_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule* sched, fp_refine_stats* stats) {
+  const bool msrv = metric == FP_METRIC_MSRV;
+  const int nov = ctx->nov, K = ctx->K;
+  if (K > VOL_KMAX) return fail(ctx, FP_ERR_UNSUPPORTED, "volume refinement supports K <= %d", VOL_KMAX);
+  if (!ctx->src_on_pins)
+    return fail(ctx, FP_ERR_UNSUPPORTED, "volume refinement needs net sources that are pins of their nets");
+  if (!msrv && ctx->order)
+    return fail(ctx, FP_ERR_UNSUPPORTED,
+                "MSV refinement visits vertex ids in order (MGV/partition.cpp:755-757); upload with the identity ordering");
+  if (ctx->pins >= (1ll << 31)) return fail(ctx, FP_ERR_UNSUPPORTED, "volume refinement needs pins < 2^31");
+  if (sched->kind != FP_SCHED_REF)
+    return fail(ctx, FP_ERR_UNSUPPORTED, "volume refinement runs the reference round schedule (FP_SCHED_REF) only");
+  const int waves = vol_eval_waves(K);
+  if (waves == 0) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d needs more LDS than a workgroup has", K);
+  const size_t lds_e = (size_t)K * 16 + (size_t)waves * 8 * (size_t)K;
+  const size_t lds_r = (size_t)(REPLAY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
+  const size_t lds_s = (size_t)3 * (size_t)K * sizeof(int);
+  const int R = std::min(sched->round, nov);
+  FP_TRY(ensure_vol_scratch(ctx, R));
+  if (R > ctx->rmax) {
+    FP_TRY(dalloc(ctx, &ctx->slot_own, (size_t)R));
+    FP_TRY(dalloc(ctx, &ctx->slot_dec, (size_t)R));
+    ctx->rmax = R;
+  }
+  auto& vs = ctx->vol;
+  fill_async<int>(ctx, ctx->counters, 4, 0);
+  fill_async<unsigned long long>(ctx, ctx->moves, 1, 0ull);
+  fill_async<int>(ctx, vs.vol, (size_t)K, 0);
+  hipEvent_t e0 = pooled_event(ctx, 0), e1 = pooled_event(ctx, 1);
+  if (!e0 || !e1) return fail(ctx, FP_ERR_HIP, "hipEventCreate failed");
+  FP_HIP_OK(hipEventRecord(e0, ctx->stream));
+  const int ngrid = std::max(1, std::min((ctx->noe + 255) / 256, 4096));
+  k_vol_init<<<ngrid, 256, 0, ctx->stream>>>(ctx->net_ptr, ctx->ent, ctx->src, ctx->part, ctx->noe, ctx->pmask,
+                                              msrv ? 1 : 0, vs.vol);
+  long long rounds = 0;
+  for (int it = 0; it < passes; it++) {
+    k_vol_maxset<<<1, 1024, 0, ctx->stream>>>(vs.vol, K, vs.mstat);
+    for (int r0 = 0; r0 < nov; r0 += R) {
+      const int nslots = std::min(R, nov - r0);
+      ctx->tag++;
+      const int par = (int)(ctx->tag & 1u);
+      VolEvalArgs a{};
+      a.vtx_ptr = ctx->vtx_ptr;
+      a.vtx_nets = ctx->vtx_nets;
+      a.vtx_seg = ctx->vtx_seg;
+      a.ev = ctx->ev;
+      a.src = ctx->src;
+      a.part = ctx->part;
+      a.order = ctx->order;
+      a.sizes = ctx->sizes;
+      a.vol = vs.vol;
+      a.mstat = vs.mstat;
+      a.K = K;
+      a.pb = ctx->pb;
+      a.msrv = msrv ? 1 : 0;
+      a.r0 = r0;
+      a.nslots = nslots;
+      a.C = sched->ldg_C;
+      a.slot_own = ctx->slot_own;
+      a.slot_dec = ctx->slot_dec;
+      a.tag = ctx->tag;
+      a.net_stamp = ctx->net_stamp;
+      a.touched = ctx->touched;
+      a.tcount = ctx->counters + par;
+      a.tcount_next = ctx->counters + (par ^ 1);
+      const int egrid = std::max(1, std::min((nslots + waves - 1) / waves, 8192));
+      k_vol_eval<<<egrid, waves * WAVE, lds_e, ctx->stream>>>(a);
+      k_vol_compact<<<1, 1024, 0, ctx->stream>>>(ctx->slot_dec, ctx->slot_own, ctx->order, ctx->vtx_ptr, r0, nslots,
+                                                  vs.slot_mi, vs.mv, vs.mstat);
+      VolReplayArgs ra{};
+      ra.net_ptr = ctx->net_ptr;
+      ra.apply_pins = ctx->apply_pins;
+      ra.slot_mi = vs.slot_mi;
+      ra.mv = vs.mv;
+      ra.touched = ctx->touched;
+      ra.tcount = ctx->counters + par;
+      ra.src = ctx->src;
+      ra.vtx_ptr = ctx->vtx_ptr;
+      ra.vtx_nets = ctx->vtx_nets;
+      ra.mstat = vs.mstat;
+      ra.vwgt = ctx->vwgt;
+      ra.ent = ctx->ent;
+      ra.ev = ctx->ev;
+      ra.part = ctx->part;
+      ra.sizes = ctx->sizes;
+      ra.moves = ctx->moves;
+      ra.evbuf = vs.evbuf;
+      ra.K = K;
+      ra.pb = ctx->pb;
+      ra.pmask = ctx->pmask;
+      ra.r0 = r0;
+      ra.r1 = r0 + nslots;
+      ra.msrv = msrv ? 1 : 0;
+      ra.net_blocks = REPLAY_NET_BLOCKS;
+      ra.commit = 0;
+      k_vol_replay<<<REPLAY_NET_BLOCKS, REPLAY_BLOCK, lds_r, ctx->stream>>>(ra);
+      k_vol_scan<<<1, WAVE, lds_s, ctx->stream>>>(vs.vol, K, vs.mv, vs.evbuf, ctx->vtx_ptr, vs.mstat);
+      ra.commit = 1;
+      k_vol_replay<<<REPLAY_NET_BLOCKS + (nslots + REPLAY_BLOCK - 1) / REPLAY_BLOCK, REPLAY_BLOCK, lds_r,
+                     ctx->stream>>>(ra);
+      rounds++;
+    }
+  }
+  FP_HIP_OK(hipGetLastError());
+  FP_HIP_OK(hipEventRecord(e1, ctx->stream));
+  unsigned long long mv = 0;
+  FP_TRY(d2h(ctx, &mv, ctx->moves, 1));
+  FP_HIP_OK(hipStreamSynchronize(ctx->stream));
+  float ms = 0;
+  FP_HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+  ctx->timed_launches = 0;
+  ctx->timed_ms = 0;
+  if (stats) {
+    stats->moves = (long long)mv;
+    stats->rounds = rounds;
+    stats->pin_visits = (long long)passes * ctx->pins;
+    stats->device_ms = ms;
+  }
+  return FP_OK;
+}
+
+}  // namespace fpe

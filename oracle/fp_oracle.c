@@ -611,3 +611,409 @@ int or_cn_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* 
   free(push); free(mv_v); free(mv_d);
   return 0;
 }
+
+/* ---------------------------------------------------------------------- */
+/* MSV (MGV directory)                                                     */
+/* ---------------------------------------------------------------------- */
+
+/* band-free LDG (MGV/partition.cpp:28-65) */
+static int ldg_plain(int K, int apc, const int* active, const double* scores, const int* sizes, double C) {
+  double max_val = 0;
+  int max_index = -1;
+  for (int i = 0; i < apc; i++) {
+    int q = active[i];
+    double penalty = 1 - (1.0 * sizes[q] / C);
+    double score = scores[q] * penalty;
+    if (score > 0) {
+      if (max_index == -1 || score > max_val || (score == max_val && sizes[q] < sizes[max_index])) {
+        max_val = score;
+        max_index = q;
+      }
+    }
+  }
+  if (max_index == -1) {
+    int min_load = INT_MAX;
+    for (int i = 0; i < K; i++)
+      if (sizes[i] < min_load) { min_load = sizes[i]; max_index = i; }
+  }
+  return max_index;
+}
+
+/* Initial partition of MGV/partition.cpp:122-189 and MSRV/partition.cpp:131-208:
+ * band-free LDG stream; the volumes are accumulated as the stream goes (the
+ * source's part is read after the decision). rv NULL: send volumes only. */
+static int vol_ldg(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                   const int* src, const int* ordering, int K, double C, int* part, int* ep_ends, int* ep_parts,
+                   int* ep_cnt, int* sizes, int* sv, int* rv) {
+  long long pins = net_ptr[noe];
+  int* marker = (int*)malloc(sizeof(int) * (size_t)K);
+  int* active = (int*)malloc(sizeof(int) * (size_t)K);
+  double* scores = (double*)calloc((size_t)K, sizeof(double));
+  if (!marker || !active || !scores) return 2;
+  for (long long i = 0; i < pins; i++) { ep_cnt[i] = 0; ep_parts[i] = -1; }
+  for (int k = 0; k < K; k++) { marker[k] = -1; sizes[k] = 0; sv[k] = 0; if (rv) rv[k] = 0; }
+  for (int v = 0; v < nov; v++) part[v] = -1;
+  for (int n = 0; n < noe; n++) ep_ends[n] = 0;
+  for (int z = 0; z < nov; z++) {
+    int v = ordering[z];
+    int apc = 0;
+    for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
+      int n = (int)vtx_nets[i];
+      int s = (int)net_ptr[n];
+      for (int j = 0; j < ep_ends[n]; j++) {
+        int q = ep_parts[s + j];
+        if (marker[q] != v) { scores[q] = 1; marker[q] = v; active[apc++] = q; }
+        else scores[q]++;
+      }
+    }
+    int d = ldg_plain(K, apc, active, scores, sizes, C);
+    sizes[d]++;
+    part[v] = d;
+    for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
+      int n = (int)vtx_nets[i];
+      int s = (int)net_ptr[n];
+      int end = ep_ends[n];
+      int sn = src[n];
+      int sp = part[sn];
+      int found = 0;
+      for (int j = 0; j < end; j++) {
+        int q = ep_parts[s + j];
+        if (sn == v && q != sp) { sv[sp]++; if (rv) rv[q]++; }
+        if (q == d) { found = 1; ep_cnt[s + j]++; }
+      }
+      if (!found) {
+        ep_parts[s + end] = d;
+        ep_cnt[s + end] = 1;
+        ep_ends[n]++;
+        if (sn != v && sp != -1) { sv[sp]++; if (rv) rv[d]++; }
+      }
+    }
+  }
+  free(marker); free(active); free(scores);
+  return 0;
+}
+
+int or_msv_ldg(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+               const int* src, const int* ordering, int K, double C, int* part, int* ep_ends, int* ep_parts,
+               int* ep_cnt, int* sizes, int* sv) {
+  return vol_ldg(nov, noe, net_ptr, vtx_ptr, vtx_nets, src, ordering, K, C, part, ep_ends, ep_parts, ep_cnt, sizes,
+                 sv, NULL);
+}
+
+int or_msrv_ldg(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                const int* src, const int* ordering, int K, double C, int* part, int* ep_ends, int* ep_parts,
+                int* ep_cnt, int* sizes, int* sv, int* rv) {
+  return vol_ldg(nov, noe, net_ptr, vtx_ptr, vtx_nets, src, ordering, K, C, part, ep_ends, ep_parts, ep_cnt, sizes,
+                 sv, rv);
+}
+
+static int vol_of(const int* sv, const int* rv, int k) { return sv[k] + (rv ? rv[k] : 0); }
+
+/* The max set (MGV/partition.cpp:712-729; MSRV/partition.cpp:772-791 on
+ * send + receive): returns the max and sets inmax[k] = 1 for its parts. */
+static int vol_max_set(int K, const int* sv, const int* rv, unsigned char* inmax) {
+  int m = -1;
+  for (int k = 0; k < K; k++) if (vol_of(sv, rv, k) > m) m = vol_of(sv, rv, k);
+  for (int k = 0; k < K; k++) inmax[k] = (unsigned char)(vol_of(sv, rv, k) == m);
+  return m;
+}
+
+/* The reference's gains and losses of moving v from its part to d
+ * (MGV/partition.cpp:800-840; with recv, MSRV/partition.cpp:852-897), added
+ * into gains[K] / losses[K]. */
+static void vol_gains(int v, int d, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                      const int* src, const int* part, const int* ep_ends, const int* ep_parts, const int* ep_cnt,
+                      int recv, int* gains, int* losses) {
+  int own = part[v];
+  for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
+    int n = (int)vtx_nets[i];
+    int s = (int)net_ptr[n];
+    int end = ep_ends[n];
+    int sn = src[n];
+    int sp = part[sn];
+    if (sn == v) { gains[own] += end - 1; losses[d] += end - 1; }
+    int found = 0;
+    for (int j = 0; j < end; j++) {
+      if (ep_parts[s + j] == own && ep_cnt[s + j] == 1) {
+        if (sn != v) { gains[sp]++; if (recv) gains[own]++; }
+        else losses[d]--;
+      }
+      if (ep_parts[s + j] == d) found = 1;
+    }
+    if (!found) {
+      if (sn != v) { losses[sp]++; if (recv) losses[d]++; }
+      else losses[d]++;
+    }
+  }
+}
+
+int or_vol_move_delta(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                      const int* src, int K, const int* part, const int* ep_ends, const int* ep_parts,
+                      const int* ep_cnt, int recv, int v, int d, int* delta) {
+  (void)noe;
+  if (v < 0 || v >= nov || d < 0 || d >= K || d == part[v]) return 1;
+  int* gains = (int*)calloc((size_t)K, sizeof(int));
+  int* losses = (int*)calloc((size_t)K, sizeof(int));
+  if (!gains || !losses) return 2;
+  vol_gains(v, d, net_ptr, vtx_ptr, vtx_nets, src, part, ep_ends, ep_parts, ep_cnt, recv, gains, losses);
+  for (int k = 0; k < K; k++) delta[k] = losses[k] - gains[k];
+  free(gains); free(losses);
+  return 0;
+}
+
+/* LDG decision and max-set test of vertex v (MGV/partition.cpp:758-856;
+ * with rv, MSRV/partition.cpp:818-920: receive terms in the gains and losses
+ * and a strict gain for the max parts). Returns the decision if the move is
+ * accepted, else -1. marker: all -1 on entry and exit; gains/losses: all 0 on
+ * entry and exit. The max test reads inmax[]/max_v; with sticky != NULL it
+ * reads the max-part list sticky[0..nsticky) instead (MSRV's serial
+ * bookkeeping, see or_msrv_refine_serial). */
+static int vol_eval(int v, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                    const int* src, int K, double C, const int* part, const int* ep_ends, const int* ep_parts,
+                    const int* ep_cnt, const int* sizes, const int* sv, const int* rv, int max_v,
+                    const unsigned char* inmax, const int* sticky, int nsticky, int* marker, int* active,
+                    double* scores, int* gains, int* losses) {
+  int own = part[v];
+  int apc = 0;
+  for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
+    int n = (int)vtx_nets[i];
+    int s = (int)net_ptr[n];
+    for (int j = 0; j < ep_ends[n]; j++) {
+      int q = ep_parts[s + j];
+      if (marker[q] != v) { scores[q] = 1; marker[q] = v; active[apc++] = q; }
+      else scores[q]++;
+    }
+  }
+  scores[own]--;
+  int d = ldg_plain(K, apc, active, scores, sizes, C);
+  for (int i = 0; i < apc; i++) marker[active[i]] = -1;
+  if (d == own) return -1;
+  vol_gains(v, d, net_ptr, vtx_ptr, vtx_nets, src, part, ep_ends, ep_parts, ep_cnt, rv != NULL, gains, losses);
+  int good = 1;
+  if (sticky) {
+    for (int i = 0; i < nsticky && good; i++)
+      if (gains[sticky[i]] <= losses[sticky[i]]) good = 0;
+  } else {
+    for (int k = 0; k < K && good; k++)
+      if (inmax[k] && (rv ? gains[k] <= losses[k] : gains[k] < losses[k])) good = 0;
+  }
+  for (int k = 0; k < K && good; k++)
+    if (!inmax[k] && vol_of(sv, rv, k) + (losses[k] - gains[k]) >= max_v) good = 0;
+  for (int k = 0; k < K; k++) { gains[k] = 0; losses[k] = 0; }
+  return good ? d : -1;
+}
+
+/* Applies v: own -> d with the volume bookkeeping of MGV/partition.cpp:897-975
+ * (with rv, MSRV/partition.cpp:952-1052); the source's part is read after
+ * part[v] = d. */
+static void vol_apply(int v, int d, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                      const int* src, int* part, int* ep_ends, int* ep_parts, int* ep_cnt, int* sizes, int* sv,
+                      int* rv) {
+  int own = part[v];
+  part[v] = d;
+  sizes[own]--;
+  sizes[d]++;
+  for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
+    int n = (int)vtx_nets[i];
+    int s = (int)net_ptr[n];
+    int end = ep_ends[n];
+    int sn = src[n];
+    int sp = part[sn];
+    if (sn == v) sv[own] -= end - 1;
+    for (int j = 0; j < end; j++) {
+      if (ep_parts[s + j] == own) {
+        ep_cnt[s + j]--;
+        if (ep_cnt[s + j] == 0) {
+          ep_parts[s + j] = ep_parts[s + end - 1];
+          ep_cnt[s + j] = ep_cnt[s + end - 1];
+          ep_parts[s + end - 1] = -1;
+          ep_cnt[s + end - 1] = 0;
+          ep_ends[n]--;
+          end--;
+          if (sn != v) { sv[sp]--; if (rv) rv[own]--; }
+        } else if (rv && sn == v) {
+          rv[own]++;
+        }
+      }
+    }
+  }
+  for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
+    int n = (int)vtx_nets[i];
+    int s = (int)net_ptr[n];
+    int end = ep_ends[n];
+    int sn = src[n];
+    int sp = part[sn];
+    if (sn == v) sv[sp] += end - 1;
+    int found = 0;
+    for (int j = 0; j < end; j++)
+      if (ep_parts[s + j] == d) {
+        found = 1;
+        ep_cnt[s + j]++;
+        if (rv && sn == v) rv[d]--;
+        break;
+      }
+    if (!found) {
+      ep_parts[s + end] = d;
+      ep_cnt[s + end] = 1;
+      ep_ends[n]++;
+      sv[sp]++;
+      if (rv && sn != v) rv[d]++;
+    }
+  }
+}
+
+typedef struct {
+  int *marker, *active, *gains, *losses;
+  double* scores;
+  unsigned char *inmax, *inmax2;
+} vol_scratch;
+
+static int vol_scratch_init(vol_scratch* w, int K) {
+  w->marker = (int*)malloc(sizeof(int) * (size_t)K);
+  w->active = (int*)malloc(sizeof(int) * (size_t)K);
+  w->gains = (int*)calloc((size_t)K, sizeof(int));
+  w->losses = (int*)calloc((size_t)K, sizeof(int));
+  w->scores = (double*)calloc((size_t)K, sizeof(double));
+  w->inmax = (unsigned char*)malloc((size_t)K);
+  w->inmax2 = (unsigned char*)malloc((size_t)K);
+  if (!w->marker || !w->active || !w->gains || !w->losses || !w->scores || !w->inmax || !w->inmax2) return 2;
+  for (int k = 0; k < K; k++) w->marker[k] = -1;
+  return 0;
+}
+
+static void vol_scratch_free(vol_scratch* w) {
+  free(w->marker); free(w->active); free(w->gains); free(w->losses); free(w->scores); free(w->inmax);
+  free(w->inmax2);
+}
+
+int or_msv_refine_serial(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr,
+                         const unsigned* vtx_nets, const int* src, const int* ordering, int K, double C,
+                         int passes, int* part, int* ep_ends, int* ep_parts, int* ep_cnt, int* sizes, int* sv) {
+  (void)noe;
+  vol_scratch w;
+  if (vol_scratch_init(&w, K)) return 2;
+  for (int it = 0; it < passes; it++) {
+    int max_v = vol_max_set(K, sv, NULL, w.inmax);
+    for (int z = 0; z < nov; z++) {
+      int v = ordering[z];
+      int d = vol_eval(v, net_ptr, vtx_ptr, vtx_nets, src, K, C, part, ep_ends, ep_parts, ep_cnt, sizes, sv, NULL,
+                       max_v, w.inmax, NULL, 0, w.marker, w.active, w.scores, w.gains, w.losses);
+      if (d < 0) continue;
+      vol_apply(v, d, net_ptr, vtx_ptr, vtx_nets, src, part, ep_ends, ep_parts, ep_cnt, sizes, sv, NULL);
+      max_v = vol_max_set(K, sv, NULL, w.inmax);
+    }
+  }
+  vol_scratch_free(&w);
+  return 0;
+}
+
+/* shared round schedule of both partition_par's (vertex = ordering ? ordering[z] : z) */
+static int vol_refine_par(int nov, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                          const int* src, const int* ordering, int K, double C, int passes, int R, int* part,
+                          int* ep_ends, int* ep_parts, int* ep_cnt, int* sizes, int* sv, int* rv, long long* moves) {
+  if (R < 1) return 1;
+  vol_scratch w;
+  if (vol_scratch_init(&w, K)) return 2;
+  int* dec = (int*)malloc(sizeof(int) * (size_t)R);
+  if (!dec) return 2;
+  long long applied = 0;
+  for (int it = 0; it < passes; it++) {
+    int max_v = vol_max_set(K, sv, rv, w.inmax);
+    for (int r0 = 0; r0 < nov; r0 += R) {
+      int r1 = r0 + R < nov ? r0 + R : nov;
+      for (int z = r0; z < r1; z++) {
+        int v = ordering ? ordering[z] : z;
+        dec[z - r0] = vol_eval(v, net_ptr, vtx_ptr, vtx_nets, src, K, C, part, ep_ends, ep_parts, ep_cnt, sizes,
+                               sv, rv, max_v, w.inmax, NULL, 0, w.marker, w.active, w.scores, w.gains, w.losses);
+      }
+      for (int z = r0; z < r1; z++) {
+        int d = dec[z - r0];
+        if (d < 0) continue;
+        int v = ordering ? ordering[z] : z;
+        vol_apply(v, d, net_ptr, vtx_ptr, vtx_nets, src, part, ep_ends, ep_parts, ep_cnt, sizes, sv, rv);
+        applied++;
+        /* MGV/partition.cpp:977-1003: a new part in the max set ends the round */
+        memcpy(w.inmax2, w.inmax, (size_t)K);
+        max_v = vol_max_set(K, sv, rv, w.inmax);
+        int entered = 0;
+        for (int k = 0; k < K; k++) if (w.inmax[k] && !w.inmax2[k]) entered = 1;
+        if (entered) break;
+      }
+    }
+  }
+  if (moves) *moves = applied;
+  free(dec);
+  vol_scratch_free(&w);
+  return 0;
+}
+
+int or_msv_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                      const int* src, int K, double C, int passes, int R, int* part, int* ep_ends, int* ep_parts,
+                      int* ep_cnt, int* sizes, int* sv, long long* moves) {
+  (void)noe;
+  return vol_refine_par(nov, net_ptr, vtx_ptr, vtx_nets, src, NULL, K, C, passes, R, part, ep_ends, ep_parts, ep_cnt,
+                        sizes, sv, NULL, moves);
+}
+
+int or_msrv_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                       const int* src, const int* ordering, int K, double C, int passes, int R, int* part,
+                       int* ep_ends, int* ep_parts, int* ep_cnt, int* sizes, int* sv, int* rv, long long* moves) {
+  (void)noe;
+  return vol_refine_par(nov, net_ptr, vtx_ptr, vtx_nets, src, ordering, K, C, passes, R, part, ep_ends, ep_parts,
+                        ep_cnt, sizes, sv, rv, moves);
+}
+
+int or_msrv_refine_serial(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr,
+                          const unsigned* vtx_nets, const int* src, const int* ordering, int K, double C,
+                          int passes, int* part, int* ep_ends, int* ep_parts, int* ep_cnt, int* sizes, int* sv,
+                          int* rv) {
+  (void)noe;
+  vol_scratch w;
+  if (vol_scratch_init(&w, K)) return 2;
+  /* the max-part list can grow past K (parts are appended again at every
+   * recomputation, MSRV/partition.cpp:496-512); capacity grows as needed */
+  int cap = K > 0 ? 2 * K : 2;
+  int* maxp = (int*)malloc(sizeof(int) * (size_t)cap);
+  int* mark = (int*)malloc(sizeof(int) * (size_t)K);
+  if (!maxp || !mark) return 2;
+  for (int it = 0; it < passes; it++) {
+    /* MSRV/partition.cpp:240-262: fresh max set at the pass start */
+    int max_v = -1, nmax = 0, changed = 0;
+    for (int k = 0; k < K; k++) mark[k] = -1;
+    for (int k = 0; k < K; k++) {
+      int a = vol_of(sv, rv, k);
+      if (a > max_v) { nmax = 1; maxp[0] = k; max_v = a; changed++; mark[k] = changed; }
+      else if (a == max_v) { maxp[nmax++] = k; mark[k] = changed; }
+    }
+    for (int z = 0; z < nov; z++) {
+      int v = ordering[z];
+      for (int k = 0; k < K; k++) w.inmax[k] = (unsigned char)(mark[k] == changed);
+      int d = vol_eval(v, net_ptr, vtx_ptr, vtx_nets, src, K, C, part, ep_ends, ep_parts, ep_cnt, sizes, sv, rv,
+                       max_v, w.inmax, maxp, nmax, w.marker, w.active, w.scores, w.gains, w.losses);
+      if (d < 0) continue;
+      vol_apply(v, d, net_ptr, vtx_ptr, vtx_nets, src, part, ep_ends, ep_parts, ep_cnt, sizes, sv, rv);
+      /* MSRV/partition.cpp:486-512: recomputed only when a listed max part
+       * changed its volume, and without resetting the list, the max or the
+       * marks: the max never falls within a pass and the list keeps growing */
+      int redo = 0;
+      for (int i = 0; i < nmax; i++) if (max_v != vol_of(sv, rv, maxp[i])) redo = 1;
+      for (int k = 0; redo && k < K; k++) {
+        int a = vol_of(sv, rv, k);
+        if (a > max_v) { nmax = 1; maxp[0] = k; max_v = a; changed++; mark[k] = changed; }
+        else if (a == max_v) {
+          if (nmax == cap) {
+            cap *= 2;
+            int* g = (int*)realloc(maxp, sizeof(int) * (size_t)cap);
+            if (!g) return 2;
+            maxp = g;
+          }
+          maxp[nmax++] = k;
+          mark[k] = changed;
+        }
+      }
+    }
+  }
+  free(maxp); free(mark);
+  vol_scratch_free(&w);
+  return 0;
+}

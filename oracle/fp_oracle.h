@@ -100,6 +100,70 @@ int or_cn_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* 
                      const unsigned* vtx_nets, int K, int passes, int* part, int* ep_ends, int* ep_parts,
                      int* ep_cnt, int* sizes);
 
+/* ---- MSV (the reference's MGV directory) -------------------------------- */
+/* Streaming initial partition with band-free LDG (MGV/partition.cpp:28-65,
+ * stream at :122-189, identical in partition_par at :583-643). sv[K] receives
+ * the send volumes the stream accumulates (MGV/partition.cpp:160-185):
+ * sv[p] = sum over nets n whose source is in p of (lambda(n) - 1). */
+int or_msv_ldg(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+               const int* src, const int* ordering, int K, double C, int* part, int* ep_ends, int* ep_parts,
+               int* ep_cnt, int* sizes, int* sv);
+/* Serial refinement, partition() (MGV/partition.cpp:208-477): each vertex in
+ * ordering order decides by LDG on the live state, the move is accepted iff
+ * every max-send-volume part gains at least what it loses and no other part
+ * reaches the max; accepted moves apply at once and the max set is recomputed. */
+int or_msv_refine_serial(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr,
+                         const unsigned* vtx_nets, const int* src, const int* ordering, int K, double C,
+                         int passes, int* part, int* ep_ends, int* ep_parts, int* ep_cnt, int* sizes, int* sv);
+/* Round schedule of partition_par (MGV/partition.cpp:706-1040) with its data
+ * race (refined_cnt_glob read before it is written, :866-872) removed: rounds
+ * of R consecutive VERTEX IDS (the refinement ignores the ordering,
+ * :755-757); every vertex of a round is evaluated on the state and max set
+ * frozen at the round start; accepted moves apply in vertex order, each
+ * followed by a recomputation of the max set; the round ends early after the
+ * first move whose max set holds a part the previous max set did not
+ * (:972-1003). R = 256*threads reproduces partition_par. moves (nullable)
+ * receives the number of applied moves. */
+int or_msv_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                      const int* src, int K, double C, int passes, int R, int* part, int* ep_ends, int* ep_parts,
+                      int* ep_cnt, int* sizes, int* sv, long long* moves);
+
+/* delta[k] = losses[k] - gains[k] that the reference's evaluation assigns to
+ * moving v to d (MGV/partition.cpp:800-840; recv = 1 adds the receive terms
+ * of MSRV/partition.cpp:852-897), on the given state. */
+int or_vol_move_delta(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                      const int* src, int K, const int* part, const int* ep_ends, const int* ep_parts,
+                      const int* ep_cnt, int recv, int v, int d, int* delta);
+
+/* ---- MSRV ---------------------------------------------------------------- */
+/* MSRV/partition.cpp is not compilable as shipped (a missing ';' at
+ * MSRV/partition.cpp:1341), so these three are restatements without a
+ * reference run to compare against: parity unpinned by the reference. The
+ * shared machinery (LDG stream, round schedule, apply, stop rule) is the MSV
+ * code above, which is pinned; the MSRV-specific terms are checked by
+ * invariants (incremental volumes equal the from-scratch metric) and by
+ * brute force on the predicted volume changes.
+ *
+ * Initial partition with send and receive volumes (MSRV/partition.cpp:131-208). */
+int or_msrv_ldg(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                const int* src, const int* ordering, int K, double C, int* part, int* ep_ends, int* ep_parts,
+                int* ep_cnt, int* sizes, int* sv, int* rv);
+/* partition_par's refinement (MSRV/partition.cpp:769-1110): as
+ * or_msv_refine_par on sv + rv, visiting ordering[z] (:815), with receive
+ * terms in the gains and losses (:869-896), a strict gain required of every
+ * max part (:903-911), and the receive bookkeeping of the apply (:952-1052). */
+int or_msrv_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                       const int* src, const int* ordering, int K, double C, int passes, int R, int* part,
+                       int* ep_ends, int* ep_parts, int* ep_cnt, int* sizes, int* sv, int* rv, long long* moves);
+/* partition()'s refinement (MSRV/partition.cpp:240-517), including its max-set
+ * bookkeeping: after a move the max list is recomputed only if a listed part's
+ * volume changed, and then without resetting the list, the max or the marks,
+ * so the max never decreases within a pass and parts are listed repeatedly. */
+int or_msrv_refine_serial(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr,
+                          const unsigned* vtx_nets, const int* src, const int* ordering, int K, double C,
+                          int passes, int* part, int* ep_ends, int* ep_parts, int* ep_cnt, int* sizes, int* sv,
+                          int* rv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,8 +56,9 @@ class State:
     def copy(self):
         s = State.__new__(State)
         s.K = self.K
-        for k in ("part", "ep_ends", "ep_parts", "ep_cnt", "sizes"):
-            setattr(s, k, getattr(self, k).copy())
+        for k in ("part", "ep_ends", "ep_parts", "ep_cnt", "sizes", "sv", "rv"):
+            if hasattr(self, k):
+                setattr(s, k, getattr(self, k).copy())
         return s
 
 
@@ -158,6 +159,94 @@ def cn_refine(hg, st, passes, serial):
            _p(st.ep_parts, _I32P), _p(st.ep_cnt, _I32P), _p(st.sizes, _I32P))
     if rc:
         raise ValueError(f"or_cn_refine rc={rc}")
+    return st
+
+
+def msv_ldg(hg, K, C, ordering=None):
+    """MGV initial partition; st.sv holds the send volumes it accumulated."""
+    st = State(hg, K)
+    st.sv = np.zeros(K, dtype=np.int32)
+    rc = lib().or_msv_ldg(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P), _p(hg.vtx_nets, _U32P),
+                          _p(hg.src, _I32P), _p(_ord(hg, ordering), _I32P), K, ctypes.c_double(C),
+                          _p(st.part, _I32P), _p(st.ep_ends, _I32P), _p(st.ep_parts, _I32P), _p(st.ep_cnt, _I32P),
+                          _p(st.sizes, _I32P), _p(st.sv, _I32P))
+    if rc:
+        raise ValueError(f"or_msv_ldg rc={rc}")
+    return st
+
+
+def msv_refine_serial(hg, st, C, passes, ordering=None):
+    rc = lib().or_msv_refine_serial(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
+                                    _p(hg.vtx_nets, _U32P), _p(hg.src, _I32P), _p(_ord(hg, ordering), _I32P), st.K,
+                                    ctypes.c_double(C), passes, _p(st.part, _I32P), _p(st.ep_ends, _I32P),
+                                    _p(st.ep_parts, _I32P), _p(st.ep_cnt, _I32P), _p(st.sizes, _I32P),
+                                    _p(st.sv, _I32P))
+    if rc:
+        raise ValueError(f"or_msv_refine_serial rc={rc}")
+    return st
+
+
+def msv_refine_par(hg, st, C, passes, R):
+    """Returns the number of applied moves."""
+    assert st.sizes.dtype == np.int32 and st.sv.dtype == np.int32
+    moves = ctypes.c_longlong(0)
+    rc = lib().or_msv_refine_par(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
+                                 _p(hg.vtx_nets, _U32P), _p(hg.src, _I32P), st.K, ctypes.c_double(C), passes, R,
+                                 _p(st.part, _I32P), _p(st.ep_ends, _I32P), _p(st.ep_parts, _I32P),
+                                 _p(st.ep_cnt, _I32P), _p(st.sizes, _I32P), _p(st.sv, _I32P), ctypes.byref(moves))
+    if rc:
+        raise ValueError(f"or_msv_refine_par rc={rc}")
+    return moves.value
+
+
+def vol_move_delta(hg, st, v, d, recv):
+    """losses - gains per part of the reference's evaluation of v -> d."""
+    out = np.zeros(st.K, dtype=np.int32)
+    rc = lib().or_vol_move_delta(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
+                                 _p(hg.vtx_nets, _U32P), _p(hg.src, _I32P), st.K, _p(st.part, _I32P),
+                                 _p(st.ep_ends, _I32P), _p(st.ep_parts, _I32P), _p(st.ep_cnt, _I32P), int(recv),
+                                 int(v), int(d), _p(out, _I32P))
+    if rc:
+        raise ValueError(f"or_vol_move_delta rc={rc}")
+    return out
+
+
+def msrv_ldg(hg, K, C, ordering=None):
+    """MSRV initial partition; st.sv / st.rv hold the send / receive volumes."""
+    st = State(hg, K)
+    st.sv = np.zeros(K, dtype=np.int32)
+    st.rv = np.zeros(K, dtype=np.int32)
+    rc = lib().or_msrv_ldg(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P), _p(hg.vtx_nets, _U32P),
+                           _p(hg.src, _I32P), _p(_ord(hg, ordering), _I32P), K, ctypes.c_double(C),
+                           _p(st.part, _I32P), _p(st.ep_ends, _I32P), _p(st.ep_parts, _I32P), _p(st.ep_cnt, _I32P),
+                           _p(st.sizes, _I32P), _p(st.sv, _I32P), _p(st.rv, _I32P))
+    if rc:
+        raise ValueError(f"or_msrv_ldg rc={rc}")
+    return st
+
+
+def msrv_refine_par(hg, st, C, passes, R, ordering=None):
+    """Returns the number of applied moves."""
+    assert st.sizes.dtype == np.int32
+    moves = ctypes.c_longlong(0)
+    rc = lib().or_msrv_refine_par(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
+                                  _p(hg.vtx_nets, _U32P), _p(hg.src, _I32P), _p(_ord(hg, ordering), _I32P), st.K,
+                                  ctypes.c_double(C), passes, R, _p(st.part, _I32P), _p(st.ep_ends, _I32P),
+                                  _p(st.ep_parts, _I32P), _p(st.ep_cnt, _I32P), _p(st.sizes, _I32P),
+                                  _p(st.sv, _I32P), _p(st.rv, _I32P), ctypes.byref(moves))
+    if rc:
+        raise ValueError(f"or_msrv_refine_par rc={rc}")
+    return moves.value
+
+
+def msrv_refine_serial(hg, st, C, passes, ordering=None):
+    rc = lib().or_msrv_refine_serial(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
+                                     _p(hg.vtx_nets, _U32P), _p(hg.src, _I32P), _p(_ord(hg, ordering), _I32P), st.K,
+                                     ctypes.c_double(C), passes, _p(st.part, _I32P), _p(st.ep_ends, _I32P),
+                                     _p(st.ep_parts, _I32P), _p(st.ep_cnt, _I32P), _p(st.sizes, _I32P),
+                                     _p(st.sv, _I32P), _p(st.rv, _I32P))
+    if rc:
+        raise ValueError(f"or_msrv_refine_serial rc={rc}")
     return st
 
 

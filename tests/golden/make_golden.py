@@ -28,6 +28,8 @@ CASES = [
     ("tv_par1", "tv", 8, 5, 1),
     ("cn_serial", "cn", 8, 5, 0),
     ("cn_par1", "cn", 8, 5, 1),
+    ("msv_serial", "mgv", 8, 5, 0),
+    ("msv_par1", "mgv", 8, 5, 1),
 ]
 PRINTED = ("TV", "CN", "TM", "MGM", "MGV", "MGAV")
 
@@ -37,9 +39,11 @@ def instance():
     return H.colnet(2000, H.DEG_FIXED, 5, seed=7)
 
 
-def main():
+def main(only=None):
     hg = instance()
     for name, m, K, passes, threads in CASES:
+        if only and name not in only:
+            continue
         if not O.ref_available(m):
             raise SystemExit(f"oracle/_ref/ref_{m} not built: run `make -C oracle ref`")
         r = O.run_reference(m, hg, K, passes, threads)
@@ -57,4 +61,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])
