@@ -1,0 +1,745 @@
+// fp_conn.hip -- TM and MSM refinement on the device: the round schedules of
+// partition_par in TM/partition.cpp:947-1265 (total messages) and
+// MSM/partition.cpp:1010-1400 (max send messages).
+//
+// Both keep the part-connection matrix pc[a][b] = number of nets whose source
+// is in part a and that have pins in part b != a; msg[a] = number of b with
+// pc[a][b] > 0. Besides pc (int, K x K) the engine keeps its nonzero pattern
+// as two bitmaps, B (rows a) and BT (rows b), so that the loss terms of a
+// candidate part k -- "receiver r with pc[k][r] == 0" and "sender s with
+// pc[s][k] == 0" -- are one bit of a 64-bit word per (r or s, 64 parts k).
+//
+// A TM round (the reference runs a whole pass as one round):
+//   k_conn_eval   one wavefront per vertex: receivers, senders and gain on the
+//                 frozen state, then the best part over all k (lanes = k)
+//   k_tm_admit    one wavefront: candidates in vertex order, skipping a
+//                 destination whose size reached C * 1.02 (TM/partition.cpp:
+//                 1115-1122); commits part vector and sizes
+//   k_tm_apply    one wavefront per touched net replays the admitted moves on
+//                 its list and adds each move's pc changes (atomics: pc's final
+//                 value does not depend on their order)
+//   k_conn_bits   B, BT and msg from pc
+// An MSM round:
+//   k_conn_eval, k_round_compact
+//   k_msm_replay  (dry) per touched net and move, the pc changes the
+//                 reference's apply makes, into per-incidence event slots
+//   k_msm_scan    one wavefront applies the moves' pc changes in order, counts
+//                 each part's messages from the 0 <-> 1 crossings, and stops
+//                 after the first move whose max-message set gains a part
+//   k_msm_replay  (commit) list edits, part vector and sizes up to the stop
+//   k_conn_bits
+#include "fp_round.h"
+
+namespace fpe {
+namespace {
+
+constexpr int CONN_BLOCK = 256;
+constexpr int CONN_KMAX = 4096;
+constexpr int REPLAY_BLOCK = 256;
+constexpr int REPLAY_NET_BLOCKS = 2048;
+
+__global__ void k_conn_init(const unsigned* __restrict__ net_ptr, const unsigned* __restrict__ ent,
+                            const int* __restrict__ src, const int* __restrict__ part, int noe, unsigned pmask, int K,
+                            int* __restrict__ pc) {
+  for (int n = (int)(blockIdx.x * blockDim.x + threadIdx.x); n < noe; n += (int)(gridDim.x * blockDim.x)) {
+    const unsigned seg = net_ptr[n] + (unsigned)n;
+    const int L = (int)ent[seg];
+    if (L < 2) continue;
+    const int sp = part[src[n]];
+    for (int t = 0; t < L; t++) {
+      const int q = (int)(ent[seg + 1 + (unsigned)t] & pmask);
+      if (q != sp) atomicAdd(&pc[(size_t)sp * (size_t)K + (size_t)q], 1);
+    }
+  }
+}
+
+// One wavefront per 64 x 64 tile of pc: the tile's words of B and BT, and the
+// popcounts of its B words into msg (zeroed before).
+__global__ void __launch_bounds__(WAVE) k_conn_bits(const int* __restrict__ pc, int K, int W,
+                                                     unsigned long long* __restrict__ B,
+                                                     unsigned long long* __restrict__ BT, int* __restrict__ msg) {
+  __shared__ int tile[WAVE][WAVE + 1];
+  const int lane = lane_id();
+  const int ta = (int)blockIdx.y, tb = (int)blockIdx.x;
+  for (int r = 0; r < WAVE; r++) {
+    const int a = ta * WAVE + r, b = tb * WAVE + lane;
+    tile[r][lane] = (a < K && b < K) ? pc[(size_t)a * (size_t)K + (size_t)b] : 0;
+  }
+  wave_lds_sync();
+  unsigned long long row = 0, col = 0;
+  for (int c = 0; c < WAVE; c++) {
+    // row a = ta*64 + lane, column b = tb*64 + c; and row b = tb*64 + lane, column a = ta*64 + c
+    if (tile[lane][c] > 0 && ta * WAVE + lane != tb * WAVE + c) row |= 1ull << c;
+    if (tile[c][lane] > 0 && ta * WAVE + c != tb * WAVE + lane) col |= 1ull << c;
+  }
+  const int a = ta * WAVE + lane, b = tb * WAVE + lane;
+  if (a < K) {
+    B[(size_t)a * (size_t)W + (size_t)tb] = row;
+    if (row) atomicAdd(&msg[a], __popcll(row));
+  }
+  if (b < K) BT[(size_t)b * (size_t)W + (size_t)ta] = col;
+}
+
+struct ConnEvalArgs {
+  const unsigned* vtx_ptr;
+  const unsigned* vtx_nets;
+  const unsigned* vtx_seg;
+  const unsigned short* ev;
+  const int* src;
+  const int* part;
+  const long long* sizes;
+  const int* pc;
+  const unsigned long long* B;
+  const unsigned long long* BT;
+  const int* msg;
+  const int* mstat;
+  int K, W, msm;
+  int r0, nslots;
+  double C;
+  int* slot_own;
+  int* slot_dec;
+  unsigned tag;
+  unsigned* net_stamp;
+  int* touched;
+  int* tcount;
+  int* tcount_next;
+};
+
+__device__ __forceinline__ bool bit_of(const unsigned* m, int p) { return (m[p >> 5] >> (p & 31)) & 1u; }
+
+// Per vertex (TM/partition.cpp:978-1084; MSM/partition.cpp:1058-1190):
+// receivers = parts v's source nets reach after v leaves (own part if it
+// keeps pins there, every other listed part); senders = source parts of v's
+// other nets; gain = source-net targets t with pc[own][t] == 1 plus, for the
+// FIRST net of each sender part s, 1 if v is own's only pin there and
+// pc[s][own] == 1. TM: best k by gain - loss; MSM: the max-set rule.
+__global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int K = a.K, KW = (K + 31) / 32;
+  const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
+  int* recv = reinterpret_cast<int*>(lds) + (size_t)wave * (size_t)(2 * K + 3 * KW);
+  int* send = recv + K;
+  unsigned* rmark = reinterpret_cast<unsigned*>(send + K);
+  unsigned* smark = rmark + KW;
+  unsigned* sgain = smark + KW;
+  for (int q = lane; q < KW; q += WAVE) {
+    rmark[q] = 0;
+    smark[q] = 0;
+    sgain[q] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && a.tcount_next) *a.tcount_next = 0;
+  wave_lds_sync();
+  const int M = a.msm ? a.mstat[0] : 0, cntM = a.msm ? a.mstat[1] : 0;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+
+  for (int slot = (int)blockIdx.x * nw + wave; slot < a.nslots; slot += (int)gridDim.x * nw) {
+    const int v = a.r0 + slot;
+    const int own = a.part[v];
+    const unsigned vb = a.vtx_ptr[v];
+    const int deg = (int)(a.vtx_ptr[v + 1] - vb);
+    int nrecv = 0, nsend = 0, gain_own = 0, gain_s = 0;
+    for (int i = 0; i < deg; i++) {
+      const int n = (int)a.vtx_nets[vb + (unsigned)i];
+      const unsigned sg = a.vtx_seg[vb + (unsigned)i];
+      const int L = (int)a.ev[sg];
+      const int s = a.src[n];
+      if (s == v) {
+        for (int t0 = 0; t0 < L; t0 += WAVE) {
+          const int t = t0 + lane;
+          const bool ok = t < L;
+          const unsigned x = ok ? a.ev[sg + 1u + (unsigned)t] : 0u;
+          const int q = (int)(x & ~EV_SINGLE);
+          const bool rec = ok && (q != own || !(x & EV_SINGLE));
+          bool isnew = false;
+          if (rec) {
+            const unsigned bit = 1u << (q & 31);
+            isnew = (atomicOr(&rmark[q >> 5], bit) & bit) == 0u;
+          }
+          const unsigned long long bm = __ballot(isnew);
+          if (isnew) recv[nrecv + __popcll(bm & lt)] = q;
+          nrecv += __popcll(bm);
+          if (ok && q != own && a.pc[(size_t)own * (size_t)K + (size_t)q] == 1) gain_own++;
+        }
+      } else {
+        const int ps = a.part[s];
+        const bool first = !bit_of(smark, ps);
+        wave_lds_sync();
+        if (first) {
+          if (lane == 0) {
+            smark[ps >> 5] |= 1u << (ps & 31);
+            send[nsend] = ps;
+          }
+          nsend++;
+          bool single = false;
+          for (int t0 = 0; t0 < L; t0 += WAVE) {
+            const int t = t0 + lane;
+            const bool ok = t < L;
+            const unsigned x = ok ? a.ev[sg + 1u + (unsigned)t] : 0u;
+            single = single || __ballot(ok && (int)(x & ~EV_SINGLE) == own && (x & EV_SINGLE)) != 0;
+          }
+          if (single && a.pc[(size_t)ps * (size_t)K + (size_t)own] == 1) {
+            gain_s++;
+            if (lane == 0) sgain[ps >> 5] |= 1u << (ps & 31);
+          }
+        }
+        wave_lds_sync();
+      }
+    }
+    gain_own = wave_sum(gain_own);
+    const int gain = gain_own + gain_s;
+    wave_lds_sync();
+    int dec = -1;
+    if (gain > 0) {
+      if (!a.msm) {
+        // TM: the largest gain - loss > 0 over parts lighter than C, the first k on ties
+        int bimp = 0, bk = INT_MAX;
+        for (int k0 = 0; k0 < K; k0 += WAVE) {
+          const int k = k0 + lane, w = k0 / WAVE;
+          int loss = 0;
+          for (int i = 0; i < nrecv; i++) {
+            const int r = recv[i];
+            loss += (r != k && !((a.BT[(size_t)r * (size_t)a.W + (size_t)w] >> lane) & 1ull)) ? 1 : 0;
+          }
+          for (int i = 0; i < nsend; i++) {
+            const int s = send[i];
+            loss += (s != k && !((a.B[(size_t)s * (size_t)a.W + (size_t)w] >> lane) & 1ull)) ? 1 : 0;
+          }
+          const bool valid = k < K && k != own && (double)a.sizes[k] < a.C;
+          if (valid && gain - loss > bimp) {
+            bimp = gain - loss;
+            bk = k;
+          }
+        }
+        int mimp = bimp, dummy = 0;
+        wave_max_count(mimp, dummy);
+        if (mimp > 0) {
+          const unsigned long long cand = __ballot(bimp == mimp);
+          int kk = INT_MAX;
+          if ((cand >> lane) & 1ull) kk = bk;
+          for (int off = WAVE / 2; off > 0; off >>= 1) kk = min(kk, __shfl_xor(kk, off));
+          dec = kk;
+        }
+      } else {
+        // MSM: every max part must gain strictly more than it loses, no other
+        // part may reach the max; improvement = sum over the max parts
+        const bool own_s = bit_of(smark, own);
+        const int g_own = gain_own + ((own_s && bit_of(sgain, own)) ? 1 : 0);
+        int bimp = -1, bk = INT_MAX;
+        for (int k0 = 0; k0 < K; k0 += WAVE) {
+          const int k = k0 + lane, w = k0 / WAVE;
+          const bool valid = k < K && k != own;
+          const int kc = valid ? k : own;
+          int lk = 0;  // receivers r with pc[k][r] == 0 (the diagonal never counts)
+          for (int i = 0; i < nrecv; i++) {
+            const int r = recv[i];
+            lk += (r != kc && !((a.BT[(size_t)r * (size_t)a.W + (size_t)w] >> lane) & 1ull)) ? 1 : 0;
+          }
+          bool bad = false;
+          int imp = 0, nmax = 0;
+          auto item = [&](int p, int g, int l) {
+            const int mp = a.msg[p];
+            if (mp == M) {
+              if (g <= l) bad = true;
+              imp += g - l;
+              nmax++;
+            } else if (mp + l - g >= M) {
+              bad = true;
+            }
+          };
+          item(own, g_own, own_s ? (int)!((a.B[(size_t)own * (size_t)a.W + (size_t)w] >> lane) & 1ull) : 0);
+          const bool k_s = valid && bit_of(smark, kc);
+          item(kc, (k_s && bit_of(sgain, kc)) ? 1 : 0, lk);
+          for (int i = 0; i < nsend; i++) {
+            const int s = send[i];
+            if (s == own || s == kc) continue;
+            item(s, bit_of(sgain, s) ? 1 : 0, (int)!((a.B[(size_t)s * (size_t)a.W + (size_t)w] >> lane) & 1ull));
+          }
+          if (nmax != cntM) bad = true;
+          if (valid && !bad && imp > bimp) {
+            bimp = imp;
+            bk = k;
+          }
+        }
+        int mimp = bimp, dummy = 0;
+        wave_max_count(mimp, dummy);
+        if (mimp > -1) {
+          const unsigned long long cand = __ballot(bimp == mimp);
+          int kk = INT_MAX;
+          if ((cand >> lane) & 1ull) kk = bk;
+          for (int off = WAVE / 2; off > 0; off >>= 1) kk = min(kk, __shfl_xor(kk, off));
+          dec = kk;
+        }
+      }
+    }
+    if (lane == 0) {
+      a.slot_own[slot] = own;
+      a.slot_dec[slot] = dec;
+    }
+    if (dec >= 0) {
+      for (int i = lane; i < deg; i += WAVE) {
+        const unsigned n = a.vtx_nets[vb + (unsigned)i];
+        if (atomicExch(&a.net_stamp[n], a.tag) != a.tag) a.touched[atomicAdd(a.tcount, 1)] = (int)n;
+      }
+    }
+    // clear this vertex's marks
+    wave_lds_sync();
+    for (int i = lane; i < nrecv; i += WAVE) rmark[recv[i] >> 5] = 0;
+    for (int i = lane; i < nsend; i += WAVE) {
+      smark[send[i] >> 5] = 0;
+      sgain[send[i] >> 5] = 0;
+    }
+    wave_lds_sync();
+  }
+}
+
+// TM apply order (TM/partition.cpp:1111-1125): candidates in vertex order;
+// one whose destination size is at least thr = C * 1.02 at that point is
+// dropped. Commits part vector and both sizes of the others.
+__global__ void __launch_bounds__(WAVE) k_tm_admit(int* __restrict__ slot_dec, const int* __restrict__ slot_own,
+                                                   const int* __restrict__ vwgt, int r0, int nslots, int K, double thr,
+                                                   long long* __restrict__ sizes, int* __restrict__ part,
+                                                   unsigned long long* __restrict__ moves) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  long long* sz = reinterpret_cast<long long*>(lds);
+  const int lane = lane_id();
+  for (int q = lane; q < K; q += WAVE) sz[q] = sizes[q];
+  wave_lds_sync();
+  unsigned long long nm = 0;
+  for (int s0 = 0; s0 < nslots; s0 += WAVE) {
+    const int s = s0 + lane;
+    const int d = s < nslots ? slot_dec[s] : -1;
+    const int o = d >= 0 ? slot_own[s] : 0;
+    const int w = d >= 0 ? (vwgt ? vwgt[r0 + s] : 1) : 0;
+    unsigned long long bm = __ballot(d >= 0);
+    while (bm) {
+      const int l = __ffsll((long long)bm) - 1;
+      bm &= bm - 1;
+      const int dd = __shfl(d, l), oo = __shfl(o, l), ww = __shfl(w, l);
+      const bool skip = (double)sz[dd] >= thr;
+      wave_lds_sync();
+      if (lane == 0) {
+        if (skip) {
+          slot_dec[s0 + l] = -1;
+        } else {
+          sz[oo] -= ww;
+          sz[dd] += ww;
+          part[r0 + s0 + l] = dd;
+        }
+      }
+      if (!skip) nm++;
+      wave_lds_sync();
+    }
+  }
+  for (int q = lane; q < K; q += WAVE) sizes[q] = sz[q];
+  if (lane == 0 && nm) atomicAdd(moves, nm);
+}
+
+// part of net n's source at the round start: slot_own for a vertex of the
+// round (part[] may already hold its new part), part[] otherwise
+__device__ __forceinline__ int frozen_part(int s, const int* part, const int* slot_own, int r0, int r1) {
+  return (s >= r0 && s < r1) ? slot_own[s - r0] : part[s];
+}
+
+struct ConnApplyArgs {
+  const unsigned* net_ptr;
+  const int* apply_pins;
+  const int* slot_dec;
+  const int* slot_own;
+  const int* slot_mi;  // MSM: move index of a slot
+  const int4* mv;
+  const int* touched;
+  const int* tcount;
+  const int* src;
+  const unsigned* vtx_ptr;
+  const unsigned* vtx_nets;
+  const unsigned* incoff;
+  const int* mstat;
+  const int* vwgt;
+  unsigned* ent;
+  unsigned short* ev;
+  int* part;
+  long long* sizes;
+  int* pc;
+  int2* evbuf;
+  unsigned long long* moves;
+  int K, pb;
+  unsigned pmask;
+  int r0, r1;
+  int mode;  // 0: TM apply, 1: MSM dry, 2: MSM commit
+  int net_blocks;
+};
+
+// The pc changes of one move v: own -> d on one net, from its list after the
+// move's edit (TM/partition.cpp:1178-1240, MSM/partition.cpp:1257-1330):
+//   v is the source:  +1 (d, t) for listed t != d; -1 (own, t) for listed
+//                     t not in {own, d}; -1 (own, d) if d has other pins
+//   else (source part sp):  +1 (sp, d) if d is new in the net; -1 (sp, own)
+//                     if own left the net
+// emit(cell, delta) is called by lanes; returns the number of calls.
+template <class Emit>
+__device__ void conn_changes(const unsigned* list, int L, int own, int d, bool is_src, int sp, int K, int pb,
+                             unsigned pmask, Emit emit) {
+  const int lane = lane_id();
+  if (is_src) {
+    for (int t0 = 0; t0 < L; t0 += WAVE) {
+      const int t = t0 + lane;
+      if (t < L) {
+        const unsigned x = list[t];
+        const int q = (int)(x & pmask);
+        const unsigned c = x >> pb;
+        emit(2 * t, q != d ? d * K + q : -1, 1);
+        emit(2 * t + 1, (q != own && q != d) ? own * K + q : ((q == d && c > 1u) ? own * K + d : -1), -1);
+      }
+    }
+  } else {
+    const int jd = wave_find(list, L, d, pmask);
+    const bool dnew = jd >= 0 && (list[jd] >> pb) == 1u;
+    const bool ogone = wave_find(list, L, own, pmask) < 0;
+    if (lane == 0) {
+      emit(0, dnew ? sp * K + d : -1, 1);
+      emit(1, ogone ? sp * K + own : -1, -1);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(REPLAY_BLOCK) k_conn_apply(ConnApplyArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
+  const int nstop = a.mode == 2 ? a.mstat[3] : 0;
+  if ((int)blockIdx.x >= a.net_blocks) {
+    // MSM commit: part vector and both sizes of the moves before the stop
+    const int mi = ((int)blockIdx.x - a.net_blocks) * (int)blockDim.x + (int)threadIdx.x;
+    const bool mvd = mi < nstop;
+    if (mvd) {
+      const int4 m = a.mv[mi];
+      const long long w = a.vwgt ? a.vwgt[m.x] : 1;
+      a.part[m.x] = m.z;
+      atomic_add_i64(&a.sizes[m.y], -w);
+      atomic_add_i64(&a.sizes[m.z], w);
+    }
+    const unsigned long long b = __ballot(mvd);
+    if (lane == 0 && b) atomicAdd(a.moves, (unsigned long long)__popcll(b));
+    return;
+  }
+  unsigned* list = reinterpret_cast<unsigned*>(lds) + (size_t)wave * (size_t)a.K;
+  const int ntouched = *a.tcount;
+  for (int i = (int)blockIdx.x * nw + wave; i < ntouched; i += a.net_blocks * nw) {
+    const int n = a.touched[i];
+    const unsigned s = a.net_ptr[n], e = a.net_ptr[n + 1];
+    const unsigned seg = s + (unsigned)n;
+    int L = wave_list_load(a.ent, seg, list);
+    const unsigned lo = wave_lower_bound(a.apply_pins, s, e, a.r0);
+    const unsigned hi = wave_lower_bound(a.apply_pins, lo, e, a.r1);
+    const int srcv = a.src[n];
+    int sp = frozen_part(srcv, a.part, a.slot_own, a.r0, a.r1);
+    bool dirty = false;
+    for (unsigned k0 = lo; k0 < hi; k0 += WAVE) {
+      const unsigned k = k0 + (unsigned)lane;
+      int slot = -1, dd = -1;
+      if (k < hi) {
+        slot = a.apply_pins[k] - a.r0;
+        if (a.mode == 0) {
+          dd = a.slot_dec[slot];
+        } else {
+          const int mi = a.slot_mi[slot];
+          dd = (mi >= 0 && (a.mode == 1 || mi < nstop)) ? mi : -1;
+        }
+      }
+      unsigned long long bm = __ballot(dd >= 0);
+      while (bm) {
+        const int l = __ffsll((long long)bm) - 1;
+        bm &= bm - 1;
+        const int sl = __shfl(slot, l);
+        const int v = a.r0 + sl;
+        int own, d;
+        if (a.mode == 0) {
+          own = a.slot_own[sl];
+          d = __shfl(dd, l);
+        } else {
+          const int4 m = a.mv[__shfl(dd, l)];
+          own = m.y;
+          d = m.z;
+        }
+        wave_list_move(list, L, own, d, a.pb, a.pmask);
+        dirty = true;
+        const bool is_src = srcv == v;
+        if (a.mode == 0) {
+          conn_changes(list, L, own, d, is_src, sp, a.K, a.pb, a.pmask, [&](int, int cell, int dl) {
+            if (cell >= 0) atomicAdd(&a.pc[cell], dl);
+          });
+        } else if (a.mode == 1) {
+          // event slots of incidence (v, n): [incoff[j], incoff[j + 1]), j = position of n in v's list
+          const unsigned vb = a.vtx_ptr[v], ve = a.vtx_ptr[v + 1];
+          const unsigned j = wave_lower_bound(reinterpret_cast<const int*>(a.vtx_nets), vb, ve, n);
+          const unsigned e0 = a.incoff[j], e1 = a.incoff[j + 1];
+          for (unsigned t = e0 + (unsigned)lane; t < e1; t += WAVE) a.evbuf[t] = make_int2(0, 0);
+          wave_lds_sync();
+          conn_changes(list, L, own, d, is_src, sp, a.K, a.pb, a.pmask, [&](int slot_i, int cell, int dl) {
+            if (cell >= 0) a.evbuf[e0 + (unsigned)slot_i] = make_int2(cell, dl);
+          });
+        }
+        if (is_src) sp = d;
+      }
+    }
+    if (a.mode != 1 && dirty) wave_list_store(a.ent, a.ev, seg, list, L, a.pb, a.pmask);
+    wave_lds_sync();
+  }
+}
+
+// MSM: the round's moves in order (MSM/partition.cpp:1230-1370). Each move's
+// pc changes go to pc with atomics; a cell crossing 0 <-> 1 changes its row's
+// message count; then the max-set step decides whether the round stops.
+__global__ void __launch_bounds__(WAVE) k_msm_scan(int* __restrict__ pc, int* __restrict__ msg_g, int K,
+                                                   const int4* __restrict__ mv, const int2* __restrict__ evbuf,
+                                                   const unsigned* __restrict__ incoff,
+                                                   const unsigned* __restrict__ vtx_ptr, int* __restrict__ mstat) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  int* msg = reinterpret_cast<int*>(lds);
+  int* delta = msg + K;
+  int* claim = delta + K;
+  const int lane = lane_id();
+  for (int q = lane; q < K; q += WAVE) {
+    msg[q] = msg_g[q];
+    delta[q] = 0;
+    claim[q] = INT_MAX;
+  }
+  wave_lds_sync();
+  int M = mstat[0], cntM = mstat[1];
+  const int nmoves = mstat[2];
+  int nstop = nmoves;
+  for (int mi = 0; mi < nmoves; mi++) {
+    const int v = mv[mi].x;
+    const unsigned e0 = incoff[vtx_ptr[v]], e1 = incoff[vtx_ptr[v + 1]];
+    const int nit = (int)(e1 - e0);
+    for (int b = 0; b < nit; b += WAVE) {
+      const int it = b + lane;
+      if (it < nit) {
+        const int2 e = evbuf[e0 + (unsigned)it];
+        if (e.y != 0) {
+          const int old = atomicAdd(&pc[e.x], e.y);
+          const int cross = (e.y > 0 && old == 0) ? 1 : ((e.y < 0 && old == 1) ? -1 : 0);
+          if (cross) atomicAdd(&delta[e.x / K], cross);
+        }
+      }
+    }
+    wave_lds_sync();
+    auto item = [&](int it) -> int {
+      if (it >= nit) return -1;
+      const int2 e = evbuf[e0 + (unsigned)it];
+      return e.y != 0 ? e.x / K : -1;
+    };
+    if (maxset_step(msg, delta, claim, K, nit, item, M, cntM)) {
+      nstop = mi + 1;
+      break;
+    }
+  }
+  if (lane == 0) {
+    mstat[0] = M;
+    mstat[1] = cntM;
+    mstat[3] = nstop;
+  }
+}
+
+int conn_eval_waves(int K) {
+  const size_t per = ((size_t)2 * (size_t)K + 3 * (size_t)((K + 31) / 32)) * sizeof(int);
+  for (int w = CONN_BLOCK / WAVE; w >= 1; w--)
+    if ((size_t)w * per <= (size_t)LDS_BUDGET) return w;
+  return 0;
+}
+
+// MSM event slots per incidence (v, n): 2 * min(|n|, K) when v is n's source,
+// else 2 (the bounds of conn_changes). Built once per upload on the host.
+fp_status ensure_incoff(fp_ctx* ctx) {
+  auto& c = ctx->conn;
+  if (c.incoff_ready) return FP_OK;
+  const int nov = ctx->nov, noe = ctx->noe;
+  std::vector<unsigned> vptr((size_t)nov + 1), vnets((size_t)std::max(ctx->pins, 1ll));
+  std::vector<int> srcs((size_t)std::max(noe, 1));
+  FP_TRY(d2h(ctx, vptr.data(), ctx->vtx_ptr, (size_t)nov + 1));
+  if (ctx->pins) FP_TRY(d2h(ctx, vnets.data(), ctx->vtx_nets, (size_t)ctx->pins));
+  if (noe) FP_TRY(d2h(ctx, srcs.data(), ctx->src, (size_t)noe));
+  FP_HIP_OK(hipStreamSynchronize(ctx->stream));
+  std::vector<unsigned> off((size_t)ctx->pins + 1);
+  unsigned long long acc = 0;
+  for (int v = 0; v < nov; v++)
+    for (unsigned j = vptr[(size_t)v]; j < vptr[(size_t)v + 1]; j++) {
+      off[j] = (unsigned)acc;
+      const int n = (int)vnets[j];
+      const unsigned sz = ctx->h_net_ptr[(size_t)n + 1] - ctx->h_net_ptr[(size_t)n];
+      acc += srcs[(size_t)n] == v ? 2ull * std::min<unsigned long long>(sz, (unsigned long long)ctx->K) : 2ull;
+    }
+  if (acc >= (1ull << 32)) return fail(ctx, FP_ERR_UNSUPPORTED, "MSM event slots exceed 2^32");
+  off[(size_t)ctx->pins] = (unsigned)acc;
+  FP_TRY(dalloc(ctx, &c.incoff, (size_t)ctx->pins + 1));
+  FP_TRY(h2d(ctx, c.incoff, off.data(), (size_t)ctx->pins + 1));
+  FP_TRY(dalloc(ctx, &c.evbuf, (size_t)std::max(acc, 1ull)));
+  c.evcap = (long long)std::max(acc, 1ull);
+  c.incoff_ready = true;
+  return FP_OK;
+}
+
+fp_status ensure_conn_scratch(fp_ctx* ctx, int R) {
+  auto& c = ctx->conn;
+  const int K = ctx->K, W = (K + 63) / 64;
+  if (K > c.kcap) {
+    FP_TRY(dalloc(ctx, &c.pc, (size_t)K * (size_t)K));
+    FP_TRY(dalloc(ctx, &c.msg, (size_t)K));
+    FP_TRY(dalloc(ctx, &c.B, (size_t)K * (size_t)W));
+    FP_TRY(dalloc(ctx, &c.BT, (size_t)K * (size_t)W));
+    FP_TRY(dalloc(ctx, &c.mstat, 8));
+    c.kcap = K;
+  }
+  if (R > c.rmax) {
+    FP_TRY(dalloc(ctx, &c.slot_mi, (size_t)R));
+    FP_TRY(dalloc(ctx, &c.mv, (size_t)R));
+    c.rmax = R;
+  }
+  if (R > ctx->rmax) {
+    FP_TRY(dalloc(ctx, &ctx->slot_own, (size_t)R));
+    FP_TRY(dalloc(ctx, &ctx->slot_dec, (size_t)R));
+    ctx->rmax = R;
+  }
+  return FP_OK;
+}
+
+void launch_bits(fp_ctx* ctx) {
+  auto& c = ctx->conn;
+  const int K = ctx->K, W = (K + 63) / 64;
+  fill_async<int>(ctx, c.msg, (size_t)K, 0);
+  k_conn_bits<<<dim3((unsigned)W, (unsigned)W), WAVE, 0, ctx->stream>>>(c.pc, K, W, c.B, c.BT, c.msg);
+}
+
+}  // namespace
+
+fp_status conn_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule* sched, fp_refine_stats* stats) {
+  const bool msm = metric == FP_METRIC_MSM;
+  const int nov = ctx->nov, K = ctx->K, W = (K + 63) / 64;
+  if (K > CONN_KMAX) return fail(ctx, FP_ERR_UNSUPPORTED, "message refinement supports K <= %d", CONN_KMAX);
+  if (!ctx->src_on_pins)
+    return fail(ctx, FP_ERR_UNSUPPORTED, "message refinement needs net sources that are pins of their nets");
+  if (ctx->order)
+    return fail(ctx, FP_ERR_UNSUPPORTED,
+                "TM/MSM refinement visits vertex ids in order (TM/partition.cpp:976); upload with the identity ordering");
+  if (sched->kind != FP_SCHED_REF)
+    return fail(ctx, FP_ERR_UNSUPPORTED, "message refinement runs the reference round schedule (FP_SCHED_REF) only");
+  const int waves = conn_eval_waves(K);
+  if (waves == 0) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d needs more LDS than a workgroup has", K);
+  const size_t lds_e = (size_t)waves * ((size_t)2 * (size_t)K + 3 * (size_t)((K + 31) / 32)) * sizeof(int);
+  const size_t lds_r = (size_t)(REPLAY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
+  const int R = std::min(sched->round, nov);
+  FP_TRY(ensure_conn_scratch(ctx, R));
+  if (msm) FP_TRY(ensure_incoff(ctx));
+  auto& c = ctx->conn;
+  fill_async<int>(ctx, ctx->counters, 4, 0);
+  fill_async<unsigned long long>(ctx, ctx->moves, 1, 0ull);
+  fill_async<int>(ctx, c.pc, (size_t)K * (size_t)K, 0);
+  hipEvent_t e0 = pooled_event(ctx, 0), e1 = pooled_event(ctx, 1);
+  if (!e0 || !e1) return fail(ctx, FP_ERR_HIP, "hipEventCreate failed");
+  FP_HIP_OK(hipEventRecord(e0, ctx->stream));
+  const int ngrid = std::max(1, std::min((ctx->noe + 255) / 256, 4096));
+  k_conn_init<<<ngrid, 256, 0, ctx->stream>>>(ctx->net_ptr, ctx->ent, ctx->src, ctx->part, ctx->noe, ctx->pmask, K,
+                                               c.pc);
+  launch_bits(ctx);
+  const double thr = sched->ldg_C * (1.02);
+  long long rounds = 0;
+  for (int it = 0; it < passes; it++) {
+    if (msm) k_round_maxset<<<1, 1024, 0, ctx->stream>>>(c.msg, K, c.mstat);
+    for (int r0 = 0; r0 < nov; r0 += R) {
+      const int nslots = std::min(R, nov - r0);
+      ctx->tag++;
+      const int par = (int)(ctx->tag & 1u);
+      ConnEvalArgs a{};
+      a.vtx_ptr = ctx->vtx_ptr;
+      a.vtx_nets = ctx->vtx_nets;
+      a.vtx_seg = ctx->vtx_seg;
+      a.ev = ctx->ev;
+      a.src = ctx->src;
+      a.part = ctx->part;
+      a.sizes = ctx->sizes;
+      a.pc = c.pc;
+      a.B = c.B;
+      a.BT = c.BT;
+      a.msg = c.msg;
+      a.mstat = c.mstat;
+      a.K = K;
+      a.W = W;
+      a.msm = msm ? 1 : 0;
+      a.r0 = r0;
+      a.nslots = nslots;
+      a.C = sched->ldg_C;
+      a.slot_own = ctx->slot_own;
+      a.slot_dec = ctx->slot_dec;
+      a.tag = ctx->tag;
+      a.net_stamp = ctx->net_stamp;
+      a.touched = ctx->touched;
+      a.tcount = ctx->counters + par;
+      a.tcount_next = ctx->counters + (par ^ 1);
+      const int egrid = std::max(1, std::min((nslots + waves - 1) / waves, 8192));
+      k_conn_eval<<<egrid, waves * WAVE, lds_e, ctx->stream>>>(a);
+      ConnApplyArgs ap{};
+      ap.net_ptr = ctx->net_ptr;
+      ap.apply_pins = ctx->apply_pins;
+      ap.slot_dec = ctx->slot_dec;
+      ap.slot_own = ctx->slot_own;
+      ap.slot_mi = c.slot_mi;
+      ap.mv = c.mv;
+      ap.touched = ctx->touched;
+      ap.tcount = ctx->counters + par;
+      ap.src = ctx->src;
+      ap.vtx_ptr = ctx->vtx_ptr;
+      ap.vtx_nets = ctx->vtx_nets;
+      ap.incoff = c.incoff;
+      ap.mstat = c.mstat;
+      ap.vwgt = ctx->vwgt;
+      ap.ent = ctx->ent;
+      ap.ev = ctx->ev;
+      ap.part = ctx->part;
+      ap.sizes = ctx->sizes;
+      ap.pc = c.pc;
+      ap.evbuf = c.evbuf;
+      ap.moves = ctx->moves;
+      ap.K = K;
+      ap.pb = ctx->pb;
+      ap.pmask = ctx->pmask;
+      ap.r0 = r0;
+      ap.r1 = r0 + nslots;
+      ap.net_blocks = REPLAY_NET_BLOCKS;
+      if (!msm) {
+        k_tm_admit<<<1, WAVE, (size_t)K * sizeof(long long), ctx->stream>>>(
+            ctx->slot_dec, ctx->slot_own, ctx->vwgt, r0, nslots, K, thr, ctx->sizes, ctx->part, ctx->moves);
+        ap.mode = 0;
+        k_conn_apply<<<REPLAY_NET_BLOCKS, REPLAY_BLOCK, lds_r, ctx->stream>>>(ap);
+      } else {
+        k_round_compact<<<1, 1024, 0, ctx->stream>>>(ctx->slot_dec, ctx->slot_own, nullptr, ctx->vtx_ptr, r0, nslots,
+                                                      c.slot_mi, c.mv, c.mstat);
+        ap.mode = 1;
+        k_conn_apply<<<REPLAY_NET_BLOCKS, REPLAY_BLOCK, lds_r, ctx->stream>>>(ap);
+        k_msm_scan<<<1, WAVE, (size_t)3 * (size_t)K * sizeof(int), ctx->stream>>>(c.pc, c.msg, K, c.mv, c.evbuf,
+                                                                                   c.incoff, ctx->vtx_ptr, c.mstat);
+        ap.mode = 2;
+        k_conn_apply<<<REPLAY_NET_BLOCKS + (nslots + REPLAY_BLOCK - 1) / REPLAY_BLOCK, REPLAY_BLOCK, lds_r,
+                       ctx->stream>>>(ap);
+      }
+      launch_bits(ctx);
+      rounds++;
+    }
+  }
+  FP_HIP_OK(hipGetLastError());
+  FP_HIP_OK(hipEventRecord(e1, ctx->stream));
+  unsigned long long mv = 0;
+  FP_TRY(d2h(ctx, &mv, ctx->moves, 1));
+  FP_HIP_OK(hipStreamSynchronize(ctx->stream));
+  float ms = 0;
+  FP_HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+  ctx->timed_launches = 0;
+  ctx->timed_ms = 0;
+  if (stats) {
+    stats->moves = (long long)mv;
+    stats->rounds = rounds;
+    stats->pin_visits = (long long)passes * ctx->pins;
+    stats->device_ms = ms;
+  }
+  return FP_OK;
+}
+
+}  // namespace fpe

@@ -293,6 +293,22 @@ struct fp_ctx {
     long long evcap = 0;
   } vol;
 
+  // TM / MSM refinement scratch (allocated on first use)
+  struct {
+    int* pc = nullptr;                 // K*K: nets with source in part a and pins in part b
+    int* msg = nullptr;                // K: parts each part sends to
+    unsigned long long* B = nullptr;   // K rows of ceil(K/64) words: bit (a, b) = pc[a][b] > 0
+    unsigned long long* BT = nullptr;  // transpose of B
+    int* mstat = nullptr;              // as vol.mstat, over msg
+    int* slot_mi = nullptr;
+    int4* mv = nullptr;
+    int2* evbuf = nullptr;             // MSM: per incidence, pc changes (cell, +-1)
+    unsigned* incoff = nullptr;        // MSM: first event slot of each incidence (pins + 1)
+    int kcap = 0, rmax = 0;
+    long long evcap = 0;
+    bool incoff_ready = false;
+  } conn;
+
   // metric scratch
   unsigned* p2p = nullptr;
   unsigned long long* mvec = nullptr;  // 5*K + 2
@@ -309,6 +325,7 @@ namespace fpe {
 fp_status fail(fp_ctx* c, fp_status st, const char* fmt, ...);
 fp_status cn_refine(fp_ctx* ctx, int passes, fp_refine_stats* stats);
 fp_status vol_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule* sched, fp_refine_stats* stats);
+fp_status conn_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule* sched, fp_refine_stats* stats);
 fp_status dalloc_bytes(fp_ctx* ctx, void** p, size_t bytes);
 void dfree_all(fp_ctx* ctx);
 hipEvent_t pooled_event(fp_ctx* ctx, size_t i);

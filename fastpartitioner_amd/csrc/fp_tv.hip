@@ -536,6 +536,10 @@ fp_status fp_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule
     FP_HIP_OK(hipSetDevice(ctx->device));
     return vol_refine(ctx, metric, passes, sched, stats);
   }
+  if (metric == FP_METRIC_TM || metric == FP_METRIC_MSM) {
+    FP_HIP_OK(hipSetDevice(ctx->device));
+    return conn_refine(ctx, metric, passes, sched, stats);
+  }
   if (metric != FP_METRIC_TV) return fail(ctx, FP_ERR_UNSUPPORTED, "refinement for metric %d not available", (int)metric);
   const int nov = ctx->nov, K = ctx->K;
   const int waves = eval_waves(K);

@@ -21,7 +21,7 @@
 //                  (MGV/partition.cpp:977-1003)
 //   k_vol_replay   (commit) list edits, part vector and both part sizes of the
 //                  moves up to the stop
-#include "fp_internal.h"
+#include "fp_round.h"
 
 namespace fpe {
 namespace {
@@ -47,53 +47,6 @@ __global__ void k_vol_init(const unsigned* __restrict__ net_ptr, const unsigned*
         const int q = (int)(ent[seg + 1 + (unsigned)t] & pmask);
         if (q != sp) atomicAdd(&vol[q], 1);
       }
-  }
-}
-
-__device__ __forceinline__ void wave_max_count(int& m, int& c) {
-  for (int off = WAVE / 2; off > 0; off >>= 1) {
-    const int om = __shfl_xor(m, off), oc = __shfl_xor(c, off);
-    if (om > m) {
-      m = om;
-      c = oc;
-    } else if (om == m) {
-      c += oc;
-    }
-  }
-}
-
-// mstat[0] = max volume, mstat[1] = number of parts holding it (MGV/partition.cpp:712-729)
-__global__ void __launch_bounds__(1024) k_vol_maxset(const int* __restrict__ vol, int K, int* __restrict__ mstat) {
-  __shared__ int s_m[16], s_c[16];
-  int m = INT_MIN, c = 0;
-  for (int q = (int)threadIdx.x; q < K; q += (int)blockDim.x) {
-    const int x = vol[q];
-    if (x > m) {
-      m = x;
-      c = 1;
-    } else if (x == m) {
-      c++;
-    }
-  }
-  wave_max_count(m, c);
-  const int lane = lane_id(), w = (int)(threadIdx.x / WAVE);
-  if (lane == 0) {
-    s_m[w] = m;
-    s_c[w] = c;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int bm = INT_MIN, bc = 0;
-    for (int i = 0; i < (int)(blockDim.x / WAVE); i++) {
-      if (s_m[i] > bm) {
-        bm = s_m[i];
-        bc = s_c[i];
-      } else if (s_m[i] == bm) {
-        bc += s_c[i];
-      }
-    }
-    mstat[0] = bm;
-    mstat[1] = bc;
   }
 }
 
@@ -328,37 +281,6 @@ __global__ void __launch_bounds__(VOL_BLOCK) k_vol_eval(VolEvalArgs a) {
   }
 }
 
-// Accepted moves of the round in visiting order: slot_mi[slot] = move index
-// (or -1), mv[i] = (vertex, own part, new part, first event slot).
-__global__ void __launch_bounds__(1024) k_vol_compact(const int* __restrict__ slot_dec, const int* __restrict__ slot_own,
-                                                      const int* __restrict__ order,
-                                                      const unsigned* __restrict__ vtx_ptr, int r0, int nslots,
-                                                      int* __restrict__ slot_mi, int4* __restrict__ mv,
-                                                      int* __restrict__ mstat) {
-  __shared__ int s_w[16];
-  int base = 0, ebase = 0;
-  for (int s0 = 0; s0 < nslots; s0 += (int)blockDim.x) {
-    const int s = s0 + (int)threadIdx.x;
-    int f = 0, deg = 0, v = 0;
-    if (s < nslots && slot_dec[s] >= 0) {
-      f = 1;
-      v = order ? order[r0 + s] : r0 + s;
-      deg = (int)(vtx_ptr[v + 1] - vtx_ptr[v]);
-    }
-    int tot, totd;
-    const int ex = block_excl_scan(f, s_w, &tot);
-    const int exd = block_excl_scan(deg, s_w, &totd);
-    if (s < nslots) slot_mi[s] = f ? base + ex : -1;
-    if (f) mv[base + ex] = make_int4(v, slot_own[s], slot_dec[s], ebase + exd);
-    base += tot;
-    ebase += totd;
-  }
-  if (threadIdx.x == 0) {
-    mstat[2] = base;
-    mstat[3] = base;
-  }
-}
-
 struct VolReplayArgs {
   const unsigned* net_ptr;
   const int* apply_pins;
@@ -499,13 +421,6 @@ __global__ void __launch_bounds__(WAVE) k_vol_scan(int* __restrict__ vol_g, int 
     const int own = m.y, d = m.z;
     const unsigned eb = (unsigned)m.w;
     const int deg = (int)(vtx_ptr[m.x + 1] - vtx_ptr[m.x]);
-    const int nit = deg + 2;
-    // item it: 0 -> own, 1 -> d, 2 + i -> source part recorded for net i (or -1)
-    auto item = [&](int it) -> int {
-      if (it == 0) return own;
-      if (it == 1) return d;
-      return it < nit ? evbuf[eb + (unsigned)(it - 2)].x : -1;
-    };
     int s_own = 0, s_dec = 0;
     for (int b = 0; b < deg; b += WAVE) {
       const int i = b + lane;
@@ -523,98 +438,13 @@ __global__ void __launch_bounds__(WAVE) k_vol_scan(int* __restrict__ vol_g, int 
       atomicAdd(&delta[d], s_dec);
     }
     wave_lds_sync();
-    for (int b = 0; b < nit; b += WAVE) {
-      const int p = item(b + lane);
-      if (p >= 0) atomicMin(&claim[p], b + lane);
-    }
-    wave_lds_sync();
-    // one representative item per affected part: old and new volumes
-    int up = INT_MIN, inS = 0, newM = 0;
-    bool enterM = false;
-    for (int b = 0; b < nit; b += WAVE) {
-      const int it = b + lane;
-      const int p = item(it);
-      if (p >= 0 && claim[p] == it) {
-        const int old = vol[p], nw = old + delta[p];
-        up = max(up, nw);
-        inS += old == M;
-        newM += nw == M;
-        enterM = enterM || (nw == M && old != M);
-      }
-    }
-    {
-      int c0 = 0;
-      wave_max_count(up, c0);
-    }
-    inS = wave_sum(inS);
-    newM = wave_sum(newM);
-    enterM = __ballot(enterM) != 0;
-    wave_lds_sync();
-    // commit the new volumes; delta still holds each part's change
-    for (int b = 0; b < nit; b += WAVE) {
-      const int it = b + lane;
-      const int p = item(it);
-      if (p >= 0 && claim[p] == it) vol[p] += delta[p];
-    }
-    wave_lds_sync();
-    bool stop;
-    int nM, nC;
-    if (up > M) {
-      // the max rose: the new max set is the affected parts at `up`
-      int cu = 0;
-      bool enter = false;
-      for (int b = 0; b < nit; b += WAVE) {
-        const int it = b + lane;
-        const int p = item(it);
-        if (p >= 0 && claim[p] == it && vol[p] == up) {
-          cu++;
-          enter = enter || (vol[p] - delta[p] != M);
-        }
-      }
-      stop = __ballot(enter) != 0;
-      nM = up;
-      nC = wave_sum(cu);
-    } else if (cntM - inS > 0 || newM > 0) {
-      // the max stays: untouched parts at M plus affected parts that land on it
-      stop = enterM;
-      nM = M;
-      nC = cntM - inS + newM;
-    } else {
-      // the max fell: full reduction; parts at the new max that were not at the old one enter
-      int mm = INT_MIN, cc = 0;
-      for (int q = lane; q < K; q += WAVE) {
-        const int x = vol[q];
-        if (x > mm) {
-          mm = x;
-          cc = 1;
-        } else if (x == mm) {
-          cc++;
-        }
-      }
-      wave_max_count(mm, cc);
-      int stay = 0;
-      for (int b = 0; b < nit; b += WAVE) {
-        const int it = b + lane;
-        const int p = item(it);
-        if (p >= 0 && claim[p] == it && vol[p] == mm && vol[p] - delta[p] == M) stay++;
-      }
-      stay = wave_sum(stay);
-      stop = cc > stay;
-      nM = mm;
-      nC = cc;
-    }
-    M = nM;
-    cntM = nC;
-    wave_lds_sync();
-    for (int b = 0; b < nit; b += WAVE) {
-      const int p = item(b + lane);
-      if (p >= 0) {
-        delta[p] = 0;
-        claim[p] = INT_MAX;
-      }
-    }
-    wave_lds_sync();
-    if (stop) {
+    // affected parts: 0 -> own, 1 -> d, 2 + i -> source part recorded for net i (or -1)
+    auto item = [&](int it) -> int {
+      if (it == 0) return own;
+      if (it == 1) return d;
+      return it < deg + 2 ? evbuf[eb + (unsigned)(it - 2)].x : -1;
+    };
+    if (maxset_step(vol, delta, claim, K, deg + 2, item, M, cntM)) {
       nstop = mi + 1;
       break;
     }
@@ -692,7 +522,7 @@ fp_status vol_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedul
                                               msrv ? 1 : 0, vs.vol);
   long long rounds = 0;
   for (int it = 0; it < passes; it++) {
-    k_vol_maxset<<<1, 1024, 0, ctx->stream>>>(vs.vol, K, vs.mstat);
+    k_round_maxset<<<1, 1024, 0, ctx->stream>>>(vs.vol, K, vs.mstat);
     for (int r0 = 0; r0 < nov; r0 += R) {
       const int nslots = std::min(R, nov - r0);
       ctx->tag++;
@@ -723,7 +553,7 @@ fp_status vol_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedul
       a.tcount_next = ctx->counters + (par ^ 1);
       const int egrid = std::max(1, std::min((nslots + waves - 1) / waves, 8192));
       k_vol_eval<<<egrid, waves * WAVE, lds_e, ctx->stream>>>(a);
-      k_vol_compact<<<1, 1024, 0, ctx->stream>>>(ctx->slot_dec, ctx->slot_own, ctx->order, ctx->vtx_ptr, r0, nslots,
+      k_round_compact<<<1, 1024, 0, ctx->stream>>>(ctx->slot_dec, ctx->slot_own, ctx->order, ctx->vtx_ptr, r0, nslots,
                                                   vs.slot_mi, vs.mv, vs.mstat);
       VolReplayArgs ra{};
       ra.net_ptr = ctx->net_ptr;

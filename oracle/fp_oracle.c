@@ -1017,3 +1017,386 @@ int or_msrv_refine_serial(int nov, int noe, const unsigned* net_ptr, const unsig
   vol_scratch_free(&w);
   return 0;
 }
+
+/* ---------------------------------------------------------------------- */
+/* TM and MSM: part connections                                            */
+/* ---------------------------------------------------------------------- */
+/* pc[a*K + b] = number of nets whose source is in part a and that have pins
+ * in part b (b != a); msg[a] = number of b with pc[a][b] > 0. */
+
+/* Greedy initial partition (TM/partition.cpp:87-235; MSM/partition.cpp:
+ * identical stream): the part saving most messages under the capacity C,
+ * ties to the lighter part, else the lightest part. */
+static int conn_init(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                     const int* src, const int* ordering, int K, double C, int diag, int* part, int* ep_ends,
+                     int* ep_parts, int* ep_cnt, int* sizes, int* pc, int* msg) {
+  long long pins = net_ptr[noe];
+  int maxdeg = 0;
+  for (int v = 0; v < nov; v++)
+    if ((int)(vtx_ptr[v + 1] - vtx_ptr[v]) > maxdeg) maxdeg = (int)(vtx_ptr[v + 1] - vtx_ptr[v]);
+  int* src_marker = (int*)malloc(sizeof(int) * (size_t)K);
+  int* src_part_of = (int*)malloc(sizeof(int) * (size_t)(noe > 0 ? noe : 1));
+  int* active_src = (int*)malloc(sizeof(int) * (size_t)(maxdeg + 1));
+  int* active_net = (int*)malloc(sizeof(int) * (size_t)(maxdeg + 1));
+  int* src_update = (int*)malloc(sizeof(int) * (size_t)(maxdeg + 1));
+  if (!src_marker || !src_part_of || !active_src || !active_net || !src_update) return 2;
+  for (long long i = 0; i < pins; i++) { ep_cnt[i] = 0; ep_parts[i] = -1; }
+  for (int k = 0; k < K; k++) { src_marker[k] = -1; sizes[k] = 0; msg[k] = 0; }
+  for (long long i = 0; i < (long long)K * K; i++) pc[i] = 0;
+  for (int k = 0; k < K; k++) pc[(size_t)k * K + k] = diag;   /* MSM/partition.cpp:99: -1 */
+  for (int v = 0; v < nov; v++) part[v] = -1;
+  for (int n = 0; n < noe; n++) { ep_ends[n] = 0; src_part_of[n] = -1; }
+  for (int z = 0; z < nov; z++) {
+    int v = ordering[z];
+    int nsrc = 0, nupd = 0, nact = 0;
+    for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
+      int n = (int)vtx_nets[i];
+      if (src[n] == v) src_update[nupd++] = n;
+      if (src_part_of[n] != -1) {
+        int sp = src_part_of[n];
+        if (src_marker[sp] != v) { src_marker[sp] = v; active_src[nsrc++] = sp; }
+        active_net[nact++] = n;
+      }
+    }
+    int best = -1, best_saved = -1;
+    for (int k = 0; k < K; k++) {
+      if (!(sizes[k] < C)) continue;
+      int saved = 0;
+      for (int i = 0; i < nsrc; i++) if (pc[(size_t)active_src[i] * K + k] > 0) saved++;
+      for (int i = 0; i < nupd; i++) {
+        int n = src_update[i];
+        int s = (int)net_ptr[n];
+        for (int j = 0; j < ep_ends[n]; j++) if (pc[(size_t)k * K + ep_parts[s + j]] > 0) saved++;
+      }
+      if (saved > best_saved || (saved == best_saved && sizes[k] < sizes[best])) { best = k; best_saved = saved; }
+    }
+    if (best == -1) {
+      int min_load = INT_MAX;
+      for (int k = 0; k < K; k++) if (sizes[k] < min_load) { min_load = sizes[k]; best = k; }
+    }
+    int d = best;
+    part[v] = d;
+    sizes[d]++;
+    for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) list_add((int)vtx_nets[i], d, net_ptr, ep_ends, ep_parts, ep_cnt);
+    /* TM/partition.cpp:190-215 */
+    for (int i = 0; i < nact; i++) {
+      int n = active_net[i];
+      int sp = src_part_of[n];
+      if (sp == d) continue;
+      size_t c = (size_t)sp * K + d;
+      if (pc[c] == 0) { msg[sp]++; pc[c] = 1; }
+      else {
+        int s = (int)net_ptr[n];
+        for (int j = 0; j < ep_ends[n]; j++)
+          if (ep_parts[s + j] == d) { if (ep_cnt[s + j] == 1) pc[c]++; break; }
+      }
+    }
+    /* TM/partition.cpp:216-235 */
+    for (int i = 0; i < nupd; i++) {
+      int n = src_update[i];
+      src_part_of[n] = d;
+      int s = (int)net_ptr[n];
+      for (int j = 0; j < ep_ends[n]; j++) {
+        int q = ep_parts[s + j];
+        if (q == d) continue;
+        size_t c = (size_t)d * K + q;
+        if (pc[c] == 0) { msg[d]++; pc[c] = 1; }
+        else pc[c]++;
+      }
+    }
+  }
+  free(src_marker); free(src_part_of); free(active_src); free(active_net); free(src_update);
+  return 0;
+}
+
+int or_tm_init(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+               const int* src, const int* ordering, int K, double C, int* part, int* ep_ends, int* ep_parts,
+               int* ep_cnt, int* sizes, int* pc, int* msg) {
+  return conn_init(nov, noe, net_ptr, vtx_ptr, vtx_nets, src, ordering, K, C, 0, part, ep_ends, ep_parts, ep_cnt,
+                   sizes, pc, msg);
+}
+
+int or_msm_init(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                const int* src, const int* ordering, int K, double C, int* part, int* ep_ends, int* ep_parts,
+                int* ep_cnt, int* sizes, int* pc, int* msg) {
+  return conn_init(nov, noe, net_ptr, vtx_ptr, vtx_nets, src, ordering, K, C, -1, part, ep_ends, ep_parts, ep_cnt,
+                   sizes, pc, msg);
+}
+
+/* Receivers, senders and the TM gain of moving v out of its part
+ * (TM/partition.cpp:318-391). gains (nullable, K): MSM's per-part split of
+ * the same terms (MSM/partition.cpp:1075-1130: the source-net terms go to
+ * the own part, the sender terms to the sender part). Returns the total gain.
+ * mrecv/msend: K scratch marks, -1 or another vertex on entry. */
+static int conn_gain(int v, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                     const int* src, int K, const int* part, const int* ep_ends, const int* ep_parts,
+                     const int* ep_cnt, const int* pc, int* mrecv, int* msend, int* recv, int* nrecv, int* send,
+                     int* nsend, int* gains) {
+  int own = part[v];
+  int gain = 0;
+  *nrecv = 0;
+  *nsend = 0;
+  for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
+    int n = (int)vtx_nets[i];
+    int s = (int)net_ptr[n];
+    int end = ep_ends[n];
+    if (src[n] == v) {
+      for (int j = 0; j < end; j++) {
+        int t = ep_parts[s + j];
+        if (t == own && ep_cnt[s + j] > 1 && mrecv[own] != v) { recv[(*nrecv)++] = own; mrecv[own] = v; }
+        if (t != own) {
+          if (mrecv[t] != v) { recv[(*nrecv)++] = t; mrecv[t] = v; }
+          /* the reference's gain_loss_2d counter is 1 here (it is raised and
+           * lowered within one net), so the test is pc[own][t] == 1 */
+          if (pc[(size_t)own * K + t] == 1) { gain++; if (gains) gains[own]++; }
+        }
+      }
+    } else {
+      int ps = part[src[n]];
+      if (msend[ps] != v) {
+        send[(*nsend)++] = ps;
+        msend[ps] = v;
+        for (int j = 0; j < end; j++)
+          if (ep_parts[s + j] == own && ep_cnt[s + j] == 1 && pc[(size_t)ps * K + own] == 1) {
+            gain++;
+            if (gains) gains[ps]++;
+          }
+      }
+    }
+  }
+  return gain;
+}
+
+/* TM decision of v on the given state (TM/partition.cpp:318-440 and
+ * :978-1084): -1 or the part with the largest improvement gain - loss > 0
+ * (the first such part on ties), among parts lighter than C. */
+static int tm_decide(int v, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                     const int* src, int K, double C, const int* part, const int* ep_ends, const int* ep_parts,
+                     const int* ep_cnt, const int* sizes, const int* pc, int* mrecv, int* msend, int* recv,
+                     int* send) {
+  int nrecv, nsend;
+  int own = part[v];
+  int gain = conn_gain(v, net_ptr, vtx_ptr, vtx_nets, src, K, part, ep_ends, ep_parts, ep_cnt, pc, mrecv, msend, recv,
+                       &nrecv, send, &nsend, NULL);
+  if (gain <= 0) return -1;
+  int best = -1, best_imp = 0;
+  for (int k = 0; k < K; k++) {
+    if (k == own || !(sizes[k] < C)) continue;
+    int loss = 0;
+    for (int i = 0; i < nrecv; i++) if (recv[i] != k && pc[(size_t)k * K + recv[i]] == 0) loss++;
+    for (int i = 0; i < nsend; i++) if (send[i] != k && pc[(size_t)send[i] * K + k] == 0) loss++;
+    if (gain > loss && gain - loss > best_imp) { best = k; best_imp = gain - loss; }
+  }
+  return best;
+}
+
+/* Moves v to d: list edits then the connection update of
+ * TM/partition.cpp:1178-1240 (identical in MSM/partition.cpp:1257-1330). */
+static void conn_apply(int v, int d, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                       const int* src, int K, int* part, int* ep_ends, int* ep_parts, int* ep_cnt, int* sizes,
+                       int* pc, int* msg) {
+  int own = part[v];
+  part[v] = d;
+  sizes[own]--;
+  sizes[d]++;
+  move_vertex(v, own, d, net_ptr, vtx_ptr, vtx_nets, ep_ends, ep_parts, ep_cnt);
+  for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
+    int n = (int)vtx_nets[i];
+    int s = (int)net_ptr[n];
+    int end = ep_ends[n];
+    if (src[n] == v) {
+      for (int j = 0; j < end; j++) {
+        int t = ep_parts[s + j];
+        if (t != d) { if (++pc[(size_t)d * K + t] == 1) msg[d]++; }
+        if (t != own && t != d) { if (--pc[(size_t)own * K + t] == 0) msg[own]--; }
+        if (t == d && ep_cnt[s + j] > 1) { if (--pc[(size_t)own * K + t] == 0) msg[own]--; }
+      }
+    } else {
+      int sp = part[src[n]];
+      int found = 0;
+      for (int j = 0; j < end; j++) {
+        if (ep_parts[s + j] == d && ep_cnt[s + j] == 1) { if (++pc[(size_t)sp * K + d] == 1) msg[sp]++; }
+        if (ep_parts[s + j] == own) found = 1;
+      }
+      if (!found) { if (--pc[(size_t)sp * K + own] == 0) msg[sp]--; }
+    }
+  }
+}
+
+typedef struct {
+  int *mrecv, *msend, *recv, *send;
+} conn_scratch;
+
+static int conn_scratch_init(conn_scratch* w, int K) {
+  w->mrecv = (int*)malloc(sizeof(int) * (size_t)K);
+  w->msend = (int*)malloc(sizeof(int) * (size_t)K);
+  w->recv = (int*)malloc(sizeof(int) * (size_t)K);
+  w->send = (int*)malloc(sizeof(int) * (size_t)K);
+  if (!w->mrecv || !w->msend || !w->recv || !w->send) return 2;
+  for (int k = 0; k < K; k++) { w->mrecv[k] = -1; w->msend[k] = -1; }
+  return 0;
+}
+
+static void conn_scratch_free(conn_scratch* w) { free(w->mrecv); free(w->msend); free(w->recv); free(w->send); }
+
+int or_tm_refine_serial(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                        const int* src, int K, double C, int passes, int* part, int* ep_ends, int* ep_parts,
+                        int* ep_cnt, int* sizes, int* pc, int* msg) {
+  (void)noe;
+  conn_scratch w;
+  if (conn_scratch_init(&w, K)) return 2;
+  for (int it = 0; it < passes; it++) {
+    for (int k = 0; k < K; k++) { w.mrecv[k] = -1; w.msend[k] = -1; }
+    for (int v = 0; v < nov; v++) {
+      int d = tm_decide(v, net_ptr, vtx_ptr, vtx_nets, src, K, C, part, ep_ends, ep_parts, ep_cnt, sizes, pc,
+                        w.mrecv, w.msend, w.recv, w.send);
+      if (d < 0) continue;
+      conn_apply(v, d, net_ptr, vtx_ptr, vtx_nets, src, K, part, ep_ends, ep_parts, ep_cnt, sizes, pc, msg);
+    }
+  }
+  conn_scratch_free(&w);
+  return 0;
+}
+
+int or_tm_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                     const int* src, int K, double C, int passes, int R, int* part, int* ep_ends, int* ep_parts,
+                     int* ep_cnt, int* sizes, int* pc, int* msg, long long* moves) {
+  (void)noe;
+  if (R < 1) return 1;
+  conn_scratch w;
+  if (conn_scratch_init(&w, K)) return 2;
+  int* dec = (int*)malloc(sizeof(int) * (size_t)R);
+  if (!dec) return 2;
+  const double thr = C * (1.02);
+  long long applied = 0;
+  for (int it = 0; it < passes; it++) {
+    for (int k = 0; k < K; k++) { w.mrecv[k] = -1; w.msend[k] = -1; }
+    for (int r0 = 0; r0 < nov; r0 += R) {
+      int r1 = r0 + R < nov ? r0 + R : nov;
+      for (int v = r0; v < r1; v++)
+        dec[v - r0] = tm_decide(v, net_ptr, vtx_ptr, vtx_nets, src, K, C, part, ep_ends, ep_parts, ep_cnt, sizes, pc,
+                                w.mrecv, w.msend, w.recv, w.send);
+      /* TM/partition.cpp:1115-1122: a destination at or above C * 1.02 is skipped */
+      for (int v = r0; v < r1; v++) {
+        int d = dec[v - r0];
+        if (d < 0 || sizes[d] >= thr) continue;
+        conn_apply(v, d, net_ptr, vtx_ptr, vtx_nets, src, K, part, ep_ends, ep_parts, ep_cnt, sizes, pc, msg);
+        applied++;
+      }
+    }
+  }
+  if (moves) *moves = applied;
+  free(dec);
+  conn_scratch_free(&w);
+  return 0;
+}
+
+/* MSM decision of v (MSM/partition.cpp:1058-1190; serial :364-500): among
+ * k != own, the part with the largest improvement = sum over the max parts of
+ * gains - losses, provided every max part gains strictly more than it loses
+ * and no other part reaches the max; the first such k on ties; -1 if none.
+ * losses[k] counts receivers r with pc[k][r] == 0, losses[s] senders s with
+ * pc[s][k] == 0 (the diagonal of pc is -1, so it never counts). */
+static int msm_decide(int v, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                      const int* src, int K, const int* part, const int* ep_ends, const int* ep_parts,
+                      const int* ep_cnt, const int* pc, const int* msg, int max_m, const unsigned char* inmax,
+                      int* mrecv, int* msend, int* recv, int* send, int* gains, int* losses) {
+  int nrecv, nsend;
+  int own = part[v];
+  for (int k = 0; k < K; k++) gains[k] = 0;
+  int gain = conn_gain(v, net_ptr, vtx_ptr, vtx_nets, src, K, part, ep_ends, ep_parts, ep_cnt, pc, mrecv, msend, recv,
+                       &nrecv, send, &nsend, gains);
+  if (gain <= 0) return -1;   /* has_gain */
+  int best = -1, best_imp = -1;
+  for (int k = 0; k < K; k++) {
+    if (k == own) continue;
+    for (int i = 0; i < K; i++) losses[i] = 0;
+    for (int i = 0; i < nrecv; i++) if (pc[(size_t)k * K + recv[i]] == 0) losses[k]++;
+    for (int i = 0; i < nsend; i++) if (pc[(size_t)send[i] * K + k] == 0) losses[send[i]]++;
+    int good = 1, imp = 0;
+    for (int m = 0; m < K && good; m++) {
+      if (!inmax[m]) continue;
+      if (gains[m] <= losses[m]) good = 0;
+      else imp += gains[m] - losses[m];
+    }
+    for (int i = 0; i < K && good; i++)
+      if (!inmax[i] && msg[i] + (losses[i] - gains[i]) >= max_m) good = 0;
+    if (good && imp > best_imp) { best = k; best_imp = imp; }
+  }
+  return best;
+}
+
+static int msm_max_set(int K, const int* msg, unsigned char* inmax) {
+  int m = -1;
+  for (int k = 0; k < K; k++) if (msg[k] > m) m = msg[k];
+  for (int k = 0; k < K; k++) inmax[k] = (unsigned char)(msg[k] == m);
+  return m;
+}
+
+int or_msm_refine_serial(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                         const int* src, int K, int passes, int* part, int* ep_ends, int* ep_parts, int* ep_cnt,
+                         int* sizes, int* pc, int* msg) {
+  (void)noe;
+  conn_scratch w;
+  if (conn_scratch_init(&w, K)) return 2;
+  int* gains = (int*)calloc((size_t)K, sizeof(int));
+  int* losses = (int*)calloc((size_t)K, sizeof(int));
+  unsigned char* inmax = (unsigned char*)malloc((size_t)K);
+  if (!gains || !losses || !inmax) return 2;
+  for (int it = 0; it < passes; it++) {
+    for (int k = 0; k < K; k++) { w.mrecv[k] = -1; w.msend[k] = -1; }
+    for (int v = 0; v < nov; v++) {
+      /* MSM/partition.cpp:364-380: the max set is recomputed before every vertex */
+      int max_m = msm_max_set(K, msg, inmax);
+      int d = msm_decide(v, net_ptr, vtx_ptr, vtx_nets, src, K, part, ep_ends, ep_parts, ep_cnt, pc, msg, max_m,
+                         inmax, w.mrecv, w.msend, w.recv, w.send, gains, losses);
+      if (d < 0) continue;
+      conn_apply(v, d, net_ptr, vtx_ptr, vtx_nets, src, K, part, ep_ends, ep_parts, ep_cnt, sizes, pc, msg);
+    }
+  }
+  free(gains); free(losses); free(inmax);
+  conn_scratch_free(&w);
+  return 0;
+}
+
+int or_msm_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                      const int* src, int K, int passes, int R, int* part, int* ep_ends, int* ep_parts, int* ep_cnt,
+                      int* sizes, int* pc, int* msg, long long* moves) {
+  (void)noe;
+  if (R < 1) return 1;
+  conn_scratch w;
+  if (conn_scratch_init(&w, K)) return 2;
+  int* gains = (int*)calloc((size_t)K, sizeof(int));
+  int* losses = (int*)calloc((size_t)K, sizeof(int));
+  unsigned char* inmax = (unsigned char*)malloc((size_t)K);
+  unsigned char* inmax2 = (unsigned char*)malloc((size_t)K);
+  int* dec = (int*)malloc(sizeof(int) * (size_t)R);
+  if (!gains || !losses || !inmax || !inmax2 || !dec) return 2;
+  long long applied = 0;
+  for (int it = 0; it < passes; it++) {
+    for (int k = 0; k < K; k++) { w.mrecv[k] = -1; w.msend[k] = -1; }
+    int max_m = msm_max_set(K, msg, inmax);
+    for (int r0 = 0; r0 < nov; r0 += R) {
+      int r1 = r0 + R < nov ? r0 + R : nov;
+      for (int v = r0; v < r1; v++)
+        dec[v - r0] = msm_decide(v, net_ptr, vtx_ptr, vtx_nets, src, K, part, ep_ends, ep_parts, ep_cnt, pc, msg,
+                                 max_m, inmax, w.mrecv, w.msend, w.recv, w.send, gains, losses);
+      for (int v = r0; v < r1; v++) {
+        int d = dec[v - r0];
+        if (d < 0) continue;
+        conn_apply(v, d, net_ptr, vtx_ptr, vtx_nets, src, K, part, ep_ends, ep_parts, ep_cnt, sizes, pc, msg);
+        applied++;
+        /* MSM/partition.cpp:1335-1370: a new part in the max set ends the round */
+        memcpy(inmax2, inmax, (size_t)K);
+        max_m = msm_max_set(K, msg, inmax);
+        int entered = 0;
+        for (int k = 0; k < K; k++) if (inmax[k] && !inmax2[k]) entered = 1;
+        if (entered) break;
+      }
+    }
+  }
+  if (moves) *moves = applied;
+  free(gains); free(losses); free(inmax); free(inmax2); free(dec);
+  conn_scratch_free(&w);
+  return 0;
+}

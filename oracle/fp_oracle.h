@@ -135,6 +135,48 @@ int or_vol_move_delta(int nov, int noe, const unsigned* net_ptr, const unsigned*
                       const int* src, int K, const int* part, const int* ep_ends, const int* ep_parts,
                       const int* ep_cnt, int recv, int v, int d, int* delta);
 
+/* ---- TM (and MSM's shared parts) ----------------------------------------- */
+/* pc[a*K + b]: nets whose source is in part a with pins in part b != a;
+ * msg[a] = number of b with pc[a*K+b] > 0 (the sends of part a).
+ * Greedy initial partition (TM/partition.cpp:87-235). */
+int or_tm_init(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+               const int* src, const int* ordering, int K, double C, int* part, int* ep_ends, int* ep_parts,
+               int* ep_cnt, int* sizes, int* pc, int* msg);
+/* Serial refinement, partition() (TM/partition.cpp:300-593): vertex ids in
+ * order, live state, moves applied at once. */
+int or_tm_refine_serial(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                        const int* src, int K, double C, int passes, int* part, int* ep_ends, int* ep_parts,
+                        int* ep_cnt, int* sizes, int* pc, int* msg);
+/* partition_par's refinement (TM/partition.cpp:947-1265): every vertex of a
+ * round of R consecutive vertex ids decides on the state frozen at the round
+ * start; moves apply in vertex order, skipping destinations whose size is at
+ * least C * 1.02 at that point. The reference evaluates a whole pass as one
+ * round: R >= nov reproduces it with one thread (its dynamic OpenMP schedule
+ * orders the moves by thread otherwise). moves (nullable) = applied moves. */
+int or_tm_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                     const int* src, int K, double C, int passes, int R, int* part, int* ep_ends, int* ep_parts,
+                     int* ep_cnt, int* sizes, int* pc, int* msg, long long* moves);
+
+/* ---- MSM ----------------------------------------------------------------- */
+/* Greedy initial partition of MSM (MSM/partition.cpp:69-235): the TM stream
+ * with pc's diagonal set to -1 (MSM/partition.cpp:99). */
+int or_msm_init(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                const int* src, const int* ordering, int K, double C, int* part, int* ep_ends, int* ep_parts,
+                int* ep_cnt, int* sizes, int* pc, int* msg);
+/* Serial refinement, partition() (MSM/partition.cpp:347-656): vertex ids in
+ * order, the max-message set recomputed before every vertex. */
+int or_msm_refine_serial(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                         const int* src, int K, int passes, int* part, int* ep_ends, int* ep_parts, int* ep_cnt,
+                         int* sizes, int* pc, int* msg);
+/* Round schedule of partition_par (MSM/partition.cpp:1010-1400) without its
+ * data race: rounds of R consecutive vertex ids decided on the frozen state
+ * and max set; moves apply in vertex order, each followed by a recomputation
+ * of the max set; the round ends after the first move whose max set holds a
+ * part the previous one did not. R = 128*threads reproduces partition_par. */
+int or_msm_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                      const int* src, int K, int passes, int R, int* part, int* ep_ends, int* ep_parts, int* ep_cnt,
+                      int* sizes, int* pc, int* msg, long long* moves);
+
 /* ---- MSRV ---------------------------------------------------------------- */
 /* MSRV/partition.cpp is not compilable as shipped (a missing ';' at
  * MSRV/partition.cpp:1341), so these three are restatements without a

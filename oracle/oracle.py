@@ -56,7 +56,7 @@ class State:
     def copy(self):
         s = State.__new__(State)
         s.K = self.K
-        for k in ("part", "ep_ends", "ep_parts", "ep_cnt", "sizes", "sv", "rv"):
+        for k in ("part", "ep_ends", "ep_parts", "ep_cnt", "sizes", "sv", "rv", "pc", "msg"):
             if hasattr(self, k):
                 setattr(s, k, getattr(self, k).copy())
         return s
@@ -209,6 +209,76 @@ def vol_move_delta(hg, st, v, d, recv):
     if rc:
         raise ValueError(f"or_vol_move_delta rc={rc}")
     return out
+
+
+def tm_init(hg, K, C, ordering=None):
+    """TM greedy initial partition; st.pc (K*K) and st.msg (K) hold the part connections."""
+    st = State(hg, K)
+    st.pc = np.zeros(K * K, dtype=np.int32)
+    st.msg = np.zeros(K, dtype=np.int32)
+    rc = lib().or_tm_init(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P), _p(hg.vtx_nets, _U32P),
+                          _p(hg.src, _I32P), _p(_ord(hg, ordering), _I32P), K, ctypes.c_double(C),
+                          _p(st.part, _I32P), _p(st.ep_ends, _I32P), _p(st.ep_parts, _I32P), _p(st.ep_cnt, _I32P),
+                          _p(st.sizes, _I32P), _p(st.pc, _I32P), _p(st.msg, _I32P))
+    if rc:
+        raise ValueError(f"or_tm_init rc={rc}")
+    return st
+
+
+def msm_init(hg, K, C, ordering=None):
+    """MSM greedy initial partition (pc diagonal -1)."""
+    st = State(hg, K)
+    st.pc = np.zeros(K * K, dtype=np.int32)
+    st.msg = np.zeros(K, dtype=np.int32)
+    rc = lib().or_msm_init(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P), _p(hg.vtx_nets, _U32P),
+                           _p(hg.src, _I32P), _p(_ord(hg, ordering), _I32P), K, ctypes.c_double(C),
+                           _p(st.part, _I32P), _p(st.ep_ends, _I32P), _p(st.ep_parts, _I32P), _p(st.ep_cnt, _I32P),
+                           _p(st.sizes, _I32P), _p(st.pc, _I32P), _p(st.msg, _I32P))
+    if rc:
+        raise ValueError(f"or_msm_init rc={rc}")
+    return st
+
+
+def msm_refine_serial(hg, st, passes):
+    rc = lib().or_msm_refine_serial(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
+                                    _p(hg.vtx_nets, _U32P), _p(hg.src, _I32P), st.K, passes, *_conn_args(hg, st))
+    if rc:
+        raise ValueError(f"or_msm_refine_serial rc={rc}")
+    return st
+
+
+def msm_refine_par(hg, st, passes, R):
+    moves = ctypes.c_longlong(0)
+    rc = lib().or_msm_refine_par(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
+                                 _p(hg.vtx_nets, _U32P), _p(hg.src, _I32P), st.K, passes, R, *_conn_args(hg, st),
+                                 ctypes.byref(moves))
+    if rc:
+        raise ValueError(f"or_msm_refine_par rc={rc}")
+    return moves.value
+
+
+def _conn_args(hg, st):
+    return (_p(st.part, _I32P), _p(st.ep_ends, _I32P), _p(st.ep_parts, _I32P), _p(st.ep_cnt, _I32P),
+            _p(st.sizes, _I32P), _p(st.pc, _I32P), _p(st.msg, _I32P))
+
+
+def tm_refine_serial(hg, st, C, passes):
+    rc = lib().or_tm_refine_serial(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
+                                   _p(hg.vtx_nets, _U32P), _p(hg.src, _I32P), st.K, ctypes.c_double(C), passes,
+                                   *_conn_args(hg, st))
+    if rc:
+        raise ValueError(f"or_tm_refine_serial rc={rc}")
+    return st
+
+
+def tm_refine_par(hg, st, C, passes, R):
+    moves = ctypes.c_longlong(0)
+    rc = lib().or_tm_refine_par(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
+                                _p(hg.vtx_nets, _U32P), _p(hg.src, _I32P), st.K, ctypes.c_double(C), passes, R,
+                                *_conn_args(hg, st), ctypes.byref(moves))
+    if rc:
+        raise ValueError(f"or_tm_refine_par rc={rc}")
+    return moves.value
 
 
 def msrv_ldg(hg, K, C, ordering=None):

@@ -22,28 +22,39 @@ from oracle import oracle as O  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 
-# (name, reference binary, K, passes, threads: 0 = partition(), 1 = partition_par with one thread)
+# (name, reference binary, K, passes, threads: 0 = partition(), 1 = partition_par
+#  with one thread, instance)
 CASES = [
-    ("tv_serial", "tv", 8, 5, 0),
-    ("tv_par1", "tv", 8, 5, 1),
-    ("cn_serial", "cn", 8, 5, 0),
-    ("cn_par1", "cn", 8, 5, 1),
-    ("msv_serial", "mgv", 8, 5, 0),
-    ("msv_par1", "mgv", 8, 5, 1),
+    ("tv_serial", "tv", 8, 5, 0, "fixed5"),
+    ("tv_par1", "tv", 8, 5, 1, "fixed5"),
+    ("cn_serial", "cn", 8, 5, 0, "fixed5"),
+    ("cn_par1", "cn", 8, 5, 1, "fixed5"),
+    ("msv_serial", "mgv", 8, 5, 0, "fixed5"),
+    ("msv_par1", "mgv", 8, 5, 1, "fixed5"),
+    ("tm_serial", "tm", 64, 4, 0, "sparse"),
+    ("tm_par1", "tm", 64, 4, 1, "sparse"),
+    ("msm_serial", "msm", 256, 6, 0, "sparse2"),
+    ("msm_par1", "msm", 256, 6, 1, "sparse2"),
 ]
 PRINTED = ("TV", "CN", "TM", "MGM", "MGV", "MGAV")
 
 
-def instance():
-    # 2,000 x 2,000 column-net matrix: diagonal + 5 uniform off-diagonals per row
-    return H.colnet(2000, H.DEG_FIXED, 5, seed=7)
+def instance(kind):
+    if kind == "fixed5":
+        # 2,000 x 2,000 column-net matrix: diagonal + 5 uniform off-diagonals per row
+        return H.colnet(2000, H.DEG_FIXED, 5, seed=7)
+    if kind == "sparse":
+        # 1,500 x 1,500: diagonal + 1..3 uniform off-diagonals (TM moves at K = 64)
+        return H.colnet(1500, H.DEG_UNIFORM, 1, 3, seed=3)
+    # 3,000 x 3,000: diagonal + 1..3 uniform off-diagonals (MSM moves at K = 256)
+    return H.colnet(3000, H.DEG_UNIFORM, 1, 3, seed=2)
 
 
 def main(only=None):
-    hg = instance()
-    for name, m, K, passes, threads in CASES:
+    for name, m, K, passes, threads, kind in CASES:
         if only and name not in only:
             continue
+        hg = instance(kind)
         if not O.ref_available(m):
             raise SystemExit(f"oracle/_ref/ref_{m} not built: run `make -C oracle ref`")
         r = O.run_reference(m, hg, K, passes, threads)

@@ -31,6 +31,22 @@ def _run_oracle(kind, hg, K, passes, threads):
     """Oracle state after the initial partition and after refinement, for the
     reference driver mode (threads 0 = partition(), 1 = partition_par(1))."""
     C, max_pw, min_pw = H.ref_constants(hg.nov, K)
+    if kind == "tm":
+        st = O.tm_init(hg, K, C)
+        init = st.copy()
+        if threads == 0:
+            O.tm_refine_serial(hg, st, C, passes)
+        else:
+            O.tm_refine_par(hg, st, C, passes, hg.nov)
+        return init, st
+    if kind == "msm":
+        st = O.msm_init(hg, K, C)
+        init = st.copy()
+        if threads == 0:
+            O.msm_refine_serial(hg, st, passes)
+        else:
+            O.msm_refine_par(hg, st, passes, 128 * threads)
+        return init, st
     if kind in ("msv", "mgv"):
         st = O.msv_ldg(hg, K, C)
         init = st.copy()
@@ -71,10 +87,10 @@ def test_golden_fixture(path):
 
 
 def test_golden_fixtures_present():
-    assert len(glob.glob(os.path.join(GOLD, "*.npz"))) >= 6
+    assert len(glob.glob(os.path.join(GOLD, "*.npz"))) >= 10
 
 
-needs_ref = pytest.mark.skipif(not all(O.ref_available(m) for m in ("tv", "cn", "mgv")),
+needs_ref = pytest.mark.skipif(not all(O.ref_available(m) for m in ("tv", "cn", "mgv", "tm", "msm")),
                                reason="reference binaries (oracle/_ref) not built here")
 
 
@@ -92,6 +108,26 @@ def test_oracle_matches_reference_binary(kind, threads, seed):
     assert np.array_equal(st.ep_ends, r["ends"])
     assert np.array_equal(st.ep_parts, r["parts"])
     assert np.array_equal(st.ep_cnt, r["cnt"])
+
+
+@needs_ref
+@pytest.mark.parametrize("kind,K,threads", [("tm", 64, 0), ("tm", 64, 1), ("tm", 128, 1), ("msm", 256, 0),
+                                            ("msm", 256, 1)])
+@pytest.mark.parametrize("seed", [2, 3])
+def test_message_oracles_match_reference_binary(kind, K, threads, seed):
+    # sparse instances, where the message objectives make moves
+    hg = H.colnet(1500 if kind == "tm" else 3000, H.DEG_UNIFORM, 1, 3, seed=seed)
+    passes = 4
+    r = O.run_reference(kind, hg, K, passes, threads)
+    init, st = _run_oracle(kind, hg, K, passes, threads)
+    assert np.array_equal(init.part, r["init_part"])
+    assert np.array_equal(init.ep_parts, r["init_parts"])
+    assert np.array_equal(st.part, r["part"])
+    assert np.array_equal(st.ep_ends, r["ends"])
+    assert np.array_equal(st.ep_parts, r["parts"])
+    assert np.array_equal(st.ep_cnt, r["cnt"])
+    # the incrementally kept connections are the from-scratch TM
+    assert int((st.msg).sum()) == O.metrics(hg, K, st.part)["TM"]
 
 
 @needs_ref
