@@ -52,7 +52,7 @@ def build_engine():
     return out
 
 
-LEGACY_OBJECTIVES = ["tv", "cn"]
+LEGACY_OBJECTIVES = ["tv", "cn", "msv", "msrv", "tm"]
 
 
 def build_legacy(objective):

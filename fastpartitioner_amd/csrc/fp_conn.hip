@@ -15,18 +15,18 @@
 //   k_tm_admit    one wavefront: candidates in vertex order, skipping a
 //                 destination whose size reached C * 1.02 (TM/partition.cpp:
 //                 1115-1122); commits part vector and sizes
-//   k_tm_apply    one wavefront per touched net replays the admitted moves on
-//                 its list and adds each move's pc changes (atomics: pc's final
-//                 value does not depend on their order)
+//   k_conn_apply  (mode 0) one wavefront per touched net replays the admitted
+//                 moves on its list and adds each move's pc changes (atomics:
+//                 pc's final value does not depend on their order)
 //   k_conn_bits   B, BT and msg from pc
 // An MSM round:
 //   k_conn_eval, k_round_compact
-//   k_msm_replay  (dry) per touched net and move, the pc changes the
+//   k_conn_apply  (mode 1, dry) per touched net and move, the pc changes the
 //                 reference's apply makes, into per-incidence event slots
 //   k_msm_scan    one wavefront applies the moves' pc changes in order, counts
 //                 each part's messages from the 0 <-> 1 crossings, and stops
 //                 after the first move whose max-message set gains a part
-//   k_msm_replay  (commit) list edits, part vector and sizes up to the stop
+//   k_conn_apply  (mode 2, commit) list edits, part vector and sizes up to the stop
 //   k_conn_bits
 #include "fp_round.h"
 
@@ -743,3 +743,27 @@ fp_status conn_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedu
 }
 
 }  // namespace fpe
+
+using namespace fpe;
+
+extern "C" fp_status fp_get_connections(fp_ctx* ctx, int* pc, int* msg) {
+  if (!ctx) return FP_ERR_ARG;
+  if (!ctx->has_state) return fail(ctx, FP_ERR_STATE, "fp_get_connections before a state was set");
+  if (ctx->K > CONN_KMAX) return fail(ctx, FP_ERR_UNSUPPORTED, "part connections support K <= %d", CONN_KMAX);
+  if (!ctx->src_on_pins)
+    return fail(ctx, FP_ERR_UNSUPPORTED, "part connections need net sources that are pins of their nets");
+  FP_HIP_OK(hipSetDevice(ctx->device));
+  const int K = ctx->K;
+  FP_TRY(ensure_conn_scratch(ctx, 1));
+  auto& c = ctx->conn;
+  fill_async<int>(ctx, c.pc, (size_t)K * (size_t)K, 0);
+  const int ngrid = std::max(1, std::min((ctx->noe + 255) / 256, 4096));
+  k_conn_init<<<ngrid, 256, 0, ctx->stream>>>(ctx->net_ptr, ctx->ent, ctx->src, ctx->part, ctx->noe, ctx->pmask, K,
+                                               c.pc);
+  launch_bits(ctx);
+  FP_HIP_OK(hipGetLastError());
+  if (pc) FP_TRY(d2h(ctx, pc, c.pc, (size_t)K * (size_t)K));
+  if (msg) FP_TRY(d2h(ctx, msg, c.msg, (size_t)K));
+  FP_HIP_OK(hipStreamSynchronize(ctx->stream));
+  return FP_OK;
+}

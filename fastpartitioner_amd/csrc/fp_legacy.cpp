@@ -1,29 +1,33 @@
-// fp_legacy.cpp -- the reference's own device entry point, partition_gpu_wrapper,
-// implemented over the C-ABI of libfastpart.so, so that the reference's
-// partition_gpu() links against this library unchanged.
+// fp_legacy.cpp -- the reference's own device entry points (partition_gpu_wrapper
+// and TM's two variants) implemented over the C-ABI of libfastpart.so, so that
+// the reference's partition_gpu() links against this library unchanged.
 //
 // The reference declares one partition_gpu_wrapper per objective, with C++
 // linkage and a signature of its own:
 //   TV    TV/partition.cpp:6-26    (defined at TV/gpu_partition.cu:332)
 //   CN    CN/partition.cpp:7-27    (defined at CN/gpu_partition.cu:263-283)
-// MSRV declares the TV signature for another objective (MSRV/partition.cpp:4-24),
-// so each objective gets a library of its own: this file is compiled once per
-// objective with -DFP_LEGACY_TV or -DFP_LEGACY_CN.
+//   MSV   MGV/partition.cpp:4-25   (its .cu definition disagrees; the declaration is what links)
+//   MSRV  MSRV/partition.cpp:4-24  (the TV signature)
+//   TM    TM/partition.cpp:3-7     (partition_gpu_wrapper, partition_gpu_optimized_wrapper,
+//                                   partition_gpu_threads_wrapper)
+// MSRV declares the TV signature for another objective, so each objective gets
+// a library of its own: this file is compiled once per objective with
+// -DFP_LEGACY_<TV|CN|MSV|MSRV|TM>.
 //
 // Contract kept from the reference:
 //   * all arrays are the caller's host arrays, borrowed for the call; refined
 //     results come back in the same arrays: part_vec, ep_ends, ep_parts,
-//     ep_cnt (slots past a net's lambda read part -1, count 0) and part_sizes
+//     ep_cnt (slots past a net's lambda read part -1, count 0), part_sizes and
+//     the objective's bookkeeping (send_volumes; part_connections, part_messages)
 //   * errors print a message and exit with the status code, as the
 //     reference's gpuAssert does (TV/gpu_partition.cu:9-17)
 // Choices the reference leaves open (its GPU kernels race, so no schedule of
-// theirs can be reproduced):
-//   * TV runs the round schedule of partition_par (TV/partition.cpp:669-896)
-//     with rounds of FASTPART_ROUND vertices; the default 8 x OpenMP threads
-//     is the round partition_par itself uses, so `-g` gives the partition that
-//     partition_par gives at that thread count without its data race
-//   * CN runs the lambda = 2 push of partition_par (CN/partition.cpp:436-836)
-//   * FASTPART_DEVICE selects the GPU (default 0)
+// theirs can be reproduced): each objective runs the round schedule of its
+// partition_par without the data race, with rounds of FASTPART_ROUND vertices.
+// The default is the round partition_par itself uses at the process's OpenMP
+// thread count (TV 8, MSV/MSRV 256 per thread; TM a whole pass), so `-g` gives
+// the partition partition_par gives at that thread count. FASTPART_DEVICE
+// selects the GPU (default 0).
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -51,63 +55,88 @@ long env_long(const char* name, long dflt) {
   return (end && *end == 0) ? v : dflt;
 }
 
-// Uploads the hypergraph and the caller's state, runs `passes` passes of
-// `metric`, and copies the state back into the caller's arrays.
-// net_ptr/net_pins may be null: the net->pins view is then the transpose of
-// the vertex->nets view (pins ascending per net).
-void refine_in_place(fp_metric metric, const unsigned* vtx_ptr, const unsigned* vtx_nets, const unsigned* net_ptr_in,
-                     const unsigned* net_pins_in, int nov, int noe, long long pins, const int* src, int K,
-                     const int* ordering, int* part_vec, int* ep_ends, int* ep_parts, int* ep_cnt, int* part_sizes,
-                     int passes, double C) {
+struct Legacy {
+  fp_metric metric;
+  const unsigned *vtx_ptr, *vtx_nets;
+  const unsigned *net_ptr, *net_pins;  // null: transpose of the vertex->nets view
+  int nov, noe;
+  long long pins;
+  const int* src;
+  int K;
+  const int* ordering;  // null: identity
+  int *part_vec, *ep_ends, *ep_parts, *ep_cnt, *part_sizes;
+  int passes;
+  double C;
+  long default_round;
+};
+
+// Uploads the hypergraph and the caller's state, runs the passes and copies
+// the state back into the caller's arrays. Returns the live context (for the
+// objective's extra outputs); the caller destroys it.
+fp_ctx* refine_in_place(const Legacy& g) {
   std::vector<unsigned> net_ptr, net_pins;
-  if (!net_ptr_in || !net_pins_in) {
-    net_ptr.resize((size_t)noe + 1);
-    net_pins.resize((size_t)(pins > 0 ? pins : 1));
-    int st = fp_csr_transpose(noe, nov, vtx_ptr, vtx_nets, net_ptr.data(), net_pins.data());
+  const unsigned *np = g.net_ptr, *pp = g.net_pins;
+  if (!np || !pp) {
+    net_ptr.resize((size_t)g.noe + 1);
+    net_pins.resize((size_t)(g.pins > 0 ? g.pins : 1));
+    int st = fp_csr_transpose(g.noe, g.nov, g.vtx_ptr, g.vtx_nets, net_ptr.data(), net_pins.data());
     if (st) die(nullptr, st, "transpose of the vertex->nets view");
-    net_ptr_in = net_ptr.data();
-    net_pins_in = net_pins.data();
+    np = net_ptr.data();
+    pp = net_pins.data();
   }
   fp_ctx* ctx = nullptr;
   int st = fp_ctx_create(&ctx, (int)env_long("FASTPART_DEVICE", 0));
   if (st) die(ctx, st, "fp_ctx_create");
-  st = fp_upload(ctx, nov, noe, net_ptr_in, net_pins_in, vtx_ptr, vtx_nets, src, nullptr, ordering);
+  st = fp_upload(ctx, g.nov, g.noe, np, pp, g.vtx_ptr, g.vtx_nets, g.src, nullptr, g.ordering);
   if (st) die(ctx, st, "fp_upload");
-  std::vector<long long> sizes((size_t)K);
-  for (int k = 0; k < K; k++) sizes[(size_t)k] = part_sizes[k];
-  st = fp_set_state(ctx, K, part_vec, ep_ends, ep_parts, ep_cnt, sizes.data());
+  std::vector<long long> sizes((size_t)g.K);
+  for (int k = 0; k < g.K; k++) sizes[(size_t)k] = g.part_sizes[k];
+  st = fp_set_state(ctx, g.K, g.part_vec, g.ep_ends, g.ep_parts, g.ep_cnt, sizes.data());
   if (st) die(ctx, st, "fp_set_state");
 
-  fp_schedule sched{};  // read by TV only
+  fp_schedule sched{};  // read by every objective but CN
   sched.kind = FP_SCHED_REF;
-  sched.round = (int)env_long("FASTPART_ROUND", 8L * omp_get_max_threads());
-  sched.ldg_C = C;
-  sched.max_pw = fp_ref_max_pw(nov, K);
-  sched.min_pw = fp_ref_min_pw(nov, K);
+  sched.round = (int)env_long("FASTPART_ROUND", g.default_round);
+  sched.ldg_C = g.C;
+  sched.max_pw = fp_ref_max_pw(g.nov, g.K);
+  sched.min_pw = fp_ref_min_pw(g.nov, g.K);
   sched.cap = 0;
   fp_refine_stats stats{};
-  st = fp_refine(ctx, metric, passes, &sched, &stats);
+  st = fp_refine(ctx, g.metric, g.passes, &sched, &stats);
   if (st) die(ctx, st, "fp_refine");
-  st = fp_get_state(ctx, part_vec, ep_ends, ep_parts, ep_cnt, sizes.data());
+  st = fp_get_state(ctx, g.part_vec, g.ep_ends, g.ep_parts, g.ep_cnt, sizes.data());
   if (st) die(ctx, st, "fp_get_state");
-  for (int k = 0; k < K; k++) part_sizes[k] = (int)sizes[(size_t)k];
-  std::printf("fastpart: %d passes on the GPU, %lld moves, %.3f ms on the device\n", passes, stats.moves,
+  for (int k = 0; k < g.K; k++) g.part_sizes[k] = (int)sizes[(size_t)k];
+  std::printf("fastpart: %d passes on the GPU, %lld moves, %.3f ms on the device\n", g.passes, stats.moves,
               stats.device_ms);
-  fp_ctx_destroy(ctx);
+  return ctx;
 }
+
+// (nov / K) * 1.05: the capacity every reference directory hard-codes (e.g.
+// MGV/partition.cpp:544-545), for the entry points that are not handed C
+double ref_C(int nov, int K) { return fp_ref_capacity(nov, K, 0.05); }
+
+long threads() { return (long)omp_get_max_threads(); }
 
 }  // namespace
 
-#if defined(FP_LEGACY_TV)
-// TV/partition.cpp:6-26. ep_starts (noe+1) are the net offsets; row_ptr/col_ind
-// the vertex->nets view; marker, active_parts, scores and edge_wghts are the
-// reference's scratch and net weights, which the engine does not need.
+#if defined(FP_LEGACY_TV) || defined(FP_LEGACY_MSRV)
+// TV/partition.cpp:6-26 and MSRV/partition.cpp:4-24. ep_starts (noe+1) are the
+// net offsets; row_ptr/col_ind the vertex->nets view; marker, active_parts,
+// scores and edge_wghts are the reference's scratch and net weights, which
+// the engine does not need.
 void partition_gpu_wrapper(unsigned int* row_ptr, unsigned int* col_ind, unsigned int* /*net_appearence*/, int nov,
                            int noe, long long pins, int* /*edge_wghts*/, int* src_nodes, int K, int* ordering,
                            int* /*marker*/, int* /*active_parts*/, int* part_sizes, int* /*ep_starts*/, int* ep_ends,
                            int* ep_parts, int* part_vec, int* ep_cnt, double* /*scores*/, int REFI_ITER, double C) {
-  refine_in_place(FP_METRIC_TV, row_ptr, col_ind, nullptr, nullptr, nov, noe, pins, src_nodes, K, ordering, part_vec,
-                  ep_ends, ep_parts, ep_cnt, part_sizes, REFI_ITER, C);
+#if defined(FP_LEGACY_TV)
+  Legacy g{FP_METRIC_TV, row_ptr, col_ind, nullptr, nullptr, nov, noe, pins, src_nodes, K, ordering, part_vec,
+           ep_ends, ep_parts, ep_cnt, part_sizes, REFI_ITER, C, 8 * threads()};
+#else
+  Legacy g{FP_METRIC_MSRV, row_ptr, col_ind, nullptr, nullptr, nov, noe, pins, src_nodes, K, ordering, part_vec,
+           ep_ends, ep_parts, ep_cnt, part_sizes, REFI_ITER, C, 256 * threads()};
+#endif
+  fp_ctx_destroy(refine_in_place(g));
 }
 #elif defined(FP_LEGACY_CN)
 // CN/partition.cpp:7-27. row_ptr_inv/col_ind_inv are the net->pins view in
@@ -117,9 +146,60 @@ void partition_gpu_wrapper(unsigned int* row_ptr, unsigned int* col_ind, unsigne
                            int /*REFI_OPT*/, int REFI_ITER, unsigned int* row_ptr_inv, unsigned int* col_ind_inv,
                            int* part_sizes, int* /*ep_starts*/, int* ep_parts, int* ep_ends, int* ep_cnt,
                            int* part_vec) {
-  refine_in_place(FP_METRIC_CN, row_ptr, col_ind, row_ptr_inv, col_ind_inv, nov, noe, pins, src_nodes, K, ordering,
-                  part_vec, ep_ends, ep_parts, ep_cnt, part_sizes, REFI_ITER, 0.0);
+  Legacy g{FP_METRIC_CN, row_ptr, col_ind, row_ptr_inv, col_ind_inv, nov, noe, pins, src_nodes, K, ordering, part_vec,
+           ep_ends, ep_parts, ep_cnt, part_sizes, REFI_ITER, ref_C(nov, K), 1};
+  fp_ctx_destroy(refine_in_place(g));
 }
+#elif defined(FP_LEGACY_MSV)
+// MGV/partition.cpp:4-25. The refinement visits vertex ids in order
+// (MGV/partition.cpp:755-757), so the ordering is not uploaded.
+// send_volumes[K] comes back as the refined send volumes.
+void partition_gpu_wrapper(unsigned int* row_ptr, unsigned int* col_ind, unsigned int* /*net_appearence*/, int nov,
+                           int noe, long long pins, int* /*edge_wgths*/, int* src_nodes, int K, int* /*ordering*/,
+                           int /*REFI_OPT*/, int REFI_ITER, int* send_volumes, double* /*scores*/, int* /*marker*/,
+                           int* /*active_parts*/, int* /*ep_starts*/, int* ep_parts, int* ep_ends, int* ep_cnt,
+                           int* part_sizes, int* part_vec) {
+  Legacy g{FP_METRIC_MSV, row_ptr, col_ind, nullptr, nullptr, nov, noe, pins, src_nodes, K, nullptr, part_vec,
+           ep_ends, ep_parts, ep_cnt, part_sizes, REFI_ITER, ref_C(nov, K), 256 * threads()};
+  fp_ctx* ctx = refine_in_place(g);
+  std::vector<long long> sv((size_t)K);
+  fp_metrics m{};
+  int st = fp_eval(ctx, &m, nullptr, nullptr, nullptr, nullptr, nullptr, sv.data());
+  if (st) die(ctx, st, "fp_eval");
+  for (int k = 0; k < K; k++) send_volumes[k] = (int)sv[(size_t)k];
+  fp_ctx_destroy(ctx);
+}
+#elif defined(FP_LEGACY_TM)
+namespace {
+// TM/partition.cpp:3-7. The refinement visits vertex ids in order
+// (TM/partition.cpp:976); part_connections[K][K] and part_messages[K] come
+// back as the refined connections.
+void tm_wrapper(unsigned int* row_ptr, unsigned int* col_ind, int nov, int noe, long long pins, int* src_nodes, int K,
+                int REFI_ITER, int* part_vec, int* ep_parts, int* ep_ends, int* ep_cnt, int** part_connections,
+                int* part_sizes, int* part_messages) {
+  Legacy g{FP_METRIC_TM, row_ptr, col_ind, nullptr, nullptr, nov, noe, pins, src_nodes, K, nullptr, part_vec,
+           ep_ends, ep_parts, ep_cnt, part_sizes, REFI_ITER, ref_C(nov, K), (long)nov};
+  fp_ctx* ctx = refine_in_place(g);
+  std::vector<int> pc((size_t)K * (size_t)K);
+  int st = fp_get_connections(ctx, pc.data(), part_messages);
+  if (st) die(ctx, st, "fp_get_connections");
+  for (int a = 0; a < K; a++)
+    for (int b = 0; b < K; b++) part_connections[a][b] = pc[(size_t)a * (size_t)K + (size_t)b];
+  fp_ctx_destroy(ctx);
+}
+}  // namespace
+
+#define FP_TM_ENTRY(name)                                                                                          \
+  void name(unsigned int* row_ptr, unsigned int* col_ind, unsigned int* /*net_appearence*/, int nov, int noe,      \
+            long long pins, int* /*edge_wgths*/, int* src_nodes, int K, int* /*ordering*/, int /*REFI_OPT*/,       \
+            int REFI_ITER, int* part_vec, int* /*ep_starts*/, int* ep_parts, int* ep_ends, int* ep_cnt,            \
+            int** part_connections, int* part_sizes, int* part_messages) {                                         \
+    tm_wrapper(row_ptr, col_ind, nov, noe, pins, src_nodes, K, REFI_ITER, part_vec, ep_parts, ep_ends, ep_cnt,     \
+               part_connections, part_sizes, part_messages);                                                      \
+  }
+FP_TM_ENTRY(partition_gpu_wrapper)
+FP_TM_ENTRY(partition_gpu_optimized_wrapper)
+FP_TM_ENTRY(partition_gpu_threads_wrapper)
 #else
-#error "compile with -DFP_LEGACY_TV or -DFP_LEGACY_CN"
+#error "compile with -DFP_LEGACY_TV, -DFP_LEGACY_CN, -DFP_LEGACY_MSV, -DFP_LEGACY_MSRV or -DFP_LEGACY_TM"
 #endif

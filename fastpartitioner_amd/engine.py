@@ -56,6 +56,9 @@ def declare(lib) -> None:
     lib.fp_refine.argtypes = [P, c.c_int, c.c_int, c.POINTER(Schedule), c.POINTER(RefineStats)]
     lib.fp_eval_moves.argtypes = [P, c.c_int, c.POINTER(Schedule), _I32P, _I32P]
     lib.fp_eval.argtypes = [P, c.POINTER(Metrics), _I32P, _I64P, _I64P, _I64P, _I64P, _I64P]
+    lib.fp_get_connections.argtypes = [P, _I32P, _I32P]
+    lib.fp_status_string.restype = c.c_char_p
+    lib.fp_status_string.argtypes = [c.c_int]
     lib.fp_set_kernel_timing.argtypes = [P, c.c_int]
     lib.fp_last_kernel_timing.argtypes = [P, c.POINTER(c.c_longlong), c.POINTER(c.c_double)]
 
@@ -173,6 +176,14 @@ class Engine:
             res["lam"] = lam
             res.update(vec)
         return res
+
+    def connections(self):
+        """(pc[K, K], msg[K]) of the resident state: nets with source in part a and pins in part b."""
+        K = self.K
+        pc = np.zeros(K * K, np.int32)
+        msg = np.zeros(K, np.int32)
+        self._check(self.lib.fp_get_connections(self.ctx, _p(pc, _I32P), _p(msg, _I32P)), "fp_get_connections")
+        return pc.reshape(K, K), msg
 
     def set_kernel_timing(self, on: bool):
         self._check(self.lib.fp_set_kernel_timing(self.ctx, int(on)), "fp_set_kernel_timing")

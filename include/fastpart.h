@@ -138,9 +138,23 @@ fp_status fp_get_state(fp_ctx* ctx, int* part, int* ep_ends, int* ep_parts, int*
 /* ---- refinement --------------------------------------------------------- */
 /* `passes` refinement passes of `metric` under `sched` on the resident state.
  * TV: vertex-centric LDG refinement of TV/partition.cpp:307-440 (decision and
- * gain test), rounds per sched.
+ * gain test), rounds per sched (FP_SCHED_REF or FP_SCHED_BAL).
  * CN: the lambda = 2 push of CN/partition.cpp:436-836; sched is not used and
- * may be NULL. stats may be NULL. */
+ * may be NULL.
+ * MSV / MSRV: partition_par of MGV/partition.cpp:706-1040 / MSRV/partition.cpp:
+ * 769-1110, FP_SCHED_REF only: rounds of sched->round positions (MSV: vertex
+ * ids, so the context's ordering must be the identity; MSRV: the ordering),
+ * band-free LDG with capacity sched->ldg_C, the max-volume guard, and the
+ * round cut after the first move that lets a new part into the max set.
+ * round = 256 * threads reproduces the reference.
+ * TM / MSM: partition_par of TM/partition.cpp:947-1265 / MSM/partition.cpp:
+ * 1010-1400, FP_SCHED_REF only, vertex ids in order (identity ordering).
+ * TM: capacity sched->ldg_C, destinations at or above 1.02 * ldg_C skipped at
+ * apply; round >= nov reproduces the reference (one frozen round per pass).
+ * MSM: the max-message guard and round cut; round = 128 * threads reproduces
+ * the reference. The volume and message objectives need K <= 4096 and net
+ * sources that are pins of their nets.
+ * stats may be NULL. */
 fp_status fp_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule* sched, fp_refine_stats* stats);
 
 /* One frozen-state evaluation of every vertex without applying anything:
@@ -158,6 +172,13 @@ fp_status fp_eval_moves(fp_ctx* ctx, fp_metric metric, const fp_schedule* sched,
  * sv_msv (MGV_Scratch send volumes). */
 fp_status fp_eval(fp_ctx* ctx, fp_metrics* out, int* lam, long long* sv, long long* rv, long long* sm,
                   long long* rm, long long* sv_msv);
+
+/* The part-connection matrix of the resident state (TM/partition.cpp:190-235,
+ * the bookkeeping of TM and MSM): pc[a*K + b] = number of nets whose source is
+ * in part a and that have pins in part b != a (pc[a*K + a] = 0); msg[a] = the
+ * number of b with pc[a*K + b] > 0, the messages part a sends. Either pointer
+ * may be NULL. Needs K <= 4096 and net sources that are pins of their nets. */
+fp_status fp_get_connections(fp_ctx* ctx, int* pc, int* msg);
 
 /* Event timing of the dominant refinement kernel in the last fp_refine:
  * launches, total device ms of those launches (HIP events around each launch,

@@ -31,6 +31,7 @@ def _run(metric, hg, K, passes, R):
         stats = eng.refine(metric, passes, schedule(SCHED_REF, R, C, max_pw, min_pw))
         got = eng.get_state()
         m = eng.metrics()
+        pc, msg = eng.connections()
     if metric == "TM":
         moves = O.tm_refine_par(hg, st, C, passes, R)
     else:
@@ -38,6 +39,10 @@ def _run(metric, hg, K, passes, R):
     _same_state(got, st)
     assert stats.moves == moves
     assert m[metric] == O.metrics(hg, K, st.part)[metric]
+    # the connection matrix the oracle kept move by move (its MSM diagonal is -1)
+    want = st.pc.reshape(K, K).copy()
+    np.fill_diagonal(want, 0)
+    assert np.array_equal(pc, want) and np.array_equal(msg, st.msg)
     return moves
 
 

@@ -63,6 +63,31 @@ def test_reference_partition_gpu_cn():
     assert f"After Refinement CN : {m['CN']}\n" in r["stdout"]
 
 
+@pytest.mark.parametrize("R", [256, 1024])
+def test_reference_partition_gpu_msv(R):
+    hg = H.colnet(3000, H.DEG_UNIFORM, 2, 9, seed=5)
+    K, passes = 12, 4
+    r = _dropin("mgv", hg, K, passes, {"FASTPART_ROUND": str(R)})
+    C, _, _ = H.ref_constants(hg.nov, K)
+    st = O.msv_ldg(hg, K, C)   # the oracle's stream reproduces the reference's initial state
+    assert np.array_equal(st.part, r["init_part"]) and np.array_equal(st.ep_parts, r["init_parts"])
+    O.msv_refine_par(hg, st, C, passes, R)
+    _assert_final(r, st)
+    assert f"After Refinement MGV : {O.metrics(hg, K, st.part)['MSV']}\n" in r["stdout"]
+
+
+def test_reference_partition_gpu_tm():
+    hg = H.colnet(1500, H.DEG_UNIFORM, 1, 3, seed=3)
+    K, passes = 64, 4
+    r = _dropin("tm", hg, K, passes)
+    C, _, _ = H.ref_constants(hg.nov, K)
+    st = O.tm_init(hg, K, C)
+    assert np.array_equal(st.part, r["init_part"]) and np.array_equal(st.ep_parts, r["init_parts"])
+    assert O.tm_refine_par(hg, st, C, passes, hg.nov) > 0
+    _assert_final(r, st)
+    assert f"After Refinement TM : {O.metrics(hg, K, st.part)['TM']}\n" in r["stdout"]
+
+
 def _recount(hg, st):
     """ep_cnt of the lists in st.ep_parts: pins of each net per listed part."""
     st.ep_cnt[:] = 0

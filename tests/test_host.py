@@ -132,27 +132,35 @@ def test_engine_reports_no_device_without_gpu():
 # The reference's partition_gpu_wrapper declarations, mangled (C++ linkage):
 # TV/partition.cpp:6-26 and CN/partition.cpp:7-27.
 LEGACY_SYMBOLS = {
-    "tv": "_Z21partition_gpu_wrapperPjS_S_iixPiS0_iS0_S0_S0_S0_S0_S0_S0_S0_S0_Pdid",
-    "cn": "_Z21partition_gpu_wrapperPjS_S_iixPiS0_iS0_iiS_S_S0_S0_S0_S0_S0_S0_",
+    "tv": ["_Z21partition_gpu_wrapperPjS_S_iixPiS0_iS0_S0_S0_S0_S0_S0_S0_S0_S0_Pdid"],
+    "cn": ["_Z21partition_gpu_wrapperPjS_S_iixPiS0_iS0_iiS_S_S0_S0_S0_S0_S0_S0_"],
+    "msv": ["_Z21partition_gpu_wrapperPjS_S_iixPiS0_iS0_iiS0_PdS0_S0_S0_S0_S0_S0_S0_S0_"],
+    "msrv": ["_Z21partition_gpu_wrapperPjS_S_iixPiS0_iS0_S0_S0_S0_S0_S0_S0_S0_S0_Pdid"],
+    "tm": ["_Z21partition_gpu_wrapperPjS_S_iixPiS0_iS0_iiS0_S0_S0_S0_S0_PS0_S0_S0_",
+           "_Z29partition_gpu_threads_wrapperPjS_S_iixPiS0_iS0_iiS0_S0_S0_S0_S0_PS0_S0_S0_",
+           "_Z31partition_gpu_optimized_wrapperPjS_S_iixPiS0_iS0_iiS0_S0_S0_S0_S0_PS0_S0_S0_"],
 }
+# the reference object each library serves (MSRV's does not compile, MSRV/partition.cpp:1341)
+LEGACY_OBJECT = {"tv": "tv", "cn": "cn", "msv": "mgv", "tm": "tm"}
 
 
 @pytest.mark.parametrize("objective", sorted(LEGACY_SYMBOLS))
 def test_legacy_library_exports_reference_entry_point(objective):
     lib = os.path.join(os.path.dirname(_native.ENGINE_LIB), f"libfastpart_legacy_{objective}.so")
     so = ctypes.CDLL(lib)
-    assert hasattr(so, LEGACY_SYMBOLS[objective])
-    # where the reference objects are built, their undefined wrapper symbol is exactly this one
-    obj = os.path.join(ROOT, "oracle", "_ref", "obj", f"{objective}_partition.o")
+    for sym in LEGACY_SYMBOLS[objective]:
+        assert hasattr(so, sym), sym
+    # where the reference objects are built, their undefined wrapper symbols are exactly these
+    obj = os.path.join(ROOT, "oracle", "_ref", "obj", f"{LEGACY_OBJECT.get(objective, '-')}_partition.o")
     if os.path.exists(obj):
         und = subprocess.run(["nm", "-u", obj], capture_output=True, text=True, check=True).stdout
-        assert [ln.split()[-1] for ln in und.splitlines() if "partition_gpu_wrapper" in ln] == \
-            [LEGACY_SYMBOLS[objective]]
+        assert sorted(ln.split()[-1] for ln in und.splitlines() if "_wrapper" in ln) == \
+            sorted(LEGACY_SYMBOLS[objective])
 
 
-@pytest.mark.parametrize("objective", sorted(LEGACY_SYMBOLS))
+@pytest.mark.parametrize("objective", sorted(LEGACY_OBJECT))
 def test_reference_partition_gpu_fails_loudly_without_gpu(objective, tmp_path):
-    exe = os.path.join(ROOT, "oracle", "_ref", f"dropin_{objective}")
+    exe = os.path.join(ROOT, "oracle", "_ref", f"dropin_{LEGACY_OBJECT[objective]}")
     if not os.path.exists(exe):
         pytest.skip("drop-in binaries not built here")
     n = ctypes.c_int(-1)
