@@ -1,7 +1,7 @@
 """In-tree build of the package's native libraries.
 
   libfastpart_host.so  g++    csrc/fp_host.cpp         (host helpers)
-  libfastpart.so       hipcc  csrc/fp_{context,tv,cn,vol,conn,metrics}.hip  (HIP engine, gfx950)
+  libfastpart.so       hipcc  csrc/fp_{context,tv,cn,vol,conn,metrics,batch}.hip  (HIP engine, gfx950)
   libfastpart_legacy_<m>.so  g++  csrc/fp_legacy.cpp  (the reference's partition_gpu_wrapper
                        for objective <m>, over libfastpart.so)
 
@@ -27,7 +27,8 @@ HIP_FLAGS = ["-O3", "-fPIC", "-shared", "-std=c++17", "-fopenmp", "-ffp-contract
              f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-ENGINE_SOURCES = ["fp_context.hip", "fp_tv.hip", "fp_cn.hip", "fp_vol.hip", "fp_conn.hip", "fp_metrics.hip"]
+ENGINE_SOURCES = ["fp_context.hip", "fp_tv.hip", "fp_cn.hip", "fp_vol.hip", "fp_conn.hip", "fp_metrics.hip",
+                  "fp_batch.hip"]
 
 
 def _run(cmd):

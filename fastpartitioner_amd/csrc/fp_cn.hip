@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(256) k_cn_pushes(const unsigned* __restrict__ 
 struct CnPrepArgs {
   const int* mv_v;
   const int* mv_net;
-  long long nmoves;
+  const unsigned long long* nmoves;  // device: the pass's push count
   const unsigned* vtx_ptr;
   const unsigned* vtx_nets;
   const int* part;
@@ -180,8 +180,8 @@ struct CnPrepArgs {
 // apply in net order, so the source of the push by net n is the new part of
 // the largest such net below n, else v's part at pass start.
 __global__ void k_cn_prep(CnPrepArgs a) {
-  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= a.nmoves) return;
+  const long long nm = (long long)*a.nmoves;
+  for (long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x; m < nm; m += (long long)gridDim.x * blockDim.x) {
   const int v = a.mv_v[m], n = a.mv_net[m];
   const int p0 = a.part[v];
   int prev = -1;
@@ -202,6 +202,18 @@ __global__ void k_cn_prep(CnPrepArgs a) {
     atomicAdd(&a.opcnt[t], 1);
   }
   atomicAdd(&a.totals[0], (unsigned long long)(ve - vb));
+  }
+}
+
+// The pass's edit lists must fit: total ops <= cap and at most CN_MAXOPS per
+// net (sorted in LDS). Otherwise flags[0] = the total needed, flags[1] = 1,
+// and the rest of the pass leaves the state unchanged.
+__global__ void k_cn_check(const unsigned long long* __restrict__ totals, long long cap, int* __restrict__ flags) {
+  const unsigned long long need = totals[0];
+  const bool ops_over = need > (unsigned long long)cap;
+  const bool net_over = totals[2] > (unsigned long long)CN_MAXOPS;
+  flags[0] = ops_over ? (int)min(need, (unsigned long long)INT_MAX) : 0;
+  flags[1] = net_over ? 1 : 0;
 }
 
 __global__ void k_cn_alloc(const int* __restrict__ touched, const int* __restrict__ tcount,
@@ -216,15 +228,18 @@ __global__ void k_cn_alloc(const int* __restrict__ touched, const int* __restric
   atomicMax(reinterpret_cast<unsigned long long*>(&totals[2]), (unsigned long long)c);
 }
 
-__global__ void k_cn_scatter(const int* __restrict__ mv_v, long long nmoves, const unsigned* __restrict__ vtx_ptr,
+__global__ void k_cn_scatter(const int* __restrict__ mv_v, const unsigned long long* __restrict__ nmoves,
+                             const int* __restrict__ flags, const unsigned* __restrict__ vtx_ptr,
                              const unsigned* __restrict__ vtx_nets, const long long* __restrict__ opoff,
                              int* __restrict__ opfill, int* __restrict__ ops) {
-  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= nmoves) return;
-  const int v = mv_v[m];
-  for (unsigned j = vtx_ptr[v]; j < vtx_ptr[v + 1]; j++) {
-    const unsigned t = vtx_nets[j];
-    ops[opoff[t] + atomicAdd(&opfill[t], 1)] = (int)m;
+  if (flags[0] | flags[1]) return;
+  const long long nm = (long long)*nmoves;
+  for (long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x; m < nm; m += (long long)gridDim.x * blockDim.x) {
+    const int v = mv_v[m];
+    for (unsigned j = vtx_ptr[v]; j < vtx_ptr[v + 1]; j++) {
+      const unsigned t = vtx_nets[j];
+      ops[opoff[t] + atomicAdd(&opfill[t], 1)] = (int)m;
+    }
   }
 }
 
@@ -243,6 +258,7 @@ struct CnApplyArgs {
   int K, pb;
   unsigned pmask;
   int* opcnt_reset;
+  const int* flags;  // overflow: only reset the op counts
 };
 
 // ascending sort of keys[0..P) in LDS by one wavefront (P a power of two)
@@ -268,8 +284,13 @@ __global__ void __launch_bounds__(WAVE) k_cn_apply(CnApplyArgs a) {
   unsigned* keys = list + a.K;
   const int lane = lane_id();
   const int nt = *a.tcount;
+  const bool skip = (a.flags[0] | a.flags[1]) != 0;
   for (int i = (int)blockIdx.x; i < nt; i += (int)gridDim.x) {
     const int t = a.touched[i];
+    if (skip) {
+      if (lane == 0) a.opcnt_reset[t] = 0;
+      continue;
+    }
     const int c = a.opcnt[t];
     const long long off = a.opoff[t];
     int P = 1;
@@ -290,26 +311,32 @@ __global__ void __launch_bounds__(WAVE) k_cn_apply(CnApplyArgs a) {
 }
 
 __global__ void k_cn_commit(const int* __restrict__ mv_v, const int* __restrict__ mv_net,
-                            const int* __restrict__ mv_from, const int* __restrict__ mv_last, long long nmoves,
+                            const int* __restrict__ mv_from, const int* __restrict__ mv_last,
+                            const unsigned long long* __restrict__ nmoves, const int* __restrict__ flags,
                             const int* __restrict__ cn_new, const int* __restrict__ vwgt, int* __restrict__ part,
-                            long long* __restrict__ sizes) {
-  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= nmoves) return;
-  const int v = mv_v[m];
-  const int to = cn_new[mv_net[m]];
-  const long long w = vwgt ? vwgt[v] : 1;
-  atomic_add_i64(&sizes[mv_from[m]], -w);
-  atomic_add_i64(&sizes[to], w);
-  if (mv_last[m]) part[v] = to;
+                            long long* __restrict__ sizes, unsigned long long* __restrict__ moves) {
+  if (flags[0] | flags[1]) return;
+  const long long nm = (long long)*nmoves;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(moves, (unsigned long long)nm);
+  for (long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x; m < nm; m += (long long)gridDim.x * blockDim.x) {
+    const int v = mv_v[m];
+    const int to = cn_new[mv_net[m]];
+    const long long w = vwgt ? vwgt[v] : 1;
+    atomic_add_i64(&sizes[mv_from[m]], -w);
+    atomic_add_i64(&sizes[to], w);
+    if (mv_last[m]) part[v] = to;
+  }
 }
 
 }  // namespace
 
-fp_status cn_refine(fp_ctx* ctx, int passes, fp_refine_stats* stats) {
+fp_status plan_cn(fp_ctx* ctx, RefinePlan& p) {
   const int noe = ctx->noe;
   const long long pins = ctx->pins;
   const int nblk = (noe + SCAN_BLOCK - 1) / SCAN_BLOCK;
   auto& s = ctx->cn;
+  const size_t lds_apply = (size_t)ctx->K * sizeof(unsigned) + (size_t)CN_MAXOPS * sizeof(unsigned);
+  if (lds_apply > (size_t)LDS_BUDGET) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d too large for the CN apply", ctx->K);
   if (!s.ready) {
     const size_t ne = (size_t)std::max(noe, 1), np = (size_t)std::max(pins, 1ll);
     FP_TRY(dalloc(ctx, &s.old_part, ne));
@@ -326,79 +353,70 @@ fp_status cn_refine(fp_ctx* ctx, int passes, fp_refine_stats* stats) {
     FP_TRY(dalloc(ctx, &s.mv_last, np));
     FP_TRY(dalloc(ctx, &s.totals, 8));
     fill_async<int>(ctx, s.opcnt, ne, 0);
+    // a pass edits the lists of a pushed vertex's nets, and a vertex is pushed
+    // by at most its degree many nets: sum of degree^2 bounds the edit list
+    const long long cap = (long long)std::max<unsigned long long>(
+        1ull, std::min<unsigned long long>(ctx->sum_deg2, 4ull * (unsigned long long)np));
+    FP_TRY(dalloc(ctx, &s.ops, (size_t)cap));
+    s.ops_cap = cap;
     s.ready = true;
   }
+  fill_async<int>(ctx, ctx->flags, 4, 0);
+  p.R = ctx->nov;
+  return FP_OK;
+}
+
+fp_status pass_cn(fp_ctx* ctx, const RefinePlan&) {
+  const int noe = ctx->noe;
+  const int nblk = (noe + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  auto& s = ctx->cn;
   const size_t lds_apply = (size_t)ctx->K * sizeof(unsigned) + (size_t)CN_MAXOPS * sizeof(unsigned);
-  if (lds_apply > (size_t)LDS_BUDGET) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d too large for the CN apply", ctx->K);
-  long long moves = 0;
-  hipEvent_t e0 = pooled_event(ctx, 0), e1 = pooled_event(ctx, 1);
-  if (!e0 || !e1) return fail(ctx, FP_ERR_HIP, "hipEventCreate failed");
-  float total_ms = 0;
-  for (int it = 0; it < passes; it++) {
-    FP_HIP_OK(hipEventRecord(e0, ctx->stream));
-    CnEvalArgs ea{ctx->net_ptr, ctx->net_pins, ctx->vtx_ptr, ctx->vtx_seg, ctx->ent, ctx->ev, ctx->part,
-                   noe, ctx->pb, ctx->pmask, s.old_part, s.new_part, s.npush};
-    const int waves_needed = (noe + WAVE - 1) / WAVE;
-    k_cn_eval<<<std::max(1, std::min((waves_needed + 3) / 4, 8192)), 256, 0, ctx->stream>>>(ea);
-    k_scan_sums<<<std::max(nblk, 1), SCAN_BLOCK, 0, ctx->stream>>>(s.npush, noe, s.sums);
-    k_scan_top<<<1, SCAN_BLOCK, 0, ctx->stream>>>(s.sums, nblk, reinterpret_cast<long long*>(s.totals + 3));
-    k_scan_apply<<<std::max(nblk, 1), SCAN_BLOCK, 0, ctx->stream>>>(s.npush, noe, s.sums, s.pbase);
-    k_cn_pushes<<<std::max(1, std::min((waves_needed + 3) / 4, 8192)), 256, 0, ctx->stream>>>(
-        ctx->net_ptr, ctx->net_pins, ctx->part, s.old_part, s.pbase, noe, s.mv_v, s.mv_net);
-    unsigned long long nm = 0;
-    FP_TRY(d2h(ctx, &nm, s.totals + 3, 1));
-    FP_HIP_OK(hipStreamSynchronize(ctx->stream));
-    const long long M = (long long)nm;
-    if (M == 0) {
-      FP_HIP_OK(hipEventRecord(e1, ctx->stream));
-      FP_HIP_OK(hipEventSynchronize(e1));
-      float ms = 0;
-      FP_HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-      total_ms += ms;
-      continue;
-    }
-    moves += M;
-    ctx->tag++;
-    const int par = (int)(ctx->tag & 1u);
-    fill_async<int>(ctx, ctx->counters + par, 1, 0);
-    fill_async<unsigned long long>(ctx, s.totals, 3, 0ull);
-    CnPrepArgs pa{s.mv_v, s.mv_net, M, ctx->vtx_ptr, ctx->vtx_nets, ctx->part, s.old_part, s.new_part, s.mv_from,
-                  s.mv_last, ctx->tag, ctx->net_stamp, ctx->touched, ctx->counters + par, s.opcnt, s.totals};
-    const int gm = (int)((M + 255) / 256);
-    k_cn_prep<<<gm, 256, 0, ctx->stream>>>(pa);
-    k_cn_alloc<<<(noe + 255) / 256, 256, 0, ctx->stream>>>(ctx->touched, ctx->counters + par, s.opcnt, s.opoff,
-                                                           s.opfill, s.totals);
-    unsigned long long tot[3] = {0, 0, 0};
-    FP_TRY(d2h(ctx, tot, s.totals, 3));
-    FP_HIP_OK(hipStreamSynchronize(ctx->stream));
-    if ((long long)tot[2] > CN_MAXOPS) {
-      ctx->has_state = false;
-      return fail(ctx, FP_ERR_UNSUPPORTED, "a net receives %llu moves in one CN pass (limit %d); state discarded",
-                  tot[2], CN_MAXOPS);
-    }
-    if ((long long)tot[0] > s.ops_cap) {
-      s.ops_cap = std::max((long long)tot[0], 2 * s.ops_cap);
-      FP_TRY(dalloc(ctx, &s.ops, (size_t)s.ops_cap));
-    }
-    k_cn_scatter<<<gm, 256, 0, ctx->stream>>>(s.mv_v, M, ctx->vtx_ptr, ctx->vtx_nets, s.opoff, s.opfill, s.ops);
-    CnApplyArgs aa{ctx->net_ptr, ctx->touched, ctx->counters + par, s.opcnt, s.opoff, s.ops, s.mv_from, s.mv_net,
-                   s.new_part, ctx->ent, ctx->ev, ctx->K, ctx->pb, ctx->pmask, s.opcnt};
-    k_cn_apply<<<8192, WAVE, lds_apply, ctx->stream>>>(aa);
-    k_cn_commit<<<gm, 256, 0, ctx->stream>>>(s.mv_v, s.mv_net, s.mv_from, s.mv_last, M, s.new_part, ctx->vwgt,
-                                              ctx->part, ctx->sizes);
-    FP_HIP_OK(hipGetLastError());
-    FP_HIP_OK(hipEventRecord(e1, ctx->stream));
-    FP_HIP_OK(hipEventSynchronize(e1));
-    float ms = 0;
-    FP_HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-    total_ms += ms;
-  }
+  const unsigned long long* nm = s.totals + 3;
+  // one round per pass: tag 1 after clearing the stamps
+  fill_async<unsigned>(ctx, ctx->net_stamp, (size_t)std::max(noe, 1), 0u);
+  fill_async<int>(ctx, ctx->counters, 4, 0);
+  fill_async<unsigned long long>(ctx, s.totals, 3, 0ull);
+  CnEvalArgs ea{ctx->net_ptr, ctx->net_pins, ctx->vtx_ptr, ctx->vtx_seg, ctx->ent, ctx->ev, ctx->part,
+                noe, ctx->pb, ctx->pmask, s.old_part, s.new_part, s.npush};
+  const int waves_needed = (noe + WAVE - 1) / WAVE;
+  k_cn_eval<<<std::max(1, std::min((waves_needed + 3) / 4, 8192)), 256, 0, ctx->stream>>>(ea);
+  k_scan_sums<<<std::max(nblk, 1), SCAN_BLOCK, 0, ctx->stream>>>(s.npush, noe, s.sums);
+  k_scan_top<<<1, SCAN_BLOCK, 0, ctx->stream>>>(s.sums, nblk, reinterpret_cast<long long*>(s.totals + 3));
+  k_scan_apply<<<std::max(nblk, 1), SCAN_BLOCK, 0, ctx->stream>>>(s.npush, noe, s.sums, s.pbase);
+  k_cn_pushes<<<std::max(1, std::min((waves_needed + 3) / 4, 8192)), 256, 0, ctx->stream>>>(
+      ctx->net_ptr, ctx->net_pins, ctx->part, s.old_part, s.pbase, noe, s.mv_v, s.mv_net);
+  const int par = 1;
+  CnPrepArgs pa{s.mv_v, s.mv_net, nm, ctx->vtx_ptr, ctx->vtx_nets, ctx->part, s.old_part, s.new_part, s.mv_from,
+                s.mv_last, 1u, ctx->net_stamp, ctx->touched, ctx->counters + par, s.opcnt, s.totals};
+  const int gm = (int)std::min<long long>((std::max(ctx->pins, 1ll) + 255) / 256, 8192);
+  k_cn_prep<<<gm, 256, 0, ctx->stream>>>(pa);
+  k_cn_alloc<<<(noe + 255) / 256, 256, 0, ctx->stream>>>(ctx->touched, ctx->counters + par, s.opcnt, s.opoff,
+                                                         s.opfill, s.totals);
+  k_cn_check<<<1, 1, 0, ctx->stream>>>(s.totals, s.ops_cap, ctx->flags);
+  k_cn_scatter<<<gm, 256, 0, ctx->stream>>>(s.mv_v, nm, ctx->flags, ctx->vtx_ptr, ctx->vtx_nets, s.opoff, s.opfill,
+                                            s.ops);
+  CnApplyArgs aa{ctx->net_ptr, ctx->touched, ctx->counters + par, s.opcnt, s.opoff, s.ops, s.mv_from, s.mv_net,
+                 s.new_part, ctx->ent, ctx->ev, ctx->K, ctx->pb, ctx->pmask, s.opcnt, ctx->flags};
+  k_cn_apply<<<8192, WAVE, lds_apply, ctx->stream>>>(aa);
+  k_cn_commit<<<gm, 256, 0, ctx->stream>>>(s.mv_v, s.mv_net, s.mv_from, s.mv_last, nm, ctx->flags, s.new_part,
+                                            ctx->vwgt, ctx->part, ctx->sizes, ctx->moves);
+  return FP_OK;
+}
+
+fp_status cn_after_pass(fp_ctx* ctx, int* redo) {
+  *redo = 0;
+  int f[2] = {0, 0};
+  FP_TRY(d2h(ctx, f, ctx->flags, 2));
   FP_HIP_OK(hipStreamSynchronize(ctx->stream));
-  if (stats) {
-    stats->moves = moves;
-    stats->rounds = passes;
-    stats->pin_visits = (long long)passes * ctx->pins;
-    stats->device_ms = total_ms;
+  if (f[1])
+    return fail(ctx, FP_ERR_UNSUPPORTED, "a net receives more than %d moves in one CN pass; that pass was not applied",
+                CN_MAXOPS);
+  if (f[0]) {
+    auto& s = ctx->cn;
+    s.ops_cap = std::max<long long>((long long)f[0], 2 * s.ops_cap);
+    FP_TRY(dalloc(ctx, &s.ops, (size_t)s.ops_cap));
+    fill_async<int>(ctx, ctx->flags, 4, 0);
+    *redo = 1;
   }
   return FP_OK;
 }

@@ -611,133 +611,123 @@ void launch_bits(fp_ctx* ctx) {
 
 }  // namespace
 
-fp_status conn_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule* sched, fp_refine_stats* stats) {
-  const bool msm = metric == FP_METRIC_MSM;
-  const int nov = ctx->nov, K = ctx->K, W = (K + 63) / 64;
+fp_status plan_conn(fp_ctx* ctx, RefinePlan& p) {
+  const bool msm = p.metric == FP_METRIC_MSM;
+  const int nov = ctx->nov, K = ctx->K;
   if (K > CONN_KMAX) return fail(ctx, FP_ERR_UNSUPPORTED, "message refinement supports K <= %d", CONN_KMAX);
   if (!ctx->src_on_pins)
     return fail(ctx, FP_ERR_UNSUPPORTED, "message refinement needs net sources that are pins of their nets");
   if (ctx->order)
     return fail(ctx, FP_ERR_UNSUPPORTED,
                 "TM/MSM refinement visits vertex ids in order (TM/partition.cpp:976); upload with the identity ordering");
-  if (sched->kind != FP_SCHED_REF)
+  if (p.sched.kind != FP_SCHED_REF)
     return fail(ctx, FP_ERR_UNSUPPORTED, "message refinement runs the reference round schedule (FP_SCHED_REF) only");
-  const int waves = conn_eval_waves(K);
+  const int waves = p.eval_waves > 0 ? std::min(p.eval_waves, conn_eval_waves(K)) : conn_eval_waves(K);
   if (waves == 0) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d needs more LDS than a workgroup has", K);
-  const size_t lds_e = (size_t)waves * ((size_t)2 * (size_t)K + 3 * (size_t)((K + 31) / 32)) * sizeof(int);
-  const size_t lds_r = (size_t)(REPLAY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
-  const int R = std::min(sched->round, nov);
-  FP_TRY(ensure_conn_scratch(ctx, R));
+  p.eval_waves = waves;
+  p.R = std::min(p.sched.round, nov);
+  FP_TRY(ensure_conn_scratch(ctx, p.R));
   if (msm) FP_TRY(ensure_incoff(ctx));
+  // the connections of the resident state, kept incrementally by the passes
   auto& c = ctx->conn;
-  fill_async<int>(ctx, ctx->counters, 4, 0);
-  fill_async<unsigned long long>(ctx, ctx->moves, 1, 0ull);
   fill_async<int>(ctx, c.pc, (size_t)K * (size_t)K, 0);
-  hipEvent_t e0 = pooled_event(ctx, 0), e1 = pooled_event(ctx, 1);
-  if (!e0 || !e1) return fail(ctx, FP_ERR_HIP, "hipEventCreate failed");
-  FP_HIP_OK(hipEventRecord(e0, ctx->stream));
   const int ngrid = std::max(1, std::min((ctx->noe + 255) / 256, 4096));
   k_conn_init<<<ngrid, 256, 0, ctx->stream>>>(ctx->net_ptr, ctx->ent, ctx->src, ctx->part, ctx->noe, ctx->pmask, K,
                                                c.pc);
   launch_bits(ctx);
-  const double thr = sched->ldg_C * (1.02);
-  long long rounds = 0;
-  for (int it = 0; it < passes; it++) {
-    if (msm) k_round_maxset<<<1, 1024, 0, ctx->stream>>>(c.msg, K, c.mstat);
-    for (int r0 = 0; r0 < nov; r0 += R) {
-      const int nslots = std::min(R, nov - r0);
-      ctx->tag++;
-      const int par = (int)(ctx->tag & 1u);
-      ConnEvalArgs a{};
-      a.vtx_ptr = ctx->vtx_ptr;
-      a.vtx_nets = ctx->vtx_nets;
-      a.vtx_seg = ctx->vtx_seg;
-      a.ev = ctx->ev;
-      a.src = ctx->src;
-      a.part = ctx->part;
-      a.sizes = ctx->sizes;
-      a.pc = c.pc;
-      a.B = c.B;
-      a.BT = c.BT;
-      a.msg = c.msg;
-      a.mstat = c.mstat;
-      a.K = K;
-      a.W = W;
-      a.msm = msm ? 1 : 0;
-      a.r0 = r0;
-      a.nslots = nslots;
-      a.C = sched->ldg_C;
-      a.slot_own = ctx->slot_own;
-      a.slot_dec = ctx->slot_dec;
-      a.tag = ctx->tag;
-      a.net_stamp = ctx->net_stamp;
-      a.touched = ctx->touched;
-      a.tcount = ctx->counters + par;
-      a.tcount_next = ctx->counters + (par ^ 1);
-      const int egrid = std::max(1, std::min((nslots + waves - 1) / waves, 8192));
-      k_conn_eval<<<egrid, waves * WAVE, lds_e, ctx->stream>>>(a);
-      ConnApplyArgs ap{};
-      ap.net_ptr = ctx->net_ptr;
-      ap.apply_pins = ctx->apply_pins;
-      ap.slot_dec = ctx->slot_dec;
-      ap.slot_own = ctx->slot_own;
-      ap.slot_mi = c.slot_mi;
-      ap.mv = c.mv;
-      ap.touched = ctx->touched;
-      ap.tcount = ctx->counters + par;
-      ap.src = ctx->src;
-      ap.vtx_ptr = ctx->vtx_ptr;
-      ap.vtx_nets = ctx->vtx_nets;
-      ap.incoff = c.incoff;
-      ap.mstat = c.mstat;
-      ap.vwgt = ctx->vwgt;
-      ap.ent = ctx->ent;
-      ap.ev = ctx->ev;
-      ap.part = ctx->part;
-      ap.sizes = ctx->sizes;
-      ap.pc = c.pc;
-      ap.evbuf = c.evbuf;
-      ap.moves = ctx->moves;
-      ap.K = K;
-      ap.pb = ctx->pb;
-      ap.pmask = ctx->pmask;
-      ap.r0 = r0;
-      ap.r1 = r0 + nslots;
-      ap.net_blocks = REPLAY_NET_BLOCKS;
-      if (!msm) {
-        k_tm_admit<<<1, WAVE, (size_t)K * sizeof(long long), ctx->stream>>>(
-            ctx->slot_dec, ctx->slot_own, ctx->vwgt, r0, nslots, K, thr, ctx->sizes, ctx->part, ctx->moves);
-        ap.mode = 0;
-        k_conn_apply<<<REPLAY_NET_BLOCKS, REPLAY_BLOCK, lds_r, ctx->stream>>>(ap);
-      } else {
-        k_round_compact<<<1, 1024, 0, ctx->stream>>>(ctx->slot_dec, ctx->slot_own, nullptr, ctx->vtx_ptr, r0, nslots,
-                                                      c.slot_mi, c.mv, c.mstat);
-        ap.mode = 1;
-        k_conn_apply<<<REPLAY_NET_BLOCKS, REPLAY_BLOCK, lds_r, ctx->stream>>>(ap);
-        k_msm_scan<<<1, WAVE, (size_t)3 * (size_t)K * sizeof(int), ctx->stream>>>(c.pc, c.msg, K, c.mv, c.evbuf,
-                                                                                   c.incoff, ctx->vtx_ptr, c.mstat);
-        ap.mode = 2;
-        k_conn_apply<<<REPLAY_NET_BLOCKS + (nslots + REPLAY_BLOCK - 1) / REPLAY_BLOCK, REPLAY_BLOCK, lds_r,
-                       ctx->stream>>>(ap);
-      }
-      launch_bits(ctx);
-      rounds++;
-    }
-  }
   FP_HIP_OK(hipGetLastError());
-  FP_HIP_OK(hipEventRecord(e1, ctx->stream));
-  unsigned long long mv = 0;
-  FP_TRY(d2h(ctx, &mv, ctx->moves, 1));
-  FP_HIP_OK(hipStreamSynchronize(ctx->stream));
-  float ms = 0;
-  FP_HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-  ctx->timed_launches = 0;
-  ctx->timed_ms = 0;
-  if (stats) {
-    stats->moves = (long long)mv;
-    stats->rounds = rounds;
-    stats->pin_visits = (long long)passes * ctx->pins;
-    stats->device_ms = ms;
+  return FP_OK;
+}
+
+fp_status pass_conn(fp_ctx* ctx, const RefinePlan& p) {
+  const bool msm = p.metric == FP_METRIC_MSM;
+  const int nov = ctx->nov, K = ctx->K, W = (K + 63) / 64, R = p.R, waves = p.eval_waves;
+  const size_t lds_e = (size_t)waves * ((size_t)2 * (size_t)K + 3 * (size_t)((K + 31) / 32)) * sizeof(int);
+  const size_t lds_r = (size_t)(REPLAY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
+  auto& c = ctx->conn;
+  const double thr = p.sched.ldg_C * (1.02);
+  fill_async<unsigned>(ctx, ctx->net_stamp, (size_t)std::max(ctx->noe, 1), 0u);
+  fill_async<int>(ctx, ctx->counters, 4, 0);
+  if (msm) k_round_maxset<<<1, 1024, 0, ctx->stream>>>(c.msg, K, c.mstat);
+  unsigned tag = 0;
+  for (int r0 = 0; r0 < nov; r0 += R) {
+    const int nslots = std::min(R, nov - r0);
+    tag++;
+    const int par = (int)(tag & 1u);
+    ConnEvalArgs a{};
+    a.vtx_ptr = ctx->vtx_ptr;
+    a.vtx_nets = ctx->vtx_nets;
+    a.vtx_seg = ctx->vtx_seg;
+    a.ev = ctx->ev;
+    a.src = ctx->src;
+    a.part = ctx->part;
+    a.sizes = ctx->sizes;
+    a.pc = c.pc;
+    a.B = c.B;
+    a.BT = c.BT;
+    a.msg = c.msg;
+    a.mstat = c.mstat;
+    a.K = K;
+    a.W = W;
+    a.msm = msm ? 1 : 0;
+    a.r0 = r0;
+    a.nslots = nslots;
+    a.C = p.sched.ldg_C;
+    a.slot_own = ctx->slot_own;
+    a.slot_dec = ctx->slot_dec;
+    a.tag = tag;
+    a.net_stamp = ctx->net_stamp;
+    a.touched = ctx->touched;
+    a.tcount = ctx->counters + par;
+    a.tcount_next = ctx->counters + (par ^ 1);
+    int egrid = std::max(1, std::min((nslots + waves - 1) / waves, 8192));
+    if (p.eval_grid > 0) egrid = std::min(egrid, p.eval_grid);
+    k_conn_eval<<<egrid, waves * WAVE, lds_e, ctx->stream>>>(a);
+    ConnApplyArgs ap{};
+    ap.net_ptr = ctx->net_ptr;
+    ap.apply_pins = ctx->apply_pins;
+    ap.slot_dec = ctx->slot_dec;
+    ap.slot_own = ctx->slot_own;
+    ap.slot_mi = c.slot_mi;
+    ap.mv = c.mv;
+    ap.touched = ctx->touched;
+    ap.tcount = ctx->counters + par;
+    ap.src = ctx->src;
+    ap.vtx_ptr = ctx->vtx_ptr;
+    ap.vtx_nets = ctx->vtx_nets;
+    ap.incoff = c.incoff;
+    ap.mstat = c.mstat;
+    ap.vwgt = ctx->vwgt;
+    ap.ent = ctx->ent;
+    ap.ev = ctx->ev;
+    ap.part = ctx->part;
+    ap.sizes = ctx->sizes;
+    ap.pc = c.pc;
+    ap.evbuf = c.evbuf;
+    ap.moves = ctx->moves;
+    ap.K = K;
+    ap.pb = ctx->pb;
+    ap.pmask = ctx->pmask;
+    ap.r0 = r0;
+    ap.r1 = r0 + nslots;
+    ap.net_blocks = REPLAY_NET_BLOCKS;
+    if (!msm) {
+      k_tm_admit<<<1, WAVE, (size_t)K * sizeof(long long), ctx->stream>>>(
+          ctx->slot_dec, ctx->slot_own, ctx->vwgt, r0, nslots, K, thr, ctx->sizes, ctx->part, ctx->moves);
+      ap.mode = 0;
+      k_conn_apply<<<REPLAY_NET_BLOCKS, REPLAY_BLOCK, lds_r, ctx->stream>>>(ap);
+    } else {
+      k_round_compact<<<1, 1024, 0, ctx->stream>>>(ctx->slot_dec, ctx->slot_own, nullptr, ctx->vtx_ptr, r0, nslots,
+                                                    c.slot_mi, c.mv, c.mstat);
+      ap.mode = 1;
+      k_conn_apply<<<REPLAY_NET_BLOCKS, REPLAY_BLOCK, lds_r, ctx->stream>>>(ap);
+      k_msm_scan<<<1, WAVE, (size_t)3 * (size_t)K * sizeof(int), ctx->stream>>>(c.pc, c.msg, K, c.mv, c.evbuf,
+                                                                                 c.incoff, ctx->vtx_ptr, c.mstat);
+      ap.mode = 2;
+      k_conn_apply<<<REPLAY_NET_BLOCKS + (nslots + REPLAY_BLOCK - 1) / REPLAY_BLOCK, REPLAY_BLOCK, lds_r,
+                     ctx->stream>>>(ap);
+    }
+    launch_bits(ctx);
   }
   return FP_OK;
 }

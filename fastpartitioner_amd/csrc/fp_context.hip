@@ -260,6 +260,11 @@ fp_status fp_upload(fp_ctx* ctx, int nov, int noe, const unsigned* net_ptr, cons
   unsigned max_vdeg = 0;
   for (int v = 0; v < nov; v++) max_vdeg = std::max(max_vdeg, h_vptr[(size_t)v + 1] - h_vptr[(size_t)v]);
   ctx->max_vdeg = max_vdeg;
+  ctx->sum_deg2 = 0;
+  for (int v = 0; v < nov; v++) {
+    const unsigned long long d = h_vptr[(size_t)v + 1] - h_vptr[(size_t)v];
+    ctx->sum_deg2 += d * d;
+  }
   // per net: ordering ranks of its pins, ascending (the apply replays moves in this order)
   ctx->h_rank_pins.resize((size_t)pins);
   bool dup = false;
@@ -336,10 +341,11 @@ fp_status fp_upload(fp_ctx* ctx, int nov, int noe, const unsigned* net_ptr, cons
   FP_TRY(dalloc(ctx, &ctx->net_stamp, (size_t)std::max(noe, 1)));
   FP_TRY(dalloc(ctx, &ctx->touched, (size_t)std::max(noe, 1)));
   FP_TRY(dalloc(ctx, &ctx->counters, 4));
+  FP_TRY(dalloc(ctx, &ctx->flags, 4));
+  fill_async<int>(ctx, ctx->flags, 4, 0);
   FP_TRY(dalloc(ctx, &ctx->moves, 1));
   FP_TRY(dalloc(ctx, &ctx->over_count, 1));
   fill_async<unsigned>(ctx, ctx->net_stamp, (size_t)std::max(noe, 1), 0u);
-  ctx->tag = 0;
   FP_HIP_OK(hipGetLastError());
   FP_HIP_OK(hipStreamSynchronize(ctx->stream));
   ctx->has_graph = true;

@@ -244,6 +244,7 @@ struct fp_ctx {
   bool src_on_pins = false;  // every non-empty net's source is one of its pins
   long long total_weight = 0;
   unsigned max_net = 0, max_vdeg = 0;
+  unsigned long long sum_deg2 = 0;  // sum over vertices of degree^2 (bounds CN's per-pass edit list)
   std::vector<unsigned> h_net_ptr;
   std::vector<int> h_rank_pins;  // per net, ordering ranks of its pins ascending
   std::vector<int> h_order;      // rank -> vertex
@@ -262,8 +263,8 @@ struct fp_ctx {
   // round scratch
   int rmax = 0;
   int *slot_own = nullptr, *slot_dec = nullptr, *touched = nullptr, *counters = nullptr;
-  unsigned* net_stamp = nullptr;
-  unsigned tag = 0;
+  int* flags = nullptr;  // [0] CN edit list overflow: the size it needed (0 = none)
+  unsigned* net_stamp = nullptr;  // round tag of the last round that touched the net (cleared every pass)
   long long *cand_tot = nullptr, *sizes_frozen = nullptr, *pref = nullptr;
   int* hist = nullptr;
   size_t hist_cap = 0;
@@ -318,14 +319,42 @@ struct fp_ctx {
   long long timed_launches = 0;
   double timed_ms = 0;
   std::vector<hipEvent_t> ev_pool;
+  size_t timed_next = 0;  // next pooled event pair for per-launch timing (events 0, 1 bracket a refine)
 };
 
 namespace fpe {
 
 fp_status fail(fp_ctx* c, fp_status st, const char* fmt, ...);
-fp_status cn_refine(fp_ctx* ctx, int passes, fp_refine_stats* stats);
-fp_status vol_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule* sched, fp_refine_stats* stats);
-fp_status conn_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule* sched, fp_refine_stats* stats);
+// A prepared refinement: what every pass of one objective on one context
+// needs, checked and allocated once. The launch shape (waves per evaluation
+// block, evaluation grid cap) does not change results; 0 = the default.
+struct RefinePlan {
+  fp_metric metric = FP_METRIC_TV;
+  fp_schedule sched{};
+  int R = 0;
+  int eval_waves = 0;
+  int eval_grid = 0;
+};
+
+// plan_*: checks, scratch, and the per-refinement initialisation kernels
+// (volumes, connections). pass_*: enqueues one pass on ctx->stream with no
+// host synchronisation and no host-visible state change, so a pass can be
+// captured into a HIP graph and replayed (every pass resets its own round
+// counters and net stamps on the device).
+fp_status plan_tv(fp_ctx* ctx, RefinePlan& p);
+fp_status pass_tv(fp_ctx* ctx, const RefinePlan& p);
+fp_status plan_cn(fp_ctx* ctx, RefinePlan& p);
+fp_status pass_cn(fp_ctx* ctx, const RefinePlan& p);
+fp_status plan_vol(fp_ctx* ctx, RefinePlan& p);
+fp_status pass_vol(fp_ctx* ctx, const RefinePlan& p);
+fp_status plan_conn(fp_ctx* ctx, RefinePlan& p);
+fp_status pass_conn(fp_ctx* ctx, const RefinePlan& p);
+// dispatch on p.metric
+fp_status plan_refine(fp_ctx* ctx, fp_metric metric, const fp_schedule* sched, RefinePlan& p);
+fp_status record_pass(fp_ctx* ctx, const RefinePlan& p);
+// CN only: after a pass, grows the edit list if the pass overflowed it (the
+// pass then left the state unchanged); *redo = 1 when it must run again
+fp_status cn_after_pass(fp_ctx* ctx, int* redo);
 fp_status dalloc_bytes(fp_ctx* ctx, void** p, size_t bytes);
 void dfree_all(fp_ctx* ctx);
 hipEvent_t pooled_event(fp_ctx* ctx, size_t i);

@@ -515,6 +515,131 @@ EvalArgs make_eval_args(fp_ctx* ctx, const fp_schedule* s) {
 }
 
 }  // namespace
+
+fp_status plan_tv(fp_ctx* ctx, RefinePlan& p) {
+  const int K = ctx->K;
+  const int waves = p.eval_waves > 0 ? std::min(p.eval_waves, eval_waves(K)) : eval_waves(K);
+  if (waves == 0) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d needs more LDS than a workgroup has", K);
+  p.eval_waves = waves;
+  const size_t lds_a = (size_t)(APPLY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
+  if (lds_a > (size_t)LDS_BUDGET) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d too large for the apply kernel", K);
+  p.R = std::min(p.sched.round, ctx->nov);
+  FP_TRY(ensure_round_scratch(ctx, p.R));
+  if (p.sched.kind == FP_SCHED_BAL) FP_TRY(ensure_bal_scratch(ctx, p.R));
+  return FP_OK;
+}
+
+fp_status pass_tv(fp_ctx* ctx, const RefinePlan& p) {
+  const int nov = ctx->nov, K = ctx->K, R = p.R, waves = p.eval_waves;
+  const fp_schedule* sched = &p.sched;
+  const bool bal = sched->kind == FP_SCHED_BAL;
+  const size_t lds_e = eval_lds(K, waves);
+  const size_t lds_a = (size_t)(APPLY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
+  const size_t lds_s = (size_t)K * sizeof(int);
+  // every pass restarts its round tags: clear the stamps and the touched counters
+  fill_async<unsigned>(ctx, ctx->net_stamp, (size_t)std::max(ctx->noe, 1), 0u);
+  fill_async<int>(ctx, ctx->counters, 4, 0);
+  unsigned tag = 0;
+  for (int r0 = 0; r0 < nov; r0 += R) {
+    const int nslots = std::min(R, nov - r0);
+    const int ntiles = (nslots + TILE - 1) / TILE;
+    tag++;
+    const int par = (int)(tag & 1u);
+    EvalArgs a = make_eval_args(ctx, sched);
+    a.r0 = r0;
+    a.nslots = nslots;
+    a.ntiles = ntiles;
+    a.tag = tag;
+    a.net_stamp = ctx->net_stamp;
+    a.touched = ctx->touched;
+    a.tcount = ctx->counters + par;
+    a.tcount_next = ctx->counters + (par ^ 1);
+    if (bal) {
+      a.cand_tot = ctx->cand_tot;
+      a.hist = ctx->hist;
+      a.sizes_frozen = ctx->sizes_frozen;
+    }
+    hipEvent_t ta = nullptr, tb = nullptr;
+    if (ctx->kernel_timing) {
+      ta = pooled_event(ctx, ctx->timed_next);
+      tb = pooled_event(ctx, ctx->timed_next + 1);
+      if (!ta || !tb) return fail(ctx, FP_ERR_HIP, "hipEventCreate failed");
+      ctx->timed_next += 2;
+      FP_HIP_OK(hipEventRecord(ta, ctx->stream));
+    }
+    int grid = eval_grid(nslots, waves);
+    if (p.eval_grid > 0) grid = std::min(grid, p.eval_grid);
+    k_tv_eval<<<grid, waves * WAVE, lds_e, ctx->stream>>>(a);
+    if (ctx->kernel_timing) FP_HIP_OK(hipEventRecord(tb, ctx->stream));
+    if (bal) {
+      k_bal_scan<<<std::max(1, std::min(K, 512)), 256, lds_s, ctx->stream>>>(
+          ctx->hist, ctx->pref, ntiles, K, ctx->sizes_frozen, ctx->cand_tot, sched->cap, ctx->over_idx,
+          ctx->over_count);
+      k_bal_admit<<<std::max(1, std::min((ntiles + 3) / 4, 4096)), 256, 0, ctx->stream>>>(
+          ctx->slot_dec, ctx->slot_own, ctx->order, ctx->vwgt, r0, nslots, ntiles, K, ctx->sizes_frozen, sched->cap,
+          ctx->over_idx, ctx->over_count, ctx->pref, ctx->hist, ctx->part, ctx->sizes, ctx->cand_tot, ctx->moves);
+    }
+    ApplyArgs ap{};
+    ap.net_ptr = ctx->net_ptr;
+    ap.apply_pins = ctx->apply_pins;
+    ap.slot_own = ctx->slot_own;
+    ap.slot_dec = ctx->slot_dec;
+    ap.touched = ctx->touched;
+    ap.tcount = ctx->counters + par;
+    ap.ent = ctx->ent;
+    ap.ev = ctx->ev;
+    ap.K = K;
+    ap.pb = ctx->pb;
+    ap.pmask = ctx->pmask;
+    ap.r0 = r0;
+    ap.r1 = r0 + nslots;
+    ap.net_blocks = APPLY_NET_BLOCKS;
+    int grid_a = APPLY_NET_BLOCKS;
+    if (!bal) {
+      ap.order = ctx->order;
+      ap.part = ctx->part;
+      ap.sizes = ctx->sizes;
+      ap.moves = ctx->moves;
+      grid_a += (nslots + APPLY_BLOCK - 1) / APPLY_BLOCK;
+    }
+    k_apply<<<grid_a, APPLY_BLOCK, lds_a, ctx->stream>>>(ap);
+  }
+  return FP_OK;
+}
+
+fp_status plan_refine(fp_ctx* ctx, fp_metric metric, const fp_schedule* sched, RefinePlan& p) {
+  if (!ctx->has_state) return fail(ctx, FP_ERR_STATE, "refinement before a state was set");
+  if (ctx->dup_pins) return fail(ctx, FP_ERR_UNSUPPORTED, "a net lists the same vertex twice; refinement needs distinct pins");
+  const int waves = p.eval_waves, grid = p.eval_grid;
+  p = RefinePlan{};
+  p.metric = metric;
+  p.eval_waves = waves;
+  p.eval_grid = grid;
+  FP_HIP_OK(hipSetDevice(ctx->device));
+  if (metric == FP_METRIC_CN) return plan_cn(ctx, p);  // the lambda = 2 push takes no schedule
+  FP_TRY(check_sched(ctx, sched));
+  p.sched = *sched;
+  FP_TRY(check_envelope(ctx));
+  switch (metric) {
+    case FP_METRIC_TV: return plan_tv(ctx, p);
+    case FP_METRIC_MSV:
+    case FP_METRIC_MSRV: return plan_vol(ctx, p);
+    case FP_METRIC_TM:
+    case FP_METRIC_MSM: return plan_conn(ctx, p);
+    default: return fail(ctx, FP_ERR_UNSUPPORTED, "refinement for metric %d not available", (int)metric);
+  }
+}
+
+fp_status record_pass(fp_ctx* ctx, const RefinePlan& p) {
+  switch (p.metric) {
+    case FP_METRIC_TV: return pass_tv(ctx, p);
+    case FP_METRIC_CN: return pass_cn(ctx, p);
+    case FP_METRIC_MSV:
+    case FP_METRIC_MSRV: return pass_vol(ctx, p);
+    default: return pass_conn(ctx, p);
+  }
+}
+
 }  // namespace fpe
 
 using namespace fpe;
@@ -523,111 +648,22 @@ extern "C" {
 
 fp_status fp_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule* sched, fp_refine_stats* stats) {
   if (!ctx) return FP_ERR_ARG;
-  if (!ctx->has_state) return fail(ctx, FP_ERR_STATE, "fp_refine before a state was set");
   if (passes < 0) return fail(ctx, FP_ERR_ARG, "passes < 0");
-  if (ctx->dup_pins) return fail(ctx, FP_ERR_UNSUPPORTED, "a net lists the same vertex twice; refinement needs distinct pins");
-  if (metric == FP_METRIC_CN) {  // the lambda = 2 push takes no schedule
-    FP_HIP_OK(hipSetDevice(ctx->device));
-    return cn_refine(ctx, passes, stats);
-  }
-  FP_TRY(check_sched(ctx, sched));
-  FP_TRY(check_envelope(ctx));
-  if (metric == FP_METRIC_MSV || metric == FP_METRIC_MSRV) {
-    FP_HIP_OK(hipSetDevice(ctx->device));
-    return vol_refine(ctx, metric, passes, sched, stats);
-  }
-  if (metric == FP_METRIC_TM || metric == FP_METRIC_MSM) {
-    FP_HIP_OK(hipSetDevice(ctx->device));
-    return conn_refine(ctx, metric, passes, sched, stats);
-  }
-  if (metric != FP_METRIC_TV) return fail(ctx, FP_ERR_UNSUPPORTED, "refinement for metric %d not available", (int)metric);
-  const int nov = ctx->nov, K = ctx->K;
-  const int waves = eval_waves(K);
-  const size_t lds_a = (size_t)(APPLY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
-  if (lds_a > (size_t)LDS_BUDGET) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d too large for the apply kernel", K);
-  FP_HIP_OK(hipSetDevice(ctx->device));
-  const int R = std::min(sched->round, nov);
-  const bool bal = sched->kind == FP_SCHED_BAL;
-  FP_TRY(ensure_round_scratch(ctx, R));
-  if (bal) FP_TRY(ensure_bal_scratch(ctx, R));
-  const size_t lds_e = eval_lds(K, waves);
-  const size_t lds_s = (size_t)K * sizeof(int);
-
-  fill_async<int>(ctx, ctx->counters, 4, 0);
+  RefinePlan p;
+  FP_TRY(plan_refine(ctx, metric, sched, p));
   fill_async<unsigned long long>(ctx, ctx->moves, 1, 0ull);
+  ctx->timed_next = 2;
+  ctx->timed_launches = 0;
+  ctx->timed_ms = 0;
   hipEvent_t e0 = pooled_event(ctx, 0), e1 = pooled_event(ctx, 1);
   if (!e0 || !e1) return fail(ctx, FP_ERR_HIP, "hipEventCreate failed");
   FP_HIP_OK(hipEventRecord(e0, ctx->stream));
-  ctx->timed_launches = 0;
-  ctx->timed_ms = 0;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> timed;
-  long long rounds = 0;
   for (int it = 0; it < passes; it++) {
-    for (int r0 = 0; r0 < nov; r0 += R) {
-      const int nslots = std::min(R, nov - r0);
-      const int ntiles = (nslots + TILE - 1) / TILE;
-      ctx->tag++;
-      const int par = (int)(ctx->tag & 1u);
-      EvalArgs a = make_eval_args(ctx, sched);
-      a.r0 = r0;
-      a.nslots = nslots;
-      a.ntiles = ntiles;
-      a.tag = ctx->tag;
-      a.net_stamp = ctx->net_stamp;
-      a.touched = ctx->touched;
-      a.tcount = ctx->counters + par;
-      a.tcount_next = ctx->counters + (par ^ 1);
-      if (bal) {
-        a.cand_tot = ctx->cand_tot;
-        a.hist = ctx->hist;
-        a.sizes_frozen = ctx->sizes_frozen;
-      }
-      hipEvent_t ta = nullptr, tb = nullptr;
-      if (ctx->kernel_timing) {
-        ta = pooled_event(ctx, 2 + 2 * timed.size());
-        tb = pooled_event(ctx, 3 + 2 * timed.size());
-        if (!ta || !tb) return fail(ctx, FP_ERR_HIP, "hipEventCreate failed");
-        FP_HIP_OK(hipEventRecord(ta, ctx->stream));
-      }
-      k_tv_eval<<<eval_grid(nslots, waves), waves * WAVE, lds_e, ctx->stream>>>(a);
-      if (ctx->kernel_timing) {
-        FP_HIP_OK(hipEventRecord(tb, ctx->stream));
-        timed.emplace_back(ta, tb);
-      }
-      if (bal) {
-        k_bal_scan<<<std::max(1, std::min(K, 512)), 256, lds_s, ctx->stream>>>(
-            ctx->hist, ctx->pref, ntiles, K, ctx->sizes_frozen, ctx->cand_tot, sched->cap, ctx->over_idx,
-            ctx->over_count);
-        k_bal_admit<<<std::max(1, std::min((ntiles + 3) / 4, 4096)), 256, 0, ctx->stream>>>(
-            ctx->slot_dec, ctx->slot_own, ctx->order, ctx->vwgt, r0, nslots, ntiles, K, ctx->sizes_frozen,
-            sched->cap, ctx->over_idx, ctx->over_count, ctx->pref, ctx->hist, ctx->part, ctx->sizes,
-            ctx->cand_tot, ctx->moves);
-      }
-      ApplyArgs ap{};
-      ap.net_ptr = ctx->net_ptr;
-      ap.apply_pins = ctx->apply_pins;
-      ap.slot_own = ctx->slot_own;
-      ap.slot_dec = ctx->slot_dec;
-      ap.touched = ctx->touched;
-      ap.tcount = ctx->counters + par;
-      ap.ent = ctx->ent;
-      ap.ev = ctx->ev;
-      ap.K = K;
-      ap.pb = ctx->pb;
-      ap.pmask = ctx->pmask;
-      ap.r0 = r0;
-      ap.r1 = r0 + nslots;
-      ap.net_blocks = APPLY_NET_BLOCKS;
-      int grid_a = APPLY_NET_BLOCKS;
-      if (!bal) {
-        ap.order = ctx->order;
-        ap.part = ctx->part;
-        ap.sizes = ctx->sizes;
-        ap.moves = ctx->moves;
-        grid_a += (nslots + APPLY_BLOCK - 1) / APPLY_BLOCK;
-      }
-      k_apply<<<grid_a, APPLY_BLOCK, lds_a, ctx->stream>>>(ap);
-      rounds++;
+    FP_TRY(record_pass(ctx, p));
+    if (metric == FP_METRIC_CN) {
+      int redo = 0;
+      FP_TRY(cn_after_pass(ctx, &redo));
+      if (redo) it--;
     }
   }
   FP_HIP_OK(hipGetLastError());
@@ -637,15 +673,15 @@ fp_status fp_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule
   FP_HIP_OK(hipStreamSynchronize(ctx->stream));
   float ms = 0;
   FP_HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-  for (auto& pr : timed) {
+  for (size_t i = 2; i + 1 < ctx->timed_next; i += 2) {
     float t = 0;
-    FP_HIP_OK(hipEventElapsedTime(&t, pr.first, pr.second));
+    FP_HIP_OK(hipEventElapsedTime(&t, ctx->ev_pool[i], ctx->ev_pool[i + 1]));
     ctx->timed_ms += t;
+    ctx->timed_launches++;
   }
-  ctx->timed_launches = (long long)timed.size();
   if (stats) {
     stats->moves = (long long)mv;
-    stats->rounds = rounds;
+    stats->rounds = metric == FP_METRIC_CN ? passes : (long long)passes * ((ctx->nov + p.R - 1) / p.R);
     stats->pin_visits = (long long)passes * ctx->pins;
     stats->device_ms = ms;
   }

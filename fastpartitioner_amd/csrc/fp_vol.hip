@@ -486,8 +486,8 @@ fp_status ensure_vol_scratch(fp_ctx* ctx, int R) {
 
 }  // namespace
 
-fp_status vol_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule* sched, fp_refine_stats* stats) {
-  const bool msrv = metric == FP_METRIC_MSRV;
+fp_status plan_vol(fp_ctx* ctx, RefinePlan& p) {
+  const bool msrv = p.metric == FP_METRIC_MSRV;
   const int nov = ctx->nov, K = ctx->K;
   if (K > VOL_KMAX) return fail(ctx, FP_ERR_UNSUPPORTED, "volume refinement supports K <= %d", VOL_KMAX);
   if (!ctx->src_on_pins)
@@ -496,113 +496,103 @@ fp_status vol_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedul
     return fail(ctx, FP_ERR_UNSUPPORTED,
                 "MSV refinement visits vertex ids in order (MGV/partition.cpp:755-757); upload with the identity ordering");
   if (ctx->pins >= (1ll << 31)) return fail(ctx, FP_ERR_UNSUPPORTED, "volume refinement needs pins < 2^31");
-  if (sched->kind != FP_SCHED_REF)
+  if (p.sched.kind != FP_SCHED_REF)
     return fail(ctx, FP_ERR_UNSUPPORTED, "volume refinement runs the reference round schedule (FP_SCHED_REF) only");
-  const int waves = vol_eval_waves(K);
+  const int waves = p.eval_waves > 0 ? std::min(p.eval_waves, vol_eval_waves(K)) : vol_eval_waves(K);
   if (waves == 0) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d needs more LDS than a workgroup has", K);
-  const size_t lds_e = (size_t)K * 16 + (size_t)waves * 8 * (size_t)K;
-  const size_t lds_r = (size_t)(REPLAY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
-  const size_t lds_s = (size_t)3 * (size_t)K * sizeof(int);
-  const int R = std::min(sched->round, nov);
-  FP_TRY(ensure_vol_scratch(ctx, R));
-  if (R > ctx->rmax) {
-    FP_TRY(dalloc(ctx, &ctx->slot_own, (size_t)R));
-    FP_TRY(dalloc(ctx, &ctx->slot_dec, (size_t)R));
-    ctx->rmax = R;
+  p.eval_waves = waves;
+  p.R = std::min(p.sched.round, nov);
+  FP_TRY(ensure_vol_scratch(ctx, p.R));
+  if (p.R > ctx->rmax) {
+    FP_TRY(dalloc(ctx, &ctx->slot_own, (size_t)p.R));
+    FP_TRY(dalloc(ctx, &ctx->slot_dec, (size_t)p.R));
+    ctx->rmax = p.R;
   }
+  // the volumes of the resident state, kept incrementally by the passes
   auto& vs = ctx->vol;
-  fill_async<int>(ctx, ctx->counters, 4, 0);
-  fill_async<unsigned long long>(ctx, ctx->moves, 1, 0ull);
   fill_async<int>(ctx, vs.vol, (size_t)K, 0);
-  hipEvent_t e0 = pooled_event(ctx, 0), e1 = pooled_event(ctx, 1);
-  if (!e0 || !e1) return fail(ctx, FP_ERR_HIP, "hipEventCreate failed");
-  FP_HIP_OK(hipEventRecord(e0, ctx->stream));
   const int ngrid = std::max(1, std::min((ctx->noe + 255) / 256, 4096));
   k_vol_init<<<ngrid, 256, 0, ctx->stream>>>(ctx->net_ptr, ctx->ent, ctx->src, ctx->part, ctx->noe, ctx->pmask,
                                               msrv ? 1 : 0, vs.vol);
-  long long rounds = 0;
-  for (int it = 0; it < passes; it++) {
-    k_round_maxset<<<1, 1024, 0, ctx->stream>>>(vs.vol, K, vs.mstat);
-    for (int r0 = 0; r0 < nov; r0 += R) {
-      const int nslots = std::min(R, nov - r0);
-      ctx->tag++;
-      const int par = (int)(ctx->tag & 1u);
-      VolEvalArgs a{};
-      a.vtx_ptr = ctx->vtx_ptr;
-      a.vtx_nets = ctx->vtx_nets;
-      a.vtx_seg = ctx->vtx_seg;
-      a.ev = ctx->ev;
-      a.src = ctx->src;
-      a.part = ctx->part;
-      a.order = ctx->order;
-      a.sizes = ctx->sizes;
-      a.vol = vs.vol;
-      a.mstat = vs.mstat;
-      a.K = K;
-      a.pb = ctx->pb;
-      a.msrv = msrv ? 1 : 0;
-      a.r0 = r0;
-      a.nslots = nslots;
-      a.C = sched->ldg_C;
-      a.slot_own = ctx->slot_own;
-      a.slot_dec = ctx->slot_dec;
-      a.tag = ctx->tag;
-      a.net_stamp = ctx->net_stamp;
-      a.touched = ctx->touched;
-      a.tcount = ctx->counters + par;
-      a.tcount_next = ctx->counters + (par ^ 1);
-      const int egrid = std::max(1, std::min((nslots + waves - 1) / waves, 8192));
-      k_vol_eval<<<egrid, waves * WAVE, lds_e, ctx->stream>>>(a);
-      k_round_compact<<<1, 1024, 0, ctx->stream>>>(ctx->slot_dec, ctx->slot_own, ctx->order, ctx->vtx_ptr, r0, nslots,
-                                                  vs.slot_mi, vs.mv, vs.mstat);
-      VolReplayArgs ra{};
-      ra.net_ptr = ctx->net_ptr;
-      ra.apply_pins = ctx->apply_pins;
-      ra.slot_mi = vs.slot_mi;
-      ra.mv = vs.mv;
-      ra.touched = ctx->touched;
-      ra.tcount = ctx->counters + par;
-      ra.src = ctx->src;
-      ra.vtx_ptr = ctx->vtx_ptr;
-      ra.vtx_nets = ctx->vtx_nets;
-      ra.mstat = vs.mstat;
-      ra.vwgt = ctx->vwgt;
-      ra.ent = ctx->ent;
-      ra.ev = ctx->ev;
-      ra.part = ctx->part;
-      ra.sizes = ctx->sizes;
-      ra.moves = ctx->moves;
-      ra.evbuf = vs.evbuf;
-      ra.K = K;
-      ra.pb = ctx->pb;
-      ra.pmask = ctx->pmask;
-      ra.r0 = r0;
-      ra.r1 = r0 + nslots;
-      ra.msrv = msrv ? 1 : 0;
-      ra.net_blocks = REPLAY_NET_BLOCKS;
-      ra.commit = 0;
-      k_vol_replay<<<REPLAY_NET_BLOCKS, REPLAY_BLOCK, lds_r, ctx->stream>>>(ra);
-      k_vol_scan<<<1, WAVE, lds_s, ctx->stream>>>(vs.vol, K, vs.mv, vs.evbuf, ctx->vtx_ptr, vs.mstat);
-      ra.commit = 1;
-      k_vol_replay<<<REPLAY_NET_BLOCKS + (nslots + REPLAY_BLOCK - 1) / REPLAY_BLOCK, REPLAY_BLOCK, lds_r,
-                     ctx->stream>>>(ra);
-      rounds++;
-    }
-  }
   FP_HIP_OK(hipGetLastError());
-  FP_HIP_OK(hipEventRecord(e1, ctx->stream));
-  unsigned long long mv = 0;
-  FP_TRY(d2h(ctx, &mv, ctx->moves, 1));
-  FP_HIP_OK(hipStreamSynchronize(ctx->stream));
-  float ms = 0;
-  FP_HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-  ctx->timed_launches = 0;
-  ctx->timed_ms = 0;
-  if (stats) {
-    stats->moves = (long long)mv;
-    stats->rounds = rounds;
-    stats->pin_visits = (long long)passes * ctx->pins;
-    stats->device_ms = ms;
+  return FP_OK;
+}
+
+fp_status pass_vol(fp_ctx* ctx, const RefinePlan& p) {
+  const bool msrv = p.metric == FP_METRIC_MSRV;
+  const int nov = ctx->nov, K = ctx->K, R = p.R, waves = p.eval_waves;
+  const size_t lds_e = (size_t)K * 16 + (size_t)waves * 8 * (size_t)K;
+  const size_t lds_r = (size_t)(REPLAY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
+  const size_t lds_s = (size_t)3 * (size_t)K * sizeof(int);
+  auto& vs = ctx->vol;
+  fill_async<unsigned>(ctx, ctx->net_stamp, (size_t)std::max(ctx->noe, 1), 0u);
+  fill_async<int>(ctx, ctx->counters, 4, 0);
+  k_round_maxset<<<1, 1024, 0, ctx->stream>>>(vs.vol, K, vs.mstat);
+  unsigned tag = 0;
+  for (int r0 = 0; r0 < nov; r0 += R) {
+    const int nslots = std::min(R, nov - r0);
+    tag++;
+    const int par = (int)(tag & 1u);
+    VolEvalArgs a{};
+    a.vtx_ptr = ctx->vtx_ptr;
+    a.vtx_nets = ctx->vtx_nets;
+    a.vtx_seg = ctx->vtx_seg;
+    a.ev = ctx->ev;
+    a.src = ctx->src;
+    a.part = ctx->part;
+    a.order = ctx->order;
+    a.sizes = ctx->sizes;
+    a.vol = vs.vol;
+    a.mstat = vs.mstat;
+    a.K = K;
+    a.pb = ctx->pb;
+    a.msrv = msrv ? 1 : 0;
+    a.r0 = r0;
+    a.nslots = nslots;
+    a.C = p.sched.ldg_C;
+    a.slot_own = ctx->slot_own;
+    a.slot_dec = ctx->slot_dec;
+    a.tag = tag;
+    a.net_stamp = ctx->net_stamp;
+    a.touched = ctx->touched;
+    a.tcount = ctx->counters + par;
+    a.tcount_next = ctx->counters + (par ^ 1);
+    int egrid = std::max(1, std::min((nslots + waves - 1) / waves, 8192));
+    if (p.eval_grid > 0) egrid = std::min(egrid, p.eval_grid);
+    k_vol_eval<<<egrid, waves * WAVE, lds_e, ctx->stream>>>(a);
+    k_round_compact<<<1, 1024, 0, ctx->stream>>>(ctx->slot_dec, ctx->slot_own, ctx->order, ctx->vtx_ptr, r0, nslots,
+                                                  vs.slot_mi, vs.mv, vs.mstat);
+    VolReplayArgs ra{};
+    ra.net_ptr = ctx->net_ptr;
+    ra.apply_pins = ctx->apply_pins;
+    ra.slot_mi = vs.slot_mi;
+    ra.mv = vs.mv;
+    ra.touched = ctx->touched;
+    ra.tcount = ctx->counters + par;
+    ra.src = ctx->src;
+    ra.vtx_ptr = ctx->vtx_ptr;
+    ra.vtx_nets = ctx->vtx_nets;
+    ra.mstat = vs.mstat;
+    ra.vwgt = ctx->vwgt;
+    ra.ent = ctx->ent;
+    ra.ev = ctx->ev;
+    ra.part = ctx->part;
+    ra.sizes = ctx->sizes;
+    ra.moves = ctx->moves;
+    ra.evbuf = vs.evbuf;
+    ra.K = K;
+    ra.pb = ctx->pb;
+    ra.pmask = ctx->pmask;
+    ra.r0 = r0;
+    ra.r1 = r0 + nslots;
+    ra.msrv = msrv ? 1 : 0;
+    ra.net_blocks = REPLAY_NET_BLOCKS;
+    ra.commit = 0;
+    k_vol_replay<<<REPLAY_NET_BLOCKS, REPLAY_BLOCK, lds_r, ctx->stream>>>(ra);
+    k_vol_scan<<<1, WAVE, lds_s, ctx->stream>>>(vs.vol, K, vs.mv, vs.evbuf, ctx->vtx_ptr, vs.mstat);
+    ra.commit = 1;
+    k_vol_replay<<<REPLAY_NET_BLOCKS + (nslots + REPLAY_BLOCK - 1) / REPLAY_BLOCK, REPLAY_BLOCK, lds_r,
+                   ctx->stream>>>(ra);
   }
   return FP_OK;
 }

@@ -59,6 +59,16 @@ def declare(lib) -> None:
     lib.fp_get_connections.argtypes = [P, _I32P, _I32P]
     lib.fp_status_string.restype = c.c_char_p
     lib.fp_status_string.argtypes = [c.c_int]
+    lib.fp_batch_create.argtypes = [c.POINTER(P), c.c_int]
+    lib.fp_batch_destroy.argtypes = [P]
+    lib.fp_batch_destroy.restype = None
+    lib.fp_batch_last_error.argtypes = [P]
+    lib.fp_batch_last_error.restype = c.c_char_p
+    lib.fp_batch_add.argtypes = [P, P, c.c_int, c.POINTER(Schedule)]
+    lib.fp_batch_size.argtypes = [P]
+    lib.fp_batch_refine.argtypes = [P, c.c_int, c.c_int, c.POINTER(c.c_double)]
+    lib.fp_batch_member.argtypes = [P, c.c_int, c.POINTER(c.c_longlong), c.POINTER(c.c_int), c.POINTER(c.c_int),
+                                    c.POINTER(c.c_double)]
     lib.fp_set_kernel_timing.argtypes = [P, c.c_int]
     lib.fp_last_kernel_timing.argtypes = [P, c.POINTER(c.c_longlong), c.POINTER(c.c_double)]
 
@@ -193,3 +203,46 @@ class Engine:
         self._check(self.lib.fp_last_kernel_timing(self.ctx, ctypes.byref(n), ctypes.byref(ms)),
                     "fp_last_kernel_timing")
         return n.value, ms.value
+
+
+class Batch:
+    """Many resident hypergraphs (Engine contexts) refined together: one HIP
+    graph launch per pass, launch shapes picked by the first tune passes."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _native.engine()
+        declare(self.lib)
+        self.b = ctypes.c_void_p()
+        _native.check(self.lib.fp_batch_create(ctypes.byref(self.b), device), "fp_batch_create")
+        self.members = []
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise _native.FastPartError(rc, f"{what}: {self.lib.fp_batch_last_error(self.b).decode()}")
+
+    def add(self, eng: "Engine", metric: str, sched: Schedule | None = None):
+        self._check(self.lib.fp_batch_add(self.b, eng.ctx, METRICS[metric],
+                                          ctypes.byref(sched) if sched is not None else None), "fp_batch_add")
+        self.members.append(eng)
+
+    def refine(self, passes: int, tune_passes: int = 0) -> float:
+        ms = ctypes.c_double()
+        self._check(self.lib.fp_batch_refine(self.b, passes, tune_passes, ctypes.byref(ms)), "fp_batch_refine")
+        return ms.value
+
+    def member(self, i: int):
+        mv, w, g, t = ctypes.c_longlong(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self._check(self.lib.fp_batch_member(self.b, i, ctypes.byref(mv), ctypes.byref(w), ctypes.byref(g),
+                                             ctypes.byref(t)), "fp_batch_member")
+        return {"moves": mv.value, "eval_waves": w.value, "eval_grid": g.value, "tuned_pass_ms": t.value}
+
+    def close(self):
+        if self.b:
+            self.lib.fp_batch_destroy(self.b)
+            self.b = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -180,6 +180,33 @@ fp_status fp_eval(fp_ctx* ctx, fp_metrics* out, int* lam, long long* sv, long lo
  * may be NULL. Needs K <= 4096 and net sources that are pins of their nets. */
 fp_status fp_get_connections(fp_ctx* ctx, int* pc, int* msg);
 
+/* ---- batched refinement ------------------------------------------------- */
+/* A batch refines many resident hypergraphs of uneven size, K and objective
+ * together. Each member's pass is captured once into a HIP graph; all members'
+ * pass graphs are children of one batch graph, launched once per pass, so the
+ * members' work overlaps on the device. Every member ends exactly where
+ * fp_refine(ctx, metric, passes, sched) alone would leave it. The reference
+ * has no counterpart (it refines one hypergraph per process). */
+typedef struct fp_batch fp_batch;
+fp_status fp_batch_create(fp_batch** batch, int device);
+void fp_batch_destroy(fp_batch* batch);
+const char* fp_batch_last_error(const fp_batch* batch);
+/* Adds a context that holds a hypergraph and a state, with the objective and
+ * schedule it is refined under (sched may be NULL for CN). The batch borrows
+ * the context; it must outlive the batch's use of it. */
+fp_status fp_batch_add(fp_batch* batch, fp_ctx* ctx, fp_metric metric, const fp_schedule* sched);
+int fp_batch_size(const fp_batch* batch);
+/* `passes` passes of every member. The first tune_passes of them run eagerly
+ * with a different candidate launch shape each (wavefronts per evaluation
+ * block, evaluation grid cap), timed per member; each member then keeps its
+ * fastest shape for the remaining passes, which run as one batch-graph launch
+ * per pass. device_ms (nullable) receives the device time of the whole call. */
+fp_status fp_batch_refine(fp_batch* batch, int passes, int tune_passes, double* device_ms);
+/* Member i after fp_batch_refine: applied moves, the launch shape it kept and
+ * its fastest tuning pass (ms). Any pointer may be NULL. */
+fp_status fp_batch_member(const fp_batch* batch, int i, long long* moves, int* eval_waves, int* eval_grid,
+                          double* tuned_pass_ms);
+
 /* Event timing of the dominant refinement kernel in the last fp_refine:
  * launches, total device ms of those launches (HIP events around each launch,
  * on the context stream). */
