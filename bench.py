@@ -87,6 +87,126 @@ def cpu_baseline(hg, K, passes, threads):
     return {"error": f"reference crashed at every thread count: {crashes}"}
 
 
+OBJECTIVES = ("TV", "CN", "TM", "MSM", "MSV", "MSRV")
+
+
+def batch_specs(count, seed):
+    """Instances of the batched workload: vertex count log-uniform in
+    [1,000, 200,000], K in {16, 64, 256}, objective uniform over the six."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(np.floor(np.exp(rng.uniform(np.log(1e3), np.log(2e5)))))
+        out.append((i, n, int(rng.choice([16, 64, 256])), OBJECTIVES[int(rng.integers(6))]))
+    return out
+
+
+def batch_schedule(metric, nov, K, C, max_pw, min_pw, cap):
+    """Round schedule per objective: TV the balanced schedule, TM a whole pass
+    per round (its reference schedule), the max-set objectives the reference
+    round schedule with rounds of 16384 vertices."""
+    from fastpartitioner_amd.engine import SCHED_BAL, SCHED_REF, schedule
+
+    if metric == "CN":
+        return None
+    if metric == "TV":
+        return schedule(SCHED_BAL, 16384, C, cap, 0, cap)
+    R = nov if metric == "TM" else 16384
+    return schedule(SCHED_REF, R, C, max_pw, min_pw)
+
+
+def run_batch(args):
+    """Batched refinement of many hypergraphs against one-at-a-time refinement
+    of the same hypergraphs from the same states: hypergraphs/s of each, and
+    whether every final part vector is identical."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    # hardware queues of this process, read when the HIP runtime starts: the
+    # batch's lanes run side by side only on distinct queues
+    os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
+    from fastpartitioner_amd import hypergraph as H
+    from fastpartitioner_amd.engine import Batch, Engine
+
+    eps = 0.05
+    specs = batch_specs(args.instances, args.seed)
+
+    def prep(spec):
+        i, n, K, metric = spec
+        hg = H.colnet(n, H.DEG_UNIFORM, 4, 32, seed=1000 + i)
+        C, max_pw, min_pw = H.ref_constants(hg.nov, K, eps)
+        part = H.initial_partition(hg, K, C, max_pw, min_pw)
+        return hg, part, batch_schedule(metric, hg.nov, K, C, max_pw, min_pw, H.balance_cap(hg.nov, K, eps))
+
+    t = time.time()
+    with ThreadPoolExecutor(max_workers=min(16, len(os.sched_getaffinity(0)))) as ex:
+        prepared = list(ex.map(prep, specs))
+    pins = sum(hg.pins for hg, _, _ in prepared)
+    print(f"# batch: {len(specs)} instances, {sum(hg.nov for hg, _, _ in prepared)} vertices, {pins} pins, "
+          f"prep {time.time() - t:.1f}s", file=sys.stderr, flush=True)
+
+    engines = []
+    try:
+        t = time.time()
+        for (i, n, K, metric), (hg, part, sched) in zip(specs, prepared):
+            eng = Engine(0)
+            engines.append(eng)
+            eng.upload(hg)
+            eng.set_partition(K, part)
+        print(f"# batch: uploaded in {time.time() - t:.1f}s", file=sys.stderr, flush=True)
+        # one at a time
+        t0 = time.perf_counter()
+        for eng, (i, n, K, metric), (hg, part, sched) in zip(engines, specs, prepared):
+            eng.refine(metric, args.passes, sched)
+        t_solo = time.perf_counter() - t0
+        solo_parts = [eng.get_state().part for eng in engines]
+        print(f"# batch: one at a time {t_solo:.2f}s", file=sys.stderr, flush=True)
+        for eng, (i, n, K, metric), (hg, part, sched) in zip(engines, specs, prepared):
+            eng.set_partition(K, part)
+        # batched
+        with Batch(0) as b:
+            b.set_lanes(args.lanes)
+            for eng, (i, n, K, metric), (hg, part, sched) in zip(engines, specs, prepared):
+                b.add(eng, metric, sched)
+            t0 = time.perf_counter()
+            dev_ms = b.refine(args.passes, tune_passes=args.tune_passes)
+            t_batch = time.perf_counter() - t0
+            shapes = {}
+            for j in range(len(engines)):
+                mm = b.member(j)
+                key = f"{mm['eval_waves']}x{mm['eval_grid']}"
+                shapes[key] = shapes.get(key, 0) + 1
+        print(f"# batch: batched {t_batch:.2f}s", file=sys.stderr, flush=True)
+        identical = all(np.array_equal(eng.get_state().part, sp) for eng, sp in zip(engines, solo_parts))
+    finally:
+        for eng in engines:
+            eng.close()
+    n = len(specs)
+    out = {
+        "metric": "batched refinement hypergraphs/s (mixed size, K and objective)",
+        "value": round(n / t_batch, 2),
+        "unit": "hypergraphs/s",
+        "n_gpus": 1,
+        "steps": args.passes,
+        "warmup": 0,
+        "ms_per_step": round(t_batch * 1e3 / args.passes, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"batch-{n}-colnet-uniform4to32-logU1e3to2e5", "instances": n, "pins": int(pins),
+                   "passes": args.passes, "tune_passes": args.tune_passes, "lanes": args.lanes,
+                   "hw_queues": args.hw_queues,
+                   "objectives": {m: sum(1 for s in specs if s[3] == m) for m in OBJECTIVES},
+                   "tuned_shapes": shapes},
+        "one_at_a_time_hgs": round(n / t_solo, 2),
+        "speedup_over_one_at_a_time": round(t_solo / t_batch, 2),
+        "identical_results": bool(identical),
+        "batch_device_ms": round(dev_ms, 2),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +218,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU baseline")
     ap.add_argument("--cpu-passes", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--workload", choices=("tv1m", "batch"), default="tv1m",
+                    help="tv1m: the 1M-vertex TV refinement (default); batch: batched vs one-at-a-time refinement")
+    ap.add_argument("--instances", type=int, default=512, help="batch: number of hypergraphs")
+    ap.add_argument("--passes", type=int, default=10, help="batch: passes per hypergraph")
+    ap.add_argument("--tune-passes", type=int, default=3, help="batch: passes that double as autotuner trials")
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--lanes", type=int, default=16, help="batch: streams the members are spread over")
+    ap.add_argument("--hw-queues", type=int, default=16, help="batch: hardware queues of the process")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -117,6 +245,10 @@ def main():
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.workload == "batch":
+        run_batch(args)
+        return
 
     from fastpartitioner_amd import hypergraph as H
     from fastpartitioner_amd.engine import SCHED_BAL, SCHED_REF, Engine, schedule

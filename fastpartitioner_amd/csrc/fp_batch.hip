@@ -1,13 +1,14 @@
 // fp_batch.hip -- batched refinement: many resident hypergraphs (contexts) of
 // uneven size, K and objective refined together.
 //
-// Each member's pass (the same kernel sequence fp_refine enqueues, with no
-// host synchronisation in it) is captured once from the member's stream into
-// a HIP graph. The members' pass graphs become independent child nodes of one
-// batch graph, so one graph launch runs a pass of every member, and members
-// that are independent of each other can overlap on the device. Results are
-// those of fp_refine on each member alone: the launch shape only changes how
-// the work is spread over the GPU.
+// Members are spread over L lanes (streams) by estimated pass cost (pins,
+// longest first onto the least loaded lane). Each lane's members' passes (the
+// kernel sequences fp_refine enqueues, with no host synchronisation in them)
+// are captured once, back to back, into one HIP graph per lane; a pass of the
+// batch is one launch of every lane graph, and the lanes run side by side on
+// the device's hardware queues (the HIP runtime's GPU_MAX_HW_QUEUES, 4 by
+// default). Results are those of fp_refine on each member alone: lanes and
+// launch shapes only change how the work is spread over the GPU.
 //
 // Autotuning: the first passes of a batch are run eagerly, each with another
 // candidate launch shape (wavefronts per evaluation block, evaluation grid
@@ -29,9 +30,11 @@ struct fp_batch {
     float tuned_ms = 0;
   };
   std::vector<Member> members;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  std::vector<hipGraph_t> member_graphs;
+  int lanes = 8;
+  std::vector<hipStream_t> lane_streams;
+  std::vector<hipGraph_t> lane_graphs;
+  std::vector<hipGraphExec_t> lane_execs;
+  std::vector<int> lane_of;  // member -> lane
   std::vector<hipEvent_t> events;
   std::vector<size_t> owned_at_capture;  // members' allocation counts the captured graph relies on
 };
@@ -53,12 +56,10 @@ fp_status bfail(fp_batch* b, fp_status st, const char* fmt, ...) {
   } while (0)
 
 void drop_graphs(fp_batch* b) {
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  for (hipGraph_t g : b->member_graphs) (void)hipGraphDestroy(g);
-  b->exec = nullptr;
-  b->graph = nullptr;
-  b->member_graphs.clear();
+  for (hipGraphExec_t e : b->lane_execs) (void)hipGraphExecDestroy(e);
+  for (hipGraph_t g : b->lane_graphs) (void)hipGraphDestroy(g);
+  b->lane_execs.clear();
+  b->lane_graphs.clear();
 }
 
 hipEvent_t batch_event(fp_batch* b, size_t i) {
@@ -109,10 +110,19 @@ void fp_batch_destroy(fp_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
   (void)hipStreamSynchronize(b->stream);
+  for (hipStream_t s : b->lane_streams) (void)hipStreamSynchronize(s);
   drop_graphs(b);
   for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
+  for (hipStream_t s : b->lane_streams) (void)hipStreamDestroy(s);
   (void)hipStreamDestroy(b->stream);
   delete b;
+}
+
+fp_status fp_batch_set_lanes(fp_batch* b, int lanes) {
+  if (!b || lanes < 1 || lanes > 64) return FP_ERR_ARG;
+  if (lanes != b->lanes) drop_graphs(b);
+  b->lanes = lanes;
+  return FP_OK;
 }
 
 const char* fp_batch_last_error(const fp_batch* b) { return b ? b->err : "null batch"; }
@@ -217,43 +227,81 @@ fp_status fp_batch_refine(fp_batch* b, int passes, int tune_passes, double* devi
   }
 
   const int rest = passes - ntune;
-  if (b->exec) {
+  if (!b->lane_execs.empty()) {
     bool same = b->owned_at_capture.size() == nm;
     for (size_t i = 0; same && i < nm; i++) same = b->owned_at_capture[i] == b->members[i].ctx->owned.size();
     if (!same) drop_graphs(b);
   }
-  if (rest > 0 && nm > 0 && !b->exec) {
-    // capture one pass of every member and join them under one graph
-    for (size_t i = 0; i < nm; i++) B_HIP_OK(hipStreamSynchronize(b->members[i].ctx->stream));
-    B_HIP_OK(hipGraphCreate(&b->graph, 0));
-    for (size_t i = 0; i < nm; i++) {
-      auto& m = b->members[i];
-      const bool timing = m.ctx->kernel_timing;
-      m.ctx->kernel_timing = false;
-      B_HIP_OK(hipStreamBeginCapture(m.ctx->stream, hipStreamCaptureModeThreadLocal));
-      fp_status st = record_pass(m.ctx, m.plan);
-      hipGraph_t g = nullptr;
-      const hipError_t ce = hipStreamEndCapture(m.ctx->stream, &g);
-      m.ctx->kernel_timing = timing;
-      if (st) return member_fail(b, i, st);
-      if (ce != hipSuccess || !g) return bfail(b, FP_ERR_HIP, "member %zu: pass capture failed: %s", i, hipGetErrorString(ce));
-      b->member_graphs.push_back(g);
-      hipGraphNode_t node = nullptr;
-      B_HIP_OK(hipGraphAddChildGraphNode(&node, b->graph, nullptr, 0, g));
+  const int L = std::max(1, std::min(b->lanes, (int)nm));
+  if (rest > 0 && nm > 0 && b->lane_execs.empty()) {
+    while ((int)b->lane_streams.size() < L) {
+      hipStream_t s = nullptr;
+      B_HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+      b->lane_streams.push_back(s);
     }
-    B_HIP_OK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    // longest first onto the least loaded lane (pins as the pass cost)
+    std::vector<size_t> order(nm);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t x, size_t y) { return b->members[x].ctx->pins > b->members[y].ctx->pins; });
+    std::vector<long long> load((size_t)L, 0);
+    b->lane_of.assign(nm, 0);
+    for (size_t i : order) {
+      const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      b->lane_of[i] = l;
+      load[(size_t)l] += b->members[i].ctx->pins + b->members[i].ctx->nov;
+    }
+    for (size_t i = 0; i < nm; i++) B_HIP_OK(hipStreamSynchronize(b->members[i].ctx->stream));
+    // capture each lane: its members' passes, recorded on the lane stream
+    for (int l = 0; l < L; l++) {
+      hipStream_t ls = b->lane_streams[(size_t)l];
+      B_HIP_OK(hipStreamBeginCapture(ls, hipStreamCaptureModeThreadLocal));
+      fp_status st = FP_OK;
+      size_t bad = 0;
+      for (size_t i = 0; i < nm && st == FP_OK; i++) {
+        if (b->lane_of[i] != l) continue;
+        auto& m = b->members[i];
+        const hipStream_t own = m.ctx->stream;
+        const bool timing = m.ctx->kernel_timing;
+        m.ctx->stream = ls;
+        m.ctx->kernel_timing = false;
+        st = record_pass(m.ctx, m.plan);
+        m.ctx->stream = own;
+        m.ctx->kernel_timing = timing;
+        bad = i;
+      }
+      hipGraph_t g = nullptr;
+      const hipError_t ce = hipStreamEndCapture(ls, &g);
+      if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return member_fail(b, bad, st);
+      }
+      if (ce != hipSuccess || !g) return bfail(b, FP_ERR_HIP, "lane %d: pass capture failed: %s", l, hipGetErrorString(ce));
+      b->lane_graphs.push_back(g);
+      hipGraphExec_t ex = nullptr;
+      B_HIP_OK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+      b->lane_execs.push_back(ex);
+    }
     b->owned_at_capture.resize(nm);
     for (size_t i = 0; i < nm; i++) b->owned_at_capture[i] = b->members[i].ctx->owned.size();
   }
   if (rest > 0 && nm > 0) {
-    // the batch stream starts after the members' planning work
+    // each lane starts after its members' planning work
     for (size_t i = 0; i < nm; i++) {
       hipEvent_t e = batch_event(b, 2 + 3 * nm + i);
       if (!e) return bfail(b, FP_ERR_HIP, "hipEventCreate failed");
       B_HIP_OK(hipEventRecord(e, b->members[i].ctx->stream));
+      B_HIP_OK(hipStreamWaitEvent(b->lane_streams[(size_t)b->lane_of[i]], e, 0));
+    }
+    for (int it = 0; it < rest; it++)
+      for (int l = 0; l < (int)b->lane_execs.size(); l++)
+        B_HIP_OK(hipGraphLaunch(b->lane_execs[(size_t)l], b->lane_streams[(size_t)l]));
+    for (int l = 0; l < (int)b->lane_execs.size(); l++) {
+      hipEvent_t e = batch_event(b, 2 + 4 * nm + (size_t)l);
+      if (!e) return bfail(b, FP_ERR_HIP, "hipEventCreate failed");
+      B_HIP_OK(hipEventRecord(e, b->lane_streams[(size_t)l]));
       B_HIP_OK(hipStreamWaitEvent(b->stream, e, 0));
     }
-    for (int it = 0; it < rest; it++) B_HIP_OK(hipGraphLaunch(b->exec, b->stream));
   }
   // the batch stream's end event waits for work left on member streams
   for (size_t i = 0; i < nm; i++) {

@@ -38,46 +38,58 @@ constexpr int CONN_KMAX = 4096;
 constexpr int REPLAY_BLOCK = 256;
 constexpr int REPLAY_NET_BLOCKS = 2048;
 
-__global__ void k_conn_init(const unsigned* __restrict__ net_ptr, const unsigned* __restrict__ ent,
-                            const int* __restrict__ src, const int* __restrict__ part, int noe, unsigned pmask, int K,
-                            int* __restrict__ pc) {
-  for (int n = (int)(blockIdx.x * blockDim.x + threadIdx.x); n < noe; n += (int)(gridDim.x * blockDim.x)) {
+// one wavefront per net: pc[source part][listed part] += 1 for listed parts
+// other than the source's
+__global__ void __launch_bounds__(256) k_conn_init(const unsigned* __restrict__ net_ptr,
+                                                   const unsigned* __restrict__ ent, const int* __restrict__ src,
+                                                   const int* __restrict__ part, int noe, unsigned pmask, int K,
+                                                   int* __restrict__ pc) {
+  const int lane = lane_id();
+  const int nw = (int)(gridDim.x * blockDim.x / WAVE);
+  for (int n = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE); n < noe; n += nw) {
     const unsigned seg = net_ptr[n] + (unsigned)n;
     const int L = (int)ent[seg];
     if (L < 2) continue;
     const int sp = part[src[n]];
-    for (int t = 0; t < L; t++) {
+    for (int t = lane; t < L; t += WAVE) {
       const int q = (int)(ent[seg + 1 + (unsigned)t] & pmask);
       if (q != sp) atomicAdd(&pc[(size_t)sp * (size_t)K + (size_t)q], 1);
     }
   }
 }
 
-// One wavefront per 64 x 64 tile of pc: the tile's words of B and BT, and the
-// popcounts of its B words into msg (zeroed before).
-__global__ void __launch_bounds__(WAVE) k_conn_bits(const int* __restrict__ pc, int K, int W,
-                                                     unsigned long long* __restrict__ B,
-                                                     unsigned long long* __restrict__ BT, int* __restrict__ msg) {
+// One block (four wavefronts) per 64 x 64 tile of pc: the tile's words of B
+// (wavefront 0) and BT (wavefront 1), and the popcounts of its B words into
+// msg (zeroed before).
+__global__ void __launch_bounds__(256) k_conn_bits(const int* __restrict__ pc, int K, int W,
+                                                    unsigned long long* __restrict__ B,
+                                                    unsigned long long* __restrict__ BT, int* __restrict__ msg) {
   __shared__ int tile[WAVE][WAVE + 1];
-  const int lane = lane_id();
+  const int lane = lane_id(), wave = (int)(threadIdx.x / WAVE);
   const int ta = (int)blockIdx.y, tb = (int)blockIdx.x;
-  for (int r = 0; r < WAVE; r++) {
+  for (int r = wave; r < WAVE; r += (int)(blockDim.x / WAVE)) {
     const int a = ta * WAVE + r, b = tb * WAVE + lane;
     tile[r][lane] = (a < K && b < K) ? pc[(size_t)a * (size_t)K + (size_t)b] : 0;
   }
-  wave_lds_sync();
-  unsigned long long row = 0, col = 0;
-  for (int c = 0; c < WAVE; c++) {
-    // row a = ta*64 + lane, column b = tb*64 + c; and row b = tb*64 + lane, column a = ta*64 + c
-    if (tile[lane][c] > 0 && ta * WAVE + lane != tb * WAVE + c) row |= 1ull << c;
-    if (tile[c][lane] > 0 && ta * WAVE + c != tb * WAVE + lane) col |= 1ull << c;
+  __syncthreads();
+  if (wave == 0) {
+    // row a = ta*64 + lane, columns b = tb*64 + c
+    unsigned long long row = 0;
+    for (int c = 0; c < WAVE; c++)
+      if (tile[lane][c] > 0 && ta * WAVE + lane != tb * WAVE + c) row |= 1ull << c;
+    const int a = ta * WAVE + lane;
+    if (a < K) {
+      B[(size_t)a * (size_t)W + (size_t)tb] = row;
+      if (row) atomicAdd(&msg[a], __popcll(row));
+    }
+  } else if (wave == 1) {
+    // row b = tb*64 + lane of the transpose, columns a = ta*64 + c
+    unsigned long long col = 0;
+    for (int c = 0; c < WAVE; c++)
+      if (tile[c][lane] > 0 && ta * WAVE + c != tb * WAVE + lane) col |= 1ull << c;
+    const int b = tb * WAVE + lane;
+    if (b < K) BT[(size_t)b * (size_t)W + (size_t)ta] = col;
   }
-  const int a = ta * WAVE + lane, b = tb * WAVE + lane;
-  if (a < K) {
-    B[(size_t)a * (size_t)W + (size_t)tb] = row;
-    if (row) atomicAdd(&msg[a], __popcll(row));
-  }
-  if (b < K) BT[(size_t)b * (size_t)W + (size_t)ta] = col;
 }
 
 struct ConnEvalArgs {
@@ -292,44 +304,42 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
   }
 }
 
-// TM apply order (TM/partition.cpp:1111-1125): candidates in vertex order;
-// one whose destination size is at least thr = C * 1.02 at that point is
-// dropped. Commits part vector and both sizes of the others.
-__global__ void __launch_bounds__(WAVE) k_tm_admit(int* __restrict__ slot_dec, const int* __restrict__ slot_own,
-                                                   const int* __restrict__ vwgt, int r0, int nslots, int K, double thr,
-                                                   long long* __restrict__ sizes, int* __restrict__ part,
-                                                   unsigned long long* __restrict__ moves) {
+// TM apply order (TM/partition.cpp:1111-1125): the round's candidates (the
+// compacted move list) in vertex order; one whose destination size is at
+// least thr = C * 1.02 at that point is dropped. Commits part vector and both
+// sizes of the others.
+__global__ void __launch_bounds__(WAVE) k_tm_admit(int* __restrict__ slot_dec, const int4* __restrict__ mv,
+                                                   const int* __restrict__ mstat, const int* __restrict__ vwgt,
+                                                   int r0, int K, double thr, long long* __restrict__ sizes,
+                                                   int* __restrict__ part, unsigned long long* __restrict__ moves) {
   extern __shared__ __align__(16) unsigned char lds[];
   long long* sz = reinterpret_cast<long long*>(lds);
   const int lane = lane_id();
   for (int q = lane; q < K; q += WAVE) sz[q] = sizes[q];
   wave_lds_sync();
+  const int nmv = mstat[2];
   unsigned long long nm = 0;
-  for (int s0 = 0; s0 < nslots; s0 += WAVE) {
-    const int s = s0 + lane;
-    const int d = s < nslots ? slot_dec[s] : -1;
-    const int o = d >= 0 ? slot_own[s] : 0;
-    const int w = d >= 0 ? (vwgt ? vwgt[r0 + s] : 1) : 0;
-    unsigned long long bm = __ballot(d >= 0);
-    while (bm) {
-      const int l = __ffsll((long long)bm) - 1;
-      bm &= bm - 1;
-      const int dd = __shfl(d, l), oo = __shfl(o, l), ww = __shfl(w, l);
-      const bool skip = (double)sz[dd] >= thr;
-      wave_lds_sync();
+  for (int m0 = 0; m0 < nmv; m0 += WAVE) {
+    const int i = m0 + lane;
+    const int4 x = i < nmv ? mv[i] : make_int4(0, 0, 0, 0);
+    const int w = i < nmv ? (vwgt ? vwgt[x.x] : 1) : 0;
+    const int cnt = min(WAVE, nmv - m0);
+    // lane 0 walks the chunk's candidates in order; only it touches sz here
+    for (int l = 0; l < cnt; l++) {
+      const int vv = __shfl(x.x, l), oo = __shfl(x.y, l), dd = __shfl(x.z, l), ww = __shfl(w, l);
       if (lane == 0) {
-        if (skip) {
-          slot_dec[s0 + l] = -1;
+        if ((double)sz[dd] >= thr) {
+          slot_dec[vv - r0] = -1;
         } else {
           sz[oo] -= ww;
           sz[dd] += ww;
-          part[r0 + s0 + l] = dd;
+          part[vv] = dd;
+          nm++;
         }
       }
-      if (!skip) nm++;
-      wave_lds_sync();
     }
   }
+  wave_lds_sync();
   for (int q = lane; q < K; q += WAVE) sizes[q] = sz[q];
   if (lane == 0 && nm) atomicAdd(moves, nm);
 }
@@ -606,7 +616,7 @@ void launch_bits(fp_ctx* ctx) {
   auto& c = ctx->conn;
   const int K = ctx->K, W = (K + 63) / 64;
   fill_async<int>(ctx, c.msg, (size_t)K, 0);
-  k_conn_bits<<<dim3((unsigned)W, (unsigned)W), WAVE, 0, ctx->stream>>>(c.pc, K, W, c.B, c.BT, c.msg);
+  k_conn_bits<<<dim3((unsigned)W, (unsigned)W), 256, 0, ctx->stream>>>(c.pc, K, W, c.B, c.BT, c.msg);
 }
 
 }  // namespace
@@ -632,7 +642,7 @@ fp_status plan_conn(fp_ctx* ctx, RefinePlan& p) {
   auto& c = ctx->conn;
   fill_async<int>(ctx, c.pc, (size_t)K * (size_t)K, 0);
   const int ngrid = std::max(1, std::min((ctx->noe + 255) / 256, 4096));
-  k_conn_init<<<ngrid, 256, 0, ctx->stream>>>(ctx->net_ptr, ctx->ent, ctx->src, ctx->part, ctx->noe, ctx->pmask, K,
+  k_conn_init<<<std::min(ngrid * 8, 4096), 256, 0, ctx->stream>>>(ctx->net_ptr, ctx->ent, ctx->src, ctx->part, ctx->noe, ctx->pmask, K,
                                                c.pc);
   launch_bits(ctx);
   FP_HIP_OK(hipGetLastError());
@@ -712,8 +722,10 @@ fp_status pass_conn(fp_ctx* ctx, const RefinePlan& p) {
     ap.r1 = r0 + nslots;
     ap.net_blocks = REPLAY_NET_BLOCKS;
     if (!msm) {
+      k_round_compact<<<1, 1024, 0, ctx->stream>>>(ctx->slot_dec, ctx->slot_own, nullptr, ctx->vtx_ptr, r0, nslots,
+                                                    c.slot_mi, c.mv, c.mstat);
       k_tm_admit<<<1, WAVE, (size_t)K * sizeof(long long), ctx->stream>>>(
-          ctx->slot_dec, ctx->slot_own, ctx->vwgt, r0, nslots, K, thr, ctx->sizes, ctx->part, ctx->moves);
+          ctx->slot_dec, c.mv, c.mstat, ctx->vwgt, r0, K, thr, ctx->sizes, ctx->part, ctx->moves);
       ap.mode = 0;
       k_conn_apply<<<REPLAY_NET_BLOCKS, REPLAY_BLOCK, lds_r, ctx->stream>>>(ap);
     } else {
@@ -748,7 +760,7 @@ extern "C" fp_status fp_get_connections(fp_ctx* ctx, int* pc, int* msg) {
   auto& c = ctx->conn;
   fill_async<int>(ctx, c.pc, (size_t)K * (size_t)K, 0);
   const int ngrid = std::max(1, std::min((ctx->noe + 255) / 256, 4096));
-  k_conn_init<<<ngrid, 256, 0, ctx->stream>>>(ctx->net_ptr, ctx->ent, ctx->src, ctx->part, ctx->noe, ctx->pmask, K,
+  k_conn_init<<<std::min(ngrid * 8, 4096), 256, 0, ctx->stream>>>(ctx->net_ptr, ctx->ent, ctx->src, ctx->part, ctx->noe, ctx->pmask, K,
                                                c.pc);
   launch_bits(ctx);
   FP_HIP_OK(hipGetLastError());

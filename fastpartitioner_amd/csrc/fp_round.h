@@ -59,26 +59,46 @@ __global__ void __launch_bounds__(1024) k_round_maxset(const int* __restrict__ v
 
 // Accepted moves of the round in visiting order: slot_mi[slot] = move index
 // (or -1), mv[i] = (vertex, own part, new part, first event slot).
+constexpr int COMPACT_RUN = 16;  // consecutive slots per thread per step of k_round_compact
+
 __global__ void __launch_bounds__(1024) k_round_compact(const int* __restrict__ slot_dec, const int* __restrict__ slot_own,
                                                       const int* __restrict__ order,
                                                       const unsigned* __restrict__ vtx_ptr, int r0, int nslots,
                                                       int* __restrict__ slot_mi, int4* __restrict__ mv,
                                                       int* __restrict__ mstat) {
+  // each thread owns a run of COMPACT_RUN consecutive slots, so one step of
+  // the block covers 16K slots and the block-wide scans are few
   __shared__ int s_w[16];
   int base = 0, ebase = 0;
-  for (int s0 = 0; s0 < nslots; s0 += (int)blockDim.x) {
-    const int s = s0 + (int)threadIdx.x;
-    int f = 0, deg = 0, v = 0;
-    if (s < nslots && slot_dec[s] >= 0) {
-      f = 1;
-      v = order ? order[r0 + s] : r0 + s;
-      deg = (int)(vtx_ptr[v + 1] - vtx_ptr[v]);
+  const int step = (int)blockDim.x * COMPACT_RUN;
+  for (int s0 = 0; s0 < nslots; s0 += step) {
+    const int first = s0 + (int)threadIdx.x * COMPACT_RUN;
+    int f = 0, deg = 0;
+    for (int j = 0; j < COMPACT_RUN; j++) {
+      const int s = first + j;
+      if (s < nslots && slot_dec[s] >= 0) {
+        const int v = order ? order[r0 + s] : r0 + s;
+        f++;
+        deg += (int)(vtx_ptr[v + 1] - vtx_ptr[v]);
+      }
     }
     int tot, totd;
-    const int ex = block_excl_scan(f, s_w, &tot);
-    const int exd = block_excl_scan(deg, s_w, &totd);
-    if (s < nslots) slot_mi[s] = f ? base + ex : -1;
-    if (f) mv[base + ex] = make_int4(v, slot_own[s], slot_dec[s], ebase + exd);
+    int ex = block_excl_scan(f, s_w, &tot);
+    int exd = block_excl_scan(deg, s_w, &totd);
+    for (int j = 0; j < COMPACT_RUN; j++) {
+      const int s = first + j;
+      if (s >= nslots) break;
+      const int d = slot_dec[s];
+      if (d >= 0) {
+        const int v = order ? order[r0 + s] : r0 + s;
+        slot_mi[s] = base + ex;
+        mv[base + ex] = make_int4(v, slot_own[s], d, ebase + exd);
+        ex++;
+        exd += (int)(vtx_ptr[v + 1] - vtx_ptr[v]);
+      } else {
+        slot_mi[s] = -1;
+      }
+    }
     base += tot;
     ebase += totd;
   }

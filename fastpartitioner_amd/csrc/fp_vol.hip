@@ -32,22 +32,32 @@ constexpr int REPLAY_BLOCK = 256;
 constexpr int REPLAY_NET_BLOCKS = 2048;
 
 // per part: volume of every net whose source is in the part, lambda - 1 (plus,
-// for MSRV, one per other part of the net: what it receives)
-__global__ void k_vol_init(const unsigned* __restrict__ net_ptr, const unsigned* __restrict__ ent,
-                           const int* __restrict__ src, const int* __restrict__ part, int noe, unsigned pmask,
-                           int msrv, int* __restrict__ vol) {
-  for (int n = (int)(blockIdx.x * blockDim.x + threadIdx.x); n < noe; n += (int)(gridDim.x * blockDim.x)) {
+// for MSRV, one per other part of the net: what it receives); one wavefront
+// per net, block-private sums in LDS
+__global__ void __launch_bounds__(256) k_vol_init(const unsigned* __restrict__ net_ptr,
+                                                  const unsigned* __restrict__ ent, const int* __restrict__ src,
+                                                  const int* __restrict__ part, int noe, unsigned pmask, int K,
+                                                  int msrv, int* __restrict__ vol) {
+  extern __shared__ int s_vol[];
+  const int lane = lane_id();
+  for (int q = (int)threadIdx.x; q < K; q += (int)blockDim.x) s_vol[q] = 0;
+  __syncthreads();
+  const int nw = (int)(gridDim.x * blockDim.x / WAVE);
+  for (int n = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE); n < noe; n += nw) {
     const unsigned seg = net_ptr[n] + (unsigned)n;
     const int L = (int)ent[seg];
     if (L == 0) continue;
     const int sp = part[src[n]];
-    if (L > 1) atomicAdd(&vol[sp], L - 1);
+    if (L > 1 && lane == 0) atomicAdd(&s_vol[sp], L - 1);
     if (msrv)
-      for (int t = 0; t < L; t++) {
+      for (int t = lane; t < L; t += WAVE) {
         const int q = (int)(ent[seg + 1 + (unsigned)t] & pmask);
-        if (q != sp) atomicAdd(&vol[q], 1);
+        if (q != sp) atomicAdd(&s_vol[q], 1);
       }
   }
+  __syncthreads();
+  for (int q = (int)threadIdx.x; q < K; q += (int)blockDim.x)
+    if (s_vol[q]) atomicAdd(&vol[q], s_vol[q]);
 }
 
 struct VolEvalArgs {
@@ -512,8 +522,8 @@ fp_status plan_vol(fp_ctx* ctx, RefinePlan& p) {
   auto& vs = ctx->vol;
   fill_async<int>(ctx, vs.vol, (size_t)K, 0);
   const int ngrid = std::max(1, std::min((ctx->noe + 255) / 256, 4096));
-  k_vol_init<<<ngrid, 256, 0, ctx->stream>>>(ctx->net_ptr, ctx->ent, ctx->src, ctx->part, ctx->noe, ctx->pmask,
-                                              msrv ? 1 : 0, vs.vol);
+  k_vol_init<<<std::min(ngrid * 8, 1024), 256, (size_t)K * sizeof(int), ctx->stream>>>(
+      ctx->net_ptr, ctx->ent, ctx->src, ctx->part, ctx->noe, ctx->pmask, K, msrv ? 1 : 0, vs.vol);
   FP_HIP_OK(hipGetLastError());
   return FP_OK;
 }

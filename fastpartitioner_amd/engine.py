@@ -66,6 +66,7 @@ def declare(lib) -> None:
     lib.fp_batch_last_error.restype = c.c_char_p
     lib.fp_batch_add.argtypes = [P, P, c.c_int, c.POINTER(Schedule)]
     lib.fp_batch_size.argtypes = [P]
+    lib.fp_batch_set_lanes.argtypes = [P, c.c_int]
     lib.fp_batch_refine.argtypes = [P, c.c_int, c.c_int, c.POINTER(c.c_double)]
     lib.fp_batch_member.argtypes = [P, c.c_int, c.POINTER(c.c_longlong), c.POINTER(c.c_int), c.POINTER(c.c_int),
                                     c.POINTER(c.c_double)]
@@ -224,6 +225,9 @@ class Batch:
         self._check(self.lib.fp_batch_add(self.b, eng.ctx, METRICS[metric],
                                           ctypes.byref(sched) if sched is not None else None), "fp_batch_add")
         self.members.append(eng)
+
+    def set_lanes(self, lanes: int):
+        self._check(self.lib.fp_batch_set_lanes(self.b, lanes), "fp_batch_set_lanes")
 
     def refine(self, passes: int, tune_passes: int = 0) -> float:
         ms = ctypes.c_double()

@@ -182,11 +182,12 @@ fp_status fp_get_connections(fp_ctx* ctx, int* pc, int* msg);
 
 /* ---- batched refinement ------------------------------------------------- */
 /* A batch refines many resident hypergraphs of uneven size, K and objective
- * together. Each member's pass is captured once into a HIP graph; all members'
- * pass graphs are children of one batch graph, launched once per pass, so the
- * members' work overlaps on the device. Every member ends exactly where
- * fp_refine(ctx, metric, passes, sched) alone would leave it. The reference
- * has no counterpart (it refines one hypergraph per process). */
+ * together. Members are spread over lanes (streams) by size; each lane's
+ * members' passes are captured once into one HIP graph, and a batch pass is
+ * one launch of each lane graph, the lanes running side by side. Every member
+ * ends exactly where fp_refine(ctx, metric, passes, sched) alone would leave
+ * it. The reference has no counterpart (it refines one hypergraph per
+ * process). */
 typedef struct fp_batch fp_batch;
 fp_status fp_batch_create(fp_batch** batch, int device);
 void fp_batch_destroy(fp_batch* batch);
@@ -196,6 +197,9 @@ const char* fp_batch_last_error(const fp_batch* batch);
  * the context; it must outlive the batch's use of it. */
 fp_status fp_batch_add(fp_batch* batch, fp_ctx* ctx, fp_metric metric, const fp_schedule* sched);
 int fp_batch_size(const fp_batch* batch);
+/* Number of lanes (1..64, default 8). More lanes than the runtime's hardware
+ * queues (GPU_MAX_HW_QUEUES, 4 by default) share queues. */
+fp_status fp_batch_set_lanes(fp_batch* batch, int lanes);
 /* `passes` passes of every member. The first tune_passes of them run eagerly
  * with a different candidate launch shape each (wavefronts per evaluation
  * block, evaluation grid cap), timed per member; each member then keeps its
