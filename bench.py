@@ -207,6 +207,124 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def run_serve(args):
+    """The serving loop under Poisson arrivals: resident hypergraphs, requests
+    with mixed K and objective admitted by HBM footprint; p50/p99 latency
+    (arrival to completion) and completed hypergraphs/s. A sample of requests
+    is rerun alone to check that results do not depend on the mix."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+
+    os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
+    from fastpartitioner_amd import hypergraph as H
+    from fastpartitioner_amd.engine import Engine, Server
+
+    eps, Ks = 0.05, (32, 128, 512)
+    rng = np.random.default_rng(args.seed)
+    sizes = [int(np.floor(np.exp(rng.uniform(np.log(5e3), np.log(1e6))))) for _ in range(args.graphs)]
+
+    def prep_graph(t):
+        return H.colnet(sizes[t], H.DEG_ZIPF, 2, 5000, 2.3, seed=2000 + t)
+
+    t = time.time()
+    workers = min(16, len(os.sched_getaffinity(0)))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        graphs = list(ex.map(prep_graph, range(args.graphs)))
+
+    def prep_part(gk):
+        g, K = gk
+        hg = graphs[g]
+        C, max_pw, min_pw = H.ref_constants(hg.nov, K, eps)
+        return H.initial_partition(hg, K, C, max_pw, min_pw)
+
+    keys = [(g, K) for g in range(args.graphs) for K in Ks]
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        parts = dict(zip(keys, ex.map(prep_part, keys)))
+    # arrivals: Poisson process at args.rate requests/s for args.duration s
+    arrivals = []
+    at = rng.exponential(1.0 / args.rate)
+    while at < args.duration:
+        g = int(rng.integers(args.graphs))
+        K = int(rng.choice(Ks))
+        metric = OBJECTIVES[int(rng.integers(6))]
+        arrivals.append((at, g, K, metric))
+        at += rng.exponential(1.0 / args.rate)
+    pins = [hg.pins for hg in graphs]
+    print(f"# serve: {args.graphs} hypergraphs, {sum(hg.nov for hg in graphs)} vertices, {sum(pins)} pins, "
+          f"{len(arrivals)} requests over {args.duration}s, prep {time.time() - t:.1f}s", file=sys.stderr, flush=True)
+
+    def sched_of(g, K, metric):
+        hg = graphs[g]
+        C, max_pw, min_pw = H.ref_constants(hg.nov, K, eps)
+        return batch_schedule(metric, hg.nov, K, C, max_pw, min_pw, H.balance_cap(hg.nov, K, eps))
+
+    with Server(0, hbm_budget=args.budget_gib << 30, workers=args.workers) as srv:
+        for hg in graphs:
+            srv.add_graph(hg)
+        rids = [None] * len(arrivals)
+        scheds = [sched_of(g, K, metric) for _, g, K, metric in arrivals]
+        start = time.perf_counter()
+
+        def submitter():
+            last = 0.0
+            for i, (a, g, K, metric) in enumerate(arrivals):
+                delay = start + a - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                rids[i] = srv.submit(g, metric, K, parts[(g, K)], scheds[i], args.passes)
+                if a - last >= 10:
+                    last = a
+                    print(f"# serve: {i + 1} submitted at {a:.0f}s", file=sys.stderr, flush=True)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        infos = []
+        for i in range(len(arrivals)):
+            while rids[i] is None:
+                time.sleep(0.001)
+            info, part = srv.wait(rids[i], with_part=i < args.check)
+            infos.append((info, part))
+        th.join()
+        stats = srv.stats()
+    lat = np.array([info["end_ms"] - info["submit_ms"] for info, _ in infos])
+    first = min(info["submit_ms"] for info, _ in infos)
+    last_end = max(info["end_ms"] for info, _ in infos)
+    span_s = (last_end - first) / 1e3
+    # the first requests rerun alone on fresh contexts
+    same = True
+    for (info, part), (a, g, K, metric) in zip(infos[:args.check], arrivals[:args.check]):
+        with Engine(0) as e:
+            e.upload(graphs[g])
+            e.set_partition(K, parts[(g, K)])
+            e.refine(metric, args.passes, sched_of(g, K, metric))
+            same = same and np.array_equal(e.get_state().part, part) and e.metrics() == info["metrics"]
+    out = {
+        "metric": "serving hypergraphs/s and latency (Poisson arrivals, mixed K and objective)",
+        "value": round(len(infos) / span_s, 2),
+        "unit": "hypergraphs/s",
+        "n_gpus": 1,
+        "steps": args.passes,
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"serve-{args.graphs}-colnet-zipf2.3-logU5e3to1e6-poisson{args.rate}",
+                   "requests": len(infos), "rate_per_s": args.rate, "duration_s": args.duration,
+                   "passes": args.passes, "K": list(Ks), "workers": args.workers, "budget_gib": args.budget_gib,
+                   "hw_queues": args.hw_queues, "resident_pins": int(sum(pins))},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2), "p99": round(float(np.percentile(lat, 99)), 2),
+                       "mean": round(float(lat.mean()), 2), "max": round(float(lat.max()), 2)},
+        "peak_inflight": stats["peak_inflight"],
+        "peak_inflight_gib": round(stats["peak_bytes"] / 2**30, 3),
+        "same_alone_and_in_mix": bool(same),
+        "checked_requests": args.check,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,14 +336,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU baseline")
     ap.add_argument("--cpu-passes", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--workload", choices=("tv1m", "batch"), default="tv1m",
-                    help="tv1m: the 1M-vertex TV refinement (default); batch: batched vs one-at-a-time refinement")
+    ap.add_argument("--workload", choices=("tv1m", "batch", "serve"), default="tv1m",
+                    help="tv1m: the 1M-vertex TV refinement (default); batch: batched vs one-at-a-time "
+                         "refinement; serve: the serving loop under Poisson arrivals")
     ap.add_argument("--instances", type=int, default=512, help="batch: number of hypergraphs")
     ap.add_argument("--passes", type=int, default=10, help="batch: passes per hypergraph")
     ap.add_argument("--tune-passes", type=int, default=3, help="batch: passes that double as autotuner trials")
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--lanes", type=int, default=16, help="batch: streams the members are spread over")
-    ap.add_argument("--hw-queues", type=int, default=16, help="batch: hardware queues of the process")
+    ap.add_argument("--hw-queues", type=int, default=16, help="batch/serve: hardware queues of the process")
+    ap.add_argument("--graphs", type=int, default=32, help="serve: resident hypergraphs")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second")
+    ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
+    ap.add_argument("--workers", type=int, default=16, help="serve: worker threads")
+    ap.add_argument("--budget-gib", type=int, default=64, help="serve: HBM budget of the requests in flight")
+    ap.add_argument("--check", type=int, default=16, help="serve: requests rerun alone for the mix check")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -248,6 +373,9 @@ def main():
 
     if args.workload == "batch":
         run_batch(args)
+        return
+    if args.workload == "serve":
+        run_serve(args)
         return
 
     from fastpartitioner_amd import hypergraph as H

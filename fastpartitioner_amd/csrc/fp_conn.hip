@@ -528,7 +528,10 @@ __global__ void __launch_bounds__(WAVE) k_msm_scan(int* __restrict__ pc, int* __
         if (e.y != 0) {
           const int old = atomicAdd(&pc[e.x], e.y);
           const int cross = (e.y > 0 && old == 0) ? 1 : ((e.y < 0 && old == 1) ? -1 : 0);
-          if (cross) atomicAdd(&delta[e.x / K], cross);
+          if (cross) {
+            atomicAdd(&delta[e.x / K], cross);
+            atomicAdd(&msg[e.x / K], cross);
+          }
         }
       }
     }

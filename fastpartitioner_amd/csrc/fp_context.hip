@@ -59,21 +59,93 @@ fp_status ensure_k(fp_ctx* ctx, int K) {
   if ((unsigned long long)ctx->max_net >= (1ull << (32 - pb)))
     return fail(ctx, FP_ERR_UNSUPPORTED, "a net of %u pins is too large for K = %d in the packed list layout",
                 ctx->max_net, K);
-  if (ctx->K != K) {
-    // K-sized arrays are (re)allocated; earlier ones stay owned until the next upload
+  if (K > ctx->kcap) {
+    // K-sized arrays grow only; a smaller K reuses them
     FP_TRY(dalloc(ctx, &ctx->sizes, (size_t)K));
     FP_TRY(dalloc(ctx, &ctx->sizes_frozen, (size_t)K));
     FP_TRY(dalloc(ctx, &ctx->cand_tot, (size_t)K));
     FP_TRY(dalloc(ctx, &ctx->over_idx, (size_t)K));
     FP_TRY(dalloc(ctx, &ctx->p2p, ((size_t)K * (size_t)K + 31) / 32));
     FP_TRY(dalloc(ctx, &ctx->mvec, (size_t)5 * (size_t)K + 2));
+    ctx->kcap = K;
+  }
+  if (ctx->K != K) {
+    // the BAL histogram is all zero between rounds whatever its shape; the
+    // candidate totals restart from zero
     fill_async<long long>(ctx, ctx->cand_tot, (size_t)K, 0);
     ctx->K = K;
-    ctx->hist_cap = 0;  // hist is K x tiles
   }
   ctx->pb = pb;
   ctx->pmask = (1u << pb) - 1u;
   return FP_OK;
+}
+
+constexpr int BUILD_KMAX = 4096;  // k_build_lists keeps four K-int arrays per wavefront in LDS
+
+// One wavefront per net: the net's connectivity list from the part vector,
+// in first-appearance order along the ordering (the state the reference's
+// streaming initial partition leaves, TV/partition.cpp:240-263). The pins are
+// visited in rank order (apply_pins holds each net's pin ranks ascending), 64
+// at a time: the first lane of each part not yet listed appends it, then every
+// lane counts its pin. Writes the packed segment and its evaluation view.
+__global__ void __launch_bounds__(256) k_build_lists(const unsigned* __restrict__ net_ptr,
+                                                     const int* __restrict__ apply_pins, const int* __restrict__ order,
+                                                     const int* __restrict__ part, int noe, int K, int pb,
+                                                     unsigned pmask, unsigned* __restrict__ ent,
+                                                     unsigned short* __restrict__ ev) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
+  int* slot = reinterpret_cast<int*>(lds) + (size_t)wave * 4 * (size_t)K;  // list position of a part, -1
+  int* claim = slot + K;                                                   // first lane of a new part, 64
+  int* lcnt = claim + K;                                                   // pin count per list position
+  int* lq = lcnt + K;                                                      // part per list position
+  for (int q = lane; q < K; q += WAVE) {
+    slot[q] = -1;
+    claim[q] = WAVE;
+  }
+  wave_lds_sync();
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  for (int n = (int)blockIdx.x * nw + wave; n < noe; n += (int)gridDim.x * nw) {
+    const unsigned s = net_ptr[n], e = net_ptr[n + 1];
+    int L = 0;
+    for (unsigned k0 = s; k0 < e; k0 += WAVE) {
+      const unsigned k = k0 + (unsigned)lane;
+      const bool ok = k < e;
+      int q = 0;
+      if (ok) {
+        const int r = apply_pins[k];
+        q = part[order ? order[r] : r];
+      }
+      const bool fresh = ok && slot[q] < 0;
+      if (fresh) atomicMin(&claim[q], lane);
+      wave_lds_sync();
+      const bool first = fresh && claim[q] == lane;
+      const unsigned long long bm = __ballot(first);
+      if (first) {
+        const int pos = L + __popcll(bm & lt);
+        slot[q] = pos;
+        lcnt[pos] = 0;
+        lq[pos] = q;
+        claim[q] = WAVE;
+      }
+      L += __popcll(bm);
+      wave_lds_sync();
+      if (ok) atomicAdd(&lcnt[slot[q]], 1);
+      wave_lds_sync();
+    }
+    const unsigned seg = s + (unsigned)n;
+    for (int t = lane; t < L; t += WAVE) {
+      const unsigned x = ((unsigned)lcnt[t] << pb) | (unsigned)lq[t];
+      ent[seg + 1 + (unsigned)t] = x;
+      ev[seg + 1 + (unsigned)t] = ev_word(x, pb, pmask);
+      slot[lq[t]] = -1;
+    }
+    if (lane == 0) {
+      ent[seg] = (unsigned)L;
+      ev[seg] = (unsigned short)L;
+    }
+    wave_lds_sync();
+  }
 }
 
 // ent: the segment array (lambda + list per net), host copy of the device layout
@@ -181,6 +253,7 @@ fp_status fp_upload(fp_ctx* ctx, int nov, int noe, const unsigned* net_ptr, cons
   dfree_all(ctx);
   ctx->has_graph = ctx->has_state = false;
   ctx->K = 0;
+  ctx->kcap = 0;
   ctx->rmax = 0;
   ctx->hist_cap = 0;
   ctx->cn.ready = false;
@@ -361,6 +434,22 @@ fp_status fp_set_partition(fp_ctx* ctx, int K, const int* part) {
     if (part[v] < 0 || part[v] >= K) return fail(ctx, FP_ERR_ARG, "part[%d] = %d out of [0, K)", v, part[v]);
   FP_TRY(ensure_k(ctx, K));
   const int noe = ctx->noe;
+  if (K <= BUILD_KMAX) {
+    // built on the device: only the part vector and the part sizes travel
+    std::vector<long long> sz((size_t)K, 0);
+    for (int v = 0; v < ctx->nov; v++) sz[(size_t)part[v]] += ctx->h_vwgt.empty() ? 1 : ctx->h_vwgt[(size_t)v];
+    FP_TRY(h2d(ctx, ctx->part, part, (size_t)ctx->nov));
+    FP_TRY(h2d(ctx, ctx->sizes, sz.data(), (size_t)K));
+    int waves = 4;
+    while (waves > 1 && (size_t)waves * 16 * (size_t)K > (size_t)LDS_BUDGET) waves--;
+    const int grid = std::max(1, std::min((noe + waves - 1) / waves, 8192));
+    k_build_lists<<<grid, waves * WAVE, (size_t)waves * 16 * (size_t)K, ctx->stream>>>(
+        ctx->net_ptr, ctx->apply_pins, ctx->order, ctx->part, noe, K, ctx->pb, ctx->pmask, ctx->ent, ctx->ev);
+    FP_HIP_OK(hipGetLastError());
+    FP_HIP_OK(hipStreamSynchronize(ctx->stream));
+    ctx->has_state = true;
+    return FP_OK;
+  }
   std::vector<unsigned> ent((size_t)ctx->pins + (size_t)noe + ENT_PAD, 0);
   const unsigned inc = 1u << ctx->pb;
   // list order = first appearance of each part along the ordering (the state

@@ -252,6 +252,7 @@ struct fp_ctx {
 
   // state
   int K = 0, pb = 0;
+  int kcap = 0;  // capacity of the K-sized arrays
   unsigned pmask = 0;
   int* part = nullptr;
   unsigned* ent = nullptr;

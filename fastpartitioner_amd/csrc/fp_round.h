@@ -109,17 +109,35 @@ __global__ void __launch_bounds__(1024) k_round_compact(const int* __restrict__ 
 }
 
 // One move's step of the max-set bookkeeping, run by a single wavefront.
-// delta[p] (LDS) holds the move's change of val[p] for every part p that
-// item(it), it in [0, nit), names (duplicates allowed, -1 = no part); claim[]
-// is all INT_MAX on entry. Commits val[p] += delta[p], updates the max M and
-// its multiplicity cntM, clears delta/claim of the items, and returns true
-// iff the new max set holds a part the previous one did not. Cases: the max
-// rose (the new set is the affected parts at the new value); it stayed (the
-// untouched parts at M plus affected parts landing on it); it fell (full
-// reduction over val).
+// val[] (LDS) already holds the move's changes; delta[p] holds the change of
+// val[p] for every part p that item(it), it in [0, nit), names (duplicates
+// allowed, -1 = no part); claim[] is all INT_MAX on entry. Updates the max M
+// and its multiplicity cntM, clears delta/claim of the items, and returns true
+// iff the new max set holds a part the previous one did not. A move that
+// leaves every affected part off the max and below it changes nothing (the
+// common case, decided with one read per item). Otherwise: the max rose (the
+// new set is the affected parts at the new value); it stayed (the untouched
+// parts at M plus affected parts landing on it); or it fell (full reduction).
 template <class Item>
 __device__ bool maxset_step(int* val, int* delta, int* claim, int K, int nit, Item item, int& M, int& cntM) {
   const int lane = lane_id();
+  bool touch = false;
+  for (int b = 0; b < nit; b += WAVE) {
+    const int p = item(b + lane);
+    if (p >= 0) {
+      const int nw = val[p], old = nw - delta[p];
+      touch = touch || old == M || nw >= M;
+    }
+  }
+  if (__ballot(touch) == 0) {
+    wave_lds_sync();
+    for (int b = 0; b < nit; b += WAVE) {
+      const int p = item(b + lane);
+      if (p >= 0) delta[p] = 0;
+    }
+    wave_lds_sync();
+    return false;
+  }
   for (int b = 0; b < nit; b += WAVE) {
     const int p = item(b + lane);
     if (p >= 0) atomicMin(&claim[p], b + lane);
@@ -132,7 +150,7 @@ __device__ bool maxset_step(int* val, int* delta, int* claim, int K, int nit, It
     const int it = b + lane;
     const int p = item(it);
     if (p >= 0 && claim[p] == it) {
-      const int old = val[p], nw = old + delta[p];
+      const int nw = val[p], old = nw - delta[p];
       up = max(up, nw);
       inS += old == M;
       newM += nw == M;
@@ -146,13 +164,6 @@ __device__ bool maxset_step(int* val, int* delta, int* claim, int K, int nit, It
   inS = wave_sum(inS);
   newM = wave_sum(newM);
   enterM = __ballot(enterM) != 0;
-  wave_lds_sync();
-  for (int b = 0; b < nit; b += WAVE) {
-    const int it = b + lane;
-    const int p = item(it);
-    if (p >= 0 && claim[p] == it) val[p] += delta[p];
-  }
-  wave_lds_sync();
   bool stop;
   int nM, nC;
   if (up > M) {

@@ -407,13 +407,18 @@ __global__ void __launch_bounds__(REPLAY_BLOCK) k_vol_replay(VolReplayArgs a) {
 
 // One wavefront: the round's moves in order, each followed by the max-set
 // update of MGV/partition.cpp:977-1003; stops after the first move whose max
-// set holds a part the previous max set did not. Writes back vol, the max and
-// its multiplicity, and the number of moves to commit (mstat[3]).
+// set holds a part the previous max set did not. The events of consecutive
+// moves are contiguous in evbuf, so they are staged into LDS a batch of moves
+// at a time (stage_cap events); a move with more events than fit is read from
+// HBM. Writes back vol, the max and its multiplicity, and the number of moves
+// to commit (mstat[3]).
 __global__ void __launch_bounds__(WAVE) k_vol_scan(int* __restrict__ vol_g, int K, const int4* __restrict__ mv,
                                                    const int4* __restrict__ evbuf,
-                                                   const unsigned* __restrict__ vtx_ptr, int* __restrict__ mstat) {
+                                                   const unsigned* __restrict__ vtx_ptr, int* __restrict__ mstat,
+                                                   int stage_cap) {
   extern __shared__ __align__(16) unsigned char lds[];
-  int* vol = reinterpret_cast<int*>(lds);
+  int4* stage = reinterpret_cast<int4*>(lds);
+  int* vol = reinterpret_cast<int*>(stage + stage_cap);
   int* delta = vol + K;
   int* claim = delta + K;
   const int lane = lane_id();
@@ -426,38 +431,62 @@ __global__ void __launch_bounds__(WAVE) k_vol_scan(int* __restrict__ vol_g, int 
   int M = mstat[0], cntM = mstat[1];
   const int nmoves = mstat[2];
   int nstop = nmoves;
-  for (int mi = 0; mi < nmoves; mi++) {
-    const int4 m = mv[mi];
-    const int own = m.y, d = m.z;
-    const unsigned eb = (unsigned)m.w;
-    const int deg = (int)(vtx_ptr[m.x + 1] - vtx_ptr[m.x]);
-    int s_own = 0, s_dec = 0;
-    for (int b = 0; b < deg; b += WAVE) {
-      const int i = b + lane;
-      if (i < deg) {
-        const int4 e = evbuf[eb + (unsigned)i];
-        s_own += e.y;
-        s_dec += e.z;
-        if (e.x >= 0 && e.w != 0) atomicAdd(&delta[e.x], e.w);
+  bool stopped = false;
+  for (int mi = 0; mi < nmoves && !stopped;) {
+    // headers of the next (up to) 64 moves; the leading ones whose events fit the stage
+    const int4 hm = mi + lane < nmoves ? mv[mi + lane] : make_int4(0, 0, 0, 0);
+    const int hdeg = mi + lane < nmoves ? (int)(vtx_ptr[hm.x + 1] - vtx_ptr[hm.x]) : 0;
+    const int cum = wave_incl_scan(hdeg);
+    int nb = __popcll(__ballot(mi + lane < nmoves && cum <= stage_cap));
+    const bool staged = nb > 0;
+    if (!staged) nb = 1;
+    const unsigned eb0 = (unsigned)__builtin_amdgcn_readfirstlane(hm.w);
+    if (staged) {
+      const int tot = __builtin_amdgcn_readlane(cum, nb - 1);
+      for (int t = lane; t < tot; t += WAVE) stage[t] = evbuf[eb0 + (unsigned)t];
+      wave_lds_sync();
+    }
+    for (int j = 0; j < nb; j++) {
+      const int own = __builtin_amdgcn_readlane(hm.y, j), d = __builtin_amdgcn_readlane(hm.z, j);
+      const unsigned eb = (unsigned)__builtin_amdgcn_readlane(hm.w, j);
+      const int deg = __builtin_amdgcn_readlane(hdeg, j);
+      const int4* evs = staged ? stage + (eb - eb0) : evbuf + eb;
+      int s_own = 0, s_dec = 0;
+      for (int b = 0; b < deg; b += WAVE) {
+        const int i = b + lane;
+        if (i < deg) {
+          const int4 e = evs[i];
+          s_own += e.y;
+          s_dec += e.z;
+          if (e.x >= 0 && e.w != 0) {
+            atomicAdd(&delta[e.x], e.w);
+            atomicAdd(&vol[e.x], e.w);
+          }
+        }
+      }
+      s_own = wave_sum(s_own);
+      s_dec = wave_sum(s_dec);
+      if (lane == 0) {
+        atomicAdd(&delta[own], s_own);
+        atomicAdd(&vol[own], s_own);
+        atomicAdd(&delta[d], s_dec);
+        atomicAdd(&vol[d], s_dec);
+      }
+      wave_lds_sync();
+      // affected parts: 0 -> own, 1 -> d, 2 + i -> source part recorded for net i (or -1)
+      auto item = [&](int it) -> int {
+        if (it == 0) return own;
+        if (it == 1) return d;
+        return it < deg + 2 ? evs[it - 2].x : -1;
+      };
+      if (maxset_step(vol, delta, claim, K, deg + 2, item, M, cntM)) {
+        nstop = mi + j + 1;
+        stopped = true;
+        break;
       }
     }
-    s_own = wave_sum(s_own);
-    s_dec = wave_sum(s_dec);
-    if (lane == 0) {
-      atomicAdd(&delta[own], s_own);
-      atomicAdd(&delta[d], s_dec);
-    }
+    mi += nb;
     wave_lds_sync();
-    // affected parts: 0 -> own, 1 -> d, 2 + i -> source part recorded for net i (or -1)
-    auto item = [&](int it) -> int {
-      if (it == 0) return own;
-      if (it == 1) return d;
-      return it < deg + 2 ? evbuf[eb + (unsigned)(it - 2)].x : -1;
-    };
-    if (maxset_step(vol, delta, claim, K, deg + 2, item, M, cntM)) {
-      nstop = mi + 1;
-      break;
-    }
   }
   for (int q = lane; q < K; q += WAVE) vol_g[q] = vol[q];
   if (lane == 0) {
@@ -533,7 +562,9 @@ fp_status pass_vol(fp_ctx* ctx, const RefinePlan& p) {
   const int nov = ctx->nov, K = ctx->K, R = p.R, waves = p.eval_waves;
   const size_t lds_e = (size_t)K * 16 + (size_t)waves * 8 * (size_t)K;
   const size_t lds_r = (size_t)(REPLAY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
-  const size_t lds_s = (size_t)3 * (size_t)K * sizeof(int);
+  const size_t lds_vol = (size_t)3 * (size_t)K * sizeof(int);
+  const int stage_cap = (int)std::min<size_t>(4096, ((size_t)LDS_BUDGET - lds_vol) / sizeof(int4));
+  const size_t lds_s = lds_vol + (size_t)stage_cap * sizeof(int4);
   auto& vs = ctx->vol;
   fill_async<unsigned>(ctx, ctx->net_stamp, (size_t)std::max(ctx->noe, 1), 0u);
   fill_async<int>(ctx, ctx->counters, 4, 0);
@@ -599,7 +630,7 @@ fp_status pass_vol(fp_ctx* ctx, const RefinePlan& p) {
     ra.net_blocks = REPLAY_NET_BLOCKS;
     ra.commit = 0;
     k_vol_replay<<<REPLAY_NET_BLOCKS, REPLAY_BLOCK, lds_r, ctx->stream>>>(ra);
-    k_vol_scan<<<1, WAVE, lds_s, ctx->stream>>>(vs.vol, K, vs.mv, vs.evbuf, ctx->vtx_ptr, vs.mstat);
+    k_vol_scan<<<1, WAVE, lds_s, ctx->stream>>>(vs.vol, K, vs.mv, vs.evbuf, ctx->vtx_ptr, vs.mstat, stage_cap);
     ra.commit = 1;
     k_vol_replay<<<REPLAY_NET_BLOCKS + (nslots + REPLAY_BLOCK - 1) / REPLAY_BLOCK, REPLAY_BLOCK, lds_r,
                    ctx->stream>>>(ra);

@@ -38,6 +38,13 @@ class Metrics(ctypes.Structure):
     _fields_ = [(k, ctypes.c_longlong) for k in ("tv", "cn", "tm", "msm", "msv", "msrv")]
 
 
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_longlong), ("graph", ctypes.c_int), ("metric", ctypes.c_int), ("K", ctypes.c_int),
+                ("passes", ctypes.c_int), ("status", ctypes.c_int), ("moves", ctypes.c_longlong),
+                ("metrics", Metrics), ("submit_ms", ctypes.c_double), ("start_ms", ctypes.c_double),
+                ("end_ms", ctypes.c_double), ("footprint", ctypes.c_longlong)]
+
+
 def declare(lib) -> None:
     c = ctypes
     P = c.c_void_p
@@ -70,6 +77,16 @@ def declare(lib) -> None:
     lib.fp_batch_refine.argtypes = [P, c.c_int, c.c_int, c.POINTER(c.c_double)]
     lib.fp_batch_member.argtypes = [P, c.c_int, c.POINTER(c.c_longlong), c.POINTER(c.c_int), c.POINTER(c.c_int),
                                     c.POINTER(c.c_double)]
+    lib.fp_server_create.argtypes = [c.POINTER(P), c.c_int, c.c_longlong, c.c_int]
+    lib.fp_server_destroy.argtypes = [P]
+    lib.fp_server_destroy.restype = None
+    lib.fp_server_last_error.argtypes = [P]
+    lib.fp_server_last_error.restype = c.c_char_p
+    lib.fp_server_add_graph.argtypes = [P, c.c_int, c.c_int, _U32P, _U32P, _I32P, _I32P, c.POINTER(c.c_int)]
+    lib.fp_server_submit.argtypes = [P, c.c_int, c.c_int, c.c_int, _I32P, c.POINTER(Schedule), c.c_int,
+                                     c.POINTER(c.c_longlong)]
+    lib.fp_server_wait.argtypes = [P, c.c_longlong, c.POINTER(RequestInfo), _I32P]
+    lib.fp_server_stats.argtypes = [P, c.POINTER(c.c_longlong), c.POINTER(c.c_int), c.POINTER(c.c_longlong)]
     lib.fp_set_kernel_timing.argtypes = [P, c.c_int]
     lib.fp_last_kernel_timing.argtypes = [P, c.POINTER(c.c_longlong), c.POINTER(c.c_double)]
 
@@ -244,6 +261,76 @@ class Batch:
         if self.b:
             self.lib.fp_batch_destroy(self.b)
             self.b = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+METRIC_NAMES = {v: k for k, v in METRICS.items()}
+
+
+class Server:
+    """Serving loop over resident hypergraphs (fp_server_*): submit requests,
+    wait for them; workers refine admitted requests concurrently."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, workers: int = 8):
+        self.lib = _native.engine()
+        declare(self.lib)
+        self.s = ctypes.c_void_p()
+        _native.check(self.lib.fp_server_create(ctypes.byref(self.s), device, hbm_budget, workers), "fp_server_create")
+        self.graphs = []
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise _native.FastPartError(rc, what, self.lib.fp_server_last_error(self.s).decode())
+
+    def add_graph(self, hg: Hypergraph) -> int:
+        gid = ctypes.c_int()
+        self._check(self.lib.fp_server_add_graph(self.s, hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.net_pins, _U32P),
+                                                 _p(hg.src, _I32P), _p(hg.vwgt, _I32P), ctypes.byref(gid)),
+                    "fp_server_add_graph")
+        self.graphs.append(hg)
+        return gid.value
+
+    def submit(self, graph: int, metric: str, K: int, part, sched: Schedule | None, passes: int) -> int:
+        part = np.ascontiguousarray(part, dtype=np.int32)
+        rid = ctypes.c_longlong()
+        self._check(self.lib.fp_server_submit(self.s, graph, METRICS[metric], K, _p(part, _I32P),
+                                              ctypes.byref(sched) if sched is not None else None, passes,
+                                              ctypes.byref(rid)), "fp_server_submit")
+        return rid.value
+
+    def wait(self, rid: int, with_part: bool = True):
+        """(info dict, final part vector or None); raises on a failed request."""
+        info = RequestInfo()
+        part = None
+        # the hypergraph is known only from the info; allocate for the largest
+        if with_part:
+            part = np.zeros(max(g.nov for g in self.graphs), np.int32)
+        rc = self.lib.fp_server_wait(self.s, rid, ctypes.byref(info), _p(part, _I32P))
+        self._check(rc, f"request {rid}")
+        m = info.metrics
+        d = {"id": info.id, "graph": info.graph, "metric": METRIC_NAMES[info.metric], "K": info.K,
+             "passes": info.passes, "moves": info.moves, "submit_ms": info.submit_ms, "start_ms": info.start_ms,
+             "end_ms": info.end_ms, "footprint": info.footprint,
+             "metrics": {"TV": m.tv, "CN": m.cn, "TM": m.tm, "MSM": m.msm, "MSV": m.msv, "MSRV": m.msrv}}
+        if with_part:
+            part = part[:self.graphs[info.graph].nov].copy()
+        return d, part
+
+    def stats(self):
+        c, p, b = ctypes.c_longlong(), ctypes.c_int(), ctypes.c_longlong()
+        self._check(self.lib.fp_server_stats(self.s, ctypes.byref(c), ctypes.byref(p), ctypes.byref(b)),
+                    "fp_server_stats")
+        return {"completed": c.value, "peak_inflight": p.value, "peak_bytes": b.value}
+
+    def close(self):
+        if self.s:
+            self.lib.fp_server_destroy(self.s)
+            self.s = ctypes.c_void_p()
 
     def __enter__(self):
         return self

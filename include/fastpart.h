@@ -211,6 +211,46 @@ fp_status fp_batch_refine(fp_batch* batch, int passes, int tune_passes, double* 
 fp_status fp_batch_member(const fp_batch* batch, int i, long long* moves, int* eval_waves, int* eval_grid,
                           double* tuned_pass_ms);
 
+/* ---- serving ------------------------------------------------------------ */
+/* A request loop over resident hypergraphs. Each registered hypergraph is
+ * uploaded once into a context of its own (CSR views and connectivity state
+ * stay in HBM). Worker threads take requests in arrival order, skipping one
+ * whose hypergraph is busy, and admit a request only while the HBM footprint
+ * of the requests in flight plus its own stays within the budget; a request
+ * sets its state from its part vector, refines and evaluates on its
+ * hypergraph's stream, so requests on different hypergraphs overlap on the
+ * device and new ones start as slots and memory free up. A request's result
+ * depends only on the request. The reference has no counterpart. */
+typedef struct fp_server fp_server;
+typedef struct {
+  long long id;
+  int graph, metric, K, passes;
+  fp_status status;
+  long long moves;
+  fp_metrics metrics;          /* the six metrics after refinement */
+  double submit_ms, start_ms, end_ms;  /* since fp_server_create */
+  long long footprint;         /* HBM bytes the admission charged */
+} fp_request_info;
+/* hbm_budget: bytes the requests in flight may hold together; workers: 1..256 */
+fp_status fp_server_create(fp_server** server, int device, long long hbm_budget, int workers);
+/* Stops the workers after their current requests; queued requests are
+ * dropped, so wait for every submitted request first. */
+void fp_server_destroy(fp_server* server);
+const char* fp_server_last_error(const fp_server* server);
+/* Registers (uploads) a hypergraph: net->pins CSR, net sources (NULL: none),
+ * vertex weights (NULL: unit). graph_id receives its index. */
+fp_status fp_server_add_graph(fp_server* server, int nov, int noe, const unsigned* net_ptr, const unsigned* net_pins,
+                              const int* src, const int* vwgt, int* graph_id);
+/* Queues a request: `passes` passes of `metric` under `sched` (NULL for CN) on
+ * hypergraph graph_id from the part vector part[nov] (copied) with K parts.
+ * FP_ERR_BUDGET if the request alone exceeds the budget. Does not block. */
+fp_status fp_server_submit(fp_server* server, int graph_id, fp_metric metric, int K, const int* part,
+                           const fp_schedule* sched, int passes, long long* request_id);
+/* Blocks until the request is done and releases it: its info, and its final
+ * part vector into part_out[nov] (nullable). Returns the request's status. */
+fp_status fp_server_wait(fp_server* server, long long request_id, fp_request_info* info, int* part_out);
+fp_status fp_server_stats(fp_server* server, long long* completed, int* peak_inflight, long long* peak_bytes);
+
 /* Event timing of the dominant refinement kernel in the last fp_refine:
  * launches, total device ms of those launches (HIP events around each launch,
  * on the context stream). */
