@@ -499,22 +499,19 @@ __global__ void __launch_bounds__(REPLAY_BLOCK) k_conn_apply(ConnApplyArgs a) {
 // MSM: the round's moves in order (MSM/partition.cpp:1230-1370). Each move's
 // pc changes go to pc with atomics; a cell crossing 0 <-> 1 changes its row's
 // message count; then the max-set step decides whether the round stops.
-__global__ void __launch_bounds__(WAVE) k_msm_scan(int* __restrict__ pc, int* __restrict__ msg_g, int K,
+__global__ void __launch_bounds__(WAVE) k_msm_scan(int* __restrict__ pc, const int* __restrict__ msg_g, int K,
                                                    const int4* __restrict__ mv, const int2* __restrict__ evbuf,
                                                    const unsigned* __restrict__ incoff,
                                                    const unsigned* __restrict__ vtx_ptr, int* __restrict__ mstat) {
   extern __shared__ __align__(16) unsigned char lds[];
   int* msg = reinterpret_cast<int*>(lds);
-  int* delta = msg + K;
-  int* claim = delta + K;
+  int* dirty = msg + K;
   const int lane = lane_id();
-  for (int q = lane; q < K; q += WAVE) {
-    msg[q] = msg_g[q];
-    delta[q] = 0;
-    claim[q] = INT_MAX;
-  }
+  for (int q = lane; q < K; q += WAVE) msg[q] = msg_g[q];
+  dirty[lane] = 0;
   wave_lds_sync();
-  int M = mstat[0], cntM = mstat[1];
+  LaneMax ms;
+  int M = maxset_init(msg, K, ms);
   const int nmoves = mstat[2];
   int nstop = nmoves;
   for (int mi = 0; mi < nmoves; mi++) {
@@ -529,23 +526,20 @@ __global__ void __launch_bounds__(WAVE) k_msm_scan(int* __restrict__ pc, int* __
           const int old = atomicAdd(&pc[e.x], e.y);
           const int cross = (e.y > 0 && old == 0) ? 1 : ((e.y < 0 && old == 1) ? -1 : 0);
           if (cross) {
-            atomicAdd(&delta[e.x / K], cross);
-            atomicAdd(&msg[e.x / K], cross);
+            const int r = e.x / K;
+            atomicAdd(&msg[r], cross);
+            dirty[r & (WAVE - 1)] = 1;
           }
         }
       }
     }
     wave_lds_sync();
-    auto item = [&](int it) -> int {
-      if (it >= nit) return -1;
-      const int2 e = evbuf[e0 + (unsigned)it];
-      return e.y != 0 ? e.x / K : -1;
-    };
-    if (maxset_step(msg, delta, claim, K, nit, item, M, cntM)) {
+    if (maxset_step(msg, dirty, K, ms, M)) {
       nstop = mi + 1;
       break;
     }
   }
+  const int cntM = maxset_count(ms);
   if (lane == 0) {
     mstat[0] = M;
     mstat[1] = cntM;
@@ -736,7 +730,7 @@ fp_status pass_conn(fp_ctx* ctx, const RefinePlan& p) {
                                                     c.slot_mi, c.mv, c.mstat);
       ap.mode = 1;
       k_conn_apply<<<REPLAY_NET_BLOCKS, REPLAY_BLOCK, lds_r, ctx->stream>>>(ap);
-      k_msm_scan<<<1, WAVE, (size_t)3 * (size_t)K * sizeof(int), ctx->stream>>>(c.pc, c.msg, K, c.mv, c.evbuf,
+      k_msm_scan<<<1, WAVE, ((size_t)K + WAVE) * sizeof(int), ctx->stream>>>(c.pc, c.msg, K, c.mv, c.evbuf,
                                                                                  c.incoff, ctx->vtx_ptr, c.mstat);
       ap.mode = 2;
       k_conn_apply<<<REPLAY_NET_BLOCKS + (nslots + REPLAY_BLOCK - 1) / REPLAY_BLOCK, REPLAY_BLOCK, lds_r,

@@ -63,6 +63,50 @@ __device__ __forceinline__ T wave_sum(T x) {
   return x;
 }
 
+// 32-bit lane exchange by a DPP pattern (no LDS traffic); lanes whose row is
+// not in ROW_MASK, or whose source lane lies outside the row, receive `old`.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ int dpp_i32(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xF, false);
+}
+
+// Wave-wide sum and max of an int through DPP: quad_perm [1,0,3,2] and
+// [2,3,0,1], row_ror 4 and 8 (row-wide), row_bcast15 into rows 1 and 3,
+// row_bcast31 into rows 2 and 3; lane 63 then holds the result, which is
+// returned to every lane.
+__device__ __forceinline__ int wave_sum(int x) {
+  x += dpp_i32<0xB1>(0, x);
+  x += dpp_i32<0x4E>(0, x);
+  x += dpp_i32<0x124>(0, x);
+  x += dpp_i32<0x128>(0, x);
+  x += dpp_i32<0x142, 0xA>(0, x);
+  x += dpp_i32<0x143, 0xC>(0, x);
+  return __builtin_amdgcn_readlane(x, WAVE - 1);
+}
+
+__device__ __forceinline__ int wave_max(int x) {
+  x = max(x, dpp_i32<0xB1>(x, x));
+  x = max(x, dpp_i32<0x4E>(x, x));
+  x = max(x, dpp_i32<0x124>(x, x));
+  x = max(x, dpp_i32<0x128>(x, x));
+  x = max(x, dpp_i32<0x142, 0xA>(x, x));
+  x = max(x, dpp_i32<0x143, 0xC>(x, x));
+  return __builtin_amdgcn_readlane(x, WAVE - 1);
+}
+
+// Inclusive prefix sum over the wave through DPP: row_shr 1, 2, 4, 8 (lanes
+// whose source is outside the row add 0), then row_bcast15 into rows 1 and 3
+// and row_bcast31 into rows 2 and 3 carry the row totals.
+__device__ __forceinline__ int wave_incl_scan(int x) {
+  x += dpp_i32<0x111>(0, x);
+  x += dpp_i32<0x112>(0, x);
+  x += dpp_i32<0x114>(0, x);
+  x += dpp_i32<0x118>(0, x);
+  x += dpp_i32<0x142, 0xA>(0, x);
+  x += dpp_i32<0x143, 0xC>(0, x);
+  return x;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_incl_scan(T x) {
   const int lane = lane_id();

@@ -108,118 +108,56 @@ __global__ void __launch_bounds__(1024) k_round_compact(const int* __restrict__ 
   }
 }
 
-// One move's step of the max-set bookkeeping, run by a single wavefront.
-// val[] (LDS) already holds the move's changes; delta[p] holds the change of
-// val[p] for every part p that item(it), it in [0, nit), names (duplicates
-// allowed, -1 = no part); claim[] is all INT_MAX on entry. Updates the max M
-// and its multiplicity cntM, clears delta/claim of the items, and returns true
-// iff the new max set holds a part the previous one did not. A move that
-// leaves every affected part off the max and below it changes nothing (the
-// common case, decided with one read per item). Otherwise: the max rose (the
-// new set is the affected parts at the new value); it stayed (the untouched
-// parts at M plus affected parts landing on it); or it fell (full reduction).
-template <class Item>
-__device__ bool maxset_step(int* val, int* delta, int* claim, int K, int nit, Item item, int& M, int& cntM) {
-  const int lane = lane_id();
-  bool touch = false;
-  for (int b = 0; b < nit; b += WAVE) {
-    const int p = item(b + lane);
-    if (p >= 0) {
-      const int nw = val[p], old = nw - delta[p];
-      touch = touch || old == M || nw >= M;
-    }
+// Max-set tracking by one wavefront over val[0..K) in LDS (K <= 64 * 64).
+// Lane l holds the parts l, l + 64, l + 128, ...: their maximum (lmax) and
+// the bitmask of those holding it (lbits, bit j = part 64 j + l). The max
+// set is the union of lbits over the lanes whose lmax equals the wave max;
+// inS is this lane's share of it.
+struct LaneMax {
+  int lmax;
+  unsigned long long lbits, inS;
+};
+
+__device__ __forceinline__ void lane_rescan(const int* val, int K, LaneMax& s) {
+  int m = INT_MIN;
+  unsigned long long bits = 0;
+  int j = 0;
+  for (int p = lane_id(); p < K; p += WAVE, j++) {
+    const int x = val[p];
+    const unsigned long long bj = 1ull << j;
+    bits = x > m ? bj : (x == m ? bits | bj : bits);
+    m = max(m, x);
   }
-  if (__ballot(touch) == 0) {
-    wave_lds_sync();
-    for (int b = 0; b < nit; b += WAVE) {
-      const int p = item(b + lane);
-      if (p >= 0) delta[p] = 0;
-    }
-    wave_lds_sync();
-    return false;
-  }
-  for (int b = 0; b < nit; b += WAVE) {
-    const int p = item(b + lane);
-    if (p >= 0) atomicMin(&claim[p], b + lane);
-  }
-  wave_lds_sync();
-  // one representative item per affected part: old and new values
-  int up = INT_MIN, inS = 0, newM = 0;
-  bool enterM = false;
-  for (int b = 0; b < nit; b += WAVE) {
-    const int it = b + lane;
-    const int p = item(it);
-    if (p >= 0 && claim[p] == it) {
-      const int nw = val[p], old = nw - delta[p];
-      up = max(up, nw);
-      inS += old == M;
-      newM += nw == M;
-      enterM = enterM || (nw == M && old != M);
-    }
-  }
-  {
-    int c0 = 0;
-    wave_max_count(up, c0);
-  }
-  inS = wave_sum(inS);
-  newM = wave_sum(newM);
-  enterM = __ballot(enterM) != 0;
-  bool stop;
-  int nM, nC;
-  if (up > M) {
-    int cu = 0;
-    bool enter = false;
-    for (int b = 0; b < nit; b += WAVE) {
-      const int it = b + lane;
-      const int p = item(it);
-      if (p >= 0 && claim[p] == it && val[p] == up) {
-        cu++;
-        enter = enter || (val[p] - delta[p] != M);
-      }
-    }
-    stop = __ballot(enter) != 0;
-    nM = up;
-    nC = wave_sum(cu);
-  } else if (cntM - inS > 0 || newM > 0) {
-    stop = enterM;
-    nM = M;
-    nC = cntM - inS + newM;
-  } else {
-    int mm = INT_MIN, cc = 0;
-    for (int q = lane; q < K; q += WAVE) {
-      const int x = val[q];
-      if (x > mm) {
-        mm = x;
-        cc = 1;
-      } else if (x == mm) {
-        cc++;
-      }
-    }
-    wave_max_count(mm, cc);
-    int stay = 0;
-    for (int b = 0; b < nit; b += WAVE) {
-      const int it = b + lane;
-      const int p = item(it);
-      if (p >= 0 && claim[p] == it && val[p] == mm && val[p] - delta[p] == M) stay++;
-    }
-    stay = wave_sum(stay);
-    stop = cc > stay;
-    nM = mm;
-    nC = cc;
-  }
-  M = nM;
-  cntM = nC;
-  wave_lds_sync();
-  for (int b = 0; b < nit; b += WAVE) {
-    const int p = item(b + lane);
-    if (p >= 0) {
-      delta[p] = 0;
-      claim[p] = INT_MAX;
-    }
-  }
-  wave_lds_sync();
-  return stop;
+  s.lmax = m;
+  s.lbits = bits;
 }
+
+// Initial state from val; returns the max.
+__device__ __forceinline__ int maxset_init(const int* val, int K, LaneMax& s) {
+  lane_rescan(val, K, s);
+  const int M = wave_max(s.lmax);
+  s.inS = s.lmax == M ? s.lbits : 0ull;
+  return M;
+}
+
+// One move's step: val holds the move's changes and dirty[l] != 0 marks the
+// lanes l whose parts changed (cleared here). Updates the max M and returns
+// true iff the new max set holds a part the previous one did not.
+__device__ __forceinline__ bool maxset_step(const int* val, int* dirty, int K, LaneMax& s, int& M) {
+  const int lane = lane_id();
+  if (dirty[lane]) {
+    dirty[lane] = 0;
+    lane_rescan(val, K, s);
+  }
+  M = wave_max(s.lmax);
+  const unsigned long long nS = s.lmax == M ? s.lbits : 0ull;
+  const bool enter = __ballot((nS & ~s.inS) != 0ull) != 0ull;
+  s.inS = nS;
+  return enter;
+}
+
+// number of parts in the max set
+__device__ __forceinline__ int maxset_count(const LaneMax& s) { return wave_sum(__popcll(s.inS)); }
 
 }  // namespace
 }  // namespace fpe

@@ -27,7 +27,7 @@ namespace fpe {
 namespace {
 
 constexpr int VOL_BLOCK = 256;
-constexpr int VOL_KMAX = 4096;  // k_vol_scan holds three K-int arrays in LDS
+constexpr int VOL_KMAX = 4096;  // k_vol_scan: one lane of the max-set tracking holds at most 64 parts
 constexpr int REPLAY_BLOCK = 256;
 constexpr int REPLAY_NET_BLOCKS = 2048;
 
@@ -419,16 +419,13 @@ __global__ void __launch_bounds__(WAVE) k_vol_scan(int* __restrict__ vol_g, int 
   extern __shared__ __align__(16) unsigned char lds[];
   int4* stage = reinterpret_cast<int4*>(lds);
   int* vol = reinterpret_cast<int*>(stage + stage_cap);
-  int* delta = vol + K;
-  int* claim = delta + K;
+  int* dirty = vol + K;
   const int lane = lane_id();
-  for (int q = lane; q < K; q += WAVE) {
-    vol[q] = vol_g[q];
-    delta[q] = 0;
-    claim[q] = INT_MAX;
-  }
+  for (int q = lane; q < K; q += WAVE) vol[q] = vol_g[q];
+  dirty[lane] = 0;
   wave_lds_sync();
-  int M = mstat[0], cntM = mstat[1];
+  LaneMax ms;
+  int M = maxset_init(vol, K, ms);
   const int nmoves = mstat[2];
   int nstop = nmoves;
   bool stopped = false;
@@ -459,27 +456,25 @@ __global__ void __launch_bounds__(WAVE) k_vol_scan(int* __restrict__ vol_g, int 
           s_own += e.y;
           s_dec += e.z;
           if (e.x >= 0 && e.w != 0) {
-            atomicAdd(&delta[e.x], e.w);
             atomicAdd(&vol[e.x], e.w);
+            dirty[e.x & (WAVE - 1)] = 1;
           }
         }
       }
       s_own = wave_sum(s_own);
       s_dec = wave_sum(s_dec);
       if (lane == 0) {
-        atomicAdd(&delta[own], s_own);
-        atomicAdd(&vol[own], s_own);
-        atomicAdd(&delta[d], s_dec);
-        atomicAdd(&vol[d], s_dec);
+        if (s_own) {
+          atomicAdd(&vol[own], s_own);
+          dirty[own & (WAVE - 1)] = 1;
+        }
+        if (s_dec) {
+          atomicAdd(&vol[d], s_dec);
+          dirty[d & (WAVE - 1)] = 1;
+        }
       }
       wave_lds_sync();
-      // affected parts: 0 -> own, 1 -> d, 2 + i -> source part recorded for net i (or -1)
-      auto item = [&](int it) -> int {
-        if (it == 0) return own;
-        if (it == 1) return d;
-        return it < deg + 2 ? evs[it - 2].x : -1;
-      };
-      if (maxset_step(vol, delta, claim, K, deg + 2, item, M, cntM)) {
+      if (maxset_step(vol, dirty, K, ms, M)) {
         nstop = mi + j + 1;
         stopped = true;
         break;
@@ -488,6 +483,7 @@ __global__ void __launch_bounds__(WAVE) k_vol_scan(int* __restrict__ vol_g, int 
     mi += nb;
     wave_lds_sync();
   }
+  const int cntM = maxset_count(ms);
   for (int q = lane; q < K; q += WAVE) vol_g[q] = vol[q];
   if (lane == 0) {
     mstat[0] = M;
@@ -562,7 +558,7 @@ fp_status pass_vol(fp_ctx* ctx, const RefinePlan& p) {
   const int nov = ctx->nov, K = ctx->K, R = p.R, waves = p.eval_waves;
   const size_t lds_e = (size_t)K * 16 + (size_t)waves * 8 * (size_t)K;
   const size_t lds_r = (size_t)(REPLAY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
-  const size_t lds_vol = (size_t)3 * (size_t)K * sizeof(int);
+  const size_t lds_vol = ((size_t)K + WAVE) * sizeof(int);
   const int stage_cap = (int)std::min<size_t>(4096, ((size_t)LDS_BUDGET - lds_vol) / sizeof(int4));
   const size_t lds_s = lds_vol + (size_t)stage_cap * sizeof(int4);
   auto& vs = ctx->vol;
