@@ -298,8 +298,8 @@ __global__ void __launch_bounds__(WAVE) k_cn_apply(CnApplyArgs a) {
     for (int j = lane; j < P; j += WAVE) keys[j] = j < c ? (unsigned)a.ops[off + j] : 0xFFFFFFFFu;
     wave_lds_sync();
     if (c > 1) wave_bitonic_sort(keys, P);
-    const unsigned seg = a.net_ptr[t] + (unsigned)t;
-    int L = wave_list_load(a.ent, seg, list);
+    const unsigned s0 = a.net_ptr[t], seg = s0 + (unsigned)t;
+    int L = wave_list_load(a.ent, seg, a.net_ptr[t + 1] - s0, list);
     for (int j = 0; j < c; j++) {
       const int m = (int)keys[j];
       wave_list_move(list, L, a.mv_from[m], a.cn_new[a.mv_net[m]], a.pb, a.pmask);

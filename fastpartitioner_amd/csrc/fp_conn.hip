@@ -437,17 +437,18 @@ __global__ void __launch_bounds__(REPLAY_BLOCK) k_conn_apply(ConnApplyArgs a) {
     const int n = a.touched[i];
     const unsigned s = a.net_ptr[n], e = a.net_ptr[n + 1];
     const unsigned seg = s + (unsigned)n;
-    int L = wave_list_load(a.ent, seg, list);
-    const unsigned lo = wave_lower_bound(a.apply_pins, s, e, a.r0);
-    const unsigned hi = wave_lower_bound(a.apply_pins, lo, e, a.r1);
+    unsigned lo, hi;
+    wave_rank_window(a.apply_pins, s, e, a.r0, a.r1, lo, hi);
+    int L = wave_list_load(a.ent, seg, e - s, list);
     const int srcv = a.src[n];
     int sp = frozen_part(srcv, a.part, a.slot_own, a.r0, a.r1);
     bool dirty = false;
     for (unsigned k0 = lo; k0 < hi; k0 += WAVE) {
       const unsigned k = k0 + (unsigned)lane;
       int slot = -1, dd = -1;
-      if (k < hi) {
-        slot = a.apply_pins[k] - a.r0;
+      const int r = k < hi ? a.apply_pins[k] : INT_MAX;
+      if (r >= a.r0 && r < a.r1) {
+        slot = r - a.r0;
         if (a.mode == 0) {
           dd = a.slot_dec[slot];
         } else {
@@ -455,21 +456,33 @@ __global__ void __launch_bounds__(REPLAY_BLOCK) k_conn_apply(ConnApplyArgs a) {
           dd = (mi >= 0 && (a.mode == 1 || mi < nstop)) ? mi : -1;
         }
       }
+      // each lane fetches its own move's parts and, for the dry replay, the
+      // event slots of (move, n); the ordered replay below reads them from the lanes
+      int own_l = 0, d_l = 0;
+      unsigned e0_l = 0, e1_l = 0;
+      if (dd >= 0) {
+        if (a.mode == 0) {
+          own_l = a.slot_own[slot];
+          d_l = dd;
+        } else {
+          const int4 m = a.mv[dd];
+          own_l = m.y;
+          d_l = m.z;
+          if (a.mode == 1) {
+            // event slots of incidence (v, n): [incoff[j], incoff[j + 1]), j = position of n in v's list
+            const int v = a.r0 + slot;
+            const unsigned j = lane_lower_bound(a.vtx_nets, a.vtx_ptr[v], a.vtx_ptr[v + 1], (unsigned)n);
+            e0_l = a.incoff[j];
+            e1_l = a.incoff[j + 1];
+          }
+        }
+      }
       unsigned long long bm = __ballot(dd >= 0);
       while (bm) {
-        const int l = __ffsll((long long)bm) - 1;
+        const int l = __builtin_amdgcn_readfirstlane(__ffsll((long long)bm) - 1);
         bm &= bm - 1;
-        const int sl = __shfl(slot, l);
-        const int v = a.r0 + sl;
-        int own, d;
-        if (a.mode == 0) {
-          own = a.slot_own[sl];
-          d = __shfl(dd, l);
-        } else {
-          const int4 m = a.mv[__shfl(dd, l)];
-          own = m.y;
-          d = m.z;
-        }
+        const int v = a.r0 + __builtin_amdgcn_readlane(slot, l);
+        const int own = __builtin_amdgcn_readlane(own_l, l), d = __builtin_amdgcn_readlane(d_l, l);
         wave_list_move(list, L, own, d, a.pb, a.pmask);
         dirty = true;
         const bool is_src = srcv == v;
@@ -478,10 +491,8 @@ __global__ void __launch_bounds__(REPLAY_BLOCK) k_conn_apply(ConnApplyArgs a) {
             if (cell >= 0) atomicAdd(&a.pc[cell], dl);
           });
         } else if (a.mode == 1) {
-          // event slots of incidence (v, n): [incoff[j], incoff[j + 1]), j = position of n in v's list
-          const unsigned vb = a.vtx_ptr[v], ve = a.vtx_ptr[v + 1];
-          const unsigned j = wave_lower_bound(reinterpret_cast<const int*>(a.vtx_nets), vb, ve, n);
-          const unsigned e0 = a.incoff[j], e1 = a.incoff[j + 1];
+          const unsigned e0 = (unsigned)__builtin_amdgcn_readlane((int)e0_l, l),
+                         e1 = (unsigned)__builtin_amdgcn_readlane((int)e1_l, l);
           for (unsigned t = e0 + (unsigned)lane; t < e1; t += WAVE) a.evbuf[t] = make_int2(0, 0);
           wave_lds_sync();
           conn_changes(list, L, own, d, is_src, sp, a.K, a.pb, a.pmask, [&](int slot_i, int cell, int dl) {

@@ -236,11 +236,41 @@ __device__ __forceinline__ void wave_list_move(unsigned* list, int& L, int p, in
 }
 
 // Stages net n's list (segment at seg = net_ptr[n] + n) into LDS; returns lambda.
-__device__ __forceinline__ int wave_list_load(const unsigned* ent, unsigned seg, unsigned* list) {
-  const int L = (int)ent[seg];
-  for (int t = lane_id(); t < L; t += WAVE) list[t] = ent[seg + 1 + (unsigned)t];
+__device__ __forceinline__ int wave_list_load(const unsigned* ent, unsigned seg, unsigned np, unsigned* list) {
+  // the segment holds the length and then np >= length entries: one load
+  // covers the length and the first 63 entries
+  const int lane = lane_id();
+  const unsigned x = (unsigned)lane <= np ? ent[seg + (unsigned)lane] : 0u;
+  const int L = __builtin_amdgcn_readfirstlane((int)x);
+  if (lane >= 1 && lane <= L) list[lane - 1] = x;
+  for (int t = WAVE - 1 + lane; t < L; t += WAVE) list[t] = ent[seg + 1 + (unsigned)t];
   wave_lds_sync();
   return L;
+}
+
+// Per-lane lower bound: first k in [lo, hi) with arr[k] >= key (arr ascending).
+__device__ __forceinline__ unsigned lane_lower_bound(const unsigned* __restrict__ arr, unsigned lo, unsigned hi,
+                                                     unsigned key) {
+  while (lo < hi) {
+    const unsigned mid = (lo + hi) >> 1;
+    if (arr[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Range [lo, hi) of apply_pins[s..e) (a net's pin ranks, ascending) that
+// may hold ranks in [r0, r1): the exact range for nets longer than a wave,
+// the whole net otherwise (the caller tests each rank against [r0, r1)).
+__device__ __forceinline__ void wave_rank_window(const int* apply_pins, unsigned s, unsigned e, int r0, int r1,
+                                                 unsigned& lo, unsigned& hi) {
+  if (e - s <= (unsigned)WAVE) {
+    lo = s;
+    hi = e;
+    return;
+  }
+  lo = wave_lower_bound(apply_pins, s, e, r0);
+  hi = wave_lower_bound(apply_pins, lo, e, r1);
 }
 
 // Writes a list back to both the packed segment and its evaluation view.

@@ -66,38 +66,62 @@ __global__ void __launch_bounds__(1024) k_round_compact(const int* __restrict__ 
                                                       const unsigned* __restrict__ vtx_ptr, int r0, int nslots,
                                                       int* __restrict__ slot_mi, int4* __restrict__ mv,
                                                       int* __restrict__ mstat) {
-  // each thread owns a run of COMPACT_RUN consecutive slots, so one step of
-  // the block covers 16K slots and the block-wide scans are few
+  // each thread owns a run of COMPACT_RUN consecutive slots, read and written
+  // with 16-byte accesses; accepted slots are few, so their vertices and
+  // degrees are loaded per set bit of the run's acceptance mask
+  static_assert(COMPACT_RUN == 16, "the run is four int4 words");
   __shared__ int s_w[16];
   int base = 0, ebase = 0;
   const int step = (int)blockDim.x * COMPACT_RUN;
   for (int s0 = 0; s0 < nslots; s0 += step) {
     const int first = s0 + (int)threadIdx.x * COMPACT_RUN;
-    int f = 0, deg = 0;
-    for (int j = 0; j < COMPACT_RUN; j++) {
-      const int s = first + j;
-      if (s < nslots && slot_dec[s] >= 0) {
-        const int v = order ? order[r0 + s] : r0 + s;
-        f++;
-        deg += (int)(vtx_ptr[v + 1] - vtx_ptr[v]);
+    const bool whole = first + COMPACT_RUN <= nslots;
+    unsigned acc = 0;
+    if (whole) {
+      const int4* p = reinterpret_cast<const int4*>(slot_dec + first);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int4 x = p[q];
+        acc |= (x.x >= 0 ? 1u : 0u) << (4 * q);
+        acc |= (x.y >= 0 ? 2u : 0u) << (4 * q);
+        acc |= (x.z >= 0 ? 4u : 0u) << (4 * q);
+        acc |= (x.w >= 0 ? 8u : 0u) << (4 * q);
       }
+    } else {
+      for (int j = 0; j < COMPACT_RUN && first + j < nslots; j++)
+        if (slot_dec[first + j] >= 0) acc |= 1u << j;
+    }
+    int deg = 0;
+    for (unsigned m = acc; m; m &= m - 1) {
+      const int s = first + __builtin_ctz(m);
+      const int v = order ? order[r0 + s] : r0 + s;
+      deg += (int)(vtx_ptr[v + 1] - vtx_ptr[v]);
     }
     int tot, totd;
-    int ex = block_excl_scan(f, s_w, &tot);
+    const int ex = block_excl_scan((int)__popc(acc), s_w, &tot);
     int exd = block_excl_scan(deg, s_w, &totd);
-    for (int j = 0; j < COMPACT_RUN; j++) {
-      const int s = first + j;
-      if (s >= nslots) break;
-      const int d = slot_dec[s];
-      if (d >= 0) {
-        const int v = order ? order[r0 + s] : r0 + s;
-        slot_mi[s] = base + ex;
-        mv[base + ex] = make_int4(v, slot_own[s], d, ebase + exd);
-        ex++;
-        exd += (int)(vtx_ptr[v + 1] - vtx_ptr[v]);
-      } else {
-        slot_mi[s] = -1;
+    if (whole) {
+      int4* q4 = reinterpret_cast<int4*>(slot_mi + first);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        int w[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const int j = 4 * q + t;
+          w[t] = (acc >> j) & 1u ? base + ex + __popc(acc & ((1u << j) - 1u)) : -1;
+        }
+        q4[q] = make_int4(w[0], w[1], w[2], w[3]);
       }
+    } else {
+      for (int j = 0; j < COMPACT_RUN && first + j < nslots; j++)
+        slot_mi[first + j] = (acc >> j) & 1u ? base + ex + __popc(acc & ((1u << j) - 1u)) : -1;
+    }
+    int i = base + ex;
+    for (unsigned m = acc; m; m &= m - 1) {
+      const int s = first + __builtin_ctz(m);
+      const int v = order ? order[r0 + s] : r0 + s;
+      mv[i++] = make_int4(v, slot_own[s], slot_dec[s], ebase + exd);
+      exd += (int)(vtx_ptr[v + 1] - vtx_ptr[v]);
     }
     base += tot;
     ebase += totd;

@@ -419,24 +419,24 @@ __global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
     const int n = a.touched[i];
     const unsigned s = a.net_ptr[n], e = a.net_ptr[n + 1];
     const unsigned seg = s + (unsigned)n;
-    int L = wave_list_load(a.ent, seg, list);
-    const unsigned lo = wave_lower_bound(a.apply_pins, s, e, a.r0);
-    const unsigned hi = wave_lower_bound(a.apply_pins, lo, e, a.r1);
+    unsigned lo, hi;
+    wave_rank_window(a.apply_pins, s, e, a.r0, a.r1, lo, hi);
+    int L = wave_list_load(a.ent, seg, e - s, list);
     bool dirty = false;
     for (unsigned k0 = lo; k0 < hi; k0 += WAVE) {
       const unsigned k = k0 + (unsigned)lane;
       int dd = -1, oo = -1;
-      if (k < hi) {
-        const int slot = a.apply_pins[k] - a.r0;
-        dd = a.slot_dec[slot];
-        if (dd >= 0) oo = a.slot_own[slot];
+      const int r = k < hi ? a.apply_pins[k] : INT_MAX;
+      if (r >= a.r0 && r < a.r1) {
+        dd = a.slot_dec[r - a.r0];
+        if (dd >= 0) oo = a.slot_own[r - a.r0];
       }
       unsigned long long m = __ballot(dd >= 0);
       while (m) {
         dirty = true;
-        const int l = __ffsll((long long)m) - 1;
+        const int l = __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1);
         m &= m - 1;
-        wave_list_move(list, L, __shfl(oo, l), __shfl(dd, l), a.pb, a.pmask);
+        wave_list_move(list, L, __builtin_amdgcn_readlane(oo, l), __builtin_amdgcn_readlane(dd, l), a.pb, a.pmask);
       }
     }
     if (dirty) wave_list_store(a.ent, a.ev, seg, list, L, a.pb, a.pmask);

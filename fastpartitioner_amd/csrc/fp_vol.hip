@@ -345,24 +345,36 @@ __global__ void __launch_bounds__(REPLAY_BLOCK) k_vol_replay(VolReplayArgs a) {
     const int n = a.touched[i];
     const unsigned s = a.net_ptr[n], e = a.net_ptr[n + 1];
     const unsigned seg = s + (unsigned)n;
-    int L = wave_list_load(a.ent, seg, list);
-    const unsigned lo = wave_lower_bound(a.apply_pins, s, e, a.r0);
-    const unsigned hi = wave_lower_bound(a.apply_pins, lo, e, a.r1);
+    unsigned lo, hi;
+    wave_rank_window(a.apply_pins, s, e, a.r0, a.r1, lo, hi);
+    int L = wave_list_load(a.ent, seg, e - s, list);
     const int srcv = a.src[n];
     int sp_cur = a.commit ? 0 : a.part[srcv];  // the source's part as the replay goes
     bool dirty = false;
     for (unsigned k0 = lo; k0 < hi; k0 += WAVE) {
       const unsigned k = k0 + (unsigned)lane;
       int mi = -1;
-      if (k < hi) mi = a.slot_mi[a.apply_pins[k] - a.r0];
+      const int r = k < hi ? a.apply_pins[k] : INT_MAX;
+      if (r >= a.r0 && r < a.r1) mi = a.slot_mi[r - a.r0];
       if (a.commit && mi >= nstop) mi = -1;
+      // each lane fetches its own move (v, own, d, evoff) and, for the dry
+      // replay, the event slot of (move, n); the ordered replay below reads
+      // them from the lanes
+      int4 mm = make_int4(0, 0, 0, 0);
+      unsigned evslot = 0;
+      if (mi >= 0) {
+        mm = a.mv[mi];
+        if (!a.commit) {
+          const unsigned vb = a.vtx_ptr[mm.x], ve = a.vtx_ptr[mm.x + 1];
+          evslot = (unsigned)mm.w + (lane_lower_bound(a.vtx_nets, vb, ve, (unsigned)n) - vb);
+        }
+      }
       unsigned long long bm = __ballot(mi >= 0);
       while (bm) {
-        const int l = __ffsll((long long)bm) - 1;
+        const int l = __builtin_amdgcn_readfirstlane(__ffsll((long long)bm) - 1);
         bm &= bm - 1;
-        const int mj = __shfl(mi, l);
-        const int4 m = a.mv[mj];  // (v, own, d, evoff)
-        const int v = m.x, own = m.y, d = m.z;
+        const int v = __builtin_amdgcn_readlane(mm.x, l), own = __builtin_amdgcn_readlane(mm.y, l),
+                  d = __builtin_amdgcn_readlane(mm.z, l);
         if (!a.commit) {
           const bool isrc = srcv == v;
           const int jo = wave_find(list, L, own, a.pmask);
@@ -391,9 +403,7 @@ __global__ void __launch_bounds__(REPLAY_BLOCK) k_vol_replay(VolReplayArgs a) {
             e_sp += 1;                                        // send_volumes[net_src_part]++
             if (a.msrv) e_dec += 1;                           // received_volumes[decision]++
           }
-          const unsigned vb = a.vtx_ptr[v], ve = a.vtx_ptr[v + 1];
-          const unsigned pos = wave_lower_bound(reinterpret_cast<const int*>(a.vtx_nets), vb, ve, n) - vb;
-          if (lane == 0) a.evbuf[(unsigned)m.w + pos] = make_int4(spx, e_own, e_dec, e_sp);
+          if (lane == 0) a.evbuf[(unsigned)__builtin_amdgcn_readlane((int)evslot, l)] = make_int4(spx, e_own, e_dec, e_sp);
           if (isrc) sp_cur = d;
         }
         wave_list_move(list, L, own, d, a.pb, a.pmask);
