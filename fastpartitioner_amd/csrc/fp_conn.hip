@@ -307,39 +307,87 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
 // TM apply order (TM/partition.cpp:1111-1125): the round's candidates (the
 // compacted move list) in vertex order; one whose destination size is at
 // least thr = C * 1.02 at that point is dropped. Commits part vector and both
-// sizes of the others.
+// sizes of the others. Chunks of 64 candidates: with in[p] / out[p] the
+// chunk's candidate weight into / out of part p, a candidate into d is
+// dropped whatever the chunk's other decisions are when sz[d] - out[d] >=
+// thr, and admitted when sz[d] + in[d] < thr; a chunk decided that way for
+// all its candidates is applied in parallel, any other chunk candidate by
+// candidate.
 __global__ void __launch_bounds__(WAVE) k_tm_admit(int* __restrict__ slot_dec, const int4* __restrict__ mv,
                                                    const int* __restrict__ mstat, const int* __restrict__ vwgt,
                                                    int r0, int K, double thr, long long* __restrict__ sizes,
                                                    int* __restrict__ part, unsigned long long* __restrict__ moves) {
   extern __shared__ __align__(16) unsigned char lds[];
   long long* sz = reinterpret_cast<long long*>(lds);
+  int* inw = reinterpret_cast<int*>(sz + K);
+  int* outw = inw + K;
   const int lane = lane_id();
-  for (int q = lane; q < K; q += WAVE) sz[q] = sizes[q];
+  for (int q = lane; q < K; q += WAVE) {
+    sz[q] = sizes[q];
+    inw[q] = 0;
+    outw[q] = 0;
+  }
   wave_lds_sync();
   const int nmv = mstat[2];
   unsigned long long nm = 0;
   for (int m0 = 0; m0 < nmv; m0 += WAVE) {
     const int i = m0 + lane;
-    const int4 x = i < nmv ? mv[i] : make_int4(0, 0, 0, 0);
-    const int w = i < nmv ? (vwgt ? vwgt[x.x] : 1) : 0;
-    const int cnt = min(WAVE, nmv - m0);
-    // lane 0 walks the chunk's candidates in order; only it touches sz here
-    for (int l = 0; l < cnt; l++) {
-      const int vv = __shfl(x.x, l), oo = __shfl(x.y, l), dd = __shfl(x.z, l), ww = __shfl(w, l);
-      if (lane == 0) {
-        if ((double)sz[dd] >= thr) {
-          slot_dec[vv - r0] = -1;
+    const bool valid = i < nmv;
+    const int4 x = valid ? mv[i] : make_int4(0, 0, 0, 0);
+    const int w = valid ? (vwgt ? vwgt[x.x] : 1) : 0;
+    const int v = x.x, o = x.y, d = x.z;
+    // chunk sums stay exact in int for weights up to 2^24
+    const bool bounded = __ballot(valid && (unsigned)w > (1u << 24)) == 0ull;
+    int status = 0;  // 1 admitted, -1 dropped, 0 open
+    if (bounded) {
+      if (valid) {
+        atomicAdd(&inw[d], w);
+        atomicAdd(&outw[o], w);
+      }
+      wave_lds_sync();
+      if (valid) {
+        const long long sd = sz[d];
+        if ((double)(sd - (long long)outw[d]) >= thr) status = -1;
+        else if ((double)(sd + (long long)inw[d]) < thr) status = 1;
+      }
+      wave_lds_sync();
+      if (valid) {
+        inw[d] = 0;
+        outw[o] = 0;
+      }
+    }
+    if (__ballot(valid && status == 0) == 0ull) {
+      if (valid) {
+        if (status < 0) {
+          slot_dec[v - r0] = -1;
         } else {
-          sz[oo] -= ww;
-          sz[dd] += ww;
-          part[vv] = dd;
-          nm++;
+          atomicAdd(reinterpret_cast<unsigned long long*>(&sz[o]), (unsigned long long)(-(long long)w));
+          atomicAdd(reinterpret_cast<unsigned long long*>(&sz[d]), (unsigned long long)(long long)w);
+          part[v] = d;
+        }
+      }
+      nm += (unsigned long long)__popcll(__ballot(valid && status > 0));
+    } else {
+      // lane 0 walks the chunk's candidates in order; only it touches sz here
+      const int cnt = min(WAVE, nmv - m0);
+      wave_lds_sync();
+      for (int l = 0; l < cnt; l++) {
+        const int vv = __builtin_amdgcn_readlane(v, l), oo = __builtin_amdgcn_readlane(o, l),
+                  dd = __builtin_amdgcn_readlane(d, l), ww = __builtin_amdgcn_readlane(w, l);
+        if (lane == 0) {
+          if ((double)sz[dd] >= thr) {
+            slot_dec[vv - r0] = -1;
+          } else {
+            sz[oo] -= ww;
+            sz[dd] += ww;
+            part[vv] = dd;
+            nm++;
+          }
         }
       }
     }
+    wave_lds_sync();
   }
-  wave_lds_sync();
   for (int q = lane; q < K; q += WAVE) sizes[q] = sz[q];
   if (lane == 0 && nm) atomicAdd(moves, nm);
 }
@@ -732,7 +780,7 @@ fp_status pass_conn(fp_ctx* ctx, const RefinePlan& p) {
     if (!msm) {
       k_round_compact<<<1, 1024, 0, ctx->stream>>>(ctx->slot_dec, ctx->slot_own, nullptr, ctx->vtx_ptr, r0, nslots,
                                                     c.slot_mi, c.mv, c.mstat);
-      k_tm_admit<<<1, WAVE, (size_t)K * sizeof(long long), ctx->stream>>>(
+      k_tm_admit<<<1, WAVE, (size_t)K * (sizeof(long long) + 2 * sizeof(int)), ctx->stream>>>(
           ctx->slot_dec, c.mv, c.mstat, ctx->vwgt, r0, K, thr, ctx->sizes, ctx->part, ctx->moves);
       ap.mode = 0;
       k_conn_apply<<<REPLAY_NET_BLOCKS, REPLAY_BLOCK, lds_r, ctx->stream>>>(ap);

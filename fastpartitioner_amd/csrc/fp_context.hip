@@ -265,6 +265,7 @@ fp_status fp_upload(fp_ctx* ctx, int nov, int noe, const unsigned* net_ptr, cons
   ctx->conn.rmax = 0;
   ctx->conn.evcap = 0;
   ctx->conn.incoff_ready = false;
+  ctx->lam = nullptr;
   if (net_ptr[0] != 0) return fail(ctx, FP_ERR_ARG, "fp_upload: net_ptr[0] != 0");
   unsigned max_net = 0;
   for (int n = 0; n < noe; n++) {

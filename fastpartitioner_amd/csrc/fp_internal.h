@@ -388,6 +388,7 @@ struct fp_ctx {
   // metric scratch
   unsigned* p2p = nullptr;
   unsigned long long* mvec = nullptr;  // 5*K + 2
+  int* lam = nullptr;                  // noe: per-net lambda of the last fp_eval
 
   // timing
   bool kernel_timing = false;

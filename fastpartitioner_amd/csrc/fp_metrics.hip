@@ -1,7 +1,8 @@
 // fp_metrics.hip -- the six metrics of shared/metric.cpp from scratch on the
 // device: one wavefront per net sweeps its pins (coalesced), deduplicates the
 // pins' parts through an LDS mark array, and accumulates lambda, TV, CN and
-// the per-part int64 send / receive volume and message vectors.
+// the per-part int64 send / receive volume and message vectors (block-private
+// in LDS up to K = 1024, straight to HBM above).
 #include "fp_internal.h"
 
 namespace fpe {
@@ -25,16 +26,20 @@ struct MetricArgs {
   unsigned long long* tvcn;    // [0] TV, [1] CN
 };
 
+constexpr int METRIC_LDS_KMAX = 1024;
+
+template <bool LDS_ACC>
 __global__ void __launch_bounds__(METRIC_BLOCK) k_metrics(MetricArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int K = a.K;
-  unsigned long long* s_send = reinterpret_cast<unsigned long long*>(lds);
-  unsigned long long* s_recv = s_send + K;
-  unsigned long long* s_msv = s_recv + K;
-  int* marks = reinterpret_cast<int*>(s_msv + K);
+  unsigned long long* s_send = LDS_ACC ? reinterpret_cast<unsigned long long*>(lds) : a.send;
+  unsigned long long* s_recv = LDS_ACC ? s_send + K : a.recv;
+  unsigned long long* s_msv = LDS_ACC ? s_recv + K : a.send_msv;
+  int* marks = LDS_ACC ? reinterpret_cast<int*>(s_msv + K) : reinterpret_cast<int*>(lds);
   const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
   int* mark = marks + (size_t)wave * (size_t)K;
-  for (int q = (int)threadIdx.x; q < K; q += (int)blockDim.x) { s_send[q] = 0; s_recv[q] = 0; s_msv[q] = 0; }
+  if (LDS_ACC)
+    for (int q = (int)threadIdx.x; q < K; q += (int)blockDim.x) { s_send[q] = 0; s_recv[q] = 0; s_msv[q] = 0; }
   for (int q = lane; q < K; q += WAVE) mark[q] = -1;
   __syncthreads();
   unsigned long long tv = 0, cn = 0;
@@ -57,8 +62,10 @@ __global__ void __launch_bounds__(METRIC_BLOCK) k_metrics(MetricArgs a) {
       if (fresh && sp >= 0 && q != sp) {
         atomicAdd(&s_recv[q], 1ull);
         const unsigned bit = (unsigned)sp * (unsigned)K + (unsigned)q;
-        const unsigned old = atomicOr(&a.p2p[bit >> 5], 1u << (bit & 31));
-        if (!(old & (1u << (bit & 31)))) {
+        // the pair is usually known already: a plain read skips the atomic
+        // (a stale 0 only costs the atomic, whose result decides)
+        const unsigned m = 1u << (bit & 31);
+        if (!(__builtin_nontemporal_load(&a.p2p[bit >> 5]) & m) && !(atomicOr(&a.p2p[bit >> 5], m) & m)) {
           atomicAdd(&a.sm[sp], 1ull);
           atomicAdd(&a.rm[q], 1ull);
         }
@@ -75,11 +82,13 @@ __global__ void __launch_bounds__(METRIC_BLOCK) k_metrics(MetricArgs a) {
       }
     }
   }
-  __syncthreads();
-  for (int q = (int)threadIdx.x; q < K; q += (int)blockDim.x) {
-    if (s_send[q]) atomicAdd(&a.send[q], s_send[q]);
-    if (s_recv[q]) atomicAdd(&a.recv[q], s_recv[q]);
-    if (s_msv[q]) atomicAdd(&a.send_msv[q], s_msv[q]);
+  if (LDS_ACC) {
+    __syncthreads();
+    for (int q = (int)threadIdx.x; q < K; q += (int)blockDim.x) {
+      if (s_send[q]) atomicAdd(&a.send[q], s_send[q]);
+      if (s_recv[q]) atomicAdd(&a.recv[q], s_recv[q]);
+      if (s_msv[q]) atomicAdd(&a.send_msv[q], s_msv[q]);
+    }
   }
   if (lane == 0) {
     if (tv) atomicAdd(&a.tvcn[0], tv);
@@ -101,8 +110,15 @@ fp_status fp_eval(fp_ctx* ctx, fp_metrics* out, int* lam, long long* sv, long lo
   if (!ctx->has_state) return fail(ctx, FP_ERR_STATE, "fp_eval before a state was set");
   FP_HIP_OK(hipSetDevice(ctx->device));
   const int K = ctx->K;
-  int* d_lam = nullptr;
-  FP_HIP_OK(hipMalloc(&d_lam, sizeof(int) * (size_t)std::max(ctx->noe, 1)));
+  const bool lds_acc = K <= METRIC_LDS_KMAX;
+  // mark arrays (K ints per wavefront) plus, up to METRIC_LDS_KMAX, three
+  // block-private int64 vectors
+  const size_t acc_bytes = lds_acc ? (size_t)3 * K * sizeof(unsigned long long) : 0;
+  int waves = METRIC_BLOCK / WAVE;
+  while (waves > 0 && acc_bytes + (size_t)waves * K * sizeof(int) > (size_t)LDS_BUDGET) waves--;
+  if (waves == 0) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d too large for the metric kernel", K);
+  if (!ctx->lam) FP_TRY(dalloc(ctx, &ctx->lam, (size_t)std::max(ctx->noe, 1)));
+  int* d_lam = ctx->lam;
   fill_async<unsigned>(ctx, ctx->p2p, ((size_t)K * (size_t)K + 31) / 32, 0u);
   fill_async<unsigned long long>(ctx, ctx->mvec, (size_t)5 * (size_t)K + 2, 0ull);
   MetricArgs m{};
@@ -121,21 +137,18 @@ fp_status fp_eval(fp_ctx* ctx, fp_metrics* out, int* lam, long long* sv, long lo
   m.rm = ctx->mvec + 3 * (size_t)K;
   m.send_msv = ctx->mvec + 4 * (size_t)K;
   m.tvcn = ctx->mvec + 5 * (size_t)K;
-  const int waves = METRIC_BLOCK / WAVE;
-  const size_t lds = (size_t)3 * K * sizeof(unsigned long long) + (size_t)waves * K * sizeof(int);
-  if (lds > (size_t)LDS_BUDGET) {
-    (void)hipFree(d_lam);
-    return fail(ctx, FP_ERR_UNSUPPORTED, "K too large for the metric kernel");
-  }
+  const size_t lds = acc_bytes + (size_t)waves * K * sizeof(int);
   const int grid = std::max(1, std::min((ctx->noe + waves - 1) / waves, 4096));
-  if (ctx->noe > 0) k_metrics<<<grid, METRIC_BLOCK, lds, ctx->stream>>>(m);
+  if (ctx->noe > 0) {
+    if (lds_acc) k_metrics<true><<<grid, waves * WAVE, lds, ctx->stream>>>(m);
+    else k_metrics<false><<<grid, waves * WAVE, lds, ctx->stream>>>(m);
+  }
   std::vector<unsigned long long> h((size_t)5 * K + 2);
   fp_status st = FP_OK;
   if (hipGetLastError() != hipSuccess) st = fail(ctx, FP_ERR_HIP, "k_metrics launch failed");
   if (st == FP_OK) st = d2h(ctx, h.data(), ctx->mvec, h.size());
   if (st == FP_OK && lam) st = d2h(ctx, lam, d_lam, (size_t)ctx->noe);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess && st == FP_OK) st = fail(ctx, FP_ERR_HIP, "sync failed");
-  (void)hipFree(d_lam);
   if (st != FP_OK) return st;
   const long long* send = reinterpret_cast<const long long*>(h.data());
   const long long* recv = send + K;

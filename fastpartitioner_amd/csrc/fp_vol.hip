@@ -291,6 +291,12 @@ __global__ void __launch_bounds__(VOL_BLOCK) k_vol_eval(VolEvalArgs a) {
   }
 }
 
+// First word of a volume event: the net's source part (or -1) and its change
+// (-1, 0 or +1) in one int.
+__device__ __forceinline__ int vol_event_word(int sp, int dsp) { return ((sp + 1) << 2) | (dsp + 1); }
+__device__ __forceinline__ int vol_event_part(int x) { return (x >> 2) - 1; }
+__device__ __forceinline__ int vol_event_change(int x) { return (x & 3) - 1; }
+
 struct VolReplayArgs {
   const unsigned* net_ptr;
   const int* apply_pins;
@@ -316,8 +322,8 @@ struct VolReplayArgs {
 
 // dry (commit = 0): per (move, net) volume changes of the reference's apply
 //   (MGV/partition.cpp:897-975; MSRV/partition.cpp:952-1052), recorded as
-//   (source part or -1, change of own part, change of new part, change of
-//   source part) at event slot mv.evoff + (position of the net in the move's
+//   (source part and its change, change of own part, change of new part,
+//   move index) at event slot mv.evoff + (position of the net in the move's
 //   vertex->nets list); the list edits stay in LDS
 // commit (commit = 1): list edits of the moves before the stop; blocks past
 //   net_blocks commit part vector and both part sizes
@@ -403,7 +409,9 @@ __global__ void __launch_bounds__(REPLAY_BLOCK) k_vol_replay(VolReplayArgs a) {
             e_sp += 1;                                        // send_volumes[net_src_part]++
             if (a.msrv) e_dec += 1;                           // received_volumes[decision]++
           }
-          if (lane == 0) a.evbuf[(unsigned)__builtin_amdgcn_readlane((int)evslot, l)] = make_int4(spx, e_own, e_dec, e_sp);
+          if (lane == 0)
+            a.evbuf[(unsigned)__builtin_amdgcn_readlane((int)evslot, l)] =
+                make_int4(vol_event_word(spx, e_sp), e_own, e_dec, __builtin_amdgcn_readlane(mi, l));
           if (isrc) sp_cur = d;
         }
         wave_list_move(list, L, own, d, a.pb, a.pmask);
@@ -415,23 +423,173 @@ __global__ void __launch_bounds__(REPLAY_BLOCK) k_vol_replay(VolReplayArgs a) {
   }
 }
 
+// LDS of k_vol_scan's batched step, after vol[K] and dirty[64]: per-part
+// bounds of a batch's changes, candidate map, and the candidates x moves
+// table of values.
+struct VolBatchLds {
+  int* pos;    // [K] sum of the batch's positive changes of a part
+  int* neg;    // [K] sum of its negative changes
+  int* cidx;   // [K] candidate index of a part or -1
+  int* cpart;  // [WAVE] part of a candidate
+  int* cflag;  // [WAVE] the candidate is in the max set at the batch start
+  int2* mvod;  // [WAVE] (own part, new part) of the batch's moves
+  int* D;      // [WAVE][WAVE + 1] per candidate, change and then value after each move
+};
+constexpr int VOL_BATCH_KMAX = 1024;  // the batched step's lane holds at most 16 parts
+constexpr int DROW = WAVE + 1;        // padded row of D: a lane per row is conflict-free
+
+__host__ __device__ constexpr size_t vol_batch_ints(int K) { return (size_t)3 * (size_t)K + 4 * WAVE + WAVE * DROW; }
+
+// Applies f(part, change) to the three part changes of a staged event of
+// move j: own part, new part and the net's source part.
+template <class F>
+__device__ __forceinline__ void vol_event_changes(const int4 x, const int2 od, F f) {
+  if (x.y) f(od.x, x.y);
+  if (x.z) f(od.y, x.z);
+  const int sp = vol_event_part(x.x), dsp = vol_event_change(x.x);
+  if (sp >= 0 && dsp) f(sp, dsp);
+}
+
+// The moves mi0 .. mi0 + nb - 1 of the round, their tot events staged (in
+// order) in stage, the max set tracked by ms. Exact without going move by
+// move: with A_low = max over the max set S of (value + negative changes),
+// a part outside S whose value + positive changes stays below A_low never
+// reaches the max within the batch (the max is at least A_low until the
+// round stops). The others (the candidates, S included) are followed move by
+// move in a candidates x moves table: before the round stops, the max after
+// move t is the largest value of S's candidates, the max set is the S
+// candidates holding it, and move t stops the round iff a candidate outside
+// S reaches it or an S candidate not in the max set after move t - 1 is in
+// it after t. Applies the moves up to the stop (all if none) and returns the
+// batch index of the stop move, -1 if none, or -2 (nothing applied) when
+// the candidates do not fit a wave.
+__device__ int vol_scan_batch(int* vol, int K, const int4* stage, int tot, int nb, int mi0, int own_l, int d_l,
+                              int cum_l, const VolBatchLds& B, LaneMax& ms, int& M) {
+  const int lane = lane_id();
+  if (lane < nb) B.mvod[lane] = make_int2(own_l, d_l);
+  wave_lds_sync();
+  for (int e = lane; e < tot; e += WAVE) {
+    const int4 x = stage[e];
+    vol_event_changes(x, B.mvod[x.w - mi0], [&](int p, int w) {
+      if (w > 0) atomicAdd(&B.pos[p], w);
+      else atomicAdd(&B.neg[p], w);
+    });
+  }
+  wave_lds_sync();
+  int al = INT_MIN;
+  for (unsigned long long b = ms.inS; b; b &= b - 1) {
+    const int p = lane + WAVE * __builtin_ctzll(b);
+    al = max(al, vol[p] + B.neg[p]);
+  }
+  const int alow = wave_max(al);
+  unsigned long long cb = 0;
+  {
+    int j = 0;
+    for (int p = lane; p < K; p += WAVE, j++) {
+      const bool inS = (ms.inS >> j) & 1ull;
+      if (inS || vol[p] + B.pos[p] >= alow) cb |= 1ull << j;
+      B.pos[p] = 0;
+      B.neg[p] = 0;
+    }
+  }
+  const int nl = __popcll(cb);
+  const int incl = wave_incl_scan(nl);
+  const int nc = __builtin_amdgcn_readlane(incl, WAVE - 1);
+  if (nc > WAVE) return -2;
+  {
+    int c = incl - nl;
+    for (unsigned long long b = cb; b; b &= b - 1, c++) {
+      const int j = __builtin_ctzll(b);
+      const int p = lane + WAVE * j;
+      B.cidx[p] = c;
+      B.cpart[c] = p;
+      B.cflag[c] = (int)((ms.inS >> j) & 1ull);
+    }
+  }
+  if (lane < nb)
+    for (int r = 0; r < nc; r++) B.D[r * DROW + lane] = 0;
+  wave_lds_sync();
+  for (int e = lane; e < tot; e += WAVE) {
+    const int4 x = stage[e];
+    const int j = x.w - mi0;
+    vol_event_changes(x, B.mvod[j], [&](int p, int w) {
+      const int ci = B.cidx[p];
+      if (ci >= 0) atomicAdd(&B.D[ci * DROW + j], w);
+    });
+  }
+  wave_lds_sync();
+  if (lane < nc) {
+    int v = vol[B.cpart[lane]];
+    for (int t = 0; t < nb; t++) {
+      v += B.D[lane * DROW + t];
+      B.D[lane * DROW + t] = v;
+    }
+  }
+  wave_lds_sync();
+  // lane t: the max and max set after move t, and whether t stops the round
+  int A = INT_MIN, Bm = INT_MIN;
+  unsigned long long mask = 0, m0 = 0;
+  for (int r = 0; r < nc; r++) {
+    const bool s0 = B.cflag[r] != 0;
+    const int v = lane < nb ? B.D[r * DROW + lane] : 0;
+    if (s0) {
+      m0 |= 1ull << r;
+      mask = v > A ? 1ull << r : (v == A ? mask | 1ull << r : mask);
+      A = max(A, v);
+    } else {
+      Bm = max(Bm, v);
+    }
+  }
+  unsigned long long prev = ((unsigned long long)(unsigned)__shfl_up((int)(unsigned)(mask >> 32), 1) << 32) |
+                            (unsigned)__shfl_up((int)(unsigned)(mask & 0xFFFFFFFFull), 1);
+  if (lane == 0) prev = m0;
+  const unsigned long long sb = __ballot(lane < nb && (Bm >= A || (mask & ~prev) != 0ull));
+  const int tstop = sb ? (int)__builtin_ctzll(sb) : -1;
+  const int eend = __builtin_amdgcn_readlane(cum_l, sb ? tstop : nb - 1);
+  for (int e = lane; e < eend; e += WAVE) {
+    const int4 x = stage[e];
+    vol_event_changes(x, B.mvod[x.w - mi0], [&](int p, int w) { atomicAdd(&vol[p], w); });
+  }
+  if (lane < nc) B.cidx[B.cpart[lane]] = -1;
+  wave_lds_sync();
+  M = maxset_init(vol, K, ms);
+  return tstop;
+}
+
 // One wavefront: the round's moves in order, each followed by the max-set
 // update of MGV/partition.cpp:977-1003; stops after the first move whose max
 // set holds a part the previous max set did not. The events of consecutive
-// moves are contiguous in evbuf, so they are staged into LDS a batch of moves
-// at a time (stage_cap events); a move with more events than fit is read from
-// HBM. Writes back vol, the max and its multiplicity, and the number of moves
-// to commit (mstat[3]).
+// moves are contiguous in evbuf, so they are staged into LDS a batch of up to
+// 64 moves at a time (stage_cap events). With batched != 0 (K <= 1024) a
+// staged batch goes through vol_scan_batch; otherwise, or when its
+// candidates do not fit, move by move (a move with more events than the
+// stage holds is read from HBM). Writes back vol, the max and its
+// multiplicity, and the number of moves to commit (mstat[3]).
 __global__ void __launch_bounds__(WAVE) k_vol_scan(int* __restrict__ vol_g, int K, const int4* __restrict__ mv,
                                                    const int4* __restrict__ evbuf,
                                                    const unsigned* __restrict__ vtx_ptr, int* __restrict__ mstat,
-                                                   int stage_cap) {
+                                                   int stage_cap, int batched) {
   extern __shared__ __align__(16) unsigned char lds[];
   int4* stage = reinterpret_cast<int4*>(lds);
   int* vol = reinterpret_cast<int*>(stage + stage_cap);
   int* dirty = vol + K;
+  VolBatchLds B{};
+  B.pos = dirty + WAVE;
+  B.neg = B.pos + K;
+  B.cidx = B.neg + K;
+  B.cpart = B.cidx + K;
+  B.cflag = B.cpart + WAVE;
+  B.mvod = reinterpret_cast<int2*>(B.cflag + WAVE);
+  B.D = reinterpret_cast<int*>(B.mvod + WAVE);
   const int lane = lane_id();
-  for (int q = lane; q < K; q += WAVE) vol[q] = vol_g[q];
+  for (int q = lane; q < K; q += WAVE) {
+    vol[q] = vol_g[q];
+    if (batched) {
+      B.pos[q] = 0;
+      B.neg[q] = 0;
+      B.cidx[q] = -1;
+    }
+  }
   dirty[lane] = 0;
   wave_lds_sync();
   LaneMax ms;
@@ -448,12 +606,19 @@ __global__ void __launch_bounds__(WAVE) k_vol_scan(int* __restrict__ vol_g, int 
     const bool staged = nb > 0;
     if (!staged) nb = 1;
     const unsigned eb0 = (unsigned)__builtin_amdgcn_readfirstlane(hm.w);
+    int tot = 0;
     if (staged) {
-      const int tot = __builtin_amdgcn_readlane(cum, nb - 1);
+      tot = __builtin_amdgcn_readlane(cum, nb - 1);
       for (int t = lane; t < tot; t += WAVE) stage[t] = evbuf[eb0 + (unsigned)t];
       wave_lds_sync();
     }
-    for (int j = 0; j < nb; j++) {
+    int bstop = -2;
+    if (batched && staged) bstop = vol_scan_batch(vol, K, stage, tot, nb, mi, hm.y, hm.z, cum, B, ms, M);
+    if (bstop >= 0) {
+      nstop = mi + bstop + 1;
+      stopped = true;
+    }
+    for (int j = 0; bstop == -2 && j < nb; j++) {
       const int own = __builtin_amdgcn_readlane(hm.y, j), d = __builtin_amdgcn_readlane(hm.z, j);
       const unsigned eb = (unsigned)__builtin_amdgcn_readlane(hm.w, j);
       const int deg = __builtin_amdgcn_readlane(hdeg, j);
@@ -465,9 +630,10 @@ __global__ void __launch_bounds__(WAVE) k_vol_scan(int* __restrict__ vol_g, int 
           const int4 e = evs[i];
           s_own += e.y;
           s_dec += e.z;
-          if (e.x >= 0 && e.w != 0) {
-            atomicAdd(&vol[e.x], e.w);
-            dirty[e.x & (WAVE - 1)] = 1;
+          const int sp = vol_event_part(e.x), dsp = vol_event_change(e.x);
+          if (sp >= 0 && dsp != 0) {
+            atomicAdd(&vol[sp], dsp);
+            dirty[sp & (WAVE - 1)] = 1;
           }
         }
       }
@@ -568,7 +734,8 @@ fp_status pass_vol(fp_ctx* ctx, const RefinePlan& p) {
   const int nov = ctx->nov, K = ctx->K, R = p.R, waves = p.eval_waves;
   const size_t lds_e = (size_t)K * 16 + (size_t)waves * 8 * (size_t)K;
   const size_t lds_r = (size_t)(REPLAY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
-  const size_t lds_vol = ((size_t)K + WAVE) * sizeof(int);
+  const int batched = K <= VOL_BATCH_KMAX ? 1 : 0;
+  const size_t lds_vol = ((size_t)K + WAVE + (batched ? vol_batch_ints(K) : 0)) * sizeof(int);
   const int stage_cap = (int)std::min<size_t>(4096, ((size_t)LDS_BUDGET - lds_vol) / sizeof(int4));
   const size_t lds_s = lds_vol + (size_t)stage_cap * sizeof(int4);
   auto& vs = ctx->vol;
@@ -636,7 +803,8 @@ fp_status pass_vol(fp_ctx* ctx, const RefinePlan& p) {
     ra.net_blocks = REPLAY_NET_BLOCKS;
     ra.commit = 0;
     k_vol_replay<<<REPLAY_NET_BLOCKS, REPLAY_BLOCK, lds_r, ctx->stream>>>(ra);
-    k_vol_scan<<<1, WAVE, lds_s, ctx->stream>>>(vs.vol, K, vs.mv, vs.evbuf, ctx->vtx_ptr, vs.mstat, stage_cap);
+    k_vol_scan<<<1, WAVE, lds_s, ctx->stream>>>(vs.vol, K, vs.mv, vs.evbuf, ctx->vtx_ptr, vs.mstat, stage_cap,
+                                                 batched);
     ra.commit = 1;
     k_vol_replay<<<REPLAY_NET_BLOCKS + (nslots + REPLAY_BLOCK - 1) / REPLAY_BLOCK, REPLAY_BLOCK, lds_r,
                    ctx->stream>>>(ra);

@@ -61,18 +61,20 @@ def test_tv_bal_schedule(R):
     assert np.array_equal(got.sizes, st.sizes)
 
 
-def test_metrics_match_oracle():
+@pytest.mark.parametrize("K", [8, 64, 1024, 2048, 4096])
+def test_metrics_match_oracle(K):
+    # K <= 1024: block-private vectors in LDS; above: straight to HBM
     hg = H.workload("c2", scale=0.02)
-    K = 64
     rng = np.random.default_rng(3)
     part = rng.integers(0, K, hg.nov).astype(np.int32)
     with Engine(0) as eng:
         eng.upload(hg)
         eng.set_partition(K, part)
         m = eng.metrics(vectors=True)
+        again = eng.metrics()
     r = O.metrics(hg, K, part)
     for k in O.METRIC_NAMES:
-        assert m[k] == r[k], k
+        assert m[k] == r[k] == again[k], k
     for k in ("lam", "sv", "rv", "sm", "rm", "sv_msv"):
         assert np.array_equal(m[k], r[k]), k
 

@@ -73,3 +73,17 @@ def test_msrv_permuted_ordering():
 def test_msrv_zipf():
     hg = H.workload("c2", scale=0.02)
     _run("MSRV", hg, 64, 4, 2048)
+
+
+@pytest.mark.parametrize("K", [512, 1024, 2048])
+def test_msv_many_parts(K):
+    # K <= 1024 takes the scan's batched step (with the move-by-move fallback
+    # when many parts tie near the max), K = 2048 the move-by-move scan
+    hg = H.colnet(20000, H.DEG_ZIPF, 2, 5000, 2.3, seed=2001)
+    assert _run("MSV", hg, K, 3, 4096) > 0
+
+
+@pytest.mark.parametrize("K", [256, 2048])
+def test_msrv_many_parts(K):
+    hg = H.colnet(20000, H.DEG_ZIPF, 2, 5000, 2.3, seed=2002)
+    _run("MSRV", hg, K, 3, 4096)
