@@ -310,9 +310,11 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
 // sizes of the others. Chunks of 64 candidates: with in[p] / out[p] the
 // chunk's candidate weight into / out of part p, a candidate into d is
 // dropped whatever the chunk's other decisions are when sz[d] - out[d] >=
-// thr, and admitted when sz[d] + in[d] < thr; a chunk decided that way for
-// all its candidates is applied in parallel, any other chunk candidate by
-// candidate.
+// thr, and admitted when sz[d] + in[d] < thr. The destinations of the
+// undecided candidates are "open"; the chunk's admitted candidates that touch
+// no open part are applied in parallel, then lane 0 walks the rest in order
+// (undecided ones get their decision from the open part's size at that
+// point).
 __global__ void __launch_bounds__(WAVE) k_tm_admit(int* __restrict__ slot_dec, const int4* __restrict__ mv,
                                                    const int* __restrict__ mstat, const int* __restrict__ vwgt,
                                                    int r0, int K, double thr, long long* __restrict__ sizes,
@@ -336,9 +338,10 @@ __global__ void __launch_bounds__(WAVE) k_tm_admit(int* __restrict__ slot_dec, c
     const int4 x = valid ? mv[i] : make_int4(0, 0, 0, 0);
     const int w = valid ? (vwgt ? vwgt[x.x] : 1) : 0;
     const int v = x.x, o = x.y, d = x.z;
-    // chunk sums stay exact in int for weights up to 2^24
+    // chunk sums stay exact in int for weights up to 2^24; otherwise every
+    // candidate of the chunk stays undecided
     const bool bounded = __ballot(valid && (unsigned)w > (1u << 24)) == 0ull;
-    int status = 0;  // 1 admitted, -1 dropped, 0 open
+    int status = 0;  // 1 admitted, -1 dropped, 0 undecided
     if (bounded) {
       if (valid) {
         atomicAdd(&inw[d], w);
@@ -355,34 +358,38 @@ __global__ void __launch_bounds__(WAVE) k_tm_admit(int* __restrict__ slot_dec, c
         inw[d] = 0;
         outw[o] = 0;
       }
-    }
-    if (__ballot(valid && status == 0) == 0ull) {
-      if (valid) {
-        if (status < 0) {
-          slot_dec[v - r0] = -1;
-        } else {
-          atomicAdd(reinterpret_cast<unsigned long long*>(&sz[o]), (unsigned long long)(-(long long)w));
-          atomicAdd(reinterpret_cast<unsigned long long*>(&sz[d]), (unsigned long long)(long long)w);
-          part[v] = d;
-        }
-      }
-      nm += (unsigned long long)__popcll(__ballot(valid && status > 0));
-    } else {
-      // lane 0 walks the chunk's candidates in order; only it touches sz here
-      const int cnt = min(WAVE, nmv - m0);
       wave_lds_sync();
-      for (int l = 0; l < cnt; l++) {
-        const int vv = __builtin_amdgcn_readlane(v, l), oo = __builtin_amdgcn_readlane(o, l),
-                  dd = __builtin_amdgcn_readlane(d, l), ww = __builtin_amdgcn_readlane(w, l);
-        if (lane == 0) {
-          if ((double)sz[dd] >= thr) {
-            slot_dec[vv - r0] = -1;
-          } else {
-            sz[oo] -= ww;
-            sz[dd] += ww;
-            part[vv] = dd;
-            nm++;
-          }
+    }
+    // open parts: inw doubles as the flag array (all 0 here)
+    if (valid && status == 0) inw[d] = 1;
+    wave_lds_sync();
+    const bool walk = valid && (status == 0 || (status > 0 && (inw[o] != 0 || inw[d] != 0)));
+    if (valid && !walk) {
+      if (status < 0) {
+        slot_dec[v - r0] = -1;
+      } else {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&sz[o]), (unsigned long long)(-(long long)w));
+        atomicAdd(reinterpret_cast<unsigned long long*>(&sz[d]), (unsigned long long)(long long)w);
+        part[v] = d;
+      }
+    }
+    nm += (unsigned long long)__popcll(__ballot(valid && !walk && status > 0));
+    wave_lds_sync();
+    if (valid && status == 0) inw[d] = 0;
+    // lane 0 walks the rest in order; only it touches sz meanwhile
+    for (unsigned long long b = __ballot(walk); b; b &= b - 1) {
+      const int l = __builtin_amdgcn_readfirstlane((int)__builtin_ctzll(b));
+      const int vv = __builtin_amdgcn_readlane(v, l), oo = __builtin_amdgcn_readlane(o, l),
+                dd = __builtin_amdgcn_readlane(d, l), ww = __builtin_amdgcn_readlane(w, l),
+                st = __builtin_amdgcn_readlane(status, l);
+      if (lane == 0) {
+        if (st == 0 && (double)sz[dd] >= thr) {
+          slot_dec[vv - r0] = -1;
+        } else {
+          sz[oo] -= ww;
+          sz[dd] += ww;
+          part[vv] = dd;
+          nm++;
         }
       }
     }
