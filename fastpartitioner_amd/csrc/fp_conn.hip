@@ -150,17 +150,27 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
     const unsigned vb = a.vtx_ptr[v];
     const int deg = (int)(a.vtx_ptr[v + 1] - vb);
     int nrecv = 0, nsend = 0, gain_own = 0, gain_s = 0;
-    for (int i = 0; i < deg; i++) {
-      const int n = (int)a.vtx_nets[vb + (unsigned)i];
-      const unsigned sg = a.vtx_seg[vb + (unsigned)i];
-      const int L = (int)a.ev[sg];
-      const int s = a.src[n];
-      if (s == v) {
-        for (int t0 = 0; t0 < L; t0 += WAVE) {
-          const int t = t0 + lane;
-          const bool ok = t < L;
-          const unsigned x = ok ? a.ev[sg + 1u + (unsigned)t] : 0u;
-          const int q = (int)(x & ~EV_SINGLE);
+    // lane j takes net j of each chunk of v's nets: its segment, source and
+    // source part; the lists are swept with several nets in flight
+    for (int c0 = 0; c0 < deg; c0 += WAVE) {
+      const int nn = min(WAVE, deg - c0);
+      unsigned sg = 0;
+      int ps = -1;
+      bool isrc = false;
+      if (lane < nn) {
+        const unsigned k = vb + (unsigned)(c0 + lane);
+        sg = a.vtx_seg[k];
+        const int s = a.src[a.vtx_nets[k]];
+        isrc = s == v;
+        if (!isrc) ps = a.part[s];
+      }
+      const unsigned long long srcmask = __ballot(isrc);
+      unsigned long long smask = 0;  // nets where v is own's only pin
+      wave_sweep_lists<8, 4>(a.ev, sg, nn, [&](int j, int, int, bool ok, unsigned x) {
+        const int q = (int)(x & ~EV_SINGLE);
+        if ((srcmask >> j) & 1ull) {
+          // v's own source net: receivers are the listed parts other than own,
+          // and own if it keeps other pins there
           const bool rec = ok && (q != own || !(x & EV_SINGLE));
           bool isnew = false;
           if (rec) {
@@ -171,32 +181,30 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
           if (isnew) recv[nrecv + __popcll(bm & lt)] = q;
           nrecv += __popcll(bm);
           if (ok && q != own && a.pc[(size_t)own * (size_t)K + (size_t)q] == 1) gain_own++;
+        } else if (__ballot(ok && q == own && (x & EV_SINGLE))) {
+          smask |= 1ull << j;
         }
-      } else {
-        const int ps = a.part[s];
-        const bool first = !bit_of(smark, ps);
-        wave_lds_sync();
-        if (first) {
-          if (lane == 0) {
-            smark[ps >> 5] |= 1u << (ps & 31);
-            send[nsend] = ps;
-          }
-          nsend++;
-          bool single = false;
-          for (int t0 = 0; t0 < L; t0 += WAVE) {
-            const int t = t0 + lane;
-            const bool ok = t < L;
-            const unsigned x = ok ? a.ev[sg + 1u + (unsigned)t] : 0u;
-            single = single || __ballot(ok && (int)(x & ~EV_SINGLE) == own && (x & EV_SINGLE)) != 0;
-          }
-          if (single && a.pc[(size_t)ps * (size_t)K + (size_t)own] == 1) {
-            gain_s++;
-            if (lane == 0) sgain[ps >> 5] |= 1u << (ps & 31);
-          }
-        }
-        wave_lds_sync();
+      });
+      // senders: the first net of each source part in v's net order
+      bool first = lane < nn && !isrc && !bit_of(smark, ps);
+      for (unsigned long long m = ~srcmask & (nn == WAVE ? ~0ull : ((1ull << nn) - 1ull)); m; m &= m - 1) {
+        const int jj = __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1);
+        if (lane > jj && ps == __builtin_amdgcn_readlane(ps, jj)) first = false;
       }
+      wave_lds_sync();
+      const unsigned long long fm = __ballot(first);
+      if (first) {
+        send[nsend + __popcll(fm & lt)] = ps;
+        atomicOr(&smark[ps >> 5], 1u << (ps & 31));
+        if (((smask >> lane) & 1ull) && a.pc[(size_t)ps * (size_t)K + (size_t)own] == 1) {
+          gain_s++;
+          atomicOr(&sgain[ps >> 5], 1u << (ps & 31));
+        }
+      }
+      nsend += __popcll(fm);
+      wave_lds_sync();
     }
+    gain_s = wave_sum(gain_s);
     gain_own = wave_sum(gain_own);
     const int gain = gain_own + gain_s;
     wave_lds_sync();

@@ -259,6 +259,56 @@ __device__ __forceinline__ unsigned lane_lower_bound(const unsigned* __restrict_
   return lo;
 }
 
+// Sweeps the 16-bit lists (evaluation view) of nn <= 64 nets whose segment
+// offsets lane j of `sg` holds: NIF segment heads are loaded together (a
+// 64-word window brings lambda in lane 0 and the first 63 entries), and
+// lists longer than 63 entries are finished after in windows of 64, LW at a
+// time. For every window f(j, L, t, ok, x) runs on all lanes: j = net index
+// (wave-uniform), L = its lambda, t = this lane's list position, ok = t < L,
+// x = the entry. A window may read up to 63 words past a segment: the view
+// has ENT_PAD words of tail padding.
+template <int NIF, int LW, class F>
+__device__ __forceinline__ void wave_sweep_lists(const unsigned short* __restrict__ ev, unsigned sg, int nn, F f) {
+  const int lane = lane_id();
+  unsigned long long longmask = 0;
+  for (int j0 = 0; j0 < nn; j0 += NIF) {
+    unsigned xv[NIF];
+#pragma unroll
+    for (int u = 0; u < NIF; u++) {
+      const unsigned sj = (unsigned)__builtin_amdgcn_readlane((int)sg, min(j0 + u, nn - 1));
+      xv[u] = ev[sj + (unsigned)lane];
+    }
+#pragma unroll
+    for (int u = 0; u < NIF; u++) {
+      const int j = j0 + u;
+      if (j < nn) {
+        const int Lj = __builtin_amdgcn_readlane((int)xv[u], 0);
+        if (Lj > WAVE - 1) longmask |= 1ull << j;
+        f(j, Lj, lane - 1, lane >= 1 && lane - 1 < Lj, xv[u]);
+      }
+    }
+  }
+  while (longmask) {
+    const int j = __builtin_amdgcn_readfirstlane(__ffsll((long long)longmask) - 1);
+    longmask &= longmask - 1;
+    const unsigned sj = (unsigned)__builtin_amdgcn_readlane((int)sg, j);
+    const int Lj = (int)ev[sj];
+    for (int t0 = WAVE - 1; t0 < Lj; t0 += WAVE * LW) {
+      unsigned xw[LW];
+#pragma unroll
+      for (int u = 0; u < LW; u++) {
+        const int t = t0 + u * WAVE + lane;
+        xw[u] = ev[sj + 1u + (unsigned)(t < Lj ? t : 0)];
+      }
+#pragma unroll
+      for (int u = 0; u < LW; u++) {
+        const int t = t0 + u * WAVE + lane;
+        f(j, Lj, t, t < Lj, xw[u]);
+      }
+    }
+  }
+}
+
 // Range [lo, hi) of apply_pins[s..e) (a net's pin ranks, ascending) that
 // may hold ranks in [r0, r1): the exact range for nets longer than a wave,
 // the whole net otherwise (the caller tests each rank against [r0, r1)).

@@ -143,18 +143,30 @@ __global__ void __launch_bounds__(VOL_BLOCK) k_vol_eval(VolEvalArgs a) {
     const int own = a.part[v];
     const unsigned vb = a.vtx_ptr[v];
     const int deg = (int)(a.vtx_ptr[v + 1] - vb);
+    // this lane's net of the first chunk: segment offset and source part
+    // (own for v's own source nets), kept for the later passes
+    const unsigned sg0 = lane < deg ? a.vtx_seg[vb + (unsigned)lane] : 0u;
+    int ip0 = -1;
+    bool isrc0 = false;
+    if (lane < deg) {
+      const int s = a.src[a.vtx_nets[vb + (unsigned)lane]];
+      isrc0 = s == v;
+      ip0 = isrc0 ? own : a.part[s];
+    }
 
     // scores: pin counts per part over v's nets, first appearance ranked by
     // (net index, list position) as the reference's scan meets them
     // (MGV/partition.cpp:765-790)
-    for (int i = 0; i < deg; i++) {
-      const unsigned sg = a.vtx_seg[vb + (unsigned)i];
-      const int L = (int)a.ev[sg];
-      for (int t = lane; t < L; t += WAVE) {
-        const int q = (int)(a.ev[sg + 1u + (unsigned)t] & ~EV_SINGLE);
-        atomicAdd(&cnt[q], 1u);
-        atomicMin(&first[q], ((unsigned)i << a.pb) | (unsigned)t);
-      }
+    for (int c0 = 0; c0 < deg; c0 += WAVE) {
+      const int nn = min(WAVE, deg - c0);
+      const unsigned sg = c0 == 0 ? sg0 : (lane < nn ? a.vtx_seg[vb + (unsigned)(c0 + lane)] : 0u);
+      wave_sweep_lists<8, 4>(a.ev, sg, nn, [&](int j, int, int t, bool ok, unsigned x) {
+        if (ok) {
+          const int q = (int)(x & ~EV_SINGLE);
+          atomicAdd(&cnt[q], 1u);
+          atomicMin(&first[q], ((unsigned)(c0 + j) << a.pb) | (unsigned)t);
+        }
+      });
     }
     wave_lds_sync();
     // LDG: lexicographic max of (score, -size, -first rank) over score > 0
@@ -191,90 +203,97 @@ __global__ void __launch_bounds__(VOL_BLOCK) k_vol_eval(VolEvalArgs a) {
     int dec = -1;
     if (d != own) {
       // gains / losses (MGV/partition.cpp:800-840; MSRV/partition.cpp:852-897),
-      // accumulated as delta[p] = losses[p] - gains[p]
+      // accumulated as delta[p] = losses[p] - gains[p]; lane j takes net j of
+      // each chunk: whether v is own's only pin there (single), whether d is
+      // listed (found), and the net's source part
       int d_own = 0, d_dec = 0;
-      for (int i = 0; i < deg; i++) {
-        const unsigned sg = a.vtx_seg[vb + (unsigned)i];
-        const int n = (int)a.vtx_nets[vb + (unsigned)i];
-        const int L = (int)a.ev[sg];
-        bool single = false, found = false;
-        for (int t0 = 0; t0 < L; t0 += WAVE) {
-          const int t = t0 + lane;
-          const bool ok = t < L;
-          const unsigned x = ok ? a.ev[sg + 1u + (unsigned)t] : 0u;
-          const int q = (int)(x & ~EV_SINGLE);
-          single = single || __ballot(ok && q == own && (x & EV_SINGLE)) != 0;
-          found = found || __ballot(ok && q == d) != 0;
-        }
-        const int s = a.src[n];
-        if (s == v) {
-          d_own -= L - 1;  // gains[part_id] += end_cnt - 1
-          d_dec += L - 1;  // losses[decision] += end_cnt - 1
-          if (single) d_dec -= 1;
-          if (!found) d_dec += 1;
-        } else {
-          const int sp = a.part[s];
-          if (single) {
-            if (lane == 0) atomicAdd(&delta[sp], -1);
-            if (a.msrv) d_own -= 1;
+      for (int c0 = 0; c0 < deg; c0 += WAVE) {
+        const int nn = min(WAVE, deg - c0);
+        unsigned sg = sg0;
+        int ip = ip0;
+        bool isrc = isrc0;
+        if (c0 > 0) {
+          sg = lane < nn ? a.vtx_seg[vb + (unsigned)(c0 + lane)] : 0u;
+          ip = -1;
+          if (lane < nn) {
+            const int s = a.src[a.vtx_nets[vb + (unsigned)(c0 + lane)]];
+            isrc = s == v;
+            ip = isrc ? own : a.part[s];
           }
-          if (!found) {
-            if (lane == 0) atomicAdd(&delta[sp], 1);
-            if (a.msrv) d_dec += 1;
+        }
+        unsigned long long smask = 0, fmask = 0;
+        int myL = 0;
+        wave_sweep_lists<8, 4>(a.ev, sg, nn, [&](int j, int L, int t, bool ok, unsigned x) {
+          const int q = (int)(x & ~EV_SINGLE);
+          if (__ballot(ok && q == own && (x & EV_SINGLE))) smask |= 1ull << j;
+          if (__ballot(ok && q == d)) fmask |= 1ull << j;
+          if (lane == j) myL = L;
+          (void)t;
+        });
+        if (lane < nn) {
+          const bool single = (smask >> lane) & 1ull, found = (fmask >> lane) & 1ull;
+          if (isrc) {
+            d_own -= myL - 1;  // gains[part_id] += end_cnt - 1
+            d_dec += myL - 1;  // losses[decision] += end_cnt - 1
+            if (single) d_dec -= 1;
+            if (!found) d_dec += 1;
+          } else {
+            const int dl = (found ? 0 : 1) - (single ? 1 : 0);
+            if (dl) atomicAdd(&delta[ip], dl);
+            if (a.msrv) {
+              if (single) d_own -= 1;
+              if (!found) d_dec += 1;
+            }
           }
         }
       }
+      d_own = wave_sum(d_own);
+      d_dec = wave_sum(d_dec);
       if (lane == 0) {
         atomicAdd(&delta[own], d_own);
         atomicAdd(&delta[d], d_dec);
       }
       wave_lds_sync();
-      // the affected parts: item 0 = own, 1 = d, 2 + i = source part of net i;
-      // every other part has gains = losses = 0
-      bool bad = false;
-      for (int b = 0; b < deg + 2; b += WAVE) {
-        const int it = b + lane;
-        int p = -1;
-        if (it == 0) p = own;
-        else if (it == 1) p = d;
-        else if (it < deg + 2) p = a.part[a.src[a.vtx_nets[vb + (unsigned)(it - 2)]]];
-        if (p >= 0) {
-          const int dl = delta[p];
-          const int vv = a.vol[p];
-          if (vv == M) {
-            if (a.msrv ? dl >= 0 : dl > 0) bad = true;  // a max part must gain (MSRV: strictly)
-            atomicMin(&first[p], (unsigned)it);
-          } else if (vv + dl >= M) {
-            bad = true;  // no other part may reach the max
+      // the affected parts: item 0 = own, 1 = d, 2 + i = source part of net i
+      // (own for v's own source nets); every other part has gains = losses = 0.
+      // g(item, part) runs for every item, net items on lane i of their chunk
+      auto for_items = [&](auto g) {
+        if (lane < 2) g(lane, lane == 0 ? own : d);
+        for (int c0 = 0; c0 < deg; c0 += WAVE) {
+          const int i = c0 + lane;
+          if (i < deg) {
+            int p = ip0;
+            if (c0 > 0) {
+              const int s = a.src[a.vtx_nets[vb + (unsigned)i]];
+              p = s == v ? own : a.part[s];
+            }
+            g(2 + i, p);
           }
         }
-      }
+      };
+      bool bad = false;
+      for_items([&](int it, int p) {
+        const int dl = delta[p];
+        const int vv = a.vol[p];
+        if (vv == M) {
+          if (a.msrv ? dl >= 0 : dl > 0) bad = true;  // a max part must gain (MSRV: strictly)
+          atomicMin(&first[p], (unsigned)it);
+        } else if (vv + dl >= M) {
+          bad = true;  // no other part may reach the max
+        }
+      });
       bad = __ballot(bad) != 0;
       wave_lds_sync();
       // MSRV: max parts untouched by the move have gains = losses = 0 and fail
       // the strict test, so every max part must be affected
       int maxhit = 0;
-      for (int b = 0; b < deg + 2; b += WAVE) {
-        const int it = b + lane;
-        int p = -1;
-        if (it == 0) p = own;
-        else if (it == 1) p = d;
-        else if (it < deg + 2) p = a.part[a.src[a.vtx_nets[vb + (unsigned)(it - 2)]]];
-        if (p >= 0) maxhit += first[p] == (unsigned)it ? 1 : 0;
-      }
+      for_items([&](int it, int p) { maxhit += first[p] == (unsigned)it ? 1 : 0; });
       maxhit = wave_sum(maxhit);
       wave_lds_sync();
-      for (int b = 0; b < deg + 2; b += WAVE) {
-        const int it = b + lane;
-        int p = -1;
-        if (it == 0) p = own;
-        else if (it == 1) p = d;
-        else if (it < deg + 2) p = a.part[a.src[a.vtx_nets[vb + (unsigned)(it - 2)]]];
-        if (p >= 0) {
-          delta[p] = 0;
-          first[p] = NOFIRST;
-        }
-      }
+      for_items([&](int, int p) {
+        delta[p] = 0;
+        first[p] = NOFIRST;
+      });
       wave_lds_sync();
       if (!bad && (!a.msrv || maxhit == cntM)) dec = d;
     }
