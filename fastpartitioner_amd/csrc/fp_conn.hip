@@ -241,11 +241,20 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
         }
       } else {
         // MSM: every max part must gain strictly more than it loses, no other
-        // part may reach the max; improvement = sum over the max parts
+        // part may reach the max; improvement = sum over the max parts.
+        // The max parts among the items number X (own and the senders) plus
+        // one if k is a max part outside them, and must be all cntM of them:
+        // with X + 1 < cntM no k passes
         const bool own_s = bit_of(smark, own);
         const int g_own = gain_own + ((own_s && bit_of(sgain, own)) ? 1 : 0);
+        int X = 0;
+        for (int i = lane; i < nsend; i += WAVE) {
+          const int sp = send[i];
+          X += (sp != own && a.msg[sp] == M) ? 1 : 0;
+        }
+        X = wave_sum(X) + (a.msg[own] == M ? 1 : 0);
         int bimp = -1, bk = INT_MAX;
-        for (int k0 = 0; k0 < K; k0 += WAVE) {
+        for (int k0 = 0; X + 1 >= cntM && k0 < K; k0 += WAVE) {
           const int k = k0 + lane, w = k0 / WAVE;
           const bool valid = k < K && k != own;
           const int kc = valid ? k : own;
