@@ -1,10 +1,10 @@
 // fp_batch.hip -- batched refinement: many resident hypergraphs (contexts) of
 // uneven size, K and objective refined together.
 //
-// Members are spread over L lanes (streams) by estimated pass cost (pins,
-// longest first onto the least loaded lane). Each lane's members' passes (the
-// kernel sequences fp_refine enqueues, with no host synchronisation in them)
-// are captured once, back to back, into one HIP graph per lane; a pass of the
+// Members are spread over L lanes (streams) by pass cost (the autotuner's
+// measured pass time, else pins; longest first onto the least loaded lane).
+// Each lane's members' passes (the kernel sequences fp_refine enqueues, with
+// no host synchronisation in them) are captured once, back to back, into one HIP graph per lane; a pass of the
 // batch is one launch of every lane graph, and the lanes run side by side on
 // the device's hardware queues (the HIP runtime's GPU_MAX_HW_QUEUES, 4 by
 // default). Results are those of fp_refine on each member alone: lanes and
@@ -164,7 +164,7 @@ fp_status fp_batch_refine(fp_batch* b, int passes, int tune_passes, double* devi
     if (st) return member_fail(b, i, st);
     fill_async<unsigned long long>(m.ctx, m.ctx->moves, 1, 0ull);
     m.moves = 0;
-    m.tuned_ms = 0;
+    if (tune_passes > 0) m.tuned_ms = 0;  // a call without trials keeps the last measured times
   }
   hipEvent_t t0 = batch_event(b, 0), t1 = batch_event(b, 1);
   if (!t0 || !t1) return bfail(b, FP_ERR_HIP, "hipEventCreate failed");
@@ -239,17 +239,23 @@ fp_status fp_batch_refine(fp_batch* b, int passes, int tune_passes, double* devi
       B_HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
       b->lane_streams.push_back(s);
     }
-    // longest first onto the least loaded lane (pins as the pass cost)
+    // longest first onto the least loaded lane; the pass cost is the time a
+    // member's tuned shape took in the autotuner trials when every member has
+    // one, its pins + vertices otherwise
+    bool timed = true;
+    for (size_t i = 0; i < nm; i++) timed = timed && b->members[i].tuned_ms > 0;
+    std::vector<double> cost(nm);
+    for (size_t i = 0; i < nm; i++)
+      cost[i] = timed ? (double)b->members[i].tuned_ms : (double)(b->members[i].ctx->pins + b->members[i].ctx->nov);
     std::vector<size_t> order(nm);
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](size_t x, size_t y) { return b->members[x].ctx->pins > b->members[y].ctx->pins; });
-    std::vector<long long> load((size_t)L, 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+    std::vector<double> load((size_t)L, 0.0);
     b->lane_of.assign(nm, 0);
     for (size_t i : order) {
       const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
       b->lane_of[i] = l;
-      load[(size_t)l] += b->members[i].ctx->pins + b->members[i].ctx->nov;
+      load[(size_t)l] += cost[i];
     }
     for (size_t i = 0; i < nm; i++) B_HIP_OK(hipStreamSynchronize(b->members[i].ctx->stream));
     // capture each lane: its members' passes, recorded on the lane stream

@@ -373,8 +373,7 @@ fp_status pass_cn(fp_ctx* ctx, const RefinePlan&) {
   const size_t lds_apply = (size_t)ctx->K * sizeof(unsigned) + (size_t)CN_MAXOPS * sizeof(unsigned);
   const unsigned long long* nm = s.totals + 3;
   // one round per pass: tag 1 after clearing the stamps
-  fill_async<unsigned>(ctx, ctx->net_stamp, (size_t)std::max(noe, 1), 0u);
-  fill_async<int>(ctx, ctx->counters, 4, 0);
+  pass_begin(ctx, nullptr, 0, nullptr);
   fill_async<unsigned long long>(ctx, s.totals, 3, 0ull);
   CnEvalArgs ea{ctx->net_ptr, ctx->net_pins, ctx->vtx_ptr, ctx->vtx_seg, ctx->ent, ctx->ev, ctx->part,
                 noe, ctx->pb, ctx->pmask, s.old_part, s.new_part, s.npush};

@@ -736,9 +736,7 @@ fp_status pass_conn(fp_ctx* ctx, const RefinePlan& p) {
   const size_t lds_r = (size_t)(REPLAY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
   auto& c = ctx->conn;
   const double thr = p.sched.ldg_C * (1.02);
-  fill_async<unsigned>(ctx, ctx->net_stamp, (size_t)std::max(ctx->noe, 1), 0u);
-  fill_async<int>(ctx, ctx->counters, 4, 0);
-  if (msm) k_round_maxset<<<1, 1024, 0, ctx->stream>>>(c.msg, K, c.mstat);
+  pass_begin(ctx, msm ? c.msg : nullptr, K, c.mstat);
   unsigned tag = 0;
   for (int r0 = 0; r0 < nov; r0 += R) {
     const int nslots = std::min(R, nov - r0);
@@ -800,23 +798,23 @@ fp_status pass_conn(fp_ctx* ctx, const RefinePlan& p) {
     ap.pmask = ctx->pmask;
     ap.r0 = r0;
     ap.r1 = r0 + nslots;
-    ap.net_blocks = REPLAY_NET_BLOCKS;
+    ap.net_blocks = net_grid(ctx, REPLAY_NET_BLOCKS, REPLAY_BLOCK / WAVE);
     if (!msm) {
       k_round_compact<<<1, 1024, 0, ctx->stream>>>(ctx->slot_dec, ctx->slot_own, nullptr, ctx->vtx_ptr, r0, nslots,
                                                     c.slot_mi, c.mv, c.mstat);
       k_tm_admit<<<1, WAVE, (size_t)K * (sizeof(long long) + 2 * sizeof(int)), ctx->stream>>>(
           ctx->slot_dec, c.mv, c.mstat, ctx->vwgt, r0, K, thr, ctx->sizes, ctx->part, ctx->moves);
       ap.mode = 0;
-      k_conn_apply<<<REPLAY_NET_BLOCKS, REPLAY_BLOCK, lds_r, ctx->stream>>>(ap);
+      k_conn_apply<<<ap.net_blocks, REPLAY_BLOCK, lds_r, ctx->stream>>>(ap);
     } else {
       k_round_compact<<<1, 1024, 0, ctx->stream>>>(ctx->slot_dec, ctx->slot_own, nullptr, ctx->vtx_ptr, r0, nslots,
                                                     c.slot_mi, c.mv, c.mstat);
       ap.mode = 1;
-      k_conn_apply<<<REPLAY_NET_BLOCKS, REPLAY_BLOCK, lds_r, ctx->stream>>>(ap);
+      k_conn_apply<<<ap.net_blocks, REPLAY_BLOCK, lds_r, ctx->stream>>>(ap);
       k_msm_scan<<<1, WAVE, ((size_t)K + WAVE) * sizeof(int), ctx->stream>>>(c.pc, c.msg, K, c.mv, c.evbuf,
                                                                                  c.incoff, ctx->vtx_ptr, c.mstat);
       ap.mode = 2;
-      k_conn_apply<<<REPLAY_NET_BLOCKS + (nslots + REPLAY_BLOCK - 1) / REPLAY_BLOCK, REPLAY_BLOCK, lds_r,
+      k_conn_apply<<<ap.net_blocks + (nslots + REPLAY_BLOCK - 1) / REPLAY_BLOCK, REPLAY_BLOCK, lds_r,
                      ctx->stream>>>(ap);
     }
     launch_bits(ctx);

@@ -4,6 +4,53 @@
 
 namespace fpe {
 
+__global__ void __launch_bounds__(1024) k_pass_begin(unsigned* __restrict__ stamp, size_t n, int* __restrict__ counters,
+                                                     const int* __restrict__ val, int K, int* __restrict__ mstat) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    stamp[i] = 0u;
+  if (blockIdx.x != 0) return;
+  if (threadIdx.x < 4) counters[threadIdx.x] = 0;
+  if (!val) return;
+  __shared__ int s_m[16], s_c[16];
+  int m = INT_MIN, c = 0;
+  for (int q = (int)threadIdx.x; q < K; q += (int)blockDim.x) {
+    const int x = val[q];
+    if (x > m) {
+      m = x;
+      c = 1;
+    } else if (x == m) {
+      c++;
+    }
+  }
+  wave_max_count(m, c);
+  const int w = (int)(threadIdx.x / WAVE);
+  if (lane_id() == 0) {
+    s_m[w] = m;
+    s_c[w] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int bm = INT_MIN, bc = 0;
+    for (int i = 0; i < (int)(blockDim.x / WAVE); i++) {
+      if (s_m[i] > bm) {
+        bm = s_m[i];
+        bc = s_c[i];
+      } else if (s_m[i] == bm) {
+        bc += s_c[i];
+      }
+    }
+    mstat[0] = bm;
+    mstat[1] = bc;
+  }
+}
+
+void pass_begin(fp_ctx* ctx, const int* val, int K, int* mstat) {
+  const size_t n = (size_t)std::max(ctx->noe, 1);
+  const int blocks = (int)std::min<size_t>((n + 1023) / 1024, 1024);
+  k_pass_begin<<<blocks, 1024, 0, ctx->stream>>>(ctx->net_stamp, n, ctx->counters, val, K, mstat);
+}
+
+
 fp_status fail(fp_ctx* c, fp_status st, const char* fmt, ...) {
   if (c) {
     va_list ap;

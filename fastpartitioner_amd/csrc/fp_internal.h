@@ -342,6 +342,26 @@ __device__ __forceinline__ void atomic_add_i64(long long* p, long long v) {
   atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
 }
 
+__device__ __forceinline__ void wave_max_count(int& m, int& c) {
+  for (int off = WAVE / 2; off > 0; off >>= 1) {
+    const int om = __shfl_xor(m, off), oc = __shfl_xor(c, off);
+    if (om > m) {
+      m = om;
+      c = oc;
+    } else if (om == m) {
+      c += oc;
+    }
+  }
+}
+
+// The start of every refinement pass in one launch: clears the net stamps
+// (round tags restart at 1 each pass) and the touched counters; with val
+// given, block 0 also writes max(val[0..K)) and the number of parts holding
+// it to mstat[0..1] (the max set of MGV/partition.cpp:712-729 and
+// MSM/partition.cpp:1030-1045). 1024 threads per block.
+__global__ void __launch_bounds__(1024) k_pass_begin(unsigned* __restrict__ stamp, size_t n, int* __restrict__ counters,
+                                                     const int* __restrict__ val, int K, int* __restrict__ mstat);
+
 template <typename T>
 __global__ void k_fill(T* p, size_t n, T v) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
@@ -522,6 +542,15 @@ void fill_async(fp_ctx* ctx, T* p, size_t n, T v) {
   if (n == 0) return;
   const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
   k_fill<T><<<blocks, 256, 0, ctx->stream>>>(p, n, v);
+}
+
+// k_pass_begin on the context's stream (val may be null)
+void pass_begin(fp_ctx* ctx, const int* val, int K, int* mstat);
+
+// blocks of a per-touched-net kernel with `waves` wavefronts per block: a
+// round touches at most noe nets
+inline int net_grid(const fp_ctx* ctx, int cap, int waves) {
+  return std::max(1, std::min(cap, (ctx->noe + waves - 1) / waves));
 }
 
 }  // namespace fpe

@@ -10,53 +10,6 @@
 namespace fpe {
 namespace {
 
-__device__ __forceinline__ void wave_max_count(int& m, int& c) {
-  for (int off = WAVE / 2; off > 0; off >>= 1) {
-    const int om = __shfl_xor(m, off), oc = __shfl_xor(c, off);
-    if (om > m) {
-      m = om;
-      c = oc;
-    } else if (om == m) {
-      c += oc;
-    }
-  }
-}
-
-// mstat[0] = max of val, mstat[1] = number of parts holding it (MGV/partition.cpp:712-729)
-__global__ void __launch_bounds__(1024) k_round_maxset(const int* __restrict__ val, int K, int* __restrict__ mstat) {
-  __shared__ int s_m[16], s_c[16];
-  int m = INT_MIN, c = 0;
-  for (int q = (int)threadIdx.x; q < K; q += (int)blockDim.x) {
-    const int x = val[q];
-    if (x > m) {
-      m = x;
-      c = 1;
-    } else if (x == m) {
-      c++;
-    }
-  }
-  wave_max_count(m, c);
-  const int lane = lane_id(), w = (int)(threadIdx.x / WAVE);
-  if (lane == 0) {
-    s_m[w] = m;
-    s_c[w] = c;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int bm = INT_MIN, bc = 0;
-    for (int i = 0; i < (int)(blockDim.x / WAVE); i++) {
-      if (s_m[i] > bm) {
-        bm = s_m[i];
-        bc = s_c[i];
-      } else if (s_m[i] == bm) {
-        bc += s_c[i];
-      }
-    }
-    mstat[0] = bm;
-    mstat[1] = bc;
-  }
-}
-
 // Accepted moves of the round in visiting order: slot_mi[slot] = move index
 // (or -1), mv[i] = (vertex, own part, new part, first event slot).
 constexpr int COMPACT_RUN = 16;  // consecutive slots per thread per step of k_round_compact

@@ -537,8 +537,7 @@ fp_status pass_tv(fp_ctx* ctx, const RefinePlan& p) {
   const size_t lds_a = (size_t)(APPLY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
   const size_t lds_s = (size_t)K * sizeof(int);
   // every pass restarts its round tags: clear the stamps and the touched counters
-  fill_async<unsigned>(ctx, ctx->net_stamp, (size_t)std::max(ctx->noe, 1), 0u);
-  fill_async<int>(ctx, ctx->counters, 4, 0);
+  pass_begin(ctx, nullptr, 0, nullptr);
   unsigned tag = 0;
   for (int r0 = 0; r0 < nov; r0 += R) {
     const int nslots = std::min(R, nov - r0);
@@ -593,8 +592,8 @@ fp_status pass_tv(fp_ctx* ctx, const RefinePlan& p) {
     ap.pmask = ctx->pmask;
     ap.r0 = r0;
     ap.r1 = r0 + nslots;
-    ap.net_blocks = APPLY_NET_BLOCKS;
-    int grid_a = APPLY_NET_BLOCKS;
+    ap.net_blocks = net_grid(ctx, APPLY_NET_BLOCKS, APPLY_BLOCK / WAVE);
+    int grid_a = ap.net_blocks;
     if (!bal) {
       ap.order = ctx->order;
       ap.part = ctx->part;

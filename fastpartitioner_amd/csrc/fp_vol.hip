@@ -758,9 +758,7 @@ fp_status pass_vol(fp_ctx* ctx, const RefinePlan& p) {
   const int stage_cap = (int)std::min<size_t>(4096, ((size_t)LDS_BUDGET - lds_vol) / sizeof(int4));
   const size_t lds_s = lds_vol + (size_t)stage_cap * sizeof(int4);
   auto& vs = ctx->vol;
-  fill_async<unsigned>(ctx, ctx->net_stamp, (size_t)std::max(ctx->noe, 1), 0u);
-  fill_async<int>(ctx, ctx->counters, 4, 0);
-  k_round_maxset<<<1, 1024, 0, ctx->stream>>>(vs.vol, K, vs.mstat);
+  pass_begin(ctx, vs.vol, K, vs.mstat);
   unsigned tag = 0;
   for (int r0 = 0; r0 < nov; r0 += R) {
     const int nslots = std::min(R, nov - r0);
@@ -819,13 +817,13 @@ fp_status pass_vol(fp_ctx* ctx, const RefinePlan& p) {
     ra.r0 = r0;
     ra.r1 = r0 + nslots;
     ra.msrv = msrv ? 1 : 0;
-    ra.net_blocks = REPLAY_NET_BLOCKS;
+    ra.net_blocks = net_grid(ctx, REPLAY_NET_BLOCKS, REPLAY_BLOCK / WAVE);
     ra.commit = 0;
-    k_vol_replay<<<REPLAY_NET_BLOCKS, REPLAY_BLOCK, lds_r, ctx->stream>>>(ra);
+    k_vol_replay<<<ra.net_blocks, REPLAY_BLOCK, lds_r, ctx->stream>>>(ra);
     k_vol_scan<<<1, WAVE, lds_s, ctx->stream>>>(vs.vol, K, vs.mv, vs.evbuf, ctx->vtx_ptr, vs.mstat, stage_cap,
                                                  batched);
     ra.commit = 1;
-    k_vol_replay<<<REPLAY_NET_BLOCKS + (nslots + REPLAY_BLOCK - 1) / REPLAY_BLOCK, REPLAY_BLOCK, lds_r,
+    k_vol_replay<<<ra.net_blocks + (nslots + REPLAY_BLOCK - 1) / REPLAY_BLOCK, REPLAY_BLOCK, lds_r,
                    ctx->stream>>>(ra);
   }
   return FP_OK;
