@@ -213,28 +213,48 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_tv_eval(EvalArgs a) {
     unsigned long long b1 = 0, b2 = ~0ull;
     int bq = -1;
     unsigned bc = 0;
-    for (int q = lane; q < K; q += WAVE) {
-      const unsigned c = cnt[q];
-      if (c) {
-        const unsigned f = first[q];
-        cnt[q] = 0;
-        first[q] = NOFIRST;
-        const int adj = (int)c - (q == own ? 1 : 0);   // scores[part_id]-- (TV/partition.cpp:345)
-        const double sc = (double)adj * s_pen[q];
-        if (sc > 0) {
-          const unsigned long long k1 = (unsigned long long)__double_as_longlong(sc);
-          const unsigned long long k2 = ((unsigned long long)s_size[q] << 32) | f;
-          if (k1 > b1 || (k1 == b1 && k2 < b2)) { b1 = k1; b2 = k2; bq = q; bc = c; }
+    // four 64-part chunks per step, without branches: their LDS reads issue
+    // together; every slot is cleared for the next vertex
+    for (int q0 = 0; q0 < K; q0 += 4 * WAVE) {
+      unsigned cv[4], fv[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int q = min(q0 + u * WAVE + lane, K - 1);
+        cv[u] = q0 + u * WAVE + lane < K ? cnt[q] : 0u;
+        fv[u] = first[q];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int q = q0 + u * WAVE + lane;
+        const int qc = min(q, K - 1);
+        if (q < K) {
+          cnt[q] = 0;
+          first[q] = NOFIRST;
         }
+        const int adj = (int)cv[u] - (q == own ? 1 : 0);   // scores[part_id]-- (TV/partition.cpp:345)
+        const double sc = (double)adj * s_pen[qc];
+        const bool ok = cv[u] != 0 && sc > 0;
+        const unsigned long long k1 = ok ? (unsigned long long)__double_as_longlong(sc) : 0ull;
+        const unsigned long long k2 = ((unsigned long long)s_size[qc] << 32) | fv[u];
+        const bool better = ok && (k1 > b1 || (k1 == b1 && k2 < b2));
+        b1 = better ? k1 : b1;
+        b2 = better ? k2 : b2;
+        bq = better ? q : bq;
+        bc = better ? cv[u] : bc;
       }
     }
     const unsigned long long m1 = wave_max_u64(b1);
-    const unsigned long long m2 = wave_min_u64(b1 == m1 ? b2 : ~0ull);
     wave_lds_sync();
     int d = fb;
     unsigned craw = cfb;
     if (m1 != 0) {
-      const int wl = __ffsll((long long)__ballot(b1 == m1 && b2 == m2)) - 1;
+      // the second key is needed only when several lanes hold the best score
+      unsigned long long cand = __ballot(b1 == m1);
+      if (__popcll(cand) > 1) {
+        const unsigned long long m2 = wave_min_u64(b1 == m1 ? b2 : ~0ull);
+        cand = __ballot(b1 == m1 && b2 == m2);
+      }
+      const int wl = __ffsll((long long)cand) - 1;
       d = __builtin_amdgcn_readlane(bq, wl);
       craw = (unsigned)__builtin_amdgcn_readlane((int)bc, wl);
     }
