@@ -78,8 +78,11 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_tv_eval(EvalArgs a) {
   __shared__ long long red_sz[EVAL_BLOCK / WAVE];
   __shared__ int red_ix[EVAL_BLOCK / WAVE];
   const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
-  unsigned* cnt = wbase + (size_t)wave * 2 * (size_t)K;
-  unsigned* first = cnt + K;
+  // per wavefront: cnt[K] and first[K], each followed by 64 spare slots that
+  // take the atomics of lanes past a list's end (no branch around them)
+  unsigned* cnt = wbase + (size_t)wave * 2 * (size_t)(K + WAVE);
+  unsigned* first = cnt + K + WAVE;
+  const int spare = K + lane_id();
 
   // penalties of the frozen part sizes: penalty = 1 - (1.0 * size / C)  (TV/partition.cpp:103)
   long long bs = LLONG_MAX;
@@ -167,10 +170,9 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_tv_eval(EvalArgs a) {
           const bool ok = lane >= 1 && t < Lj;
           const unsigned x = xv[u];
           const int q = (int)(x & ~EV_SINGLE);
-          if (ok) {
-            atomicAdd(&cnt[q], 1u);
-            atomicMin(&first[q], ((unsigned)(c0 + j) << a.pb) | (unsigned)t);
-          }
+          const int qs = ok ? q : spare;
+          atomicAdd(&cnt[qs], 1u);
+          atomicMin(&first[qs], ((unsigned)(c0 + j) << a.pb) | (unsigned)t);
           // leave gain: nets where v is the only pin of its part (TV/partition.cpp:365-368)
           lg += __popcll(__ballot(ok && q == own && (x & EV_SINGLE)));
         }
@@ -194,10 +196,9 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_tv_eval(EvalArgs a) {
             const bool ok = t < Lj;
             const unsigned x = xv[u];
             const int q = (int)(x & ~EV_SINGLE);
-            if (ok) {
-              atomicAdd(&cnt[q], 1u);
-              atomicMin(&first[q], ((unsigned)(c0 + j) << a.pb) | (unsigned)t);
-            }
+            const int qs = ok ? q : spare;
+            atomicAdd(&cnt[qs], 1u);
+            atomicMin(&first[qs], ((unsigned)(c0 + j) << a.pb) | (unsigned)t);
             lg += __popcll(__ballot(ok && q == own && (x & EV_SINGLE)));
           }
         }
@@ -464,14 +465,15 @@ __global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
   }
 }
 
+size_t eval_lds(int K, int waves) { return (size_t)K * 16 + (size_t)waves * 8 * (size_t)(K + WAVE); }
+
 int eval_waves(int K) {
-  // LDS: 16 bytes per part shared by the block + 8 bytes per part per wavefront
+  // LDS: 16 bytes per part shared by the block + 8 bytes per part (and per
+  // spare slot) per wavefront
   for (int w = EVAL_BLOCK / WAVE; w >= 1; w--)
-    if ((size_t)K * 16 + (size_t)w * 8 * (size_t)K <= (size_t)LDS_BUDGET) return w;
+    if (eval_lds(K, w) <= (size_t)LDS_BUDGET) return w;
   return 0;
 }
-
-size_t eval_lds(int K, int waves) { return (size_t)K * 16 + (size_t)waves * 8 * (size_t)K; }
 
 int eval_grid(int nslots, int waves) { return std::max(1, std::min((nslots + waves - 1) / waves, 8192)); }
 
