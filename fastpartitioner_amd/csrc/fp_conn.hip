@@ -304,12 +304,7 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
       a.slot_own[slot] = own;
       a.slot_dec[slot] = dec;
     }
-    if (dec >= 0) {
-      for (int i = lane; i < deg; i += WAVE) {
-        const unsigned n = a.vtx_nets[vb + (unsigned)i];
-        if (atomicExch(&a.net_stamp[n], a.tag) != a.tag) a.touched[atomicAdd(a.tcount, 1)] = (int)n;
-      }
-    }
+    if (dec >= 0) wave_touch_nets(a.vtx_nets + vb, deg, a.tag, a.net_stamp, a.touched, a.tcount);
     // clear this vertex's marks
     wave_lds_sync();
     for (int i = lane; i < nrecv; i += WAVE) rmark[recv[i] >> 5] = 0;

@@ -309,6 +309,32 @@ __device__ __forceinline__ void wave_sweep_lists(const unsigned short* __restric
   }
 }
 
+// Adds a vertex's nets vn[0..deg) to the round's touched-net list, each net
+// once per round (net_stamp holds the round tag of its last addition), with
+// one counter atomic per 64 nets instead of one per net. Called by the whole
+// wavefront.
+__device__ __forceinline__ void wave_touch_nets(const unsigned* __restrict__ vn, int deg, unsigned tag,
+                                                unsigned* __restrict__ net_stamp, int* __restrict__ touched,
+                                                int* __restrict__ tcount) {
+  const int lane = lane_id();
+  for (int i0 = 0; i0 < deg; i0 += WAVE) {
+    const int i = i0 + lane;
+    unsigned n = 0;
+    bool isnew = false;
+    if (i < deg) {
+      n = vn[i];
+      isnew = atomicExch(&net_stamp[n], tag) != tag;
+    }
+    const unsigned long long b = __ballot(isnew);
+    if (b) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(tcount, (int)__popcll(b));
+      base = __builtin_amdgcn_readfirstlane(base);
+      if (isnew) touched[base + (int)__popcll(b & ((1ull << lane) - 1ull))] = (int)n;
+    }
+  }
+}
+
 // Range [lo, hi) of apply_pins[s..e) (a net's pin ranks, ascending) that
 // may hold ranks in [r0, r1): the exact range for nets longer than a wave,
 // the whole net otherwise (the caller tests each rank against [r0, r1)).

@@ -274,12 +274,7 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_tv_eval(EvalArgs a) {
         atomicAdd(&a.hist[(size_t)d * (size_t)a.ntiles + (size_t)(slot / TILE)], w);
       }
     }
-    if (accept && a.touched) {
-      for (int i = lane; i < deg; i += WAVE) {
-        const unsigned n = a.vtx_nets[vb + (unsigned)i];
-        if (atomicExch(&a.net_stamp[n], a.tag) != a.tag) a.touched[atomicAdd(a.tcount, 1)] = (int)n;
-      }
-    }
+    if (accept && a.touched) wave_touch_nets(a.vtx_nets + vb, deg, a.tag, a.net_stamp, a.touched, a.tcount);
     slot += stride;
     hv[0] = hv[1]; hown[0] = hown[1]; hvb[0] = hvb[1]; hdeg[0] = hdeg[1];
     hv[1] = hv[2]; hown[1] = hown[2]; hvb[1] = hvb[2]; hdeg[1] = hdeg[2];

@@ -301,12 +301,7 @@ __global__ void __launch_bounds__(VOL_BLOCK) k_vol_eval(VolEvalArgs a) {
       a.slot_own[slot] = own;
       a.slot_dec[slot] = dec;
     }
-    if (dec >= 0) {
-      for (int i = lane; i < deg; i += WAVE) {
-        const unsigned n = a.vtx_nets[vb + (unsigned)i];
-        if (atomicExch(&a.net_stamp[n], a.tag) != a.tag) a.touched[atomicAdd(a.tcount, 1)] = (int)n;
-      }
-    }
+    if (dec >= 0) wave_touch_nets(a.vtx_nets + vb, deg, a.tag, a.net_stamp, a.touched, a.tcount);
   }
 }
 
