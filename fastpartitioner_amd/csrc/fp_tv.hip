@@ -27,7 +27,7 @@ namespace {
 constexpr int EVAL_BLOCK = 256;  // 4 wavefronts
 constexpr int TILE = 64;         // slots per admission tile
 #ifndef FP_NETS_IN_FLIGHT
-#define FP_NETS_IN_FLIGHT 8
+#define FP_NETS_IN_FLIGHT 4
 #endif
 constexpr int NETS_IN_FLIGHT = FP_NETS_IN_FLIGHT;  // segment head windows loaded together per wave
 constexpr int LONG_WINDOWS = 4;                    // windows of one long list loaded together
