@@ -364,9 +364,9 @@ __global__ void __launch_bounds__(256) k_bal_admit(int* __restrict__ slot_dec, c
     if (__ballot(o >= 0)) {
       // weight of earlier candidates into the same part within the tile
       long long pre = 0;
+      const int wi = (int)w;  // vertex weights are int
       for (int j = 0; j < WAVE; j++) {
-        const int dj = __shfl(d, j);
-        const long long wj = __shfl(w, j);
+        const int dj = __builtin_amdgcn_readlane(d, j), wj = __builtin_amdgcn_readlane(wi, j);
         if (j < lane && dj == d) pre += wj;
       }
       if (o >= 0) adm = sizes_frozen[d] + pref[(size_t)o * (size_t)ntiles + (size_t)tile] + pre + w <= cap;
