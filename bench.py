@@ -423,6 +423,17 @@ def main():
         m1 = eng.metrics(vectors=True)
         lam1 = lambda_visits(hg, m1["lam"])
         sizes = eng.get_state().sizes
+        # the same instance's CN refinement (the lambda = 2 push) from the same
+        # initial partition, as many passes, timed the same way
+        eng.set_partition(K, part0)
+        cn0 = eng.metrics()["CN"]
+        if args.warmup:
+            eng.refine("CN", args.warmup)
+            eng.set_partition(K, part0)
+        tc0 = time.perf_counter()
+        cst = eng.refine("CN", args.steps)
+        tc1 = time.perf_counter()
+        cn1 = eng.metrics()["CN"]
     secs = t1 - t0
     value = args.steps * hg.pins / secs
 
@@ -471,6 +482,9 @@ def main():
                    "device_ms": round(st.device_ms, 3)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "cn_same_instance": {"passes": args.steps, "ms_per_step": round((tc1 - tc0) * 1e3 / args.steps, 4),
+                             "pin_visits_per_s": round(args.steps * hg.pins / (tc1 - tc0), 1),
+                             "cn_initial": int(cn0), "cn_refined": int(cn1), "moves": int(cst.moves)},
     }
     if cpu and "value" in cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 2)
