@@ -1,0 +1,126 @@
+"""Ragged inputs and the edges of the supported envelope, all six objectives
+against the oracle (bit-exact states and metrics): isolated vertices,
+single-pin nets, one net holding almost every vertex, K = 1 and 2; and K
+beyond what an objective's kernels hold fails with FP_ERR_UNSUPPORTED and
+leaves the context usable."""
+import numpy as np
+import pytest
+
+from fastpartitioner_amd import hypergraph as H
+from fastpartitioner_amd._native import FastPartError
+from fastpartitioner_amd.engine import SCHED_REF, Engine, schedule
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FP_ERR_UNSUPPORTED = 5
+
+
+def _ragged(seed=5, nov=3000, isolated=100):
+    """One net over every vertex but the last `isolated` ones, a single-pin
+    net on every 7th vertex, and random nets of 2..20 pins; every net's
+    source is one of its pins (its first); the last vertices are in no net."""
+    rng = np.random.default_rng(seed)
+    live = nov - isolated
+    nets = [np.arange(live)]
+    nets += [np.array([v]) for v in range(0, live, 7)]
+    for _ in range(2 * live):
+        nets.append(rng.choice(live, size=int(rng.integers(2, 21)), replace=False))
+    net_ptr = np.zeros(len(nets) + 1, dtype=np.uint32)
+    net_ptr[1:] = np.cumsum([len(n) for n in nets])
+    net_pins = np.concatenate(nets).astype(np.uint32)
+    src = np.array([n[0] for n in nets], dtype=np.int32)
+    return H.Hypergraph.from_nets(nov, net_ptr, net_pins, src)
+
+
+def _same_state(got, st):
+    assert np.array_equal(got.part, st.part)
+    assert np.array_equal(got.ep_ends, st.ep_ends)
+    assert np.array_equal(got.ep_parts, st.ep_parts)
+    assert np.array_equal(got.ep_cnt, st.ep_cnt)
+    assert np.array_equal(got.sizes, st.sizes.astype(np.int64))
+
+
+def _initial(metric, hg, K, C, max_pw, min_pw):
+    if metric == "TV":
+        return O.tv_ldg(hg, K, C, max_pw, min_pw, par_skip=True)[0]
+    if metric == "CN":
+        return O.cn_ldg(hg, K, C)
+    if metric == "TM":
+        return O.tm_init(hg, K, C)
+    if metric == "MSM":
+        return O.msm_init(hg, K, C)
+    if metric == "MSV":
+        return O.msv_ldg(hg, K, C)
+    return O.msrv_ldg(hg, K, C)
+
+
+def _oracle_refine(metric, hg, st, C, max_pw, min_pw, passes, R):
+    if metric == "TV":
+        O.tv_refine_par(hg, st, C, max_pw, min_pw, passes, R)
+    elif metric == "CN":
+        O.cn_refine(hg, st, passes, serial=False)
+    elif metric == "TM":
+        O.tm_refine_par(hg, st, C, passes, R)
+    elif metric == "MSM":
+        O.msm_refine_par(hg, st, passes, R)
+    elif metric == "MSV":
+        O.msv_refine_par(hg, st, C, passes, R)
+    else:
+        O.msrv_refine_par(hg, st, C, passes, R)
+
+
+@pytest.mark.parametrize("K", [1, 2, 16])
+@pytest.mark.parametrize("metric", ["TV", "CN", "TM", "MSM", "MSV", "MSRV"])
+def test_ragged_matches_oracle(metric, K):
+    hg = _ragged()
+    C, max_pw, min_pw = H.ref_constants(hg.nov, K)
+    R = hg.nov if metric == "TM" else 256
+    passes = 3
+    print(f"{metric}: ragged nov={hg.nov} noe={hg.noe} pins={hg.pins} K={K} R={R}", flush=True)
+    st = _initial(metric, hg, K, C, max_pw, min_pw)
+    with Engine(0) as eng:
+        eng.upload(hg)
+        eng.set_state(K, st.part, st.ep_ends, st.ep_parts, st.ep_cnt, st.sizes)
+        sched = None if metric == "CN" else schedule(SCHED_REF, R, C, max_pw, min_pw)
+        stats = eng.refine(metric, passes, sched)
+        got = eng.get_state()
+        m = eng.metrics(vectors=True)
+    _oracle_refine(metric, hg, st, C, max_pw, min_pw, passes, R)
+    _same_state(got, st)
+    ref = O.metrics(hg, K, st.part)
+    for k in O.METRIC_NAMES:
+        assert m[k] == ref[k], k
+    for k in ("lam", "sv", "rv", "sm", "rm", "sv_msv"):
+        assert np.array_equal(m[k], ref[k]), k
+    assert stats.moves >= 0
+
+
+@pytest.mark.parametrize("metric", ["TV", "TM", "MSM", "MSV", "MSRV"])
+def test_k_beyond_the_kernels_fails_cleanly(metric):
+    hg = H.colnet(20000, H.DEG_UNIFORM, 2, 9, seed=7)
+    K = 8192
+    C, max_pw, min_pw = H.ref_constants(hg.nov, K)
+    assert C > 0
+    part = np.arange(hg.nov, dtype=np.int32) % K
+    with Engine(0) as eng:
+        eng.upload(hg)
+        eng.set_partition(K, part)
+        with pytest.raises(FastPartError) as e:
+            eng.refine(metric, 1, schedule(SCHED_REF, 512, C, max_pw, min_pw))
+        assert e.value.code == FP_ERR_UNSUPPORTED
+        # the metrics still evaluate at this K, and the context takes a new state
+        m = eng.metrics()
+        assert m == {k: v for k, v in O.metrics(hg, K, part).items() if k in O.METRIC_NAMES}
+        K2 = 16
+        C2, max2, min2 = H.ref_constants(hg.nov, K2)
+        eng.set_partition(K2, part % K2)
+        eng.refine("TV", 1, schedule(SCHED_REF, 512, C2, max2, min2))
+
+
+def test_k_beyond_the_packing_is_rejected():
+    hg = H.colnet(2000, H.DEG_UNIFORM, 2, 9, seed=7)
+    with Engine(0) as eng:
+        eng.upload(hg)
+        with pytest.raises(FastPartError):
+            eng.set_partition(16385, np.arange(hg.nov, dtype=np.int32))
