@@ -270,6 +270,7 @@ def run_serve(args):
         return batch_schedule(metric, hg.nov, K, C, max_pw, min_pw, H.balance_cap(hg.nov, K, eps))
 
     with Server(0, hbm_budget=args.budget_gib << 30, workers=args.workers) as srv:
+        srv.set_replicas(args.replicas)
         for hg in graphs:
             srv.add_graph(hg)
         rids = [None] * len(arrivals)
@@ -324,7 +325,8 @@ def run_serve(args):
         "data": "synthetic",
         "config": {"workload": f"serve-{args.graphs}-colnet-zipf2.3-logU5e3to1e6-poisson{args.rate}",
                    "requests": len(infos), "rate_per_s": args.rate, "duration_s": args.duration,
-                   "passes": args.passes, "K": list(Ks), "workers": args.workers, "budget_gib": args.budget_gib,
+                   "passes": args.passes, "K": list(Ks), "workers": args.workers, "replicas": args.replicas,
+                   "budget_gib": args.budget_gib,
                    "hw_queues": args.hw_queues, "resident_pins": int(sum(pins))},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2), "p99": round(float(np.percentile(lat, 99)), 2),
                        "mean": round(float(lat.mean()), 2), "max": round(float(lat.max()), 2)},
@@ -360,6 +362,7 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second")
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
     ap.add_argument("--workers", type=int, default=16, help="serve: worker threads")
+    ap.add_argument("--replicas", type=int, default=1, help="serve: contexts per hypergraph at most")
     ap.add_argument("--budget-gib", type=int, default=64, help="serve: HBM budget of the requests in flight")
     ap.add_argument("--check", type=int, default=16, help="serve: requests rerun alone for the mix check")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",

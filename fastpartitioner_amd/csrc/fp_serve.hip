@@ -1,13 +1,15 @@
 // fp_serve.hip -- the serving loop: resident hypergraphs, a request queue and
 // worker threads that refine requests concurrently on one GPU.
 //
-// Every registered hypergraph is uploaded once into a context of its own
-// (both CSR views plus the connectivity state arrays stay in HBM). A request
-// names a hypergraph, K, an objective, a schedule, an initial part vector and
-// a pass count. Workers take requests in arrival order, skipping a request
-// whose hypergraph is busy with another request (that one waits for it), and
-// admit a request only while the HBM footprint of the requests in flight plus
-// its own stays within the server's budget. A running request sets its state
+// Every registered hypergraph is uploaded into a context of its own (both CSR
+// views plus the connectivity state arrays stay in HBM); a hypergraph whose
+// contexts are all busy gets another one (a replica, uploaded from the host
+// copy the server keeps) while it has fewer than the server's replica limit.
+// A request names a hypergraph, K, an objective, a schedule, an initial part
+// vector and a pass count. Workers take requests in arrival order, skipping a
+// request whose hypergraph has no free context and may not get another (that
+// one waits for it), and admit a request only while the HBM footprint of the
+// requests in flight plus its own stays within the server's budget. A running request sets its state
 // from the part vector, refines, and evaluates the six metrics on its
 // hypergraph's stream; requests on different hypergraphs overlap on the device,
 // and new requests are admitted as slots and memory free up (continuous
@@ -30,11 +32,14 @@ struct fp_server {
   std::chrono::steady_clock::time_point t0;
 
   struct Graph {
-    fp_ctx* ctx = nullptr;
-    bool busy = false;
-    int nov = 0;
+    std::vector<fp_ctx*> ctxs;  // the hypergraph's contexts (replicas), all with the same upload
+    std::vector<char> busy;     // per context
+    int creating = 0;           // replicas being uploaded by workers
+    int nov = 0, noe = 0;
     long long pins = 0;
-    long long bytes = 0;  // device bytes of the resident hypergraph and state arrays (after upload)
+    long long bytes = 0;  // device bytes of one resident copy with its state arrays (after upload)
+    std::vector<unsigned> net_ptr, net_pins;  // host copy for replicas
+    std::vector<int> src, vwgt;
   };
   struct Request {
     long long id = 0;
@@ -55,9 +60,10 @@ struct fp_server {
     double submit_ms = 0, start_ms = 0, end_ms = 0;
   };
 
+  int max_replicas = 1;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
-  std::vector<Graph> graphs;
+  std::vector<std::unique_ptr<Graph>> graphs;  // stable addresses: workers read host copies unlocked
   std::deque<long long> queue;
   std::unordered_map<long long, std::unique_ptr<Request>> reqs;
   long long next_id = 1;
@@ -101,6 +107,21 @@ long long scratch_bytes(const fp_server::Graph& g, fp_metric metric, int K, int 
   return b;
 }
 
+// a new context holding graph g's upload (the caller holds no lock)
+fp_status upload_replica(fp_server* s, const fp_server::Graph& g, fp_ctx** out) {
+  fp_ctx* ctx = nullptr;
+  fp_status st = fp_ctx_create(&ctx, s->device);
+  if (st) return st;
+  st = fp_upload(ctx, g.nov, g.noe, g.net_ptr.data(), g.net_pins.data(), nullptr, nullptr,
+                 g.src.empty() ? nullptr : g.src.data(), g.vwgt.empty() ? nullptr : g.vwgt.data(), nullptr);
+  if (st) {
+    fp_ctx_destroy(ctx);
+    return st;
+  }
+  *out = ctx;
+  return FP_OK;
+}
+
 void run_request(fp_server* s, fp_server::Request& q, fp_ctx* ctx) {
   fp_status st = fp_set_partition(ctx, q.K, q.part.data());
   fp_refine_stats stats{};
@@ -117,42 +138,75 @@ void worker_loop(fp_server* s) {
   (void)hipSetDevice(s->device);
   for (;;) {
     fp_server::Request* q = nullptr;
-    int gi = -1;
+    int gi = -1, ci = -1;
     fp_ctx* ctx = nullptr;
     {
       std::unique_lock<std::mutex> lk(s->mu);
       for (;;) {
         if (s->stop) return;
-        // first request in arrival order whose hypergraph is free and that
-        // fits the budget (a request larger than the budget runs alone)
+        // first request in arrival order whose hypergraph has a free context,
+        // or may get another, and that fits the budget (a request larger
+        // than the budget runs alone)
         for (auto it = s->queue.begin(); it != s->queue.end(); ++it) {
           fp_server::Request* c = s->reqs[*it].get();
-          fp_server::Graph& cg = s->graphs[(size_t)c->graph];
-          if (cg.busy) continue;
+          fp_server::Graph& cg = *s->graphs[(size_t)c->graph];
+          int free_ci = -1;
+          for (size_t k = 0; k < cg.ctxs.size() && free_ci < 0; k++)
+            if (!cg.busy[k]) free_ci = (int)k;
+          const bool grow = free_ci < 0 && (int)cg.ctxs.size() + cg.creating < s->max_replicas;
+          if (free_ci < 0 && !grow) continue;
           if (s->inflight > 0 && s->inflight_bytes + c->footprint > s->budget) break;
           q = c;
           gi = c->graph;
+          ci = free_ci;
           s->queue.erase(it);
           break;
         }
         if (q) break;
         s->cv_work.wait(lk);
       }
-      // graphs may be added meanwhile: index the vector under the lock only
-      s->graphs[(size_t)gi].busy = true;
-      ctx = s->graphs[(size_t)gi].ctx;
+      fp_server::Graph& g = *s->graphs[(size_t)gi];
+      if (ci >= 0) {
+        g.busy[(size_t)ci] = 1;
+        ctx = g.ctxs[(size_t)ci];
+      } else {
+        g.creating++;
+      }
       s->inflight++;
       s->inflight_bytes += q->footprint;
       s->peak_inflight = std::max(s->peak_inflight, s->inflight);
       s->peak_bytes = std::max(s->peak_bytes, s->inflight_bytes);
       q->start_ms = s->now_ms();
     }
-    run_request(s, *q, ctx);
+    if (ci < 0) {
+      // a new replica for this request, uploaded outside the lock from the
+      // host copy (which the server never changes after registration)
+      const fp_server::Graph* gp = nullptr;
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        gp = s->graphs[(size_t)gi].get();
+      }
+      fp_ctx* nc = nullptr;
+      const fp_status st = upload_replica(s, *gp, &nc);
+      std::lock_guard<std::mutex> lk(s->mu);
+      fp_server::Graph& g = *s->graphs[(size_t)gi];
+      g.creating--;
+      if (st == FP_OK) {
+        g.ctxs.push_back(nc);
+        g.busy.push_back(1);
+        ci = (int)g.ctxs.size() - 1;
+        ctx = nc;
+      } else {
+        q->status = st;
+        std::snprintf(q->err, sizeof(q->err), "replica upload failed");
+      }
+    }
+    if (ctx) run_request(s, *q, ctx);
     {
       std::lock_guard<std::mutex> lk(s->mu);
       q->end_ms = s->now_ms();
       q->done = true;
-      s->graphs[(size_t)gi].busy = false;
+      if (ci >= 0) s->graphs[(size_t)gi]->busy[(size_t)ci] = 0;
       s->inflight--;
       s->inflight_bytes -= q->footprint;
       s->completed++;
@@ -194,8 +248,16 @@ void fp_server_destroy(fp_server* s) {
   }
   s->cv_work.notify_all();
   for (auto& t : s->workers) t.join();
-  for (auto& g : s->graphs) fp_ctx_destroy(g.ctx);
+  for (auto& g : s->graphs)
+    for (fp_ctx* c : g->ctxs) fp_ctx_destroy(c);
   delete s;
+}
+
+fp_status fp_server_set_replicas(fp_server* s, int max_replicas) {
+  if (!s || max_replicas < 1 || max_replicas > 64) return FP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->max_replicas = max_replicas;
+  return FP_OK;
 }
 
 const char* fp_server_last_error(const fp_server* s) { return s ? s->err : "null server"; }
@@ -212,13 +274,20 @@ fp_status fp_server_add_graph(fp_server* s, int nov, int noe, const unsigned* ne
     fp_ctx_destroy(ctx);
     return st;
   }
+  auto g = std::make_unique<fp_server::Graph>();
+  g->ctxs.push_back(ctx);
+  g->busy.push_back(0);
+  g->nov = nov;
+  g->noe = noe;
+  g->pins = net_ptr[noe];
+  g->bytes = fp_ctx_device_bytes(ctx);
+  // host copy for replicas
+  g->net_ptr.assign(net_ptr, net_ptr + noe + 1);
+  g->net_pins.assign(net_pins, net_pins + g->pins);
+  if (src) g->src.assign(src, src + noe);
+  if (vwgt) g->vwgt.assign(vwgt, vwgt + nov);
   std::lock_guard<std::mutex> lk(s->mu);
-  fp_server::Graph g;
-  g.ctx = ctx;
-  g.nov = nov;
-  g.pins = net_ptr[noe];
-  g.bytes = fp_ctx_device_bytes(ctx);
-  s->graphs.push_back(g);
+  s->graphs.push_back(std::move(g));
   *graph_id = (int)s->graphs.size() - 1;
   return FP_OK;
 }
@@ -231,7 +300,7 @@ fp_status fp_server_submit(fp_server* s, int graph_id, fp_metric metric, int K, 
   {
     std::lock_guard<std::mutex> lk(s->mu);
     if (graph_id < 0 || graph_id >= (int)s->graphs.size()) return sfail(s, FP_ERR_ARG, "no hypergraph %d", graph_id);
-    const fp_server::Graph& g = s->graphs[(size_t)graph_id];
+    const fp_server::Graph& g = *s->graphs[(size_t)graph_id];
     q->part.assign(part, part + g.nov);
     q->footprint = g.bytes + scratch_bytes(g, metric, K, sched ? sched->round : 0);
     if (q->footprint > s->budget)

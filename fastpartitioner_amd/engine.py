@@ -82,6 +82,7 @@ def declare(lib) -> None:
     lib.fp_server_destroy.restype = None
     lib.fp_server_last_error.argtypes = [P]
     lib.fp_server_last_error.restype = c.c_char_p
+    lib.fp_server_set_replicas.argtypes = [P, c.c_int]
     lib.fp_server_add_graph.argtypes = [P, c.c_int, c.c_int, _U32P, _U32P, _I32P, _I32P, c.POINTER(c.c_int)]
     lib.fp_server_submit.argtypes = [P, c.c_int, c.c_int, c.c_int, _I32P, c.POINTER(Schedule), c.c_int,
                                      c.POINTER(c.c_longlong)]
@@ -286,6 +287,9 @@ class Server:
     def _check(self, rc, what):
         if rc != 0:
             raise _native.FastPartError(rc, what, self.lib.fp_server_last_error(self.s).decode())
+
+    def set_replicas(self, max_replicas: int) -> None:
+        self._check(self.lib.fp_server_set_replicas(self.s, max_replicas), "fp_server_set_replicas")
 
     def add_graph(self, hg: Hypergraph) -> int:
         gid = ctypes.c_int()

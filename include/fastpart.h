@@ -237,8 +237,15 @@ fp_status fp_server_create(fp_server** server, int device, long long hbm_budget,
  * dropped, so wait for every submitted request first. */
 void fp_server_destroy(fp_server* server);
 const char* fp_server_last_error(const fp_server* server);
+/* Up to max_replicas contexts (1..64, default 1) per hypergraph: a request
+ * for a hypergraph whose contexts are all busy gets a new one, uploaded from
+ * the server's host copy, instead of waiting. Replicas stay resident until
+ * fp_server_destroy, outside the budget of the requests in flight. No
+ * counterpart in the reference. */
+fp_status fp_server_set_replicas(fp_server* server, int max_replicas);
 /* Registers (uploads) a hypergraph: net->pins CSR, net sources (NULL: none),
- * vertex weights (NULL: unit). graph_id receives its index. */
+ * vertex weights (NULL: unit); the server keeps a host copy for replicas.
+ * graph_id receives its index. */
 fp_status fp_server_add_graph(fp_server* server, int nov, int noe, const unsigned* net_ptr, const unsigned* net_pins,
                               const int* src, const int* vwgt, int* graph_id);
 /* Queues a request: `passes` passes of `metric` under `sched` (NULL for CN) on

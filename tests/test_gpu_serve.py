@@ -76,3 +76,19 @@ def test_serve_budget_bounds_the_bytes_in_flight():
     assert stats["peak_bytes"] <= biggest
     for (i1, p1), (i2, p2) in zip(out, out2):
         assert np.array_equal(p1, p2) and i1["metrics"] == i2["metrics"]
+
+
+def test_replicas_run_one_hypergraphs_requests_side_by_side():
+    graphs, reqs = _workload()
+    one = [(0, metric, K, part, sched, passes) for g, metric, K, part, sched, passes in reqs if g == 0]
+    with Server(0, hbm_budget=1 << 36, workers=6) as srv:
+        srv.set_replicas(3)
+        srv.add_graph(graphs[0])
+        rids = [srv.submit(g, metric, K, part, sched, passes) for g, metric, K, part, sched, passes in one]
+        out = [srv.wait(r) for r in rids]
+        stats = srv.stats()
+    assert stats["peak_inflight"] > 1  # one hypergraph, several requests at once
+    for (info, part), (g, metric, K, p0, sched, passes) in zip(out, one):
+        want_part, want_m = _alone(graphs[0], metric, K, p0, sched, passes)
+        assert np.array_equal(part, want_part), (metric, K)
+        assert info["metrics"] == want_m, (metric, K)
