@@ -676,7 +676,7 @@ fp_status ensure_conn_scratch(fp_ctx* ctx, int R) {
   }
   if (R > c.rmax) {
     FP_TRY(dalloc(ctx, &c.slot_mi, (size_t)R));
-    FP_TRY(dalloc(ctx, &c.mv, (size_t)R));
+    FP_TRY(dalloc(ctx, &c.mv, (size_t)R + 1));  // + the end sentinel k_round_compact writes
     c.rmax = R;
   }
   if (R > ctx->rmax) {

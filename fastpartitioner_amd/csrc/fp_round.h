@@ -11,7 +11,9 @@ namespace fpe {
 namespace {
 
 // Accepted moves of the round in visiting order: slot_mi[slot] = move index
-// (or -1), mv[i] = (vertex, own part, new part, first event slot).
+// (or -1), mv[i] = (vertex, own part, new part, first event slot), and after
+// the last move a sentinel whose event slot is the round's event total (mv
+// holds one more entry than the round has slots).
 constexpr int COMPACT_RUN = 16;  // consecutive slots per thread per step of k_round_compact
 
 __global__ void __launch_bounds__(1024) k_round_compact(const int* __restrict__ slot_dec, const int* __restrict__ slot_own,
@@ -82,6 +84,7 @@ __global__ void __launch_bounds__(1024) k_round_compact(const int* __restrict__ 
   if (threadIdx.x == 0) {
     mstat[2] = base;
     mstat[3] = base;
+    mv[base] = make_int4(-1, -1, -1, ebase);  // end sentinel: move i has mv[i + 1].w - mv[i].w events
   }
 }
 

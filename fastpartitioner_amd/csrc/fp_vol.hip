@@ -470,7 +470,8 @@ __device__ __forceinline__ void vol_event_changes(const int4 x, const int2 od, F
 // a part outside S whose value + positive changes stays below A_low never
 // reaches the max within the batch (the max is at least A_low until the
 // round stops). The others (the candidates, S included) are followed move by
-// move in a candidates x moves table: before the round stops, the max after
+// move in a candidates x moves table of changes, summed along the moves by
+// wave scans: before the round stops, the max after
 // move t is the largest value of S's candidates, the max set is the S
 // candidates holding it, and move t stops the round iff a candidate outside
 // S reaches it or an S candidate not in the max set after move t - 1 is in
@@ -532,20 +533,14 @@ __device__ int vol_scan_batch(int* vol, int K, const int4* stage, int tot, int n
     });
   }
   wave_lds_sync();
-  if (lane < nc) {
-    int v = vol[B.cpart[lane]];
-    for (int t = 0; t < nb; t++) {
-      v += B.D[lane * DROW + t];
-      B.D[lane * DROW + t] = v;
-    }
-  }
-  wave_lds_sync();
-  // lane t: the max and max set after move t, and whether t stops the round
+  // lane t: the max and max set after move t, and whether t stops the
+  // round; candidate r's value after move t is its value before the batch
+  // plus the inclusive prefix of its row up to column t
   int A = INT_MIN, Bm = INT_MIN;
   unsigned long long mask = 0, m0 = 0;
   for (int r = 0; r < nc; r++) {
     const bool s0 = B.cflag[r] != 0;
-    const int v = lane < nb ? B.D[r * DROW + lane] : 0;
+    const int v = vol[B.cpart[r]] + wave_incl_scan(lane < nb ? B.D[r * DROW + lane] : 0);
     if (s0) {
       m0 |= 1ull << r;
       mask = v > A ? 1ull << r : (v == A ? mask | 1ull << r : mask);
@@ -580,8 +575,7 @@ __device__ int vol_scan_batch(int* vol, int K, const int4* stage, int tot, int n
 // stage holds is read from HBM). Writes back vol, the max and its
 // multiplicity, and the number of moves to commit (mstat[3]).
 __global__ void __launch_bounds__(WAVE) k_vol_scan(int* __restrict__ vol_g, int K, const int4* __restrict__ mv,
-                                                   const int4* __restrict__ evbuf,
-                                                   const unsigned* __restrict__ vtx_ptr, int* __restrict__ mstat,
+                                                   const int4* __restrict__ evbuf, int* __restrict__ mstat,
                                                    int stage_cap, int batched) {
   extern __shared__ __align__(16) unsigned char lds[];
   int4* stage = reinterpret_cast<int4*>(lds);
@@ -611,15 +605,27 @@ __global__ void __launch_bounds__(WAVE) k_vol_scan(int* __restrict__ vol_g, int 
   const int nmoves = mstat[2];
   int nstop = nmoves;
   bool stopped = false;
+  // headers of 64 moves from m0: (vertex, own, new, event offset) and the
+  // event count, the next offset minus this one (mv ends with a sentinel)
+  auto load_head = [&](int m0, int4& h, int& dg) {
+    const bool in = m0 + lane < nmoves;
+    h = in ? mv[m0 + lane] : make_int4(0, 0, 0, 0);
+    dg = in ? mv[m0 + lane + 1].w - h.w : 0;
+  };
+  int4 hm;
+  int hdeg;
+  load_head(0, hm, hdeg);
   for (int mi = 0; mi < nmoves && !stopped;) {
-    // headers of the next (up to) 64 moves; the leading ones whose events fit the stage
-    const int4 hm = mi + lane < nmoves ? mv[mi + lane] : make_int4(0, 0, 0, 0);
-    const int hdeg = mi + lane < nmoves ? (int)(vtx_ptr[hm.x + 1] - vtx_ptr[hm.x]) : 0;
+    // this batch: the leading moves of the 64 headers at hand whose events fit the stage
     const int cum = wave_incl_scan(hdeg);
     int nb = __popcll(__ballot(mi + lane < nmoves && cum <= stage_cap));
     const bool staged = nb > 0;
     if (!staged) nb = 1;
     const unsigned eb0 = (unsigned)__builtin_amdgcn_readfirstlane(hm.w);
+    // the next batch's headers, in flight while this batch runs
+    int4 hm_n;
+    int hdeg_n;
+    load_head(mi + nb, hm_n, hdeg_n);
     int tot = 0;
     if (staged) {
       tot = __builtin_amdgcn_readlane(cum, nb - 1);
@@ -671,6 +677,8 @@ __global__ void __launch_bounds__(WAVE) k_vol_scan(int* __restrict__ vol_g, int 
       }
     }
     mi += nb;
+    hm = hm_n;
+    hdeg = hdeg_n;
     wave_lds_sync();
   }
   const int cntM = maxset_count(ms);
@@ -698,7 +706,7 @@ fp_status ensure_vol_scratch(fp_ctx* ctx, int R) {
   }
   if (R > s.rmax) {
     FP_TRY(dalloc(ctx, &s.slot_mi, (size_t)R));
-    FP_TRY(dalloc(ctx, &s.mv, (size_t)R));
+    FP_TRY(dalloc(ctx, &s.mv, (size_t)R + 1));  // + the end sentinel k_round_compact writes
     s.rmax = R;
   }
   const long long need = std::max(ctx->pins, 1ll);
@@ -815,8 +823,7 @@ fp_status pass_vol(fp_ctx* ctx, const RefinePlan& p) {
     ra.net_blocks = net_grid(ctx, REPLAY_NET_BLOCKS, REPLAY_BLOCK / WAVE);
     ra.commit = 0;
     k_vol_replay<<<ra.net_blocks, REPLAY_BLOCK, lds_r, ctx->stream>>>(ra);
-    k_vol_scan<<<1, WAVE, lds_s, ctx->stream>>>(vs.vol, K, vs.mv, vs.evbuf, ctx->vtx_ptr, vs.mstat, stage_cap,
-                                                 batched);
+    k_vol_scan<<<1, WAVE, lds_s, ctx->stream>>>(vs.vol, K, vs.mv, vs.evbuf, vs.mstat, stage_cap, batched);
     ra.commit = 1;
     k_vol_replay<<<ra.net_blocks + (nslots + REPLAY_BLOCK - 1) / REPLAY_BLOCK, REPLAY_BLOCK, lds_r,
                    ctx->stream>>>(ra);
