@@ -30,7 +30,10 @@ constexpr int TILE = 64;         // slots per admission tile
 #define FP_NETS_IN_FLIGHT 4
 #endif
 constexpr int NETS_IN_FLIGHT = FP_NETS_IN_FLIGHT;  // segment head windows loaded together per wave
-constexpr int LONG_WINDOWS = 4;                    // windows of one long list loaded together
+#ifndef FP_LONG_WINDOWS
+#define FP_LONG_WINDOWS 2
+#endif
+constexpr int LONG_WINDOWS = FP_LONG_WINDOWS;      // windows of one long list loaded together
 constexpr int APPLY_BLOCK = 256;
 constexpr int APPLY_NET_BLOCKS = 2048;
 
