@@ -12,9 +12,11 @@
 // A TM round (the reference runs a whole pass as one round):
 //   k_conn_eval   one wavefront per vertex: receivers, senders and gain on the
 //                 frozen state, then the best part over all k (lanes = k)
+//   k_round_compact the candidates in vertex order
 //   k_tm_admit    one wavefront: candidates in vertex order, skipping a
 //                 destination whose size reached C * 1.02 (TM/partition.cpp:
-//                 1115-1122); commits part vector and sizes
+//                 1115-1122), decided 64 at a time by size bounds where they
+//                 allow; commits part vector and sizes
 //   k_conn_apply  (mode 0) one wavefront per touched net replays the admitted
 //                 moves on its list and adds each move's pc changes (atomics:
 //                 pc's final value does not depend on their order)

@@ -6,7 +6,7 @@
 //                software-pipelined over the wave's vertices) sweeps the lists
 //                of the vertex's nets in their 16-bit evaluation view -- one
 //                128-byte read per net brings its lambda and first 63
-//                entries, 8 nets in flight -- into LDS histograms
+//                entries, 4 nets in flight -- into LDS histograms
 //                (pin count + first-appearance rank per part), then picks the
 //                LDG2 decision by a lexicographic wave reduction and applies
 //                the TV gain test. Candidates stamp their nets into the

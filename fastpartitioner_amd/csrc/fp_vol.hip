@@ -11,14 +11,15 @@
 //                  reference's per-part gain/loss terms, accepted iff every
 //                  max part gains at least (MSV) / more than (MSRV) it loses
 //                  and no other part reaches the max
-//   k_vol_compact  accepted moves in visiting order, with event offsets
+//   k_round_compact accepted moves in visiting order, with event offsets
 //   k_vol_replay   (dry) one wavefront per touched net replays the round's
 //                  moves on its list in order and records, per (move, net),
 //                  the volume changes the reference's apply makes
 //   k_vol_scan     one wavefront applies the moves' volume changes in order,
 //                  tracks the max set, and stops after the first move whose
 //                  max set holds a part the previous one did not
-//                  (MGV/partition.cpp:977-1003)
+//                  (MGV/partition.cpp:977-1003); batches of 64 moves are
+//                  decided together where bounds allow
 //   k_vol_replay   (commit) list edits, part vector and both part sizes of the
 //                  moves up to the stop
 #include "fp_round.h"
