@@ -131,7 +131,9 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int K = a.K, KW = (K + 31) / 32;
   const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
-  int* recv = reinterpret_cast<int*>(lds) + (size_t)wave * (size_t)(2 * K + 3 * KW);
+  // block-shared: bit k of vmask = part k is lighter than C (TM's candidates)
+  unsigned long long* vmask = reinterpret_cast<unsigned long long*>(lds);
+  int* recv = reinterpret_cast<int*>(vmask + a.W) + (size_t)wave * (size_t)(2 * K + 3 * KW);
   int* send = recv + K;
   unsigned* rmark = reinterpret_cast<unsigned*>(send + K);
   unsigned* smark = rmark + KW;
@@ -142,7 +144,13 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
     sgain[q] = 0;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0 && a.tcount_next) *a.tcount_next = 0;
-  wave_lds_sync();
+  if (!a.msm)
+    for (int w = wave; w < a.W; w += nw) {
+      const int k = w * WAVE + lane;
+      const unsigned long long b = __ballot(k < K && (double)a.sizes[k] < a.C);
+      if (lane == 0) vmask[w] = b;
+    }
+  __syncthreads();
   const int M = a.msm ? a.mstat[0] : 0, cntM = a.msm ? a.mstat[1] : 0;
   const unsigned long long lt = (1ull << lane) - 1ull;
 
@@ -213,9 +221,36 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
     int dec = -1;
     if (gain > 0) {
       if (!a.msm) {
-        // TM: the largest gain - loss > 0 over parts lighter than C, the first k on ties
+        // TM: the largest gain - loss > 0 over parts lighter than C, the first
+        // k on ties. A part connected to every receiver and every sender has
+        // loss 0, so improvement = gain, the largest possible: the first such
+        // candidate is found by ANDing their bitmap words, 64 parts at a time
+        // (an item's own bit counts as connected: its loss term skips k ==
+        // the item)
+        int kfast = -1;
+        for (int w = 0; w < a.W && kfast < 0; w++) {
+          unsigned long long allc = vmask[w];
+          if ((own >> 6) == w) allc &= ~(1ull << (own & 63));
+          for (int i = 0; i < nrecv && allc; i++) {
+            const int r = __builtin_amdgcn_readfirstlane(recv[i]);
+            unsigned long long x = a.BT[(size_t)r * (size_t)a.W + (size_t)w];
+            if ((r >> 6) == w) x |= 1ull << (r & 63);
+            allc &= x;
+          }
+          for (int i = 0; i < nsend && allc; i++) {
+            const int sp = __builtin_amdgcn_readfirstlane(send[i]);
+            unsigned long long x = a.B[(size_t)sp * (size_t)a.W + (size_t)w];
+            if ((sp >> 6) == w) x |= 1ull << (sp & 63);
+            allc &= x;
+          }
+          if (allc) kfast = w * WAVE + __builtin_ctzll(allc);
+        }
         int bimp = 0, bk = INT_MAX;
-        for (int k0 = 0; k0 < K; k0 += WAVE) {
+        if (kfast >= 0) {
+          bimp = gain;
+          bk = kfast;
+        }
+        for (int k0 = 0; kfast < 0 && k0 < K; k0 += WAVE) {
           const int k = k0 + lane, w = k0 / WAVE;
           int loss = 0;
           for (int i = 0; i < nrecv; i++) {
@@ -627,10 +662,16 @@ __global__ void __launch_bounds__(WAVE) k_msm_scan(int* __restrict__ pc, const i
   }
 }
 
-int conn_eval_waves(int K) {
+// LDS of k_conn_eval: the block's candidate mask, then per wavefront the
+// receiver and sender lists and three K-bit marks
+size_t conn_eval_lds(int K, int waves) {
   const size_t per = ((size_t)2 * (size_t)K + 3 * (size_t)((K + 31) / 32)) * sizeof(int);
+  return (size_t)((K + 63) / 64) * sizeof(unsigned long long) + (size_t)waves * per;
+}
+
+int conn_eval_waves(int K) {
   for (int w = CONN_BLOCK / WAVE; w >= 1; w--)
-    if ((size_t)w * per <= (size_t)LDS_BUDGET) return w;
+    if (conn_eval_lds(K, w) <= (size_t)LDS_BUDGET) return w;
   return 0;
 }
 
@@ -729,7 +770,7 @@ fp_status plan_conn(fp_ctx* ctx, RefinePlan& p) {
 fp_status pass_conn(fp_ctx* ctx, const RefinePlan& p) {
   const bool msm = p.metric == FP_METRIC_MSM;
   const int nov = ctx->nov, K = ctx->K, W = (K + 63) / 64, R = p.R, waves = p.eval_waves;
-  const size_t lds_e = (size_t)waves * ((size_t)2 * (size_t)K + 3 * (size_t)((K + 31) / 32)) * sizeof(int);
+  const size_t lds_e = conn_eval_lds(K, waves);
   const size_t lds_r = (size_t)(REPLAY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
   auto& c = ctx->conn;
   const double thr = p.sched.ldg_C * (1.02);
