@@ -356,14 +356,15 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
 // TM apply order (TM/partition.cpp:1111-1125): the round's candidates (the
 // compacted move list) in vertex order; one whose destination size is at
 // least thr = C * 1.02 at that point is dropped. Commits part vector and both
-// sizes of the others. Chunks of 64 candidates: with in[p] / out[p] the
-// chunk's candidate weight into / out of part p, a candidate into d is
-// dropped whatever the chunk's other decisions are when sz[d] - out[d] >=
-// thr, and admitted when sz[d] + in[d] < thr. The destinations of the
-// undecided candidates are "open"; the chunk's admitted candidates that touch
-// no open part are applied in parallel, then lane 0 walks the rest in order
-// (undecided ones get their decision from the open part's size at that
-// point).
+// sizes of the others. Sizes are integers, so size < thr is size < T with
+// T = ceil(thr). Chunks of 64 candidates: with in[p] / out[p] the chunk's
+// candidate weight into / out of part p, a candidate into d is dropped
+// whatever the chunk's other decisions are when sz[d] - out[d] >= T, and
+// admitted when sz[d] + in[d] < T. The destinations of the undecided
+// candidates are "open"; each open part gets an owner lane that holds its
+// size in a register. The undecided candidates and the admitted ones that
+// touch an open part are walked in order, reading and updating only those
+// registers; then every admitted move of the chunk is added to the sizes.
 __global__ void __launch_bounds__(WAVE) k_tm_admit(int* __restrict__ slot_dec, const int4* __restrict__ mv,
                                                    const int* __restrict__ mstat, const int* __restrict__ vwgt,
                                                    int r0, int K, double thr, long long* __restrict__ sizes,
@@ -373,6 +374,7 @@ __global__ void __launch_bounds__(WAVE) k_tm_admit(int* __restrict__ slot_dec, c
   int* inw = reinterpret_cast<int*>(sz + K);
   int* outw = inw + K;
   const int lane = lane_id();
+  const long long T = thr < 9.0e18 ? (long long)ceil(thr) : LLONG_MAX;
   for (int q = lane; q < K; q += WAVE) {
     sz[q] = sizes[q];
     inw[q] = 0;
@@ -399,8 +401,8 @@ __global__ void __launch_bounds__(WAVE) k_tm_admit(int* __restrict__ slot_dec, c
       wave_lds_sync();
       if (valid) {
         const long long sd = sz[d];
-        if ((double)(sd - (long long)outw[d]) >= thr) status = -1;
-        else if ((double)(sd + (long long)inw[d]) < thr) status = 1;
+        if (sd - (long long)outw[d] >= T) status = -1;
+        else if (sd + (long long)inw[d] < T) status = 1;
       }
       wave_lds_sync();
       if (valid) {
@@ -409,39 +411,43 @@ __global__ void __launch_bounds__(WAVE) k_tm_admit(int* __restrict__ slot_dec, c
       }
       wave_lds_sync();
     }
-    // open parts: inw doubles as the flag array (all 0 here)
-    if (valid && status == 0) inw[d] = 1;
+    // owners of the open parts: inw[p] = owner lane + 1 (0: p is not open)
+    const bool und = valid && status == 0;
+    if (und) inw[d] = lane + 1;
     wave_lds_sync();
-    const bool walk = valid && (status == 0 || (status > 0 && (inw[o] != 0 || inw[d] != 0)));
-    if (valid && !walk) {
-      if (status < 0) {
-        slot_dec[v - r0] = -1;
-      } else {
+    const int down = valid ? inw[d] : 0, oown = valid ? inw[o] : 0;
+    long long mysz = und && down == lane + 1 ? sz[d] : 0;
+    const bool walk = und || (valid && status > 0 && (down | oown) != 0);
+    bool adm = valid && status > 0;
+    const int pk = down | (oown << 8) | (und ? 1 << 16 : 0);
+    for (unsigned long long b = __ballot(walk); b; b &= b - 1) {
+      const int l = __builtin_amdgcn_readfirstlane((int)__builtin_ctzll(b));
+      const int q = __builtin_amdgcn_readlane(pk, l), ww = __builtin_amdgcn_readlane(w, l);
+      const int dl = (q & 0xFF) - 1, ol = ((q >> 8) & 0xFF) - 1;
+      bool a = true;
+      if (q >> 16) {  // undecided: its destination's size at this point decides
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(unsigned long long)mysz, dl);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((unsigned long long)mysz >> 32), dl);
+        a = (long long)(((unsigned long long)hi << 32) | lo) < T;
+      }
+      if (a) {
+        if (lane == dl) mysz += ww;
+        if (lane == ol) mysz -= ww;
+      }
+      if (lane == l) adm = a;
+    }
+    wave_lds_sync();
+    if (und) inw[d] = 0;
+    if (valid) {
+      if (adm) {
         atomicAdd(reinterpret_cast<unsigned long long*>(&sz[o]), (unsigned long long)(-(long long)w));
         atomicAdd(reinterpret_cast<unsigned long long*>(&sz[d]), (unsigned long long)(long long)w);
         part[v] = d;
+      } else {
+        slot_dec[v - r0] = -1;
       }
     }
-    nm += (unsigned long long)__popcll(__ballot(valid && !walk && status > 0));
-    wave_lds_sync();
-    if (valid && status == 0) inw[d] = 0;
-    // lane 0 walks the rest in order; only it touches sz meanwhile
-    for (unsigned long long b = __ballot(walk); b; b &= b - 1) {
-      const int l = __builtin_amdgcn_readfirstlane((int)__builtin_ctzll(b));
-      const int vv = __builtin_amdgcn_readlane(v, l), oo = __builtin_amdgcn_readlane(o, l),
-                dd = __builtin_amdgcn_readlane(d, l), ww = __builtin_amdgcn_readlane(w, l),
-                st = __builtin_amdgcn_readlane(status, l);
-      if (lane == 0) {
-        if (st == 0 && (double)sz[dd] >= thr) {
-          slot_dec[vv - r0] = -1;
-        } else {
-          sz[oo] -= ww;
-          sz[dd] += ww;
-          part[vv] = dd;
-          nm++;
-        }
-      }
-    }
+    nm += (unsigned long long)__popcll(__ballot(valid && adm));
     wave_lds_sync();
   }
   for (int q = lane; q < K; q += WAVE) sizes[q] = sz[q];
@@ -722,6 +728,7 @@ fp_status ensure_conn_scratch(fp_ctx* ctx, int R) {
     FP_TRY(dalloc(ctx, &c.mv, (size_t)R + 1));  // + the end sentinel k_round_compact writes
     c.rmax = R;
   }
+  FP_TRY(ensure_compact_tiles(ctx, R));
   if (R > ctx->rmax) {
     FP_TRY(dalloc(ctx, &ctx->slot_own, (size_t)R));
     FP_TRY(dalloc(ctx, &ctx->slot_dec, (size_t)R));
@@ -838,15 +845,13 @@ fp_status pass_conn(fp_ctx* ctx, const RefinePlan& p) {
     ap.r1 = r0 + nslots;
     ap.net_blocks = net_grid(ctx, REPLAY_NET_BLOCKS, REPLAY_BLOCK / WAVE);
     if (!msm) {
-      k_round_compact<<<1, 1024, 0, ctx->stream>>>(ctx->slot_dec, ctx->slot_own, nullptr, ctx->vtx_ptr, r0, nslots,
-                                                    c.slot_mi, c.mv, c.mstat);
+      launch_round_compact<true>(ctx, nullptr, r0, nslots, c.slot_mi, c.mv, c.mstat);
       k_tm_admit<<<1, WAVE, (size_t)K * (sizeof(long long) + 2 * sizeof(int)), ctx->stream>>>(
           ctx->slot_dec, c.mv, c.mstat, ctx->vwgt, r0, K, thr, ctx->sizes, ctx->part, ctx->moves);
       ap.mode = 0;
       k_conn_apply<<<ap.net_blocks, REPLAY_BLOCK, lds_r, ctx->stream>>>(ap);
     } else {
-      k_round_compact<<<1, 1024, 0, ctx->stream>>>(ctx->slot_dec, ctx->slot_own, nullptr, ctx->vtx_ptr, r0, nslots,
-                                                    c.slot_mi, c.mv, c.mstat);
+      launch_round_compact<false>(ctx, nullptr, r0, nslots, c.slot_mi, c.mv, c.mstat);
       ap.mode = 1;
       k_conn_apply<<<ap.net_blocks, REPLAY_BLOCK, lds_r, ctx->stream>>>(ap);
       k_msm_scan<<<1, WAVE, ((size_t)K + WAVE) * sizeof(int), ctx->stream>>>(c.pc, c.msg, K, c.mv, c.evbuf,

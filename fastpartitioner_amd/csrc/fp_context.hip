@@ -303,6 +303,7 @@ fp_status fp_upload(fp_ctx* ctx, int nov, int noe, const unsigned* net_ptr, cons
   ctx->kcap = 0;
   ctx->rmax = 0;
   ctx->hist_cap = 0;
+  ctx->compact_cap = 0;
   ctx->cn.ready = false;
   ctx->cn.ops_cap = 0;
   ctx->vol.rmax = 0;

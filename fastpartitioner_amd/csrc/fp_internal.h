@@ -441,6 +441,8 @@ struct fp_ctx {
   size_t hist_cap = 0;
   int *over_idx = nullptr, *over_count = nullptr;
   unsigned long long* moves = nullptr;
+  int2* compact_tiles = nullptr;  // per-tile move / event totals of a round's compaction
+  int compact_cap = 0;
 
   // CN refinement scratch (allocated on the first CN refinement of an upload)
   struct {

@@ -710,6 +710,7 @@ fp_status ensure_vol_scratch(fp_ctx* ctx, int R) {
     FP_TRY(dalloc(ctx, &s.mv, (size_t)R + 1));  // + the end sentinel k_round_compact writes
     s.rmax = R;
   }
+  FP_TRY(ensure_compact_tiles(ctx, R));
   const long long need = std::max(ctx->pins, 1ll);
   if (need > s.evcap) {
     FP_TRY(dalloc(ctx, &s.evbuf, (size_t)need));
@@ -795,8 +796,7 @@ fp_status pass_vol(fp_ctx* ctx, const RefinePlan& p) {
     int egrid = std::max(1, std::min((nslots + waves - 1) / waves, 8192));
     if (p.eval_grid > 0) egrid = std::min(egrid, p.eval_grid);
     k_vol_eval<<<egrid, waves * WAVE, lds_e, ctx->stream>>>(a);
-    k_round_compact<<<1, 1024, 0, ctx->stream>>>(ctx->slot_dec, ctx->slot_own, ctx->order, ctx->vtx_ptr, r0, nslots,
-                                                  vs.slot_mi, vs.mv, vs.mstat);
+    launch_round_compact<false>(ctx, ctx->order, r0, nslots, vs.slot_mi, vs.mv, vs.mstat);
     VolReplayArgs ra{};
     ra.net_ptr = ctx->net_ptr;
     ra.apply_pins = ctx->apply_pins;
