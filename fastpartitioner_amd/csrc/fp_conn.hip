@@ -12,7 +12,7 @@
 // A TM round (the reference runs a whole pass as one round):
 //   k_conn_eval   one wavefront per vertex: receivers, senders and gain on the
 //                 frozen state, then the best part over all k (lanes = k)
-//   k_round_compact the candidates in vertex order
+//   k_round_compact the candidates in vertex order, with their weights
 //   k_tm_admit    one wavefront: candidates in vertex order, skipping a
 //                 destination whose size reached C * 1.02 (TM/partition.cpp:
 //                 1115-1122), decided 64 at a time by size bounds where they
@@ -365,10 +365,14 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
 // size in a register. The undecided candidates and the admitted ones that
 // touch an open part are walked in order, reading and updating only those
 // registers; then every admitted move of the chunk is added to the sizes.
+// The moves (vertex, own part, new part, weight) are read TM_PREFETCH chunks
+// at a time, the next group's loads in flight while a group is decided.
+constexpr int TM_PREFETCH = 4;
+
 __global__ void __launch_bounds__(WAVE) k_tm_admit(int* __restrict__ slot_dec, const int4* __restrict__ mv,
-                                                   const int* __restrict__ mstat, const int* __restrict__ vwgt,
-                                                   int r0, int K, double thr, long long* __restrict__ sizes,
-                                                   int* __restrict__ part, unsigned long long* __restrict__ moves) {
+                                                   const int* __restrict__ mstat, int r0, int K, double thr,
+                                                   long long* __restrict__ sizes, int* __restrict__ part,
+                                                   unsigned long long* __restrict__ moves) {
   extern __shared__ __align__(16) unsigned char lds[];
   long long* sz = reinterpret_cast<long long*>(lds);
   int* inw = reinterpret_cast<int*>(sz + K);
@@ -383,12 +387,9 @@ __global__ void __launch_bounds__(WAVE) k_tm_admit(int* __restrict__ slot_dec, c
   wave_lds_sync();
   const int nmv = mstat[2];
   unsigned long long nm = 0;
-  for (int m0 = 0; m0 < nmv; m0 += WAVE) {
-    const int i = m0 + lane;
-    const bool valid = i < nmv;
-    const int4 x = valid ? mv[i] : make_int4(0, 0, 0, 0);
-    const int w = valid ? (vwgt ? vwgt[x.x] : 1) : 0;
-    const int v = x.x, o = x.y, d = x.z;
+  auto chunk = [&](int m0, int4 x) {
+    const bool valid = m0 + lane < nmv;
+    const int v = x.x, o = x.y, d = x.z, w = valid ? x.w : 0;
     // chunk sums stay exact in int for weights up to 2^24; otherwise every
     // candidate of the chunk stays undecided
     const bool bounded = __ballot(valid && (unsigned)w > (1u << 24)) == 0ull;
@@ -449,6 +450,25 @@ __global__ void __launch_bounds__(WAVE) k_tm_admit(int* __restrict__ slot_dec, c
     }
     nm += (unsigned long long)__popcll(__ballot(valid && adm));
     wave_lds_sync();
+  };
+  int4 cur[TM_PREFETCH];
+#pragma unroll
+  for (int j = 0; j < TM_PREFETCH; j++) {
+    const int i = j * WAVE + lane;
+    cur[j] = i < nmv ? mv[i] : make_int4(0, 0, 0, 0);
+  }
+  for (int b0 = 0; b0 < nmv; b0 += TM_PREFETCH * WAVE) {
+    int4 nxt[TM_PREFETCH];
+#pragma unroll
+    for (int j = 0; j < TM_PREFETCH; j++) {
+      const int i = b0 + (TM_PREFETCH + j) * WAVE + lane;
+      nxt[j] = i < nmv ? mv[i] : make_int4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < TM_PREFETCH; j++)
+      if (b0 + j * WAVE < nmv) chunk(b0 + j * WAVE, cur[j]);
+#pragma unroll
+    for (int j = 0; j < TM_PREFETCH; j++) cur[j] = nxt[j];
   }
   for (int q = lane; q < K; q += WAVE) sizes[q] = sz[q];
   if (lane == 0 && nm) atomicAdd(moves, nm);
@@ -847,7 +867,7 @@ fp_status pass_conn(fp_ctx* ctx, const RefinePlan& p) {
     if (!msm) {
       launch_round_compact<true>(ctx, nullptr, r0, nslots, c.slot_mi, c.mv, c.mstat);
       k_tm_admit<<<1, WAVE, (size_t)K * (sizeof(long long) + 2 * sizeof(int)), ctx->stream>>>(
-          ctx->slot_dec, c.mv, c.mstat, ctx->vwgt, r0, K, thr, ctx->sizes, ctx->part, ctx->moves);
+          ctx->slot_dec, c.mv, c.mstat, r0, K, thr, ctx->sizes, ctx->part, ctx->moves);
       ap.mode = 0;
       k_conn_apply<<<ap.net_blocks, REPLAY_BLOCK, lds_r, ctx->stream>>>(ap);
     } else {

@@ -50,11 +50,15 @@ __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE)); }
 
 // orders the LDS traffic of one wavefront: everything before is visible to
-// every lane of the wave after
+// every lane of the wave after. The LDS performs one wavefront's operations
+// in issue order, so wavefront-scope fences need no wait: this only keeps the
+// compiler from moving LDS accesses across it (a workgroup-scope fence would
+// wait for every outstanding LDS access here). Not for data shared between
+// the waves of a workgroup: that needs __syncthreads().
 __device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <typename T>

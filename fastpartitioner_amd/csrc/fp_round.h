@@ -13,29 +13,30 @@ namespace {
 // Accepted moves of the round in visiting order: slot_mi[slot] = move index
 // (or -1), mv[i] = (vertex, own part, new part, first event slot), and after
 // the last move a sentinel whose event slot is the round's event total (mv
-// holds one more entry than the round has slots). One block per tile of
-// COMPACT_TILE slots; with several tiles, k_compact_count first writes each
-// tile's move and event totals and every block adds up the ones before its
-// own.
+// holds one more entry than the round has slots). WEIGHTS rounds (TM, whose
+// apply reads no event slots and most of whose slots hold a move) store the
+// vertex weight in place of the event slot, and 0 in the sentinel. One block
+// per tile of COMPACT_TILE slots; with several tiles, k_compact_count first
+// writes each tile's move and event totals and every block adds up the ones
+// before its own.
 constexpr int COMPACT_RUN = 16;  // consecutive slots per thread
 constexpr int COMPACT_BLOCK = 1024;
 constexpr int COMPACT_TILE = COMPACT_BLOCK * COMPACT_RUN;
 
 // One thread's run of COMPACT_RUN slots: the acceptance mask (bit j: slot
-// first + j holds a move) and the decisions. DENSE (most slots hold a move,
-// as in a TM round): the accepted slots' vertices and degrees are loaded
-// here, all loads of the run issued together, the indexes of the slots
-// without a move clamped to the run's first slot so every load stays in
-// bounds. Otherwise (few moves per round) they are loaded per set bit where
-// they are used.
+// first + j holds a move) and the decisions. WEIGHTS: the accepted slots'
+// vertices and weights are loaded here, all loads of the run issued
+// together, the indexes of the slots without a move clamped to the run's
+// first slot so that every load stays in bounds. Otherwise (few moves per
+// round) vertices and degrees are loaded per set bit where they are used.
 struct CompactRun {
   unsigned acc;
-  int dec[COMPACT_RUN], v[COMPACT_RUN], deg[COMPACT_RUN];
+  int dec[COMPACT_RUN], v[COMPACT_RUN], w[COMPACT_RUN];
 };
 
-template <bool DENSE>
+template <bool WEIGHTS>
 __device__ __forceinline__ void compact_load(const int* __restrict__ slot_dec, const int* __restrict__ order,
-                                             const unsigned* __restrict__ vtx_ptr, int r0, int nslots, int first,
+                                             const int* __restrict__ vwgt, int r0, int nslots, int first,
                                              CompactRun& r) {
   static_assert(COMPACT_RUN == 16, "the run is four int4 words");
   r.acc = 0;
@@ -60,28 +61,22 @@ __device__ __forceinline__ void compact_load(const int* __restrict__ slot_dec, c
   }
 #pragma unroll
   for (int j = 0; j < COMPACT_RUN; j++) r.acc |= (r.dec[j] >= 0 ? 1u : 0u) << j;
-  if (!DENSE || !r.acc) return;
+  if (!WEIGHTS || !r.acc) return;
 #pragma unroll
   for (int j = 0; j < COMPACT_RUN; j++) {
     const int s = (r.acc >> j) & 1u ? first + j : first;
     r.v[j] = order ? order[r0 + s] : r0 + s;
   }
 #pragma unroll
-  for (int j = 0; j < COMPACT_RUN; j++) {
-    const int d = (int)(vtx_ptr[r.v[j] + 1] - vtx_ptr[r.v[j]]);
-    r.deg[j] = (r.acc >> j) & 1u ? d : 0;
-  }
+  for (int j = 0; j < COMPACT_RUN; j++) r.w[j] = vwgt ? vwgt[r.v[j]] : 1;
 }
 
-template <bool DENSE>
-__device__ __forceinline__ int compact_deg(const CompactRun& r, const int* __restrict__ order,
-                                           const unsigned* __restrict__ vtx_ptr, int r0, int first) {
+// events of the run's moves (their vertices' degrees); none for WEIGHTS
+template <bool WEIGHTS>
+__device__ __forceinline__ int compact_events(const CompactRun& r, const int* __restrict__ order,
+                                              const unsigned* __restrict__ vtx_ptr, int r0, int first) {
   int d = 0;
-  if (DENSE) {
-    if (!r.acc) return 0;
-#pragma unroll
-    for (int j = 0; j < COMPACT_RUN; j++) d += r.deg[j];
-  } else {
+  if (!WEIGHTS) {
     for (unsigned m = r.acc; m; m &= m - 1) {
       const int s = first + __builtin_ctz(m);
       const int v = order ? order[r0 + s] : r0 + s;
@@ -91,7 +86,7 @@ __device__ __forceinline__ int compact_deg(const CompactRun& r, const int* __res
   return d;
 }
 
-template <bool DENSE>
+template <bool WEIGHTS>
 __global__ void __launch_bounds__(COMPACT_BLOCK) k_compact_count(const int* __restrict__ slot_dec,
                                                                  const int* __restrict__ order,
                                                                  const unsigned* __restrict__ vtx_ptr, int r0,
@@ -99,20 +94,21 @@ __global__ void __launch_bounds__(COMPACT_BLOCK) k_compact_count(const int* __re
   __shared__ int s_w[COMPACT_BLOCK / WAVE];
   const int first = (int)blockIdx.x * COMPACT_TILE + (int)threadIdx.x * COMPACT_RUN;
   CompactRun r;
-  compact_load<DENSE>(slot_dec, order, vtx_ptr, r0, nslots, first, r);
+  compact_load<false>(slot_dec, order, nullptr, r0, nslots, first, r);
   int tot, totd;
   block_excl_scan((int)__popc(r.acc), s_w, &tot);
-  block_excl_scan(compact_deg<DENSE>(r, order, vtx_ptr, r0, first), s_w, &totd);
+  block_excl_scan(compact_events<WEIGHTS>(r, order, vtx_ptr, r0, first), s_w, &totd);
   if (threadIdx.x == 0) tiles[blockIdx.x] = make_int2(tot, totd);
 }
 
 // tiles: the per-tile totals of k_compact_count, or null for a single tile
-template <bool DENSE>
+template <bool WEIGHTS>
 __global__ void __launch_bounds__(COMPACT_BLOCK) k_round_compact(const int* __restrict__ slot_dec,
                                                                  const int* __restrict__ slot_own,
                                                                  const int* __restrict__ order,
-                                                                 const unsigned* __restrict__ vtx_ptr, int r0,
-                                                                 int nslots, const int2* __restrict__ tiles,
+                                                                 const unsigned* __restrict__ vtx_ptr,
+                                                                 const int* __restrict__ vwgt, int r0, int nslots,
+                                                                 const int2* __restrict__ tiles,
                                                                  int* __restrict__ slot_mi, int4* __restrict__ mv,
                                                                  int* __restrict__ mstat) {
   __shared__ int s_w[COMPACT_BLOCK / WAVE];
@@ -129,10 +125,10 @@ __global__ void __launch_bounds__(COMPACT_BLOCK) k_round_compact(const int* __re
   }
   const int first = (int)blockIdx.x * COMPACT_TILE + (int)threadIdx.x * COMPACT_RUN;
   CompactRun r;
-  compact_load<DENSE>(slot_dec, order, vtx_ptr, r0, nslots, first, r);
+  compact_load<WEIGHTS>(slot_dec, order, vwgt, r0, nslots, first, r);
   int tot, totd;
   const int ex = block_excl_scan((int)__popc(r.acc), s_w, &tot);
-  const int exd = block_excl_scan(compact_deg<DENSE>(r, order, vtx_ptr, r0, first), s_w, &totd);
+  const int exd = block_excl_scan(compact_events<WEIGHTS>(r, order, vtx_ptr, r0, first), s_w, &totd);
   const unsigned acc = r.acc;
   if (first + COMPACT_RUN <= nslots) {
     int4* q4 = reinterpret_cast<int4*>(slot_mi + first);
@@ -150,20 +146,16 @@ __global__ void __launch_bounds__(COMPACT_BLOCK) k_round_compact(const int* __re
     for (int j = 0; j < COMPACT_RUN && first + j < nslots; j++)
       slot_mi[first + j] = (acc >> j) & 1u ? base + ex + __popc(acc & ((1u << j) - 1u)) : -1;
   }
-  if (DENSE && acc) {
+  if (WEIGHTS && acc) {
     int own[COMPACT_RUN];
 #pragma unroll
     for (int j = 0; j < COMPACT_RUN; j++) own[j] = slot_own[(acc >> j) & 1u ? first + j : first];
-    int i = base + ex, e = ebase + exd;
+    int i = base + ex;
 #pragma unroll
     for (int j = 0; j < COMPACT_RUN; j++) {
-      if ((acc >> j) & 1u) {
-        mv[i] = make_int4(r.v[j], own[j], r.dec[j], e);
-        i++;
-        e += r.deg[j];
-      }
+      if ((acc >> j) & 1u) mv[i++] = make_int4(r.v[j], own[j], r.dec[j], r.w[j]);
     }
-  } else if (!DENSE) {
+  } else if (!WEIGHTS) {
     int i = base + ex, e = ebase + exd;
     for (unsigned m = acc; m; m &= m - 1) {
       const int s = first + __builtin_ctz(m);
@@ -190,16 +182,16 @@ inline fp_status ensure_compact_tiles(fp_ctx* ctx, int R) {
   return FP_OK;
 }
 
-template <bool DENSE>
+template <bool WEIGHTS>
 inline void launch_round_compact(fp_ctx* ctx, const int* order, int r0, int nslots, int* slot_mi, int4* mv,
                                  int* mstat) {
   const int nb = std::max(1, (nslots + COMPACT_TILE - 1) / COMPACT_TILE);
   if (nb > 1)
-    k_compact_count<DENSE><<<nb, COMPACT_BLOCK, 0, ctx->stream>>>(ctx->slot_dec, order, ctx->vtx_ptr, r0, nslots,
-                                                                  ctx->compact_tiles);
-  k_round_compact<DENSE><<<nb, COMPACT_BLOCK, 0, ctx->stream>>>(ctx->slot_dec, ctx->slot_own, order, ctx->vtx_ptr,
-                                                                r0, nslots, nb > 1 ? ctx->compact_tiles : nullptr,
-                                                                slot_mi, mv, mstat);
+    k_compact_count<WEIGHTS><<<nb, COMPACT_BLOCK, 0, ctx->stream>>>(ctx->slot_dec, order, ctx->vtx_ptr, r0, nslots,
+                                                                    ctx->compact_tiles);
+  k_round_compact<WEIGHTS><<<nb, COMPACT_BLOCK, 0, ctx->stream>>>(
+      ctx->slot_dec, ctx->slot_own, order, ctx->vtx_ptr, ctx->vwgt, r0, nslots, nb > 1 ? ctx->compact_tiles : nullptr,
+      slot_mi, mv, mstat);
 }
 
 // Max-set tracking by one wavefront over val[0..K) in LDS (K <= 64 * 64).
