@@ -239,6 +239,66 @@ __device__ __forceinline__ void wave_list_move(unsigned* list, int& L, int p, in
   wave_lds_sync();
 }
 
+// A net's list in LDS with a part -> position index (one wavefront): at[p]
+// is the position of part p's entry, or LIST_NONE. Between nets every slot of
+// at holds LIST_NONE: wave_index_set fills the slots of a loaded list and
+// wave_index_clear empties those of the final one.
+constexpr unsigned short LIST_NONE = 0xFFFF;
+
+__device__ __forceinline__ void wave_index_init(unsigned short* at, int K) {
+  for (int q = lane_id(); q < K; q += WAVE) at[q] = LIST_NONE;
+  wave_lds_sync();
+}
+
+__device__ __forceinline__ void wave_index_set(const unsigned* list, int L, unsigned short* at, unsigned pmask) {
+  for (int t = lane_id(); t < L; t += WAVE) at[list[t] & pmask] = (unsigned short)t;
+  wave_lds_sync();
+}
+
+__device__ __forceinline__ void wave_index_clear(const unsigned* list, int L, unsigned short* at, unsigned pmask) {
+  for (int t = lane_id(); t < L; t += WAVE) at[list[t] & pmask] = LIST_NONE;
+  wave_lds_sync();
+}
+
+// wave_list_move through the index (same edits, same list order): two
+// dependent LDS reads per move instead of scans of the list. Returns in co
+// the pin count of p's entry before the move (0 without one) and in found
+// whether q had an entry.
+__device__ __forceinline__ void wave_ilist_move(unsigned* list, unsigned short* at, int& L, int p, int q, int pb,
+                                                unsigned pmask, unsigned& co, bool& found) {
+  const int lane = lane_id();
+  const unsigned inc = 1u << pb;
+  const int jp = __builtin_amdgcn_readfirstlane((int)at[p]);
+  int jq = __builtin_amdgcn_readfirstlane((int)at[q]);
+  const bool hasp = jp != (int)LIST_NONE;
+  const unsigned x = (unsigned)__builtin_amdgcn_readfirstlane((int)list[hasp ? jp : 0]);
+  const unsigned y = (unsigned)__builtin_amdgcn_readfirstlane((int)list[L > 0 ? L - 1 : 0]);
+  co = hasp ? x >> pb : 0u;
+  found = jq != (int)LIST_NONE;
+  if (co > 1u) {
+    if (lane == 0) list[jp] = x - inc;
+  } else if (hasp) {
+    // the last entry takes p's slot
+    if (lane == 0) {
+      list[jp] = y;
+      at[y & pmask] = (unsigned short)jp;
+      at[p] = LIST_NONE;
+    }
+    L--;
+    if (jq == L) jq = jp;
+  }
+  if (found) {
+    if (lane == 0) atomicAdd(&list[jq], inc);
+  } else {
+    if (lane == 0) {
+      list[L] = inc | (unsigned)q;
+      at[q] = (unsigned short)L;
+    }
+    L++;
+  }
+  wave_lds_sync();
+}
+
 // Stages net n's list (segment at seg = net_ptr[n] + n) into LDS; returns lambda.
 __device__ __forceinline__ int wave_list_load(const unsigned* ent, unsigned seg, unsigned np, unsigned* list) {
   // the segment holds the length and then np >= length entries: one load

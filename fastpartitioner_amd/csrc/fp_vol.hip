@@ -342,6 +342,9 @@ struct VolReplayArgs {
 //   vertex->nets list); the list edits stay in LDS
 // commit (commit = 1): list edits of the moves before the stop; blocks past
 //   net_blocks commit part vector and both part sizes
+// INDEXED: each wave keeps a part -> position index of its list in LDS
+//   (2 bytes per part after the lists), for K where it fits
+template <bool INDEXED>
 __global__ void __launch_bounds__(REPLAY_BLOCK) k_vol_replay(VolReplayArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
@@ -361,6 +364,9 @@ __global__ void __launch_bounds__(REPLAY_BLOCK) k_vol_replay(VolReplayArgs a) {
     return;
   }
   unsigned* list = reinterpret_cast<unsigned*>(lds) + (size_t)wave * (size_t)a.K;
+  unsigned short* at = reinterpret_cast<unsigned short*>(reinterpret_cast<unsigned*>(lds) + (size_t)nw * (size_t)a.K) +
+                       (size_t)wave * (size_t)a.K;
+  if (INDEXED) wave_index_init(at, a.K);
   const int ntouched = *a.tcount;
   for (int i = (int)blockIdx.x * nw + wave; i < ntouched; i += a.net_blocks * nw) {
     const int n = a.touched[i];
@@ -369,6 +375,7 @@ __global__ void __launch_bounds__(REPLAY_BLOCK) k_vol_replay(VolReplayArgs a) {
     unsigned lo, hi;
     wave_rank_window(a.apply_pins, s, e, a.r0, a.r1, lo, hi);
     int L = wave_list_load(a.ent, seg, e - s, list);
+    if (INDEXED) wave_index_set(list, L, at, a.pmask);
     const int srcv = a.src[n];
     int sp_cur = a.commit ? 0 : a.part[srcv];  // the source's part as the replay goes
     bool dirty = false;
@@ -398,11 +405,18 @@ __global__ void __launch_bounds__(REPLAY_BLOCK) k_vol_replay(VolReplayArgs a) {
                   d = __builtin_amdgcn_readlane(mm.z, l);
         if (!a.commit) {
           const bool isrc = srcv == v;
-          const int jo = wave_find(list, L, own, a.pmask);
-          const unsigned co = jo >= 0 ? (list[jo] >> a.pb) : 0u;
-          const bool found = wave_find(list, L, d, a.pmask) >= 0;
           const int lam_b = L;
-          const int lam_mid = co == 1u ? L - 1 : L;
+          unsigned co;
+          bool found;
+          if (INDEXED) {
+            wave_ilist_move(list, at, L, own, d, a.pb, a.pmask, co, found);
+          } else {
+            const int jo = wave_find(list, L, own, a.pmask);
+            co = jo >= 0 ? (list[jo] >> a.pb) : 0u;
+            found = wave_find(list, L, d, a.pmask) >= 0;
+            wave_list_move(list, L, own, d, a.pb, a.pmask);
+          }
+          const int lam_mid = co == 1u ? lam_b - 1 : lam_b;
           int e_own = 0, e_dec = 0, e_sp = 0, spx = -1;
           if (isrc) e_own -= lam_b - 1;                       // send_volumes[part_id] -= end_cnt - 1
           if (co == 1u) {
@@ -428,12 +442,18 @@ __global__ void __launch_bounds__(REPLAY_BLOCK) k_vol_replay(VolReplayArgs a) {
             a.evbuf[(unsigned)__builtin_amdgcn_readlane((int)evslot, l)] =
                 make_int4(vol_event_word(spx, e_sp), e_own, e_dec, __builtin_amdgcn_readlane(mi, l));
           if (isrc) sp_cur = d;
+        } else if (INDEXED) {
+          unsigned co;
+          bool found;
+          wave_ilist_move(list, at, L, own, d, a.pb, a.pmask, co, found);
+        } else {
+          wave_list_move(list, L, own, d, a.pb, a.pmask);
         }
-        wave_list_move(list, L, own, d, a.pb, a.pmask);
         dirty = true;
       }
     }
     if (a.commit && dirty) wave_list_store(a.ent, a.ev, seg, list, L, a.pb, a.pmask);
+    if (INDEXED) wave_index_clear(list, L, at, a.pmask);
     wave_lds_sync();
   }
 }
@@ -757,7 +777,12 @@ fp_status pass_vol(fp_ctx* ctx, const RefinePlan& p) {
   const bool msrv = p.metric == FP_METRIC_MSRV;
   const int nov = ctx->nov, K = ctx->K, R = p.R, waves = p.eval_waves;
   const size_t lds_e = (size_t)K * 16 + (size_t)waves * 8 * (size_t)K;
-  const size_t lds_r = (size_t)(REPLAY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
+  // the replay's part -> position index where it fits the LDS budget
+  const bool indexed = (size_t)(REPLAY_BLOCK / WAVE) * (size_t)K * (sizeof(unsigned) + sizeof(unsigned short)) <=
+                       (size_t)LDS_BUDGET;
+  const size_t lds_r = (size_t)(REPLAY_BLOCK / WAVE) * (size_t)K *
+                       (sizeof(unsigned) + (indexed ? sizeof(unsigned short) : 0));
+  auto replay = indexed ? k_vol_replay<true> : k_vol_replay<false>;
   const int batched = K <= VOL_BATCH_KMAX ? 1 : 0;
   const size_t lds_vol = ((size_t)K + WAVE + (batched ? vol_batch_ints(K) : 0)) * sizeof(int);
   const int stage_cap = (int)std::min<size_t>(4096, ((size_t)LDS_BUDGET - lds_vol) / sizeof(int4));
@@ -823,11 +848,10 @@ fp_status pass_vol(fp_ctx* ctx, const RefinePlan& p) {
     ra.msrv = msrv ? 1 : 0;
     ra.net_blocks = net_grid(ctx, REPLAY_NET_BLOCKS, REPLAY_BLOCK / WAVE);
     ra.commit = 0;
-    k_vol_replay<<<ra.net_blocks, REPLAY_BLOCK, lds_r, ctx->stream>>>(ra);
+    replay<<<ra.net_blocks, REPLAY_BLOCK, lds_r, ctx->stream>>>(ra);
     k_vol_scan<<<1, WAVE, lds_s, ctx->stream>>>(vs.vol, K, vs.mv, vs.evbuf, vs.mstat, stage_cap, batched);
     ra.commit = 1;
-    k_vol_replay<<<ra.net_blocks + (nslots + REPLAY_BLOCK - 1) / REPLAY_BLOCK, REPLAY_BLOCK, lds_r,
-                   ctx->stream>>>(ra);
+    replay<<<ra.net_blocks + (nslots + REPLAY_BLOCK - 1) / REPLAY_BLOCK, REPLAY_BLOCK, lds_r, ctx->stream>>>(ra);
   }
   return FP_OK;
 }
