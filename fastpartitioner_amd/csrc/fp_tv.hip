@@ -42,6 +42,7 @@ struct EvalArgs {
   const unsigned* vtx_nets;
   const unsigned* vtx_seg;
   const unsigned short* ev;
+  unsigned pad_seg;  // the 64 zero words after the last segment: a list of length 0
   const int* part;
   const int* order;  // nullable: identity
   const int* vwgt;   // nullable: unit
@@ -134,7 +135,7 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_tv_eval(EvalArgs a) {
     }
   };
   auto load_segs = [&](int k) -> unsigned {
-    return lane < min(hdeg[k], WAVE) ? a.vtx_seg[hvb[k] + (unsigned)lane] : 0u;
+    return lane < min(hdeg[k], WAVE) ? a.vtx_seg[hvb[k] + (unsigned)lane] : a.pad_seg;
   };
   load_vertex(0, slot);
   load_vertex(1, slot + stride);
@@ -151,24 +152,26 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_tv_eval(EvalArgs a) {
     // the order in which the reference's scan meets the parts
     // (TV/partition.cpp:318-338). One 64-word read of a segment brings lambda
     // (lane 0) and the first 63 entries (lanes 1..63); NETS_IN_FLIGHT such
-    // reads are issued together, longer lists are finished after.
-    int lg = 0;
+    // reads are issued together, longer lists are finished after. Lanes past
+    // the vertex's nets hold the zero padding's offset, so the windows past
+    // the last net see lambda = 0. The leave-gain count is kept per lane.
+    int lgl = 0;
     for (int c0 = 0; c0 < deg; c0 += WAVE) {
       const int nn = min(WAVE, deg - c0);
-      const unsigned sg = c0 == 0 ? sg0 : (lane < nn ? a.vtx_seg[vb + (unsigned)(c0 + lane)] : 0u);
-      unsigned long long longmask = 0;  // nets whose list has more than 63 entries
+      const unsigned sg = c0 == 0 ? sg0 : (lane < nn ? a.vtx_seg[vb + (unsigned)(c0 + lane)] : a.pad_seg);
+      int lmax = 0;  // longest list of the chunk
       for (int j0 = 0; j0 < nn; j0 += NETS_IN_FLIGHT) {
         unsigned xv[NETS_IN_FLIGHT];
 #pragma unroll
         for (int u = 0; u < NETS_IN_FLIGHT; u++) {
-          const unsigned sj = (unsigned)__builtin_amdgcn_readlane((int)sg, min(j0 + u, nn - 1));
+          const unsigned sj = (unsigned)__builtin_amdgcn_readlane((int)sg, j0 + u);
           xv[u] = a.ev[sj + (unsigned)lane];
         }
 #pragma unroll
         for (int u = 0; u < NETS_IN_FLIGHT; u++) {
           const int j = j0 + u;
-          const int Lj = __builtin_amdgcn_readlane((int)xv[u], 0) & (j < nn ? -1 : 0);
-          if (Lj > WAVE - 1) longmask |= 1ull << j;
+          const int Lj = __builtin_amdgcn_readlane((int)xv[u], 0);
+          lmax = max(lmax, Lj);
           const int t = lane - 1;
           const bool ok = lane >= 1 && t < Lj;
           const unsigned x = xv[u];
@@ -177,10 +180,13 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_tv_eval(EvalArgs a) {
           atomicAdd(&cnt[qs], 1u);
           atomicMin(&first[qs], ((unsigned)(c0 + j) << a.pb) | (unsigned)t);
           // leave gain: nets where v is the only pin of its part (TV/partition.cpp:365-368)
-          lg += __popcll(__ballot(ok && q == own && (x & EV_SINGLE)));
+          lgl += ok && q == own && (x & EV_SINGLE) ? 1 : 0;
         }
       }
-      // the rest of the long lists: entries 63.. in windows of 64
+      // the rest of the long lists (more than 63 entries): entries 63.. in
+      // windows of 64; a segment's first word is its lambda
+      unsigned long long longmask = 0;
+      if (lmax > WAVE - 1) longmask = __ballot((int)a.ev[sg] > WAVE - 1);
       while (longmask) {
         const int j = __ffsll((long long)longmask) - 1;
         longmask &= longmask - 1;
@@ -202,11 +208,12 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_tv_eval(EvalArgs a) {
             const int qs = ok ? q : spare;
             atomicAdd(&cnt[qs], 1u);
             atomicMin(&first[qs], ((unsigned)(c0 + j) << a.pb) | (unsigned)t);
-            lg += __popcll(__ballot(ok && q == own && (x & EV_SINGLE)));
+            lgl += ok && q == own && (x & EV_SINGLE) ? 1 : 0;
           }
         }
       }
     }
+    const int lg = wave_sum(lgl);
     wave_lds_sync();
     // candidates: lexicographic max of (score, -size, -first rank) over score >
     // 0 equals the sequential LDG2 scan over the active parts
@@ -515,6 +522,7 @@ fp_status check_envelope(fp_ctx* ctx) {
 
 EvalArgs make_eval_args(fp_ctx* ctx, const fp_schedule* s) {
   EvalArgs a{};
+  a.pad_seg = (unsigned)((size_t)ctx->pins + (size_t)ctx->noe);
   a.vtx_ptr = ctx->vtx_ptr;
   a.vtx_nets = ctx->vtx_nets;
   a.vtx_seg = ctx->vtx_seg;
