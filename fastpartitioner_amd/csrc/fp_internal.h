@@ -77,7 +77,9 @@ __device__ __forceinline__ int dpp_i32(int old, int v) {
 // Wave-wide sum and max of an int through DPP: quad_perm [1,0,3,2] and
 // [2,3,0,1], row_ror 4 and 8 (row-wide), row_bcast15 into rows 1 and 3,
 // row_bcast31 into rows 2 and 3; lane 63 then holds the result, which is
-// returned to every lane.
+// returned to every lane. The value a step moves into lanes it does not
+// reach is the operation's identity (0, INT_MIN), which lets the compiler
+// fold each step into one DPP-modified add / max.
 __device__ __forceinline__ int wave_sum(int x) {
   x += dpp_i32<0xB1>(0, x);
   x += dpp_i32<0x4E>(0, x);
@@ -89,13 +91,34 @@ __device__ __forceinline__ int wave_sum(int x) {
 }
 
 __device__ __forceinline__ int wave_max(int x) {
-  x = max(x, dpp_i32<0xB1>(x, x));
-  x = max(x, dpp_i32<0x4E>(x, x));
-  x = max(x, dpp_i32<0x124>(x, x));
-  x = max(x, dpp_i32<0x128>(x, x));
-  x = max(x, dpp_i32<0x142, 0xA>(x, x));
-  x = max(x, dpp_i32<0x143, 0xC>(x, x));
+  x = max(x, dpp_i32<0xB1>(INT_MIN, x));
+  x = max(x, dpp_i32<0x4E>(INT_MIN, x));
+  x = max(x, dpp_i32<0x124>(INT_MIN, x));
+  x = max(x, dpp_i32<0x128>(INT_MIN, x));
+  x = max(x, dpp_i32<0x142, 0xA>(INT_MIN, x));
+  x = max(x, dpp_i32<0x143, 0xC>(INT_MIN, x));
   return __builtin_amdgcn_readlane(x, WAVE - 1);
+}
+
+// The same for unsigned max (identity 0) and min (identity ~0).
+__device__ __forceinline__ unsigned wave_max_u32(unsigned x) {
+  x = max(x, (unsigned)dpp_i32<0xB1>(0, (int)x));
+  x = max(x, (unsigned)dpp_i32<0x4E>(0, (int)x));
+  x = max(x, (unsigned)dpp_i32<0x124>(0, (int)x));
+  x = max(x, (unsigned)dpp_i32<0x128>(0, (int)x));
+  x = max(x, (unsigned)dpp_i32<0x142, 0xA>(0, (int)x));
+  x = max(x, (unsigned)dpp_i32<0x143, 0xC>(0, (int)x));
+  return (unsigned)__builtin_amdgcn_readlane((int)x, WAVE - 1);
+}
+
+__device__ __forceinline__ unsigned wave_min_u32(unsigned x) {
+  x = min(x, (unsigned)dpp_i32<0xB1>(-1, (int)x));
+  x = min(x, (unsigned)dpp_i32<0x4E>(-1, (int)x));
+  x = min(x, (unsigned)dpp_i32<0x124>(-1, (int)x));
+  x = min(x, (unsigned)dpp_i32<0x128>(-1, (int)x));
+  x = min(x, (unsigned)dpp_i32<0x142, 0xA>(-1, (int)x));
+  x = min(x, (unsigned)dpp_i32<0x143, 0xC>(-1, (int)x));
+  return (unsigned)__builtin_amdgcn_readlane((int)x, WAVE - 1);
 }
 
 // Inclusive prefix sum over the wave through DPP: row_shr 1, 2, 4, 8 (lanes
@@ -141,43 +164,20 @@ __device__ __forceinline__ T block_excl_scan(T x, T* scratch, T* total) {
   return base + incl - x;
 }
 
-// 64-bit lane shuffle by a DPP pattern (no LDS traffic). Rows not in ROW_MASK
-// keep their own value.
-template <int CTRL, int ROW_MASK = 0xF>
-__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v) {
-  const int lo = (int)(unsigned)(v & 0xFFFFFFFFull), hi = (int)(unsigned)(v >> 32);
-  const int rlo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xF, false);
-  const int rhi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xF, false);
-  return ((unsigned long long)(unsigned)rhi << 32) | (unsigned)rlo;
-}
-
-__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long v, int l) {
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v & 0xFFFFFFFFull), l);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), l);
-  return ((unsigned long long)hi << 32) | lo;
-}
-
-// Wave-wide max / min of a 64-bit key through DPP: quad_perm [1,0,3,2] and
-// [2,3,0,1], row_ror 4 and 8 (row-wide), row_bcast15 into rows 1 and 3,
-// row_bcast31 into rows 2 and 3; lane 63 then holds the result.
+// Wave-wide max / min of a 64-bit key: the high words first, then the low
+// words of the lanes holding the extreme high word (two 32-bit reductions).
 __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long x) {
-  x = max(x, dpp_u64<0xB1>(x));
-  x = max(x, dpp_u64<0x4E>(x));
-  x = max(x, dpp_u64<0x124>(x));
-  x = max(x, dpp_u64<0x128>(x));
-  x = max(x, dpp_u64<0x142, 0xA>(x));
-  x = max(x, dpp_u64<0x143, 0xC>(x));
-  return readlane_u64(x, WAVE - 1);
+  const unsigned hi = (unsigned)(x >> 32), lo = (unsigned)x;
+  const unsigned mh = wave_max_u32(hi);
+  const unsigned ml = wave_max_u32(hi == mh ? lo : 0u);
+  return ((unsigned long long)mh << 32) | ml;
 }
 
 __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long x) {
-  x = min(x, dpp_u64<0xB1>(x));
-  x = min(x, dpp_u64<0x4E>(x));
-  x = min(x, dpp_u64<0x124>(x));
-  x = min(x, dpp_u64<0x128>(x));
-  x = min(x, dpp_u64<0x142, 0xA>(x));
-  x = min(x, dpp_u64<0x143, 0xC>(x));
-  return readlane_u64(x, WAVE - 1);
+  const unsigned hi = (unsigned)(x >> 32), lo = (unsigned)x;
+  const unsigned mh = wave_min_u32(hi);
+  const unsigned ml = wave_min_u32(hi == mh ? lo : ~0u);
+  return ((unsigned long long)mh << 32) | ml;
 }
 
 // first index k in [lo, hi) with arr[k] >= key (arr ascending); wave-uniform result
