@@ -160,13 +160,20 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_tv_eval(EvalArgs a) {
       const int nn = min(WAVE, deg - c0);
       const unsigned sg = c0 == 0 ? sg0 : (lane < nn ? a.vtx_seg[vb + (unsigned)(c0 + lane)] : a.pad_seg);
       int lmax = 0;  // longest list of the chunk
-      for (int j0 = 0; j0 < nn; j0 += NETS_IN_FLIGHT) {
-        unsigned xv[NETS_IN_FLIGHT];
+      // the next group's windows are in flight while a group is counted
+      unsigned xv[NETS_IN_FLIGHT];
+      auto load_group = [&](int j0, unsigned (&dst)[NETS_IN_FLIGHT]) {
 #pragma unroll
         for (int u = 0; u < NETS_IN_FLIGHT; u++) {
           const unsigned sj = (unsigned)__builtin_amdgcn_readlane((int)sg, j0 + u);
-          xv[u] = a.ev[sj + (unsigned)lane];
+          dst[u] = a.ev[sj + (unsigned)lane];
         }
+      };
+      load_group(0, xv);
+      for (int j0 = 0; j0 < nn; j0 += NETS_IN_FLIGHT) {
+        unsigned xn[NETS_IN_FLIGHT];
+        const bool more = j0 + NETS_IN_FLIGHT < nn;
+        if (more) load_group(j0 + NETS_IN_FLIGHT, xn);
 #pragma unroll
         for (int u = 0; u < NETS_IN_FLIGHT; u++) {
           const int j = j0 + u;
@@ -181,6 +188,10 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_tv_eval(EvalArgs a) {
           atomicMin(&first[qs], ((unsigned)(c0 + j) << a.pb) | (unsigned)t);
           // leave gain: nets where v is the only pin of its part (TV/partition.cpp:365-368)
           lgl += ok && q == own && (x & EV_SINGLE) ? 1 : 0;
+        }
+        if (more) {
+#pragma unroll
+          for (int u = 0; u < NETS_IN_FLIGHT; u++) xv[u] = xn[u];
         }
       }
       // the rest of the long lists (more than 63 entries): entries 63.. in
