@@ -431,6 +431,9 @@ struct ApplyArgs {
 // increment or append); one wavefront per net, the list staged in LDS.
 // Blocks past net_blocks commit the REF part vector and source-part sizes
 // (TV/partition.cpp:818-830: the destination's size is left unchanged).
+// INDEXED: each wave keeps a part -> position index of its list in LDS (2
+// bytes per part after the lists), for K where it fits.
+template <bool INDEXED>
 __global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
@@ -451,6 +454,9 @@ __global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
     return;
   }
   unsigned* list = reinterpret_cast<unsigned*>(lds) + (size_t)wave * (size_t)a.K;
+  unsigned short* at = reinterpret_cast<unsigned short*>(reinterpret_cast<unsigned*>(lds) + (size_t)nw * (size_t)a.K) +
+                       (size_t)wave * (size_t)a.K;
+  if (INDEXED) wave_index_init(at, a.K);
   const int ntouched = *a.tcount;
   for (int i = (int)blockIdx.x * nw + wave; i < ntouched; i += a.net_blocks * nw) {
     const int n = a.touched[i];
@@ -459,6 +465,7 @@ __global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
     unsigned lo, hi;
     wave_rank_window(a.apply_pins, s, e, a.r0, a.r1, lo, hi);
     int L = wave_list_load(a.ent, seg, e - s, list);
+    if (INDEXED) wave_index_set(list, L, at, a.pmask);
     bool dirty = false;
     for (unsigned k0 = lo; k0 < hi; k0 += WAVE) {
       const unsigned k = k0 + (unsigned)lane;
@@ -473,12 +480,30 @@ __global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
         dirty = true;
         const int l = __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1);
         m &= m - 1;
-        wave_list_move(list, L, __builtin_amdgcn_readlane(oo, l), __builtin_amdgcn_readlane(dd, l), a.pb, a.pmask);
+        const int o = __builtin_amdgcn_readlane(oo, l), d = __builtin_amdgcn_readlane(dd, l);
+        if (INDEXED) {
+          unsigned co;
+          bool found;
+          wave_ilist_move(list, at, L, o, d, a.pb, a.pmask, co, found);
+        } else {
+          wave_list_move(list, L, o, d, a.pb, a.pmask);
+        }
       }
     }
     if (dirty) wave_list_store(a.ent, a.ev, seg, list, L, a.pb, a.pmask);
+    if (INDEXED) wave_index_clear(list, L, at, a.pmask);
     wave_lds_sync();
   }
+}
+
+// the apply's LDS: a K-entry list per wave, and the part -> position index
+// where it fits the budget
+bool apply_indexed(int K) {
+  return (size_t)(APPLY_BLOCK / WAVE) * (size_t)K * (sizeof(unsigned) + sizeof(unsigned short)) <= (size_t)LDS_BUDGET;
+}
+
+size_t apply_lds(int K) {
+  return (size_t)(APPLY_BLOCK / WAVE) * (size_t)K * (sizeof(unsigned) + (apply_indexed(K) ? sizeof(unsigned short) : 0));
 }
 
 size_t eval_lds(int K, int waves) { return (size_t)K * 16 + (size_t)waves * 8 * (size_t)(K + WAVE); }
@@ -560,8 +585,7 @@ fp_status plan_tv(fp_ctx* ctx, RefinePlan& p) {
   const int waves = p.eval_waves > 0 ? std::min(p.eval_waves, eval_waves(K)) : eval_waves(K);
   if (waves == 0) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d needs more LDS than a workgroup has", K);
   p.eval_waves = waves;
-  const size_t lds_a = (size_t)(APPLY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
-  if (lds_a > (size_t)LDS_BUDGET) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d too large for the apply kernel", K);
+  if (apply_lds(K) > (size_t)LDS_BUDGET) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d too large for the apply kernel", K);
   p.R = std::min(p.sched.round, ctx->nov);
   FP_TRY(ensure_round_scratch(ctx, p.R));
   if (p.sched.kind == FP_SCHED_BAL) FP_TRY(ensure_bal_scratch(ctx, p.R));
@@ -573,7 +597,8 @@ fp_status pass_tv(fp_ctx* ctx, const RefinePlan& p) {
   const fp_schedule* sched = &p.sched;
   const bool bal = sched->kind == FP_SCHED_BAL;
   const size_t lds_e = eval_lds(K, waves);
-  const size_t lds_a = (size_t)(APPLY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
+  const size_t lds_a = apply_lds(K);
+  auto apply = apply_indexed(K) ? k_apply<true> : k_apply<false>;
   const size_t lds_s = (size_t)K * sizeof(int);
   // every pass restarts its round tags: clear the stamps and the touched counters
   pass_begin(ctx, nullptr, 0, nullptr);
@@ -640,7 +665,7 @@ fp_status pass_tv(fp_ctx* ctx, const RefinePlan& p) {
       ap.moves = ctx->moves;
       grid_a += (nslots + APPLY_BLOCK - 1) / APPLY_BLOCK;
     }
-    k_apply<<<grid_a, APPLY_BLOCK, lds_a, ctx->stream>>>(ap);
+    apply<<<grid_a, APPLY_BLOCK, lds_a, ctx->stream>>>(ap);
   }
   return FP_OK;
 }
