@@ -278,11 +278,16 @@ __device__ void wave_bitonic_sort(unsigned* keys, int P) {
   }
 }
 
+// INDEXED: a part -> position index of the list in LDS (2 bytes per part
+// after the keys), for K where it fits.
+template <bool INDEXED>
 __global__ void __launch_bounds__(WAVE) k_cn_apply(CnApplyArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   unsigned* list = reinterpret_cast<unsigned*>(lds);
   unsigned* keys = list + a.K;
+  unsigned short* at = reinterpret_cast<unsigned short*>(keys + CN_MAXOPS);
   const int lane = lane_id();
+  if (INDEXED) wave_index_init(at, a.K);
   const int nt = *a.tcount;
   const bool skip = (a.flags[0] | a.flags[1]) != 0;
   for (int i = (int)blockIdx.x; i < nt; i += (int)gridDim.x) {
@@ -300,14 +305,44 @@ __global__ void __launch_bounds__(WAVE) k_cn_apply(CnApplyArgs a) {
     if (c > 1) wave_bitonic_sort(keys, P);
     const unsigned s0 = a.net_ptr[t], seg = s0 + (unsigned)t;
     int L = wave_list_load(a.ent, seg, a.net_ptr[t + 1] - s0, list);
-    for (int j = 0; j < c; j++) {
-      const int m = (int)keys[j];
-      wave_list_move(list, L, a.mv_from[m], a.cn_new[a.mv_net[m]], a.pb, a.pmask);
+    if (INDEXED) wave_index_set(list, L, at, a.pmask);
+    for (int j0 = 0; j0 < c; j0 += WAVE) {
+      // 64 moves at a time: each lane loads one move's old and new part
+      int from_l = 0, to_l = 0;
+      if (j0 + lane < c) {
+        const int m = (int)keys[j0 + lane];
+        from_l = a.mv_from[m];
+        to_l = a.cn_new[a.mv_net[m]];
+      }
+      const int nj = min(WAVE, c - j0);
+      for (int l = 0; l < nj; l++) {
+        const int p = __builtin_amdgcn_readlane(from_l, l), q = __builtin_amdgcn_readlane(to_l, l);
+        if (INDEXED) {
+          unsigned co;
+          bool found;
+          wave_ilist_move(list, at, L, p, q, a.pb, a.pmask, co, found);
+        } else {
+          wave_list_move(list, L, p, q, a.pb, a.pmask);
+        }
+      }
     }
     wave_list_store(a.ent, a.ev, seg, list, L, a.pb, a.pmask);
+    if (INDEXED) wave_index_clear(list, L, at, a.pmask);
     if (lane == 0) a.opcnt_reset[t] = 0;
     wave_lds_sync();
   }
+}
+
+// the CN apply's LDS: list, sort keys, and the part -> position index where
+// it fits the budget
+bool cn_apply_indexed(int K) {
+  return (size_t)K * (sizeof(unsigned) + sizeof(unsigned short)) + (size_t)CN_MAXOPS * sizeof(unsigned) <=
+         (size_t)LDS_BUDGET;
+}
+
+size_t cn_apply_lds(int K) {
+  return (size_t)K * sizeof(unsigned) + (size_t)CN_MAXOPS * sizeof(unsigned) +
+         (cn_apply_indexed(K) ? (size_t)K * sizeof(unsigned short) : 0);
 }
 
 __global__ void k_cn_commit(const int* __restrict__ mv_v, const int* __restrict__ mv_net,
@@ -335,8 +370,8 @@ fp_status plan_cn(fp_ctx* ctx, RefinePlan& p) {
   const long long pins = ctx->pins;
   const int nblk = (noe + SCAN_BLOCK - 1) / SCAN_BLOCK;
   auto& s = ctx->cn;
-  const size_t lds_apply = (size_t)ctx->K * sizeof(unsigned) + (size_t)CN_MAXOPS * sizeof(unsigned);
-  if (lds_apply > (size_t)LDS_BUDGET) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d too large for the CN apply", ctx->K);
+  if (cn_apply_lds(ctx->K) > (size_t)LDS_BUDGET)
+    return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d too large for the CN apply", ctx->K);
   if (!s.ready) {
     const size_t ne = (size_t)std::max(noe, 1), np = (size_t)std::max(pins, 1ll);
     FP_TRY(dalloc(ctx, &s.old_part, ne));
@@ -370,7 +405,8 @@ fp_status pass_cn(fp_ctx* ctx, const RefinePlan&) {
   const int noe = ctx->noe;
   const int nblk = (noe + SCAN_BLOCK - 1) / SCAN_BLOCK;
   auto& s = ctx->cn;
-  const size_t lds_apply = (size_t)ctx->K * sizeof(unsigned) + (size_t)CN_MAXOPS * sizeof(unsigned);
+  const size_t lds_apply = cn_apply_lds(ctx->K);
+  auto cn_apply = cn_apply_indexed(ctx->K) ? k_cn_apply<true> : k_cn_apply<false>;
   const unsigned long long* nm = s.totals + 3;
   // one round per pass: tag 1 after clearing the stamps
   pass_begin(ctx, nullptr, 0, nullptr);
@@ -396,7 +432,7 @@ fp_status pass_cn(fp_ctx* ctx, const RefinePlan&) {
                                             s.ops);
   CnApplyArgs aa{ctx->net_ptr, ctx->touched, ctx->counters + par, s.opcnt, s.opoff, s.ops, s.mv_from, s.mv_net,
                  s.new_part, ctx->ent, ctx->ev, ctx->K, ctx->pb, ctx->pmask, s.opcnt, ctx->flags};
-  k_cn_apply<<<8192, WAVE, lds_apply, ctx->stream>>>(aa);
+  cn_apply<<<8192, WAVE, lds_apply, ctx->stream>>>(aa);
   k_cn_commit<<<gm, 256, 0, ctx->stream>>>(s.mv_v, s.mv_net, s.mv_from, s.mv_last, nm, ctx->flags, s.new_part,
                                             ctx->vwgt, ctx->part, ctx->sizes, ctx->moves);
   return FP_OK;

@@ -275,6 +275,7 @@ __device__ __forceinline__ void wave_ilist_move(unsigned* list, unsigned short* 
   const unsigned y = (unsigned)__builtin_amdgcn_readfirstlane((int)list[L > 0 ? L - 1 : 0]);
   co = hasp ? x >> pb : 0u;
   found = jq != (int)LIST_NONE;
+  bool qin = found;  // q has an entry after the removal
   if (co > 1u) {
     if (lane == 0) list[jp] = x - inc;
   } else if (hasp) {
@@ -285,9 +286,10 @@ __device__ __forceinline__ void wave_ilist_move(unsigned* list, unsigned short* 
       at[p] = LIST_NONE;
     }
     L--;
-    if (jq == L) jq = jp;
+    if (q == p) qin = false;       // q's entry was the one removed
+    else if (jq == L) jq = jp;     // q's entry was the last one: it moved into p's slot
   }
-  if (found) {
+  if (qin) {
     if (lane == 0) atomicAdd(&list[jq], inc);
   } else {
     if (lane == 0) {
