@@ -61,6 +61,13 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A load at a wave-uniform address, through the scalar cache: for arrays the
+// kernel only reads (the scalar cache starts every launch empty).
+template <typename T>
+__device__ __forceinline__ T uniform_load(const T* p) {
+  return *(const __attribute__((address_space(4))) T*)p;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T x) {
   for (int off = WAVE / 2; off > 0; off >>= 1) x += __shfl_xor(x, off);

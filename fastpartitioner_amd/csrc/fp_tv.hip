@@ -73,7 +73,10 @@ __device__ __forceinline__ bool better(double sc, long long sz, unsigned fi, dou
 }
 
 // Per vertex: sweep, LDG2 decision, TV gain test (TV/partition.cpp:732-805).
-__global__ void __launch_bounds__(EVAL_BLOCK) k_tv_eval(EvalArgs a) {
+// Eight waves per SIMD (the hardware's maximum) hide more of the dependent
+// list reads than the registers a freer allocation would save: the compiler
+// keeps within 64 VGPRs by parking cold scalar values in VGPR lanes.
+__global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) k_tv_eval(EvalArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int K = a.K;
   double* s_pen = reinterpret_cast<double*>(lds);
@@ -125,11 +128,11 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_tv_eval(EvalArgs a) {
   unsigned hvb[3];
   auto load_vertex = [&](int k, int sl) {
     if (sl < slot_end) {
-      const int vv = a.order ? a.order[a.r0 + sl] : a.r0 + sl;
+      const int vv = a.order ? uniform_load(a.order + (a.r0 + sl)) : a.r0 + sl;
       hv[k] = vv;
-      hown[k] = a.part[vv];
-      hvb[k] = a.vtx_ptr[vv];
-      hdeg[k] = (int)(a.vtx_ptr[vv + 1] - hvb[k]);
+      hown[k] = uniform_load(a.part + vv);
+      hvb[k] = uniform_load(a.vtx_ptr + vv);
+      hdeg[k] = (int)(uniform_load(a.vtx_ptr + vv + 1) - hvb[k]);
     } else {
       hv[k] = 0; hown[k] = 0; hvb[k] = 0; hdeg[k] = 0;
     }
@@ -137,9 +140,20 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_tv_eval(EvalArgs a) {
   auto load_segs = [&](int k) -> unsigned {
     return lane < min(hdeg[k], WAVE) ? a.vtx_seg[hvb[k] + (unsigned)lane] : a.pad_seg;
   };
+  // the first window group of the vertex about to be swept, loaded while the
+  // previous vertex picks its candidate
+  unsigned xpre[NETS_IN_FLIGHT];
+  auto load_first = [&](unsigned sgv) {
+#pragma unroll
+    for (int u = 0; u < NETS_IN_FLIGHT; u++) {
+      const unsigned sj = (unsigned)__builtin_amdgcn_readlane((int)sgv, u);
+      xpre[u] = a.ev[sj + (unsigned)lane];
+    }
+  };
   load_vertex(0, slot);
   load_vertex(1, slot + stride);
   unsigned sg0 = load_segs(0);
+  load_first(sg0);
   while (slot < slot_end) {
     load_vertex(2, slot + 2 * stride);
     const unsigned sg1 = load_segs(1);
@@ -169,7 +183,12 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_tv_eval(EvalArgs a) {
           dst[u] = a.ev[sj + (unsigned)lane];
         }
       };
-      load_group(0, xv);
+      if (c0 == 0) {
+#pragma unroll
+        for (int u = 0; u < NETS_IN_FLIGHT; u++) xv[u] = xpre[u];
+      } else {
+        load_group(0, xv);
+      }
       for (int j0 = 0; j0 < nn; j0 += NETS_IN_FLIGHT) {
         unsigned xn[NETS_IN_FLIGHT];
         const bool more = j0 + NETS_IN_FLIGHT < nn;
@@ -225,6 +244,7 @@ __global__ void __launch_bounds__(EVAL_BLOCK) k_tv_eval(EvalArgs a) {
       }
     }
     const int lg = wave_sum(lgl);
+    load_first(sg1);
     wave_lds_sync();
     // candidates: lexicographic max of (score, -size, -first rank) over score >
     // 0 equals the sequential LDG2 scan over the active parts
