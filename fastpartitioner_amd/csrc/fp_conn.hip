@@ -387,7 +387,8 @@ __global__ void __launch_bounds__(WAVE) k_tm_admit(int* __restrict__ slot_dec, c
   wave_lds_sync();
   const int nmv = mstat[2];
   unsigned long long nm = 0;
-  auto chunk = [&](int m0, int4 x) {
+  // decides one chunk in LDS and registers only; returns this lane's decision
+  auto chunk = [&](int m0, int4 x) -> bool {
     const bool valid = m0 + lane < nmv;
     const int v = x.x, o = x.y, d = x.z, w = valid ? x.w : 0;
     // chunk sums stay exact in int for weights up to 2^24; otherwise every
@@ -439,36 +440,44 @@ __global__ void __launch_bounds__(WAVE) k_tm_admit(int* __restrict__ slot_dec, c
     }
     wave_lds_sync();
     if (und) inw[d] = 0;
-    if (valid) {
-      if (adm) {
-        atomicAdd(reinterpret_cast<unsigned long long*>(&sz[o]), (unsigned long long)(-(long long)w));
-        atomicAdd(reinterpret_cast<unsigned long long*>(&sz[d]), (unsigned long long)(long long)w);
-        part[v] = d;
-      } else {
-        slot_dec[v - r0] = -1;
-      }
+    if (valid && adm) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(&sz[o]), (unsigned long long)(-(long long)w));
+      atomicAdd(reinterpret_cast<unsigned long long*>(&sz[d]), (unsigned long long)(long long)w);
     }
     nm += (unsigned long long)__popcll(__ballot(valid && adm));
     wave_lds_sync();
+    return valid && adm;
   };
-  int4 cur[TM_PREFETCH];
+  // Two register buffers in turn, and a group's decisions written after the
+  // group (a store per candidate into part and slot_dec, the unchanged value
+  // where nothing changes), so that deciding a group issues no global memory
+  // operation: the wait for the next group's moves then covers only
+  // operations issued before them. Loads past the end read the sentinel.
+  int4 ga[TM_PREFETCH], gb[TM_PREFETCH];
+  auto load_group = [&](int g0, int4 (&dst)[TM_PREFETCH]) {
 #pragma unroll
-  for (int j = 0; j < TM_PREFETCH; j++) {
-    const int i = j * WAVE + lane;
-    cur[j] = i < nmv ? mv[i] : make_int4(0, 0, 0, 0);
-  }
-  for (int b0 = 0; b0 < nmv; b0 += TM_PREFETCH * WAVE) {
-    int4 nxt[TM_PREFETCH];
+    for (int j = 0; j < TM_PREFETCH; j++) dst[j] = mv[min(g0 + j * WAVE + lane, nmv)];
+  };
+  auto run_group = [&](int g0, const int4 (&src)[TM_PREFETCH]) {
+    bool adm[TM_PREFETCH];
+#pragma unroll
+    for (int j = 0; j < TM_PREFETCH; j++) adm[j] = g0 + j * WAVE < nmv ? chunk(g0 + j * WAVE, src[j]) : false;
 #pragma unroll
     for (int j = 0; j < TM_PREFETCH; j++) {
-      const int i = b0 + (TM_PREFETCH + j) * WAVE + lane;
-      nxt[j] = i < nmv ? mv[i] : make_int4(0, 0, 0, 0);
+      if (g0 + j * WAVE + lane < nmv) {
+        const int4 x = src[j];
+        part[x.x] = adm[j] ? x.z : x.y;
+        slot_dec[x.x - r0] = adm[j] ? x.z : -1;
+      }
     }
-#pragma unroll
-    for (int j = 0; j < TM_PREFETCH; j++)
-      if (b0 + j * WAVE < nmv) chunk(b0 + j * WAVE, cur[j]);
-#pragma unroll
-    for (int j = 0; j < TM_PREFETCH; j++) cur[j] = nxt[j];
+  };
+  constexpr int G = TM_PREFETCH * WAVE;
+  load_group(0, ga);
+  for (int b0 = 0; b0 < nmv; b0 += 2 * G) {
+    load_group(b0 + G, gb);
+    run_group(b0, ga);
+    load_group(b0 + 2 * G, ga);
+    run_group(b0 + G, gb);
   }
   for (int q = lane; q < K; q += WAVE) sizes[q] = sz[q];
   if (lane == 0 && nm) atomicAdd(moves, nm);
