@@ -220,6 +220,28 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
     wave_lds_sync();
     int dec = -1;
     if (gain > 0) {
+      // The items' bitmap rows are read 64 items at a time, one item per lane,
+      // so that the loads are in flight together; a lane (a part k of the 64
+      // being scored) then reads the words from the lanes that hold them.
+      // Losses of k: the items i != k whose row word has bit k clear.
+      auto count_losses = [&](const int* items, int nitems, const unsigned long long* rows, int w, int k) {
+        int loss = 0;
+        for (int i0 = 0; i0 < nitems; i0 += WAVE) {
+          const int n = min(WAVE, nitems - i0);
+          int it = -1;
+          unsigned long long x = ~0ull;
+          if (lane < n) {
+            it = items[i0 + lane];
+            x = rows[(size_t)it * (size_t)a.W + (size_t)w];
+          }
+          for (int i = 0; i < n; i++) {
+            const int r = __builtin_amdgcn_readlane(it, i);
+            const unsigned long long xi = readlane_u64(x, i);
+            loss += (r != k && !((xi >> lane) & 1ull)) ? 1 : 0;
+          }
+        }
+        return loss;
+      };
       if (!a.msm) {
         // TM: the largest gain - loss > 0 over parts lighter than C, the first
         // k on ties. A part connected to every receiver and every sender has
@@ -227,22 +249,25 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
         // candidate is found by ANDing their bitmap words, 64 parts at a time
         // (an item's own bit counts as connected: its loss term skips k ==
         // the item)
+        auto and_rows = [&](const int* items, int nitems, const unsigned long long* rows, int w) {
+          unsigned long long all = ~0ull;
+          for (int i0 = 0; i0 < nitems; i0 += WAVE) {
+            unsigned long long x = ~0ull;
+            if (i0 + lane < nitems) {
+              const int it = items[i0 + lane];
+              x = rows[(size_t)it * (size_t)a.W + (size_t)w];
+              if ((it >> 6) == w) x |= 1ull << (it & 63);
+            }
+            all &= wave_and_u64(x);
+          }
+          return all;
+        };
         int kfast = -1;
         for (int w = 0; w < a.W && kfast < 0; w++) {
           unsigned long long allc = vmask[w];
           if ((own >> 6) == w) allc &= ~(1ull << (own & 63));
-          for (int i = 0; i < nrecv && allc; i++) {
-            const int r = __builtin_amdgcn_readfirstlane(recv[i]);
-            unsigned long long x = a.BT[(size_t)r * (size_t)a.W + (size_t)w];
-            if ((r >> 6) == w) x |= 1ull << (r & 63);
-            allc &= x;
-          }
-          for (int i = 0; i < nsend && allc; i++) {
-            const int sp = __builtin_amdgcn_readfirstlane(send[i]);
-            unsigned long long x = a.B[(size_t)sp * (size_t)a.W + (size_t)w];
-            if ((sp >> 6) == w) x |= 1ull << (sp & 63);
-            allc &= x;
-          }
+          if (allc) allc &= and_rows(recv, nrecv, a.BT, w);
+          if (allc) allc &= and_rows(send, nsend, a.B, w);
           if (allc) kfast = w * WAVE + __builtin_ctzll(allc);
         }
         int bimp = 0, bk = INT_MAX;
@@ -252,15 +277,7 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
         }
         for (int k0 = 0; kfast < 0 && k0 < K; k0 += WAVE) {
           const int k = k0 + lane, w = k0 / WAVE;
-          int loss = 0;
-          for (int i = 0; i < nrecv; i++) {
-            const int r = recv[i];
-            loss += (r != k && !((a.BT[(size_t)r * (size_t)a.W + (size_t)w] >> lane) & 1ull)) ? 1 : 0;
-          }
-          for (int i = 0; i < nsend; i++) {
-            const int s = send[i];
-            loss += (s != k && !((a.B[(size_t)s * (size_t)a.W + (size_t)w] >> lane) & 1ull)) ? 1 : 0;
-          }
+          const int loss = count_losses(recv, nrecv, a.BT, w, k) + count_losses(send, nsend, a.B, w, k);
           const bool valid = k < K && k != own && (double)a.sizes[k] < a.C;
           if (valid && gain - loss > bimp) {
             bimp = gain - loss;
@@ -290,20 +307,17 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
           X += (sp != own && a.msg[sp] == M) ? 1 : 0;
         }
         X = wave_sum(X) + (a.msg[own] == M ? 1 : 0);
+        const int m_own = a.msg[own];
         int bimp = -1, bk = INT_MAX;
         for (int k0 = 0; X + 1 >= cntM && k0 < K; k0 += WAVE) {
           const int k = k0 + lane, w = k0 / WAVE;
           const bool valid = k < K && k != own;
           const int kc = valid ? k : own;
-          int lk = 0;  // receivers r with pc[k][r] == 0 (the diagonal never counts)
-          for (int i = 0; i < nrecv; i++) {
-            const int r = recv[i];
-            lk += (r != kc && !((a.BT[(size_t)r * (size_t)a.W + (size_t)w] >> lane) & 1ull)) ? 1 : 0;
-          }
+          // receivers r with pc[k][r] == 0 (the diagonal never counts)
+          const int lk = count_losses(recv, nrecv, a.BT, w, kc);
           bool bad = false;
           int imp = 0, nmax = 0;
-          auto item = [&](int p, int g, int l) {
-            const int mp = a.msg[p];
+          auto item = [&](int mp, int g, int l) {
             if (mp == M) {
               if (g <= l) bad = true;
               imp += g - l;
@@ -312,13 +326,28 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
               bad = true;
             }
           };
-          item(own, g_own, own_s ? (int)!((a.B[(size_t)own * (size_t)a.W + (size_t)w] >> lane) & 1ull) : 0);
+          item(m_own, g_own, own_s ? (int)!((a.B[(size_t)own * (size_t)a.W + (size_t)w] >> lane) & 1ull) : 0);
           const bool k_s = valid && bit_of(smark, kc);
-          item(kc, (k_s && bit_of(sgain, kc)) ? 1 : 0, lk);
-          for (int i = 0; i < nsend; i++) {
-            const int s = send[i];
-            if (s == own || s == kc) continue;
-            item(s, bit_of(sgain, s) ? 1 : 0, (int)!((a.B[(size_t)s * (size_t)a.W + (size_t)w] >> lane) & 1ull));
+          item(a.msg[kc], (k_s && bit_of(sgain, kc)) ? 1 : 0, lk);
+          // the senders other than own and k, 64 at a time: part, its
+          // messages, its gain bit and its B row word
+          for (int i0 = 0; i0 < nsend; i0 += WAVE) {
+            const int n = min(WAVE, nsend - i0);
+            int sl = -1, ml = 0, gl = 0;
+            unsigned long long xl = ~0ull;
+            if (lane < n) {
+              sl = send[i0 + lane];
+              ml = a.msg[sl];
+              gl = bit_of(sgain, sl) ? 1 : 0;
+              xl = a.B[(size_t)sl * (size_t)a.W + (size_t)w];
+            }
+            for (int i = 0; i < n; i++) {
+              const int sp = __builtin_amdgcn_readlane(sl, i);
+              if (sp == own) continue;
+              const unsigned long long xi = readlane_u64(xl, i);
+              if (sp != kc)
+                item(__builtin_amdgcn_readlane(ml, i), __builtin_amdgcn_readlane(gl, i), (int)!((xi >> lane) & 1ull));
+            }
           }
           if (nmax != cntM) bad = true;
           if (valid && !bad && imp > bimp) {

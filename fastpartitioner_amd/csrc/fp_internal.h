@@ -128,6 +128,29 @@ __device__ __forceinline__ unsigned wave_min_u32(unsigned x) {
   return (unsigned)__builtin_amdgcn_readlane((int)x, WAVE - 1);
 }
 
+// Wave-wide AND of an unsigned (identity ~0), and of a 64-bit word as two
+// 32-bit reductions.
+__device__ __forceinline__ unsigned wave_and_u32(unsigned x) {
+  x &= (unsigned)dpp_i32<0xB1>(-1, (int)x);
+  x &= (unsigned)dpp_i32<0x4E>(-1, (int)x);
+  x &= (unsigned)dpp_i32<0x124>(-1, (int)x);
+  x &= (unsigned)dpp_i32<0x128>(-1, (int)x);
+  x &= (unsigned)dpp_i32<0x142, 0xA>(-1, (int)x);
+  x &= (unsigned)dpp_i32<0x143, 0xC>(-1, (int)x);
+  return (unsigned)__builtin_amdgcn_readlane((int)x, WAVE - 1);
+}
+
+__device__ __forceinline__ unsigned long long wave_and_u64(unsigned long long x) {
+  return ((unsigned long long)wave_and_u32((unsigned)(x >> 32)) << 32) | wave_and_u32((unsigned)x);
+}
+
+// lane l's 64-bit value, to every lane
+__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long v, int l) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), l);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
 // Inclusive prefix sum over the wave through DPP: row_shr 1, 2, 4, 8 (lanes
 // whose source is outside the row add 0), then row_bcast15 into rows 1 and 3
 // and row_bcast31 into rows 2 and 3 carry the row totals.
