@@ -470,6 +470,7 @@ struct VolBatchLds {
   int2* mvod;  // [WAVE] (own part, new part) of the batch's moves
   int* D;      // [WAVE][WAVE + 1] per candidate, change and then value after each move
 };
+constexpr int VOL_STAGE_LOADS = 8;      // event loads per lane in flight while staging
 constexpr int VOL_BATCH_KMAX = 1024;  // the batched step's lane holds at most 16 parts
 constexpr int DROW = WAVE + 1;        // padded row of D: a lane per row is conflict-free
 
@@ -650,7 +651,20 @@ __global__ void __launch_bounds__(WAVE) k_vol_scan(int* __restrict__ vol_g, int 
     int tot = 0;
     if (staged) {
       tot = __builtin_amdgcn_readlane(cum, nb - 1);
-      for (int t = lane; t < tot; t += WAVE) stage[t] = evbuf[eb0 + (unsigned)t];
+      // VOL_STAGE_LOADS loads per lane issued together, then stored
+      for (int t0 = 0; t0 < tot; t0 += VOL_STAGE_LOADS * WAVE) {
+        int4 e[VOL_STAGE_LOADS];
+        // past the end a lane reads and writes the batch's last event again
+        // (no branch, so the loads are not moved next to their stores)
+#pragma unroll
+        for (int u = 0; u < VOL_STAGE_LOADS; u++) e[u] = evbuf[eb0 + (unsigned)min(t0 + u * WAVE + lane, tot - 1)];
+        // an empty asm that takes every loaded value: all loads issue before
+        // the first store instead of each waiting next to its own store
+#pragma unroll
+        for (int u = 0; u < VOL_STAGE_LOADS; u++) asm volatile("" ::"v"(e[u].x), "v"(e[u].y), "v"(e[u].z), "v"(e[u].w));
+#pragma unroll
+        for (int u = 0; u < VOL_STAGE_LOADS; u++) stage[min(t0 + u * WAVE + lane, tot - 1)] = e[u];
+      }
       wave_lds_sync();
     }
     int bstop = -2;
