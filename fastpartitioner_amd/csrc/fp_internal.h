@@ -445,6 +445,30 @@ __device__ __forceinline__ void wave_rank_window(const int* apply_pins, unsigned
   hi = wave_lower_bound(apply_pins, lo, e, r1);
 }
 
+// Walks a wave's share of the touched nets (entries first, first + stride,
+// ... of touched): their ids and pin ranges are loaded 64 nets at a time,
+// one net per lane, then f(n, s, e) runs on every lane for each net in
+// order, with [s, e) the net's range in net_ptr.
+template <class F>
+__device__ __forceinline__ void wave_for_touched(const int* __restrict__ touched, int ntouched,
+                                                 const unsigned* __restrict__ net_ptr, int first, int stride, F f) {
+  const int lane = lane_id();
+  for (long long b0 = first; b0 < ntouched; b0 += (long long)WAVE * stride) {
+    const long long ij = b0 + (long long)lane * stride;
+    int nj = 0;
+    unsigned sj = 0, ej = 0;
+    if (ij < ntouched) {
+      nj = touched[ij];
+      sj = net_ptr[nj];
+      ej = net_ptr[nj + 1];
+    }
+    const int cnt = (int)min((long long)WAVE, ((long long)ntouched - b0 + stride - 1) / stride);
+    for (int j = 0; j < cnt; j++)
+      f(__builtin_amdgcn_readlane(nj, j), (unsigned)__builtin_amdgcn_readlane((int)sj, j),
+        (unsigned)__builtin_amdgcn_readlane((int)ej, j));
+  }
+}
+
 // Writes a list back to both the packed segment and its evaluation view.
 __device__ __forceinline__ void wave_list_store(unsigned* ent, unsigned short* ev, unsigned seg, const unsigned* list,
                                                 int L, int pb, unsigned pmask) {

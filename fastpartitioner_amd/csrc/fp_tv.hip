@@ -478,9 +478,8 @@ __global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
                        (size_t)wave * (size_t)a.K;
   if (INDEXED) wave_index_init(at, a.K);
   const int ntouched = *a.tcount;
-  for (int i = (int)blockIdx.x * nw + wave; i < ntouched; i += a.net_blocks * nw) {
-    const int n = a.touched[i];
-    const unsigned s = a.net_ptr[n], e = a.net_ptr[n + 1];
+  wave_for_touched(a.touched, ntouched, a.net_ptr, (int)blockIdx.x * nw + wave, a.net_blocks * nw,
+                   [&](int n, unsigned s, unsigned e) {
     const unsigned seg = s + (unsigned)n;
     unsigned lo, hi;
     wave_rank_window(a.apply_pins, s, e, a.r0, a.r1, lo, hi);
@@ -513,7 +512,7 @@ __global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
     if (dirty) wave_list_store(a.ent, a.ev, seg, list, L, a.pb, a.pmask);
     if (INDEXED) wave_index_clear(list, L, at, a.pmask);
     wave_lds_sync();
-  }
+  });
 }
 
 // the apply's LDS: a K-entry list per wave, and the part -> position index
