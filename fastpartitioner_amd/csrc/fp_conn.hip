@@ -595,8 +595,7 @@ __global__ void __launch_bounds__(REPLAY_BLOCK) k_conn_apply(ConnApplyArgs a) {
       atomic_add_i64(&a.sizes[m.y], -w);
       atomic_add_i64(&a.sizes[m.z], w);
     }
-    const unsigned long long b = __ballot(mvd);
-    if (lane == 0 && b) atomicAdd(a.moves, (unsigned long long)__popcll(b));
+    block_count_add(a.moves, mvd);
     return;
   }
   unsigned* list = reinterpret_cast<unsigned*>(lds) + (size_t)wave * (size_t)a.K;

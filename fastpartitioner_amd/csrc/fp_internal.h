@@ -484,6 +484,19 @@ __device__ __forceinline__ void wave_list_store(unsigned* ent, unsigned short* e
   }
 }
 
+// Adds the number of the block's threads with `flag` set to *counter, with
+// one global atomic per block (same-address atomics from every wave queue at
+// one L2 slice). Every thread of the block calls it.
+__device__ __forceinline__ void block_count_add(unsigned long long* counter, bool flag) {
+  __shared__ unsigned long long s_cnt;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  const unsigned long long b = __ballot(flag);
+  if (lane_id() == 0 && b) atomicAdd(&s_cnt, (unsigned long long)__popcll(b));
+  __syncthreads();
+  if (threadIdx.x == 0 && s_cnt) atomicAdd(counter, s_cnt);
+}
+
 __device__ __forceinline__ void atomic_add_i64(long long* p, long long v) {
   atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
 }

@@ -388,6 +388,11 @@ __global__ void __launch_bounds__(256) k_bal_admit(int* __restrict__ slot_dec, c
                                                    unsigned long long* __restrict__ moves) {
   const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
   const int nover = *over_count;
+  // the block's admitted moves, added to the global count once per block
+  // (same-address atomics from every wave would queue at one L2 slice)
+  __shared__ unsigned long long s_moves;
+  if (threadIdx.x == 0) s_moves = 0;
+  __syncthreads();
   if (blockIdx.x == 0)
     for (int d = (int)threadIdx.x; d < K; d += (int)blockDim.x) cand_tot[d] = 0;
   for (int tile = (int)blockIdx.x * nw + wave; tile < ntiles; tile += (int)gridDim.x * nw) {
@@ -422,8 +427,10 @@ __global__ void __launch_bounds__(256) k_bal_admit(int* __restrict__ slot_dec, c
       atomic_add_i64(&sizes[d], w);
     }
     const unsigned long long am = __ballot(adm);
-    if (lane == 0 && am) atomicAdd(moves, (unsigned long long)__popcll(am));
+    if (lane == 0 && am) atomicAdd(&s_moves, (unsigned long long)__popcll(am));
   }
+  __syncthreads();
+  if (threadIdx.x == 0 && s_moves) atomicAdd(moves, s_moves);
 }
 
 struct ApplyArgs {
@@ -469,8 +476,7 @@ __global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
         mv = true;
       }
     }
-    const unsigned long long m = __ballot(mv);
-    if (lane == 0 && m) atomicAdd(a.moves, (unsigned long long)__popcll(m));
+    block_count_add(a.moves, mv);
     return;
   }
   unsigned* list = reinterpret_cast<unsigned*>(lds) + (size_t)wave * (size_t)a.K;
