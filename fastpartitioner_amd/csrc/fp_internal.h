@@ -431,6 +431,52 @@ __device__ __forceinline__ void wave_touch_nets(const unsigned* __restrict__ vn,
   }
 }
 
+// A wave's newly touched nets, staged in its own LDS buffer of TOUCH_BUF
+// entries and appended to the round's touched list a buffer at a time: one
+// atomic on the list's counter per flush instead of one per vertex (atomics
+// from every wave on that one address queue at one L2 slice). The wave must
+// flush before it exits.
+constexpr int TOUCH_BUF = 256;
+
+struct WaveTouch {
+  unsigned* buf;  // this wave's LDS buffer
+  int n;          // staged nets (wave-uniform)
+};
+
+__device__ __forceinline__ void wave_touch_flush(WaveTouch& t, int* __restrict__ touched, int* __restrict__ tcount) {
+  if (t.n == 0) return;
+  const int lane = lane_id();
+  int base = 0;
+  if (lane == 0) base = atomicAdd(tcount, t.n);
+  base = __builtin_amdgcn_readfirstlane(base);
+  wave_lds_sync();
+  for (int i = lane; i < t.n; i += WAVE) touched[base + i] = (int)t.buf[i];
+  wave_lds_sync();
+  t.n = 0;
+}
+
+// wave_touch_nets through the wave's buffer
+__device__ __forceinline__ void wave_touch_nets_staged(const unsigned* __restrict__ vn, int deg, unsigned tag,
+                                                       unsigned* __restrict__ net_stamp, WaveTouch& t,
+                                                       int* __restrict__ touched, int* __restrict__ tcount) {
+  const int lane = lane_id();
+  for (int i0 = 0; i0 < deg; i0 += WAVE) {
+    const int i = i0 + lane;
+    unsigned n = 0;
+    bool isnew = false;
+    if (i < deg) {
+      n = vn[i];
+      isnew = atomicExch(&net_stamp[n], tag) != tag;
+    }
+    const unsigned long long b = __ballot(isnew);
+    const int c = (int)__popcll(b);
+    if (c == 0) continue;
+    if (t.n + c > TOUCH_BUF) wave_touch_flush(t, touched, tcount);
+    if (isnew) t.buf[t.n + (int)__popcll(b & ((1ull << lane) - 1ull))] = n;
+    t.n += c;
+  }
+}
+
 // Range [lo, hi) of apply_pins[s..e) (a net's pin ranks, ascending) that
 // may hold ranks in [r0, r1): the exact range for nets longer than a wave,
 // the whole net otherwise (the caller tests each rank against [r0, r1)).

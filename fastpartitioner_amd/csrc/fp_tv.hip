@@ -90,6 +90,8 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
   unsigned* cnt = wbase + (size_t)wave * 2 * (size_t)(K + WAVE);
   unsigned* first = cnt + K + WAVE;
   const int spare = K + lane_id();
+  // per wavefront: the nets its accepted vertices touch, before they go to the touched list
+  WaveTouch tb{wbase + (size_t)nw * 2 * (size_t)(K + WAVE) + (size_t)wave * TOUCH_BUF, 0};
 
   // penalties of the frozen part sizes: penalty = 1 - (1.0 * size / C)  (TV/partition.cpp:103)
   long long bs = LLONG_MAX;
@@ -315,12 +317,14 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
         atomicAdd(&a.hist[(size_t)d * (size_t)a.ntiles + (size_t)(slot / TILE)], w);
       }
     }
-    if (accept && a.touched) wave_touch_nets(a.vtx_nets + vb, deg, a.tag, a.net_stamp, a.touched, a.tcount);
+    if (accept && a.touched)
+      wave_touch_nets_staged(a.vtx_nets + vb, deg, a.tag, a.net_stamp, tb, a.touched, a.tcount);
     slot += stride;
     hv[0] = hv[1]; hown[0] = hown[1]; hvb[0] = hvb[1]; hdeg[0] = hdeg[1];
     hv[1] = hv[2]; hown[1] = hown[2]; hvb[1] = hvb[2]; hdeg[1] = hdeg[2];
     sg0 = sg1;
   }
+  if (a.touched) wave_touch_flush(tb, a.touched, a.tcount);
 }
 
 // BAL step 1: over-subscribed parts (candidate weight > cap - frozen size) get
@@ -531,11 +535,13 @@ size_t apply_lds(int K) {
   return (size_t)(APPLY_BLOCK / WAVE) * (size_t)K * (sizeof(unsigned) + (apply_indexed(K) ? sizeof(unsigned short) : 0));
 }
 
-size_t eval_lds(int K, int waves) { return (size_t)K * 16 + (size_t)waves * 8 * (size_t)(K + WAVE); }
+size_t eval_lds(int K, int waves) {
+  return (size_t)K * 16 + (size_t)waves * (8 * (size_t)(K + WAVE) + 4 * (size_t)TOUCH_BUF);
+}
 
 int eval_waves(int K) {
   // LDS: 16 bytes per part shared by the block + 8 bytes per part (and per
-  // spare slot) per wavefront
+  // spare slot) and the touched-net buffer per wavefront
   for (int w = EVAL_BLOCK / WAVE; w >= 1; w--)
     if (eval_lds(K, w) <= (size_t)LDS_BUDGET) return w;
   return 0;
