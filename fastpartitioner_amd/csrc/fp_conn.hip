@@ -138,6 +138,10 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
   unsigned* rmark = reinterpret_cast<unsigned*>(send + K);
   unsigned* smark = rmark + KW;
   unsigned* sgain = smark + KW;
+  // per wavefront: the nets its moving vertices touch, before they go to the touched list
+  WaveTouch tb{reinterpret_cast<unsigned*>(vmask + a.W) + (size_t)nw * (size_t)(2 * K + 3 * KW) +
+                   (size_t)wave * TOUCH_BUF,
+               0};
   for (int q = lane; q < KW; q += WAVE) {
     rmark[q] = 0;
     smark[q] = 0;
@@ -370,7 +374,7 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
       a.slot_own[slot] = own;
       a.slot_dec[slot] = dec;
     }
-    if (dec >= 0) wave_touch_nets(a.vtx_nets + vb, deg, a.tag, a.net_stamp, a.touched, a.tcount);
+    if (dec >= 0) wave_touch_nets_staged(a.vtx_nets + vb, deg, a.tag, a.net_stamp, tb, a.touched, a.tcount);
     // clear this vertex's marks
     wave_lds_sync();
     for (int i = lane; i < nrecv; i += WAVE) rmark[recv[i] >> 5] = 0;
@@ -380,6 +384,7 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
     }
     wave_lds_sync();
   }
+  wave_touch_flush(tb, a.touched, a.tcount);
 }
 
 // TM apply order (TM/partition.cpp:1111-1125): the round's candidates (the
@@ -726,9 +731,10 @@ __global__ void __launch_bounds__(WAVE) k_msm_scan(int* __restrict__ pc, const i
 }
 
 // LDS of k_conn_eval: the block's candidate mask, then per wavefront the
-// receiver and sender lists and three K-bit marks
+// receiver and sender lists and three K-bit marks, then per wavefront the
+// touched-net buffer
 size_t conn_eval_lds(int K, int waves) {
-  const size_t per = ((size_t)2 * (size_t)K + 3 * (size_t)((K + 31) / 32)) * sizeof(int);
+  const size_t per = ((size_t)2 * (size_t)K + 3 * (size_t)((K + 31) / 32) + TOUCH_BUF) * sizeof(int);
   return (size_t)((K + 63) / 64) * sizeof(unsigned long long) + (size_t)waves * per;
 }
 
