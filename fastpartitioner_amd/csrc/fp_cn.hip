@@ -179,7 +179,15 @@ struct CnPrepArgs {
 // pass (every accepted net whose old part is v's part at pass start); pushes
 // apply in net order, so the source of the push by net n is the new part of
 // the largest such net below n, else v's part at pass start.
-__global__ void k_cn_prep(CnPrepArgs a) {
+// The touched-net appends are counted per wave and the op total per block:
+// one atomic each on those counters instead of one per push (same-address
+// atomics queue at one L2 slice).
+__global__ void __launch_bounds__(256) k_cn_prep(CnPrepArgs a) {
+  __shared__ unsigned long long s_ops;
+  if (threadIdx.x == 0) s_ops = 0;
+  __syncthreads();
+  const int lane = lane_id();
+  unsigned long long ops = 0;
   const long long nm = (long long)*a.nmoves;
   for (long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x; m < nm; m += (long long)gridDim.x * blockDim.x) {
   const int v = a.mv_v[m], n = a.mv_net[m];
@@ -198,11 +206,22 @@ __global__ void k_cn_prep(CnPrepArgs a) {
   a.mv_last[m] = last ? 1 : 0;
   for (unsigned j = vb; j < ve; j++) {
     const unsigned t = a.vtx_nets[j];
-    if (atomicExch(&a.net_stamp[t], a.tag) != a.tag) a.touched[atomicAdd(a.tcount, 1)] = (int)t;
+    const bool isnew = atomicExch(&a.net_stamp[t], a.tag) != a.tag;
+    // the lanes still in this loop append their new nets with one atomic
+    const unsigned long long b = __ballot(isnew);
+    if (b) {
+      int base = 0;
+      if (lane == (int)__builtin_ctzll(__ballot(true))) base = atomicAdd(a.tcount, (int)__popcll(b));
+      base = __builtin_amdgcn_readfirstlane(base);
+      if (isnew) a.touched[base + (int)__popcll(b & ((1ull << lane) - 1ull))] = (int)t;
+    }
     atomicAdd(&a.opcnt[t], 1);
   }
-  atomicAdd(&a.totals[0], (unsigned long long)(ve - vb));
+  ops += ve - vb;
   }
+  if (ops) atomicAdd(&s_ops, ops);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_ops) atomicAdd(&a.totals[0], s_ops);
 }
 
 // The pass's edit lists must fit: total ops <= cap and at most CN_MAXOPS per
@@ -216,16 +235,29 @@ __global__ void k_cn_check(const unsigned long long* __restrict__ totals, long l
   flags[1] = net_over ? 1 : 0;
 }
 
-__global__ void k_cn_alloc(const int* __restrict__ touched, const int* __restrict__ tcount,
-                           const int* __restrict__ opcnt, long long* __restrict__ opoff, int* __restrict__ opfill,
-                           unsigned long long* __restrict__ totals) {
+// Each touched net's op range: the block's nets take consecutive ranges
+// (a block scan), the block's start comes from one atomic on the running total.
+__global__ void __launch_bounds__(256) k_cn_alloc(const int* __restrict__ touched, const int* __restrict__ tcount,
+                                                  const int* __restrict__ opcnt, long long* __restrict__ opoff,
+                                                  int* __restrict__ opfill, unsigned long long* __restrict__ totals) {
+  __shared__ long long s_w[256 / WAVE];
+  __shared__ long long s_base;
+  __shared__ int s_max;
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (i >= *tcount) return;
-  const int t = touched[i];
-  const int c = opcnt[t];
-  opoff[t] = (long long)atomicAdd(&totals[1], (unsigned long long)c);
-  opfill[t] = 0;
-  atomicMax(reinterpret_cast<unsigned long long*>(&totals[2]), (unsigned long long)c);
+  const bool valid = i < *tcount;
+  const int t = valid ? touched[i] : 0;
+  const int c = valid ? opcnt[t] : 0;
+  if (threadIdx.x == 0) s_max = 0;
+  long long tot;
+  const long long ex = block_excl_scan((long long)c, s_w, &tot);
+  if (c) atomicMax(&s_max, c);
+  if (threadIdx.x == 0) s_base = tot ? (long long)atomicAdd(&totals[1], (unsigned long long)tot) : 0;
+  __syncthreads();
+  if (valid) {
+    opoff[t] = s_base + ex;
+    opfill[t] = 0;
+  }
+  if (threadIdx.x == 0 && s_max) atomicMax(reinterpret_cast<unsigned long long*>(&totals[2]), (unsigned long long)s_max);
 }
 
 __global__ void k_cn_scatter(const int* __restrict__ mv_v, const unsigned long long* __restrict__ nmoves,
