@@ -499,18 +499,39 @@ __device__ __forceinline__ void vol_event_changes(const int4 x, const int2 od, F
 // it after t. Applies the moves up to the stop (all if none) and returns the
 // batch index of the stop move, -1 if none, or -2 (nothing applied) when
 // the candidates do not fit a wave.
+// Runs f(event, (own, new) of its move, its move index - mi0) over the staged
+// events [0, n), VOL_EV_UNROLL per lane at a time: their reads are issued
+// before f touches LDS (the compiler keeps LDS reads and atomics in program
+// order, so one event at a time would wait out each read).
+constexpr int VOL_EV_UNROLL = 4;
+
+template <class F>
+__device__ __forceinline__ void for_staged_events(const int4* stage, int n, const int2* mvod, int mi0, F f) {
+  const int lane = lane_id();
+  for (int e0 = 0; e0 < n; e0 += VOL_EV_UNROLL * WAVE) {
+    int4 x[VOL_EV_UNROLL];
+    int2 od[VOL_EV_UNROLL];
+#pragma unroll
+    for (int u = 0; u < VOL_EV_UNROLL; u++) x[u] = stage[min(e0 + u * WAVE + lane, n - 1)];
+#pragma unroll
+    for (int u = 0; u < VOL_EV_UNROLL; u++) od[u] = mvod[x[u].w - mi0];
+#pragma unroll
+    for (int u = 0; u < VOL_EV_UNROLL; u++)
+      if (e0 + u * WAVE + lane < n) f(x[u], od[u], x[u].w - mi0);
+  }
+}
+
 __device__ int vol_scan_batch(int* vol, int K, const int4* stage, int tot, int nb, int mi0, int own_l, int d_l,
                               int cum_l, const VolBatchLds& B, LaneMax& ms, int& M) {
   const int lane = lane_id();
   if (lane < nb) B.mvod[lane] = make_int2(own_l, d_l);
   wave_lds_sync();
-  for (int e = lane; e < tot; e += WAVE) {
-    const int4 x = stage[e];
-    vol_event_changes(x, B.mvod[x.w - mi0], [&](int p, int w) {
+  for_staged_events(stage, tot, B.mvod, mi0, [&](int4 x, int2 od, int) {
+    vol_event_changes(x, od, [&](int p, int w) {
       if (w > 0) atomicAdd(&B.pos[p], w);
       else atomicAdd(&B.neg[p], w);
     });
-  }
+  });
   wave_lds_sync();
   int al = INT_MIN;
   for (unsigned long long b = ms.inS; b; b &= b - 1) {
@@ -545,14 +566,12 @@ __device__ int vol_scan_batch(int* vol, int K, const int4* stage, int tot, int n
   if (lane < nb)
     for (int r = 0; r < nc; r++) B.D[r * DROW + lane] = 0;
   wave_lds_sync();
-  for (int e = lane; e < tot; e += WAVE) {
-    const int4 x = stage[e];
-    const int j = x.w - mi0;
-    vol_event_changes(x, B.mvod[j], [&](int p, int w) {
+  for_staged_events(stage, tot, B.mvod, mi0, [&](int4 x, int2 od, int j) {
+    vol_event_changes(x, od, [&](int p, int w) {
       const int ci = B.cidx[p];
       if (ci >= 0) atomicAdd(&B.D[ci * DROW + j], w);
     });
-  }
+  });
   wave_lds_sync();
   // lane t: the max and max set after move t, and whether t stops the
   // round; candidate r's value after move t is its value before the batch
@@ -576,10 +595,8 @@ __device__ int vol_scan_batch(int* vol, int K, const int4* stage, int tot, int n
   const unsigned long long sb = __ballot(lane < nb && (Bm >= A || (mask & ~prev) != 0ull));
   const int tstop = sb ? (int)__builtin_ctzll(sb) : -1;
   const int eend = __builtin_amdgcn_readlane(cum_l, sb ? tstop : nb - 1);
-  for (int e = lane; e < eend; e += WAVE) {
-    const int4 x = stage[e];
-    vol_event_changes(x, B.mvod[x.w - mi0], [&](int p, int w) { atomicAdd(&vol[p], w); });
-  }
+  for_staged_events(stage, eend, B.mvod, mi0,
+                    [&](int4 x, int2 od, int) { vol_event_changes(x, od, [&](int p, int w) { atomicAdd(&vol[p], w); }); });
   if (lane < nc) B.cidx[B.cpart[lane]] = -1;
   wave_lds_sync();
   M = maxset_init(vol, K, ms);
