@@ -27,6 +27,10 @@ struct MetricArgs {
 };
 
 constexpr int METRIC_LDS_KMAX = 1024;
+#ifndef FP_METRIC_GRID
+#define FP_METRIC_GRID 4096
+#endif
+constexpr int METRIC_GRID = FP_METRIC_GRID;  // blocks at most: each flushes its per-part sums once
 
 template <bool LDS_ACC>
 __global__ void __launch_bounds__(METRIC_BLOCK) k_metrics(MetricArgs a) {
@@ -90,9 +94,21 @@ __global__ void __launch_bounds__(METRIC_BLOCK) k_metrics(MetricArgs a) {
       if (s_msv[q]) atomicAdd(&a.send_msv[q], s_msv[q]);
     }
   }
+  // TV and CN: summed over the block, then one atomic each
+  __shared__ unsigned long long s_tv, s_cn;
+  if (threadIdx.x == 0) {
+    s_tv = 0;
+    s_cn = 0;
+  }
+  __syncthreads();
   if (lane == 0) {
-    if (tv) atomicAdd(&a.tvcn[0], tv);
-    if (cn) atomicAdd(&a.tvcn[1], cn);
+    if (tv) atomicAdd(&s_tv, tv);
+    if (cn) atomicAdd(&s_cn, cn);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_tv) atomicAdd(&a.tvcn[0], s_tv);
+    if (s_cn) atomicAdd(&a.tvcn[1], s_cn);
   }
 }
 
@@ -138,7 +154,7 @@ fp_status fp_eval(fp_ctx* ctx, fp_metrics* out, int* lam, long long* sv, long lo
   m.send_msv = ctx->mvec + 4 * (size_t)K;
   m.tvcn = ctx->mvec + 5 * (size_t)K;
   const size_t lds = acc_bytes + (size_t)waves * K * sizeof(int);
-  const int grid = std::max(1, std::min((ctx->noe + waves - 1) / waves, 4096));
+  const int grid = std::max(1, std::min((ctx->noe + waves - 1) / waves, METRIC_GRID));
   if (ctx->noe > 0) {
     if (lds_acc) k_metrics<true><<<grid, waves * WAVE, lds, ctx->stream>>>(m);
     else k_metrics<false><<<grid, waves * WAVE, lds, ctx->stream>>>(m);
