@@ -610,15 +610,18 @@ __global__ void __launch_bounds__(REPLAY_BLOCK) k_conn_apply(ConnApplyArgs a) {
     const unsigned s = a.net_ptr[n], e = a.net_ptr[n + 1];
     const unsigned seg = s + (unsigned)n;
     unsigned lo, hi;
+    // a net of at most 64 pins: its ranks (and its source) are read together with its list
+    const bool small = e - s <= (unsigned)WAVE;
+    const int r_small = small && (unsigned)lane < e - s ? a.apply_pins[s + (unsigned)lane] : INT_MAX;
+    const int srcv = a.src[n];
     wave_rank_window(a.apply_pins, s, e, a.r0, a.r1, lo, hi);
     int L = wave_list_load(a.ent, seg, e - s, list);
-    const int srcv = a.src[n];
     int sp = frozen_part(srcv, a.part, a.slot_own, a.r0, a.r1);
     bool dirty = false;
     for (unsigned k0 = lo; k0 < hi; k0 += WAVE) {
       const unsigned k = k0 + (unsigned)lane;
       int slot = -1, dd = -1;
-      const int r = k < hi ? a.apply_pins[k] : INT_MAX;
+      const int r = small ? r_small : (k < hi ? a.apply_pins[k] : INT_MAX);
       if (r >= a.r0 && r < a.r1) {
         slot = r - a.r0;
         if (a.mode == 0) {

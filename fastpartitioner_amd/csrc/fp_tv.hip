@@ -491,6 +491,9 @@ __global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
   wave_for_touched(a.touched, ntouched, a.net_ptr, (int)blockIdx.x * nw + wave, a.net_blocks * nw,
                    [&](int n, unsigned s, unsigned e) {
     const unsigned seg = s + (unsigned)n;
+    // a net of at most 64 pins: its ranks are read together with its list
+    const bool small = e - s <= (unsigned)WAVE;
+    const int r_small = small && (unsigned)lane < e - s ? a.apply_pins[s + (unsigned)lane] : INT_MAX;
     unsigned lo, hi;
     wave_rank_window(a.apply_pins, s, e, a.r0, a.r1, lo, hi);
     int L = wave_list_load(a.ent, seg, e - s, list);
@@ -499,7 +502,7 @@ __global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
     for (unsigned k0 = lo; k0 < hi; k0 += WAVE) {
       const unsigned k = k0 + (unsigned)lane;
       int dd = -1, oo = -1;
-      const int r = k < hi ? a.apply_pins[k] : INT_MAX;
+      const int r = small ? r_small : (k < hi ? a.apply_pins[k] : INT_MAX);
       if (r >= a.r0 && r < a.r1) {
         dd = a.slot_dec[r - a.r0];
         if (dd >= 0) oo = a.slot_own[r - a.r0];

@@ -372,16 +372,19 @@ __global__ void __launch_bounds__(REPLAY_BLOCK) k_vol_replay(VolReplayArgs a) {
     const unsigned s = a.net_ptr[n], e = a.net_ptr[n + 1];
     const unsigned seg = s + (unsigned)n;
     unsigned lo, hi;
+    // a net of at most 64 pins: its ranks (and its source) are read together with its list
+    const bool small = e - s <= (unsigned)WAVE;
+    const int r_small = small && (unsigned)lane < e - s ? a.apply_pins[s + (unsigned)lane] : INT_MAX;
+    const int srcv = a.src[n];
     wave_rank_window(a.apply_pins, s, e, a.r0, a.r1, lo, hi);
     int L = wave_list_load(a.ent, seg, e - s, list);
     if (INDEXED) wave_index_set(list, L, at, a.pmask);
-    const int srcv = a.src[n];
     int sp_cur = a.commit ? 0 : a.part[srcv];  // the source's part as the replay goes
     bool dirty = false;
     for (unsigned k0 = lo; k0 < hi; k0 += WAVE) {
       const unsigned k = k0 + (unsigned)lane;
       int mi = -1;
-      const int r = k < hi ? a.apply_pins[k] : INT_MAX;
+      const int r = small ? r_small : (k < hi ? a.apply_pins[k] : INT_MAX);
       if (r >= a.r0 && r < a.r1) mi = a.slot_mi[r - a.r0];
       if (a.commit && mi >= nstop) mi = -1;
       // each lane fetches its own move (v, own, d, evoff) and, for the dry
