@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(256) k_conn_init(const unsigned* __restrict__ 
 
 // One block (four wavefronts) per 64 x 64 tile of pc: the tile's words of B
 // (wavefront 0) and BT (wavefront 1), and the popcounts of its B words into
-// msg (zeroed before).
+// msg (zeroed before; msg null: left as it is).
 __global__ void __launch_bounds__(256) k_conn_bits(const int* __restrict__ pc, int K, int W,
                                                     unsigned long long* __restrict__ B,
                                                     unsigned long long* __restrict__ BT, int* __restrict__ msg) {
@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(256) k_conn_bits(const int* __restrict__ pc, i
     const int a = ta * WAVE + lane;
     if (a < K) {
       B[(size_t)a * (size_t)W + (size_t)tb] = row;
-      if (row) atomicAdd(&msg[a], __popcll(row));
+      if (row && msg) atomicAdd(&msg[a], __popcll(row));
     }
   } else if (wave == 1) {
     // row b = tb*64 + lane of the transpose, columns a = ta*64 + c
@@ -685,7 +685,7 @@ __global__ void __launch_bounds__(REPLAY_BLOCK) k_conn_apply(ConnApplyArgs a) {
 // MSM: the round's moves in order (MSM/partition.cpp:1230-1370). Each move's
 // pc changes go to pc with atomics; a cell crossing 0 <-> 1 changes its row's
 // message count; then the max-set step decides whether the round stops.
-__global__ void __launch_bounds__(WAVE) k_msm_scan(int* __restrict__ pc, const int* __restrict__ msg_g, int K,
+__global__ void __launch_bounds__(WAVE) k_msm_scan(int* __restrict__ pc, int* __restrict__ msg_g, int K,
                                                    const int4* __restrict__ mv, const int2* __restrict__ evbuf,
                                                    const unsigned* __restrict__ incoff,
                                                    const unsigned* __restrict__ vtx_ptr, int* __restrict__ mstat) {
@@ -726,6 +726,8 @@ __global__ void __launch_bounds__(WAVE) k_msm_scan(int* __restrict__ pc, const i
     }
   }
   const int cntM = maxset_count(ms);
+  // the messages after the moves up to the stop: what a recount from pc gives
+  for (int q = lane; q < K; q += WAVE) msg_g[q] = msg[q];
   if (lane == 0) {
     mstat[0] = M;
     mstat[1] = cntM;
@@ -803,11 +805,14 @@ fp_status ensure_conn_scratch(fp_ctx* ctx, int R) {
   return FP_OK;
 }
 
-void launch_bits(fp_ctx* ctx) {
+// B and BT from pc, and msg recounted unless the caller has it (an MSM round's
+// scan leaves the exact counts)
+void launch_bits(fp_ctx* ctx, bool recount_msg = true) {
   auto& c = ctx->conn;
   const int K = ctx->K, W = (K + 63) / 64;
-  fill_async<int>(ctx, c.msg, (size_t)K, 0);
-  k_conn_bits<<<dim3((unsigned)W, (unsigned)W), 256, 0, ctx->stream>>>(c.pc, K, W, c.B, c.BT, c.msg);
+  if (recount_msg) fill_async<int>(ctx, c.msg, (size_t)K, 0);
+  k_conn_bits<<<dim3((unsigned)W, (unsigned)W), 256, 0, ctx->stream>>>(c.pc, K, W, c.B, c.BT,
+                                                                       recount_msg ? c.msg : nullptr);
 }
 
 }  // namespace
@@ -926,7 +931,7 @@ fp_status pass_conn(fp_ctx* ctx, const RefinePlan& p) {
       k_conn_apply<<<ap.net_blocks + (nslots + REPLAY_BLOCK - 1) / REPLAY_BLOCK, REPLAY_BLOCK, lds_r,
                      ctx->stream>>>(ap);
     }
-    launch_bits(ctx);
+    launch_bits(ctx, !msm);
   }
   return FP_OK;
 }
