@@ -34,6 +34,7 @@ constexpr int NETS_IN_FLIGHT = FP_NETS_IN_FLIGHT;  // segment head windows loade
 #define FP_LONG_WINDOWS 2
 #endif
 constexpr int LONG_WINDOWS = FP_LONG_WINDOWS;      // windows of one long list loaded together
+constexpr int SCAN_RUN = 16;     // consecutive tiles per thread in the BAL scan
 constexpr int APPLY_BLOCK = 256;
 constexpr int APPLY_NET_BLOCKS = 2048;
 
@@ -338,7 +339,6 @@ __global__ void __launch_bounds__(256) k_bal_scan(int* __restrict__ hist, long l
   int* s_list = reinterpret_cast<int*>(lds);  // K
   __shared__ int s_wi[16];
   __shared__ long long s_wl[16];
-  __shared__ long long s_carry;
   int base = 0;
   for (int d0 = 0; d0 < K; d0 += (int)blockDim.x) {
     const int d = d0 + (int)threadIdx.x;
@@ -354,24 +354,35 @@ __global__ void __launch_bounds__(256) k_bal_scan(int* __restrict__ hist, long l
   const int nover = base;
   if (blockIdx.x == 0 && threadIdx.x == 0) *over_count = nover;
   __syncthreads();
+  // each thread scans SCAN_RUN consecutive tiles, loaded together, so a
+  // 4096-tile row takes one round trip to memory and one block scan
   for (int o = (int)blockIdx.x; o < nover; o += (int)gridDim.x) {
     const int d = s_list[o];
     int* row = hist + (size_t)d * (size_t)ntiles;
     long long* out = pref + (size_t)o * (size_t)ntiles;
-    if (threadIdx.x == 0) s_carry = 0;
-    __syncthreads();
-    for (int t0 = 0; t0 < ntiles; t0 += (int)blockDim.x) {
-      const int t = t0 + (int)threadIdx.x;
-      const long long x = t < ntiles ? row[t] : 0;
-      long long tot;
-      const long long ex = block_excl_scan(x, s_wl, &tot);
-      if (t < ntiles) {
-        out[t] = s_carry + ex;
-        row[t] = 0;
+    long long carry = 0;
+    for (int t0 = 0; t0 < ntiles; t0 += (int)blockDim.x * SCAN_RUN) {
+      const int tb = t0 + (int)threadIdx.x * SCAN_RUN;
+      int x[SCAN_RUN];
+#pragma unroll
+      for (int p = 0; p < SCAN_RUN; p++) {
+        const int v = row[min(tb + p, ntiles - 1)];
+        x[p] = tb + p < ntiles ? v : 0;
       }
-      __syncthreads();
-      if (threadIdx.x == 0) s_carry += tot;
-      __syncthreads();
+      long long run = 0;
+#pragma unroll
+      for (int p = 0; p < SCAN_RUN; p++) run += x[p];
+      long long tot;
+      long long acc = carry + block_excl_scan(run, s_wl, &tot);
+#pragma unroll
+      for (int p = 0; p < SCAN_RUN; p++) {
+        if (tb + p < ntiles) {
+          out[tb + p] = acc;
+          row[tb + p] = 0;
+        }
+        acc += x[p];
+      }
+      carry += tot;
     }
   }
 }
