@@ -6,7 +6,9 @@
 //                software-pipelined over the wave's vertices) sweeps the lists
 //                of the vertex's nets in their 16-bit evaluation view -- one
 //                128-byte read per net brings its lambda and first 63
-//                entries, 4 nets in flight -- into LDS histograms
+//                entries, 4 nets in flight; the further windows of long
+//                lists follow as one task list over all of the vertex's long
+//                nets, 2 windows in flight -- into LDS histograms
 //                (pin count + first-appearance rank per part), then picks the
 //                LDG2 decision by a lexicographic wave reduction and applies
 //                the TV gain test. Candidates stamp their nets into the
@@ -30,10 +32,10 @@ constexpr int TILE = 64;         // slots per admission tile
 #define FP_NETS_IN_FLIGHT 4
 #endif
 constexpr int NETS_IN_FLIGHT = FP_NETS_IN_FLIGHT;  // segment head windows loaded together per wave
-#ifndef FP_LONG_WINDOWS
-#define FP_LONG_WINDOWS 2
+#ifndef FP_LONG_TASKS
+#define FP_LONG_TASKS 2
 #endif
-constexpr int LONG_WINDOWS = FP_LONG_WINDOWS;      // windows of one long list loaded together
+constexpr int LONG_TASKS = FP_LONG_TASKS;  // windows of long lists (any of the chunk's nets) loaded together
 constexpr int SCAN_RUN = 16;     // consecutive tiles per thread in the BAL scan
 constexpr int APPLY_BLOCK = 256;
 constexpr int APPLY_NET_BLOCKS = 2048;
@@ -177,6 +179,7 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
       const int nn = min(WAVE, deg - c0);
       const unsigned sg = c0 == 0 ? sg0 : (lane < nn ? a.vtx_seg[vb + (unsigned)(c0 + lane)] : a.pad_seg);
       int lmax = 0;  // longest list of the chunk
+      int lj = 0;    // lane j: lambda of the chunk's net j
       // the next group's windows are in flight while a group is counted
       unsigned xv[NETS_IN_FLIGHT];
       auto load_group = [&](int j0, unsigned (&dst)[NETS_IN_FLIGHT]) {
@@ -201,6 +204,7 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
           const int j = j0 + u;
           const int Lj = __builtin_amdgcn_readlane((int)xv[u], 0);
           lmax = max(lmax, Lj);
+          lj = lane == j ? Lj : lj;
           const int t = lane - 1;
           const bool ok = lane >= 1 && t < Lj;
           const unsigned x = xv[u];
@@ -217,30 +221,40 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
         }
       }
       // the rest of the long lists (more than 63 entries): entries 63.. in
-      // windows of 64; a segment's first word is its lambda
-      unsigned long long longmask = 0;
-      if (lmax > WAVE - 1) longmask = __ballot((int)a.ev[sg] > WAVE - 1);
-      while (longmask) {
-        const int j = __ffsll((long long)longmask) - 1;
-        longmask &= longmask - 1;
-        const unsigned sj = (unsigned)__builtin_amdgcn_readlane((int)sg, j);
-        const int Lj = (int)a.ev[sj];
-        for (int t0 = WAVE - 1; t0 < Lj; t0 += WAVE * LONG_WINDOWS) {
-          unsigned xv[LONG_WINDOWS];
+      // windows of 64. Each (net, window) pair of the chunk is a task,
+      // numbered net by net by a scan of the per-net window counts, and
+      // LONG_TASKS tasks' windows are loaded together, whichever nets they
+      // belong to; the lambdas come from the first windows.
+      if (lmax > WAVE - 1) {
+        const int nwin = lj >> 6;  // ceil((lambda - 63) / 64) windows past the first
+        const int incl = wave_incl_scan(nwin);
+        const int ntask = __builtin_amdgcn_readlane(incl, WAVE - 1);
+        for (int k0 = 0; k0 < ntask; k0 += LONG_TASKS) {
+          unsigned xw[LONG_TASKS];
+          int tj[LONG_TASKS], tw[LONG_TASKS], tl[LONG_TASKS];
 #pragma unroll
-          for (int u = 0; u < LONG_WINDOWS; u++) {
-            const int t = t0 + u * WAVE + lane;
-            xv[u] = a.ev[sj + 1u + (unsigned)(t < Lj ? t : 0)];
+          for (int u = 0; u < LONG_TASKS; u++) {
+            // past the last task: a repeat of it, with no entries counted
+            const int k = min(k0 + u, ntask - 1);
+            const int j = (int)__popcll(__ballot(incl <= k));  // its net: incl is non-decreasing
+            const int w = k - __builtin_amdgcn_readlane(incl - nwin, j);
+            const unsigned sj = (unsigned)__builtin_amdgcn_readlane((int)sg, j);
+            const int Lj = __builtin_amdgcn_readlane(lj, j);
+            tj[u] = j;
+            tw[u] = WAVE - 1 + w * WAVE;
+            tl[u] = k0 + u < ntask ? Lj : 0;
+            const int t = tw[u] + lane;
+            xw[u] = a.ev[sj + 1u + (unsigned)(t < Lj ? t : 0)];
           }
 #pragma unroll
-          for (int u = 0; u < LONG_WINDOWS; u++) {
-            const int t = t0 + u * WAVE + lane;
-            const bool ok = t < Lj;
-            const unsigned x = xv[u];
+          for (int u = 0; u < LONG_TASKS; u++) {
+            const int t = tw[u] + lane;
+            const bool ok = t < tl[u];
+            const unsigned x = xw[u];
             const int q = (int)(x & ~EV_SINGLE);
             const int qs = ok ? q : spare;
             atomicAdd(&cnt[qs], 1u);
-            atomicMin(&first[qs], ((unsigned)(c0 + j) << a.pb) | (unsigned)t);
+            atomicMin(&first[qs], ((unsigned)(c0 + tj[u]) << a.pb) | (unsigned)t);
             lgl += ok && q == own && (x & EV_SINGLE) ? 1 : 0;
           }
         }
