@@ -180,7 +180,7 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
       }
       const unsigned long long srcmask = __ballot(isrc);
       unsigned long long smask = 0;  // nets where v is own's only pin
-      wave_sweep_lists<8, 4>(a.ev, sg, nn, [&](int j, int, int, bool ok, unsigned x) {
+      wave_sweep_lists<8, SWEEP_LONG_TASKS>(a.ev, sg, nn, [&](int j, int, int, bool ok, unsigned x) {
         const int q = (int)(x & ~EV_SINGLE);
         if ((srcmask >> j) & 1ull) {
           // v's own source net: receivers are the listed parts other than own,

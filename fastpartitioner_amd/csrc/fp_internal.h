@@ -357,16 +357,24 @@ __device__ __forceinline__ unsigned lane_lower_bound(const unsigned* __restrict_
 
 // Sweeps the 16-bit lists (evaluation view) of nn <= 64 nets whose segment
 // offsets lane j of `sg` holds: NIF segment heads are loaded together (a
-// 64-word window brings lambda in lane 0 and the first 63 entries), and
-// lists longer than 63 entries are finished after in windows of 64, LW at a
-// time. For every window f(j, L, t, ok, x) runs on all lanes: j = net index
-// (wave-uniform), L = its lambda, t = this lane's list position, ok = t < L,
-// x = the entry. A window may read up to 63 words past a segment: the view
-// has ENT_PAD words of tail padding.
-template <int NIF, int LW, class F>
+// 64-word window brings lambda in lane 0 and the first 63 entries). The
+// further windows of the lists longer than 63 entries follow as one task list
+// over all those nets (numbered net by net by a scan of their window counts),
+// LT windows loaded together, with the lambdas of the first windows. For every
+// window f(j, L, t, ok, x) runs on all lanes: j = net index (wave-uniform), L =
+// its lambda, t = this lane's list position, ok = t < L, x = the entry. A
+// window may read up to 63 words past a segment: the view has ENT_PAD words of
+// tail padding.
+#ifndef FP_SWEEP_LONG_TASKS
+#define FP_SWEEP_LONG_TASKS 2
+#endif
+constexpr int SWEEP_LONG_TASKS = FP_SWEEP_LONG_TASKS;  // long-list windows in flight in the sweeps
+
+template <int NIF, int LT, class F>
 __device__ __forceinline__ void wave_sweep_lists(const unsigned short* __restrict__ ev, unsigned sg, int nn, F f) {
   const int lane = lane_id();
-  unsigned long long longmask = 0;
+  int lmax = 0;  // longest list
+  int lj = 0;    // lane j: lambda of net j
   for (int j0 = 0; j0 < nn; j0 += NIF) {
     unsigned xv[NIF];
 #pragma unroll
@@ -379,27 +387,36 @@ __device__ __forceinline__ void wave_sweep_lists(const unsigned short* __restric
       const int j = j0 + u;
       if (j < nn) {
         const int Lj = __builtin_amdgcn_readlane((int)xv[u], 0);
-        if (Lj > WAVE - 1) longmask |= 1ull << j;
+        lmax = max(lmax, Lj);
+        lj = lane == j ? Lj : lj;
         f(j, Lj, lane - 1, lane >= 1 && lane - 1 < Lj, xv[u]);
       }
     }
   }
-  while (longmask) {
-    const int j = __builtin_amdgcn_readfirstlane(__ffsll((long long)longmask) - 1);
-    longmask &= longmask - 1;
-    const unsigned sj = (unsigned)__builtin_amdgcn_readlane((int)sg, j);
-    const int Lj = (int)ev[sj];
-    for (int t0 = WAVE - 1; t0 < Lj; t0 += WAVE * LW) {
-      unsigned xw[LW];
+  if (lmax > WAVE - 1) {
+    const int nwin = lj >> 6;  // ceil((lambda - 63) / 64) windows past the first
+    const int incl = wave_incl_scan(nwin);
+    const int ntask = __builtin_amdgcn_readlane(incl, WAVE - 1);
+    for (int k0 = 0; k0 < ntask; k0 += LT) {
+      unsigned xw[LT];
+      int tj[LT], tw[LT], tl[LT];
 #pragma unroll
-      for (int u = 0; u < LW; u++) {
-        const int t = t0 + u * WAVE + lane;
+      for (int u = 0; u < LT; u++) {
+        const int k = min(k0 + u, ntask - 1);
+        const int j = (int)__popcll(__ballot(incl <= k));  // its net: incl is non-decreasing
+        const int w = k - __builtin_amdgcn_readlane(incl - nwin, j);
+        const unsigned sj = (unsigned)__builtin_amdgcn_readlane((int)sg, j);
+        const int Lj = __builtin_amdgcn_readlane(lj, j);
+        tj[u] = j;
+        tw[u] = WAVE - 1 + w * WAVE;
+        tl[u] = Lj;
+        const int t = tw[u] + lane;
         xw[u] = ev[sj + 1u + (unsigned)(t < Lj ? t : 0)];
       }
 #pragma unroll
-      for (int u = 0; u < LW; u++) {
-        const int t = t0 + u * WAVE + lane;
-        f(j, Lj, t, t < Lj, xw[u]);
+      for (int u = 0; u < LT; u++) {
+        const int t = tw[u] + lane;
+        if (k0 + u < ntask) f(tj[u], tl[u], t, t < tl[u], xw[u]);
       }
     }
   }

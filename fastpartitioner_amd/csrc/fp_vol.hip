@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(VOL_BLOCK) k_vol_eval(VolEvalArgs a) {
     for (int c0 = 0; c0 < deg; c0 += WAVE) {
       const int nn = min(WAVE, deg - c0);
       const unsigned sg = c0 == 0 ? sg0 : (lane < nn ? a.vtx_seg[vb + (unsigned)(c0 + lane)] : 0u);
-      wave_sweep_lists<8, 4>(a.ev, sg, nn, [&](int j, int, int t, bool ok, unsigned x) {
+      wave_sweep_lists<8, SWEEP_LONG_TASKS>(a.ev, sg, nn, [&](int j, int, int t, bool ok, unsigned x) {
         if (ok) {
           const int q = (int)(x & ~EV_SINGLE);
           atomicAdd(&cnt[q], 1u);
@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(VOL_BLOCK) k_vol_eval(VolEvalArgs a) {
         }
         unsigned long long smask = 0, fmask = 0;
         int myL = 0;
-        wave_sweep_lists<8, 4>(a.ev, sg, nn, [&](int j, int L, int t, bool ok, unsigned x) {
+        wave_sweep_lists<8, SWEEP_LONG_TASKS>(a.ev, sg, nn, [&](int j, int L, int t, bool ok, unsigned x) {
           const int q = (int)(x & ~EV_SINGLE);
           if (__ballot(ok && q == own && (x & EV_SINGLE))) smask |= 1ull << j;
           if (__ballot(ok && q == d)) fmask |= 1ull << j;
