@@ -331,17 +331,28 @@ __device__ __forceinline__ void wave_ilist_move(unsigned* list, unsigned short* 
   wave_lds_sync();
 }
 
-// Stages net n's list (segment at seg = net_ptr[n] + n) into LDS; returns lambda.
-__device__ __forceinline__ int wave_list_load(const unsigned* ent, unsigned seg, unsigned np, unsigned* list) {
-  // the segment holds the length and then np >= length entries: one load
-  // covers the length and the first 63 entries
+// First 64 words of net n's segment (seg = net_ptr[n] + n): the segment holds
+// the length and then np >= length entries, so one load covers the length and
+// the first 63 entries.
+__device__ __forceinline__ unsigned wave_list_head(const unsigned* ent, unsigned seg, unsigned np) {
   const int lane = lane_id();
-  const unsigned x = (unsigned)lane <= np ? ent[seg + (unsigned)lane] : 0u;
+  return (unsigned)lane <= np ? ent[seg + (unsigned)lane] : 0u;
+}
+
+// Stages a list into LDS from its head x (wave_list_head) and the rest of its
+// segment; returns lambda.
+__device__ __forceinline__ int wave_list_stage(const unsigned* ent, unsigned seg, unsigned x, unsigned* list) {
+  const int lane = lane_id();
   const int L = __builtin_amdgcn_readfirstlane((int)x);
   if (lane >= 1 && lane <= L) list[lane - 1] = x;
   for (int t = WAVE - 1 + lane; t < L; t += WAVE) list[t] = ent[seg + 1 + (unsigned)t];
   wave_lds_sync();
   return L;
+}
+
+// Stages net n's list (segment at seg = net_ptr[n] + n) into LDS; returns lambda.
+__device__ __forceinline__ int wave_list_load(const unsigned* ent, unsigned seg, unsigned np, unsigned* list) {
+  return wave_list_stage(ent, seg, wave_list_head(ent, seg, np), list);
 }
 
 // Per-lane lower bound: first k in [lo, hi) with arr[k] >= key (arr ascending).
@@ -529,6 +540,38 @@ __device__ __forceinline__ void wave_for_touched(const int* __restrict__ touched
     for (int j = 0; j < cnt; j++)
       f(__builtin_amdgcn_readlane(nj, j), (unsigned)__builtin_amdgcn_readlane((int)sj, j),
         (unsigned)__builtin_amdgcn_readlane((int)ej, j));
+  }
+}
+
+// wave_for_touched with a look-ahead: h = pre(n, s, e) issues the loads of a
+// net's head, and is called for the next net before f(n, s, e, h) runs on
+// the current one, so that those loads are in flight during f.
+template <class P, class F>
+__device__ __forceinline__ void wave_for_touched_ahead(const int* __restrict__ touched, int ntouched,
+                                                       const unsigned* __restrict__ net_ptr, int first, int stride,
+                                                       P pre, F f) {
+  const int lane = lane_id();
+  for (long long b0 = first; b0 < ntouched; b0 += (long long)WAVE * stride) {
+    const long long ij = b0 + (long long)lane * stride;
+    int nj = 0;
+    unsigned sj = 0, ej = 0;
+    if (ij < ntouched) {
+      nj = touched[ij];
+      sj = net_ptr[nj];
+      ej = net_ptr[nj + 1];
+    }
+    const int cnt = (int)min((long long)WAVE, ((long long)ntouched - b0 + stride - 1) / stride);
+    auto cur = pre(__builtin_amdgcn_readlane(nj, 0), (unsigned)__builtin_amdgcn_readlane((int)sj, 0),
+                   (unsigned)__builtin_amdgcn_readlane((int)ej, 0));
+    for (int j = 0; j < cnt; j++) {
+      decltype(cur) nxt = cur;
+      if (j + 1 < cnt)
+        nxt = pre(__builtin_amdgcn_readlane(nj, j + 1), (unsigned)__builtin_amdgcn_readlane((int)sj, j + 1),
+                  (unsigned)__builtin_amdgcn_readlane((int)ej, j + 1));
+      f(__builtin_amdgcn_readlane(nj, j), (unsigned)__builtin_amdgcn_readlane((int)sj, j),
+        (unsigned)__builtin_amdgcn_readlane((int)ej, j), cur);
+      cur = nxt;
+    }
   }
 }
 

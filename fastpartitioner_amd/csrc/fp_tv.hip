@@ -513,15 +513,26 @@ __global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
                        (size_t)wave * (size_t)a.K;
   if (INDEXED) wave_index_init(at, a.K);
   const int ntouched = *a.tcount;
-  wave_for_touched(a.touched, ntouched, a.net_ptr, (int)blockIdx.x * nw + wave, a.net_blocks * nw,
-                   [&](int n, unsigned s, unsigned e) {
+  // a net's head: the first 64 words of its segment and, for a net of at
+  // most 64 pins, its ranks; loaded while the wave's previous net is replayed
+  struct NetHead {
+    unsigned x;
+    int r;
+  };
+  auto head = [&](int n, unsigned s, unsigned e) -> NetHead {
+    NetHead h;
+    h.x = wave_list_head(a.ent, s + (unsigned)n, e - s);
+    h.r = e - s <= (unsigned)WAVE && (unsigned)lane < e - s ? a.apply_pins[s + (unsigned)lane] : INT_MAX;
+    return h;
+  };
+  wave_for_touched_ahead(a.touched, ntouched, a.net_ptr, (int)blockIdx.x * nw + wave, a.net_blocks * nw, head,
+                         [&](int n, unsigned s, unsigned e, const NetHead& h) {
     const unsigned seg = s + (unsigned)n;
-    // a net of at most 64 pins: its ranks are read together with its list
     const bool small = e - s <= (unsigned)WAVE;
-    const int r_small = small && (unsigned)lane < e - s ? a.apply_pins[s + (unsigned)lane] : INT_MAX;
+    const int r_small = h.r;
     unsigned lo, hi;
     wave_rank_window(a.apply_pins, s, e, a.r0, a.r1, lo, hi);
-    int L = wave_list_load(a.ent, seg, e - s, list);
+    int L = wave_list_stage(a.ent, seg, h.x, list);
     if (INDEXED) wave_index_set(list, L, at, a.pmask);
     bool dirty = false;
     for (unsigned k0 = lo; k0 < hi; k0 += WAVE) {
@@ -530,7 +541,7 @@ __global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
       const int r = small ? r_small : (k < hi ? a.apply_pins[k] : INT_MAX);
       if (r >= a.r0 && r < a.r1) {
         dd = a.slot_dec[r - a.r0];
-        if (dd >= 0) oo = a.slot_own[r - a.r0];
+        oo = a.slot_own[r - a.r0];
       }
       unsigned long long m = __ballot(dd >= 0);
       while (m) {
