@@ -404,7 +404,11 @@ __global__ void __launch_bounds__(256) k_bal_scan(int* __restrict__ hist, long l
 // BAL step 2, one wavefront per tile: a candidate into an over-subscribed part
 // d is admitted iff frozen_size[d] + (weight of earlier candidates into d) + w
 // <= cap; every other candidate is admitted. Commits part vector and sizes of
-// admitted moves; clears the tile's histogram cells of non-over parts.
+// admitted moves; clears the tile's histogram cells of non-over parts. AGG:
+// the block sums its size changes per part in LDS first, so that sizes takes
+// one atomic per part and block (same-address atomics from every wave would
+// queue at one L2 slice).
+template <bool AGG>
 __global__ void __launch_bounds__(256) k_bal_admit(int* __restrict__ slot_dec, const int* __restrict__ slot_own,
                                                    const int* __restrict__ order, const int* __restrict__ vwgt,
                                                    int r0, int nslots, int ntiles, int K,
@@ -420,7 +424,12 @@ __global__ void __launch_bounds__(256) k_bal_admit(int* __restrict__ slot_dec, c
   // the block's admitted moves, added to the global count once per block
   // (same-address atomics from every wave would queue at one L2 slice)
   __shared__ unsigned long long s_moves;
+  // AGG: the block's size changes per part, added to sizes once per block
+  extern __shared__ __align__(16) unsigned char lds[];
+  unsigned long long* s_dsz = reinterpret_cast<unsigned long long*>(lds);
   if (threadIdx.x == 0) s_moves = 0;
+  if (AGG)
+    for (int d = (int)threadIdx.x; d < K; d += (int)blockDim.x) s_dsz[d] = 0;
   __syncthreads();
   if (blockIdx.x == 0)
     for (int d = (int)threadIdx.x; d < K; d += (int)blockDim.x) cand_tot[d] = 0;
@@ -452,14 +461,24 @@ __global__ void __launch_bounds__(256) k_bal_admit(int* __restrict__ slot_dec, c
     }
     if (adm) {
       part[v] = d;
-      atomic_add_i64(&sizes[slot_own[slot]], -w);
-      atomic_add_i64(&sizes[d], w);
+      if (AGG) {
+        atomicAdd(&s_dsz[slot_own[slot]], (unsigned long long)-w);
+        atomicAdd(&s_dsz[d], (unsigned long long)w);
+      } else {
+        atomic_add_i64(&sizes[slot_own[slot]], -w);
+        atomic_add_i64(&sizes[d], w);
+      }
     }
     const unsigned long long am = __ballot(adm);
     if (lane == 0 && am) atomicAdd(&s_moves, (unsigned long long)__popcll(am));
   }
   __syncthreads();
   if (threadIdx.x == 0 && s_moves) atomicAdd(moves, s_moves);
+  if (AGG)
+    for (int d = (int)threadIdx.x; d < K; d += (int)blockDim.x) {
+      const long long x = (long long)s_dsz[d];
+      if (x) atomic_add_i64(&sizes[d], x);
+    }
 }
 
 struct ApplyArgs {
@@ -708,9 +727,16 @@ fp_status pass_tv(fp_ctx* ctx, const RefinePlan& p) {
       k_bal_scan<<<std::max(1, std::min(K, 512)), 256, lds_s, ctx->stream>>>(
           ctx->hist, ctx->pref, ntiles, K, ctx->sizes_frozen, ctx->cand_tot, sched->cap, ctx->over_idx,
           ctx->over_count);
-      k_bal_admit<<<std::max(1, std::min((ntiles + 3) / 4, 4096)), 256, 0, ctx->stream>>>(
-          ctx->slot_dec, ctx->slot_own, ctx->order, ctx->vwgt, r0, nslots, ntiles, K, ctx->sizes_frozen, sched->cap,
-          ctx->over_idx, ctx->over_count, ctx->pref, ctx->hist, ctx->part, ctx->sizes, ctx->cand_tot, ctx->moves);
+      // the size changes are summed per block in LDS where K parts fit
+      const int agrid = std::max(1, std::min((ntiles + 3) / 4, 4096));
+      if ((size_t)K * sizeof(long long) <= (size_t)LDS_BUDGET)
+        k_bal_admit<true><<<agrid, 256, (size_t)K * sizeof(long long), ctx->stream>>>(
+            ctx->slot_dec, ctx->slot_own, ctx->order, ctx->vwgt, r0, nslots, ntiles, K, ctx->sizes_frozen, sched->cap,
+            ctx->over_idx, ctx->over_count, ctx->pref, ctx->hist, ctx->part, ctx->sizes, ctx->cand_tot, ctx->moves);
+      else
+        k_bal_admit<false><<<agrid, 256, 0, ctx->stream>>>(
+            ctx->slot_dec, ctx->slot_own, ctx->order, ctx->vwgt, r0, nslots, ntiles, K, ctx->sizes_frozen, sched->cap,
+            ctx->over_idx, ctx->over_count, ctx->pref, ctx->hist, ctx->part, ctx->sizes, ctx->cand_tot, ctx->moves);
     }
     ApplyArgs ap{};
     ap.net_ptr = ctx->net_ptr;
