@@ -543,38 +543,6 @@ __device__ __forceinline__ void wave_for_touched(const int* __restrict__ touched
   }
 }
 
-// wave_for_touched with a look-ahead: h = pre(n, s, e) issues the loads of a
-// net's head, and is called for the next net before f(n, s, e, h) runs on
-// the current one, so that those loads are in flight during f.
-template <class P, class F>
-__device__ __forceinline__ void wave_for_touched_ahead(const int* __restrict__ touched, int ntouched,
-                                                       const unsigned* __restrict__ net_ptr, int first, int stride,
-                                                       P pre, F f) {
-  const int lane = lane_id();
-  for (long long b0 = first; b0 < ntouched; b0 += (long long)WAVE * stride) {
-    const long long ij = b0 + (long long)lane * stride;
-    int nj = 0;
-    unsigned sj = 0, ej = 0;
-    if (ij < ntouched) {
-      nj = touched[ij];
-      sj = net_ptr[nj];
-      ej = net_ptr[nj + 1];
-    }
-    const int cnt = (int)min((long long)WAVE, ((long long)ntouched - b0 + stride - 1) / stride);
-    auto cur = pre(__builtin_amdgcn_readlane(nj, 0), (unsigned)__builtin_amdgcn_readlane((int)sj, 0),
-                   (unsigned)__builtin_amdgcn_readlane((int)ej, 0));
-    for (int j = 0; j < cnt; j++) {
-      decltype(cur) nxt = cur;
-      if (j + 1 < cnt)
-        nxt = pre(__builtin_amdgcn_readlane(nj, j + 1), (unsigned)__builtin_amdgcn_readlane((int)sj, j + 1),
-                  (unsigned)__builtin_amdgcn_readlane((int)ej, j + 1));
-      f(__builtin_amdgcn_readlane(nj, j), (unsigned)__builtin_amdgcn_readlane((int)sj, j),
-        (unsigned)__builtin_amdgcn_readlane((int)ej, j), cur);
-      cur = nxt;
-    }
-  }
-}
-
 // Writes a list back to both the packed segment and its evaluation view.
 __device__ __forceinline__ void wave_list_store(unsigned* ent, unsigned short* ev, unsigned seg, const unsigned* list,
                                                 int L, int pb, unsigned pmask) {

@@ -18,9 +18,10 @@
 //                dense index; for each, an exclusive scan over tiles
 //   k_bal_admit  BAL: prefix admission per part in ordering order, commit of
 //                part vector and sizes
-//   k_apply      one wavefront per touched net replays the round's moves on
-//                its list in ordering order (plus, for REF, the commit of the
-//                part vector and the source-part sizes in extra blocks)
+//   k_apply      one wavefront per touched net (one lane per net of at most 8
+//                pins) replays the round's moves on its list in ordering
+//                order (plus, for REF, the commit of the part vector and the
+//                source-part sizes in extra blocks)
 #include "fp_internal.h"
 
 namespace fpe {
@@ -38,6 +39,7 @@ constexpr int NETS_IN_FLIGHT = FP_NETS_IN_FLIGHT;  // segment head windows loade
 constexpr int LONG_TASKS = FP_LONG_TASKS;  // windows of long lists (any of the chunk's nets) loaded together
 constexpr int SCAN_RUN = 16;     // consecutive tiles per thread in the BAL scan
 constexpr int APPLY_BLOCK = 256;
+constexpr int SMALL_NET = 8;     // nets of at most this many pins are replayed one per lane
 constexpr int APPLY_NET_BLOCKS = 2048;
 
 struct EvalArgs {
@@ -508,6 +510,77 @@ struct ApplyArgs {
 // (TV/partition.cpp:818-830: the destination's size is left unchanged).
 // INDEXED: each wave keeps a part -> position index of its list in LDS (2
 // bytes per part after the lists), for K where it fits.
+// bytes of the per-wave lists (and indexes) at the start of the apply's LDS
+template <bool INDEXED>
+__host__ __device__ __forceinline__ size_t apply_list_bytes(int K, int nw) {
+  return (size_t)nw * (size_t)K * (sizeof(unsigned) + (INDEXED ? sizeof(unsigned short) : 0));
+}
+
+// One lane replays the round's moves on a net of at most SMALL_NET pins, in
+// the rank order of its pins, on a copy of the list in the lane's own LDS
+// slots: the same edits, in the same order, as the wavefront replay (remove
+// = decrement or swap-with-last, add = increment or append). The list and the
+// ranks load together, then the moves of the ranked pins.
+__device__ __forceinline__ void lane_replay_small(const ApplyArgs& a, int n, unsigned s, unsigned e, unsigned* my) {
+  const unsigned np = e - s;
+  const unsigned seg = s + (unsigned)n;
+  int r[SMALL_NET];
+  unsigned x[SMALL_NET + 1];
+#pragma unroll
+  for (int i = 0; i < SMALL_NET; i++) r[i] = (unsigned)i < np ? a.apply_pins[s + (unsigned)i] : INT_MAX;
+#pragma unroll
+  for (int t = 0; t <= SMALL_NET; t++) x[t] = (unsigned)t <= np ? a.ent[seg + (unsigned)t] : 0u;
+  int dd[SMALL_NET], oo[SMALL_NET];
+#pragma unroll
+  for (int i = 0; i < SMALL_NET; i++) {
+    const bool in = r[i] >= a.r0 && r[i] < a.r1;
+    dd[i] = in ? a.slot_dec[r[i] - a.r0] : -1;
+    oo[i] = in ? a.slot_own[r[i] - a.r0] : 0;
+  }
+  int L = (int)x[0];
+#pragma unroll
+  for (int t = 0; t < SMALL_NET; t++)
+    if (t < L) my[t] = x[t + 1];
+  bool any = false;
+#pragma unroll
+  for (int i = 0; i < SMALL_NET; i++) any |= dd[i] >= 0;
+  if (!any) return;
+  const unsigned inc = 1u << a.pb;
+#pragma unroll
+  for (int i = 0; i < SMALL_NET; i++) {
+    if (dd[i] < 0) continue;
+    const int p = oo[i], q = dd[i];
+    int jp = -1;
+    for (int t = 0; t < L; t++)
+      if ((int)(my[t] & a.pmask) == p) jp = t;
+    if (jp >= 0) {
+      const unsigned y = my[jp];
+      if ((y >> a.pb) > 1u) {
+        my[jp] = y - inc;
+      } else {
+        my[jp] = my[L - 1];
+        L--;
+      }
+    }
+    int jq = -1;
+    for (int t = 0; t < L; t++)
+      if ((int)(my[t] & a.pmask) == q) jq = t;
+    if (jq >= 0) {
+      my[jq] += inc;
+    } else {
+      my[L] = inc | (unsigned)q;
+      L++;
+    }
+  }
+  for (int t = 0; t < L; t++) {
+    const unsigned y = my[t];
+    a.ent[seg + 1 + (unsigned)t] = y;
+    a.ev[seg + 1 + (unsigned)t] = ev_word(y, a.pb, a.pmask);
+  }
+  a.ent[seg] = (unsigned)L;
+  a.ev[seg] = (unsigned short)L;
+}
+
 template <bool INDEXED>
 __global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
@@ -530,6 +603,9 @@ __global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
   unsigned* list = reinterpret_cast<unsigned*>(lds) + (size_t)wave * (size_t)a.K;
   unsigned short* at = reinterpret_cast<unsigned short*>(reinterpret_cast<unsigned*>(lds) + (size_t)nw * (size_t)a.K) +
                        (size_t)wave * (size_t)a.K;
+  // per lane: SMALL_NET + 1 words (the odd stride keeps the lanes on distinct banks)
+  unsigned* my = reinterpret_cast<unsigned*>(lds + apply_list_bytes<INDEXED>(a.K, nw)) +
+                 (size_t)(wave * WAVE + lane) * (size_t)(SMALL_NET + 1);
   if (INDEXED) wave_index_init(at, a.K);
   const int ntouched = *a.tcount;
   // a net's head: the first 64 words of its segment and, for a net of at
@@ -544,8 +620,7 @@ __global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
     h.r = e - s <= (unsigned)WAVE && (unsigned)lane < e - s ? a.apply_pins[s + (unsigned)lane] : INT_MAX;
     return h;
   };
-  wave_for_touched_ahead(a.touched, ntouched, a.net_ptr, (int)blockIdx.x * nw + wave, a.net_blocks * nw, head,
-                         [&](int n, unsigned s, unsigned e, const NetHead& h) {
+  auto replay_net = [&](int n, unsigned s, unsigned e, const NetHead& h) {
     const unsigned seg = s + (unsigned)n;
     const bool small = e - s <= (unsigned)WAVE;
     const int r_small = h.r;
@@ -580,17 +655,71 @@ __global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
     if (dirty) wave_list_store(a.ent, a.ev, seg, list, L, a.pb, a.pmask);
     if (INDEXED) wave_index_clear(list, L, at, a.pmask);
     wave_lds_sync();
-  });
+  };
+  // the wave's share of the touched nets, 64 at a time, one per lane: nets of
+  // at most SMALL_NET pins are replayed by their lanes, the others by the
+  // whole wavefront one after another, the next one's head loading meanwhile
+  const int stride = a.net_blocks * nw;
+  for (long long b0 = (int)blockIdx.x * nw + wave; b0 < ntouched; b0 += (long long)WAVE * stride) {
+    const long long ij = b0 + (long long)lane * stride;
+    int nj = 0;
+    unsigned sj = 0, ej = 0;
+    if (ij < ntouched) {
+      nj = a.touched[ij];
+      sj = a.net_ptr[nj];
+      ej = a.net_ptr[nj + 1];
+    }
+    const bool sm = ij < ntouched && ej - sj <= (unsigned)SMALL_NET;
+    unsigned long long big = __ballot(ij < ntouched && !sm);
+    auto net_at = [&](int j, int& n, unsigned& s, unsigned& e) {
+      n = __builtin_amdgcn_readlane(nj, j);
+      s = (unsigned)__builtin_amdgcn_readlane((int)sj, j);
+      e = (unsigned)__builtin_amdgcn_readlane((int)ej, j);
+    };
+    int j = -1, n = 0;
+    unsigned s = 0, e = 0;
+    NetHead cur{0u, INT_MAX};
+    if (big) {
+      j = __builtin_amdgcn_readfirstlane(__ffsll((long long)big) - 1);
+      big &= big - 1;
+      net_at(j, n, s, e);
+      cur = head(n, s, e);
+    }
+    if (sm) lane_replay_small(a, nj, sj, ej, my);
+    while (j >= 0) {
+      int jn = -1, nn = 0;
+      unsigned sn = 0, en = 0;
+      NetHead nxt = cur;
+      if (big) {
+        jn = __builtin_amdgcn_readfirstlane(__ffsll((long long)big) - 1);
+        big &= big - 1;
+        net_at(jn, nn, sn, en);
+        nxt = head(nn, sn, en);
+      }
+      replay_net(n, s, e, cur);
+      j = jn;
+      n = nn;
+      s = sn;
+      e = en;
+      cur = nxt;
+    }
+  }
 }
 
 // the apply's LDS: a K-entry list per wave, and the part -> position index
 // where it fits the budget
+// LDS of the apply: per wave a K-entry list and, where it fits the budget,
+// the part -> position index; then SMALL_NET + 1 words per lane
+constexpr size_t APPLY_SMALL_BYTES = (size_t)APPLY_BLOCK * (SMALL_NET + 1) * sizeof(unsigned);
+
 bool apply_indexed(int K) {
-  return (size_t)(APPLY_BLOCK / WAVE) * (size_t)K * (sizeof(unsigned) + sizeof(unsigned short)) <= (size_t)LDS_BUDGET;
+  return apply_list_bytes<true>(K, APPLY_BLOCK / WAVE) + APPLY_SMALL_BYTES <= (size_t)LDS_BUDGET;
 }
 
 size_t apply_lds(int K) {
-  return (size_t)(APPLY_BLOCK / WAVE) * (size_t)K * (sizeof(unsigned) + (apply_indexed(K) ? sizeof(unsigned short) : 0));
+  return (apply_indexed(K) ? apply_list_bytes<true>(K, APPLY_BLOCK / WAVE)
+                           : apply_list_bytes<false>(K, APPLY_BLOCK / WAVE)) +
+         APPLY_SMALL_BYTES;
 }
 
 size_t eval_lds(int K, int waves) {
