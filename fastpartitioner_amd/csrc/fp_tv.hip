@@ -6,7 +6,7 @@
 //                software-pipelined over the wave's vertices) sweeps the lists
 //                of the vertex's nets in their 16-bit evaluation view -- one
 //                128-byte read per net brings its lambda and first 63
-//                entries, 4 nets in flight; the further windows of long
+//                entries, 3 nets in flight; the further windows of long
 //                lists follow as one task list over all of the vertex's long
 //                nets, 2 windows in flight -- into LDS histograms
 //                (pin count + first-appearance rank per part), then picks the
@@ -30,9 +30,13 @@ namespace {
 constexpr int EVAL_BLOCK = 256;  // 4 wavefronts
 constexpr int TILE = 64;         // slots per admission tile
 #ifndef FP_NETS_IN_FLIGHT
-#define FP_NETS_IN_FLIGHT 4
+#define FP_NETS_IN_FLIGHT 3
 #endif
 constexpr int NETS_IN_FLIGHT = FP_NETS_IN_FLIGHT;  // segment head windows loaded together per wave
+// nets per chunk of a vertex's net list: a multiple of NETS_IN_FLIGHT, so that
+// no group of a chunk reaches past the wave's last lane (lanes from nn on hold
+// the zero padding: lists of length 0)
+constexpr int CHUNK = (WAVE / NETS_IN_FLIGHT) * NETS_IN_FLIGHT;
 #ifndef FP_LONG_TASKS
 #define FP_LONG_TASKS 2
 #endif
@@ -145,7 +149,7 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
     }
   };
   auto load_segs = [&](int k) -> unsigned {
-    return lane < min(hdeg[k], WAVE) ? a.vtx_seg[hvb[k] + (unsigned)lane] : a.pad_seg;
+    return lane < min(hdeg[k], CHUNK) ? a.vtx_seg[hvb[k] + (unsigned)lane] : a.pad_seg;
   };
   // the first window group of the vertex about to be swept, loaded while the
   // previous vertex picks its candidate
@@ -177,8 +181,8 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
     // the vertex's nets hold the zero padding's offset, so the windows past
     // the last net see lambda = 0. The leave-gain count is kept per lane.
     int lgl = 0;
-    for (int c0 = 0; c0 < deg; c0 += WAVE) {
-      const int nn = min(WAVE, deg - c0);
+    for (int c0 = 0; c0 < deg; c0 += CHUNK) {
+      const int nn = min(CHUNK, deg - c0);
       const unsigned sg = c0 == 0 ? sg0 : (lane < nn ? a.vtx_seg[vb + (unsigned)(c0 + lane)] : a.pad_seg);
       int lmax = 0;  // longest list of the chunk
       int lj = 0;    // lane j: lambda of the chunk's net j

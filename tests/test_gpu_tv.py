@@ -108,3 +108,45 @@ def test_cn_refinement_matches_oracle(name, scale, K):
     assert np.array_equal(got.sizes, st.sizes.astype(np.int64))
     assert m["CN"] == O.metrics(hg, K, st.part)["CN"]
     assert stats.moves > 0
+
+
+def _hubs(seed=11, nov=3000, degrees=(61, 62, 63, 64, 65, 66, 127, 128, 129, 200)):
+    """Random nets of 2..12 pins, plus hub vertices 0, 1, ... that belong to
+    the given numbers of nets: degrees at and around the 64-net chunk of the
+    evaluation, so that its net groups reach the end of a chunk."""
+    rng = np.random.default_rng(seed)
+    nets = [list(rng.choice(np.arange(len(degrees), nov), size=int(rng.integers(2, 13)), replace=False))
+            for _ in range(3 * nov)]
+    for h, d in enumerate(degrees):
+        for i in rng.choice(len(nets), size=d, replace=False):
+            nets[i].append(h)
+    net_ptr = np.zeros(len(nets) + 1, dtype=np.uint32)
+    net_ptr[1:] = np.cumsum([len(n) for n in nets])
+    net_pins = np.concatenate([np.array(n) for n in nets]).astype(np.uint32)
+    src = np.array([n[0] for n in nets], dtype=np.int32)
+    return H.Hypergraph.from_nets(nov, net_ptr, net_pins, src)
+
+
+@pytest.mark.parametrize("sched,R", [(SCHED_REF, 64), (SCHED_REF, 1000), (SCHED_BAL, 256)])
+def test_tv_hub_vertices(sched, R):
+    hg = _hubs()
+    assert int(np.diff(hg.vtx_ptr)[:10].min()) == 61 and int(np.diff(hg.vtx_ptr)[:10].max()) == 200
+    K = 16
+    st, C, max_pw, min_pw = _ldg_state(hg, K)
+    cap = H.balance_cap(hg.nov, K, 0.05)
+    if sched == SCHED_BAL:
+        st = O.build_state(hg, K, st.part)
+    with Engine(0) as eng:
+        eng.upload(hg)
+        eng.set_state(K, st.part, st.ep_ends, st.ep_parts, st.ep_cnt)
+        if sched == SCHED_REF:
+            eng.refine("TV", 6, schedule(SCHED_REF, R, C, max_pw, min_pw))
+        else:
+            eng.refine("TV", 6, schedule(SCHED_BAL, R, C, cap, 0, cap))
+        got = eng.get_state()
+    if sched == SCHED_REF:
+        O.tv_refine_par(hg, st, C, max_pw, min_pw, 6, R)
+    else:
+        O.tv_refine_bal(hg, st, C, cap, 0, cap, 6, R)
+    _same_state(got, st)
+    assert np.array_equal(got.sizes, st.sizes.astype(np.int64))
