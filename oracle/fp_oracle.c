@@ -161,35 +161,40 @@ int or_metrics(int nov, int noe, const unsigned* net_ptr, const unsigned* net_pi
 
 /* LDG2 decision (TV/partition.cpp:89-145). Returns the decision, or -2 when
  * the reference would index part_sizes[-1] (max band hit while the vertex is
- * still unassigned: undefined in the reference). */
-static int ldg2(int K, int apc, const int* active, const double* scores, const int* sizes, double C, int max_pw,
-                int min_pw, int own) {
-  double max_val = 0;
-  int max_index = -1;
-  for (int i = 0; i < apc; i++) {
-    int q = active[i];
-    double penalty = 1 - (1.0 * sizes[q] / C);
-    double score = scores[q] * penalty;
-    if (score > 0) {
-      if (max_index == -1 || score > max_val || (score == max_val && sizes[q] < sizes[max_index])) {
-        max_val = score;
-        max_index = q;
-      }
-    }
+ * still unassigned: undefined in the reference). One body serves int part
+ * sizes (the reference's own) and int64 weighted sizes; the score is the same
+ * double expression either way (1.0 * size converts exactly below 2^53). */
+#define DEFINE_LDG2(NAME, T, T_MAX)                                                                   \
+  static int NAME(int K, int apc, const int* active, const double* scores, const T* sizes, double C, \
+                  T max_pw, T min_pw, int own) {                                                      \
+    double max_val = 0;                                                                               \
+    int max_index = -1;                                                                               \
+    for (int i = 0; i < apc; i++) {                                                                   \
+      int q = active[i];                                                                              \
+      double penalty = 1 - (1.0 * sizes[q] / C);                                                      \
+      double score = scores[q] * penalty;                                                             \
+      if (score > 0) {                                                                                \
+        if (max_index == -1 || score > max_val || (score == max_val && sizes[q] < sizes[max_index])) { \
+          max_val = score;                                                                            \
+          max_index = q;                                                                              \
+        }                                                                                             \
+      }                                                                                               \
+    }                                                                                                 \
+    if (max_index == -1) {                                                                            \
+      T min_load = T_MAX;                                                                             \
+      for (int i = 0; i < K; i++)                                                                     \
+        if (sizes[i] < min_load) { min_load = sizes[i]; max_index = i; }                              \
+    }                                                                                                 \
+    int d = max_index;                                                                                \
+    if (sizes[d] > max_pw) {                                                                          \
+      d = own;                                                                                        \
+      if (d < 0) return -2;                                                                           \
+    }                                                                                                 \
+    if (sizes[d] < min_pw && own != -1) d = own;                                                      \
+    return d;                                                                                         \
   }
-  if (max_index == -1) {
-    int min_load = INT_MAX;
-    for (int i = 0; i < K; i++)
-      if (sizes[i] < min_load) { min_load = sizes[i]; max_index = i; }
-  }
-  int d = max_index;
-  if (sizes[d] > max_pw) {
-    d = own;
-    if (d < 0) return -2;
-  }
-  if (sizes[d] < min_pw && own != -1) d = own;
-  return d;
-}
+DEFINE_LDG2(ldg2, int, INT_MAX)
+DEFINE_LDG2(ldg2_w, long long, LLONG_MAX)
 
 int or_tv_ldg(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
               const int* ordering, int K, double C, int max_pw, int min_pw, int par_skip, int* part, int* ep_ends,
@@ -284,11 +289,12 @@ int or_tv_refine_serial(int nov, int noe, const unsigned* net_ptr, const unsigne
  * Scores start fresh for every vertex (the per-thread marker of partition_par
  * is reset each pass and each vertex is visited once per pass,
  * TV/partition.cpp:714). Returns the LDG2 decision and writes the gain test
- * terms. sizes_i is the int view used by LDG2. */
+ * terms. Part sizes are int64 so that the weighted balanced schedule
+ * evaluates through this same function. */
 static int tv_eval(int v, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets, int K,
-                   double C, int max_pw, int min_pw, const int* part, const int* ep_ends, const int* ep_parts,
-                   const int* ep_cnt, const int* sizes_i, int* marker, int* active, double* scores, int* lg,
-                   int* al) {
+                   double C, long long max_pw, long long min_pw, const int* part, const int* ep_ends,
+                   const int* ep_parts, const int* ep_cnt, const long long* sizes, int* marker, int* active,
+                   double* scores, int* lg, int* al) {
   int own = part[v];
   int apc = 0;
   for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
@@ -301,7 +307,7 @@ static int tv_eval(int v, const unsigned* net_ptr, const unsigned* vtx_ptr, cons
     }
   }
   scores[own]--;
-  int d = ldg2(K, apc, active, scores, sizes_i, C, max_pw, min_pw, own);
+  int d = ldg2_w(K, apc, active, scores, sizes, C, max_pw, min_pw, own);
   /* reset this vertex's marks so the next vertex starts fresh */
   for (int i = 0; i < apc; i++) marker[active[i]] = -1;
   *lg = 0; *al = 0;
@@ -329,18 +335,20 @@ int or_tv_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* 
   int* active = (int*)malloc(sizeof(int) * (size_t)K);
   double* scores = (double*)calloc((size_t)K, sizeof(double));
   int* dec = (int*)malloc(sizeof(int) * (size_t)R);
-  if (!marker || !active || !scores || !dec) return 2;
+  long long* frozen = (long long*)malloc(sizeof(long long) * (size_t)K);
+  if (!marker || !active || !scores || !dec || !frozen) return 2;
   for (int k = 0; k < K; k++) marker[k] = -1;
   for (int it = 0; it < passes; it++) {
     for (int r0 = 0; r0 < nov; r0 += R) {
       int r1 = r0 + R < nov ? r0 + R : nov;
       /* compute phase on the frozen state (TV/partition.cpp:725-805) */
+      for (int k = 0; k < K; k++) frozen[k] = sizes[k];
       for (int z = r0; z < r1; z++) {
         int v = ordering[z];
         int lg, al;
         int d = tv_eval(v, net_ptr, vtx_ptr, vtx_nets, K, C, max_pw, min_pw, part, ep_ends, ep_parts, ep_cnt,
-                        sizes, marker, active, scores, &lg, &al);
-        if (d == -2) { free(marker); free(active); free(scores); free(dec); return 3; }
+                        frozen, marker, active, scores, &lg, &al);
+        if (d == -2) { free(marker); free(active); free(scores); free(dec); free(frozen); return 3; }
         if (gains) gains[v] = (d == part[v]) ? INT_MIN : lg - al;
         dec[z - r0] = (d != part[v] && lg >= al) ? d : -1;
       }
@@ -360,35 +368,19 @@ int or_tv_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* 
       }
     }
   }
-  free(marker); free(active); free(scores); free(dec);
+  free(marker); free(active); free(scores); free(dec); free(frozen);
   return 0;
 }
 
-/* LDG2 on int64 weighted sizes (same double expression, TV/partition.cpp:103-104) */
-static int ldg2_w(int K, int apc, const int* active, const double* scores, const long long* sizes, double C,
-                  long long max_pw, long long min_pw, int own) {
-  double max_val = 0;
-  int max_index = -1;
-  for (int i = 0; i < apc; i++) {
-    int q = active[i];
-    double penalty = 1 - (1.0 * (double)sizes[q] / C);
-    double score = scores[q] * penalty;
-    if (score > 0) {
-      if (max_index == -1 || score > max_val || (score == max_val && sizes[q] < sizes[max_index])) {
-        max_val = score;
-        max_index = q;
-      }
-    }
-  }
-  if (max_index == -1) {
-    long long min_load = LLONG_MAX;
-    for (int i = 0; i < K; i++)
-      if (sizes[i] < min_load) { min_load = sizes[i]; max_index = i; }
-  }
-  int d = max_index;
-  if (sizes[d] > max_pw) d = own;
-  if (sizes[d] < min_pw) d = own;
-  return d;
+/* Test hook: a nonzero code makes the balanced admission below deliberately
+ * wrong, so that tests can show their checks catch such a mistake. 0 = the
+ * rule as specified. */
+static int g_plant = 0;
+void or_plant(int code) { g_plant = code; }
+
+void or_apply_move(int v, int p, int q, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                   int* ep_ends, int* ep_parts, int* ep_cnt) {
+  move_vertex(v, p, q, net_ptr, vtx_ptr, vtx_nets, ep_ends, ep_parts, ep_cnt);
 }
 
 int or_tv_refine_bal(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
@@ -407,49 +399,30 @@ int or_tv_refine_bal(int nov, int noe, const unsigned* net_ptr, const unsigned* 
   for (int it = 0; it < passes; it++) {
     for (int r0 = 0; r0 < nov; r0 += R) {
       int r1 = r0 + R < nov ? r0 + R : nov;
+      /* decisions and gain test of the reference round schedule on the frozen state */
       for (int z = r0; z < r1; z++) {
         int v = ordering[z];
-        int own = part[v];
-        int apc = 0;
-        for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
-          int n = (int)vtx_nets[i];
-          int s = (int)net_ptr[n];
-          for (int j = 0; j < ep_ends[n]; j++) {
-            int q = ep_parts[s + j];
-            if (marker[q] != v) { scores[q] = 1; marker[q] = v; active[apc++] = q; }
-            else scores[q]++;
-          }
-        }
-        scores[own]--;
-        int d = ldg2_w(K, apc, active, scores, sizes, C, max_pw, min_pw, own);
-        for (int i = 0; i < apc; i++) marker[active[i]] = -1;
-        dec[z - r0] = -1;
-        if (gains) gains[v] = INT_MIN;
-        if (d == own) continue;
-        int lg = 0, al = 0;
-        for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
-          int n = (int)vtx_nets[i];
-          int s = (int)net_ptr[n];
-          int found = 0;
-          for (int j = 0; j < ep_ends[n]; j++) {
-            int t = ep_parts[s + j];
-            if (t == own && ep_cnt[s + j] == 1) lg++;
-            else if (t == d) found = 1;
-          }
-          if (!found) al++;
-        }
-        if (gains) gains[v] = lg - al;
-        if (lg >= al) dec[z - r0] = d;
+        int lg, al;
+        int d = tv_eval(v, net_ptr, vtx_ptr, vtx_nets, K, C, max_pw, min_pw, part, ep_ends, ep_parts, ep_cnt,
+                        sizes, marker, active, scores, &lg, &al);
+        if (d == -2) { free(marker); free(active); free(scores); free(dec); free(inflow); return 3; }
+        if (gains) gains[v] = (d == part[v]) ? INT_MIN : lg - al;
+        dec[z - r0] = (d != part[v] && lg >= al) ? d : -1;
       }
-      /* per-destination prefix admission in ordering order */
+      /* per-destination prefix admission in ordering order: a candidate into d
+       * is admitted iff size[d] + (weight of every earlier candidate into d,
+       * admitted or not) + its own weight <= cap */
       for (int k = 0; k < K; k++) inflow[k] = 0;
       for (int z = r0; z < r1; z++) {
         int d = dec[z - r0];
         if (d < 0) continue;
         long long w = vwgt ? vwgt[ordering[z]] : 1;
         long long before = inflow[d];
-        inflow[d] += w;
-        if (sizes[d] + before + w > cap) dec[z - r0] = -1;
+        int admit = sizes[d] + before + w <= cap;
+        if (g_plant == 1) admit = 1;
+        if (g_plant == 3) admit = sizes[d] + before + w < cap;
+        if (g_plant != 2 || admit) inflow[d] += w;
+        if (!admit) dec[z - r0] = -1;
       }
       for (int z = r0; z < r1; z++) {
         int d = dec[z - r0];

@@ -74,15 +74,28 @@ int or_tv_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* 
                      int* part, int* ep_ends, int* ep_parts, int* ep_cnt, int* sizes, int* gains);
 
 /* Balanced deterministic round schedule (this build's production schedule;
- * no reference counterpart): same per-round decisions as or_tv_refine_par on
- * weighted part sizes, then per destination part d the candidates in
- * ordering order are admitted while sizes[d] + (weights of earlier
- * candidates into d this round) + w <= cap; admitted moves apply in ordering
- * order and update both part sizes. vwgt NULL = unit weights. */
+ * no reference counterpart): the per-round decisions and gain test are
+ * or_tv_refine_par's (the same evaluation function) on weighted part sizes;
+ * then per destination part d the candidates in ordering order are admitted
+ * while sizes[d] + (weights of every earlier candidate into d this round,
+ * admitted or not) + w <= cap, so the admitted set is the longest prefix of
+ * d's candidates that fits; admitted moves apply in ordering order and update
+ * both part sizes. vwgt NULL = unit weights. */
 int or_tv_refine_bal(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
                      const int* ordering, const int* vwgt, int K, double C, long long max_pw, long long min_pw,
                      long long cap, int passes, int R, int* part, int* ep_ends, int* ep_parts, int* ep_cnt,
                      long long* sizes, int* gains);
+
+/* The reference's move of vertex v from part p to q on the lists: every
+ * removal over v's nets, then every add (TV/partition.cpp:387-436). */
+void or_apply_move(int v, int p, int q, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                   int* ep_ends, int* ep_parts, int* ep_cnt);
+
+/* Test hook. A nonzero code plants a known mistake in or_tv_refine_bal's
+ * admission (1: admit every candidate; 2: count only admitted weight in the
+ * prefix; 3: a strict < cap test), so a test can show that its checks fail
+ * on it. 0 restores the rule. */
+void or_plant(int code);
 
 /* ---- CN ---------------------------------------------------------------- */
 /* Initial partition of CN/partition.cpp:132-221 (LDG over uncut nets). */

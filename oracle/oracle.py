@@ -141,6 +141,18 @@ def tv_refine_bal(hg, st, C, max_pw, min_pw, cap, passes, R, ordering=None, gain
     return st
 
 
+def apply_move(hg, st, v, p, q):
+    """The reference's list edits of one move (TV/partition.cpp:387-436)."""
+    lib().or_apply_move(int(v), int(p), int(q), _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
+                        _p(hg.vtx_nets, _U32P), _p(st.ep_ends, _I32P), _p(st.ep_parts, _I32P),
+                        _p(st.ep_cnt, _I32P))
+
+
+def plant(code: int):
+    """Test hook: plant a known mistake in the balanced admission (0 = none)."""
+    lib().or_plant(int(code))
+
+
 def cn_ldg(hg, K, C, ordering=None):
     st = State(hg, K)
     rc = lib().or_cn_ldg(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P), _p(hg.vtx_nets, _U32P),
@@ -327,24 +339,30 @@ def ref_available(metric: str, binary: str = "ref") -> bool:
 
 
 def run_reference(metric: str, hg, K: int, passes: int, threads: int, timeout: float = 600.0,
-                  env_extra=None, binary: str = "ref"):
+                  env_extra=None, binary: str = "ref", ordering=None):
     """Run the reference's own partition()/partition_par() on `hg` and return
     the states it reached: dict with init_part/init_ends/init_parts and
     part/ends/parts/cnt (after refinement), the wall time and the
     'Refinement took' seconds it printed.
 
     binary="dropin" with threads=-1 runs the reference's partition_gpu(),
-    whose device entry point is served by libfastpart_legacy_<metric>.so."""
+    whose device entry point is served by libfastpart_legacy_<metric>.so.
+    ordering: a vertex permutation handed to the reference as its visiting
+    order (default: the identity, as the reference's drivers build it)."""
     exe = os.path.join(REF_DIR, f"{binary}_{metric}")
     with tempfile.TemporaryDirectory() as d:
         inst = os.path.join(d, "g.fphg")
         out = os.path.join(d, "g.fprs")
         hg.write_fphg(inst)
+        cmd = [exe, inst, str(K), str(passes), str(threads), out]
+        if ordering is not None:
+            opath = os.path.join(d, "g.ord")
+            np.ascontiguousarray(ordering, dtype="<i4").tofile(opath)
+            cmd.append(opath)
         env = dict(os.environ)
         if env_extra:
             env.update(env_extra)
-        p = subprocess.run([exe, inst, str(K), str(passes), str(threads), out], capture_output=True, text=True,
-                           timeout=timeout, env=env)
+        p = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, env=env)
         if p.returncode != 0:
             raise RuntimeError(f"{exe} failed rc={p.returncode}: {p.stderr[-2000:]}")
         with open(out, "rb") as f:

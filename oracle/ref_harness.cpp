@@ -15,13 +15,17 @@
 // and forward to the real function.
 //
 // The vertex->nets CSR is built exactly as TV/tv.cpp:76-100 does (counting
-// sort over the net->pins CSR); ordering is the identity (TV/tv.cpp:122-128).
+// sort over the net->pins CSR); ordering is the identity (TV/tv.cpp:122-128)
+// unless an ordering file is given.
 // net_appearence is given NET degrees (consistent arrays), not the vertex
 // degrees tv.cpp passes (TV/tv.cpp:71), so MGV_Scratch's inverse is valid.
 //
-// Usage: ref_<m> <instance.fphg> <K> <passes> <threads> <out.fprs>
+// Usage: ref_<m> <instance.fphg> <K> <passes> <threads> <out.fprs> [ordering.i32]
 //   threads == 0 -> partition()            (serial schedule)
 //   threads >= 1 -> partition_par(threads)  (round schedule, OpenMP)
+//   ordering.i32: nov raw int32 vertex ids, the visiting order handed to the
+//   reference in place of the identity (TV/tv.cpp:122-128 builds the identity
+//   or, with -o 1, a BFS order).
 //
 // Built with -DREF_DROPIN (binaries dropin_<m>): threads == -1 runs the
 // reference's partition_gpu() (TV/partition.cpp:984, CN/partition.cpp:1599),
@@ -86,7 +90,7 @@ extern "C" int __wrap__Z10TV_RefinedPiS_ii(int* ep_starts, int* ep_cnt, int noe,
 static void die(const char* m) { fprintf(stderr, "ref_harness: %s\n", m); exit(2); }
 
 int main(int argc, char** argv) {
-  if (argc != 6) die("usage: ref_<m> <instance.fphg> <K> <passes> <threads> <out.fprs>");
+  if (argc != 6 && argc != 7) die("usage: ref_<m> <instance.fphg> <K> <passes> <threads> <out.fprs> [ordering.i32]");
   FILE* f = fopen(argv[1], "rb");
   if (!f) die("cannot open instance");
   char magic[4];
@@ -124,6 +128,16 @@ int main(int argc, char** argv) {
   for (int i = 0; i < noe; i++) edge_w[i] = 1;
   int* ordering = new int[nov];
   for (int i = 0; i < nov; i++) ordering[i] = i;
+  if (argc == 7) {
+    FILE* fo = fopen(argv[6], "rb");
+    if (!fo || fread(ordering, 4, nov, fo) != (size_t)nov) die("short ordering file");
+    fclose(fo);
+    vector<char> seen(nov, 0);
+    for (int i = 0; i < nov; i++) {
+      if (ordering[i] < 0 || ordering[i] >= nov || seen[ordering[i]]) die("ordering is not a permutation");
+      seen[ordering[i]] = 1;
+    }
+  }
   string fname = string(argv[5]) + ".ref";
   int refi_opt = passes > 0 ? 1 : 0, refi_iter = passes;
 

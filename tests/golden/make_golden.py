@@ -23,7 +23,8 @@ from oracle import oracle as O  # noqa: E402
 OUT = os.path.dirname(os.path.abspath(__file__))
 
 # (name, reference binary, K, passes, threads: 0 = partition(), 1 = partition_par
-#  with one thread, instance)
+#  with one thread, instance). Names ending in "_perm" hand the reference a
+#  seeded random visiting order (stored in the fixture as `ordering`).
 CASES = [
     ("tv_serial", "tv", 8, 5, 0, "fixed5"),
     ("tv_par1", "tv", 8, 5, 1, "fixed5"),
@@ -35,6 +36,9 @@ CASES = [
     ("tm_par1", "tm", 64, 4, 1, "sparse"),
     ("msm_serial", "msm", 256, 6, 0, "sparse2"),
     ("msm_par1", "msm", 256, 6, 1, "sparse2"),
+    ("msrv_serial", "msrv", 4, 6, 0, "thin"),
+    ("msrv_par1_perm", "msrv", 6, 6, 1, "mid"),
+    ("tv_par1_perm", "tv", 8, 5, 1, "fixed5"),
 ]
 PRINTED = ("TV", "CN", "TM", "MGM", "MGV", "MGAV")
 
@@ -46,6 +50,12 @@ def instance(kind):
     if kind == "sparse":
         # 1,500 x 1,500: diagonal + 1..3 uniform off-diagonals (TM moves at K = 64)
         return H.colnet(1500, H.DEG_UNIFORM, 1, 3, seed=3)
+    if kind == "thin":
+        # 1,500 x 1,500: diagonal + 1..2 uniform off-diagonals (MSRV moves at K = 4)
+        return H.colnet(1500, H.DEG_UNIFORM, 1, 2, seed=2)
+    if kind == "mid":
+        # 2,000 x 2,000: diagonal + 2..5 uniform off-diagonals
+        return H.colnet(2000, H.DEG_UNIFORM, 2, 5, seed=2)
     # 3,000 x 3,000: diagonal + 1..3 uniform off-diagonals (MSM moves at K = 256)
     return H.colnet(3000, H.DEG_UNIFORM, 1, 3, seed=2)
 
@@ -57,13 +67,17 @@ def main(only=None):
         hg = instance(kind)
         if not O.ref_available(m):
             raise SystemExit(f"oracle/_ref/ref_{m} not built: run `make -C oracle ref`")
-        r = O.run_reference(m, hg, K, passes, threads)
+        ordering = None
+        if name.endswith("_perm"):
+            ordering = np.random.default_rng(K * 1000 + passes).permutation(hg.nov).astype(np.int32)
+        r = O.run_reference(m, hg, K, passes, threads, ordering=ordering)
         printed = {}
         for key in PRINTED:
             mm = re.findall(rf"^After Refinement {key} : (-?\d+)$", r["stdout"], re.M)
             printed[key] = int(mm[-1])
+        extra = {} if ordering is None else {"ordering": ordering}
         np.savez_compressed(
-            os.path.join(OUT, f"{name}.npz"),
+            os.path.join(OUT, f"{name}.npz"), **extra,
             nov=hg.nov, net_ptr=hg.net_ptr, net_pins=hg.net_pins, src=hg.src, K=K, passes=passes,
             threads=threads, init_part=r["init_part"], init_ends=r["init_ends"], init_parts=r["init_parts"],
             part=r["part"], ends=r["ends"], parts=r["parts"], cnt=r["cnt"],

@@ -27,12 +27,21 @@ def _hg_from(z):
     return H.Hypergraph.from_nets(int(z["nov"]), z["net_ptr"], z["net_pins"], z["src"])
 
 
-def _run_oracle(kind, hg, K, passes, threads):
+def _run_oracle(kind, hg, K, passes, threads, ordering=None):
     """Oracle state after the initial partition and after refinement, for the
-    reference driver mode (threads 0 = partition(), 1 = partition_par(1))."""
+    reference driver mode (threads 0 = partition(), 1 = partition_par(1)),
+    visiting vertices in `ordering` (default: the identity)."""
     C, max_pw, min_pw = H.ref_constants(hg.nov, K)
+    if kind == "msrv":
+        st = O.msrv_ldg(hg, K, C, ordering=ordering)
+        init = st.copy()
+        if threads == 0:
+            O.msrv_refine_serial(hg, st, C, passes, ordering=ordering)
+        else:
+            O.msrv_refine_par(hg, st, C, passes, 256 * threads, ordering=ordering)
+        return init, st
     if kind == "tm":
-        st = O.tm_init(hg, K, C)
+        st = O.tm_init(hg, K, C, ordering=ordering)
         init = st.copy()
         if threads == 0:
             O.tm_refine_serial(hg, st, C, passes)
@@ -40,7 +49,7 @@ def _run_oracle(kind, hg, K, passes, threads):
             O.tm_refine_par(hg, st, C, passes, hg.nov)
         return init, st
     if kind == "msm":
-        st = O.msm_init(hg, K, C)
+        st = O.msm_init(hg, K, C, ordering=ordering)
         init = st.copy()
         if threads == 0:
             O.msm_refine_serial(hg, st, passes)
@@ -48,22 +57,22 @@ def _run_oracle(kind, hg, K, passes, threads):
             O.msm_refine_par(hg, st, passes, 128 * threads)
         return init, st
     if kind in ("msv", "mgv"):
-        st = O.msv_ldg(hg, K, C)
+        st = O.msv_ldg(hg, K, C, ordering=ordering)
         init = st.copy()
         if threads == 0:
-            O.msv_refine_serial(hg, st, C, passes)
+            O.msv_refine_serial(hg, st, C, passes, ordering=ordering)
         else:
             O.msv_refine_par(hg, st, C, passes, 256 * threads)
         return init, st
     if kind == "tv":
-        st, marker = O.tv_ldg(hg, K, C, max_pw, min_pw, par_skip=threads > 0)
+        st, marker = O.tv_ldg(hg, K, C, max_pw, min_pw, par_skip=threads > 0, ordering=ordering)
         init = st.copy()
         if threads == 0:
-            O.tv_refine_serial(hg, st, marker, C, max_pw, min_pw, passes)
+            O.tv_refine_serial(hg, st, marker, C, max_pw, min_pw, passes, ordering=ordering)
         else:
-            O.tv_refine_par(hg, st, C, max_pw, min_pw, passes, 8 * threads)
+            O.tv_refine_par(hg, st, C, max_pw, min_pw, passes, 8 * threads, ordering=ordering)
     else:
-        st = O.cn_ldg(hg, K, C)
+        st = O.cn_ldg(hg, K, C, ordering=ordering)
         init = st.copy()
         O.cn_refine(hg, st, passes, serial=threads == 0)
     return init, st
@@ -74,7 +83,8 @@ def test_golden_fixture(path):
     z = np.load(path)
     hg = _hg_from(z)
     kind = os.path.basename(path).split("_")[0]
-    init, st = _run_oracle(kind, hg, int(z["K"]), int(z["passes"]), int(z["threads"]))
+    ordering = z["ordering"] if "ordering" in z.files else None
+    init, st = _run_oracle(kind, hg, int(z["K"]), int(z["passes"]), int(z["threads"]), ordering)
     assert np.array_equal(init.part, z["init_part"])
     assert np.array_equal(init.ep_ends, z["init_ends"])
     assert np.array_equal(init.ep_parts, z["init_parts"])
@@ -87,21 +97,25 @@ def test_golden_fixture(path):
 
 
 def test_golden_fixtures_present():
-    assert len(glob.glob(os.path.join(GOLD, "*.npz"))) >= 10
+    names = {os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLD, "*.npz"))}
+    for kind in ("tv", "cn", "msv", "msrv", "tm", "msm"):
+        assert f"{kind}_serial" in names and any(n.startswith(f"{kind}_par1") for n in names), kind
 
 
-needs_ref = pytest.mark.skipif(not all(O.ref_available(m) for m in ("tv", "cn", "mgv", "tm", "msm")),
+needs_ref = pytest.mark.skipif(not all(O.ref_available(m) for m in ("tv", "cn", "mgv", "tm", "msm", "msrv")),
                                reason="reference binaries (oracle/_ref) not built here")
 
 
 @needs_ref
-@pytest.mark.parametrize("kind,threads", [("tv", 0), ("tv", 1), ("cn", 0), ("cn", 1), ("mgv", 0), ("mgv", 1)])
-@pytest.mark.parametrize("seed", [3, 11])
-def test_oracle_matches_reference_binary(kind, threads, seed):
+@pytest.mark.parametrize("kind,threads", [("tv", 0), ("tv", 1), ("cn", 0), ("cn", 1), ("mgv", 0), ("mgv", 1),
+                                          ("msrv", 0), ("msrv", 1)])
+@pytest.mark.parametrize("seed,permuted", [(3, False), (11, False), (5, True)])
+def test_oracle_matches_reference_binary(kind, threads, seed, permuted):
     hg = H.colnet(1500, H.DEG_UNIFORM, 2, 9, seed=seed)
     K, passes = 12, 4
-    r = O.run_reference(kind, hg, K, passes, threads)
-    init, st = _run_oracle(kind, hg, K, passes, threads)
+    ordering = np.random.default_rng(seed).permutation(hg.nov).astype(np.int32) if permuted else None
+    r = O.run_reference(kind, hg, K, passes, threads, ordering=ordering)
+    init, st = _run_oracle(kind, hg, K, passes, threads, ordering)
     assert np.array_equal(init.part, r["init_part"])
     assert np.array_equal(init.ep_parts, r["init_parts"])
     assert np.array_equal(st.part, r["part"])
@@ -113,13 +127,15 @@ def test_oracle_matches_reference_binary(kind, threads, seed):
 @needs_ref
 @pytest.mark.parametrize("kind,K,threads", [("tm", 64, 0), ("tm", 64, 1), ("tm", 128, 1), ("msm", 256, 0),
                                             ("msm", 256, 1)])
-@pytest.mark.parametrize("seed", [2, 3])
-def test_message_oracles_match_reference_binary(kind, K, threads, seed):
-    # sparse instances, where the message objectives make moves
+@pytest.mark.parametrize("seed,permuted", [(2, False), (3, False), (4, True)])
+def test_message_oracles_match_reference_binary(kind, K, threads, seed, permuted):
+    # sparse instances, where the message objectives make moves; the visiting
+    # order reaches the initial greedy (refinement runs in vertex order)
     hg = H.colnet(1500 if kind == "tm" else 3000, H.DEG_UNIFORM, 1, 3, seed=seed)
     passes = 4
-    r = O.run_reference(kind, hg, K, passes, threads)
-    init, st = _run_oracle(kind, hg, K, passes, threads)
+    ordering = np.random.default_rng(seed).permutation(hg.nov).astype(np.int32) if permuted else None
+    r = O.run_reference(kind, hg, K, passes, threads, ordering=ordering)
+    init, st = _run_oracle(kind, hg, K, passes, threads, ordering)
     assert np.array_equal(init.part, r["init_part"])
     assert np.array_equal(init.ep_parts, r["init_parts"])
     assert np.array_equal(st.part, r["part"])
@@ -331,10 +347,26 @@ def test_volume_gain_model_brute_force(seed):
     assert checked > 50
 
 
+@needs_ref
+@pytest.mark.parametrize("seed,K,lo,hi", [(2, 4, 1, 2), (7, 6, 2, 5), (9, 8, 1, 3)])
+def test_msrv_oracle_matches_reference_on_moving_instances(seed, K, lo, hi):
+    # thin instances where MSRV's strict rule lets tens of moves through, in a
+    # permuted visiting order (MSRV/partition.cpp:822 visits ordering[z])
+    hg = H.colnet(1500, H.DEG_UNIFORM, lo, hi, seed=seed)
+    ordering = np.random.default_rng(seed).permutation(hg.nov).astype(np.int32)
+    for threads in (0, 1):
+        r = O.run_reference("msrv", hg, K, 6, threads, ordering=ordering)
+        init, st = _run_oracle("msrv", hg, K, 6, threads, ordering)
+        assert np.count_nonzero(r["part"] != r["init_part"]) >= 5
+        assert np.array_equal(st.part, r["part"])
+        assert np.array_equal(st.ep_ends, r["ends"])
+        assert np.array_equal(st.ep_parts, r["parts"])
+        assert np.array_equal(st.ep_cnt, r["cnt"])
+
+
 @pytest.mark.parametrize("seed", [3, 11])
 def test_msrv_volumes_stay_canonical(seed):
-    # MSRV's reference cannot be compiled (MSRV/partition.cpp:1341), so its
-    # refinement is pinned by invariants: the incrementally kept send/receive
+    # Beyond the reference runs above: the incrementally kept send/receive
     # volumes equal the from-scratch MGAV vectors after every schedule, and
     # the objective never rises under the round schedule's strict-gain rule.
     hg = H.colnet(1500, H.DEG_UNIFORM, 2, 9, seed=seed)
@@ -361,3 +393,78 @@ def test_msv_volumes_match_scratch():
     st = O.msv_ldg(hg, K, C)
     O.msv_refine_par(hg, st, C, 4, 512)
     assert np.array_equal(O.metrics(hg, K, st.part)["sv_msv"], st.sv)
+
+
+def _bal_by_definition(hg, st, C, cap, passes, R, ordering):
+    """The balanced schedule from its definition, built on reference-pinned
+    pieces only. Per round: the candidates and their destinations are what the
+    reference round schedule (or_tv_refine_par, pinned by the tv goldens) moves
+    when it evaluates every vertex on the frozen state with max_pw = cap and
+    min_pw = 0; per destination d the admitted candidates are the longest
+    ordering-order prefix of d's candidates whose cumulative weight fits in
+    cap - size[d]; the admitted moves are applied in ordering order with the
+    reference's list edits."""
+    nov = hg.nov
+    for _ in range(passes):
+        for r0 in range(0, nov, R):
+            frozen = st.copy()
+            frozen.sizes = st.sizes.astype(np.int32)
+            O.tv_refine_par(hg, frozen, C, int(cap), 0, 1, nov, ordering=ordering)
+            verts = ordering[r0:r0 + R]
+            cand = [(v, int(frozen.part[v])) for v in verts if frozen.part[v] != st.part[v]]
+            admitted = set()
+            for d in {d for _, d in cand}:
+                into = [v for v, dd in cand if dd == d]
+                cum = np.cumsum([int(hg.vwgt[v]) for v in into])
+                fits = int(np.searchsorted(cum, cap - st.sizes[d], side="right"))
+                admitted.update(into[:fits])
+            for v, d in cand:
+                if v in admitted:
+                    own = int(st.part[v])
+                    O.apply_move(hg, st, v, own, d)
+                    st.part[v] = d
+                    st.sizes[own] -= hg.vwgt[v]
+                    st.sizes[d] += hg.vwgt[v]
+    return st
+
+
+def _bal_case(seed):
+    hg = H.colnet(600, H.DEG_UNIFORM, 1, 6, seed=seed, weights=(1, 5))
+    K = 6
+    W = hg.total_weight()
+    C, _, _ = H.ref_constants(W, K, 0.03)
+    cap = H.balance_cap(W, K, 0.0)
+    part = np.random.default_rng(seed).integers(0, K, hg.nov).astype(np.int32)
+    ordering = np.random.default_rng(seed + 100).permutation(hg.nov).astype(np.int32)
+    return hg, K, C, cap, O.build_state(hg, K, part, ordering), ordering
+
+
+def _bal_matches(seed, R, passes=3):
+    hg, K, C, cap, st0, ordering = _bal_case(seed)
+    got = O.tv_refine_bal(hg, st0.copy(), C, cap, 0, cap, passes, R, ordering=ordering)
+    want = _bal_by_definition(hg, st0.copy(), C, cap, passes, R, ordering)
+    moved = int(np.count_nonzero(want.part != st0.part))
+    same = (np.array_equal(got.part, want.part) and np.array_equal(got.sizes, want.sizes)
+            and np.array_equal(got.ep_ends, want.ep_ends) and np.array_equal(got.ep_parts, want.ep_parts)
+            and np.array_equal(got.ep_cnt, want.ep_cnt))
+    return same, moved
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("R", [37, 200, 600])
+def test_bal_admission_equals_its_definition(seed, R):
+    same, moved = _bal_matches(seed, R)
+    assert moved >= 10
+    assert same
+
+
+def test_bal_admission_check_catches_planted_mistakes():
+    # each planted admission mistake must make the definition check fail on
+    # at least one of the cases the test above runs
+    try:
+        for code in (1, 2, 3):
+            O.plant(code)
+            caught = any(not _bal_matches(seed, R)[0] for seed in (1, 2, 3) for R in (37, 200, 600))
+            assert caught, f"planted mistake {code} not caught"
+    finally:
+        O.plant(0)
