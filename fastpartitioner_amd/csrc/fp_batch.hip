@@ -36,7 +36,7 @@ struct fp_batch {
   std::vector<hipGraphExec_t> lane_execs;
   std::vector<int> lane_of;  // member -> lane
   std::vector<hipEvent_t> events;
-  std::vector<size_t> owned_at_capture;  // members' allocation counts the captured graph relies on
+  std::vector<unsigned long long> gen_at_capture;  // members' generations the captured graphs were recorded at
 };
 
 namespace {
@@ -151,7 +151,8 @@ fp_status fp_batch_refine(fp_batch* b, int passes, int tune_passes, double* devi
   B_HIP_OK(hipSetDevice(b->device));
   const size_t nm = b->members.size();
   // a captured batch graph is reused when no tuning is asked for and no
-  // member's plan allocated anything since the capture
+  // member changed anything its recorded kernel arguments rely on (device
+  // buffers, K, the list layout) since the capture
   if (tune_passes > 0) drop_graphs(b);
   // plan every member (initialisation kernels run on the member streams)
   for (size_t i = 0; i < nm; i++) {
@@ -228,8 +229,8 @@ fp_status fp_batch_refine(fp_batch* b, int passes, int tune_passes, double* devi
 
   const int rest = passes - ntune;
   if (!b->lane_execs.empty()) {
-    bool same = b->owned_at_capture.size() == nm;
-    for (size_t i = 0; same && i < nm; i++) same = b->owned_at_capture[i] == b->members[i].ctx->owned.size();
+    bool same = b->gen_at_capture.size() == nm;
+    for (size_t i = 0; same && i < nm; i++) same = b->gen_at_capture[i] == b->members[i].ctx->gen;
     if (!same) drop_graphs(b);
   }
   const int L = std::max(1, std::min(b->lanes, (int)nm));
@@ -288,8 +289,8 @@ fp_status fp_batch_refine(fp_batch* b, int passes, int tune_passes, double* devi
       B_HIP_OK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
       b->lane_execs.push_back(ex);
     }
-    b->owned_at_capture.resize(nm);
-    for (size_t i = 0; i < nm; i++) b->owned_at_capture[i] = b->members[i].ctx->owned.size();
+    b->gen_at_capture.resize(nm);
+    for (size_t i = 0; i < nm; i++) b->gen_at_capture[i] = b->members[i].ctx->gen;
   }
   if (rest > 0 && nm > 0) {
     // each lane starts after its members' planning work

@@ -71,6 +71,7 @@ fp_status dalloc_bytes(fp_ctx* ctx, void** p, size_t bytes) {
   }
   ctx->owned.push_back(q);
   ctx->dev_bytes += (long long)bytes;
+  ctx->gen++;
   *p = q;
   return FP_OK;
 }
@@ -79,6 +80,7 @@ void dfree_all(fp_ctx* ctx) {
   for (void* p : ctx->owned) (void)hipFree(p);
   ctx->owned.clear();
   ctx->dev_bytes = 0;
+  ctx->gen++;
 }
 
 hipEvent_t pooled_event(fp_ctx* ctx, size_t i) {
@@ -112,8 +114,6 @@ fp_status ensure_k(fp_ctx* ctx, int K) {
     FP_TRY(dalloc(ctx, &ctx->sizes_frozen, (size_t)K));
     FP_TRY(dalloc(ctx, &ctx->cand_tot, (size_t)K));
     FP_TRY(dalloc(ctx, &ctx->over_idx, (size_t)K));
-    FP_TRY(dalloc(ctx, &ctx->p2p, ((size_t)K * (size_t)K + 31) / 32));
-    FP_TRY(dalloc(ctx, &ctx->mvec, (size_t)5 * (size_t)K + 2));
     ctx->kcap = K;
   }
   if (ctx->K != K) {
@@ -121,7 +121,9 @@ fp_status ensure_k(fp_ctx* ctx, int K) {
     // candidate totals restart from zero
     fill_async<long long>(ctx, ctx->cand_tot, (size_t)K, 0);
     ctx->K = K;
+    ctx->gen++;
   }
+  if (ctx->pb != pb) ctx->gen++;
   ctx->pb = pb;
   ctx->pmask = (1u << pb) - 1u;
   return FP_OK;
@@ -314,6 +316,10 @@ fp_status fp_upload(fp_ctx* ctx, int nov, int noe, const unsigned* net_ptr, cons
   ctx->conn.evcap = 0;
   ctx->conn.incoff_ready = false;
   ctx->lam = nullptr;
+  ctx->met_scratch = nullptr;
+  ctx->met_words = 0;
+  ctx->met_out = nullptr;
+  ctx->met_out_cap = 0;
   if (net_ptr[0] != 0) return fail(ctx, FP_ERR_ARG, "fp_upload: net_ptr[0] != 0");
   unsigned max_net = 0;
   for (int n = 0; n < noe; n++) {

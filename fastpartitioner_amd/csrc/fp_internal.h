@@ -611,6 +611,9 @@ struct fp_ctx {
   char err[512] = {0};
   long long dev_bytes = 0;
   std::vector<void*> owned;
+  // bumped by every change that can invalidate recorded kernel arguments
+  // (device pointers, K, the list layout): allocation, freeing, a new K
+  unsigned long long gen = 0;
 
   // hypergraph
   int nov = 0, noe = 0;
@@ -690,10 +693,12 @@ struct fp_ctx {
     bool incoff_ready = false;
   } conn;
 
-  // metric scratch
-  unsigned* p2p = nullptr;
-  unsigned long long* mvec = nullptr;  // 5*K + 2
-  int* lam = nullptr;                  // noe: per-net lambda of the last fp_eval
+  // metric scratch (grown on demand by fp_eval)
+  unsigned* met_scratch = nullptr;  // work counter, message bitmap, per-block partial sums
+  size_t met_words = 0;
+  long long* met_out = nullptr;     // 6 scalars + 5 K-vectors
+  size_t met_out_cap = 0;
+  int* lam = nullptr;               // noe: per-net lambda of the last fp_eval that asked for it
 
   // timing
   bool kernel_timing = false;

@@ -1,117 +1,461 @@
 // fp_metrics.hip -- the six metrics of shared/metric.cpp from scratch on the
-// device: one wavefront per net sweeps its pins (coalesced), deduplicates the
-// pins' parts through an LDS mark array, and accumulates lambda, TV, CN and
-// the per-part int64 send / receive volume and message vectors (block-private
-// in LDS up to K = 1024, straight to HBM above).
+// device, in three launches:
+//
+//  k_metrics   wavefronts take chunks of 64 consecutive nets from a work
+//              counter. Nets of at most 64 pins are packed whole into 64-lane
+//              windows, one pin per lane (coalesced pin loads, one part gather
+//              per pin); the lanes' (net, part) keys are sorted across the
+//              wavefront (bitonic, in registers), so a key that differs from
+//              its left neighbour is the first pin of its part in its net.
+//              Longer nets go 64 pins at a time through a per-wavefront LDS
+//              bitmap of the parts seen. Per-part send / receive / MGV send
+//              volumes are summed in block-private LDS counters and stored
+//              as one row of partial sums per block (above K = 4096, in one
+//              row in HBM by atomics); each (source part,
+//              receiving part) pair sets a bit of the message bitmap (a plain
+//              read first, so known pairs cost no atomic).
+//  k_metrics_reduce  column sums of the partial rows, and the messages sent
+//              and received per part as popcounts of the bitmap's rows and
+//              columns.
+//  k_metrics_final   TM, MSM, MSV and MSRV from the per-part vectors.
+//
+// Algorithmic bytes per evaluation: 8 per pin (pin id + its part) + 12 per
+// net (two offsets read per chunk lane, source id and its part; lambda when
+// requested).
 #include "fp_internal.h"
 
 namespace fpe {
 namespace {
 
-constexpr int METRIC_BLOCK = 256;
+constexpr int MET_WAVES = 8;             // wavefronts per block
+constexpr int MET_BLOCK = MET_WAVES * WAVE;
+constexpr int MET_CHUNK = 64;            // nets per work item
+constexpr int MET_GRID_MAX = 1024;       // blocks at most (rows of partial sums)
+constexpr int MET_LDS_KMAX = 4096;      // per-part counters in LDS up to here, in HBM above
+constexpr int MET_KMAX = 16384;
+constexpr unsigned KEY_NONE = 0xFFFFFFFFu;
+
+// Offsets (in words) of the per-evaluation scratch: the work counter, the
+// message bitmap (K rows of W = ceil(K/32) words) and the partial rows of
+// PS = 3K + 2 words (send as int, receive, MGV send, TV, CN).
+struct MetLayout {
+  int K, W, PS, grid;
+  size_t bitmap_off, partial_off, words;
+};
+
+MetLayout met_layout(int K, int noe) {
+  MetLayout L{};
+  L.K = K;
+  L.W = (K + 31) / 32;
+  L.PS = 3 * K + 2;
+  const int chunks = std::max(1, (noe + MET_CHUNK - 1) / MET_CHUNK);
+  L.grid = std::max(1, std::min(MET_GRID_MAX, (chunks + MET_WAVES - 1) / MET_WAVES));
+  L.bitmap_off = 64;  // the counter has a 256-byte line of its own
+  L.partial_off = L.bitmap_off + (size_t)K * (size_t)L.W;
+  L.words = L.partial_off + (size_t)(K <= MET_LDS_KMAX ? L.grid : 1) * (size_t)L.PS;
+  return L;
+}
 
 struct MetricArgs {
   const unsigned* net_ptr;
   const unsigned* net_pins;
   const int* part;
   const int* src;
-  int noe, K, nov;
-  int* lam;
-  unsigned* p2p;               // K*K bits, zeroed
-  unsigned long long* send;    // MGAV send, K
-  unsigned long long* recv;    // K
-  unsigned long long* sm;      // K
-  unsigned long long* rm;      // K
-  unsigned long long* send_msv;  // MGV send, K
-  unsigned long long* tvcn;    // [0] TV, [1] CN
+  int noe, K, nov, pb, W, PS;
+  int* lam;             // noe, nullable
+  unsigned* counter;    // work counter, zeroed
+  unsigned* bitmap;     // K x W words, zeroed
+  unsigned* partial;    // gridDim.x rows of PS words
 };
 
-constexpr int METRIC_LDS_KMAX = 1024;
-#ifndef FP_METRIC_GRID
-#define FP_METRIC_GRID 4096
-#endif
-constexpr int METRIC_GRID = FP_METRIC_GRID;  // blocks at most: each flushes its per-part sums once
+// Inclusive max-scan over the wave (values >= -1); the DPP steps move -1 into
+// lanes they do not reach.
+__device__ __forceinline__ int wave_incl_max(int x) {
+  x = max(x, dpp_i32<0x111>(-1, x));
+  x = max(x, dpp_i32<0x112>(-1, x));
+  x = max(x, dpp_i32<0x114>(-1, x));
+  x = max(x, dpp_i32<0x118>(-1, x));
+  x = max(x, dpp_i32<0x142, 0xA>(-1, x));
+  x = max(x, dpp_i32<0x143, 0xC>(-1, x));
+  return x;
+}
 
+// Ascending bitonic sort of one key per lane across the wavefront.
+__device__ __forceinline__ unsigned wave_sort_u32(unsigned key) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int k = 2; k <= WAVE; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const unsigned other = (unsigned)__shfl_xor((int)key, j);
+      const bool up = (lane & k) == 0;
+      const bool lower = (lane & j) == 0;
+      key = (lower == up) ? min(key, other) : max(key, other);
+    }
+  }
+  return key;
+}
+
+// Each wavefront queues the messages (sp -> q) it finds in LDS as bit
+// indices of the message bitmap and sets them MSG_Q at a time: the bitmap
+// words load together, then an atomic OR, whose result nobody waits for,
+// goes to each bit a word does not hold yet. A plain read can be stale
+// (another L2 took the atomic); that costs a redundant atomic, never a bit.
+constexpr int MSG_Q = 128;
+
+__device__ __forceinline__ void msg_flush(unsigned* __restrict__ bitmap, const unsigned* q, int& n) {
+  const int lane = lane_id();
+  unsigned idx[MSG_Q / WAVE], w[MSG_Q / WAVE];
+#pragma unroll
+  for (int u = 0; u < MSG_Q / WAVE; u++) {
+    const int t = u * WAVE + lane;
+    idx[u] = t < n ? q[t] : 0xFFFFFFFFu;
+    w[u] = idx[u] != 0xFFFFFFFFu ? bitmap[idx[u] >> 5] : ~0u;
+  }
+#pragma unroll
+  for (int u = 0; u < MSG_Q / WAVE; u++) {
+    const unsigned bit = 1u << (idx[u] & 31);
+    if (idx[u] != 0xFFFFFFFFu && !(w[u] & bit)) atomicOr(&bitmap[idx[u] >> 5], bit);
+  }
+  wave_lds_sync();
+  n = 0;
+}
+
+// Appends the message (sp -> q) of every lane with `want` to the wave's
+// queue (flushed first when it could overflow).
+__device__ __forceinline__ void msg_push(unsigned* __restrict__ bitmap, unsigned* q, int& n, bool want, int sp,
+                                         int qp, int W) {
+  const unsigned long long m = __ballot(want);
+  if (!m) return;
+  const int c = (int)__popcll(m);
+  if (n + c > MSG_Q) msg_flush(bitmap, q, n);
+  if (want) q[n + (int)__popcll(m & ((1ull << lane_id()) - 1ull))] = (unsigned)sp * (unsigned)W * 32u + (unsigned)qp;
+  n += c;
+  wave_lds_sync();
+}
+
+constexpr int MET_GROUP = 4;  // windows (or long-net steps of 64 pins) loaded together
+
+// LDS_ACC: per-part counters block-private in LDS, stored as row blockIdx.x
+// of the partial sums; otherwise (K above MET_LDS_KMAX) atomics go straight
+// to partial row 0, zeroed beforehand.
 template <bool LDS_ACC>
-__global__ void __launch_bounds__(METRIC_BLOCK) k_metrics(MetricArgs a) {
+__global__ void __launch_bounds__(MET_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) k_metrics(MetricArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
-  const int K = a.K;
-  unsigned long long* s_send = LDS_ACC ? reinterpret_cast<unsigned long long*>(lds) : a.send;
-  unsigned long long* s_recv = LDS_ACC ? s_send + K : a.recv;
-  unsigned long long* s_msv = LDS_ACC ? s_recv + K : a.send_msv;
-  int* marks = LDS_ACC ? reinterpret_cast<int*>(s_msv + K) : reinterpret_cast<int*>(lds);
-  const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
-  int* mark = marks + (size_t)wave * (size_t)K;
+  const int K = a.K, W = a.W, pb = a.pb;
+  const unsigned pmask = (1u << pb) - 1u;
+  unsigned* acc = LDS_ACC ? reinterpret_cast<unsigned*>(lds) : a.partial;
+  int* s_send = reinterpret_cast<int*>(acc);
+  unsigned* s_recv = acc + K;
+  unsigned* s_msv = s_recv + K;
+  unsigned* s_tvcn = LDS_ACC ? s_msv + K : reinterpret_cast<unsigned*>(lds);  // [0] TV, [1] CN
+  const int lane = lane_id(), wave = wave_id();
+  unsigned* seen = s_tvcn + 2 + (size_t)wave * (size_t)(W + WAVE + MSG_Q);  // W words: parts of the current long net
+  int* winbuf = reinterpret_cast<int*>(seen + W);  // 64: net starting at each window lane
+  unsigned* mq = seen + W + WAVE;                  // MSG_Q: queued message bits
   if (LDS_ACC)
-    for (int q = (int)threadIdx.x; q < K; q += (int)blockDim.x) { s_send[q] = 0; s_recv[q] = 0; s_msv[q] = 0; }
-  for (int q = lane; q < K; q += WAVE) mark[q] = -1;
+    for (int t = (int)threadIdx.x; t < 3 * K; t += (int)blockDim.x) acc[t] = 0;
+  if (threadIdx.x < 2) s_tvcn[threadIdx.x] = 0;
+  for (int t = lane; t < W; t += WAVE) seen[t] = 0;
   __syncthreads();
-  unsigned long long tv = 0, cn = 0;
-  for (int n = (int)blockIdx.x * nw + wave; n < a.noe; n += (int)gridDim.x * nw) {
-    const unsigned s = a.net_ptr[n], e = a.net_ptr[n + 1];
-    const int sv = a.src[n];
-    const int sp = (sv >= 0 && sv < a.nov) ? a.part[sv] : -1;
-    int l = 0, occ = 0;
-    for (unsigned k0 = s; k0 < e; k0 += WAVE) {
-      const unsigned k = k0 + (unsigned)lane;
-      bool fresh = false;
-      int q = -1;
-      if (k < e) {
-        const int v = (int)a.net_pins[k];
-        q = a.part[v];
-        occ += (v == sv) ? 1 : 0;
-        fresh = atomicExch(&mark[q], n) != n;   // first pin of part q in this net
-      }
-      l += __popcll(__ballot(fresh));
-      if (fresh && sp >= 0 && q != sp) {
-        atomicAdd(&s_recv[q], 1ull);
-        const unsigned bit = (unsigned)sp * (unsigned)K + (unsigned)q;
-        // the pair is usually known already: a plain read skips the atomic
-        // (a stale 0 only costs the atomic, whose result decides)
-        const unsigned m = 1u << (bit & 31);
-        if (!(__builtin_nontemporal_load(&a.p2p[bit >> 5]) & m) && !(atomicOr(&a.p2p[bit >> 5], m) & m)) {
-          atomicAdd(&a.sm[sp], 1ull);
-          atomicAdd(&a.rm[q], 1ull);
+  unsigned tv = 0, cn = 0;  // this wave's running sums (every lane holds them)
+  int mqn = 0;              // queued messages
+
+  int chunk = 0;
+  if (lane == 0) chunk = (int)atomicAdd(a.counter, 1u);
+  chunk = __builtin_amdgcn_readfirstlane(chunk);
+  while ((long long)chunk * MET_CHUNK < a.noe) {
+    // the next chunk's index is fetched while this one is processed
+    int next = 0;
+    if (lane == 0) next = (int)atomicAdd(a.counter, 1u);
+    const int c = chunk * MET_CHUNK;
+    const int nc = min(MET_CHUNK, a.noe - c);
+    // lane j: net c + j
+    unsigned start = 0, end = 0;
+    int sj = -1, spj = -1;
+    if (lane < nc) {
+      start = a.net_ptr[c + lane];
+      end = a.net_ptr[c + lane + 1];
+      sj = a.src[c + lane];
+    }
+    if (sj >= 0 && sj < a.nov) spj = a.part[sj];
+    const int len = (int)(end - start);
+    int lamj = 0, occj = 0;  // lane j: lambda and source occurrences of net c + j
+    int i = 0;
+    while (i < nc) {
+      // up to MET_GROUP windows of whole nets of at most 64 pins, from net i on
+      int wi[MET_GROUP], wc[MET_GROUP];
+      unsigned wbase[MET_GROUP], wp[MET_GROUP];
+      int ng = 0;
+#pragma unroll
+      for (int u = 0; u < MET_GROUP; u++) {
+        wi[u] = i;
+        wc[u] = 0;
+        wbase[u] = 0;
+        wp[u] = 0;
+        if (ng == u && i < nc) {
+          const unsigned base = (unsigned)__builtin_amdgcn_readlane((int)start, i);
+          const unsigned long long fit = __ballot(lane >= i && lane < nc && end - base <= (unsigned)WAVE);
+          const int cnt = (int)__popcll(fit);
+          if (cnt > 0) {
+            wbase[u] = base;
+            wc[u] = cnt;
+            wp[u] = (unsigned)__builtin_amdgcn_readlane((int)end, i + cnt - 1) - base;
+            i += cnt;
+            ng++;
+          }
         }
       }
-    }
-    occ = wave_sum(occ);
-    if (lane == 0) {
-      a.lam[n] = l;
-      if (l != 0) tv += (unsigned long long)(l - 1);
-      if (l > 1) cn++;
-      if (sp >= 0) {
-        atomicAdd(&s_send[sp], (unsigned long long)(long long)(l - 1));
-        if (occ) atomicAdd(&s_msv[sp], (unsigned long long)((long long)(l - 1) * occ));
+      int vv[MET_GROUP], qq[MET_GROUP];
+#pragma unroll
+      for (int u = 0; u < MET_GROUP; u++) vv[u] = (u < ng && (unsigned)lane < wp[u]) ? (int)a.net_pins[wbase[u] + lane] : -1;
+#pragma unroll
+      for (int u = 0; u < MET_GROUP; u++) qq[u] = vv[u] >= 0 ? a.part[vv[u]] : 0;
+#pragma unroll
+      for (int u = 0; u < MET_GROUP; u++) {
+        if (u >= ng) break;
+        const int i0 = wi[u], cnt = wc[u];
+        const unsigned base = wbase[u];
+        const bool in_win = lane >= i0 && lane < i0 + cnt;
+        winbuf[lane] = -1;
+        wave_lds_sync();
+        if (in_win && len > 0) winbuf[start - base] = lane;
+        wave_lds_sync();
+        const int owner = max(wave_incl_max(winbuf[lane]), 0);  // net (chunk lane) of this lane's pin
+        const int srcp = __shfl(sj, owner);
+        unsigned key = KEY_NONE;
+        bool is_src = false;
+        if (vv[u] >= 0) {
+          key = ((unsigned)(owner - i0) << pb) | (unsigned)qq[u];
+          is_src = vv[u] == srcp;
+        }
+        const unsigned long long occm = __ballot(is_src);
+        // sorting keeps every net on its own lanes: net j's keys stay in [start_j, end_j)
+        key = wave_sort_u32(key);
+        const unsigned prev = (unsigned)__shfl_up((int)key, 1);
+        const bool fresh = key != KEY_NONE && (lane == 0 || prev != key);
+        const unsigned long long fm = __ballot(fresh);
+        const int kn = i0 + (int)(key == KEY_NONE ? 0u : key >> pb);
+        const int spk = __shfl(spj, min(kn, WAVE - 1));
+        const int q = (int)(key & pmask);
+        const bool recv = fresh && spk >= 0 && q != spk;
+        if (recv) atomicAdd(&s_recv[q], 1u);
+        msg_push(a.bitmap, mq, mqn, recv, spk, q, W);
+        if (in_win) {
+          const unsigned o = start - base;
+          const unsigned long long seg = len >= WAVE ? ~0ull : (((1ull << len) - 1ull) << o);
+          lamj = (int)__popcll(fm & seg);
+          occj = (int)__popcll(occm & seg);
+        }
+      }
+      if (ng < MET_GROUP && i < nc) {
+        // net i is longer than a wave: MET_GROUP x 64 pins at a time through
+        // the LDS bitmap of the parts seen
+        const unsigned s0 = (unsigned)__builtin_amdgcn_readlane((int)start, i);
+        const unsigned e = (unsigned)__builtin_amdgcn_readlane((int)end, i);
+        const int srcL = __builtin_amdgcn_readlane(sj, i);
+        const int spL = __builtin_amdgcn_readlane(spj, i);
+        int l = 0, occ = 0;
+        for (unsigned k0 = s0; k0 < e; k0 += MET_GROUP * WAVE) {
+          int lv[MET_GROUP], lq[MET_GROUP];
+#pragma unroll
+          for (int u = 0; u < MET_GROUP; u++) {
+            const unsigned k = k0 + (unsigned)(u * WAVE + lane);
+            lv[u] = k < e ? (int)a.net_pins[k] : -1;
+          }
+#pragma unroll
+          for (int u = 0; u < MET_GROUP; u++) lq[u] = lv[u] >= 0 ? a.part[lv[u]] : 0;
+#pragma unroll
+          for (int u = 0; u < MET_GROUP; u++) {
+            bool fresh = false;
+            if (lv[u] >= 0) {
+              occ += lv[u] == srcL ? 1 : 0;
+              const unsigned bit = 1u << (lq[u] & 31);
+              fresh = !(atomicOr(&seen[lq[u] >> 5], bit) & bit);
+            }
+            l += (int)__popcll(__ballot(fresh));
+            const bool recv = fresh && spL >= 0 && lq[u] != spL;
+            if (recv) atomicAdd(&s_recv[lq[u]], 1u);
+            msg_push(a.bitmap, mq, mqn, recv, spL, lq[u], W);
+          }
+        }
+        occ = wave_sum(occ);
+        wave_lds_sync();
+        for (int t = lane; t < W; t += WAVE) seen[t] = 0;
+        wave_lds_sync();
+        if (lane == i) {
+          lamj = l;
+          occj = occ;
+        }
+        i++;
       }
     }
-  }
-  if (LDS_ACC) {
-    __syncthreads();
-    for (int q = (int)threadIdx.x; q < K; q += (int)blockDim.x) {
-      if (s_send[q]) atomicAdd(&a.send[q], s_send[q]);
-      if (s_recv[q]) atomicAdd(&a.recv[q], s_recv[q]);
-      if (s_msv[q]) atomicAdd(&a.send_msv[q], s_msv[q]);
+    // per net: lambda, TV, CN, send volumes
+    if (lane < nc) {
+      if (a.lam) a.lam[c + lane] = lamj;
+      if (spj >= 0) {
+        atomicAdd(&s_send[spj], lamj - 1);
+        if (occj) atomicAdd(&s_msv[spj], (unsigned)((lamj - 1) * occj));
+      }
     }
+    tv += (unsigned)wave_sum(lane < nc && lamj != 0 ? lamj - 1 : 0);
+    cn += (unsigned)wave_sum(lane < nc && lamj > 1 ? 1 : 0);
+    chunk = __builtin_amdgcn_readfirstlane(next);
   }
-  // TV and CN: summed over the block, then one atomic each
-  __shared__ unsigned long long s_tv, s_cn;
-  if (threadIdx.x == 0) {
-    s_tv = 0;
-    s_cn = 0;
-  }
-  __syncthreads();
+  if (mqn) msg_flush(a.bitmap, mq, mqn);
   if (lane == 0) {
-    if (tv) atomicAdd(&s_tv, tv);
-    if (cn) atomicAdd(&s_cn, cn);
+    atomicAdd(&s_tvcn[0], tv);
+    atomicAdd(&s_tvcn[1], cn);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    if (s_tv) atomicAdd(&a.tvcn[0], s_tv);
-    if (s_cn) atomicAdd(&a.tvcn[1], s_cn);
+  if (LDS_ACC) {
+    unsigned* row = a.partial + (size_t)blockIdx.x * (size_t)a.PS;
+    for (int t = (int)threadIdx.x; t < a.PS; t += (int)blockDim.x) row[t] = acc[t];
+  } else if (threadIdx.x < 2) {
+    atomicAdd(&a.partial[3 * (size_t)K + threadIdx.x], s_tvcn[threadIdx.x]);
   }
 }
 
+// Blocks [0, colb): column sums of the partial rows, 64 columns per block and
+// 16 row groups. Blocks [colb, colb + kb): messages sent per part (popcount
+// of bitmap row a, one thread per part). Blocks [colb + kb, colb + 2 kb):
+// messages received (popcount of bitmap column b: the 64 lanes of a wave read
+// two words per row). out = [6 scalars][send K][recv K][sm K][rm K][msv K].
+__global__ void __launch_bounds__(1024) k_metrics_reduce(const unsigned* __restrict__ partial, int rows, int K, int W,
+                                                         const unsigned* __restrict__ bitmap, int colb, int kb,
+                                                         long long* __restrict__ out) {
+  const int PS = 3 * K + 2;
+  const int b = (int)blockIdx.x;
+  if (b < colb) {
+    __shared__ long long acc[16][64];
+    const int cl = (int)threadIdx.x & 63, g = (int)threadIdx.x >> 6;
+    const int col = b * 64 + cl;
+    long long s = 0;
+    if (col < PS) {
+      const bool is_signed = col < K;  // send: int
+      for (int r = g; r < rows; r += 16) {
+        const unsigned x = partial[(size_t)r * (size_t)PS + (size_t)col];
+        s += is_signed ? (long long)(int)x : (long long)x;
+      }
+    }
+    acc[g][cl] = s;
+    __syncthreads();
+    if (g == 0 && col < PS) {
+      for (int h = 1; h < 16; h++) s += acc[h][cl];
+      long long* dst;
+      if (col < K) dst = out + 6 + col;                               // send
+      else if (col < 2 * K) dst = out + 6 + K + (col - K);            // recv
+      else if (col < 3 * K) dst = out + 6 + 4 * (size_t)K + (col - 2 * K);  // MGV send
+      else dst = out + (col - 3 * K);                                 // TV, CN
+      *dst = s;
+    }
+    return;
+  }
+  if (b < colb + kb) {
+    const int p = (b - colb) * 1024 + (int)threadIdx.x;
+    if (p >= K) return;
+    long long s = 0;
+    for (int w = 0; w < W; w++) s += __popc(bitmap[(size_t)p * (size_t)W + w]);
+    out[6 + 2 * (size_t)K + p] = s;
+    return;
+  }
+  const int q = (b - colb - kb) * 1024 + (int)threadIdx.x;
+  if (q >= K) return;
+  const unsigned bit = 1u << (q & 31);
+  long long s = 0;
+  for (int p = 0; p < K; p++) s += (bitmap[(size_t)p * (size_t)W + (q >> 5)] & bit) ? 1 : 0;
+  out[6 + 3 * (size_t)K + q] = s;
+}
+
+// TM = sum of messages sent, MSM = their max, MSV = max MGV send volume,
+// MSRV = max of send + receive volume (shared/metric.cpp:122-391).
+__global__ void __launch_bounds__(1024) k_metrics_final(int K, long long* __restrict__ out) {
+  __shared__ long long s_r[4][16];
+  const long long* send = out + 6;
+  const long long* recv = send + K;
+  const long long* sm = send + 2 * (size_t)K;
+  const long long* msv = send + 4 * (size_t)K;
+  long long tm = 0, msm = -1, mv = -1, msrv = -1;
+  for (int k = (int)threadIdx.x; k < K; k += (int)blockDim.x) {
+    tm += sm[k];
+    msm = max(msm, sm[k]);
+    mv = max(mv, msv[k]);
+    msrv = max(msrv, send[k] + recv[k]);
+  }
+  for (int off = WAVE / 2; off > 0; off >>= 1) {
+    tm += __shfl_xor(tm, off);
+    msm = max(msm, __shfl_xor(msm, off));
+    mv = max(mv, __shfl_xor(mv, off));
+    msrv = max(msrv, __shfl_xor(msrv, off));
+  }
+  const int w = (int)threadIdx.x / WAVE;
+  if (lane_id() == 0) {
+    s_r[0][w] = tm;
+    s_r[1][w] = msm;
+    s_r[2][w] = mv;
+    s_r[3][w] = msrv;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (int)blockDim.x / WAVE;
+    for (int h = 1; h < nw; h++) {
+      tm += s_r[0][h];
+      msm = max(msm, s_r[1][h]);
+      mv = max(mv, s_r[2][h]);
+      msrv = max(msrv, s_r[3][h]);
+    }
+    out[2] = tm;
+    out[3] = msm;
+    out[4] = mv;
+    out[5] = msrv;
+  }
+}
+
+size_t met_lds_bytes(int K, int W) {
+  const size_t acc = K <= MET_LDS_KMAX ? (size_t)3 * K : 0;
+  return (acc + 2) * sizeof(unsigned) + (size_t)MET_WAVES * (size_t)(W + WAVE + MSG_Q) * sizeof(unsigned);
+}
+
+fp_status met_check(fp_ctx* ctx, int K) {
+  if (K > MET_KMAX) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d above %d for the metric kernel", K, MET_KMAX);
+  // per-block partial sums are 32-bit: a block's sums stay below the pin
+  // count, except the MGV send volume (lambda - 1) * occurrences of the
+  // source among a net's pins, which only repeated pins can push past it
+  if (ctx->pins >= (1ll << 31) || (ctx->dup_pins && (unsigned long long)ctx->max_net * ctx->max_net >= (1ull << 31)))
+    return fail(ctx, FP_ERR_UNSUPPORTED, "hypergraph too large for the metric kernel's 32-bit partial sums");
+  return FP_OK;
+}
+
+// Enqueues the three launches on ctx->stream. scratch: L.words words; out: 6 + 5K.
+fp_status met_launch(fp_ctx* ctx, const MetLayout& L, unsigned* scratch, long long* out, int* lam) {
+  const bool lds_acc = L.K <= MET_LDS_KMAX;
+  // the counter and the bitmap; without LDS counters also partial row 0
+  fill_async<unsigned>(ctx, scratch, L.partial_off + (lds_acc ? 0 : (size_t)L.PS), 0u);
+  MetricArgs m{};
+  m.net_ptr = ctx->net_ptr;
+  m.net_pins = ctx->net_pins;
+  m.part = ctx->part;
+  m.src = ctx->src;
+  m.noe = ctx->noe;
+  m.K = L.K;
+  m.nov = ctx->nov;
+  m.pb = ctx->pb;
+  m.W = L.W;
+  m.PS = L.PS;
+  m.lam = lam;
+  m.counter = scratch;
+  m.bitmap = scratch + L.bitmap_off;
+  m.partial = scratch + L.partial_off;
+  const size_t lds = met_lds_bytes(L.K, L.W);
+  if (lds_acc) k_metrics<true><<<L.grid, MET_BLOCK, lds, ctx->stream>>>(m);
+  else k_metrics<false><<<L.grid, MET_BLOCK, lds, ctx->stream>>>(m);
+  const int colb = (L.PS + 63) / 64, kb = (L.K + 1023) / 1024;
+  k_metrics_reduce<<<colb + 2 * kb, 1024, 0, ctx->stream>>>(m.partial, lds_acc ? L.grid : 1, L.K, L.W, m.bitmap,
+                                                            colb, kb, out);
+  k_metrics_final<<<1, 1024, 0, ctx->stream>>>(L.K, out);
+  if (hipGetLastError() != hipSuccess) return fail(ctx, FP_ERR_HIP, "metric kernel launch failed");
+  return FP_OK;
+}
 
 }  // namespace
 }  // namespace fpe
@@ -126,73 +470,38 @@ fp_status fp_eval(fp_ctx* ctx, fp_metrics* out, int* lam, long long* sv, long lo
   if (!ctx->has_state) return fail(ctx, FP_ERR_STATE, "fp_eval before a state was set");
   FP_HIP_OK(hipSetDevice(ctx->device));
   const int K = ctx->K;
-  const bool lds_acc = K <= METRIC_LDS_KMAX;
-  // mark arrays (K ints per wavefront) plus, up to METRIC_LDS_KMAX, three
-  // block-private int64 vectors
-  const size_t acc_bytes = lds_acc ? (size_t)3 * K * sizeof(unsigned long long) : 0;
-  int waves = METRIC_BLOCK / WAVE;
-  while (waves > 0 && acc_bytes + (size_t)waves * K * sizeof(int) > (size_t)LDS_BUDGET) waves--;
-  if (waves == 0) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d too large for the metric kernel", K);
-  if (!ctx->lam) FP_TRY(dalloc(ctx, &ctx->lam, (size_t)std::max(ctx->noe, 1)));
-  int* d_lam = ctx->lam;
-  fill_async<unsigned>(ctx, ctx->p2p, ((size_t)K * (size_t)K + 31) / 32, 0u);
-  fill_async<unsigned long long>(ctx, ctx->mvec, (size_t)5 * (size_t)K + 2, 0ull);
-  MetricArgs m{};
-  m.net_ptr = ctx->net_ptr;
-  m.net_pins = ctx->net_pins;
-  m.part = ctx->part;
-  m.src = ctx->src;
-  m.noe = ctx->noe;
-  m.K = K;
-  m.nov = ctx->nov;
-  m.lam = d_lam;
-  m.p2p = ctx->p2p;
-  m.send = ctx->mvec;
-  m.recv = ctx->mvec + K;
-  m.sm = ctx->mvec + 2 * (size_t)K;
-  m.rm = ctx->mvec + 3 * (size_t)K;
-  m.send_msv = ctx->mvec + 4 * (size_t)K;
-  m.tvcn = ctx->mvec + 5 * (size_t)K;
-  const size_t lds = acc_bytes + (size_t)waves * K * sizeof(int);
-  const int grid = std::max(1, std::min((ctx->noe + waves - 1) / waves, METRIC_GRID));
-  if (ctx->noe > 0) {
-    if (lds_acc) k_metrics<true><<<grid, waves * WAVE, lds, ctx->stream>>>(m);
-    else k_metrics<false><<<grid, waves * WAVE, lds, ctx->stream>>>(m);
+  FP_TRY(met_check(ctx, K));
+  const MetLayout L = met_layout(K, ctx->noe);
+  if (L.words > ctx->met_words) {
+    FP_TRY(dalloc(ctx, &ctx->met_scratch, L.words));
+    ctx->met_words = L.words;
   }
-  std::vector<unsigned long long> h((size_t)5 * K + 2);
-  fp_status st = FP_OK;
-  if (hipGetLastError() != hipSuccess) st = fail(ctx, FP_ERR_HIP, "k_metrics launch failed");
-  if (st == FP_OK) st = d2h(ctx, h.data(), ctx->mvec, h.size());
-  if (st == FP_OK && lam) st = d2h(ctx, lam, d_lam, (size_t)ctx->noe);
+  if ((size_t)(6 + 5 * K) > ctx->met_out_cap) {
+    FP_TRY(dalloc(ctx, &ctx->met_out, (size_t)(6 + 5 * K)));
+    ctx->met_out_cap = (size_t)(6 + 5 * K);
+  }
+  if (lam && !ctx->lam) FP_TRY(dalloc(ctx, &ctx->lam, (size_t)std::max(ctx->noe, 1)));
+  FP_TRY(met_launch(ctx, L, ctx->met_scratch, ctx->met_out, lam ? ctx->lam : nullptr));
+  std::vector<long long> h((size_t)6 + 5 * (size_t)K);
+  fp_status st = d2h(ctx, h.data(), ctx->met_out, h.size());
+  if (st == FP_OK && lam) st = d2h(ctx, lam, ctx->lam, (size_t)ctx->noe);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess && st == FP_OK) st = fail(ctx, FP_ERR_HIP, "sync failed");
   if (st != FP_OK) return st;
-  const long long* send = reinterpret_cast<const long long*>(h.data());
-  const long long* recv = send + K;
-  const long long* smv = send + 2 * (size_t)K;
-  const long long* rmv = send + 3 * (size_t)K;
-  const long long* msv = send + 4 * (size_t)K;
+  const long long* v = h.data() + 6;
   if (out) {
-    out->tv = (long long)h[5 * (size_t)K];
-    out->cn = (long long)h[5 * (size_t)K + 1];
-    long long tm = 0, m_sm = -1, m_sv = -1, m_srv = -1;
-    for (int k = 0; k < K; k++) {
-      tm += smv[k];
-      m_sm = std::max(m_sm, smv[k]);
-      m_sv = std::max(m_sv, msv[k]);
-      m_srv = std::max(m_srv, send[k] + recv[k]);
-    }
-    out->tm = tm;
-    out->msm = m_sm;
-    out->msv = m_sv;
-    out->msrv = m_srv;
+    out->tv = h[0];
+    out->cn = h[1];
+    out->tm = h[2];
+    out->msm = h[3];
+    out->msv = h[4];
+    out->msrv = h[5];
   }
-  if (sv) std::copy(send, send + K, sv);
-  if (rv) std::copy(recv, recv + K, rv);
-  if (sm) std::copy(smv, smv + K, sm);
-  if (rm) std::copy(rmv, rmv + K, rm);
-  if (sv_msv) std::copy(msv, msv + K, sv_msv);
+  if (sv) std::copy(v, v + K, sv);
+  if (rv) std::copy(v + K, v + 2 * (size_t)K, rv);
+  if (sm) std::copy(v + 2 * (size_t)K, v + 3 * (size_t)K, sm);
+  if (rm) std::copy(v + 3 * (size_t)K, v + 4 * (size_t)K, rm);
+  if (sv_msv) std::copy(v + 4 * (size_t)K, v + 5 * (size_t)K, sv_msv);
   return FP_OK;
 }
-
 
 }  // extern "C"

@@ -12,6 +12,13 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Vertex weights of the refinements and initial partitions below (NULL =
+ * unit, the reference's only case): a move changes the part sizes by the
+ * vertex's weight, as the engine does. Set by or_set_vertex_weights. */
+static const int* g_vwgt = NULL;
+#define VW(v) (g_vwgt ? g_vwgt[(v)] : 1)
+void or_set_vertex_weights(const int* vwgt) { g_vwgt = vwgt; }
+
 /* ---------------------------------------------------------------------- */
 /* connectivity list edits (identical in every reference directory)        */
 /* ---------------------------------------------------------------------- */
@@ -225,7 +232,7 @@ int or_tv_ldg(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr
     int d = ldg2(K, apc, active, scores, sizes, C, max_pw, min_pw, part[v]);
     if (d == -2) { rc = 3; break; }
     if (par_skip && sizes[d] > max_pw) continue;   /* TV/partition.cpp:598-602 */
-    sizes[d]++;
+    sizes[d] += VW(v);
     part[v] = d;
     for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) list_add((int)vtx_nets[i], d, net_ptr, ep_ends, ep_parts, ep_cnt);
   }
@@ -275,8 +282,8 @@ int or_tv_refine_serial(int nov, int noe, const unsigned* net_ptr, const unsigne
       }
       if (leave_gain >= arrival_loss) {   /* TV/partition.cpp:380-386 */
         part[v] = d;
-        sizes[own]--;
-        sizes[d]++;
+        sizes[own] -= VW(v);
+        sizes[d] += VW(v);
         move_vertex(v, own, d, net_ptr, vtx_ptr, vtx_nets, ep_ends, ep_parts, ep_cnt);
       }
     }
@@ -334,7 +341,7 @@ int or_tv_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* 
   int* marker = (int*)malloc(sizeof(int) * (size_t)K);
   int* active = (int*)malloc(sizeof(int) * (size_t)K);
   double* scores = (double*)calloc((size_t)K, sizeof(double));
-  int* dec = (int*)malloc(sizeof(int) * (size_t)R);
+  int* dec = (int*)malloc(sizeof(int) * (size_t)(R < nov ? R : (nov > 0 ? nov : 1)));  /* a round holds at most nov */
   long long* frozen = (long long*)malloc(sizeof(long long) * (size_t)K);
   if (!marker || !active || !scores || !dec || !frozen) return 2;
   for (int k = 0; k < K; k++) marker[k] = -1;
@@ -362,7 +369,7 @@ int or_tv_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* 
           int own = part[v];
           if (sizes[d] > max_pw) break;   /* "Weird": ends this thread's list */
           part[v] = d;
-          sizes[own]--;
+          sizes[own] -= VW(v);
           move_vertex(v, own, d, net_ptr, vtx_ptr, vtx_nets, ep_ends, ep_parts, ep_cnt);
         }
       }
@@ -392,7 +399,7 @@ int or_tv_refine_bal(int nov, int noe, const unsigned* net_ptr, const unsigned* 
   int* marker = (int*)malloc(sizeof(int) * (size_t)K);
   int* active = (int*)malloc(sizeof(int) * (size_t)K);
   double* scores = (double*)calloc((size_t)K, sizeof(double));
-  int* dec = (int*)malloc(sizeof(int) * (size_t)R);
+  int* dec = (int*)malloc(sizeof(int) * (size_t)(R < nov ? R : (nov > 0 ? nov : 1)));  /* a round holds at most nov */
   long long* inflow = (long long*)calloc((size_t)K, sizeof(long long));
   if (!marker || !active || !scores || !dec || !inflow) return 2;
   for (int k = 0; k < K; k++) marker[k] = -1;
@@ -487,7 +494,7 @@ int or_cn_ldg(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr
         if (sizes[i] < min_load) { min_load = sizes[i]; max_index = i; }
     }
     int d = max_index;
-    sizes[d]++;
+    sizes[d] += VW(v);
     part[v] = d;
     for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) list_add((int)vtx_nets[i], d, net_ptr, ep_ends, ep_parts, ep_cnt);
   }
@@ -539,8 +546,8 @@ int or_cn_refine_serial(int nov, int noe, const unsigned* net_ptr, const unsigne
         int v = push[i];
         int own = part[v];
         part[v] = nd;
-        sizes[own]--;
-        sizes[nd]++;
+        sizes[own] -= VW(v);
+        sizes[nd] += VW(v);
         move_vertex(v, own, nd, net_ptr, vtx_ptr, vtx_nets, ep_ends, ep_parts, ep_cnt);
       }
     }
@@ -576,8 +583,8 @@ int or_cn_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* 
       int v = mv_v[i], nd = mv_d[i];
       int own = part[v];
       part[v] = nd;
-      sizes[own]--;
-      sizes[nd]++;
+      sizes[own] -= VW(v);
+      sizes[nd] += VW(v);
       move_vertex(v, own, nd, net_ptr, vtx_ptr, vtx_nets, ep_ends, ep_parts, ep_cnt);
     }
   }
@@ -640,7 +647,7 @@ static int vol_ldg(int nov, int noe, const unsigned* net_ptr, const unsigned* vt
       }
     }
     int d = ldg_plain(K, apc, active, scores, sizes, C);
-    sizes[d]++;
+    sizes[d] += VW(v);
     part[v] = d;
     for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
       int n = (int)vtx_nets[i];
@@ -784,8 +791,8 @@ static void vol_apply(int v, int d, const unsigned* net_ptr, const unsigned* vtx
                       int* rv) {
   int own = part[v];
   part[v] = d;
-  sizes[own]--;
-  sizes[d]++;
+  sizes[own] -= VW(v);
+  sizes[d] += VW(v);
   for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
     int n = (int)vtx_nets[i];
     int s = (int)net_ptr[n];
@@ -887,7 +894,7 @@ static int vol_refine_par(int nov, const unsigned* net_ptr, const unsigned* vtx_
   if (R < 1) return 1;
   vol_scratch w;
   if (vol_scratch_init(&w, K)) return 2;
-  int* dec = (int*)malloc(sizeof(int) * (size_t)R);
+  int* dec = (int*)malloc(sizeof(int) * (size_t)(R < nov ? R : (nov > 0 ? nov : 1)));  /* a round holds at most nov */
   if (!dec) return 2;
   long long applied = 0;
   for (int it = 0; it < passes; it++) {
@@ -1049,7 +1056,7 @@ static int conn_init(int nov, int noe, const unsigned* net_ptr, const unsigned* 
     }
     int d = best;
     part[v] = d;
-    sizes[d]++;
+    sizes[d] += VW(v);
     for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) list_add((int)vtx_nets[i], d, net_ptr, ep_ends, ep_parts, ep_cnt);
     /* TM/partition.cpp:190-215 */
     for (int i = 0; i < nact; i++) {
@@ -1170,8 +1177,8 @@ static void conn_apply(int v, int d, const unsigned* net_ptr, const unsigned* vt
                        int* pc, int* msg) {
   int own = part[v];
   part[v] = d;
-  sizes[own]--;
-  sizes[d]++;
+  sizes[own] -= VW(v);
+  sizes[d] += VW(v);
   move_vertex(v, own, d, net_ptr, vtx_ptr, vtx_nets, ep_ends, ep_parts, ep_cnt);
   for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
     int n = (int)vtx_nets[i];
@@ -1238,7 +1245,7 @@ int or_tm_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* 
   if (R < 1) return 1;
   conn_scratch w;
   if (conn_scratch_init(&w, K)) return 2;
-  int* dec = (int*)malloc(sizeof(int) * (size_t)R);
+  int* dec = (int*)malloc(sizeof(int) * (size_t)(R < nov ? R : (nov > 0 ? nov : 1)));  /* a round holds at most nov */
   if (!dec) return 2;
   const double thr = C * (1.02);
   long long applied = 0;
@@ -1343,7 +1350,7 @@ int or_msm_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned*
   int* losses = (int*)calloc((size_t)K, sizeof(int));
   unsigned char* inmax = (unsigned char*)malloc((size_t)K);
   unsigned char* inmax2 = (unsigned char*)malloc((size_t)K);
-  int* dec = (int*)malloc(sizeof(int) * (size_t)R);
+  int* dec = (int*)malloc(sizeof(int) * (size_t)(R < nov ? R : (nov > 0 ? nov : 1)));  /* a round holds at most nov */
   if (!gains || !losses || !inmax || !inmax2 || !dec) return 2;
   long long applied = 0;
   for (int it = 0; it < passes; it++) {

@@ -91,6 +91,12 @@ int or_tv_refine_bal(int nov, int noe, const unsigned* net_ptr, const unsigned* 
 void or_apply_move(int v, int p, int q, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
                    int* ep_ends, int* ep_parts, int* ep_cnt);
 
+/* Vertex weights (nov ints, NULL = unit) that the refinements and initial
+ * partitions apply to part sizes from now on: a move changes the sizes by the
+ * vertex's weight. With unit weights every function is the reference's. The
+ * balanced schedule and or_build_state take their weights as an argument. */
+void or_set_vertex_weights(const int* vwgt);
+
 /* Test hook. A nonzero code plants a known mistake in or_tv_refine_bal's
  * admission (1: admit every candidate; 2: count only admitted weight in the
  * prefix; 3: a strict < cap test), so a test can show that its checks fail

@@ -36,6 +36,12 @@ def _p(a, t):
     return None if a is None else a.ctypes.data_as(t)
 
 
+def _wts(hg):
+    """Hands hg's vertex weights (None = unit) to the oracle's refinements and
+    initial partitions, which change part sizes by the moved vertex's weight."""
+    lib().or_set_vertex_weights(_p(hg.vwgt, _I32P))
+
+
 def _ord(hg, ordering):
     if ordering is None:
         ordering = np.arange(hg.nov, dtype=np.int32)
@@ -94,6 +100,7 @@ def metrics(hg, K, part):
 
 
 def tv_ldg(hg, K, C, max_pw, min_pw, par_skip=False, ordering=None):
+    _wts(hg)
     st = State(hg, K)
     marker = np.zeros(K, dtype=np.int32)
     rc = lib().or_tv_ldg(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P), _p(hg.vtx_nets, _U32P),
@@ -106,6 +113,7 @@ def tv_ldg(hg, K, C, max_pw, min_pw, par_skip=False, ordering=None):
 
 
 def tv_refine_serial(hg, st, marker, C, max_pw, min_pw, passes, ordering=None):
+    _wts(hg)
     rc = lib().or_tv_refine_serial(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
                                    _p(hg.vtx_nets, _U32P), _p(_ord(hg, ordering), _I32P), st.K,
                                    ctypes.c_double(C), max_pw, min_pw, passes, _p(st.part, _I32P),
@@ -117,6 +125,7 @@ def tv_refine_serial(hg, st, marker, C, max_pw, min_pw, passes, ordering=None):
 
 
 def tv_refine_par(hg, st, C, max_pw, min_pw, passes, R, chunk=8, ordering=None, gains=None):
+    _wts(hg)
     assert st.sizes.dtype == np.int32
     rc = lib().or_tv_refine_par(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
                                 _p(hg.vtx_nets, _U32P), _p(_ord(hg, ordering), _I32P), st.K, ctypes.c_double(C),
@@ -129,6 +138,7 @@ def tv_refine_par(hg, st, C, max_pw, min_pw, passes, R, chunk=8, ordering=None, 
 
 
 def tv_refine_bal(hg, st, C, max_pw, min_pw, cap, passes, R, ordering=None, gains=None):
+    _wts(hg)
     assert st.sizes.dtype == np.int64
     L = ctypes.c_longlong
     rc = lib().or_tv_refine_bal(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
@@ -154,6 +164,7 @@ def plant(code: int):
 
 
 def cn_ldg(hg, K, C, ordering=None):
+    _wts(hg)
     st = State(hg, K)
     rc = lib().or_cn_ldg(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P), _p(hg.vtx_nets, _U32P),
                          _p(_ord(hg, ordering), _I32P), K, ctypes.c_double(C), _p(st.part, _I32P),
@@ -165,6 +176,7 @@ def cn_ldg(hg, K, C, ordering=None):
 
 
 def cn_refine(hg, st, passes, serial):
+    _wts(hg)
     f = lib().or_cn_refine_serial if serial else lib().or_cn_refine_par
     rc = f(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.net_pins, _U32P), _p(hg.vtx_ptr, _U32P),
            _p(hg.vtx_nets, _U32P), st.K, passes, _p(st.part, _I32P), _p(st.ep_ends, _I32P),
@@ -176,6 +188,7 @@ def cn_refine(hg, st, passes, serial):
 
 def msv_ldg(hg, K, C, ordering=None):
     """MGV initial partition; st.sv holds the send volumes it accumulated."""
+    _wts(hg)
     st = State(hg, K)
     st.sv = np.zeros(K, dtype=np.int32)
     rc = lib().or_msv_ldg(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P), _p(hg.vtx_nets, _U32P),
@@ -188,6 +201,7 @@ def msv_ldg(hg, K, C, ordering=None):
 
 
 def msv_refine_serial(hg, st, C, passes, ordering=None):
+    _wts(hg)
     rc = lib().or_msv_refine_serial(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
                                     _p(hg.vtx_nets, _U32P), _p(hg.src, _I32P), _p(_ord(hg, ordering), _I32P), st.K,
                                     ctypes.c_double(C), passes, _p(st.part, _I32P), _p(st.ep_ends, _I32P),
@@ -200,6 +214,7 @@ def msv_refine_serial(hg, st, C, passes, ordering=None):
 
 def msv_refine_par(hg, st, C, passes, R):
     """Returns the number of applied moves."""
+    _wts(hg)
     assert st.sizes.dtype == np.int32 and st.sv.dtype == np.int32
     moves = ctypes.c_longlong(0)
     rc = lib().or_msv_refine_par(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
@@ -213,6 +228,7 @@ def msv_refine_par(hg, st, C, passes, R):
 
 def vol_move_delta(hg, st, v, d, recv):
     """losses - gains per part of the reference's evaluation of v -> d."""
+    _wts(hg)
     out = np.zeros(st.K, dtype=np.int32)
     rc = lib().or_vol_move_delta(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
                                  _p(hg.vtx_nets, _U32P), _p(hg.src, _I32P), st.K, _p(st.part, _I32P),
@@ -225,6 +241,7 @@ def vol_move_delta(hg, st, v, d, recv):
 
 def tm_init(hg, K, C, ordering=None):
     """TM greedy initial partition; st.pc (K*K) and st.msg (K) hold the part connections."""
+    _wts(hg)
     st = State(hg, K)
     st.pc = np.zeros(K * K, dtype=np.int32)
     st.msg = np.zeros(K, dtype=np.int32)
@@ -239,6 +256,7 @@ def tm_init(hg, K, C, ordering=None):
 
 def msm_init(hg, K, C, ordering=None):
     """MSM greedy initial partition (pc diagonal -1)."""
+    _wts(hg)
     st = State(hg, K)
     st.pc = np.zeros(K * K, dtype=np.int32)
     st.msg = np.zeros(K, dtype=np.int32)
@@ -252,6 +270,7 @@ def msm_init(hg, K, C, ordering=None):
 
 
 def msm_refine_serial(hg, st, passes):
+    _wts(hg)
     rc = lib().or_msm_refine_serial(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
                                     _p(hg.vtx_nets, _U32P), _p(hg.src, _I32P), st.K, passes, *_conn_args(hg, st))
     if rc:
@@ -260,6 +279,7 @@ def msm_refine_serial(hg, st, passes):
 
 
 def msm_refine_par(hg, st, passes, R):
+    _wts(hg)
     moves = ctypes.c_longlong(0)
     rc = lib().or_msm_refine_par(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
                                  _p(hg.vtx_nets, _U32P), _p(hg.src, _I32P), st.K, passes, R, *_conn_args(hg, st),
@@ -275,6 +295,7 @@ def _conn_args(hg, st):
 
 
 def tm_refine_serial(hg, st, C, passes):
+    _wts(hg)
     rc = lib().or_tm_refine_serial(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
                                    _p(hg.vtx_nets, _U32P), _p(hg.src, _I32P), st.K, ctypes.c_double(C), passes,
                                    *_conn_args(hg, st))
@@ -284,6 +305,7 @@ def tm_refine_serial(hg, st, C, passes):
 
 
 def tm_refine_par(hg, st, C, passes, R):
+    _wts(hg)
     moves = ctypes.c_longlong(0)
     rc = lib().or_tm_refine_par(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
                                 _p(hg.vtx_nets, _U32P), _p(hg.src, _I32P), st.K, ctypes.c_double(C), passes, R,
@@ -295,6 +317,7 @@ def tm_refine_par(hg, st, C, passes, R):
 
 def msrv_ldg(hg, K, C, ordering=None):
     """MSRV initial partition; st.sv / st.rv hold the send / receive volumes."""
+    _wts(hg)
     st = State(hg, K)
     st.sv = np.zeros(K, dtype=np.int32)
     st.rv = np.zeros(K, dtype=np.int32)
@@ -309,6 +332,7 @@ def msrv_ldg(hg, K, C, ordering=None):
 
 def msrv_refine_par(hg, st, C, passes, R, ordering=None):
     """Returns the number of applied moves."""
+    _wts(hg)
     assert st.sizes.dtype == np.int32
     moves = ctypes.c_longlong(0)
     rc = lib().or_msrv_refine_par(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
@@ -322,6 +346,7 @@ def msrv_refine_par(hg, st, C, passes, R, ordering=None):
 
 
 def msrv_refine_serial(hg, st, C, passes, ordering=None):
+    _wts(hg)
     rc = lib().or_msrv_refine_serial(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
                                      _p(hg.vtx_nets, _U32P), _p(hg.src, _I32P), _p(_ord(hg, ordering), _I32P), st.K,
                                      ctypes.c_double(C), passes, _p(st.part, _I32P), _p(st.ep_ends, _I32P),

@@ -1,11 +1,15 @@
 """The serving loop: requests on resident hypergraphs, admitted by HBM
-footprint and run concurrently by worker threads, give the same part vector
-and metrics as the same request run alone through a fresh context."""
+footprint and run concurrently by worker threads, give the oracle's
+refinement of the request, and the same part vector and metrics as the same
+request run alone through a fresh context."""
 import numpy as np
 import pytest
 
 from fastpartitioner_amd import hypergraph as H
 from fastpartitioner_amd.engine import SCHED_BAL, SCHED_REF, Engine, Server, schedule
+from oracle import oracle as O
+
+import oracle_refine as R
 
 pytestmark = pytest.mark.gpu
 
@@ -64,6 +68,15 @@ def test_serve_same_alone_and_in_the_mix():
         assert np.array_equal(part, want_part), (g, metric, K)
         assert info["metrics"] == want_m, (g, metric, K)
         assert info["start_ms"] >= info["submit_ms"] and info["end_ms"] >= info["start_ms"]
+        # the oracle's refinement of the request's state (the first-appearance
+        # lists of its part vector)
+        hg = graphs[g]
+        st, moves = R.refine(hg, K, metric, sched, O.build_state(hg, K, p0), passes)
+        assert np.array_equal(part, st.part), (g, metric, K)
+        ref_m = O.metrics(hg, K, st.part)
+        assert info["metrics"] == {k: ref_m[k] for k in O.METRIC_NAMES}, (g, metric, K)
+        if moves is not None:
+            assert info["moves"] == moves, (g, metric, K)
 
 
 def test_serve_budget_bounds_the_bytes_in_flight():

@@ -468,3 +468,39 @@ def test_bal_admission_check_catches_planted_mistakes():
             assert caught, f"planted mistake {code} not caught"
     finally:
         O.plant(0)
+
+
+@pytest.mark.parametrize("kind", ["tv", "cn", "msv", "msrv", "tm", "msm"])
+def test_weighted_sizes_follow_the_moves(kind):
+    # with vertex weights every move changes the part sizes by the vertex's
+    # weight (unit weights are the reference's case): after the initial
+    # partition and after refinement the sizes are the weighted histogram
+    hg = H.colnet(1500 if kind != "msm" else 3000, H.DEG_UNIFORM, 1, 3, seed=5 if kind != "msm" else 2,
+                  weights=(1, 8))
+    K = {"tm": 64, "msm": 64}.get(kind, 12)
+    C, max_pw, min_pw = H.ref_constants(hg.total_weight(), K)
+    if kind == "tv":
+        # the serial schedule updates both sizes (the round schedule keeps the
+        # reference's source-only update, TV/partition.cpp:827-829)
+        st, marker = O.tv_ldg(hg, K, C, max_pw, min_pw)
+        init = st.copy()
+        O.tv_refine_serial(hg, st, marker, C, max_pw, min_pw, 4)
+    elif kind == "cn":
+        st = O.cn_ldg(hg, K, C)
+        init = st.copy()
+        O.cn_refine(hg, st, 4, serial=False)
+    elif kind in ("msv", "msrv"):
+        st = (O.msv_ldg if kind == "msv" else O.msrv_ldg)(hg, K, C)
+        init = st.copy()
+        (O.msv_refine_par if kind == "msv" else O.msrv_refine_par)(hg, st, C, 4, 256)
+    else:
+        st = (O.tm_init if kind == "tm" else O.msm_init)(hg, K, C)
+        init = st.copy()
+        if kind == "tm":
+            O.tm_refine_par(hg, st, C, 4, hg.nov)
+        else:
+            O.msm_refine_par(hg, st, 4, 128)
+    for s in (init, st):
+        want = np.bincount(s.part, weights=hg.vwgt, minlength=K).astype(np.int64)
+        assert np.array_equal(s.sizes.astype(np.int64), want), kind
+    assert np.count_nonzero(init.part != st.part) > 0, kind
