@@ -131,10 +131,11 @@ def run_batch(args):
     specs = batch_specs(args.instances, args.seed)
 
     def prep(spec):
+        # each member starts from its objective's own initial stream
         i, n, K, metric = spec
         hg = H.colnet(n, H.DEG_UNIFORM, 4, 32, seed=1000 + i)
         C, max_pw, min_pw = H.ref_constants(hg.nov, K, eps)
-        part = H.initial_partition(hg, K, C, max_pw, min_pw)
+        part = H.initial_partition_for(metric, hg, K, eps)
         return hg, part, batch_schedule(metric, hg.nov, K, C, max_pw, min_pw, H.balance_cap(hg.nov, K, eps))
 
     t = time.time()
@@ -242,15 +243,21 @@ def run_serve(args):
     with ThreadPoolExecutor(max_workers=workers) as ex:
         graphs = list(ex.map(prep_graph, range(args.graphs)))
 
-    def prep_part(gk):
-        g, K = gk
+    # a request starts from its objective's own initial stream (TV LDG2, the
+    # CN stream, band-free LDG for MSV/MSRV, the message greedy for TM/MSM)
+    def prep_part(key):
+        g, K, mode = key
         hg = graphs[g]
         C, max_pw, min_pw = H.ref_constants(hg.nov, K, eps)
-        return H.initial_partition(hg, K, C, max_pw, min_pw)
+        return H.initial_partition_mode(hg, mode, K, C, max_pw, min_pw)
 
-    keys = [(g, K) for g in range(args.graphs) for K in Ks]
+    modes = sorted(set(H.INIT_OF.values()))
+    keys = [(g, K, m) for g in range(args.graphs) for K in Ks for m in modes]
     with ThreadPoolExecutor(max_workers=workers) as ex:
         parts = dict(zip(keys, ex.map(prep_part, keys)))
+
+    def part_of(g, K, metric):
+        return parts[(g, K, H.INIT_OF[metric])]
     # arrivals: Poisson process at args.rate requests/s for args.duration s
     arrivals = []
     at = rng.exponential(1.0 / args.rate)
@@ -283,7 +290,7 @@ def run_serve(args):
                 delay = start + a - time.perf_counter()
                 if delay > 0:
                     time.sleep(delay)
-                rids[i] = srv.submit(g, metric, K, parts[(g, K)], scheds[i], args.passes)
+                rids[i] = srv.submit(g, metric, K, part_of(g, K, metric), scheds[i], args.passes)
                 if a - last >= 10:
                     last = a
                     print(f"# serve: {i + 1} submitted at {a:.0f}s", file=sys.stderr, flush=True)
@@ -307,7 +314,7 @@ def run_serve(args):
     for (info, part), (a, g, K, metric) in zip(infos[:args.check], arrivals[:args.check]):
         with Engine(0) as e:
             e.upload(graphs[g])
-            e.set_partition(K, parts[(g, K)])
+            e.set_partition(K, part_of(g, K, metric))
             e.refine(metric, args.passes, sched_of(g, K, metric))
             same = same and np.array_equal(e.get_state().part, part) and e.metrics() == info["metrics"]
     out = {

@@ -182,16 +182,26 @@ int fp_csr_transpose(int nov, int noe, const unsigned* net_ptr, const unsigned* 
   return FP_OK;
 }
 
-// Streaming initial partition. mode FP_INIT_LDG_BANDS is the LDG2 rule of
-// TV/partition.cpp:89-145 as used by the TV stream (TV/partition.cpp:207-264);
-// FP_INIT_LDG is the band-free LDG of MGV/partition.cpp:28-65. Only the part
-// vector is produced; the connectivity lists the stream leaves are the
-// first-appearance order that fp_set_partition rebuilds on the device.
+// Streaming initial partitions (only the part vector is produced; the
+// connectivity lists a stream leaves are rebuilt on the device by
+// fp_set_partition in first-appearance order).
+//
+//   FP_INIT_LDG_BANDS  LDG2 of TV/partition.cpp:89-145 as the TV stream uses it
+//                      (TV/partition.cpp:207-264)
+//   FP_INIT_LDG        band-free LDG of MGV/partition.cpp:28-65 (the MSV and
+//                      MSRV streams, MGV/partition.cpp:122-189)
+//   FP_INIT_CN_LDG     the CN stream (CN/partition.cpp:132-221): only nets that
+//                      are not cut (one part so far) vote, no bands
+//   FP_INIT_MSG_GREEDY the TM/MSM greedy (TM/partition.cpp:87-250; the MSM
+//                      stream is the same, MSM/partition.cpp:146-308): the part
+//                      among those lighter than C that saves the most messages
 int fp_initial_partition(int mode, int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr,
                          const unsigned* vtx_nets, const int* ordering, const int* vwgt, int K, double C,
                          long long max_pw, long long min_pw, int* part) {
   if (nov < 1 || noe < 0 || K < 1 || !net_ptr || !vtx_ptr || !vtx_nets || !part) return FP_ERR_ARG;
-  if (mode != FP_INIT_LDG_BANDS && mode != FP_INIT_LDG) return FP_ERR_ARG;
+  if (mode == FP_INIT_MSG_GREEDY) return FP_ERR_ARG;  // needs the net sources: fp_initial_partition_msg
+  if (mode != FP_INIT_LDG_BANDS && mode != FP_INIT_LDG && mode != FP_INIT_CN_LDG) return FP_ERR_ARG;
+  const bool cut_free = mode == FP_INIT_CN_LDG;
   const long long pins = net_ptr[noe];
   std::vector<int> lam((size_t)noe, 0), lp((size_t)pins, -1);
   std::vector<long long> sizes((size_t)K, 0);
@@ -200,12 +210,14 @@ int fp_initial_partition(int mode, int nov, int noe, const unsigned* net_ptr, co
   for (int v = 0; v < nov; v++) part[v] = -1;
   for (int z = 0; z < nov; z++) {
     int v = ordering ? ordering[z] : z;
-    if (v < 0 || v >= nov) return FP_ERR_ARG;
+    if (v < 0 || v >= nov || part[v] != -1) return FP_ERR_ARG;
     active.clear();
     for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
       int n = (int)vtx_nets[i];
       const int* L = &lp[net_ptr[n]];
-      for (int j = 0; j < lam[(size_t)n]; j++) {
+      const int l = lam[(size_t)n];
+      if (cut_free && l != 1) continue;  // CN/partition.cpp:140-143: a cut net has no say
+      for (int j = 0; j < l; j++) {
         int q = L[j];
         if (seen[(size_t)q] != v) {
           seen[(size_t)q] = v;
@@ -241,6 +253,147 @@ int fp_initial_partition(int mode, int nov, int noe, const unsigned* net_ptr, co
       int l = lam[(size_t)n], j = 0;
       while (j < l && L[j] != d) j++;
       if (j == l) { L[l] = d; lam[(size_t)n] = l + 1; }
+    }
+  }
+  return FP_OK;
+}
+
+namespace {
+
+// K counters of up to 2^levels - 1 each, bit-sliced over K-bit rows: adding a
+// row of K bits costs W = ceil(K/64) words per level the carry reaches.
+struct SlicedCounts {
+  int W = 0, levels = 0;
+  std::vector<uint64_t> c;  // levels x W
+  void reset(int w, int lv) {
+    W = w;
+    levels = lv;
+    c.assign((size_t)lv * (size_t)w, 0ull);
+  }
+  void add(const uint64_t* row) {
+    for (int w = 0; w < W; w++) {
+      uint64_t carry = row[w];
+      for (int b = 0; b < levels && carry; b++) {
+        uint64_t& x = c[(size_t)b * (size_t)W + (size_t)w];
+        const uint64_t t = x & carry;
+        x ^= carry;
+        carry = t;
+      }
+    }
+  }
+  int get(int k) const {
+    int v = 0;
+    const int w = k >> 6, sh = k & 63;
+    for (int b = 0; b < levels; b++) v |= (int)((c[(size_t)b * (size_t)W + (size_t)w] >> sh) & 1ull) << b;
+    return v;
+  }
+};
+
+}  // namespace
+
+// The TM/MSM greedy stream (TM/partition.cpp:87-250). The message bookkeeping
+// of the reference -- part_connections[a][b], the number of nets with their
+// source in a and a pin in b != a, and part_messages[a] -- is kept as counts
+// plus the rows and columns of its nonzero pattern as bitmaps, so that a
+// vertex's savings for all K parts are sums of bitmap rows:
+//   saved[k] = #{source parts s of v's nets placed so far : pc[s][k] > 0}
+//            + sum over v's own nets (v is their source) of
+//              #{parts q in the net's list : pc[k][q] > 0}
+// The vertex goes to the part lighter than C with the most savings, ties to
+// the smaller size, then the lower index; with none lighter than C, to the
+// lowest-index part of minimum size.
+int fp_initial_partition_msg(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr,
+                             const unsigned* vtx_nets, const int* src, const int* ordering, const int* vwgt, int K,
+                             double C, int* part) {
+  if (nov < 1 || noe < 0 || K < 1 || !net_ptr || !vtx_ptr || !vtx_nets || !src || !part) return FP_ERR_ARG;
+  if (K > 4096) return FP_ERR_UNSUPPORTED;  // a K x K count matrix
+  const long long pins = net_ptr[noe];
+  const int W = (K + 63) / 64;
+  std::vector<int> pc((size_t)K * (size_t)K, 0);
+  std::vector<uint64_t> rows((size_t)K * (size_t)W, 0ull), cols((size_t)K * (size_t)W, 0ull);
+  std::vector<int> lam((size_t)noe, 0), lp((size_t)pins, -1), lc((size_t)pins, 0), src_part((size_t)noe, -1);
+  std::vector<long long> sizes((size_t)K, 0);
+  std::vector<int> smark((size_t)K, -1);
+  std::vector<int> sparts, own_nets;
+  SlicedCounts cnt;
+  for (int v = 0; v < nov; v++) part[v] = -1;
+  auto set_pc = [&](int a, int b) {  // pc[a][b]++ (a != b)
+    int& x = pc[(size_t)a * (size_t)K + (size_t)b];
+    if (x++ == 0) {
+      rows[(size_t)a * (size_t)W + (size_t)(b >> 6)] |= 1ull << (b & 63);
+      cols[(size_t)b * (size_t)W + (size_t)(a >> 6)] |= 1ull << (a & 63);
+    }
+  };
+  for (int z = 0; z < nov; z++) {
+    const int v = ordering ? ordering[z] : z;
+    if (v < 0 || v >= nov || part[v] != -1) return FP_ERR_ARG;
+    sparts.clear();
+    own_nets.clear();
+    int terms = 0;
+    for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
+      const int n = (int)vtx_nets[i];
+      if (src[n] == v) {
+        own_nets.push_back(n);
+        terms += lam[(size_t)n];
+      }
+      const int sp = src_part[(size_t)n];
+      if (sp >= 0 && smark[(size_t)sp] != v) {
+        smark[(size_t)sp] = v;
+        sparts.push_back(sp);
+        terms++;
+      }
+    }
+    int levels = 1;
+    while ((1 << levels) <= terms) levels++;
+    cnt.reset(W, levels);
+    for (int sp : sparts) cnt.add(&rows[(size_t)sp * (size_t)W]);
+    for (int n : own_nets) {
+      const int* L = &lp[net_ptr[n]];
+      for (int j = 0; j < lam[(size_t)n]; j++) cnt.add(&cols[(size_t)L[j] * (size_t)W]);
+    }
+    int d = -1, best = -1;
+    for (int k = 0; k < K; k++) {
+      if (!((double)sizes[(size_t)k] < C)) continue;
+      const int saved = cnt.get(k);
+      if (saved > best || (saved == best && sizes[(size_t)k] < sizes[(size_t)d])) {
+        best = saved;
+        d = k;
+      }
+    }
+    if (d == -1) {
+      long long m = LLONG_MAX;
+      for (int k = 0; k < K; k++)
+        if (sizes[(size_t)k] < m) { m = sizes[(size_t)k]; d = k; }
+    }
+    part[v] = d;
+    sizes[(size_t)d] += vwgt ? vwgt[v] : 1;
+    // the vertex joins its nets' lists, then the messages follow
+    // (TM/partition.cpp:175-235): a net whose source is placed gains a
+    // message to d when d is new to it; v's own nets send from d to every
+    // other part in them
+    for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
+      const int n = (int)vtx_nets[i];
+      int* L = &lp[net_ptr[n]];
+      int* Lc = &lc[net_ptr[n]];
+      const int l = lam[(size_t)n];
+      int j = 0;
+      while (j < l && L[j] != d) j++;
+      const bool fresh = j == l;
+      if (fresh) {
+        L[l] = d;
+        Lc[l] = 1;
+        lam[(size_t)n] = l + 1;
+      } else {
+        Lc[j]++;
+      }
+      const int sp = src_part[(size_t)n];
+      if (sp >= 0 && sp != d && (fresh || pc[(size_t)sp * (size_t)K + (size_t)d] == 0)) set_pc(sp, d);
+    }
+    for (int n : own_nets) {
+      src_part[(size_t)n] = d;
+      const int* L = &lp[net_ptr[n]];
+      for (int j = 0; j < lam[(size_t)n]; j++)
+        if (L[j] != d) set_pc(d, L[j]);
     }
   }
   return FP_OK;

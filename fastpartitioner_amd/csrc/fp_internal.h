@@ -44,6 +44,16 @@ __host__ __device__ __forceinline__ unsigned short ev_word(unsigned x, int pb, u
   return (unsigned short)((x & pmask) | ((x >> pb) == 1u ? EV_SINGLE : 0u));
 }
 
+// A part size as an unsigned key that orders like the signed size: the
+// reference compares int part sizes (TV/partition.cpp:108), and its round
+// schedule's source-only size update (TV/partition.cpp:827-829) can drive a
+// size below zero. Sizes outside the int range, which the reference's own
+// counters cannot hold, saturate.
+__host__ __device__ __forceinline__ unsigned size_key(long long sz) {
+  const long long c = sz < (long long)INT_MIN ? (long long)INT_MIN : (sz > (long long)INT_MAX ? (long long)INT_MAX : sz);
+  return (unsigned)(int)c ^ 0x80000000u;
+}
+
 // ---- wave helpers ---------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
 // index of the wavefront in its block, as a wave-uniform (scalar) value
@@ -782,6 +792,24 @@ void fill_async(fp_ctx* ctx, T* p, size_t n, T v) {
   if (n == 0) return;
   const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
   k_fill<T><<<blocks, 256, 0, ctx->stream>>>(p, n, v);
+}
+
+// With ctx->kernel_timing, events around the next launch of a pass's
+// dominant kernel (fp_last_kernel_timing adds them up after fp_refine):
+// timing_begin before the launch, timing_end after it.
+inline fp_status timing_begin(fp_ctx* ctx, hipEvent_t* tb) {
+  *tb = nullptr;
+  if (!ctx->kernel_timing) return FP_OK;
+  hipEvent_t ta = pooled_event(ctx, ctx->timed_next);
+  *tb = pooled_event(ctx, ctx->timed_next + 1);
+  if (!ta || !*tb) return fail(ctx, FP_ERR_HIP, "hipEventCreate failed");
+  ctx->timed_next += 2;
+  if (hipEventRecord(ta, ctx->stream) != hipSuccess) return fail(ctx, FP_ERR_HIP, "hipEventRecord failed");
+  return FP_OK;
+}
+inline fp_status timing_end(fp_ctx* ctx, hipEvent_t tb) {
+  if (tb && hipEventRecord(tb, ctx->stream) != hipSuccess) return fail(ctx, FP_ERR_HIP, "hipEventRecord failed");
+  return FP_OK;
 }
 
 // k_pass_begin on the context's stream (val may be null)

@@ -1,8 +1,8 @@
 // fp_metrics.hip -- the six metrics of shared/metric.cpp from scratch on the
 // device, in three launches:
 //
-//  k_metrics   wavefronts take chunks of 64 consecutive nets from a work
-//              counter. Nets of at most 64 pins are packed whole into 64-lane
+//  k_metrics   wavefronts take chunks of 64 consecutive nets round robin.
+//              Nets of at most 64 pins are packed whole into 64-lane
 //              windows, one pin per lane (coalesced pin loads, one part gather
 //              per pin); the lanes' (net, part) keys are sorted across the
 //              wavefront (bitonic, in registers), so a key that differs from
@@ -14,9 +14,9 @@
 //              row in HBM by atomics); each (source part,
 //              receiving part) pair sets a bit of the message bitmap (a plain
 //              read first, so known pairs cost no atomic).
-//  k_metrics_reduce  column sums of the partial rows, and the messages sent
-//              and received per part as popcounts of the bitmap's rows and
-//              columns.
+//  k_metrics_colsum   column sums of the partial rows;
+//  k_metrics_msgcount the messages sent and received per part, popcounts of
+//              the bitmap's rows and columns;
 //  k_metrics_final   TM, MSM, MSV and MSRV from the per-part vectors.
 //
 // Algorithmic bytes per evaluation: 8 per pin (pin id + its part) + 12 per
@@ -35,24 +35,31 @@ constexpr int MET_LDS_KMAX = 4096;      // per-part counters in LDS up to here, 
 constexpr int MET_KMAX = 16384;
 constexpr unsigned KEY_NONE = 0xFFFFFFFFu;
 
-// Offsets (in words) of the per-evaluation scratch: the work counter, the
-// message bitmap (K rows of W = ceil(K/32) words) and the partial rows of
-// PS = 3K + 2 words (send as int, receive, MGV send, TV, CN).
+constexpr int MET_LDS_MSG_KMAX = 512;   // per-block message bitmaps in LDS up to here
+
+// Offsets (in words) of the per-evaluation scratch: the message bitmap (K
+// rows of W = ceil(K/32) words) and the partial rows of PR words: PS = 3K + 2
+// counters (send as int, receive, MGV send, TV, CN), then, with per-block
+// message bitmaps, the block's K x W bitmap.
 struct MetLayout {
-  int K, W, PS, grid;
+  int K, W, PS, PR, grid;
+  bool lds_acc, lds_msg;
   size_t bitmap_off, partial_off, words;
 };
 
-MetLayout met_layout(int K, int noe) {
+MetLayout met_layout(int K, int noe, int grid_max) {
   MetLayout L{};
   L.K = K;
   L.W = (K + 31) / 32;
   L.PS = 3 * K + 2;
+  L.lds_acc = K <= MET_LDS_KMAX;
+  L.lds_msg = K <= MET_LDS_MSG_KMAX;
+  L.PR = L.PS + (L.lds_msg ? K * L.W : 0);
   const int chunks = std::max(1, (noe + MET_CHUNK - 1) / MET_CHUNK);
-  L.grid = std::max(1, std::min(MET_GRID_MAX, (chunks + MET_WAVES - 1) / MET_WAVES));
-  L.bitmap_off = 64;  // the counter has a 256-byte line of its own
+  L.grid = std::max(1, std::min(grid_max, (chunks + MET_WAVES - 1) / MET_WAVES));
+  L.bitmap_off = 0;
   L.partial_off = L.bitmap_off + (size_t)K * (size_t)L.W;
-  L.words = L.partial_off + (size_t)(K <= MET_LDS_KMAX ? L.grid : 1) * (size_t)L.PS;
+  L.words = L.partial_off + (size_t)(L.lds_acc ? L.grid : 1) * (size_t)L.PR;
   return L;
 }
 
@@ -61,11 +68,10 @@ struct MetricArgs {
   const unsigned* net_pins;
   const int* part;
   const int* src;
-  int noe, K, nov, pb, W, PS;
+  int noe, K, nov, pb, W, PS, PR;
   int* lam;             // noe, nullable
-  unsigned* counter;    // work counter, zeroed
   unsigned* bitmap;     // K x W words, zeroed
-  unsigned* partial;    // gridDim.x rows of PS words
+  unsigned* partial;    // gridDim.x rows of PR words
 };
 
 // Inclusive max-scan over the wave (values >= -1); the DPP steps move -1 into
@@ -138,8 +144,11 @@ constexpr int MET_GROUP = 4;  // windows (or long-net steps of 64 pins) loaded t
 
 // LDS_ACC: per-part counters block-private in LDS, stored as row blockIdx.x
 // of the partial sums; otherwise (K above MET_LDS_KMAX) atomics go straight
-// to partial row 0, zeroed beforehand.
-template <bool LDS_ACC>
+// to partial row 0, zeroed beforehand. LDS_MSG: the block's messages are bits
+// of its own bitmap in LDS, stored after its counters (K up to
+// MET_LDS_MSG_KMAX); otherwise each wavefront queues them for the global
+// bitmap.
+template <bool LDS_ACC, bool LDS_MSG>
 __global__ void __launch_bounds__(MET_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) k_metrics(MetricArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int K = a.K, W = a.W, pb = a.pb;
@@ -149,25 +158,35 @@ __global__ void __launch_bounds__(MET_BLOCK) __attribute__((amdgpu_waves_per_eu(
   unsigned* s_recv = acc + K;
   unsigned* s_msv = s_recv + K;
   unsigned* s_tvcn = LDS_ACC ? s_msv + K : reinterpret_cast<unsigned*>(lds);  // [0] TV, [1] CN
+  unsigned* s_msg = s_tvcn + 2;                                                // LDS_MSG: K x W bits
   const int lane = lane_id(), wave = wave_id();
-  unsigned* seen = s_tvcn + 2 + (size_t)wave * (size_t)(W + WAVE + MSG_Q);  // W words: parts of the current long net
+  const int per_wave = W + WAVE + (LDS_MSG ? 0 : MSG_Q);
+  unsigned* seen = s_msg + (LDS_MSG ? (size_t)K * (size_t)W : 0) + (size_t)wave * (size_t)per_wave;  // W words
   int* winbuf = reinterpret_cast<int*>(seen + W);  // 64: net starting at each window lane
-  unsigned* mq = seen + W + WAVE;                  // MSG_Q: queued message bits
+  unsigned* mq = seen + W + WAVE;                  // !LDS_MSG: MSG_Q queued message bits
   if (LDS_ACC)
     for (int t = (int)threadIdx.x; t < 3 * K; t += (int)blockDim.x) acc[t] = 0;
+  if (LDS_MSG)
+    for (int t = (int)threadIdx.x; t < K * W; t += (int)blockDim.x) s_msg[t] = 0;
   if (threadIdx.x < 2) s_tvcn[threadIdx.x] = 0;
   for (int t = lane; t < W; t += WAVE) seen[t] = 0;
   __syncthreads();
   unsigned tv = 0, cn = 0;  // this wave's running sums (every lane holds them)
   int mqn = 0;              // queued messages
+  // the message (sp -> qp) of every lane with `want`
+  auto message = [&](bool want, int sp, int qp) {
+    if (LDS_MSG) {
+      if (want) atomicOr(&s_msg[(size_t)sp * (size_t)W + (unsigned)(qp >> 5)], 1u << (qp & 31));
+    } else {
+      msg_push(a.bitmap, mq, mqn, want, sp, qp, W);
+    }
+  };
 
-  int chunk = 0;
-  if (lane == 0) chunk = (int)atomicAdd(a.counter, 1u);
-  chunk = __builtin_amdgcn_readfirstlane(chunk);
-  while ((long long)chunk * MET_CHUNK < a.noe) {
-    // the next chunk's index is fetched while this one is processed
-    int next = 0;
-    if (lane == 0) next = (int)atomicAdd(a.counter, 1u);
+  // chunks go to the wavefronts round robin: one counter that every
+  // wavefront took its next chunk from serialised those atomics at one
+  // address and cost the kernel half its time
+  const int waves_total = (int)gridDim.x * MET_WAVES;
+  for (int chunk = (int)blockIdx.x * MET_WAVES + wave; (long long)chunk * MET_CHUNK < a.noe; chunk += waves_total) {
     const int c = chunk * MET_CHUNK;
     const int nc = min(MET_CHUNK, a.noe - c);
     // lane j: net c + j
@@ -240,7 +259,7 @@ __global__ void __launch_bounds__(MET_BLOCK) __attribute__((amdgpu_waves_per_eu(
         const int q = (int)(key & pmask);
         const bool recv = fresh && spk >= 0 && q != spk;
         if (recv) atomicAdd(&s_recv[q], 1u);
-        msg_push(a.bitmap, mq, mqn, recv, spk, q, W);
+        message(recv, spk, q);
         if (in_win) {
           const unsigned o = start - base;
           const unsigned long long seg = len >= WAVE ? ~0ull : (((1ull << len) - 1ull) << o);
@@ -276,7 +295,7 @@ __global__ void __launch_bounds__(MET_BLOCK) __attribute__((amdgpu_waves_per_eu(
             l += (int)__popcll(__ballot(fresh));
             const bool recv = fresh && spL >= 0 && lq[u] != spL;
             if (recv) atomicAdd(&s_recv[lq[u]], 1u);
-            msg_push(a.bitmap, mq, mqn, recv, spL, lq[u], W);
+            message(recv, spL, lq[u]);
           }
         }
         occ = wave_sum(occ);
@@ -300,71 +319,113 @@ __global__ void __launch_bounds__(MET_BLOCK) __attribute__((amdgpu_waves_per_eu(
     }
     tv += (unsigned)wave_sum(lane < nc && lamj != 0 ? lamj - 1 : 0);
     cn += (unsigned)wave_sum(lane < nc && lamj > 1 ? 1 : 0);
-    chunk = __builtin_amdgcn_readfirstlane(next);
   }
-  if (mqn) msg_flush(a.bitmap, mq, mqn);
+  if (!LDS_MSG && mqn) msg_flush(a.bitmap, mq, mqn);
   if (lane == 0) {
     atomicAdd(&s_tvcn[0], tv);
     atomicAdd(&s_tvcn[1], cn);
   }
   __syncthreads();
   if (LDS_ACC) {
-    unsigned* row = a.partial + (size_t)blockIdx.x * (size_t)a.PS;
-    for (int t = (int)threadIdx.x; t < a.PS; t += (int)blockDim.x) row[t] = acc[t];
+    // counters, TV, CN and (LDS_MSG) the block's bitmap are contiguous in LDS
+    unsigned* row = a.partial + (size_t)blockIdx.x * (size_t)a.PR;
+    for (int t = (int)threadIdx.x; t < a.PR; t += (int)blockDim.x) row[t] = acc[t];
   } else if (threadIdx.x < 2) {
     atomicAdd(&a.partial[3 * (size_t)K + threadIdx.x], s_tvcn[threadIdx.x]);
   }
 }
 
-// Blocks [0, colb): column sums of the partial rows, 64 columns per block and
-// 16 row groups. Blocks [colb, colb + kb): messages sent per part (popcount
-// of bitmap row a, one thread per part). Blocks [colb + kb, colb + 2 kb):
-// messages received (popcount of bitmap column b: the 64 lanes of a wave read
-// two words per row). out = [6 scalars][send K][recv K][sm K][rm K][msv K].
-__global__ void __launch_bounds__(1024) k_metrics_reduce(const unsigned* __restrict__ partial, int rows, int K, int W,
-                                                         const unsigned* __restrict__ bitmap, int colb, int kb,
-                                                         long long* __restrict__ out) {
+// Column sums of the partial rows into out (zeroed before): block (c, r)
+// sums the 64 columns c*64.. over its share of the rows, 16 row groups of 64
+// threads with RED_UNROLL loads in flight each, then one atomic per column.
+// out = [6 scalars][send K][recv K][sm K][rm K][msv K].
+constexpr int RED_ROW_BLOCKS = 16;
+constexpr int RED_UNROLL = 4;
+
+__global__ void __launch_bounds__(1024) k_metrics_colsum(const unsigned* __restrict__ partial, int rows, int PR,
+                                                         int K, long long* __restrict__ out) {
   const int PS = 3 * K + 2;
-  const int b = (int)blockIdx.x;
-  if (b < colb) {
-    __shared__ long long acc[16][64];
-    const int cl = (int)threadIdx.x & 63, g = (int)threadIdx.x >> 6;
-    const int col = b * 64 + cl;
-    long long s = 0;
-    if (col < PS) {
-      const bool is_signed = col < K;  // send: int
-      for (int r = g; r < rows; r += 16) {
-        const unsigned x = partial[(size_t)r * (size_t)PS + (size_t)col];
-        s += is_signed ? (long long)(int)x : (long long)x;
-      }
-    }
-    acc[g][cl] = s;
-    __syncthreads();
-    if (g == 0 && col < PS) {
-      for (int h = 1; h < 16; h++) s += acc[h][cl];
-      long long* dst;
-      if (col < K) dst = out + 6 + col;                               // send
-      else if (col < 2 * K) dst = out + 6 + K + (col - K);            // recv
-      else if (col < 3 * K) dst = out + 6 + 4 * (size_t)K + (col - 2 * K);  // MGV send
-      else dst = out + (col - 3 * K);                                 // TV, CN
-      *dst = s;
-    }
-    return;
-  }
-  if (b < colb + kb) {
-    const int p = (b - colb) * 1024 + (int)threadIdx.x;
-    if (p >= K) return;
-    long long s = 0;
-    for (int w = 0; w < W; w++) s += __popc(bitmap[(size_t)p * (size_t)W + w]);
-    out[6 + 2 * (size_t)K + p] = s;
-    return;
-  }
-  const int q = (b - colb - kb) * 1024 + (int)threadIdx.x;
-  if (q >= K) return;
-  const unsigned bit = 1u << (q & 31);
+  __shared__ long long acc[16][64];
+  const int cl = (int)threadIdx.x & 63, g = (int)threadIdx.x >> 6;
+  const int col = (int)blockIdx.x * 64 + cl;
+  const int per = (rows + (int)gridDim.y - 1) / (int)gridDim.y;
+  const int r_lo = (int)blockIdx.y * per, r_hi = min(rows, r_lo + per);
   long long s = 0;
-  for (int p = 0; p < K; p++) s += (bitmap[(size_t)p * (size_t)W + (q >> 5)] & bit) ? 1 : 0;
-  out[6 + 3 * (size_t)K + q] = s;
+  if (col < PS) {
+    const bool is_signed = col < K;  // send volumes: int
+    for (int r0 = r_lo + g; r0 < r_hi; r0 += 16 * RED_UNROLL) {
+      unsigned x[RED_UNROLL];
+#pragma unroll
+      for (int u = 0; u < RED_UNROLL; u++) {
+        const int r = r0 + 16 * u;
+        x[u] = r < r_hi ? partial[(size_t)r * (size_t)PR + (size_t)col] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < RED_UNROLL; u++) s += is_signed ? (long long)(int)x[u] : (long long)x[u];
+    }
+  }
+  acc[g][cl] = s;
+  __syncthreads();
+  if (g == 0 && col < PS) {
+    for (int h = 1; h < 16; h++) s += acc[h][cl];
+    long long* dst;
+    if (col < K) dst = out + 6 + col;                                   // send
+    else if (col < 2 * K) dst = out + 6 + K + (col - K);                // recv
+    else if (col < 3 * K) dst = out + 6 + 4 * (size_t)K + (col - 2 * K);  // MGV send
+    else dst = out + (col - 3 * K);                                     // TV, CN
+    if (s) atomicAdd(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)s);
+  }
+}
+
+// The message bitmap (zeroed before) as the OR of the blocks' bitmaps, which
+// follow their counters in the partial rows: block (c, r) ORs words c*1024..
+// over its share of the rows, RED_UNROLL loads in flight, then one atomic OR
+// per word with bits.
+__global__ void __launch_bounds__(1024) k_metrics_bitor(const unsigned* __restrict__ partial, int rows, int PR,
+                                                        int PS, int words, unsigned* __restrict__ bitmap) {
+  const int w = (int)blockIdx.x * 1024 + (int)threadIdx.x;
+  if (w >= words) return;
+  const int per = (rows + (int)gridDim.y - 1) / (int)gridDim.y;
+  const int r_lo = (int)blockIdx.y * per, r_hi = min(rows, r_lo + per);
+  unsigned m = 0;
+  for (int r0 = r_lo; r0 < r_hi; r0 += RED_UNROLL) {
+    unsigned x[RED_UNROLL];
+#pragma unroll
+    for (int u = 0; u < RED_UNROLL; u++)
+      x[u] = r0 + u < r_hi ? partial[(size_t)(r0 + u) * (size_t)PR + (size_t)PS + (size_t)w] : 0u;
+#pragma unroll
+    for (int u = 0; u < RED_UNROLL; u++) m |= x[u];
+  }
+  if (m) atomicOr(&bitmap[w], m);
+}
+
+// Messages sent and received per part from the message bitmap (K rows of W
+// words, zeroed out[6 + 2K ..] before): block b takes rows b*16 .. b*16+15;
+// sent: one wavefront per row sums its words' popcounts; received: thread t
+// counts column t (and t + 1024, ...) over the block's rows, one atomic per
+// column with a count.
+constexpr int MSG_ROWS = 16;
+
+__global__ void __launch_bounds__(1024) k_metrics_msgcount(const unsigned* __restrict__ bitmap, int K, int W,
+                                                           long long* __restrict__ out) {
+  const int p0 = (int)blockIdx.x * MSG_ROWS, p1 = min(K, p0 + MSG_ROWS);
+  const int lane = lane_id(), wave = (int)threadIdx.x / WAVE;
+  for (int p = p0 + wave; p < p1; p += (int)blockDim.x / WAVE) {
+    int c = 0;
+    for (int w = lane; w < W; w += WAVE) c += __popc(bitmap[(size_t)p * (size_t)W + w]);
+    c = wave_sum(c);
+    if (lane == 0) out[6 + 2 * (size_t)K + p] = c;
+  }
+  for (int q = (int)threadIdx.x; q < K; q += (int)blockDim.x) {
+    const unsigned bit = 1u << (q & 31);
+    unsigned x[MSG_ROWS];
+#pragma unroll
+    for (int u = 0; u < MSG_ROWS; u++) x[u] = p0 + u < p1 ? bitmap[(size_t)(p0 + u) * (size_t)W + (q >> 5)] : 0u;
+    int c = 0;
+#pragma unroll
+    for (int u = 0; u < MSG_ROWS; u++) c += (x[u] & bit) ? 1 : 0;
+    if (c) atomicAdd(reinterpret_cast<unsigned long long*>(out + 6 + 3 * (size_t)K + q), (unsigned long long)c);
+  }
 }
 
 // TM = sum of messages sent, MSM = their max, MSV = max MGV send volume,
@@ -411,9 +472,11 @@ __global__ void __launch_bounds__(1024) k_metrics_final(int K, long long* __rest
   }
 }
 
-size_t met_lds_bytes(int K, int W) {
-  const size_t acc = K <= MET_LDS_KMAX ? (size_t)3 * K : 0;
-  return (acc + 2) * sizeof(unsigned) + (size_t)MET_WAVES * (size_t)(W + WAVE + MSG_Q) * sizeof(unsigned);
+size_t met_lds_bytes(const MetLayout& L) {
+  const size_t acc = L.lds_acc ? (size_t)3 * L.K : 0;
+  const size_t msg = L.lds_msg ? (size_t)L.K * (size_t)L.W : 0;
+  const size_t per_wave = (size_t)L.W + WAVE + (L.lds_msg ? 0 : MSG_Q);
+  return (acc + 2 + msg + (size_t)MET_WAVES * per_wave) * sizeof(unsigned);
 }
 
 fp_status met_check(fp_ctx* ctx, int K) {
@@ -428,9 +491,8 @@ fp_status met_check(fp_ctx* ctx, int K) {
 
 // Enqueues the three launches on ctx->stream. scratch: L.words words; out: 6 + 5K.
 fp_status met_launch(fp_ctx* ctx, const MetLayout& L, unsigned* scratch, long long* out, int* lam) {
-  const bool lds_acc = L.K <= MET_LDS_KMAX;
-  // the counter and the bitmap; without LDS counters also partial row 0
-  fill_async<unsigned>(ctx, scratch, L.partial_off + (lds_acc ? 0 : (size_t)L.PS), 0u);
+  // the bitmap; without LDS counters also partial row 0
+  fill_async<unsigned>(ctx, scratch, L.partial_off + (L.lds_acc ? 0 : (size_t)L.PR), 0u);
   MetricArgs m{};
   m.net_ptr = ctx->net_ptr;
   m.net_pins = ctx->net_pins;
@@ -442,16 +504,26 @@ fp_status met_launch(fp_ctx* ctx, const MetLayout& L, unsigned* scratch, long lo
   m.pb = ctx->pb;
   m.W = L.W;
   m.PS = L.PS;
+  m.PR = L.PR;
   m.lam = lam;
-  m.counter = scratch;
   m.bitmap = scratch + L.bitmap_off;
   m.partial = scratch + L.partial_off;
-  const size_t lds = met_lds_bytes(L.K, L.W);
-  if (lds_acc) k_metrics<true><<<L.grid, MET_BLOCK, lds, ctx->stream>>>(m);
-  else k_metrics<false><<<L.grid, MET_BLOCK, lds, ctx->stream>>>(m);
-  const int colb = (L.PS + 63) / 64, kb = (L.K + 1023) / 1024;
-  k_metrics_reduce<<<colb + 2 * kb, 1024, 0, ctx->stream>>>(m.partial, lds_acc ? L.grid : 1, L.K, L.W, m.bitmap,
-                                                            colb, kb, out);
+  const size_t lds = met_lds_bytes(L);
+  if (L.lds_msg) k_metrics<true, true><<<L.grid, MET_BLOCK, lds, ctx->stream>>>(m);
+  else if (L.lds_acc) k_metrics<true, false><<<L.grid, MET_BLOCK, lds, ctx->stream>>>(m);
+  else k_metrics<false, false><<<L.grid, MET_BLOCK, lds, ctx->stream>>>(m);
+  // the output vectors are sums of atomics: zero first (a fill kernel, so
+  // that the evaluation can be captured and replayed)
+  fill_async<long long>(ctx, out, (size_t)6 + 5 * (size_t)L.K, 0ll);
+  const int rows = L.lds_acc ? L.grid : 1;
+  const unsigned ry = (unsigned)std::min(RED_ROW_BLOCKS, rows);
+  k_metrics_colsum<<<dim3((unsigned)((L.PS + 63) / 64), ry), 1024, 0, ctx->stream>>>(m.partial, rows, L.PR, L.K, out);
+  if (L.lds_msg) {
+    const int words = L.K * L.W;
+    k_metrics_bitor<<<dim3((unsigned)((words + 1023) / 1024), ry), 1024, 0, ctx->stream>>>(m.partial, rows, L.PR,
+                                                                                           L.PS, words, m.bitmap);
+  }
+  k_metrics_msgcount<<<(L.K + MSG_ROWS - 1) / MSG_ROWS, 1024, 0, ctx->stream>>>(m.bitmap, L.K, L.W, out);
   k_metrics_final<<<1, 1024, 0, ctx->stream>>>(L.K, out);
   if (hipGetLastError() != hipSuccess) return fail(ctx, FP_ERR_HIP, "metric kernel launch failed");
   return FP_OK;
@@ -471,7 +543,7 @@ fp_status fp_eval(fp_ctx* ctx, fp_metrics* out, int* lam, long long* sv, long lo
   FP_HIP_OK(hipSetDevice(ctx->device));
   const int K = ctx->K;
   FP_TRY(met_check(ctx, K));
-  const MetLayout L = met_layout(K, ctx->noe);
+  const MetLayout L = met_layout(K, ctx->noe, MET_GRID_MAX);
   if (L.words > ctx->met_words) {
     FP_TRY(dalloc(ctx, &ctx->met_scratch, L.words));
     ctx->met_words = L.words;

@@ -272,8 +272,8 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
     // candidates: lexicographic max of (score, -size, -first rank) over score >
     // 0 equals the sequential LDG2 scan over the active parts
     // (TV/partition.cpp:98-116). Scores are positive doubles, so their bit
-    // patterns order like the values: key1 = score bits (max), key2 = size <<
-    // 32 | first rank (min among key1 ties).
+    // patterns order like the values: key1 = score bits (max), key2 =
+    // size_key(size) << 32 | first rank (min among key1 ties).
     const unsigned cfb = cnt[fb];
     unsigned long long b1 = 0, b2 = ~0ull;
     int bq = -1;
@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
         const double sc = (double)adj * s_pen[qc];
         const bool ok = cv[u] != 0 && sc > 0;
         const unsigned long long k1 = ok ? (unsigned long long)__double_as_longlong(sc) : 0ull;
-        const unsigned long long k2 = ((unsigned long long)s_size[qc] << 32) | fv[u];
+        const unsigned long long k2 = ((unsigned long long)size_key(s_size[qc]) << 32) | fv[u];
         const bool better = ok && (k1 > b1 || (k1 == b1 && k2 < b2));
         b1 = better ? k1 : b1;
         b2 = better ? k2 : b2;
@@ -774,7 +774,7 @@ fp_status check_envelope(fp_ctx* ctx) {
   if (eval_waves(ctx->K) == 0) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d needs more LDS than a workgroup has", ctx->K);
   if ((unsigned long long)ctx->max_vdeg + 1 >= (1ull << (32 - ctx->pb)))
     return fail(ctx, FP_ERR_UNSUPPORTED, "a vertex of degree %u is too large for K = %d", ctx->max_vdeg, ctx->K);
-  if (ctx->total_weight >= (1ll << 32)) return fail(ctx, FP_ERR_UNSUPPORTED, "total vertex weight >= 2^32");
+  if (ctx->total_weight >= (1ll << 31)) return fail(ctx, FP_ERR_UNSUPPORTED, "total vertex weight >= 2^31");
   return FP_OK;
 }
 

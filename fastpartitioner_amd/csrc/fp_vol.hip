@@ -183,7 +183,7 @@ __global__ void __launch_bounds__(VOL_BLOCK) k_vol_eval(VolEvalArgs a) {
         const double sc = (double)adj * s_pen[q];
         if (sc > 0) {
           const unsigned long long k1 = (unsigned long long)__double_as_longlong(sc);
-          const unsigned long long k2 = ((unsigned long long)s_size[q] << 32) | f;
+          const unsigned long long k2 = ((unsigned long long)size_key(s_size[q]) << 32) | f;
           if (k1 > b1 || (k1 == b1 && k2 < b2)) {
             b1 = k1;
             b2 = k2;
@@ -853,7 +853,10 @@ fp_status pass_vol(fp_ctx* ctx, const RefinePlan& p) {
     a.tcount_next = ctx->counters + (par ^ 1);
     int egrid = std::max(1, std::min((nslots + waves - 1) / waves, 8192));
     if (p.eval_grid > 0) egrid = std::min(egrid, p.eval_grid);
+    hipEvent_t tb = nullptr;
+    FP_TRY(timing_begin(ctx, &tb));
     k_vol_eval<<<egrid, waves * WAVE, lds_e, ctx->stream>>>(a);
+    FP_TRY(timing_end(ctx, tb));
     launch_round_compact<false>(ctx, ctx->order, r0, nslots, vs.slot_mi, vs.mv, vs.mstat);
     VolReplayArgs ra{};
     ra.net_ptr = ctx->net_ptr;

@@ -153,6 +153,42 @@ def initial_partition(hg: Hypergraph, K: int, C: float, max_pw: int, min_pw: int
     return part
 
 
+INIT_LDG_BANDS, INIT_LDG, INIT_CN_LDG, INIT_MSG_GREEDY = 0, 1, 2, 3
+# each objective's own initial stream in the reference
+INIT_OF = {"TV": INIT_LDG_BANDS, "CN": INIT_CN_LDG, "MSV": INIT_LDG, "MSRV": INIT_LDG,
+           "TM": INIT_MSG_GREEDY, "MSM": INIT_MSG_GREEDY}
+
+
+def initial_partition_mode(hg: Hypergraph, mode: int, K: int, C: float, max_pw: int = 0, min_pw: int = 0,
+                           ordering=None) -> np.ndarray:
+    """Streaming initial partition of the given mode (fp_initial_partition,
+    fp_initial_partition_msg): LDG2 with bands, band-free LDG, the CN stream or
+    the TM/MSM message greedy."""
+    lib = _native.host()
+    part = np.zeros(hg.nov, dtype=np.int32)
+    ordp = None if ordering is None else _ptr(np.ascontiguousarray(ordering, dtype=np.int32), _I32P)
+    vw = None if hg.vwgt is None else _ptr(hg.vwgt, _I32P)
+    if mode == INIT_MSG_GREEDY:
+        rc = lib.fp_initial_partition_msg(hg.nov, hg.noe, _ptr(hg.net_ptr, _U32P), _ptr(hg.vtx_ptr, _U32P),
+                                          _ptr(hg.vtx_nets, _U32P), _ptr(hg.src, _I32P), ordp, vw, K,
+                                          ctypes.c_double(C), _ptr(part, _I32P))
+        _native.check(rc, "fp_initial_partition_msg")
+        return part
+    rc = lib.fp_initial_partition(mode, hg.nov, hg.noe, _ptr(hg.net_ptr, _U32P), _ptr(hg.vtx_ptr, _U32P),
+                                  _ptr(hg.vtx_nets, _U32P), ordp, vw, K, ctypes.c_double(C),
+                                  ctypes.c_longlong(max_pw), ctypes.c_longlong(min_pw), _ptr(part, _I32P))
+    _native.check(rc, "fp_initial_partition")
+    return part
+
+
+def initial_partition_for(metric: str, hg: Hypergraph, K: int, eps: float = 0.05, ordering=None) -> np.ndarray:
+    """The initial partition the reference refines `metric` from, with the
+    reference's constants at imbalance eps (C, max_pw, min_pw of the total
+    vertex weight)."""
+    C, max_pw, min_pw = ref_constants(hg.total_weight(), K, eps)
+    return initial_partition_mode(hg, INIT_OF[metric], K, C, max_pw, min_pw, ordering)
+
+
 def ref_constants(total_weight: int, K: int, eps: float = 0.05):
     """(C, max_pw, min_pw): the reference's LDG constants (TV/partition.cpp:95-96,159-160)."""
     lib = _native.host()

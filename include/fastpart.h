@@ -258,9 +258,10 @@ fp_status fp_server_submit(fp_server* server, int graph_id, fp_metric metric, in
 fp_status fp_server_wait(fp_server* server, long long request_id, fp_request_info* info, int* part_out);
 fp_status fp_server_stats(fp_server* server, long long* completed, int* peak_inflight, long long* peak_bytes);
 
-/* Event timing of the dominant refinement kernel in the last fp_refine:
- * launches, total device ms of those launches (HIP events around each launch,
- * on the context stream). */
+/* Event timing of the dominant refinement kernel in the last fp_refine (the
+ * evaluation: TV k_tv_eval, MSV/MSRV k_vol_eval, TM/MSM k_conn_eval; CN has
+ * none): launches, total device ms of those launches (HIP events around each
+ * launch, on the context stream). */
 fp_status fp_last_kernel_timing(fp_ctx* ctx, long long* launches, double* total_ms);
 /* Enables (1) or disables (0) per-launch event timing of fp_refine's
  * dominant kernel (adds events around every launch). */

@@ -60,14 +60,28 @@ int fp_gen_weights(int n, unsigned long long seed, int lo, int hi, int* vwgt);
 int fp_csr_transpose(int nov, int noe, const unsigned* net_ptr, const unsigned* net_pins, unsigned* vtx_ptr,
                      unsigned* vtx_nets);
 
-/* Streaming initial partition over `ordering` (NULL = identity).
+/* Streaming initial partition over `ordering` (NULL = identity), the state
+ * each objective's refinement starts from in the reference.
  *   FP_INIT_LDG_BANDS: LDG2 of TV/partition.cpp:89-145 (stream at :207-264)
- *   FP_INIT_LDG:       band-free LDG of MGV/partition.cpp:28-65
- * vwgt NULL = unit weights. Writes part[nov]. */
-enum { FP_INIT_LDG_BANDS = 0, FP_INIT_LDG = 1 };
+ *   FP_INIT_LDG:       band-free LDG of MGV/partition.cpp:28-65 (MSV, MSRV:
+ *                      MGV/partition.cpp:122-189, MSRV/partition.cpp:123-196)
+ *   FP_INIT_CN_LDG:    the CN stream, CN/partition.cpp:132-221 (votes from
+ *                      nets with one part so far only)
+ * vwgt NULL = unit weights; sizes grow by the vertex weight. Writes part[nov].
+ * FP_INIT_MSG_GREEDY is fp_initial_partition_msg's (it needs the sources). */
+enum { FP_INIT_LDG_BANDS = 0, FP_INIT_LDG = 1, FP_INIT_CN_LDG = 2, FP_INIT_MSG_GREEDY = 3 };
 int fp_initial_partition(int mode, int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr,
                          const unsigned* vtx_nets, const int* ordering, const int* vwgt, int K, double C,
                          long long max_pw, long long min_pw, int* part);
+
+/* The TM / MSM greedy stream (TM/partition.cpp:87-250, the MSM stream is the
+ * same): each vertex goes to the part lighter than C that saves the most
+ * messages given the parts placed so far (ties: smaller size, then lower
+ * index; none lighter than C: lowest-index part of minimum size). src[noe]
+ * = net sources. K <= 4096. Writes part[nov]. */
+int fp_initial_partition_msg(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr,
+                             const unsigned* vtx_nets, const int* src, const int* ordering, const int* vwgt, int K,
+                             double C, int* part);
 
 /* Balance constants. With eps = 0.05 and unit weights these are exactly the
  * reference's hard-coded C (TV/partition.cpp:159-160), max_pw and min_pw

@@ -34,7 +34,7 @@ def _workload():
         metric = OBJ[i % 6]
         K = (16, 64, 32)[i % 3]
         C, max_pw, min_pw = H.ref_constants(hg.nov, K)
-        part = H.initial_partition(hg, K, C, max_pw, min_pw)
+        part = H.initial_partition_for(metric, hg, K)   # the objective's own initial stream
         reqs.append((g, metric, K, part, _sched(metric, hg, K, C, max_pw, min_pw, H.balance_cap(hg.nov, K, 0.05)), 3))
     return graphs, reqs
 

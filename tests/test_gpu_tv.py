@@ -271,3 +271,24 @@ def test_tv_bal_headline_shape():
     assert stats.moves > 1000
     _same_state(got, st)
     assert np.array_equal(got.sizes, st.sizes)
+
+
+@pytest.mark.parametrize("scale,K,R,passes", [(0.2, 8, 16, 40), (0.1, 16, 8, 80)])
+def test_tv_ref_sizes_below_zero(scale, K, R, passes):
+    # the round schedule lowers only the source part's size
+    # (TV/partition.cpp:827-829): with the size bands open, a long run drives
+    # sizes below zero, where LDG2 still breaks score ties by the smaller
+    # size as a signed int (TV/partition.cpp:108)
+    hg = H.workload("c1", scale=scale)
+    st, C, _, _ = _ldg_state(hg, K)
+    big = 1 << 30
+    print(f"negative sizes: nov={hg.nov} K={K} R={R} passes={passes}", flush=True)
+    with Engine(0) as eng:
+        eng.upload(hg)
+        eng.set_state(K, st.part, st.ep_ends, st.ep_parts, st.ep_cnt)
+        eng.refine("TV", passes, schedule(SCHED_REF, R, C, big, -big))
+        got = eng.get_state()
+    O.tv_refine_par(hg, st, C, big, -big, passes, R)
+    assert st.sizes.min() < 0
+    _same_state(got, st)
+    assert np.array_equal(got.sizes, st.sizes.astype(np.int64))

@@ -102,6 +102,38 @@ def test_initial_partition_matches_oracle_ldg(seed):
     assert np.array_equal(H.initial_partition(hg, K, C, max_pw, min_pw), st.part)
 
 
+_INIT_CASES = [
+    # (instance, K, weights, permuted ordering)
+    (lambda: H.colnet(5000, H.DEG_UNIFORM, 2, 10, seed=1), 16, False),
+    (lambda: H.colnet(3000, H.DEG_UNIFORM, 1, 3, seed=2), 256, True),
+    (lambda: H.colnet(4000, H.DEG_ZIPF, 2, 400, 2.1, seed=3, weights=(1, 8)), 64, True),
+]
+
+
+@pytest.mark.parametrize("case", range(len(_INIT_CASES)))
+@pytest.mark.parametrize("metric", ["TV", "CN", "MSV", "MSRV", "TM", "MSM"])
+def test_objective_initial_partitions_match_the_oracle(metric, case):
+    # each objective's own stream (LDG2, the CN stream, band-free LDG, the
+    # TM/MSM message greedy) against the oracle's, which the reference's own
+    # runs pin (tests/test_oracle.py); weighted and permuted orders included
+    make, K, permuted = _INIT_CASES[case]
+    hg = make()
+    ordering = np.random.default_rng(case).permutation(hg.nov).astype(np.int32) if permuted else None
+    C, max_pw, min_pw = H.ref_constants(hg.total_weight(), K)
+    got = H.initial_partition_for(metric, hg, K, ordering=ordering)
+    if metric == "TV":
+        want = O.tv_ldg(hg, K, C, max_pw, min_pw, ordering=ordering)[0]
+    elif metric == "CN":
+        want = O.cn_ldg(hg, K, C, ordering=ordering)
+    elif metric == "MSV":
+        want = O.msv_ldg(hg, K, C, ordering=ordering)
+    elif metric == "MSRV":
+        want = O.msrv_ldg(hg, K, C, ordering=ordering)
+    else:
+        want = (O.tm_init if metric == "TM" else O.msm_init)(hg, K, C, ordering=ordering)
+    assert np.array_equal(got, want.part)
+
+
 def _declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
