@@ -24,6 +24,8 @@
 // requested).
 #include "fp_internal.h"
 
+#include <mutex>
+
 namespace fpe {
 namespace {
 
@@ -529,6 +531,49 @@ fp_status met_launch(fp_ctx* ctx, const MetLayout& L, unsigned* scratch, long lo
   return FP_OK;
 }
 
+// ---- the allocation-free evaluation's scratch -------------------------------
+// fp_eval_noalloc keeps the message bitmap and the partial rows in this
+// file-scope array instead of context memory: K <= NOALLOC_KMAX, at most
+// NOALLOC_GRID blocks. Each GPU the library runs on holds its own copy from
+// the library's load until the process ends (5 MB).
+constexpr int NOALLOC_KMAX = 1024;
+constexpr int NOALLOC_GRID = 128;
+
+constexpr size_t noalloc_words(int K) {
+  return (size_t)K * (size_t)((K + 31) / 32) +
+         (size_t)NOALLOC_GRID * ((size_t)(3 * K + 2) + (K <= MET_LDS_MSG_KMAX ? (size_t)K * (size_t)((K + 31) / 32) : 0));
+}
+// the largest layout over K <= NOALLOC_KMAX: at the last K with block
+// bitmaps or at the maximum K
+constexpr size_t NOALLOC_WORDS =
+    noalloc_words(MET_LDS_MSG_KMAX) > noalloc_words(NOALLOC_KMAX) ? noalloc_words(MET_LDS_MSG_KMAX)
+                                                                   : noalloc_words(NOALLOC_KMAX);
+__device__ __align__(256) unsigned scratch[NOALLOC_WORDS];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on ctx's device, looked up once per device
+// with that device current (ctx's calls set it) and kept; host code uses
+// only this address, never the array's name.
+fp_status noalloc_scratch(fp_ctx* ctx, unsigned** out) {
+  if (ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES)
+    return fail(ctx, FP_ERR_UNSUPPORTED, "device %d beyond the scratch table", ctx->device);
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[ctx->device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return fail(ctx, FP_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    }
+    g_scratch_ptr[ctx->device] = static_cast<unsigned*>(p);
+  }
+  *out = g_scratch_ptr[ctx->device];
+  return FP_OK;
+}
+
 }  // namespace
 }  // namespace fpe
 
@@ -574,6 +619,21 @@ fp_status fp_eval(fp_ctx* ctx, fp_metrics* out, int* lam, long long* sv, long lo
   if (rm) std::copy(v + 3 * (size_t)K, v + 4 * (size_t)K, rm);
   if (sv_msv) std::copy(v + 4 * (size_t)K, v + 5 * (size_t)K, sv_msv);
   return FP_OK;
+}
+
+fp_status fp_eval_noalloc(fp_ctx* ctx, long long* d_out, int* d_lam) {
+  if (!ctx || !d_out) return FP_ERR_ARG;
+  if (!ctx->has_state) return fail(ctx, FP_ERR_STATE, "fp_eval_noalloc before a state was set");
+  FP_HIP_OK(hipSetDevice(ctx->device));
+  const int K = ctx->K;
+  if (K > NOALLOC_KMAX)
+    return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d above %d, the allocation-free evaluation's maximum", K, NOALLOC_KMAX);
+  FP_TRY(met_check(ctx, K));
+  const MetLayout L = met_layout(K, ctx->noe, NOALLOC_GRID);
+  if (L.words > NOALLOC_WORDS) return fail(ctx, FP_ERR_UNSUPPORTED, "metric scratch beyond the static maximum");
+  unsigned* sp = nullptr;
+  FP_TRY(noalloc_scratch(ctx, &sp));
+  return met_launch(ctx, L, sp, d_out, d_lam);
 }
 
 }  // extern "C"

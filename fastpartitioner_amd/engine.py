@@ -63,6 +63,7 @@ def declare(lib) -> None:
     lib.fp_refine.argtypes = [P, c.c_int, c.c_int, c.POINTER(Schedule), c.POINTER(RefineStats)]
     lib.fp_eval_moves.argtypes = [P, c.c_int, c.POINTER(Schedule), _I32P, _I32P]
     lib.fp_eval.argtypes = [P, c.POINTER(Metrics), _I32P, _I64P, _I64P, _I64P, _I64P, _I64P]
+    lib.fp_eval_noalloc.argtypes = [P, P, P]
     lib.fp_get_connections.argtypes = [P, _I32P, _I32P]
     lib.fp_status_string.restype = c.c_char_p
     lib.fp_status_string.argtypes = [c.c_int]

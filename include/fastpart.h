@@ -173,6 +173,20 @@ fp_status fp_eval_moves(fp_ctx* ctx, fp_metric metric, const fp_schedule* sched,
 fp_status fp_eval(fp_ctx* ctx, fp_metrics* out, int* lam, long long* sv, long long* rv, long long* sm,
                   long long* rm, long long* sv_msv);
 
+/* fp_eval without allocating: no counterpart in the reference. For callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops that must not call hipMalloc). The same six metrics and per-part
+ * vectors, written to DEVICE memory on the context's stream, with no host
+ * synchronisation: d_out[6 + 5K] = tv, cn, tm, msm, msv, msrv, then sv[K],
+ * rv[K], sm[K], rm[K], sv_msv[K] (as fp_eval's outputs); d_lam (nullable,
+ * device, noe ints) = lambda per net. The kernels' scratch is a static device
+ * array, one per GPU, held until the process ends: K <= 1024 (larger K:
+ * FP_ERR_UNSUPPORTED). Every call on a GPU shares that GPU's array, so calls
+ * that overlap on one GPU (contexts on different streams or threads) race;
+ * calls in sequence on one stream do not. The context must already hold a
+ * state (fp_set_partition / fp_set_state, which allocate). */
+fp_status fp_eval_noalloc(fp_ctx* ctx, long long* d_out, int* d_lam);
+
 /* The part-connection matrix of the resident state (TM/partition.cpp:190-235,
  * the bookkeeping of TM and MSM): pc[a*K + b] = number of nets whose source is
  * in part a and that have pins in part b != a (pc[a*K + a] = 0); msg[a] = the
