@@ -53,7 +53,7 @@ def build_engine():
     return out
 
 
-LEGACY_OBJECTIVES = ["tv", "cn", "msv", "msrv", "tm"]
+LEGACY_OBJECTIVES = ["tv", "cn", "msv", "msrv", "tm", "msm"]
 
 
 def build_legacy(objective):

@@ -16,6 +16,8 @@
 #include <string>
 #include <vector>
 
+#include <omp.h>
+
 namespace {
 
 // splitmix64: one independent stream per (seed, row)
@@ -429,57 +431,285 @@ int fp_read_patoh_header(const char* path, int* index_base, int* nov, int* noe, 
   return ok ? FP_OK : FP_ERR_IO;
 }
 
+namespace {
+
+bool slurp(const char* path, std::vector<char>& buf) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return false;
+  std::fseek(f, 0, SEEK_END);
+  const long sz = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  if (sz < 0) { std::fclose(f); return false; }
+  buf.resize((size_t)sz + 1);
+  const size_t got = std::fread(buf.data(), 1, (size_t)sz, f);
+  std::fclose(f);
+  buf.resize(got + 1);
+  buf[got] = '\0';
+  return true;
+}
+
+inline bool is_space(char c) { return c == ' ' || c == '\t' || c == '\r' || c == '\n' || c == '\v' || c == '\f'; }
+
+}  // namespace
+
+// PaToH-like text, parsed in parallel with the rules of read_patohlike
+// (shared/hypergraph_rw.cpp:97-183): header "index_base nov noe pins" (the
+// rest of its line ignored); then the first noe non-blank lines are the nets
+// (blank lines skipped), ids minus 1 when the base is not 0; noe becomes the
+// number of nets read; a pin count other than the header's fails. Lines are
+// found, then their tokens counted, then parsed into place, each step over
+// all threads.
 int fp_read_patoh(const char* path, int nov_cap, int noe_cap, long long pins_cap, int* nov, int* noe,
                   long long* pins, unsigned* net_ptr, unsigned* net_pins) {
-  FILE* f = std::fopen(path, "r");
-  if (!f) return FP_ERR_IO;
+  std::vector<char> buf;
+  if (!slurp(path, buf)) return FP_ERR_IO;
+  const char* text = buf.data();
+  const size_t len = buf.size() - 1;
   int base, hv, he;
   long long hp;
-  if (std::fscanf(f, "%d %d %d %lld", &base, &hv, &he, &hp) != 4) { std::fclose(f); return FP_ERR_IO; }
-  // rest of the header line is ignored (shared/hypergraph_rw.cpp:138)
-  int ch;
-  while ((ch = std::fgetc(f)) != EOF && ch != '\n') {}
-  const int salt = base == 0 ? 0 : -1;
-  long long v_count = 0;
-  int n_count = 0;
+  int hdr_end = 0;
+  if (std::sscanf(text, "%d %d %d %lld%n", &base, &hv, &he, &hp, &hdr_end) != 4) return FP_ERR_IO;
+  size_t pos = (size_t)hdr_end;
+  while (pos < len && text[pos] != '\n') pos++;  // rest of the header line (:138)
+  if (pos < len) pos++;
+  // line starts: per-thread newline counts over equal chunks, then positions
+  const int T = std::max(1, omp_get_max_threads());
+  const size_t body = len - pos;
+  std::vector<size_t> nl((size_t)T + 1, 0);
+#pragma omp parallel num_threads(T)
+  {
+    const int t = omp_get_thread_num();
+    const size_t a = pos + body * (size_t)t / (size_t)T, b = pos + body * (size_t)(t + 1) / (size_t)T;
+    size_t c = 0;
+    for (size_t i = a; i < b; i++) c += text[i] == '\n';
+    nl[(size_t)t + 1] = c;
+  }
+  for (int t = 0; t < T; t++) nl[(size_t)t + 1] += nl[(size_t)t];
+  const size_t nlines = nl[(size_t)T] + 1;  // the last line may lack a newline
+  std::vector<size_t> start(nlines + 1);
+  start[0] = pos;
+#pragma omp parallel num_threads(T)
+  {
+    const int t = omp_get_thread_num();
+    const size_t a = pos + body * (size_t)t / (size_t)T, b = pos + body * (size_t)(t + 1) / (size_t)T;
+    size_t k = nl[(size_t)t];
+    for (size_t i = a; i < b; i++)
+      if (text[i] == '\n') start[++k] = i + 1;
+  }
+  start[nlines] = len + 1;
+  // tokens per line
+  std::vector<int> tok(nlines, 0);
+#pragma omp parallel for num_threads(T) schedule(dynamic, 4096)
+  for (long long l = 0; l < (long long)nlines; l++) {
+    const size_t e = std::min(start[(size_t)l + 1], len);
+    int c = 0;
+    bool in = false;
+    for (size_t i = start[(size_t)l]; i < e; i++) {
+      const bool sp = is_space(text[i]);
+      if (!sp && !in) c++;
+      in = !sp;
+    }
+    tok[(size_t)l] = c;
+  }
+  // the first `he` non-blank lines are the nets
+  std::vector<size_t> net_line;
+  net_line.reserve((size_t)std::max(he, 0));
+  long long total = 0;
+  for (size_t l = 0; l < nlines && (int)net_line.size() < he; l++) {
+    if (tok[l] == 0) continue;
+    if ((int)net_line.size() >= noe_cap) return FP_ERR_ARG;
+    net_ptr[net_line.size()] = (unsigned)total;
+    total += tok[l];
+    if (total > pins_cap || total >= (1ll << 32)) return FP_ERR_ARG;
+    net_line.push_back(l);
+  }
+  const int n_read = (int)net_line.size();
+  net_ptr[n_read] = (unsigned)total;
+  if (total != hp) return FP_ERR_ARG;  // pin-count mismatch fails the read (:167-173)
+  if (hv > nov_cap) return FP_ERR_ARG;
+  const long salt = base == 0 ? 0 : -1;
+  int bad = 0;
+#pragma omp parallel for num_threads(T) schedule(dynamic, 1024) reduction(| : bad)
+  for (int n = 0; n < n_read; n++) {
+    const size_t l = net_line[(size_t)n];
+    const char* q = text + start[l];
+    const char* e = text + std::min(start[l + 1], len);
+    unsigned k = net_ptr[n];
+    while (q < e) {
+      while (q < e && is_space(*q)) q++;
+      if (q >= e) break;
+      char* endp;
+      const long x = std::strtol(q, &endp, 10);
+      if (endp == q) { bad = 1; break; }
+      q = endp;
+      const long v = x + salt;
+      if (v < 0 || v >= hv) { bad = 1; break; }
+      net_pins[k++] = (unsigned)v;
+    }
+  }
+  if (bad) return FP_ERR_ARG;
+  *nov = hv;
+  *noe = n_read;  // noe is corrected to the nets actually read (:159-165)
+  *pins = total;
+  return FP_OK;
+}
+
+// ---- the .plbin cache (shared/hypergraph_rw.cpp:185-240) ------------------
+// u32 noe, u32 nov, i64 pins, then per net: u32 end offset, u32 pins[...].
+
+int fp_write_plbin(const char* path, int nov, int noe, const unsigned* net_ptr, const unsigned* net_pins) {
+  if (!path || nov < 0 || noe < 0 || !net_ptr) return FP_ERR_ARG;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return FP_ERR_IO;
+  const unsigned un = (unsigned)noe, uv = (unsigned)nov;
+  const long long pins = net_ptr[noe];
+  bool ok = std::fwrite(&un, 4, 1, f) == 1 && std::fwrite(&uv, 4, 1, f) == 1 && std::fwrite(&pins, 8, 1, f) == 1;
+  std::vector<unsigned> rec;
+  for (int n = 0; n < noe && ok; n++) {
+    rec.assign(1, net_ptr[n + 1]);
+    rec.insert(rec.end(), net_pins + net_ptr[n], net_pins + net_ptr[n + 1]);
+    ok = std::fwrite(rec.data(), 4, rec.size(), f) == rec.size();
+  }
+  ok = std::fclose(f) == 0 && ok;
+  return ok ? FP_OK : FP_ERR_IO;
+}
+
+int fp_read_plbin_header(const char* path, int* nov, int* noe, long long* pins) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return FP_ERR_IO;
+  unsigned un = 0, uv = 0;
+  long long p = 0;
+  const bool ok = std::fread(&un, 4, 1, f) == 1 && std::fread(&uv, 4, 1, f) == 1 && std::fread(&p, 8, 1, f) == 1;
+  std::fclose(f);
+  if (!ok) return FP_ERR_IO;
+  *noe = (int)un;
+  *nov = (int)uv;
+  *pins = p;
+  return FP_OK;
+}
+
+int fp_read_plbin(const char* path, int nov_cap, int noe_cap, long long pins_cap, int* nov, int* noe, long long* pins,
+                  unsigned* net_ptr, unsigned* net_pins) {
+  std::vector<char> buf;
+  if (!slurp(path, buf) || buf.size() < 17) return FP_ERR_IO;
+  const char* b = buf.data();
+  unsigned un, uv;
+  long long p;
+  std::memcpy(&un, b, 4);
+  std::memcpy(&uv, b + 4, 4);
+  std::memcpy(&p, b + 8, 8);
+  if ((long long)un > noe_cap || (long long)uv > nov_cap || p > pins_cap || p < 0) return FP_ERR_ARG;
+  const size_t words = (buf.size() - 1 - 16) / 4;
+  const unsigned* w = reinterpret_cast<const unsigned*>(b + 16);
+  std::vector<unsigned> tmp;
+  if (reinterpret_cast<uintptr_t>(w) % alignof(unsigned)) {
+    tmp.resize(words);
+    std::memcpy(tmp.data(), b + 16, words * 4);
+    w = tmp.data();
+  }
+  size_t k = 0;
   net_ptr[0] = 0;
-  std::string line;
-  char buf[1 << 16];
-  while (n_count < he) {
-    line.clear();
-    bool got = false;
-    while (std::fgets(buf, sizeof buf, f)) {
-      got = true;
-      line += buf;
-      if (!line.empty() && line.back() == '\n') break;
+  for (unsigned n = 0; n < un; n++) {
+    if (k >= words) return FP_ERR_IO;
+    const unsigned end = w[k++];
+    if (end < net_ptr[n] || (long long)end > p || k + (end - net_ptr[n]) > words) return FP_ERR_IO;
+    for (unsigned j = net_ptr[n]; j < end; j++) {
+      const unsigned v = w[k++];
+      if (v >= uv) return FP_ERR_ARG;
+      net_pins[j] = v;
     }
-    if (!got) break;
-    const char* s = line.c_str();
-    char* e;
-    int local = 0;
-    for (;;) {
-      while (*s == ' ' || *s == '\t' || *s == '\r' || *s == '\n') s++;
-      if (!*s) break;
-      long x = std::strtol(s, &e, 10);
-      if (e == s) break;
-      s = e;
-      long v = x + salt;
-      if (v < 0 || v >= hv || v_count >= pins_cap) { std::fclose(f); return FP_ERR_ARG; }
-      net_pins[v_count++] = (unsigned)v;
-      local++;
-    }
-    if (local == 0) continue;  // blank lines are skipped (shared/hypergraph_rw.cpp:152-156)
-    if (n_count >= noe_cap) { std::fclose(f); return FP_ERR_ARG; }
-    net_ptr[n_count + 1] = net_ptr[n_count] + (unsigned)local;
-    n_count++;
+    net_ptr[n + 1] = end;
+  }
+  if ((long long)net_ptr[un] != p) return FP_ERR_IO;
+  *nov = (int)uv;
+  *noe = (int)un;
+  *pins = p;
+  return FP_OK;
+}
+
+// ---- .ws weights and sources (shared/hypergraph_rw.cpp:434-494) ------------
+
+int fp_ws_path(const char* hg_path, char* out, int cap) {
+  if (!hg_path || !out || cap < 1) return FP_ERR_ARG;
+  std::string s(hg_path);
+  const size_t dot = s.rfind('.');
+  const std::string w = (dot == std::string::npos ? s : s.substr(0, dot)) + ".ws";
+  if ((int)w.size() + 1 > cap) return FP_ERR_ARG;
+  std::memcpy(out, w.c_str(), w.size() + 1);
+  return FP_OK;
+}
+
+int fp_read_ws(const char* path, int noe, int* edge_w, int* src) {
+  FILE* f = std::fopen(path, "r");
+  if (!f) return FP_ERR_IO;
+  int n = 0, w, v;
+  char line[256];
+  while (n < noe && std::fgets(line, sizeof line, f)) {
+    if (std::sscanf(line, "%d %d", &w, &v) != 2) continue;
+    if (edge_w) edge_w[n] = w;
+    if (src) src[n] = v;
+    n++;
   }
   std::fclose(f);
-  if (v_count != hp) return FP_ERR_ARG;  // pin-count mismatch fails the read (:167-173)
-  if (hv > nov_cap) return FP_ERR_ARG;
-  *nov = hv;
-  *noe = n_count;  // noe is corrected to the nets actually read (:159-165)
-  *pins = v_count;
+  return n == noe ? FP_OK : FP_ERR_IO;
+}
+
+int fp_write_ws(const char* path, int noe, const int* edge_w, const int* src) {
+  if (!path || noe < 0 || !src) return FP_ERR_ARG;
+  FILE* f = std::fopen(path, "w");
+  if (!f) return FP_ERR_IO;
+  for (int n = 0; n < noe; n++) std::fprintf(f, "%d %d\n", edge_w ? edge_w[n] : 1, src[n]);
+  return std::fclose(f) == 0 ? FP_OK : FP_ERR_IO;
+}
+
+// ---- part vectors (CN/partition.cpp:425-433, shared/utils.cpp:592-608) -----
+
+int fp_write_part_vector(const char* path, int nov, const int* part) {
+  if (!path || nov < 0 || !part) return FP_ERR_ARG;
+  FILE* f = std::fopen(path, "w");
+  if (!f) return FP_ERR_IO;
+  std::vector<char> out;
+  out.reserve((size_t)nov * 4);
+  char tmp[16];
+  for (int v = 0; v < nov; v++) {
+    const int k = std::snprintf(tmp, sizeof tmp, "%d\n", part[v]);
+    out.insert(out.end(), tmp, tmp + k);
+  }
+  const bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
+  return (std::fclose(f) == 0 && ok) ? FP_OK : FP_ERR_IO;
+}
+
+int fp_read_part_vector(const char* path, int nov, int* part) {
+  std::vector<char> buf;
+  if (!slurp(path, buf)) return FP_ERR_IO;
+  const char* q = buf.data();
+  for (int v = 0; v < nov; v++) {
+    char* e;
+    const long x = std::strtol(q, &e, 10);
+    if (e == q) return FP_ERR_IO;
+    part[v] = (int)x;
+    q = e;
+  }
   return FP_OK;
+}
+
+// The reference's reader with its cache: <path>.plbin when it exists, else
+// the text, after which the cache is written when write_cache is set
+// (read_patohlike, shared/hypergraph_rw.cpp:97-242).
+int fp_read_hypergraph(const char* path, int write_cache, int nov_cap, int noe_cap, long long pins_cap, int* nov,
+                       int* noe, long long* pins, unsigned* net_ptr, unsigned* net_pins, int* from_cache) {
+  if (!path) return FP_ERR_ARG;
+  const std::string bin = std::string(path) + ".plbin";
+  FILE* probe = std::fopen(bin.c_str(), "rb");
+  if (probe) {
+    std::fclose(probe);
+    if (from_cache) *from_cache = 1;
+    return fp_read_plbin(bin.c_str(), nov_cap, noe_cap, pins_cap, nov, noe, pins, net_ptr, net_pins);
+  }
+  if (from_cache) *from_cache = 0;
+  int st = fp_read_patoh(path, nov_cap, noe_cap, pins_cap, nov, noe, pins, net_ptr, net_pins);
+  if (st == FP_OK && write_cache) st = fp_write_plbin(bin.c_str(), *nov, *noe, net_ptr, net_pins);
+  return st;
 }
 
 int fp_write_patoh(const char* path, int index_base, int nov, int noe, const unsigned* net_ptr,

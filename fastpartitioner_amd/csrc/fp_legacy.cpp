@@ -10,9 +10,15 @@
 //   MSRV  MSRV/partition.cpp:4-24  (the TV signature)
 //   TM    TM/partition.cpp:3-7     (partition_gpu_wrapper, partition_gpu_optimized_wrapper,
 //                                   partition_gpu_threads_wrapper)
+//   MSM   no device entry in the reference: MSM/msm.cpp:67 calls a
+//         partition_gpu(...) that MSM/partition.h does not declare and no
+//         file defines. This library defines it (the signature msm.cpp calls,
+//         that of TM/partition.h:16), covering what TM's partition_gpu does
+//         around its wrapper: the message greedy stream, the device
+//         refinement, the six metrics printed in the reference's format.
 // MSRV declares the TV signature for another objective, so each objective gets
 // a library of its own: this file is compiled once per objective with
-// -DFP_LEGACY_<TV|CN|MSV|MSRV|TM>.
+// -DFP_LEGACY_<TV|CN|MSV|MSRV|TM|MSM>.
 //
 // Contract kept from the reference:
 //   * all arrays are the caller's host arrays, borrowed for the call; refined
@@ -28,8 +34,10 @@
 // thread count (TV 8, MSV/MSRV 256 per thread; TM a whole pass), so `-g` gives
 // the partition partition_par gives at that thread count. FASTPART_DEVICE
 // selects the GPU (default 0).
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include <omp.h>
@@ -200,6 +208,70 @@ void tm_wrapper(unsigned int* row_ptr, unsigned int* col_ind, int nov, int noe, 
 FP_TM_ENTRY(partition_gpu_wrapper)
 FP_TM_ENTRY(partition_gpu_optimized_wrapper)
 FP_TM_ENTRY(partition_gpu_threads_wrapper)
+#elif defined(FP_LEGACY_MSM)
+// The MSM driver's device path (see the top of this file). row_ptr/col_ind:
+// the vertex->nets view; src_nodes: net sources; ordering: the stream's
+// visiting order. Rounds of 128 vertices per OpenMP thread, as MSM's
+// partition_par (MSM/partition.cpp:1048-1056), unless FASTPART_ROUND is set.
+// FASTPART_PARTVEC, when set, names a file that receives the refined part
+// vector (one id per line); the reference's MSM driver writes none.
+void partition_gpu(unsigned int* row_ptr, unsigned int* col_ind, unsigned int* /*net_appearence*/, int nov, int noe,
+                   long long pins, int* /*edge_wgths*/, int* src_nodes, int K, int* ordering, std::string /*f_name*/,
+                   int REFI_OPT, int REFI_ITER) {
+  std::vector<unsigned> net_ptr((size_t)noe + 1), net_pins((size_t)(pins > 0 ? pins : 1));
+  int st = fp_csr_transpose(noe, nov, row_ptr, col_ind, net_ptr.data(), net_pins.data());
+  if (st) die(nullptr, st, "transpose of the vertex->nets view");
+  const double C = ref_C(nov, K);
+  std::vector<int> part((size_t)nov);
+  auto t0 = std::chrono::steady_clock::now();
+  st = fp_initial_partition_msg(nov, noe, net_ptr.data(), row_ptr, col_ind, src_nodes, ordering, nullptr, K, C,
+                                part.data());
+  if (st) die(nullptr, st, "fp_initial_partition_msg");
+  auto t1 = std::chrono::steady_clock::now();
+  std::printf("Paritioning took : %g seconds.\n", std::chrono::duration<double>(t1 - t0).count());
+  fp_ctx* ctx = nullptr;
+  st = fp_ctx_create(&ctx, (int)env_long("FASTPART_DEVICE", 0));
+  if (st) die(ctx, st, "fp_ctx_create");
+  st = fp_upload(ctx, nov, noe, net_ptr.data(), net_pins.data(), row_ptr, col_ind, src_nodes, nullptr, ordering);
+  if (st) die(ctx, st, "fp_upload");
+  st = fp_set_partition(ctx, K, part.data());
+  if (st) die(ctx, st, "fp_set_partition");
+  fp_metrics m{};
+  st = fp_eval(ctx, &m, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+  if (st) die(ctx, st, "fp_eval");
+  // the reference's report (MSM/partition.cpp:318-330 and :664-700)
+  std::printf("TV : %lld\nCN : %lld\nTM : %lld\nMGM : %lld\nMGV : %lld\nMGAV : %lld\n", m.tv, m.cn, m.tm, m.msm,
+              m.msv, m.msrv);
+  if (REFI_OPT) {
+    fp_schedule sched{};
+    sched.kind = FP_SCHED_REF;
+    sched.round = (int)env_long("FASTPART_ROUND", 128 * threads());
+    sched.ldg_C = C;
+    fp_refine_stats stats{};
+    st = fp_refine(ctx, FP_METRIC_MSM, REFI_ITER, &sched, &stats);
+    if (st) die(ctx, st, "fp_refine");
+    std::printf("Refinement took : %g seconds.\n", stats.device_ms / 1000.0);
+    fp_metrics r{};
+    st = fp_eval(ctx, &r, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (st) die(ctx, st, "fp_eval");
+    std::printf("After Refinement TV : %lld\nAfter Refinement CN : %lld\nAfter Refinement TM : %lld\n"
+                "After Refinement MGM : %lld\nAfter Refinement MGV : %lld\nAfter Refinement MGAV : %lld\n",
+                r.tv, r.cn, r.tm, r.msm, r.msv, r.msrv);
+    st = fp_get_state(ctx, part.data(), nullptr, nullptr, nullptr, nullptr);
+    if (st) die(ctx, st, "fp_get_state");
+  } else {
+    std::printf("After Refinement TV : -1\nAfter Refinement CN : -1\nAfter Refinement TM : -1\n"
+                "After Refinement MGM : -1\nAfter Refinement MGV : -1\nAfter Refinement MGAV : -1\n");
+  }
+  std::printf("Iterated for : %d times\n", REFI_ITER);
+  std::fflush(stdout);
+  const char* pv = std::getenv("FASTPART_PARTVEC");
+  if (pv && *pv) {
+    st = fp_write_part_vector(pv, nov, part.data());
+    if (st) die(ctx, st, "fp_write_part_vector");
+  }
+  fp_ctx_destroy(ctx);
+}
 #else
-#error "compile with -DFP_LEGACY_TV, -DFP_LEGACY_CN, -DFP_LEGACY_MSV, -DFP_LEGACY_MSRV or -DFP_LEGACY_TM"
+#error "compile with -DFP_LEGACY_TV, -DFP_LEGACY_CN, -DFP_LEGACY_MSV, -DFP_LEGACY_MSRV, -DFP_LEGACY_TM or -DFP_LEGACY_MSM"
 #endif

@@ -132,6 +132,67 @@ def read_patoh(path: str, src=None) -> Hypergraph:
     return Hypergraph.from_nets(rn.value, net_ptr[: re_.value + 1], net_pins[: rp.value], src)
 
 
+def read_hypergraph(path: str, src=None, write_cache: bool = True) -> tuple[Hypergraph, bool]:
+    """The reference reader's behaviour (fp_read_hypergraph): <path>.plbin
+    when it exists, else the text (then the cache is written). Returns the
+    hypergraph and whether it came from the cache."""
+    lib = _native.host()
+    bin_path = path + ".plbin"
+    nov, noe, pins = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()
+    if os.path.exists(bin_path):
+        _native.check(lib.fp_read_plbin_header(bin_path.encode(), ctypes.byref(nov), ctypes.byref(noe),
+                                               ctypes.byref(pins)), "fp_read_plbin_header")
+    else:
+        base = ctypes.c_int()
+        _native.check(lib.fp_read_patoh_header(path.encode(), ctypes.byref(base), ctypes.byref(nov),
+                                               ctypes.byref(noe), ctypes.byref(pins)), "fp_read_patoh_header")
+    cap_e = max(noe.value, 1)
+    net_ptr = np.zeros(cap_e + 1, dtype=np.uint32)
+    net_pins = np.zeros(max(pins.value, 1), dtype=np.uint32)
+    rn, re_, rp, cached = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong(), ctypes.c_int()
+    _native.check(lib.fp_read_hypergraph(path.encode(), int(write_cache), nov.value, cap_e, pins.value,
+                                         ctypes.byref(rn), ctypes.byref(re_), ctypes.byref(rp), _ptr(net_ptr, _U32P),
+                                         _ptr(net_pins, _U32P), ctypes.byref(cached)), "fp_read_hypergraph")
+    return Hypergraph.from_nets(rn.value, net_ptr[: re_.value + 1], net_pins[: rp.value], src), bool(cached.value)
+
+
+def write_plbin(hg: Hypergraph, path: str) -> None:
+    _native.check(_native.host().fp_write_plbin(path.encode(), hg.nov, hg.noe, _ptr(hg.net_ptr, _U32P),
+                                                _ptr(hg.net_pins, _U32P)), "fp_write_plbin")
+
+
+def ws_path(hg_path: str) -> str:
+    buf = ctypes.create_string_buffer(len(hg_path) + 8)
+    _native.check(_native.host().fp_ws_path(hg_path.encode(), buf, len(buf)), "fp_ws_path")
+    return buf.value.decode()
+
+
+def read_ws(path: str, noe: int):
+    """(edge weights, sources) of a .ws file."""
+    w = np.zeros(noe, np.int32)
+    src = np.zeros(noe, np.int32)
+    _native.check(_native.host().fp_read_ws(path.encode(), noe, _ptr(w, _I32P), _ptr(src, _I32P)), "fp_read_ws")
+    return w, src
+
+
+def write_ws(path: str, src, edge_w=None) -> None:
+    src = np.ascontiguousarray(src, dtype=np.int32)
+    ew = None if edge_w is None else _ptr(np.ascontiguousarray(edge_w, dtype=np.int32), _I32P)
+    _native.check(_native.host().fp_write_ws(path.encode(), len(src), ew, _ptr(src, _I32P)), "fp_write_ws")
+
+
+def write_part_vector(path: str, part) -> None:
+    part = np.ascontiguousarray(part, dtype=np.int32)
+    _native.check(_native.host().fp_write_part_vector(path.encode(), len(part), _ptr(part, _I32P)),
+                  "fp_write_part_vector")
+
+
+def read_part_vector(path: str, nov: int) -> np.ndarray:
+    part = np.zeros(nov, np.int32)
+    _native.check(_native.host().fp_read_part_vector(path.encode(), nov, _ptr(part, _I32P)), "fp_read_part_vector")
+    return part
+
+
 def write_patoh(hg: Hypergraph, path: str, index_base: int = 0) -> None:
     lib = _native.host()
     _native.check(lib.fp_write_patoh(path.encode(), index_base, hg.nov, hg.noe, _ptr(hg.net_ptr, _U32P),

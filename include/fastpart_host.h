@@ -4,7 +4,8 @@
  * initial partition to it:
  *   - seeded synthetic column-net generators (the benchmark workloads),
  *   - the net->pins -> vertex->nets CSR transpose,
- *   - PaToH-like text reading/writing,
+ *   - PaToH-like text reading/writing, the .plbin cache, .ws weight and
+ *     source files, part-vector files,
  *   - the streaming initial partition, and the reference's balance constants.
  * All functions return an fp_status code (0 = FP_OK); none calls exit().
  */
@@ -92,12 +93,44 @@ long long fp_ref_min_pw(long long total_weight, int K);
 long long fp_balance_cap(long long total_weight, int K, double eps);
 
 /* PaToH-like text (header "index_base nov noe pins", one net per line),
- * read with the rules of shared/hypergraph_rw.cpp:97-183. */
+ * read with the rules of shared/hypergraph_rw.cpp:97-183 (the first noe
+ * non-blank lines are the nets; noe is corrected to the nets read; a pin
+ * count other than the header's fails), parsed on all threads. The header
+ * gives the capacities to allocate. */
 int fp_read_patoh_header(const char* path, int* index_base, int* nov, int* noe, long long* pins);
 int fp_read_patoh(const char* path, int nov_cap, int noe_cap, long long pins_cap, int* nov, int* noe,
                   long long* pins, unsigned* net_ptr, unsigned* net_pins);
 int fp_write_patoh(const char* path, int index_base, int nov, int noe, const unsigned* net_ptr,
                    const unsigned* net_pins);
+
+/* The reference's binary cache of a text hypergraph, <file>.plbin
+ * (shared/hypergraph_rw.cpp:185-240): u32 noe, u32 nov, i64 pins, then per
+ * net its u32 end offset followed by its u32 pins. Files written here are
+ * byte-identical to the reference's. */
+int fp_write_plbin(const char* path, int nov, int noe, const unsigned* net_ptr, const unsigned* net_pins);
+int fp_read_plbin_header(const char* path, int* nov, int* noe, long long* pins);
+int fp_read_plbin(const char* path, int nov_cap, int noe_cap, long long pins_cap, int* nov, int* noe, long long* pins,
+                  unsigned* net_ptr, unsigned* net_pins);
+/* read_patohlike's behaviour (shared/hypergraph_rw.cpp:97-242): <path>.plbin
+ * when it exists, else the text, then (write_cache != 0) the cache is
+ * written. *from_cache (nullable) tells which was read. Capacities as for
+ * fp_read_patoh (the cache's header, or the text's, gives them). */
+int fp_read_hypergraph(const char* path, int write_cache, int nov_cap, int noe_cap, long long pins_cap, int* nov,
+                       int* noe, long long* pins, unsigned* net_ptr, unsigned* net_pins, int* from_cache);
+
+/* Net weights and sources, <base>.ws with one "weight source" line per net
+ * (shared/hypergraph_rw.cpp:434-494; base = the hypergraph path up to its
+ * last '.'). fp_read_ws reads noe lines (edge_w / src nullable);
+ * fp_write_ws writes them (edge_w NULL = weight 1). */
+int fp_ws_path(const char* hg_path, char* out, int cap);
+int fp_read_ws(const char* path, int noe, int* edge_w, int* src);
+int fp_write_ws(const char* path, int noe, const int* edge_w, const int* src);
+
+/* Part vectors as the reference writes them: one part id per line, vertex
+ * order (CN/partition.cpp:425-433 "<file>.fastpart.<K>",
+ * shared/utils.cpp:592-608). */
+int fp_write_part_vector(const char* path, int nov, const int* part);
+int fp_read_part_vector(const char* path, int nov, int* part);
 
 #ifdef __cplusplus
 }

@@ -414,3 +414,29 @@ def run_reference(metric: str, hg, K: int, passes: int, threads: int, timeout: f
     res["refine_seconds"] = float(m[-1]) if m else None
     res["stdout"] = p.stdout
     return res
+
+
+def run_reference_reader(path: str, timeout: float = 300.0):
+    """The reference's read_patohlike (oracle/_ref/ref_reader) on `path`: dict
+    with ok, nov, noe, pins, net_ptr, net_pins, vertex_appearence, stdout.
+    Like the reference, it reads <path>.plbin when that exists and otherwise
+    writes it after reading the text."""
+    exe = os.path.join(REF_DIR, "ref_reader")
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "r.bin")
+        p = subprocess.run([exe, path, out], capture_output=True, text=True, timeout=timeout)
+        if p.returncode != 0:
+            raise RuntimeError(f"{exe} failed rc={p.returncode}: {p.stderr[-2000:]}")
+        with open(out, "rb") as f:
+            raw = f.read()
+    ok, nov, noe = struct.unpack_from("<iII", raw, 0)
+    pins = struct.unpack_from("<q", raw, 12)[0]
+    res = {"ok": bool(ok), "nov": nov, "noe": noe, "pins": pins, "stdout": p.stdout}
+    if ok:
+        off = 20
+        res["net_ptr"] = np.frombuffer(raw, dtype="<u4", count=noe + 1, offset=off).copy()
+        off += 4 * (noe + 1)
+        res["net_pins"] = np.frombuffer(raw, dtype="<u4", count=pins, offset=off).copy()
+        off += 4 * pins
+        res["vertex_appearence"] = np.frombuffer(raw, dtype="<u4", count=nov, offset=off).copy()
+    return res

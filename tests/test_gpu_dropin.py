@@ -8,6 +8,9 @@ host, hands its arrays to the wrapper, and prints metrics from what comes
 back. The state it ends with must equal the oracle's refinement of the state
 it handed over.
 """
+import os
+import subprocess
+
 import numpy as np
 import pytest
 
@@ -96,3 +99,96 @@ def _recount(hg, st):
         slot = {int(st.ep_parts[s + j]): s + j for j in range(L)}
         for v in hg.net_pins[hg.net_ptr[n]:hg.net_ptr[n + 1]]:
             st.ep_cnt[slot[int(st.part[v])]] += 1
+
+
+def _cli(metric, path, K, passes, env):
+    exe = os.path.join(O.REF_DIR, f"cli_{metric}")
+    if not os.path.exists(exe):
+        pytest.skip(f"oracle/_ref/cli_{metric} not built (needs the reference sources at build time)")
+    cmd = [exe, "-f", path, "-k", str(K), "-r", "1", "-i", str(passes), "-g", "1"]
+    print(" ".join(cmd[1:]), env, flush=True)
+    full = dict(os.environ)
+    full.update(env)
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=120, env=full)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    return p.stdout
+
+
+def _symmetric_colnet(n, per_row, seed):
+    """Column-net hypergraph of a structurally symmetric n x n matrix (the
+    diagonal plus random pairs (i, j), (j, i)): vertex degrees equal net
+    degrees. TV/tv.cpp hands MGV_Scratch the vertex degrees where net degrees
+    belong (TV/tv.cpp:51 via read_patohlike's vertex counts), which corrupts
+    that function's inverse CSR, and the heap, on other inputs."""
+    rng = np.random.default_rng(seed)
+    rows = [{i} for i in range(n)]
+    for _ in range(n * per_row // 2):
+        i, j = (int(x) for x in rng.integers(0, n, 2))
+        rows[i].add(j)
+        rows[j].add(i)
+    ptr = np.cumsum([0] + [len(r) for r in rows]).astype(np.uint32)
+    pins = np.concatenate([np.array(sorted(r), dtype=np.uint32) for r in rows])
+    return H.Hypergraph.from_nets(n, ptr, pins, np.arange(n, dtype=np.int32))
+
+
+@pytest.mark.parametrize("R", [8, 512])
+def test_reference_tv_driver_with_gpu_flag(tmp_path, R):
+    # TV/tv.cpp unchanged: its CLI11 flags, read_patohlike (1-based file),
+    # its .plbin cache, the host LDG stream of partition_gpu
+    # (TV/partition.cpp:1035-1095), then -g 1 -> the engine
+    hg = _symmetric_colnet(4000, 6, seed=12)
+    path = str(tmp_path / "g.hg")
+    H.write_patoh(hg, path, index_base=1)
+    K, passes = 16, 4
+    pv = str(tmp_path / "g.part")
+    out = _cli("tv", path, K, passes, {"FASTPART_ROUND": str(R), "FASTPART_PARTVEC": pv})
+    got = H.read_part_vector(pv, hg.nov)
+    C, max_pw, min_pw = H.ref_constants(hg.nov, K)
+    st, _ = O.tv_ldg(hg, K, C, max_pw, min_pw, par_skip=False)
+    O.tv_refine_par(hg, st, C, max_pw, min_pw, passes, R)
+    assert np.array_equal(got, st.part)
+    assert f"After Refinement TV : {O.metrics(hg, K, st.part)['TV']}\n" in out
+    assert os.path.exists(path + ".plbin")   # the reference reader cached the text
+
+
+def test_reference_cn_driver_with_gpu_flag(tmp_path):
+    # CN/cn.cpp unchanged; it writes <file>.fastpart.<K> itself (CN/partition.cpp:1814)
+    hg = H.colnet(4000, H.DEG_UNIFORM, 2, 9, seed=13)
+    path = str(tmp_path / "g.hg")
+    H.write_patoh(hg, path, index_base=0)
+    K, passes = 12, 4
+    _cli("cn", path, K, passes, {})
+    got = H.read_part_vector(path + f".fastpart.{K}", hg.nov)
+    C, _, _ = H.ref_constants(hg.nov, K)
+    st = O.cn_ldg(hg, K, C)
+    O.cn_refine(hg, st, passes, serial=False)
+    assert np.array_equal(got, st.part)
+
+
+@pytest.mark.parametrize("R", [128, 1024])
+def test_msm_partition_gpu_entry(tmp_path, R):
+    # the partition_gpu that MSM/msm.cpp:67 calls and the reference never
+    # defines, from libfastpart_legacy_msm.so: the message greedy stream
+    # (the oracle's, pinned by the reference's MSM runs), then the device
+    # refinement in rounds of R
+    exe = os.path.join(O.REF_DIR, "dropin_msm")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/dropin_msm not built (needs the reference sources at build time)")
+    hg = H.colnet(3000, H.DEG_UNIFORM, 1, 3, seed=2)
+    K, passes = 256, 6
+    inst = str(tmp_path / "g.fphg")
+    hg.write_fphg(inst)
+    pv = str(tmp_path / "g.part")
+    env = dict(os.environ, FASTPART_ROUND=str(R), FASTPART_PARTVEC=pv)
+    print(f"dropin_msm: nov={hg.nov} K={K} passes={passes} R={R}", flush=True)
+    p = subprocess.run([exe, inst, str(K), str(passes), "-1", str(tmp_path / "o")], capture_output=True, text=True,
+                       timeout=120, env=env)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    got = H.read_part_vector(pv, hg.nov)
+    C, _, _ = H.ref_constants(hg.nov, K)
+    st = O.msm_init(hg, K, C)
+    moves = O.msm_refine_par(hg, st, passes, R)
+    assert moves > 0
+    assert np.array_equal(got, st.part)
+    m = O.metrics(hg, K, st.part)
+    assert f"After Refinement MGM : {m['MSM']}\n" in p.stdout

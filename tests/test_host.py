@@ -171,9 +171,15 @@ LEGACY_SYMBOLS = {
     "tm": ["_Z21partition_gpu_wrapperPjS_S_iixPiS0_iS0_iiS0_S0_S0_S0_S0_PS0_S0_S0_",
            "_Z29partition_gpu_threads_wrapperPjS_S_iixPiS0_iS0_iiS0_S0_S0_S0_S0_PS0_S0_S0_",
            "_Z31partition_gpu_optimized_wrapperPjS_S_iixPiS0_iS0_iiS0_S0_S0_S0_S0_PS0_S0_S0_"],
+    # MSM: the partition_gpu MSM/msm.cpp:67 calls (the signature of TM/partition.h:16)
+    "msm": ["_Z13partition_gpuPjS_S_iixPiS0_iS0_NSt7__cxx1112basic_stringIcSt11char_traitsIcESaIcEEEii"],
 }
-# the reference object each library serves (MSRV's does not compile, MSRV/partition.cpp:1341)
+# the reference object that calls each library's wrapper (MSRV's caller,
+# partition_gpu, does not compile, MSRV/partition.cpp:1341; MSM's caller is
+# msm.cpp, which does not compile, MSM/msm.cpp:67)
 LEGACY_OBJECT = {"tv": "tv", "cn": "cn", "msv": "mgv", "tm": "tm"}
+# the harness binaries that run each library's device entry point
+DROPIN_OBJECT = dict(LEGACY_OBJECT, msm="msm")
 
 
 @pytest.mark.parametrize("objective", sorted(LEGACY_SYMBOLS))
@@ -190,9 +196,9 @@ def test_legacy_library_exports_reference_entry_point(objective):
             sorted(LEGACY_SYMBOLS[objective])
 
 
-@pytest.mark.parametrize("objective", sorted(LEGACY_OBJECT))
+@pytest.mark.parametrize("objective", sorted(DROPIN_OBJECT))
 def test_reference_partition_gpu_fails_loudly_without_gpu(objective, tmp_path):
-    exe = os.path.join(ROOT, "oracle", "_ref", f"dropin_{LEGACY_OBJECT[objective]}")
+    exe = os.path.join(ROOT, "oracle", "_ref", f"dropin_{DROPIN_OBJECT[objective]}")
     if not os.path.exists(exe):
         pytest.skip("drop-in binaries not built here")
     n = ctypes.c_int(-1)
