@@ -39,16 +39,93 @@ def lambda_visits(hg, lam) -> int:
     return int(np.dot(sizes, lam.astype(np.int64)))
 
 
-def measured_traffic(args):
-    """HBM bytes per k_tv_eval launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this exact workload and round size, or None."""
-    path = os.path.join(ROOT, "profiles", "tv_eval_traffic.json")
-    key = f"colnet-zipf2.1-1M-K256-TV/{args.sched}/{args.round}"
-    if args.scale != 1.0 or not os.path.exists(path):
-        return None
-    with open(path) as f:
-        rec = json.load(f).get(key)
-    return None if rec is None else rec["traffic_bytes"]
+def cpu_info():
+    """CPU model, the machine's physical cores and the threads this process
+    may run on."""
+    model, cores = "unknown", set()
+    try:
+        phys = core = None
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                k, _, v = ln.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model == "unknown":
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+    except OSError:
+        pass
+    return {"model": model, "physical_cores": len(cores) or None, "affinity": len(os.sched_getaffinity(0))}
+
+
+# The box's CPU share for one GPU: the reference's OpenMP baseline runs on at
+# most this many threads (all of them, bound to cores).
+CPU_SHARE = 16
+
+
+def _pmc_values(csv_dir, counter, kernel):
+    """Per-dispatch values of a PMC counter for the dispatches of `kernel`
+    (rocprofv3 counter_collection.csv)."""
+    import csv
+    import glob
+
+    vals = {}
+    for path in glob.glob(os.path.join(csv_dir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") == counter and kernel in row.get("Kernel_Name", ""):
+                    key = row.get("Dispatch_Id")
+                    vals[key] = vals.get(key, 0.0) + float(row["Counter_Value"])
+    return list(vals.values())
+
+
+# FETCH_SIZE / WRITE_SIZE read, for a plain device-to-device copy of known
+# size, half / all of the bytes the copy moves (three 1 GiB copies: 1,572,909
+# KiB fetched, 3,145,728 KiB written; calibration run and command in
+# profiles/tv_eval_c2_1M_K256_R262144.md): the factors that turn the counters
+# into HBM bytes.
+COUNTER_FACTOR = {"FETCH_SIZE": 2.0, "WRITE_SIZE": 1.0}
+
+
+def measure_traffic(args):
+    """HBM bytes per k_tv_eval launch, measured in this run: rocprofv3 --pmc
+    FETCH_SIZE, then WRITE_SIZE (KiB per dispatch) over a 2-pass probe of the
+    same workload (--traffic-probe), each scaled by COUNTER_FACTOR. Returns
+    (bytes, detail)."""
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    rocprof = shutil.which("rocprofv3")
+    if not rocprof:
+        return None, {"error": "rocprofv3 not found"}
+    env = dict(os.environ, TMPDIR="/tmp")
+    out = {}
+    try:
+        with tempfile.TemporaryDirectory(dir="/tmp") as d:
+            for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+                odir = os.path.join(d, counter)
+                cmd = [rocprof, "--pmc", counter, "-f", "csv", "-d", odir, "-o", "run", "--", sys.executable, "-u",
+                       os.path.abspath(__file__), "--traffic-probe", "--round", str(args.round), "--sched",
+                       args.sched]
+                p = subprocess.run(cmd, capture_output=True, text=True, timeout=240, cwd="/tmp", env=env)
+                if not glob.glob(os.path.join(odir, "**", "*counter_collection.csv"), recursive=True):
+                    return None, {"error": f"rocprofv3 --pmc {counter} rc={p.returncode}: {p.stderr[-300:]}"}
+                kv = _pmc_values(odir, counter, "k_tv_eval")
+                if not kv:
+                    return None, {"error": f"no {counter} values for k_tv_eval"}
+                out[counter] = {"kib_per_dispatch": round(sum(kv) / len(kv), 1), "dispatches": len(kv),
+                                "factor": COUNTER_FACTOR[counter]}
+    except Exception as e:  # reported, never fatal for the GPU number
+        return None, {"error": str(e)[:300]}
+    traffic = sum(v["kib_per_dispatch"] * 1024.0 * v["factor"] for v in out.values())
+    return int(traffic), out
 
 
 def cpu_baseline(hg, K, passes, threads):
@@ -345,6 +422,300 @@ def run_serve(args):
     print(json.dumps(out), flush=True)
 
 
+def tv1m_instance(args):
+    from fastpartitioner_amd import hypergraph as H
+    from fastpartitioner_amd.engine import SCHED_BAL, SCHED_REF, schedule
+
+    K, eps = 256, 0.03
+    hg = H.workload("c2", seed=1, scale=args.scale)
+    W = hg.total_weight()
+    C, max_pw, min_pw = H.ref_constants(W, K, eps)
+    cap = H.balance_cap(W, K, eps)
+    part0 = H.initial_partition(hg, K, C, max_pw, min_pw)
+    if args.sched == "bal":
+        sched = schedule(SCHED_BAL, args.round, C, cap, 0, cap)
+    else:
+        sched = schedule(SCHED_REF, args.round, C, max_pw, min_pw)
+    return hg, K, eps, cap, part0, sched
+
+
+def cn_pass_bytes(hg, part, lam):
+    """Algorithmic bytes of one CN pass's evaluation on a state (the per-unit
+    model of the lambda = 2 push, CN/partition.cpp:629-690): 6 B per net (its
+    offset and lambda), 8 B per pin of a lambda = 2 net (pin id and part), and
+    6 B per incident net of each pin on the smaller side (net id and lambda)
+    -- the uncut-net check."""
+    sizes = np.diff(hg.net_ptr.astype(np.int64))
+    two = np.nonzero(lam == 2)[0]
+    if len(two) == 0:
+        return 6 * hg.noe
+    starts, lens = hg.net_ptr[two].astype(np.int64), sizes[two]
+    idx = np.repeat(starts, lens) + (np.arange(int(lens.sum())) - np.repeat(np.cumsum(lens) - lens, lens))
+    pins = hg.net_pins[idx]
+    q = part[pins]
+    first = np.repeat(q[np.cumsum(lens) - lens], lens)
+    is_a = q == first
+    ca = np.add.reduceat(is_a.astype(np.int64), np.cumsum(lens) - lens)
+    small_is_a = np.repeat(ca < lens - ca, lens)
+    in_small = is_a == small_is_a
+    deg = np.diff(hg.vtx_ptr.astype(np.int64))
+    return int(6 * hg.noe + 8 * lens.sum() + 6 * deg[pins[in_small]].sum())
+
+
+def run_tv1m(args):
+    from fastpartitioner_amd.engine import Engine
+
+    t = time.time()
+    hg, K, eps, cap, part0, sched = tv1m_instance(args)
+    print(f"# workload: nov={hg.nov} pins={hg.pins} K={K} prep {time.time() - t:.1f}s", file=sys.stderr, flush=True)
+
+    if args.traffic_probe:
+        # the 2-pass probe that rocprofv3 --pmc runs (measure_traffic)
+        with Engine(0) as eng:
+            eng.upload(hg)
+            eng.set_partition(K, part0)
+            eng.refine("TV", 2, sched)
+        print(json.dumps({"traffic_probe": True}))
+        return
+
+    with Engine(0) as eng:
+        eng.upload(hg)
+        eng.set_partition(K, part0)
+        if args.warmup:
+            eng.refine("TV", args.warmup, sched)
+            eng.set_partition(K, part0)
+        m0 = eng.metrics(vectors=True)
+        lam0 = lambda_visits(hg, m0["lam"])
+        eng.set_kernel_timing(True)
+        t0 = time.perf_counter()
+        st = eng.refine("TV", args.steps, sched)
+        t1 = time.perf_counter()
+        launches, eval_ms = eng.kernel_timing()
+        eng.set_kernel_timing(False)
+        m1 = eng.metrics(vectors=True)
+        lam1 = lambda_visits(hg, m1["lam"])
+        sizes = eng.get_state().sizes
+        # the same instance's CN refinement (the lambda = 2 push) from the same
+        # initial partition, as many passes, timed the same way
+        eng.set_partition(K, part0)
+        mc0 = eng.metrics(vectors=True)
+        cn0 = mc0["CN"]
+        if args.warmup:
+            eng.refine("CN", args.warmup)
+            eng.set_partition(K, part0)
+        tc0 = time.perf_counter()
+        cst = eng.refine("CN", args.steps)
+        tc1 = time.perf_counter()
+        cn1 = eng.metrics()["CN"]
+        # the evaluation kernel alone, timed over a few evaluations of the final state
+        te0 = time.perf_counter()
+        for _ in range(10):
+            eng.metrics()
+        te1 = time.perf_counter()
+    secs = t1 - t0
+    value = args.steps * hg.pins / secs
+
+    # algorithmic bytes per pass of the evaluation step, per-unit model of the
+    # refinement's data in the reference layout (int32 part + int32 count per
+    # list entry), each byte counted once per use, no cache credit:
+    # 12 B per vertex, 12 B per (vertex, net) incidence, 8 B per list entry
+    # visited (Lambda = sum over nets of |n| * lambda(n))
+    lam_avg = 0.5 * (lam0 + lam1)
+    bytes_pass = 12 * hg.nov + 12 * hg.pins + 8 * lam_avg
+    avg_launch_ms = eval_ms / max(launches, 1)
+    bytes_launch = bytes_pass * args.steps / max(launches, 1)
+    achieved = bytes_launch / (avg_launch_ms * 1e-3) / 1e9
+    traffic, traffic_detail = (None, {"skipped": "--no-traffic"}) if args.no_traffic else measure_traffic(args)
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": "k_tv_eval", "launches": launches, "avg_launch_us": round(avg_launch_ms * 1e3, 2),
+            "bytes_per_launch": int(bytes_launch), "traffic_counters": traffic_detail,
+            "pass_frac": round(bytes_pass / (secs / args.steps) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    cpu = None
+    info = cpu_info()
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(CPU_SHARE, info["affinity"], info["physical_cores"] or CPU_SHARE)
+        try:
+            cpu = cpu_baseline(hg, K, args.cpu_passes, threads)
+            if cpu and "value" in cpu:
+                cpu["cpu_model"] = info["model"]
+                cpu["machine_physical_cores"] = info["physical_cores"]
+                cpu["passes_measured"] = args.cpu_passes
+                cpu["extrapolated_20_pass_s"] = round(20 * hg.pins / cpu["value"], 2)
+                one = cpu_baseline(hg, K, 1, 1)
+                if one and "value" in one:
+                    cpu["one_thread"] = {"value": one["value"], "sample": one["sample"]}
+        except Exception as e:  # reported, never fatal for the GPU number
+            cpu = {"error": str(e)[:300]}
+
+    cn_bytes = cn_pass_bytes(hg, part0, mc0["lam"])
+    cn_ms = (tc1 - tc0) * 1e3 / args.steps
+    out = {
+        "metric": "refine pin-visits/s (1M-vtx, K=256, TV)",
+        "value": round(value, 1),
+        "unit": "pin-visits/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(secs * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "colnet-zipf2.1-1M-K256-TV", "nov": hg.nov, "pins": hg.pins, "K": K,
+                   "eps": eps, "schedule": args.sched, "round": args.round, "passes": args.steps,
+                   "lambda_visits_per_pass": int(lam_avg),
+                   "tv_initial": int(m0["TV"]), "tv_refined": int(m1["TV"]),
+                   "max_part": int(sizes.max()), "cap": int(cap),
+                   "moves": int(st.moves), "rounds": int(st.rounds),
+                   "device_ms": round(st.device_ms, 3)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "cn_same_instance": {"passes": args.steps, "ms_per_step": round(cn_ms, 4),
+                             "pin_visits_per_s": round(args.steps * hg.pins / (tc1 - tc0), 1),
+                             "cn_initial": int(cn0), "cn_refined": int(cn1), "moves": int(cst.moves),
+                             "roofline": {"bound": "hbm", "scope": "whole pass", "bytes_per_pass": cn_bytes,
+                                          "achieved": round(cn_bytes / (cn_ms * 1e-3) / 1e9, 1),
+                                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                          "frac": round(cn_bytes / (cn_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
+        "metrics_evaluation": {"ms": round((te1 - te0) * 1e3 / 10, 4),
+                               "bytes": int(8 * hg.pins + 12 * hg.noe)},
+    }
+    if cpu and "value" in cpu:
+        out["gpu_over_cpu"] = round(value / cpu["value"], 2)
+    print(json.dumps(out), flush=True)
+
+
+C3_OBJECTIVES = ("TM", "MSM", "MSV", "MSRV")
+
+
+def run_c3(args):
+    """The source-based objectives on the 2M-vertex workload: column nets with
+    the diagonal plus 12 uniform off-diagonals (26M pins), vertex weights
+    U{1..8}, K = 1024, eps = 0.05, 15 passes of each of TM, MSM, MSV, MSRV from
+    its own initial stream, reference round schedules (TM one round per pass,
+    the others rounds of 16384). One line per run with a record per objective;
+    `value` = pin-visits/s over the four objectives' passes together."""
+    from fastpartitioner_amd import hypergraph as H
+    from fastpartitioner_amd.engine import SCHED_REF, Engine, schedule
+
+    K, eps, passes = 1024, 0.05, args.steps
+    t = time.time()
+    hg = H.workload("c3", seed=1, scale=args.scale)
+    W = hg.total_weight()
+    C, max_pw, min_pw = H.ref_constants(W, K, eps)
+    parts = {}
+    for mode in sorted(set(H.INIT_OF[m] for m in C3_OBJECTIVES)):
+        parts[mode] = H.initial_partition_mode(hg, mode, K, C, max_pw, min_pw)
+    print(f"# c3: nov={hg.nov} pins={hg.pins} K={K} prep {time.time() - t:.1f}s", file=sys.stderr, flush=True)
+    per = {}
+    total_s = 0.0
+    with Engine(0) as eng:
+        eng.upload(hg)
+        for metric in C3_OBJECTIVES:
+            part = parts[H.INIT_OF[metric]]
+            sched = schedule(SCHED_REF, hg.nov if metric == "TM" else 16384, C, max_pw, min_pw)
+            if args.warmup:
+                eng.set_partition(K, part)
+                eng.refine(metric, args.warmup, sched)
+            eng.set_partition(K, part)
+            m0 = eng.metrics(vectors=True)
+            eng.set_kernel_timing(True)
+            t0 = time.perf_counter()
+            st = eng.refine(metric, passes, sched)
+            t1 = time.perf_counter()
+            launches, eval_ms = eng.kernel_timing()
+            eng.set_kernel_timing(False)
+            m1 = eng.metrics()
+            total_s += t1 - t0
+            lam = lambda_visits(hg, m0["lam"])
+            # evaluation bytes per launch (rounds of R vertices): the volume
+            # evaluation reads the lists twice (12 B/vertex + 12 B/incidence +
+            # 16 B/entry); the connection evaluation 12 B/vertex + 8 B/incidence
+            # + 2 B/entry (its bitmap words are per gaining vertex, not counted)
+            if metric in ("MSV", "MSRV"):
+                eval_bytes = (12 * hg.nov + 12 * hg.pins + 16 * lam) * passes
+            else:
+                eval_bytes = (12 * hg.nov + 8 * hg.pins + 2 * lam) * passes
+            per[metric] = {
+                "ms_per_pass": round((t1 - t0) * 1e3 / passes, 3),
+                "pin_visits_per_s": round(passes * hg.pins / (t1 - t0), 1),
+                "moves": int(st.moves), "rounds": int(st.rounds),
+                "initial": int(m0[metric]), "refined": int(m1[metric]),
+                "eval_kernel": {"launches": int(launches), "avg_launch_us": round(eval_ms * 1e3 / max(launches, 1), 2),
+                                "achieved_gbs": round(eval_bytes / max(eval_ms * 1e-3, 1e-9) / 1e9, 1),
+                                "frac": round(eval_bytes / max(eval_ms * 1e-3, 1e-9) / 1e9 / HBM_PEAK_GBS, 4)},
+            }
+    cpu = None
+    if not args.no_cpu:
+        cpu = c3_cpu_baseline(args, K)
+    value = len(C3_OBJECTIVES) * passes * hg.pins / total_s
+    out = {
+        "metric": "refine pin-visits/s (2M-vtx, K=1024, TM+MSM+MSV+MSRV)",
+        "value": round(value, 1),
+        "unit": "pin-visits/s",
+        "n_gpus": 1,
+        "steps": passes,
+        "warmup": args.warmup,
+        "ms_per_step": round(total_s * 1e3 / (passes * len(C3_OBJECTIVES)), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "colnet-uniform12-2M-K1024-w1to8", "nov": hg.nov, "pins": hg.pins, "K": K,
+                   "eps": eps, "passes": passes, "objectives": per},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def c3_cpu_baseline(args, K):
+    """The reference's own partition_par for TM, MSM, MGV (MSV) and MSRV, built
+    from its sources (oracle/_ref), on a bounded sample: the same generator at
+    200,000 vertices (the reference has no vertex weights: unit weights), one
+    refinement pass, the box's CPU share of threads. Its initial streams at
+    K = 1024 (the TM/MSM greedy costs O(K) per vertex and net) bound the
+    sample size."""
+    from fastpartitioner_amd import hypergraph as H
+    from oracle import oracle as O
+
+    info = cpu_info()
+    threads = min(CPU_SHARE, info["affinity"], info["physical_cores"] or CPU_SHARE)
+    hg = H.colnet(200_000, H.DEG_FIXED, 12, seed=1)
+    res = {"sample": f"colnet diag+12, 200,000 vertices ({hg.pins} pins), unit weights, K={K}, 1 pass, "
+                     f"{threads} OpenMP threads", "cores": threads, "cpu_model": info["model"], "kind": "reference",
+           "unit": "pin-visits/s"}
+    for metric, binary in (("TM", "tm"), ("MSM", "msm"), ("MSV", "mgv"), ("MSRV", "msrv")):
+        if not O.ref_available(binary):
+            res[metric] = {"error": f"oracle/_ref/ref_{binary} not built"}
+            continue
+        # partition_par races like TV's (its per-thread counts are read
+        # before every thread wrote them) and can crash: a crashed run is
+        # retried, then the thread count is halved
+        crashes, t = [], threads
+        while t >= 1 and metric not in res:
+            for _ in range(2):
+                env = {"OMP_PROC_BIND": "close", "OMP_PLACES": "cores", "OMP_NUM_THREADS": str(t)}
+                try:
+                    r = O.run_reference(binary, hg, K, 1, t, timeout=600, env_extra=env)
+                except Exception as e:
+                    crashes.append(f"{t} threads: " + ("segfault" if "rc=-11" in str(e) else str(e)[-60:]))
+                    continue
+                secs = r["refine_seconds"]
+                res[metric] = {"refine_s": secs, "threads": t,
+                               "pin_visits_per_s": round(hg.pins / secs, 1) if secs else None}
+                if crashes:
+                    res[metric]["crashed_runs_before"] = crashes
+                break
+            t //= 2
+        if metric not in res:
+            res[metric] = {"error": f"crashed at every thread count: {crashes}"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -356,9 +727,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU baseline")
     ap.add_argument("--cpu-passes", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--workload", choices=("tv1m", "batch", "serve"), default="tv1m",
-                    help="tv1m: the 1M-vertex TV refinement (default); batch: batched vs one-at-a-time "
-                         "refinement; serve: the serving loop under Poisson arrivals")
+    ap.add_argument("--workload", choices=("tv1m", "c3", "batch", "serve"), default="tv1m",
+                    help="tv1m: the 1M-vertex TV refinement (default); c3: TM/MSM/MSV/MSRV on the 2M-vertex "
+                         "K=1024 workload; batch: batched vs one-at-a-time refinement; serve: the serving loop "
+                         "under Poisson arrivals")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM traffic measurement")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--instances", type=int, default=512, help="batch: number of hypergraphs")
     ap.add_argument("--passes", type=int, default=10, help="batch: passes per hypergraph")
     ap.add_argument("--tune-passes", type=int, default=3, help="batch: passes that double as autotuner trials")
@@ -398,107 +772,12 @@ def main():
     if args.workload == "serve":
         run_serve(args)
         return
-
-    from fastpartitioner_amd import hypergraph as H
-    from fastpartitioner_amd.engine import SCHED_BAL, SCHED_REF, Engine, schedule
-
-    K, eps = 256, 0.03
-    t = time.time()
-    hg = H.workload("c2", seed=1, scale=args.scale)
-    W = hg.total_weight()
-    C, max_pw, min_pw = H.ref_constants(W, K, eps)
-    cap = H.balance_cap(W, K, eps)
-    part0 = H.initial_partition(hg, K, C, max_pw, min_pw)
-    print(f"# workload: nov={hg.nov} pins={hg.pins} K={K} prep {time.time() - t:.1f}s", file=sys.stderr, flush=True)
-
-    if args.sched == "bal":
-        sched = schedule(SCHED_BAL, args.round, C, cap, 0, cap)
-    else:
-        sched = schedule(SCHED_REF, args.round, C, max_pw, min_pw)
-
-    with Engine(0) as eng:
-        eng.upload(hg)
-        eng.set_partition(K, part0)
-        if args.warmup:
-            eng.refine("TV", args.warmup, sched)
-            eng.set_partition(K, part0)
-        m0 = eng.metrics(vectors=True)
-        lam0 = lambda_visits(hg, m0["lam"])
-        eng.set_kernel_timing(True)
-        t0 = time.perf_counter()
-        st = eng.refine("TV", args.steps, sched)
-        t1 = time.perf_counter()
-        launches, eval_ms = eng.kernel_timing()
-        eng.set_kernel_timing(False)
-        m1 = eng.metrics(vectors=True)
-        lam1 = lambda_visits(hg, m1["lam"])
-        sizes = eng.get_state().sizes
-        # the same instance's CN refinement (the lambda = 2 push) from the same
-        # initial partition, as many passes, timed the same way
-        eng.set_partition(K, part0)
-        cn0 = eng.metrics()["CN"]
-        if args.warmup:
-            eng.refine("CN", args.warmup)
-            eng.set_partition(K, part0)
-        tc0 = time.perf_counter()
-        cst = eng.refine("CN", args.steps)
-        tc1 = time.perf_counter()
-        cn1 = eng.metrics()["CN"]
-    secs = t1 - t0
-    value = args.steps * hg.pins / secs
-
-    # algorithmic bytes per pass of the evaluation step, per-unit model of the
-    # refinement's data in the reference layout (int32 part + int32 count per
-    # list entry), each byte counted once per use, no cache credit:
-    # 12 B per vertex, 12 B per (vertex, net) incidence, 8 B per list entry
-    # visited (Lambda = sum over nets of |n| * lambda(n))
-    lam_avg = 0.5 * (lam0 + lam1)
-    bytes_pass = 12 * hg.nov + 12 * hg.pins + 8 * lam_avg
-    avg_launch_ms = eval_ms / max(launches, 1)
-    bytes_launch = bytes_pass * args.steps / max(launches, 1)
-    achieved = bytes_launch / (avg_launch_ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(args),
-            "kernel": "k_tv_eval", "launches": launches, "avg_launch_us": round(avg_launch_ms * 1e3, 2),
-            "bytes_per_launch": int(bytes_launch)}
-
-    cpu = None
-    if not args.no_cpu:
-        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
-        try:
-            cpu = cpu_baseline(hg, K, args.cpu_passes, threads)
-        except Exception as e:  # reported, never fatal for the GPU number
-            cpu = {"error": str(e)[:300]}
-
-    out = {
-        "metric": "refine pin-visits/s (1M-vtx, K=256, TV)",
-        "value": round(value, 1),
-        "unit": "pin-visits/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(secs * 1e3 / args.steps, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int32",
-        "data": "synthetic",
-        "config": {"workload": "colnet-zipf2.1-1M-K256-TV", "nov": hg.nov, "pins": hg.pins, "K": K,
-                   "eps": eps, "schedule": args.sched, "round": args.round, "passes": args.steps,
-                   "lambda_visits_per_pass": int(lam_avg),
-                   "tv_initial": int(m0["TV"]), "tv_refined": int(m1["TV"]),
-                   "max_part": int(sizes.max()), "cap": int(cap),
-                   "moves": int(st.moves), "rounds": int(st.rounds),
-                   "device_ms": round(st.device_ms, 3)},
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "cn_same_instance": {"passes": args.steps, "ms_per_step": round((tc1 - tc0) * 1e3 / args.steps, 4),
-                             "pin_visits_per_s": round(args.steps * hg.pins / (tc1 - tc0), 1),
-                             "cn_initial": int(cn0), "cn_refined": int(cn1), "moves": int(cst.moves)},
-    }
-    if cpu and "value" in cpu:
-        out["gpu_over_cpu"] = round(value / cpu["value"], 2)
-    print(json.dumps(out), flush=True)
+    if args.workload == "c3":
+        if args.steps == 20 and "--steps" not in sys.argv:
+            args.steps = 15
+        run_c3(args)
+        return
+    run_tv1m(args)
 
 
 if __name__ == "__main__":

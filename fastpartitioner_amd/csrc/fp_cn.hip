@@ -16,15 +16,18 @@
 //                 vertex can be pushed by several nets) and the ops it puts
 //                 on each of the vertex's nets
 //   k_cn_alloc / k_cn_scatter   per touched net, the op list (push indices)
+//   k_cn_bigsort  nets with more than CN_MAXOPS ops: their push indices in
+//                 order through a bitmap over the pass's pushes (one block)
 //   k_cn_apply    one wavefront per touched net: sorts its ops by push index
-//                 in LDS and replays remove/add on the list
+//                 in LDS (or reads the big nets' ordered ops) and replays
+//                 remove/add on the list
 //   k_cn_commit   part vector and part sizes (CN/partition.cpp:786-789)
 #include "fp_internal.h"
 
 namespace fpe {
 namespace {
 
-constexpr int CN_MAXOPS = 4096;  // per net and pass, sorted in LDS
+constexpr int CN_MAXOPS = 4096;  // ops per net and pass sorted in LDS; more go through k_cn_bigsort
 constexpr int SCAN_BLOCK = 1024;
 
 struct CnEvalArgs {
@@ -224,13 +227,16 @@ __global__ void __launch_bounds__(256) k_cn_prep(CnPrepArgs a) {
   if (threadIdx.x == 0 && s_ops) atomicAdd(&a.totals[0], s_ops);
 }
 
-// The pass's edit lists must fit: total ops <= cap and at most CN_MAXOPS per
-// net (sorted in LDS). Otherwise flags[0] = the total needed, flags[1] = 1,
-// and the rest of the pass leaves the state unchanged.
-__global__ void k_cn_check(const unsigned long long* __restrict__ totals, long long cap, int* __restrict__ flags) {
+// The pass's edit lists must fit: total ops <= cap. A net's ops are distinct
+// push indices unless a pin repeats in a net (dup_pins): then a net with more
+// than CN_MAXOPS ops cannot be ordered by k_cn_bigsort's bitmap. Otherwise
+// flags[0] = the total needed, flags[1] = 1, and the rest of the pass leaves
+// the state unchanged.
+__global__ void k_cn_check(const unsigned long long* __restrict__ totals, long long cap, int dup_pins,
+                           int* __restrict__ flags) {
   const unsigned long long need = totals[0];
   const bool ops_over = need > (unsigned long long)cap;
-  const bool net_over = totals[2] > (unsigned long long)CN_MAXOPS;
+  const bool net_over = dup_pins && totals[2] > (unsigned long long)CN_MAXOPS;
   flags[0] = ops_over ? (int)min(need, (unsigned long long)INT_MAX) : 0;
   flags[1] = net_over ? 1 : 0;
 }
@@ -239,7 +245,8 @@ __global__ void k_cn_check(const unsigned long long* __restrict__ totals, long l
 // (a block scan), the block's start comes from one atomic on the running total.
 __global__ void __launch_bounds__(256) k_cn_alloc(const int* __restrict__ touched, const int* __restrict__ tcount,
                                                   const int* __restrict__ opcnt, long long* __restrict__ opoff,
-                                                  int* __restrict__ opfill, unsigned long long* __restrict__ totals) {
+                                                  int* __restrict__ opfill, unsigned long long* __restrict__ totals,
+                                                  int* __restrict__ big, long long big_cap) {
   __shared__ long long s_w[256 / WAVE];
   __shared__ long long s_base;
   __shared__ int s_max;
@@ -256,6 +263,11 @@ __global__ void __launch_bounds__(256) k_cn_alloc(const int* __restrict__ touche
   if (valid) {
     opoff[t] = s_base + ex;
     opfill[t] = 0;
+    // too many ops to sort in LDS: the net goes on the big list (totals[4])
+    if (c > CN_MAXOPS) {
+      const unsigned long long k = atomicAdd(&totals[4], 1ull);
+      if ((long long)k < big_cap) big[k] = t;
+    }
   }
   if (threadIdx.x == 0 && s_max) atomicMax(reinterpret_cast<unsigned long long*>(&totals[2]), (unsigned long long)s_max);
 }
@@ -272,6 +284,52 @@ __global__ void k_cn_scatter(const int* __restrict__ mv_v, const unsigned long l
       const unsigned t = vtx_nets[j];
       ops[opoff[t] + atomicAdd(&opfill[t], 1)] = (int)m;
     }
+  }
+}
+
+// The big nets' op lists in push order: for each net on the big list, its
+// push indices set bits of a bitmap over the pass's pushes, then the block
+// writes the set bits back in order (each thread a run of words, a block scan
+// of the runs' counts). One block; the big nets one after another.
+constexpr int BIGSORT_BLOCK = 1024;
+
+__global__ void __launch_bounds__(BIGSORT_BLOCK) k_cn_bigsort(const int* __restrict__ big, long long big_cap,
+                                                              const unsigned long long* __restrict__ totals,
+                                                              const unsigned long long* __restrict__ nmoves,
+                                                              const int* __restrict__ flags, const int* __restrict__ opcnt,
+                                                              const long long* __restrict__ opoff,
+                                                              unsigned* __restrict__ bits, int* __restrict__ ops) {
+  __shared__ long long s_w[BIGSORT_BLOCK / WAVE];
+  if (flags[0] | flags[1]) return;
+  const int nbig = (int)min(totals[4], (unsigned long long)big_cap);
+  const long long W = ((long long)*nmoves + 31) / 32;
+  const int tid = (int)threadIdx.x;
+  const long long per = (W + BIGSORT_BLOCK - 1) / BIGSORT_BLOCK;
+  const long long w0 = min(W, (long long)tid * per), w1 = min(W, w0 + per);
+  for (int b = 0; b < nbig; b++) {
+    const int t = big[b];
+    const int c = opcnt[t];
+    const long long off = opoff[t];
+    for (long long w = tid; w < W; w += BIGSORT_BLOCK) bits[w] = 0u;
+    __syncthreads();
+    for (int j = tid; j < c; j += BIGSORT_BLOCK) {
+      const int m = ops[off + j];
+      atomicOr(&bits[m >> 5], 1u << (m & 31));
+    }
+    __syncthreads();
+    long long cnt = 0;
+    for (long long w = w0; w < w1; w++) cnt += __popc(bits[w]);
+    long long tot;
+    long long at = block_excl_scan(cnt, s_w, &tot);
+    for (long long w = w0; w < w1; w++) {
+      unsigned x = bits[w];
+      while (x) {
+        const int k = __ffs((int)x) - 1;
+        x &= x - 1u;
+        ops[off + at++] = (int)(w * 32 + k);
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -330,11 +388,16 @@ __global__ void __launch_bounds__(WAVE) k_cn_apply(CnApplyArgs a) {
     }
     const int c = a.opcnt[t];
     const long long off = a.opoff[t];
-    int P = 1;
-    while (P < c) P <<= 1;
-    for (int j = lane; j < P; j += WAVE) keys[j] = j < c ? (unsigned)a.ops[off + j] : 0xFFFFFFFFu;
-    wave_lds_sync();
-    if (c > 1) wave_bitonic_sort(keys, P);
+    // up to CN_MAXOPS ops are sorted here; a big net's ops come in order
+    // from k_cn_bigsort and are read straight from the op list
+    const bool small = c <= CN_MAXOPS;
+    if (small) {
+      int P = 1;
+      while (P < c) P <<= 1;
+      for (int j = lane; j < P; j += WAVE) keys[j] = j < c ? (unsigned)a.ops[off + j] : 0xFFFFFFFFu;
+      wave_lds_sync();
+      if (c > 1) wave_bitonic_sort(keys, P);
+    }
     const unsigned s0 = a.net_ptr[t], seg = s0 + (unsigned)t;
     int L = wave_list_load(a.ent, seg, a.net_ptr[t + 1] - s0, list);
     if (INDEXED) wave_index_set(list, L, at, a.pmask);
@@ -342,7 +405,7 @@ __global__ void __launch_bounds__(WAVE) k_cn_apply(CnApplyArgs a) {
       // 64 moves at a time: each lane loads one move's old and new part
       int from_l = 0, to_l = 0;
       if (j0 + lane < c) {
-        const int m = (int)keys[j0 + lane];
+        const int m = small ? (int)keys[j0 + lane] : a.ops[off + j0 + lane];
         from_l = a.mv_from[m];
         to_l = a.cn_new[a.mv_net[m]];
       }
@@ -419,6 +482,12 @@ fp_status plan_cn(fp_ctx* ctx, RefinePlan& p) {
     FP_TRY(dalloc(ctx, &s.mv_from, np));
     FP_TRY(dalloc(ctx, &s.mv_last, np));
     FP_TRY(dalloc(ctx, &s.totals, 8));
+    // a pass's pushes are at most its pins (one per (net, pin)): the big
+    // nets' bitmap; its ops at most sum of degree^2 (below): the big list, a
+    // big net holding more than CN_MAXOPS of them
+    FP_TRY(dalloc(ctx, &s.bigbits, (size_t)(np / 32 + 1)));
+    s.big_cap = (long long)(std::max<unsigned long long>(ctx->sum_deg2, 1ull) / (unsigned long long)CN_MAXOPS + 1);
+    FP_TRY(dalloc(ctx, &s.big, (size_t)s.big_cap));
     fill_async<int>(ctx, s.opcnt, ne, 0);
     // a pass edits the lists of a pushed vertex's nets, and a vertex is pushed
     // by at most its degree many nets: sum of degree^2 bounds the edit list
@@ -442,7 +511,7 @@ fp_status pass_cn(fp_ctx* ctx, const RefinePlan&) {
   const unsigned long long* nm = s.totals + 3;
   // one round per pass: tag 1 after clearing the stamps
   pass_begin(ctx, nullptr, 0, nullptr);
-  fill_async<unsigned long long>(ctx, s.totals, 3, 0ull);
+  fill_async<unsigned long long>(ctx, s.totals, 5, 0ull);
   CnEvalArgs ea{ctx->net_ptr, ctx->net_pins, ctx->vtx_ptr, ctx->vtx_seg, ctx->ent, ctx->ev, ctx->part,
                 noe, ctx->pb, ctx->pmask, s.old_part, s.new_part, s.npush};
   const int waves_needed = (noe + WAVE - 1) / WAVE;
@@ -458,10 +527,12 @@ fp_status pass_cn(fp_ctx* ctx, const RefinePlan&) {
   const int gm = (int)std::min<long long>((std::max(ctx->pins, 1ll) + 255) / 256, 8192);
   k_cn_prep<<<gm, 256, 0, ctx->stream>>>(pa);
   k_cn_alloc<<<(noe + 255) / 256, 256, 0, ctx->stream>>>(ctx->touched, ctx->counters + par, s.opcnt, s.opoff,
-                                                         s.opfill, s.totals);
-  k_cn_check<<<1, 1, 0, ctx->stream>>>(s.totals, s.ops_cap, ctx->flags);
+                                                         s.opfill, s.totals, s.big, s.big_cap);
+  k_cn_check<<<1, 1, 0, ctx->stream>>>(s.totals, s.ops_cap, ctx->dup_pins ? 1 : 0, ctx->flags);
   k_cn_scatter<<<gm, 256, 0, ctx->stream>>>(s.mv_v, nm, ctx->flags, ctx->vtx_ptr, ctx->vtx_nets, s.opoff, s.opfill,
                                             s.ops);
+  k_cn_bigsort<<<1, BIGSORT_BLOCK, 0, ctx->stream>>>(s.big, s.big_cap, s.totals, nm, ctx->flags, s.opcnt, s.opoff,
+                                                     s.bigbits, s.ops);
   CnApplyArgs aa{ctx->net_ptr, ctx->touched, ctx->counters + par, s.opcnt, s.opoff, s.ops, s.mv_from, s.mv_net,
                  s.new_part, ctx->ent, ctx->ev, ctx->K, ctx->pb, ctx->pmask, s.opcnt, ctx->flags};
   cn_apply<<<8192, WAVE, lds_apply, ctx->stream>>>(aa);
@@ -476,7 +547,8 @@ fp_status cn_after_pass(fp_ctx* ctx, int* redo) {
   FP_TRY(d2h(ctx, f, ctx->flags, 2));
   FP_HIP_OK(hipStreamSynchronize(ctx->stream));
   if (f[1])
-    return fail(ctx, FP_ERR_UNSUPPORTED, "a net receives more than %d moves in one CN pass; that pass was not applied",
+    return fail(ctx, FP_ERR_UNSUPPORTED,
+                "a net with a repeated pin receives more than %d moves in one CN pass; that pass was not applied",
                 CN_MAXOPS);
   if (f[0]) {
     auto& s = ctx->cn;

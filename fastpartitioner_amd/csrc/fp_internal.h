@@ -672,7 +672,10 @@ struct fp_ctx {
     int *mv_v = nullptr, *mv_net = nullptr, *mv_from = nullptr, *mv_last = nullptr;
     int *opcnt = nullptr, *opfill = nullptr, *ops = nullptr;
     long long ops_cap = 0;
-    unsigned long long* totals = nullptr;
+    unsigned long long* totals = nullptr;  // [0] ops needed [1] ops placed [2] max per net [3] pushes [4] big nets
+    unsigned* bigbits = nullptr;           // bitmap over a pass's pushes (k_cn_bigsort)
+    int* big = nullptr;                    // nets with more ops than an LDS sort takes
+    long long big_cap = 0;
   } cn;
 
   // MSV / MSRV refinement scratch (allocated on first use)

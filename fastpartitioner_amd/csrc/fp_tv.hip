@@ -89,8 +89,8 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
   extern __shared__ __align__(16) unsigned char lds[];
   const int K = a.K;
   double* s_pen = reinterpret_cast<double*>(lds);
-  long long* s_size = reinterpret_cast<long long*>(s_pen + K);
-  unsigned* wbase = reinterpret_cast<unsigned*>(s_size + K);
+  unsigned* s_skey = reinterpret_cast<unsigned*>(s_pen + K);  // size_key of each frozen part size
+  unsigned* wbase = s_skey + K;
   __shared__ long long red_sz[EVAL_BLOCK / WAVE];
   __shared__ int red_ix[EVAL_BLOCK / WAVE];
   const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
   int bi = INT_MAX;
   for (int q = (int)threadIdx.x; q < K; q += (int)blockDim.x) {
     const long long sz = a.sizes[q];
-    s_size[q] = sz;
+    s_skey[q] = size_key(sz);
     s_pen[q] = 1.0 - ((double)sz / a.C);
     if (sz < bs) { bs = sz; bi = q; }
     if (blockIdx.x == 0 && a.sizes_frozen) a.sizes_frozen[q] = sz;
@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
         const double sc = (double)adj * s_pen[qc];
         const bool ok = cv[u] != 0 && sc > 0;
         const unsigned long long k1 = ok ? (unsigned long long)__double_as_longlong(sc) : 0ull;
-        const unsigned long long k2 = ((unsigned long long)size_key(s_size[qc]) << 32) | fv[u];
+        const unsigned long long k2 = ((unsigned long long)s_skey[qc] << 32) | fv[u];
         const bool better = ok && (k1 > b1 || (k1 == b1 && k2 < b2));
         b1 = better ? k1 : b1;
         b2 = better ? k2 : b2;
@@ -323,8 +323,9 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
       d = __builtin_amdgcn_readlane(bq, wl);
       craw = (unsigned)__builtin_amdgcn_readlane((int)bc, wl);
     }
-    if (s_size[d] > a.max_pw) d = own;   // TV/partition.cpp:135-138
-    if (s_size[d] < a.min_pw) d = own;   // TV/partition.cpp:140-143
+    // the bands, compared as size keys (the same order)
+    if (s_skey[d] > size_key(a.max_pw)) d = own;   // TV/partition.cpp:135-138
+    if (s_skey[d] < size_key(a.min_pw)) d = own;   // TV/partition.cpp:140-143
     const int al = deg - (int)craw;      // arrival loss: nets of v lacking d (TV/partition.cpp:369-377)
     const bool accept = d != own && lg >= al;   // TV/partition.cpp:380
     if (lane == 0) {
@@ -727,11 +728,11 @@ size_t apply_lds(int K) {
 }
 
 size_t eval_lds(int K, int waves) {
-  return (size_t)K * 16 + (size_t)waves * (8 * (size_t)(K + WAVE) + 4 * (size_t)TOUCH_BUF);
+  return (size_t)K * 12 + (size_t)waves * (8 * (size_t)(K + WAVE) + 4 * (size_t)TOUCH_BUF);
 }
 
 int eval_waves(int K) {
-  // LDS: 16 bytes per part shared by the block + 8 bytes per part (and per
+  // LDS: 12 bytes per part shared by the block + 8 bytes per part (and per
   // spare slot) and the touched-net buffer per wavefront
   for (int w = EVAL_BLOCK / WAVE; w >= 1; w--)
     if (eval_lds(K, w) <= (size_t)LDS_BUDGET) return w;

@@ -292,3 +292,28 @@ def test_tv_ref_sizes_below_zero(scale, K, R, passes):
     assert st.sizes.min() < 0
     _same_state(got, st)
     assert np.array_equal(got.sizes, st.sizes.astype(np.int64))
+
+
+def test_cn_net_with_more_moves_than_an_lds_sort():
+    # net 0 holds every vertex and is cut; 5,000 two-pin nets (2i, 2i+1) with
+    # their pins in parts 0 and 1 push one pin each, so net 0 receives 5,000
+    # list edits in one pass: more than the apply sorts in LDS (4,096)
+    nov = 10000
+    nets = [list(range(nov))] + [[2 * i, 2 * i + 1] for i in range(nov // 2)]
+    ptr = np.cumsum([0] + [len(n) for n in nets]).astype(np.uint32)
+    hg = H.Hypergraph.from_nets(nov, ptr, np.concatenate([np.array(n) for n in nets]).astype(np.uint32),
+                                np.array([n[0] for n in nets], dtype=np.int32))
+    K = 4
+    part = np.array([(v % 2) if v < 6000 else 2 + (v % 2) for v in range(nov)], dtype=np.int32)
+    st = O.build_state(hg, K, part)
+    print(f"cn big net: nov={nov} nets={hg.noe}", flush=True)
+    with Engine(0) as eng:
+        eng.upload(hg)
+        eng.set_state(K, st.part, st.ep_ends, st.ep_parts, st.ep_cnt)
+        stats = eng.refine("CN", 2, schedule(SCHED_REF, 1, 1.0, 0, 0))
+        got = eng.get_state()
+    st.sizes = st.sizes.astype(np.int32)
+    O.cn_refine(hg, st, 2, serial=False)
+    assert stats.moves >= 4097
+    _same_state(got, st)
+    assert np.array_equal(got.sizes, st.sizes.astype(np.int64))
