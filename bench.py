@@ -198,8 +198,9 @@ def run_batch(args):
     whether every final part vector is identical."""
     from concurrent.futures import ThreadPoolExecutor
 
-    # hardware queues of this process, read when the HIP runtime starts: the
-    # batch's lanes run side by side only on distinct queues
+    # hardware queues of this process, read when the HIP runtime starts (more
+    # than the runtime's default of 4 made the batch and the one-at-a-time
+    # loop slower alike on MI355X)
     os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     from fastpartitioner_amd import hypergraph as H
     from fastpartitioner_amd.engine import Batch, Engine
@@ -738,7 +739,8 @@ def main():
     ap.add_argument("--tune-passes", type=int, default=3, help="batch: passes that double as autotuner trials")
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--lanes", type=int, default=16, help="batch: streams the members are spread over")
-    ap.add_argument("--hw-queues", type=int, default=16, help="batch/serve: hardware queues of the process")
+    ap.add_argument("--hw-queues", type=int, default=4, help="batch/serve: hardware queues of the process "
+                    "(4, the HIP runtime's default, measured fastest for the batch)")
     ap.add_argument("--graphs", type=int, default=32, help="serve: resident hypergraphs")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second")
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")

@@ -1,14 +1,15 @@
 // fp_batch.hip -- batched refinement: many resident hypergraphs (contexts) of
 // uneven size, K and objective refined together.
 //
-// Members are spread over L lanes (streams) by pass cost (the autotuner's
-// measured pass time, else pins; longest first onto the least loaded lane).
-// Each lane's members' passes (the kernel sequences fp_refine enqueues, with
-// no host synchronisation in them) are captured once, back to back, into one HIP graph per lane; a pass of the
-// batch is one launch of every lane graph, and the lanes run side by side on
-// the device's hardware queues (the HIP runtime's GPU_MAX_HW_QUEUES, 4 by
-// default). Results are those of fp_refine on each member alone: lanes and
-// launch shapes only change how the work is spread over the GPU.
+// Members are spread over L lanes by pass cost (the autotuner's measured pass
+// time, else pins; longest first onto the least loaded lane). One pass of the
+// whole batch (the kernel sequences fp_refine enqueues for every member, with
+// no host synchronisation in them) is captured once into a single HIP graph:
+// the batch stream forks into the lanes, each lane holds its members' passes
+// back to back, and the lanes join again. A pass of the batch is one launch
+// of that graph; its lanes are independent branches that the runtime runs
+// side by side. Results are those of fp_refine on each member alone: lanes
+// and launch shapes only change how the work is spread over the GPU.
 //
 // Autotuning: the first passes of a batch are run eagerly, each with another
 // candidate launch shape (wavefronts per evaluation block, evaluation grid
@@ -32,8 +33,8 @@ struct fp_batch {
   std::vector<Member> members;
   int lanes = 8;
   std::vector<hipStream_t> lane_streams;
-  std::vector<hipGraph_t> lane_graphs;
-  std::vector<hipGraphExec_t> lane_execs;
+  hipGraph_t graph = nullptr;          // one pass of every member, lanes as parallel branches
+  hipGraphExec_t exec = nullptr;
   std::vector<int> lane_of;  // member -> lane
   std::vector<hipEvent_t> events;
   std::vector<unsigned long long> gen_at_capture;  // members' generations the captured graphs were recorded at
@@ -56,10 +57,10 @@ fp_status bfail(fp_batch* b, fp_status st, const char* fmt, ...) {
   } while (0)
 
 void drop_graphs(fp_batch* b) {
-  for (hipGraphExec_t e : b->lane_execs) (void)hipGraphExecDestroy(e);
-  for (hipGraph_t g : b->lane_graphs) (void)hipGraphDestroy(g);
-  b->lane_execs.clear();
-  b->lane_graphs.clear();
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
 }
 
 hipEvent_t batch_event(fp_batch* b, size_t i) {
@@ -228,13 +229,13 @@ fp_status fp_batch_refine(fp_batch* b, int passes, int tune_passes, double* devi
   }
 
   const int rest = passes - ntune;
-  if (!b->lane_execs.empty()) {
+  if (b->exec) {
     bool same = b->gen_at_capture.size() == nm;
     for (size_t i = 0; same && i < nm; i++) same = b->gen_at_capture[i] == b->members[i].ctx->gen;
     if (!same) drop_graphs(b);
   }
   const int L = std::max(1, std::min(b->lanes, (int)nm));
-  if (rest > 0 && nm > 0 && b->lane_execs.empty()) {
+  if (rest > 0 && nm > 0 && !b->exec) {
     while ((int)b->lane_streams.size() < L) {
       hipStream_t s = nullptr;
       B_HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -259,56 +260,60 @@ fp_status fp_batch_refine(fp_batch* b, int passes, int tune_passes, double* devi
       load[(size_t)l] += cost[i];
     }
     for (size_t i = 0; i < nm; i++) B_HIP_OK(hipStreamSynchronize(b->members[i].ctx->stream));
-    // capture each lane: its members' passes, recorded on the lane stream
-    for (int l = 0; l < L; l++) {
-      hipStream_t ls = b->lane_streams[(size_t)l];
-      B_HIP_OK(hipStreamBeginCapture(ls, hipStreamCaptureModeThreadLocal));
+    {
+      // one graph: the batch stream forks into the lanes, each lane records
+      // its members' passes back to back, and the lanes join back
+      hipStream_t os = b->stream;
+      hipEvent_t fork = batch_event(b, 2 + 4 * nm);
+      if (!fork) return bfail(b, FP_ERR_HIP, "hipEventCreate failed");
+      for (int l = 0; l < L; l++)
+        if (!batch_event(b, 2 + 4 * nm + 1 + (size_t)l)) return bfail(b, FP_ERR_HIP, "hipEventCreate failed");
+      B_HIP_OK(hipStreamBeginCapture(os, hipStreamCaptureModeThreadLocal));
       fp_status st = FP_OK;
       size_t bad = 0;
-      for (size_t i = 0; i < nm && st == FP_OK; i++) {
-        if (b->lane_of[i] != l) continue;
+      hipError_t he = hipEventRecord(fork, os);
+      for (int l = 0; l < L && he == hipSuccess; l++) he = hipStreamWaitEvent(b->lane_streams[(size_t)l], fork, 0);
+      for (size_t i = 0; i < nm && st == FP_OK && he == hipSuccess; i++) {
         auto& m = b->members[i];
         const hipStream_t own = m.ctx->stream;
         const bool timing = m.ctx->kernel_timing;
-        m.ctx->stream = ls;
+        m.ctx->stream = b->lane_streams[(size_t)b->lane_of[i]];
         m.ctx->kernel_timing = false;
         st = record_pass(m.ctx, m.plan);
         m.ctx->stream = own;
         m.ctx->kernel_timing = timing;
         bad = i;
       }
+      for (int l = 0; l < L && he == hipSuccess; l++) {
+        hipEvent_t j = b->events[2 + 4 * nm + 1 + (size_t)l];
+        he = hipEventRecord(j, b->lane_streams[(size_t)l]);
+        if (he == hipSuccess) he = hipStreamWaitEvent(os, j, 0);
+      }
       hipGraph_t g = nullptr;
-      const hipError_t ce = hipStreamEndCapture(ls, &g);
+      const hipError_t ce = hipStreamEndCapture(os, &g);
       if (st) {
         if (g) (void)hipGraphDestroy(g);
         return member_fail(b, bad, st);
       }
-      if (ce != hipSuccess || !g) return bfail(b, FP_ERR_HIP, "lane %d: pass capture failed: %s", l, hipGetErrorString(ce));
-      b->lane_graphs.push_back(g);
-      hipGraphExec_t ex = nullptr;
-      B_HIP_OK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
-      b->lane_execs.push_back(ex);
+      if (he != hipSuccess || ce != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        return bfail(b, FP_ERR_HIP, "pass capture failed: %s", hipGetErrorString(he != hipSuccess ? he : ce));
+      }
+      b->graph = g;
+      B_HIP_OK(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
     }
     b->gen_at_capture.resize(nm);
     for (size_t i = 0; i < nm; i++) b->gen_at_capture[i] = b->members[i].ctx->gen;
   }
   if (rest > 0 && nm > 0) {
-    // each lane starts after its members' planning work
+    // the pass graph starts after the members' planning work
     for (size_t i = 0; i < nm; i++) {
       hipEvent_t e = batch_event(b, 2 + 3 * nm + i);
       if (!e) return bfail(b, FP_ERR_HIP, "hipEventCreate failed");
       B_HIP_OK(hipEventRecord(e, b->members[i].ctx->stream));
-      B_HIP_OK(hipStreamWaitEvent(b->lane_streams[(size_t)b->lane_of[i]], e, 0));
-    }
-    for (int it = 0; it < rest; it++)
-      for (int l = 0; l < (int)b->lane_execs.size(); l++)
-        B_HIP_OK(hipGraphLaunch(b->lane_execs[(size_t)l], b->lane_streams[(size_t)l]));
-    for (int l = 0; l < (int)b->lane_execs.size(); l++) {
-      hipEvent_t e = batch_event(b, 2 + 4 * nm + (size_t)l);
-      if (!e) return bfail(b, FP_ERR_HIP, "hipEventCreate failed");
-      B_HIP_OK(hipEventRecord(e, b->lane_streams[(size_t)l]));
       B_HIP_OK(hipStreamWaitEvent(b->stream, e, 0));
     }
+    for (int it = 0; it < rest; it++) B_HIP_OK(hipGraphLaunch(b->exec, b->stream));
   }
   // the batch stream's end event waits for work left on member streams
   for (size_t i = 0; i < nm; i++) {

@@ -196,9 +196,9 @@ fp_status fp_get_connections(fp_ctx* ctx, int* pc, int* msg);
 
 /* ---- batched refinement ------------------------------------------------- */
 /* A batch refines many resident hypergraphs of uneven size, K and objective
- * together. Members are spread over lanes (streams) by size; each lane's
- * members' passes are captured once into one HIP graph, and a batch pass is
- * one launch of each lane graph, the lanes running side by side. Every member
+ * together. Members are spread over lanes by pass cost; one pass of every
+ * member is captured once into a single HIP graph whose lanes are parallel
+ * branches, and a batch pass is one launch of that graph. Every member
  * ends exactly where fp_refine(ctx, metric, passes, sched) alone would leave
  * it. The reference has no counterpart (it refines one hypergraph per
  * process). */
@@ -211,8 +211,8 @@ const char* fp_batch_last_error(const fp_batch* batch);
  * the context; it must outlive the batch's use of it. */
 fp_status fp_batch_add(fp_batch* batch, fp_ctx* ctx, fp_metric metric, const fp_schedule* sched);
 int fp_batch_size(const fp_batch* batch);
-/* Number of lanes (1..64, default 8). More lanes than the runtime's hardware
- * queues (GPU_MAX_HW_QUEUES, 4 by default) share queues. */
+/* Number of lanes, the parallel branches of the pass graph (1..64, default
+ * 8). */
 fp_status fp_batch_set_lanes(fp_batch* batch, int lanes);
 /* `passes` passes of every member. The first tune_passes of them run eagerly
  * with a different candidate launch shape each (wavefronts per evaluation
