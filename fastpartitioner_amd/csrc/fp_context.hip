@@ -100,6 +100,8 @@ int bits_for(int K) {
   return b;
 }
 
+}  // namespace
+
 // K-sized arrays and the packed-entry layout for K parts
 fp_status ensure_k(fp_ctx* ctx, int K) {
   if (K < 1) return fail(ctx, FP_ERR_ARG, "K must be >= 1");
@@ -128,6 +130,8 @@ fp_status ensure_k(fp_ctx* ctx, int K) {
   ctx->pmask = (1u << pb) - 1u;
   return FP_OK;
 }
+
+namespace {
 
 constexpr int BUILD_KMAX = 4096;  // k_build_lists keeps four K-int arrays per wavefront in LDS
 

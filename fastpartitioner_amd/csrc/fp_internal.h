@@ -308,7 +308,7 @@ __device__ __forceinline__ void wave_ilist_move(unsigned* list, unsigned short* 
                                                 unsigned pmask, unsigned& co, bool& found) {
   const int lane = lane_id();
   const unsigned inc = 1u << pb;
-  const int jp = __builtin_amdgcn_readfirstlane((int)at[p]);
+  const int jp = p < 0 ? (int)LIST_NONE : __builtin_amdgcn_readfirstlane((int)at[p]);  // p -1: no part yet
   int jq = __builtin_amdgcn_readfirstlane((int)at[q]);
   const bool hasp = jp != (int)LIST_NONE;
   const unsigned x = (unsigned)__builtin_amdgcn_readfirstlane((int)list[hasp ? jp : 0]);
@@ -815,6 +815,8 @@ inline fp_status timing_end(fp_ctx* ctx, hipEvent_t tb) {
   return FP_OK;
 }
 
+// K-sized arrays and the packed-entry layout for K parts
+fp_status ensure_k(fp_ctx* ctx, int K);
 // k_pass_begin on the context's stream (val may be null)
 void pass_begin(fp_ctx* ctx, const int* val, int K, int* mstat);
 

@@ -75,16 +75,33 @@ struct EvalArgs {
   long long* cand_tot;
   int* hist;
   long long* sizes_frozen;
+  // initial partition (nullable): the lowest-index part of minimum frozen size
+  int* fb_out;
 };
+
+// slot decision of an unplaced vertex whose part the fallback assigns
+constexpr int SLOT_FALLBACK = -2;
 
 __device__ __forceinline__ bool better(double sc, long long sz, unsigned fi, double bsc, long long bsz, unsigned bfi) {
   return sc > bsc || (sc == bsc && (sz < bsz || (sz == bsz && fi < bfi)));
 }
 
+// The evaluation's modes: a refinement round (TV/partition.cpp:732-805), or a
+// sub-round of the streaming initial partition, whose unplaced vertices
+// (part -1) are scored over every list of their nets (LDG2,
+// TV/partition.cpp:214-233; plain LDG, MGV/partition.cpp:28-65) or only over
+// the lists of their uncut nets (CN/partition.cpp:138-156).
+enum EvalMode { EVAL_REFINE = 0, EVAL_INIT = 1, EVAL_INIT_CN = 2 };
+
 // Per vertex: sweep, LDG2 decision, TV gain test (TV/partition.cpp:732-805).
 // Eight waves per SIMD (the hardware's maximum) hide more of the dependent
 // list reads than the registers a freer allocation would save: the compiler
 // keeps within 64 VGPRs by parking cold scalar values in VGPR lanes.
+// EVAL_INIT*: a vertex already placed is skipped (no candidate); an unplaced
+// one is a candidate for its LDG choice, the lowest-index part of minimum
+// size standing in where the choice is above max_pw (where the reference
+// would index part_sizes[-1]); the min_pw band does not apply (own == -1).
+template <int MODE>
 __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) k_tv_eval(EvalArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int K = a.K;
@@ -128,6 +145,7 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
     for (int w = 1; w < nw; w++)
       if (red_sz[w] < b || (red_sz[w] == b && red_ix[w] < fb)) { b = red_sz[w]; fb = red_ix[w]; }
   }
+  if (MODE != EVAL_REFINE && blockIdx.x == 0 && threadIdx.x == 0) *a.fb_out = fb;
 
   // One wavefront per vertex, grid-stride over the round's slots, software
   // pipelined over the wave's vertices: while vertex X is swept, the segment
@@ -212,7 +230,8 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
           lmax = max(lmax, Lj);
           lj = lane == j ? Lj : lj;
           const int t = lane - 1;
-          const bool ok = lane >= 1 && t < Lj;
+          // EVAL_INIT_CN: only nets of one part vote (lambda == 1)
+          const bool ok = lane >= 1 && t < Lj && (MODE != EVAL_INIT_CN || Lj == 1);
           const unsigned x = xv[u];
           const int q = (int)(x & ~EV_SINGLE);
           const int qs = ok ? q : spare;
@@ -231,7 +250,7 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
       // numbered net by net by a scan of the per-net window counts, and
       // LONG_TASKS tasks' windows are loaded together, whichever nets they
       // belong to; the lambdas come from the first windows.
-      if (lmax > WAVE - 1) {
+      if (MODE != EVAL_INIT_CN && lmax > WAVE - 1) {
         const int nwin = lj >> 6;  // ceil((lambda - 63) / 64) windows past the first
         const int incl = wave_incl_scan(nwin);
         const int ntask = __builtin_amdgcn_readlane(incl, WAVE - 1);
@@ -324,16 +343,26 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
       craw = (unsigned)__builtin_amdgcn_readlane((int)bc, wl);
     }
     // the bands, compared as size keys (the same order)
-    if (s_skey[d] > size_key(a.max_pw)) d = own;   // TV/partition.cpp:135-138
-    if (s_skey[d] < size_key(a.min_pw)) d = own;   // TV/partition.cpp:140-143
-    const int al = deg - (int)craw;      // arrival loss: nets of v lacking d (TV/partition.cpp:369-377)
-    const bool accept = d != own && lg >= al;   // TV/partition.cpp:380
+    bool accept;
+    int dd;  // the slot's decision: a part, -1 (none) or SLOT_FALLBACK
+    if (MODE == EVAL_REFINE) {
+      if (s_skey[d] > size_key(a.max_pw)) d = own;   // TV/partition.cpp:135-138
+      if (s_skey[d] < size_key(a.min_pw)) d = own;   // TV/partition.cpp:140-143
+      const int al = deg - (int)craw;      // arrival loss: nets of v lacking d (TV/partition.cpp:369-377)
+      accept = d != own && lg >= al;       // TV/partition.cpp:380
+      dd = accept ? d : -1;
+    } else {
+      // an unplaced vertex with no part above score 0, or whose choice is
+      // above the band, takes its turn in the fallback
+      accept = own < 0;
+      dd = !accept ? -1 : (m1 == 0 || s_skey[d] > size_key(a.max_pw) ? SLOT_FALLBACK : d);
+    }
     if (lane == 0) {
       a.slot_own[slot] = own;
-      a.slot_dec[slot] = accept ? d : -1;
-      if (a.dec_out) a.dec_out[v] = d;
-      if (a.gain_out) a.gain_out[v] = d == own ? INT_MIN : lg - al;
-      if (accept && a.cand_tot) {
+      a.slot_dec[slot] = dd;
+      if (MODE == EVAL_REFINE && a.dec_out) a.dec_out[v] = d;
+      if (MODE == EVAL_REFINE && a.gain_out) a.gain_out[v] = d == own ? INT_MIN : lg - (deg - (int)craw);
+      if (dd >= 0 && a.cand_tot) {
         const int w = a.vwgt ? a.vwgt[v] : 1;
         atomic_add_i64(&a.cand_tot[d], w);
         atomicAdd(&a.hist[(size_t)d * (size_t)a.ntiles + (size_t)(slot / TILE)], w);
@@ -408,10 +437,40 @@ __global__ void __launch_bounds__(256) k_bal_scan(int* __restrict__ hist, long l
   }
 }
 
+// Initial partition, between the evaluation and the admission (one block):
+// the j-th fallback slot of the sub-round, in slot order, takes part
+// (fb + j) mod K, fb the lowest-index part of minimum frozen size, and its
+// weight joins the candidate totals and tile histogram.
+__global__ void __launch_bounds__(1024) k_init_fallback(int* __restrict__ slot_dec, const int* __restrict__ order,
+                                                        const int* __restrict__ vwgt, int r0, int nslots, int ntiles,
+                                                        int K, const int* __restrict__ fb_in,
+                                                        long long* __restrict__ cand_tot, int* __restrict__ hist) {
+  __shared__ int s_wi[16];
+  const int fb = *fb_in;
+  int carry = 0;
+  for (int b0 = 0; b0 < nslots; b0 += (int)blockDim.x) {
+    const int slot = b0 + (int)threadIdx.x;
+    const bool f = slot < nslots && slot_dec[slot] == SLOT_FALLBACK;
+    int tot;
+    const int j = carry + block_excl_scan(f ? 1 : 0, s_wi, &tot);
+    if (f) {
+      const int d = (int)(((long long)fb + j) % K);
+      slot_dec[slot] = d;
+      const int z = r0 + slot;
+      const int w = vwgt ? vwgt[order ? order[z] : z] : 1;
+      atomic_add_i64(&cand_tot[d], w);
+      atomicAdd(&hist[(size_t)d * (size_t)ntiles + (size_t)(slot / TILE)], w);
+    }
+    carry += tot;
+  }
+}
+
 // BAL step 2, one wavefront per tile: a candidate into an over-subscribed part
 // d is admitted iff frozen_size[d] + (weight of earlier candidates into d) + w
-// <= cap; every other candidate is admitted. Commits part vector and sizes of
-// admitted moves; clears the tile's histogram cells of non-over parts. AGG:
+// <= cap (first_free: or it is the first candidate into d); every other
+// candidate is admitted. Commits part vector and sizes of admitted moves (a
+// vertex without a part, own -1, only adds to its new part's size); clears
+// the tile's histogram cells of non-over parts. AGG:
 // the block sums its size changes per part in LDS first, so that sizes takes
 // one atomic per part and block (same-address atomics from every wave would
 // queue at one L2 slice).
@@ -420,7 +479,7 @@ __global__ void __launch_bounds__(256) k_bal_admit(int* __restrict__ slot_dec, c
                                                    const int* __restrict__ order, const int* __restrict__ vwgt,
                                                    int r0, int nslots, int ntiles, int K,
                                                    const long long* __restrict__ sizes_frozen, long long cap,
-                                                   const int* __restrict__ over_idx,
+                                                   bool first_free, const int* __restrict__ over_idx,
                                                    const int* __restrict__ over_count,
                                                    const long long* __restrict__ pref, int* __restrict__ hist,
                                                    int* __restrict__ part, long long* __restrict__ sizes,
@@ -460,7 +519,10 @@ __global__ void __launch_bounds__(256) k_bal_admit(int* __restrict__ slot_dec, c
         const int dj = __builtin_amdgcn_readlane(d, j), wj = __builtin_amdgcn_readlane(wi, j);
         if (j < lane && dj == d) pre += wj;
       }
-      if (o >= 0) adm = sizes_frozen[d] + pref[(size_t)o * (size_t)ntiles + (size_t)tile] + pre + w <= cap;
+      if (o >= 0) {
+        const long long before = pref[(size_t)o * (size_t)ntiles + (size_t)tile] + pre;
+        adm = sizes_frozen[d] + before + w <= cap || (first_free && before == 0);
+      }
     }
     if (d >= 0) {
       if (o < 0) hist[(size_t)d * (size_t)ntiles + (size_t)tile] = 0;
@@ -468,11 +530,12 @@ __global__ void __launch_bounds__(256) k_bal_admit(int* __restrict__ slot_dec, c
     }
     if (adm) {
       part[v] = d;
+      const int own = slot_own[slot];
       if (AGG) {
-        atomicAdd(&s_dsz[slot_own[slot]], (unsigned long long)-w);
+        if (own >= 0) atomicAdd(&s_dsz[own], (unsigned long long)-w);
         atomicAdd(&s_dsz[d], (unsigned long long)w);
       } else {
-        atomic_add_i64(&sizes[slot_own[slot]], -w);
+        if (own >= 0) atomic_add_i64(&sizes[own], -w);
         atomic_add_i64(&sizes[d], w);
       }
     }
@@ -855,7 +918,7 @@ fp_status pass_tv(fp_ctx* ctx, const RefinePlan& p) {
     }
     int grid = eval_grid(nslots, waves);
     if (p.eval_grid > 0) grid = std::min(grid, p.eval_grid);
-    k_tv_eval<<<grid, waves * WAVE, lds_e, ctx->stream>>>(a);
+    k_tv_eval<EVAL_REFINE><<<grid, waves * WAVE, lds_e, ctx->stream>>>(a);
     if (ctx->kernel_timing) FP_HIP_OK(hipEventRecord(tb, ctx->stream));
     if (bal) {
       k_bal_scan<<<std::max(1, std::min(K, 512)), 256, lds_s, ctx->stream>>>(
@@ -866,11 +929,11 @@ fp_status pass_tv(fp_ctx* ctx, const RefinePlan& p) {
       if ((size_t)K * sizeof(long long) <= (size_t)LDS_BUDGET)
         k_bal_admit<true><<<agrid, 256, (size_t)K * sizeof(long long), ctx->stream>>>(
             ctx->slot_dec, ctx->slot_own, ctx->order, ctx->vwgt, r0, nslots, ntiles, K, ctx->sizes_frozen, sched->cap,
-            ctx->over_idx, ctx->over_count, ctx->pref, ctx->hist, ctx->part, ctx->sizes, ctx->cand_tot, ctx->moves);
+            false, ctx->over_idx, ctx->over_count, ctx->pref, ctx->hist, ctx->part, ctx->sizes, ctx->cand_tot, ctx->moves);
       else
         k_bal_admit<false><<<agrid, 256, 0, ctx->stream>>>(
             ctx->slot_dec, ctx->slot_own, ctx->order, ctx->vwgt, r0, nslots, ntiles, K, ctx->sizes_frozen, sched->cap,
-            ctx->over_idx, ctx->over_count, ctx->pref, ctx->hist, ctx->part, ctx->sizes, ctx->cand_tot, ctx->moves);
+            false, ctx->over_idx, ctx->over_count, ctx->pref, ctx->hist, ctx->part, ctx->sizes, ctx->cand_tot, ctx->moves);
     }
     ApplyArgs ap{};
     ap.net_ptr = ctx->net_ptr;
@@ -981,6 +1044,150 @@ fp_status fp_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule
   return FP_OK;
 }
 
+fp_status fp_initial_partition_device(fp_ctx* ctx, int mode, int K, int round, double C, long long max_pw,
+                                      long long cap, fp_init_stats* stats) {
+  if (!ctx) return FP_ERR_ARG;
+  if (!ctx->has_graph) return fail(ctx, FP_ERR_STATE, "fp_initial_partition_device before fp_upload");
+  if (mode < 0 || mode > 2) return fail(ctx, FP_ERR_ARG, "unknown initial partition mode %d", mode);
+  if (round < 1) return fail(ctx, FP_ERR_ARG, "round size must be >= 1");
+  if (!(C > 0)) return fail(ctx, FP_ERR_ARG, "C must be > 0");
+  if (ctx->dup_pins) return fail(ctx, FP_ERR_UNSUPPORTED, "a net lists the same vertex twice");
+  FP_HIP_OK(hipSetDevice(ctx->device));
+  FP_TRY(ensure_k(ctx, K));
+  FP_TRY(check_envelope(ctx));
+  if (apply_lds(K) > (size_t)LDS_BUDGET) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d too large for the apply kernel", K);
+  const int nov = ctx->nov;
+  const int R = std::max(1, std::min(round, nov));
+  const int waves = eval_waves(K);
+  FP_TRY(ensure_round_scratch(ctx, R));
+  FP_TRY(ensure_bal_scratch(ctx, R));
+  // the empty state: no part, empty lists, zero sizes
+  const size_t segs = (size_t)ctx->pins + (size_t)ctx->noe + ENT_PAD;
+  fill_async<int>(ctx, ctx->part, (size_t)std::max(nov, 1), -1);
+  fill_async<unsigned>(ctx, ctx->ent, segs, 0u);
+  fill_async<unsigned short>(ctx, ctx->ev, segs, (unsigned short)0);
+  fill_async<long long>(ctx, ctx->sizes, (size_t)K, 0);
+  fill_async<long long>(ctx, ctx->cand_tot, (size_t)K, 0);
+  fill_async<unsigned long long>(ctx, ctx->moves, 1, 0ull);
+  pass_begin(ctx, nullptr, 0, nullptr);
+  hipEvent_t e0 = pooled_event(ctx, 0), e1 = pooled_event(ctx, 1);
+  if (!e0 || !e1) return fail(ctx, FP_ERR_HIP, "hipEventCreate failed");
+  FP_HIP_OK(hipEventRecord(e0, ctx->stream));
+  const int eval_mode = mode == 2 ? EVAL_INIT_CN : EVAL_INIT;
+  const size_t lds_e = eval_lds(K, waves);
+  const size_t lds_a = apply_lds(K);
+  auto apply = apply_indexed(K) ? k_apply<true> : k_apply<false>;
+  const bool agg = (size_t)K * sizeof(long long) <= (size_t)LDS_BUDGET;
+  int* fb = ctx->counters + 2;
+  unsigned tag = 0;
+  long long subrounds = 0, rounds = 0;
+  int burst = 1;  // sub-rounds enqueued before the host looks at the progress
+  auto subround = [&](int r0, int nslots) -> fp_status {
+    const int ntiles = (nslots + TILE - 1) / TILE;
+    tag++;
+    const int par = (int)(tag & 1u);
+    EvalArgs a{};
+    a.pad_seg = (unsigned)((size_t)ctx->pins + (size_t)ctx->noe);
+    a.vtx_ptr = ctx->vtx_ptr;
+    a.vtx_nets = ctx->vtx_nets;
+    a.vtx_seg = ctx->vtx_seg;
+    a.ev = ctx->ev;
+    a.part = ctx->part;
+    a.order = ctx->order;
+    a.vwgt = ctx->vwgt;
+    a.sizes = ctx->sizes;
+    a.K = K;
+    a.pb = ctx->pb;
+    a.pmask = ctx->pmask;
+    a.C = C;
+    a.max_pw = mode == 0 ? max_pw : LLONG_MAX;
+    a.min_pw = LLONG_MIN;
+    a.slot_own = ctx->slot_own;
+    a.slot_dec = ctx->slot_dec;
+    a.r0 = r0;
+    a.nslots = nslots;
+    a.ntiles = ntiles;
+    a.tag = tag;
+    a.net_stamp = ctx->net_stamp;
+    a.touched = ctx->touched;
+    a.tcount = ctx->counters + par;
+    a.tcount_next = ctx->counters + (par ^ 1);
+    a.cand_tot = ctx->cand_tot;
+    a.hist = ctx->hist;
+    a.sizes_frozen = ctx->sizes_frozen;
+    a.fb_out = fb;
+    const int grid = eval_grid(nslots, waves);
+    if (eval_mode == EVAL_INIT_CN)
+      k_tv_eval<EVAL_INIT_CN><<<grid, waves * WAVE, lds_e, ctx->stream>>>(a);
+    else
+      k_tv_eval<EVAL_INIT><<<grid, waves * WAVE, lds_e, ctx->stream>>>(a);
+    k_init_fallback<<<1, 1024, 0, ctx->stream>>>(ctx->slot_dec, ctx->order, ctx->vwgt, r0, nslots, ntiles, K, fb,
+                                                 ctx->cand_tot, ctx->hist);
+    k_bal_scan<<<std::max(1, std::min(K, 512)), 256, (size_t)K * sizeof(int), ctx->stream>>>(
+        ctx->hist, ctx->pref, ntiles, K, ctx->sizes_frozen, ctx->cand_tot, cap, ctx->over_idx, ctx->over_count);
+    const int agrid = std::max(1, std::min((ntiles + 3) / 4, 4096));
+    if (agg)
+      k_bal_admit<true><<<agrid, 256, (size_t)K * sizeof(long long), ctx->stream>>>(
+          ctx->slot_dec, ctx->slot_own, ctx->order, ctx->vwgt, r0, nslots, ntiles, K, ctx->sizes_frozen, cap, true,
+          ctx->over_idx, ctx->over_count, ctx->pref, ctx->hist, ctx->part, ctx->sizes, ctx->cand_tot, ctx->moves);
+    else
+      k_bal_admit<false><<<agrid, 256, 0, ctx->stream>>>(
+          ctx->slot_dec, ctx->slot_own, ctx->order, ctx->vwgt, r0, nslots, ntiles, K, ctx->sizes_frozen, cap, true,
+          ctx->over_idx, ctx->over_count, ctx->pref, ctx->hist, ctx->part, ctx->sizes, ctx->cand_tot, ctx->moves);
+    ApplyArgs ap{};
+    ap.net_ptr = ctx->net_ptr;
+    ap.apply_pins = ctx->apply_pins;
+    ap.slot_own = ctx->slot_own;
+    ap.slot_dec = ctx->slot_dec;
+    ap.touched = ctx->touched;
+    ap.tcount = ctx->counters + par;
+    ap.ent = ctx->ent;
+    ap.ev = ctx->ev;
+    ap.K = K;
+    ap.pb = ctx->pb;
+    ap.pmask = ctx->pmask;
+    ap.r0 = r0;
+    ap.r1 = r0 + nslots;
+    ap.net_blocks = net_grid(ctx, APPLY_NET_BLOCKS, APPLY_BLOCK / WAVE);
+    apply<<<ap.net_blocks, APPLY_BLOCK, lds_a, ctx->stream>>>(ap);
+    FP_HIP_OK(hipGetLastError());
+    subrounds++;
+    return FP_OK;
+  };
+  for (int r0 = 0; r0 < nov; r0 += R) {
+    const int nslots = std::min(R, nov - r0);
+    rounds++;
+    // every sub-round places at least one vertex of the round: a round takes
+    // at most nslots of them
+    int launched = 0;
+    for (;;) {
+      const int n = std::min(burst, nslots - launched);
+      for (int i = 0; i < n; i++) FP_TRY(subround(r0, nslots));
+      launched += n;
+      unsigned long long placed = 0;
+      FP_TRY(d2h(ctx, &placed, ctx->moves, 1));
+      FP_HIP_OK(hipStreamSynchronize(ctx->stream));
+      if (placed == (unsigned long long)r0 + (unsigned long long)nslots) break;
+      if (launched >= nslots)
+        return fail(ctx, FP_ERR_STATE, "initial partition round at %d did not place its vertices", r0);
+    }
+    // the next round starts with as many sub-rounds as this one took (one
+    // fewer when this one was done at its first look, to probe downwards)
+    burst = launched == burst ? std::max(1, burst - 1) : launched;
+  }
+  FP_HIP_OK(hipEventRecord(e1, ctx->stream));
+  FP_HIP_OK(hipStreamSynchronize(ctx->stream));
+  ctx->has_state = true;
+  if (stats) {
+    float ms = 0;
+    FP_HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    stats->rounds = rounds;
+    stats->subrounds = subrounds;
+    stats->device_ms = ms;
+  }
+  return FP_OK;
+}
+
 fp_status fp_eval_moves(fp_ctx* ctx, fp_metric metric, const fp_schedule* sched, int* dec, int* gain) {
   if (!ctx) return FP_ERR_ARG;
   if (!ctx->has_state) return fail(ctx, FP_ERR_STATE, "fp_eval_moves before a state was set");
@@ -1003,7 +1210,7 @@ fp_status fp_eval_moves(fp_ctx* ctx, fp_metric metric, const fp_schedule* sched,
   a.ntiles = (nov + TILE - 1) / TILE;
   a.dec_out = d_dec;
   a.gain_out = d_gain;
-  k_tv_eval<<<eval_grid(nov, waves), waves * WAVE, eval_lds(K, waves), ctx->stream>>>(a);
+  k_tv_eval<EVAL_REFINE><<<eval_grid(nov, waves), waves * WAVE, eval_lds(K, waves), ctx->stream>>>(a);
   fp_status st = FP_OK;
   if (hipGetLastError() != hipSuccess) st = fail(ctx, FP_ERR_HIP, "k_tv_eval launch failed");
   if (st == FP_OK && dec) st = d2h(ctx, dec, d_dec, (size_t)nov);

@@ -34,6 +34,10 @@ class RefineStats(ctypes.Structure):
                 ("pin_visits", ctypes.c_longlong), ("device_ms", ctypes.c_double)]
 
 
+class InitStats(ctypes.Structure):
+    _fields_ = [("rounds", ctypes.c_longlong), ("subrounds", ctypes.c_longlong), ("device_ms", ctypes.c_double)]
+
+
 class Metrics(ctypes.Structure):
     _fields_ = [(k, ctypes.c_longlong) for k in ("tv", "cn", "tm", "msm", "msv", "msrv")]
 
@@ -58,6 +62,8 @@ def declare(lib) -> None:
     lib.fp_device_count.argtypes = [c.POINTER(c.c_int)]
     lib.fp_upload.argtypes = [P, c.c_int, c.c_int, _U32P, _U32P, _U32P, _U32P, _I32P, _I32P, _I32P]
     lib.fp_set_partition.argtypes = [P, c.c_int, _I32P]
+    lib.fp_initial_partition_device.argtypes = [P, c.c_int, c.c_int, c.c_int, c.c_double, c.c_longlong,
+                                                 c.c_longlong, c.POINTER(InitStats)]
     lib.fp_set_state.argtypes = [P, c.c_int, _I32P, _I32P, _I32P, _I32P, _I64P]
     lib.fp_get_state.argtypes = [P, _I32P, _I32P, _I32P, _I32P, _I64P]
     lib.fp_refine.argtypes = [P, c.c_int, c.c_int, c.POINTER(Schedule), c.POINTER(RefineStats)]
@@ -162,6 +168,17 @@ class Engine:
         self.K = K
         part = np.ascontiguousarray(part, dtype=np.int32)
         self._check(self.lib.fp_set_partition(self.ctx, K, _p(part, _I32P)), "fp_set_partition")
+
+    def initial_partition(self, mode: int, K: int, round: int, C: float, max_pw: int, cap: int) -> InitStats:
+        """The streaming initial partition on the device (fp_initial_partition_device):
+        mode 0 LDG2 with the max_pw band, 1 band-free LDG, 2 CN's LDG over
+        uncut nets; rounds of `round` ordering positions (1 = the reference's
+        sequential stream). Leaves the state resident."""
+        self.K = K
+        st = InitStats()
+        self._check(self.lib.fp_initial_partition_device(self.ctx, mode, K, round, C, max_pw, cap, ctypes.byref(st)),
+                    "fp_initial_partition_device")
+        return st
 
     def set_state(self, K: int, part, ep_ends, ep_parts, ep_cnt, sizes=None):
         self.K = K

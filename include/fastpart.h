@@ -124,6 +124,34 @@ fp_status fp_upload(fp_ctx* ctx, int nov, int noe, const unsigned* net_ptr, cons
  * streaming initial partition leaves (TV/partition.cpp:240-263). */
 fp_status fp_set_partition(fp_ctx* ctx, int K, const int* part);
 
+/* Streaming initial partition on the device, leaving the part vector, the
+ * connectivity lists (in placement order, as the reference's stream leaves
+ * them: TV/partition.cpp:240-263) and the part sizes resident, ready for
+ * fp_refine. Replaces the host stream of TV/partition.cpp:207-264 (mode
+ * FP_INIT_LDG_BANDS = 0, LDG2 with the max_pw band), MGV/partition.cpp:122-189
+ * and MSRV/partition.cpp:131-208 (FP_INIT_LDG = 1, band-free LDG) and
+ * CN/partition.cpp:132-221 (FP_INIT_CN_LDG = 2, votes of uncut nets only);
+ * the mode values are those of fastpart_host.h's fp_initial_partition.
+ * Buffered rounds: the ordering is cut into rounds of `round` positions; each
+ * sub-round decides every unplaced vertex of the round on the frozen state;
+ * candidates with no positive score fall back in turn, the j-th (ordering
+ * order) taking part (f + j) mod K, f the lowest-index part of minimum size;
+ * per part, the longest ordering-order prefix of its candidates within cap is
+ * admitted (the first candidate into a part always); the admitted are placed
+ * in ordering order and the rest decided again. round = 1 is the reference's
+ * sequential stream exactly (where the reference is defined: a choice above
+ * max_pw, where it would index part_sizes[-1], falls back here). C is the
+ * LDG capacity (TV/partition.cpp:159-160), max_pw the band (mode 0 only),
+ * cap the admission cap (fp_balance_cap). stats (nullable) receives the
+ * rounds, the sub-rounds and the device time. */
+typedef struct {
+  long long rounds;
+  long long subrounds;
+  double device_ms;
+} fp_init_stats;
+fp_status fp_initial_partition_device(fp_ctx* ctx, int mode, int K, int round, double C, long long max_pw,
+                                      long long cap, fp_init_stats* stats);
+
 /* State in the reference's layout (TV/partition.cpp:150-158): ep_ends[noe]
  * (lambda), and per net n the slots net_ptr[n] .. net_ptr[n]+ep_ends[n]-1 of
  * ep_parts[pins]/ep_cnt[pins]. sizes (K, nullable) are the part sizes to

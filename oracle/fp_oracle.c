@@ -16,6 +16,9 @@
  * unit, the reference's only case): a move changes the part sizes by the
  * vertex's weight, as the engine does. Set by or_set_vertex_weights. */
 static const int* g_vwgt = NULL;
+/* test hook: a planted mistake (0 = none), see or_plant */
+static int g_plant = 0;
+void or_plant(int code) { g_plant = code; }
 #define VW(v) (g_vwgt ? g_vwgt[(v)] : 1)
 void or_set_vertex_weights(const int* vwgt) { g_vwgt = vwgt; }
 
@@ -240,6 +243,133 @@ int or_tv_ldg(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr
   return rc;
 }
 
+/* Buffered round streaming initial partition: the engine's device stream.
+ * The ordering is cut into rounds of R positions. Each sub-round of a round
+ * decides every still unplaced vertex of the round against the state frozen
+ * at the sub-round's start (lists and part sizes), with the reference's
+ * stream scoring for the mode:
+ *   mode 0: LDG2 over every list of the vertex's nets (TV/partition.cpp:89-145,
+ *           214-233), a choice above max_pw counting as no choice (where the
+ *           reference would index part_sizes[-1])
+ *   mode 1: band-free LDG over every list (MGV/partition.cpp:28-65)
+ *   mode 2: LDG over the lists of uncut nets only (CN/partition.cpp:138-191)
+ * Candidates without a choice (no part scores above 0) fall back in turn:
+ * the j-th of them in ordering order takes part (f + j) mod K, f the
+ * lowest-index part of minimum size (the reference's fallback when j = 0).
+ * Then per part, in ordering order, the longest prefix of its candidates
+ * whose weight fits is admitted: candidate i into part d is admitted iff no
+ * earlier candidate of the sub-round chose d, or the candidates into d all
+ * fit (their total <= cap - size[d]), or size[d] + (weight of the earlier
+ * candidates into d) + w(i) <= cap. The admitted are placed in ordering
+ * order (part, size, list appends: TV/partition.cpp:240-263); the rest are
+ * decided again in the next sub-round. Each sub-round places at least the
+ * first candidate into each chosen part, so a round ends; R = 1 is the
+ * reference's sequential stream exactly. Sizes are weights (int64). */
+int or_round_ldg(int mode, int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr,
+                 const unsigned* vtx_nets, const int* ordering, int K, double C, long long max_pw, int R,
+                 long long cap, int* part, int* ep_ends, int* ep_parts, int* ep_cnt, long long* sizes,
+                 long long* subrounds) {
+  long long pins = net_ptr[noe];
+  if (R < 1 || K < 1 || mode < 0 || mode > 2) return 1;
+  const int RR = R < nov ? R : (nov > 0 ? nov : 1);
+  long long* marker = (long long*)malloc(sizeof(long long) * (size_t)K);
+  int* active = (int*)malloc(sizeof(int) * (size_t)K);
+  double* scores = (double*)calloc((size_t)K, sizeof(double));
+  long long* tot = (long long*)malloc(sizeof(long long) * (size_t)K);
+  long long did = 0;
+  long long* run = (long long*)malloc(sizeof(long long) * (size_t)K);
+  int* cz = (int*)malloc(sizeof(int) * (size_t)RR);
+  int* cd = (int*)malloc(sizeof(int) * (size_t)RR);
+  unsigned char* adm = (unsigned char*)malloc((size_t)RR);
+  if (!marker || !active || !scores || !tot || !run || !cz || !cd || !adm) return 2;
+  for (long long i = 0; i < pins; i++) { ep_cnt[i] = 0; ep_parts[i] = -1; }
+  for (int k = 0; k < K; k++) { marker[k] = -1; sizes[k] = 0; }
+  for (int v = 0; v < nov; v++) part[v] = -1;
+  for (int n = 0; n < noe; n++) ep_ends[n] = 0;
+  *subrounds = 0;
+  int rc = 0;
+  const long long band = mode == 0 ? max_pw : LLONG_MAX;
+  for (int r0 = 0; r0 < nov; r0 += R) {
+    const int ns = R < nov - r0 ? R : nov - r0;
+    for (;;) {
+      int nc = 0;
+      for (int z = r0; z < r0 + ns; z++) {
+        const int v = ordering[z];
+        if (part[v] >= 0) continue;
+        int apc = 0;
+        did++;
+        for (unsigned i = vtx_ptr[v]; i < vtx_ptr[v + 1]; i++) {
+          const int n = (int)vtx_nets[i];
+          const int s = (int)net_ptr[n];
+          if (mode == 2 && ep_ends[n] != 1) continue;
+          for (int j = 0; j < ep_ends[n]; j++) {
+            const int q = ep_parts[s + j];
+            /* the marker is the decision's number, unique as v is per
+             * vertex in the sequential stream */
+            if (marker[q] != did) { scores[q] = 1; marker[q] = did; active[apc++] = q; }
+            else scores[q]++;
+          }
+        }
+        /* LDG2's own scan: a positive score, or -1 (no part scores) */
+        double best = 0;
+        int d = -1;
+        for (int i = 0; i < apc; i++) {
+          const int q = active[i];
+          const double sc = scores[q] * (1 - (1.0 * sizes[q] / C));
+          if (sc > 0 && (d == -1 || sc > best || (sc == best && sizes[q] < sizes[d]))) { best = sc; d = q; }
+        }
+        if (d >= 0 && sizes[d] > band) d = -1;
+        cz[nc] = z;
+        cd[nc] = d;
+        nc++;
+      }
+      if (nc == 0) break;
+      /* fallback candidates (no positive score, or a choice above the band):
+       * the j-th of the sub-round, in ordering order, takes part (f + j) mod
+       * K, f the lowest-index part of minimum size -- with one candidate the
+       * reference's fallback (TV/partition.cpp:118-132) */
+      {
+        int nf = 0;
+        for (int i = 0; i < nc; i++) nf += cd[i] < 0;
+        if (nf > 0) {
+          int f0 = 0;
+          for (int k = 1; k < K; k++) if (sizes[k] < sizes[f0]) f0 = k;
+          int j = 0;
+          for (int i = 0; i < nc; i++)
+            if (cd[i] < 0) cd[i] = (f0 + (g_plant == 6 ? 0 : j++)) % K;
+        }
+      }
+      (*subrounds)++;
+      /* admission on the frozen sizes */
+      for (int k = 0; k < K; k++) { tot[k] = 0; run[k] = 0; }
+      for (int i = 0; i < nc; i++) tot[cd[i]] += VW(ordering[cz[i]]);
+      for (int i = 0; i < nc; i++) {
+        const int d = cd[i];
+        const long long w = VW(ordering[cz[i]]);
+        const long long before = run[d];
+        run[d] += w;
+        adm[i] = (unsigned char)((g_plant != 4 && before == 0) || tot[d] <= cap - sizes[d] ||
+                                 (g_plant == 5 ? sizes[d] + before + w < cap : sizes[d] + before + w <= cap));
+      }
+      int placed = 0;
+      for (int ii = 0; ii < nc; ii++) {
+        const int i = g_plant == 7 ? nc - 1 - ii : ii;
+        if (!adm[i]) continue;
+        placed++;
+        const int v = ordering[cz[i]], d = cd[i];
+        part[v] = d;
+        sizes[d] += VW(v);
+        for (unsigned j = vtx_ptr[v]; j < vtx_ptr[v + 1]; j++)
+          list_add((int)vtx_nets[j], d, net_ptr, ep_ends, ep_parts, ep_cnt);
+      }
+      if (placed == 0) { rc = 4; goto done; }   /* only a planted mistake stalls a round */
+    }
+  }
+done:
+  free(marker); free(active); free(scores); free(tot); free(run); free(cz); free(cd); free(adm);
+  return rc;
+}
+
 int or_tv_refine_serial(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr,
                         const unsigned* vtx_nets, const int* ordering, int K, double C, int max_pw, int min_pw,
                         int passes, int* part, int* ep_ends, int* ep_parts, int* ep_cnt, int* sizes, int* marker) {
@@ -382,8 +512,6 @@ int or_tv_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned* 
 /* Test hook: a nonzero code makes the balanced admission below deliberately
  * wrong, so that tests can show their checks catch such a mistake. 0 = the
  * rule as specified. */
-static int g_plant = 0;
-void or_plant(int code) { g_plant = code; }
 
 void or_apply_move(int v, int p, int q, const unsigned* net_ptr, const unsigned* vtx_ptr, const unsigned* vtx_nets,
                    int* ep_ends, int* ep_parts, int* ep_cnt) {

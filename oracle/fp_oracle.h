@@ -53,6 +53,17 @@ int or_tv_ldg(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr
               const int* ordering, int K, double C, int max_pw, int min_pw, int par_skip, int* part, int* ep_ends,
               int* ep_parts, int* ep_cnt, int* sizes, int* marker);
 
+/* Buffered round streaming initial partition (the engine's device stream):
+ * rounds of R ordering positions; sub-rounds decide the round's unplaced
+ * vertices on the frozen state (mode 0 LDG2 with the max band, 1 band-free
+ * LDG, 2 CN's LDG over uncut nets) and admit per part the longest prefix of
+ * candidates within cap (the first candidate into a part always). R = 1 is
+ * the sequential stream of or_tv_ldg / or_msv_ldg / or_cn_ldg. */
+int or_round_ldg(int mode, int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr,
+                 const unsigned* vtx_nets, const int* ordering, int K, double C, long long max_pw, int R,
+                 long long cap, int* part, int* ep_ends, int* ep_parts, int* ep_cnt, long long* sizes,
+                 long long* subrounds);
+
 /* Serial refinement, partition() (TV/partition.cpp:307-440), continuing with
  * the marker array left by the initial partition (stale-marker semantics). */
 int or_tv_refine_serial(int nov, int noe, const unsigned* net_ptr, const unsigned* vtx_ptr,
@@ -97,10 +108,13 @@ void or_apply_move(int v, int p, int q, const unsigned* net_ptr, const unsigned*
  * balanced schedule and or_build_state take their weights as an argument. */
 void or_set_vertex_weights(const int* vwgt);
 
-/* Test hook. A nonzero code plants a known mistake in or_tv_refine_bal's
- * admission (1: admit every candidate; 2: count only admitted weight in the
- * prefix; 3: a strict < cap test), so a test can show that its checks fail
- * on it. 0 restores the rule. */
+/* Test hook. A nonzero code plants a known mistake, so a test can show that
+ * its checks fail on it: in or_tv_refine_bal's admission 1 admits every
+ * candidate, 2 counts only admitted weight in the prefix, 3 tests < cap; in
+ * or_round_ldg 4 drops the first-candidate-free rule (a stalled round returns
+ * 4), 5 tests < cap, 6 gives every fallback candidate the minimum part, 7
+ * places a sub-round's admitted vertices in reverse order. 0 restores the
+ * rules. */
 void or_plant(int code);
 
 /* ---- CN ---------------------------------------------------------------- */

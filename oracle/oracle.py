@@ -112,6 +112,25 @@ def tv_ldg(hg, K, C, max_pw, min_pw, par_skip=False, ordering=None):
     return st, marker
 
 
+ROUND_LDG_MODES = {"ldg_bands": 0, "ldg": 1, "cn_ldg": 2}
+
+
+def round_ldg(hg, K, mode, C, max_pw, R, cap, ordering=None):
+    """Buffered round stream of fp_oracle.c's or_round_ldg (the engine's
+    device initial partition); returns (State with int64 sizes, sub-rounds)."""
+    _wts(hg)
+    st = State(hg, K, np.int64)
+    sub = ctypes.c_longlong()
+    rc = lib().or_round_ldg(ROUND_LDG_MODES[mode], hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),
+                            _p(hg.vtx_nets, _U32P), _p(_ord(hg, ordering), _I32P), K, ctypes.c_double(C),
+                            ctypes.c_longlong(max_pw), R, ctypes.c_longlong(cap), _p(st.part, _I32P),
+                            _p(st.ep_ends, _I32P), _p(st.ep_parts, _I32P), _p(st.ep_cnt, _I32P),
+                            _p(st.sizes, _I64P), ctypes.byref(sub))
+    if rc:
+        raise ValueError(f"or_round_ldg rc={rc}")
+    return st, sub.value
+
+
 def tv_refine_serial(hg, st, marker, C, max_pw, min_pw, passes, ordering=None):
     _wts(hg)
     rc = lib().or_tv_refine_serial(hg.nov, hg.noe, _p(hg.net_ptr, _U32P), _p(hg.vtx_ptr, _U32P),

@@ -504,3 +504,144 @@ def test_weighted_sizes_follow_the_moves(kind):
         want = np.bincount(s.part, weights=hg.vwgt, minlength=K).astype(np.int64)
         assert np.array_equal(s.sizes.astype(np.int64), want), kind
     assert np.count_nonzero(init.part != st.part) > 0, kind
+
+
+# ---- buffered round streaming initial partition (the device stream) --------
+
+_SEQ_STREAM = {"ldg_bands": lambda hg, K, C, mx, mn, o: O.tv_ldg(hg, K, C, mx, mn, ordering=o)[0],
+               "ldg": lambda hg, K, C, mx, mn, o: O.msv_ldg(hg, K, C, ordering=o),
+               "cn_ldg": lambda hg, K, C, mx, mn, o: O.cn_ldg(hg, K, C, ordering=o)}
+
+
+def _ldg_case(seed, n=1500, weights=None):
+    hg = H.colnet(n, H.DEG_UNIFORM, 2, 9, seed=seed, weights=weights)
+    K = 12
+    W = hg.total_weight()
+    C, max_pw, min_pw = H.ref_constants(W, K)
+    cap = H.balance_cap(W, K, 0.05)
+    ordering = np.random.default_rng(seed + 7).permutation(hg.nov).astype(np.int32)
+    return hg, K, C, max_pw, min_pw, cap, ordering
+
+
+@pytest.mark.parametrize("mode", ["ldg_bands", "ldg", "cn_ldg"])
+@pytest.mark.parametrize("seed", [1, 2])
+@pytest.mark.parametrize("permuted", [False, True])
+def test_round_ldg_one_vertex_rounds_are_the_reference_stream(mode, seed, permuted):
+    # rounds of one vertex are the sequential streams the reference runs
+    # (pinned by the goldens and live reference runs above)
+    hg, K, C, max_pw, min_pw, cap, ordering = _ldg_case(seed)
+    o = ordering if permuted else None
+    want = _SEQ_STREAM[mode](hg, K, C, max_pw, min_pw, o)
+    got, sub = O.round_ldg(hg, K, mode, C, max_pw, 1, cap, ordering=o)
+    assert sub == hg.nov
+    for k in ("part", "ep_ends", "ep_parts", "ep_cnt"):
+        assert np.array_equal(getattr(got, k), getattr(want, k)), k
+    assert np.array_equal(got.sizes, want.sizes.astype(np.int64))
+
+
+def _round_ldg_by_definition(hg, K, mode, C, max_pw, R, cap, ordering):
+    """The buffered round stream from its definition, in plain Python: the
+    reference's LDG scoring (TV/partition.cpp:89-145 and 214-233, CN's uncut
+    nets only for cn_ldg) of every unplaced vertex of the round on the frozen
+    state, the fallback turn, the per-part prefix admission, placement in
+    ordering order."""
+    w = hg.vwgt if hg.vwgt is not None else np.ones(hg.nov, np.int32)
+    nets = [hg.vtx_nets[hg.vtx_ptr[v]:hg.vtx_ptr[v + 1]].tolist() for v in range(hg.nov)]
+    part = [-1] * hg.nov
+    sizes = [0] * K
+    lists = [[] for _ in range(hg.noe)]   # [part, count] in list order
+    band = max_pw if mode == "ldg_bands" else float("inf")
+    subrounds = 0
+    for r0 in range(0, hg.nov, R):
+        slots = [int(v) for v in ordering[r0:r0 + R]]
+        while True:
+            cands = []
+            for v in slots:
+                if part[v] >= 0:
+                    continue
+                score, seen = {}, []
+                for n in nets[v]:
+                    if mode == "cn_ldg" and len(lists[n]) != 1:
+                        continue
+                    for q, _ in lists[n]:
+                        if q not in score:
+                            score[q] = 0
+                            seen.append(q)
+                        score[q] += 1
+                best, d = 0.0, -1
+                for q in seen:
+                    sc = score[q] * (1 - (1.0 * sizes[q] / C))
+                    if sc > 0 and (d == -1 or sc > best or (sc == best and sizes[q] < sizes[d])):
+                        best, d = sc, q
+                if d >= 0 and sizes[d] > band:
+                    d = -1
+                cands.append([v, d])
+            if not cands:
+                break
+            subrounds += 1
+            f0 = min(range(K), key=lambda k: (sizes[k], k))
+            j = 0
+            for c in cands:
+                if c[1] < 0:
+                    c[1] = (f0 + j) % K
+                    j += 1
+            tot = [0] * K
+            for v, d in cands:
+                tot[d] += int(w[v])
+            run = [0] * K
+            adm = []
+            for v, d in cands:
+                before = run[d]
+                run[d] += int(w[v])
+                adm.append(before == 0 or tot[d] <= cap - sizes[d] or sizes[d] + before + int(w[v]) <= cap)
+            for (v, d), a in zip(cands, adm):
+                if not a:
+                    continue
+                part[v] = d
+                sizes[d] += int(w[v])
+                for n in nets[v]:
+                    for e in lists[n]:
+                        if e[0] == d:
+                            e[1] += 1
+                            break
+                    else:
+                        lists[n].append([d, 1])
+    return part, sizes, lists, subrounds
+
+
+def _round_ldg_matches(mode, seed, R, weights):
+    hg, K, C, max_pw, min_pw, cap, ordering = _ldg_case(seed, n=400, weights=weights)
+    try:
+        got, sub = O.round_ldg(hg, K, mode, C, max_pw, R, cap, ordering=ordering)
+    except ValueError:
+        return False, 0
+    part, sizes, lists, want_sub = _round_ldg_by_definition(hg, K, mode, C, max_pw, R, cap, ordering)
+    same = got.part.tolist() == part and got.sizes.tolist() == sizes and sub == want_sub
+    for n in range(hg.noe):
+        s, L = int(hg.net_ptr[n]), int(got.ep_ends[n])
+        same = same and [[int(got.ep_parts[s + t]), int(got.ep_cnt[s + t])] for t in range(L)] == lists[n]
+    return same, want_sub
+
+
+@pytest.mark.parametrize("mode", ["ldg_bands", "ldg", "cn_ldg"])
+@pytest.mark.parametrize("R", [1, 9, 64, 400])
+@pytest.mark.parametrize("weights", [None, (1, 5)])
+def test_round_ldg_equals_its_definition(mode, R, weights):
+    assert _round_ldg_matches(mode, 3, R, weights)[0]
+
+
+def test_round_ldg_cases_reject_candidates():
+    # the cases above must exercise the admission's rejections: some round
+    # takes more than one sub-round
+    assert any(_round_ldg_matches(m, 3, R, w)[1] > (400 + R - 1) // R
+               for m in ("ldg_bands", "ldg", "cn_ldg") for R in (9, 64) for w in (None, (1, 5)))
+
+
+def test_round_ldg_check_catches_planted_mistakes():
+    cases = [(m, R, w) for m in ("ldg_bands", "cn_ldg") for R in (9, 64, 400) for w in (None, (1, 5))]
+    try:
+        for code in (4, 5, 6, 7):
+            O.plant(code)
+            assert any(not _round_ldg_matches(m, 3, R, w)[0] for m, R, w in cases), f"planted mistake {code} not caught"
+    finally:
+        O.plant(0)
