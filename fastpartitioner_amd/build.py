@@ -1,7 +1,7 @@
 """In-tree build of the package's native libraries.
 
   libfastpart_host.so  g++    csrc/fp_host.cpp         (host helpers)
-  libfastpart.so       hipcc  csrc/fp_{context,tv,cn,vol,conn,metrics,batch,serve}.hip  (HIP engine, gfx950)
+  libfastpart.so       hipcc  csrc/fp_{context,tv,cn,vol,conn,metrics,batch,serve,bfs}.hip  (HIP engine, gfx950)
   libfastpart_legacy_<m>.so  g++  csrc/fp_legacy.cpp  (the reference's partition_gpu_wrapper
                        for objective <m>, over libfastpart.so)
 
@@ -28,7 +28,7 @@ HIP_FLAGS = ["-O3", "-fPIC", "-shared", "-std=c++17", "-fopenmp", "-ffp-contract
 
 
 ENGINE_SOURCES = ["fp_context.hip", "fp_tv.hip", "fp_cn.hip", "fp_vol.hip", "fp_conn.hip", "fp_metrics.hip",
-                  "fp_batch.hip", "fp_serve.hip"]
+                  "fp_batch.hip", "fp_serve.hip", "fp_bfs.hip"]
 
 
 def _run(cmd):

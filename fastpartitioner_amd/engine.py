@@ -38,6 +38,13 @@ class InitStats(ctypes.Structure):
     _fields_ = [("rounds", ctypes.c_longlong), ("subrounds", ctypes.c_longlong), ("device_ms", ctypes.c_double)]
 
 
+class BfsStats(ctypes.Structure):
+    _fields_ = [("reached", ctypes.c_longlong), ("levels", ctypes.c_longlong), ("device_ms", ctypes.c_double)]
+
+
+BFS_PINS_ASCENDING, BFS_PINS_UPLOADED = 0, 1
+
+
 class Metrics(ctypes.Structure):
     _fields_ = [(k, ctypes.c_longlong) for k in ("tv", "cn", "tm", "msm", "msv", "msrv")]
 
@@ -62,6 +69,8 @@ def declare(lib) -> None:
     lib.fp_device_count.argtypes = [c.POINTER(c.c_int)]
     lib.fp_upload.argtypes = [P, c.c_int, c.c_int, _U32P, _U32P, _U32P, _U32P, _I32P, _I32P, _I32P]
     lib.fp_set_partition.argtypes = [P, c.c_int, _I32P]
+    lib.fp_bfs_ordering.argtypes = [P, c.c_int, c.c_int, _I32P, c.POINTER(c.c_int), c.POINTER(c.c_int),
+                                     c.POINTER(BfsStats)]
     lib.fp_initial_partition_device.argtypes = [P, c.c_int, c.c_int, c.c_int, c.c_double, c.c_longlong,
                                                  c.c_longlong, c.POINTER(InitStats)]
     lib.fp_set_state.argtypes = [P, c.c_int, _I32P, _I32P, _I32P, _I32P, _I64P]
@@ -168,6 +177,15 @@ class Engine:
         self.K = K
         part = np.ascontiguousarray(part, dtype=np.int32)
         self._check(self.lib.fp_set_partition(self.ctx, K, _p(part, _I32P)), "fp_set_partition")
+
+    def bfs_ordering(self, start: int, view: int = BFS_PINS_ASCENDING):
+        """The reference's BFS ordering on the device (fp_bfs_ordering):
+        (ordering, farthest_first, farthest_all, BfsStats)."""
+        order = np.zeros(self.hg.nov, dtype=np.int32)
+        f1, f2, st = ctypes.c_int(), ctypes.c_int(), BfsStats()
+        self._check(self.lib.fp_bfs_ordering(self.ctx, start, view, _p(order, _I32P), ctypes.byref(f1),
+                                             ctypes.byref(f2), ctypes.byref(st)), "fp_bfs_ordering")
+        return order, f1.value, f2.value, st
 
     def initial_partition(self, mode: int, K: int, round: int, C: float, max_pw: int, cap: int) -> InitStats:
         """The streaming initial partition on the device (fp_initial_partition_device):

@@ -152,6 +152,32 @@ typedef struct {
 fp_status fp_initial_partition_device(fp_ctx* ctx, int mode, int K, int round, double C, long long max_pw,
                                       long long cap, fp_init_stats* stats);
 
+/* BFS vertex ordering on the device: the reference's -o 1 ordering
+ * (shared/utils.cpp:433-590 as TV/tv.cpp:110-128 drives it: a BFS from
+ * start, one from the farthest vertex it found -- the lowest-index vertex of
+ * the largest distance -- and the visit order of a BFS from the farthest
+ * vertex of that one), equal to the reference's with one thread (its
+ * several-thread order depends on thread timing). The BFS goes vertex -> its
+ * nets -> each net's pins in the order of `view`:
+ *   FP_BFS_PINS_ASCENDING: ascending vertex order, the createInverse view
+ *     TV/MGV/TM expand (shared/utils.cpp:32-56); needs a context uploaded
+ *     with the identity ordering
+ *   FP_BFS_PINS_UPLOADED: the uploaded (reader) order, CN/cn.cpp:345-354's view
+ * Vertices the BFS does not reach, which the reference leaves at -1, follow
+ * in ascending order, so ordering[nov] is always a permutation to hand to
+ * fp_upload. The reference starts from rand() % nov (glibc: 1804289383 %
+ * nov). farthest_first / farthest_all / stats (nullable) receive the two
+ * farthest vertices, and the reached count, levels of the last BFS and
+ * device time. Needs max vertex degree * max net size within 32 bits. */
+enum { FP_BFS_PINS_ASCENDING = 0, FP_BFS_PINS_UPLOADED = 1 };
+typedef struct {
+  long long reached;
+  long long levels;
+  double device_ms;
+} fp_bfs_stats;
+fp_status fp_bfs_ordering(fp_ctx* ctx, int start, int view, int* ordering, int* farthest_first, int* farthest_all,
+                          fp_bfs_stats* stats);
+
 /* State in the reference's layout (TV/partition.cpp:150-158): ep_ends[noe]
  * (lambda), and per net n the slots net_ptr[n] .. net_ptr[n]+ep_ends[n]-1 of
  * ep_parts[pins]/ep_cnt[pins]. sizes (K, nullable) are the part sizes to

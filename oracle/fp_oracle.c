@@ -1508,3 +1508,67 @@ int or_msm_refine_par(int nov, int noe, const unsigned* net_ptr, const unsigned*
   conn_scratch_free(&w);
   return 0;
 }
+
+/* ---------------------------------------------------------------------- */
+/* BFS vertex ordering                                                     */
+/* ---------------------------------------------------------------------- */
+
+/* One BFS over the vertex -> net -> pin adjacency as topDown
+ * (shared/utils.cpp:543-590) runs it on one thread: level by level, each
+ * vertex of the level in queue order, its nets in vertex -> nets order, each
+ * net's pins in the order of the net view given, an unvisited pin joining the
+ * next level. queue[] receives the visit order; returns the vertices reached.
+ * A net is expanded the first time only: after that every pin of it is
+ * visited, so the reference's later expansions of it find nothing. */
+static int bfs_run(int start, int nov, int noe, const unsigned* vtx_ptr, const unsigned* vtx_nets,
+                   const unsigned* net_ptr, const unsigned* pins, int* dist, int* queue, unsigned char* done) {
+  for (int v = 0; v < nov; v++) dist[v] = -1;
+  for (int n = 0; n < noe; n++) done[n] = 0;
+  int head = 0, tail = 0;
+  dist[start] = 0;
+  queue[tail++] = start;
+  while (head < tail) {
+    const int v = queue[head++];
+    for (unsigned j = vtx_ptr[v]; j < vtx_ptr[v + 1]; j++) {
+      const unsigned e = vtx_nets[j];
+      if (done[e]) continue;
+      done[e] = 1;
+      for (unsigned k = net_ptr[e]; k < net_ptr[e + 1]; k++) {
+        const int u = (int)pins[k];
+        if (dist[u] < 0) { dist[u] = dist[v] + 1; queue[tail++] = u; }
+      }
+    }
+  }
+  return tail;
+}
+
+/* the lowest-index vertex of the largest distance (shared/utils.cpp:466-473) */
+static int bfs_farthest(int nov, const int* dist) {
+  int far = -1, fd = -1;
+  for (int i = 0; i < nov; i++)
+    if (dist[i] > fd) { far = i; fd = dist[i]; }
+  return far;
+}
+
+int or_bfs_ordering(int nov, int noe, const unsigned* vtx_ptr, const unsigned* vtx_nets, const unsigned* net_ptr,
+                    const unsigned* pins, int start, int* ordering, int* farthest_first, int* farthest_all) {
+  if (nov <= 0 || start < 0 || start >= nov) return 1;
+  int* dist = (int*)malloc(sizeof(int) * (size_t)nov);
+  unsigned char* seen = (unsigned char*)calloc((size_t)nov, 1);
+  unsigned char* done = (unsigned char*)malloc((size_t)(noe > 0 ? noe : 1));
+  if (!dist || !seen || !done) return 2;
+  bfs_run(start, nov, noe, vtx_ptr, vtx_nets, net_ptr, pins, dist, ordering, done);
+  *farthest_first = bfs_farthest(nov, dist);
+  bfs_run(*farthest_first, nov, noe, vtx_ptr, vtx_nets, net_ptr, pins, dist, ordering, done);
+  *farthest_all = bfs_farthest(nov, dist);
+  int n = bfs_run(*farthest_all, nov, noe, vtx_ptr, vtx_nets, net_ptr, pins, dist, ordering, done);
+  /* vertices the BFS does not reach (the reference leaves them -1) follow
+   * in ascending order */
+  for (int i = 0; i < n; i++) seen[ordering[i]] = 1;
+  for (int v = 0; v < nov; v++)
+    if (!seen[v]) ordering[n++] = v;
+  free(dist);
+  free(seen);
+  free(done);
+  return 0;
+}

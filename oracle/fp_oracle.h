@@ -239,6 +239,17 @@ int or_msrv_refine_serial(int nov, int noe, const unsigned* net_ptr, const unsig
                           int passes, int* part, int* ep_ends, int* ep_parts, int* ep_cnt, int* sizes, int* sv,
                           int* rv);
 
+/* ---- BFS ordering --------------------------------------------------------- */
+/* The reference's BFS ordering on one thread (shared/utils.cpp:433-590,
+ * driven as TV/tv.cpp:110-128): BFS from start, from the farthest vertex it
+ * found (the lowest-index vertex of the largest distance), then the visit
+ * order of a BFS from the farthest vertex of that one. The BFS goes vertex ->
+ * its nets (vtx_nets order) -> each net's pins in the order of the view
+ * given (net_ptr / pins). Vertices it does not reach, which the reference
+ * leaves at -1, follow in ascending order. */
+int or_bfs_ordering(int nov, int noe, const unsigned* vtx_ptr, const unsigned* vtx_nets, const unsigned* net_ptr,
+                    const unsigned* pins, int start, int* ordering, int* farthest_first, int* farthest_all);
+
 #ifdef __cplusplus
 }
 #endif

@@ -459,3 +459,51 @@ def run_reference_reader(path: str, timeout: float = 300.0):
         off += 4 * pins
         res["vertex_appearence"] = np.frombuffer(raw, dtype="<u4", count=nov, offset=off).copy()
     return res
+
+
+def bfs_net_view(hg, view: str):
+    """The net -> pins view a BFS expands: "ascending" = each net's pins in
+    ascending vertex order (createInverse, shared/utils.cpp:32-56, which
+    TV/MGV/TM drive the BFS with), "file" = the reader's pin order (CN/cn.cpp
+    drives it with col_ptr/row_ind)."""
+    if view == "file":
+        return np.ascontiguousarray(hg.net_pins, dtype=np.uint32)
+    if view != "ascending":
+        raise ValueError(view)
+    out = np.array(hg.net_pins, dtype=np.uint32)
+    for n in range(hg.noe):
+        s, e = int(hg.net_ptr[n]), int(hg.net_ptr[n + 1])
+        out[s:e].sort()
+    return out
+
+
+def bfs_ordering(hg, start: int, view: str = "ascending"):
+    """or_bfs_ordering: (ordering, farthest_first, farthest_all)."""
+    pins = bfs_net_view(hg, view)
+    ordering = np.zeros(hg.nov, dtype=np.int32)
+    f1, f2 = ctypes.c_int(), ctypes.c_int()
+    rc = lib().or_bfs_ordering(hg.nov, hg.noe, _p(hg.vtx_ptr, _U32P), _p(hg.vtx_nets, _U32P), _p(hg.net_ptr, _U32P),
+                               _p(pins, _U32P), int(start), _p(ordering, _I32P), ctypes.byref(f1), ctypes.byref(f2))
+    if rc:
+        raise ValueError(f"or_bfs_ordering rc={rc}")
+    return ordering, f1.value, f2.value
+
+
+def run_reference_bfs(hg, start: int, view: str = "ascending", threads: int = 1, timeout: float = 300.0):
+    """The reference's BFS ordering (oracle/_ref/ref_bfs): (ordering with -1
+    for unreached vertices, farthest_first, farthest_all)."""
+    exe = os.path.join(REF_DIR, "ref_bfs")
+    pins = bfs_net_view(hg, view)
+    with tempfile.TemporaryDirectory() as d:
+        src, out = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(src, "wb") as f:
+            f.write(struct.pack("<IIq", hg.nov, hg.noe, hg.pins))
+            for a in (hg.vtx_ptr, hg.vtx_nets, hg.net_ptr, pins):
+                f.write(np.ascontiguousarray(a, dtype="<u4").tobytes())
+            f.write(struct.pack("<ii", int(start), int(threads)))
+        p = subprocess.run([exe, src, out], capture_output=True, text=True, timeout=timeout)
+        if p.returncode != 0:
+            raise RuntimeError(f"{exe} failed rc={p.returncode}: {p.stderr[-2000:]}")
+        raw = open(out, "rb").read()
+    f1, f2 = struct.unpack_from("<ii", raw, 0)
+    return np.frombuffer(raw, dtype="<i4", count=hg.nov, offset=8).copy(), f1, f2

@@ -5,6 +5,7 @@ instance (net->pins CSR and sources) and the states the reference reached.
 
 Run from the repository root where the reference tree is available:
     make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle cli && python tests/golden/make_golden.py bfs
 """
 from __future__ import annotations
 
@@ -85,5 +86,29 @@ def main(only=None):
         print(name, {k: printed[k] for k in PRINTED})
 
 
+def write_bfs():
+    """bfs.npz: the reference's BFS ordering (oracle/_ref/ref_bfs, built by
+    `make -C oracle cli`) on an instance whose nets list their pins in a
+    shuffled order, through both net views."""
+    base = H.colnet(4000, H.DEG_UNIFORM, 1, 3, seed=26)
+    rng = np.random.default_rng(26)
+    pins = np.array(base.net_pins, dtype=np.uint32)
+    for n in range(base.noe):
+        rng.shuffle(pins[int(base.net_ptr[n]):int(base.net_ptr[n + 1])])
+    hg = H.Hypergraph.from_nets(base.nov, base.net_ptr, pins, base.src)
+    start = 1804289383 % hg.nov
+    out = {}
+    for view, key in (("ascending", "asc"), ("file", "file")):
+        order, f1, f2 = O.run_reference_bfs(hg, start, view)
+        out[f"{key}_ordering"] = order
+        out[f"{key}_farthest"] = np.array([f1, f2], dtype=np.int32)
+    np.savez_compressed(os.path.join(OUT, "bfs.npz"), nov=hg.nov, net_ptr=hg.net_ptr, net_pins=hg.net_pins,
+                        src=hg.src, start=start, **out)
+    print("bfs", {k: v.tolist() for k, v in out.items() if k.endswith("farthest")})
+
+
 if __name__ == "__main__":
-    main(sys.argv[1:])
+    if sys.argv[1:] == ["bfs"]:
+        write_bfs()
+    else:
+        main(sys.argv[1:])

@@ -78,7 +78,10 @@ def _run_oracle(kind, hg, K, passes, threads, ordering=None):
     return init, st
 
 
-@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "*.npz"))), ids=os.path.basename)
+_REFINE_GOLDEN = sorted(p for p in glob.glob(os.path.join(GOLD, "*.npz")) if os.path.basename(p) != "bfs.npz")
+
+
+@pytest.mark.parametrize("path", _REFINE_GOLDEN, ids=os.path.basename)
 def test_golden_fixture(path):
     z = np.load(path)
     hg = _hg_from(z)
@@ -645,3 +648,81 @@ def test_round_ldg_check_catches_planted_mistakes():
             assert any(not _round_ldg_matches(m, 3, R, w)[0] for m, R, w in cases), f"planted mistake {code} not caught"
     finally:
         O.plant(0)
+
+
+# ---- BFS ordering ------------------------------------------------------------
+
+def _shuffled_pins(hg, seed):
+    """hg with each net's pins in a random order (the reader keeps file order)."""
+    rng = np.random.default_rng(seed)
+    pins = np.array(hg.net_pins, dtype=np.uint32)
+    for n in range(hg.noe):
+        s, e = int(hg.net_ptr[n]), int(hg.net_ptr[n + 1])
+        rng.shuffle(pins[s:e])
+    return H.Hypergraph.from_nets(hg.nov, hg.net_ptr, pins, hg.src)
+
+
+_BFS_CASES = {
+    "uniform": lambda: H.colnet(6000, H.DEG_UNIFORM, 1, 4, seed=21),
+    "shuffled": lambda: _shuffled_pins(H.colnet(6000, H.DEG_UNIFORM, 1, 5, seed=22), 3),
+    "zipf": lambda: H.colnet(50000, H.DEG_ZIPF, 2, 800, alpha=2.1, seed=23),
+    "sparse": lambda: H.colnet(4000, H.DEG_FIXED, 1, seed=24),   # long paths, many levels
+}
+
+
+@pytest.mark.parametrize("case", list(_BFS_CASES))
+@pytest.mark.parametrize("view", ["ascending", "file"])
+def test_bfs_ordering_matches_reference(case, view):
+    if not os.path.exists(os.path.join(O.REF_DIR, "ref_bfs")):
+        pytest.skip("oracle/_ref/ref_bfs not built (needs the reference sources at build time)")
+    hg = _BFS_CASES[case]()
+    start = 1804289383 % hg.nov   # glibc's first rand(), TV/tv.cpp:120
+    got, f1, f2 = O.bfs_ordering(hg, start, view)
+    want, g1, g2 = O.run_reference_bfs(hg, start, view)
+    assert (f1, f2) == (g1, g2)
+    n = int(np.count_nonzero(want >= 0))
+    assert np.array_equal(got[:n], want[:n])
+    assert np.all(want[n:] == -1)
+    # unreached vertices follow in ascending order; the whole is a permutation
+    assert np.array_equal(got[n:], np.sort(got[n:]))
+    assert np.array_equal(np.sort(got), np.arange(hg.nov))
+
+
+def test_bfs_ordering_golden():
+    z = np.load(os.path.join(GOLD, "bfs.npz"))
+    hg = H.Hypergraph.from_nets(int(z["nov"]), z["net_ptr"], z["net_pins"], z["src"])
+    for view, key in (("ascending", "asc"), ("file", "file")):
+        got, f1, f2 = O.bfs_ordering(hg, int(z["start"]), view)
+        n = int(np.count_nonzero(z[f"{key}_ordering"] >= 0))
+        assert (f1, f2) == tuple(int(x) for x in z[f"{key}_farthest"])
+        assert np.array_equal(got[:n], z[f"{key}_ordering"][:n])
+
+
+def test_bfs_ordering_is_a_bfs():
+    # what the mathematics fixes whatever the order: each vertex's level is
+    # one more than its parent's, levels never decrease along the ordering,
+    # and the start of each BFS is the farthest vertex of the one before
+    hg = H.colnet(3000, H.DEG_UNIFORM, 1, 3, seed=25)
+    order, f1, f2 = O.bfs_ordering(hg, 7, "ascending")
+    adj = [set() for _ in range(hg.nov)]
+    for n in range(hg.noe):
+        p = hg.net_pins[hg.net_ptr[n]:hg.net_ptr[n + 1]].tolist()
+        for u in p:
+            adj[u].update(p)
+    def levels(s):
+        d = {s: 0}
+        q = [s]
+        for v in q:
+            for u in sorted(adj[v]):
+                if u not in d:
+                    d[u] = d[v] + 1
+                    q.append(u)
+        return d
+    d0 = levels(7)
+    assert f1 == min(v for v in d0 if d0[v] == max(d0.values()))
+    d1 = levels(f1)
+    assert f2 == min(v for v in d1 if d1[v] == max(d1.values()))
+    d2 = levels(f2)
+    assert order[0] == f2
+    lv = [d2[v] for v in order[:len(d2)]]
+    assert lv == sorted(lv)
