@@ -432,7 +432,10 @@ def tv1m_instance(args):
     W = hg.total_weight()
     C, max_pw, min_pw = H.ref_constants(W, K, eps)
     cap = H.balance_cap(W, K, eps)
+    t = time.perf_counter()
     part0 = H.initial_partition(hg, K, C, max_pw, min_pw)
+    args.host_stream_s = time.perf_counter() - t
+    args.ldg = (C, max_pw)
     if args.sched == "bal":
         sched = schedule(SCHED_BAL, args.round, C, cap, 0, cap)
     else:
@@ -513,6 +516,17 @@ def run_tv1m(args):
         for _ in range(10):
             eng.metrics()
         te1 = time.perf_counter()
+        # both sides of the refinement on the device: the BFS ordering (from
+        # glibc's first rand() % nov, as TV/tv.cpp:120) and the streaming
+        # initial partition (LDG2, rounds of 16384) from which TV refines
+        t = time.perf_counter()
+        _, _, _, bst = eng.bfs_ordering(1804289383 % hg.nov)
+        t_bfs = time.perf_counter() - t
+        C, max_pw = args.ldg
+        t = time.perf_counter()
+        ist = eng.initial_partition(0, K, 16384, C, max_pw, cap)
+        t_init = time.perf_counter() - t
+        tv_dev_init = eng.metrics()["TV"]
     secs = t1 - t0
     value = args.steps * hg.pins / secs
 
@@ -583,6 +597,14 @@ def run_tv1m(args):
                                           "frac": round(cn_bytes / (cn_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
         "metrics_evaluation": {"ms": round((te1 - te0) * 1e3 / 10, 4),
                                "bytes": int(8 * hg.pins + 12 * hg.noe)},
+        "host_path_on_device": {
+            "bfs_ordering": {"device_ms": round(bst.device_ms, 3), "wall_ms": round(t_bfs * 1e3, 3),
+                             "levels": int(bst.levels), "reached": int(bst.reached)},
+            "initial_partition": {"stream": "LDG2 with bands", "round": 16384,
+                                  "device_ms": round(ist.device_ms, 3), "wall_ms": round(t_init * 1e3, 3),
+                                  "subrounds": int(ist.subrounds), "tv": int(tv_dev_init),
+                                  "tv_sequential_stream": int(m0["TV"]),
+                                  "host_sequential_stream_s": round(args.host_stream_s, 3)}},
     }
     if cpu and "value" in cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 2)
