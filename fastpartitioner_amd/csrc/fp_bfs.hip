@@ -56,10 +56,10 @@ __global__ void __launch_bounds__(BFS_BLOCK) k_bfs_claim(const int* __restrict__
       int e = 0;
       if (j < deg) {
         e = (int)vtx_nets[vb + j];
-        if (!net_done[e]) {
-          const unsigned long long key = ((unsigned long long)p << 32) | j;
-          first = atomicMin(&net_owner[e], key) == NO_KEY;
-        }
+        // a claim at or above the one already read cannot win: no atomic
+        // (a stale read only costs an atomic, as claims only decrease)
+        const unsigned long long key = ((unsigned long long)p << 32) | j;
+        if (!net_done[e] && key < net_owner[e]) first = atomicMin(&net_owner[e], key) == NO_KEY;
       }
       wave_append(first, e, level_nets, nlevel);
     }
