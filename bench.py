@@ -627,7 +627,8 @@ def run_c3(args):
     the diagonal plus 12 uniform off-diagonals (26M pins), vertex weights
     U{1..8}, K = 1024, eps = 0.05, 15 passes of each of TM, MSM, MSV, MSRV from
     its own initial stream, reference round schedules (TM one round per pass,
-    the others rounds of 16384). One line per run with a record per objective;
+    the others rounds of --c3-round vertices). One line per run with a record
+    per objective;
     `value` = pin-visits/s over the four objectives' passes together."""
     from fastpartitioner_amd import hypergraph as H
     from fastpartitioner_amd.engine import SCHED_REF, Engine, schedule
@@ -647,7 +648,7 @@ def run_c3(args):
         eng.upload(hg)
         for metric in C3_OBJECTIVES:
             part = parts[H.INIT_OF[metric]]
-            sched = schedule(SCHED_REF, hg.nov if metric == "TM" else 16384, C, max_pw, min_pw)
+            sched = schedule(SCHED_REF, hg.nov if metric == "TM" else args.c3_round, C, max_pw, min_pw)
             if args.warmup:
                 eng.set_partition(K, part)
                 eng.refine(metric, args.warmup, sched)
@@ -697,7 +698,7 @@ def run_c3(args):
         "dtype": "int32",
         "data": "synthetic",
         "config": {"workload": "colnet-uniform12-2M-K1024-w1to8", "nov": hg.nov, "pins": hg.pins, "K": K,
-                   "eps": eps, "passes": passes, "objectives": per},
+                   "eps": eps, "passes": passes, "round": args.c3_round, "objectives": per},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -768,6 +769,9 @@ def main():
     ap.add_argument("--passes", type=int, default=10, help="batch: passes per hypergraph")
     ap.add_argument("--tune-passes", type=int, default=3, help="batch: passes that double as autotuner trials")
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--c3-round", type=int, default=262144,
+                    help="c3: round size of the MSM, MSV and MSRV schedules (TM: one round per pass); "
+                         "16384 gave the same refined objectives at half the rate")
     ap.add_argument("--lanes", type=int, default=16, help="batch: streams the members are spread over")
     ap.add_argument("--hw-queues", type=int, default=4, help="batch/serve: hardware queues of the process "
                     "(4, the HIP runtime's default, measured fastest for the batch)")
