@@ -8,10 +8,12 @@
 //                128-byte read per net brings its lambda and first 63
 //                entries, 3 nets in flight; the further windows of long
 //                lists follow as one task list over all of the vertex's long
-//                nets, 2 windows in flight -- into LDS histograms
-//                (pin count + first-appearance rank per part), then picks the
-//                LDG2 decision by a lexicographic wave reduction and applies
-//                the TV gain test. Candidates stamp their nets into the
+//                nets, 2 windows in flight -- into LDS histograms of pin
+//                counts per part, then picks the LDG2 decision by a
+//                lexicographic wave reduction (a vertex whose best score and
+//                size two parts share is swept again with the
+//                first-appearance ranks that break the tie) and applies the
+//                TV gain test. Candidates stamp their nets into the
 //                round's touched-net list; BAL also accumulates candidate
 //                weights per part and per (part, 64-slot tile).
 //   k_bal_scan   BAL: parts whose candidate weight exceeds cap - size get a
@@ -84,6 +86,56 @@ constexpr int SLOT_FALLBACK = -2;
 
 __device__ __forceinline__ bool better(double sc, long long sz, unsigned fi, double bsc, long long bsz, unsigned bfi) {
   return sc > bsc || (sc == bsc && (sz < bsz || (sz == bsz && fi < bfi)));
+}
+
+// The LDG2 decision of one vertex with first-appearance ranks, for a vertex
+// whose best (score, size) two parts share: its lists are swept again into
+// pin counts and ranks (i << pb | t, i the net's index in the vertex's list,
+// t the part's position in the net's list: the order in which the reference's
+// scan meets the parts, TV/partition.cpp:318-338), and the lexicographic max
+// of (score, -size, -rank) over score > 0 picks the part. cnt and first are
+// left clear (0, NOFIRST).
+template <int MODE>
+__device__ __forceinline__ void tie_decision(const EvalArgs& a, unsigned vb, int deg, int own, unsigned* cnt,
+                                          unsigned* first, const double* s_pen, const unsigned* s_skey, int& d,
+                                          unsigned& craw) {
+  const int lane = lane_id(), K = a.K;
+  for (int c0 = 0; c0 < deg; c0 += WAVE) {
+    const int nn = min(WAVE, deg - c0);
+    const unsigned sg = lane < nn ? a.vtx_seg[vb + (unsigned)(c0 + lane)] : a.pad_seg;
+    wave_sweep_lists<1, 1>(a.ev, sg, nn, [&](int j, int L, int t, bool ok, unsigned x) {
+      if (ok && (MODE != 2 || L == 1)) {  // mode 2, CN's stream: uncut nets only
+        const int q = (int)(x & ~EV_SINGLE);
+        atomicAdd(&cnt[q], 1u);
+        atomicMin(&first[q], ((unsigned)(c0 + j) << a.pb) | (unsigned)t);
+      }
+    });
+  }
+  wave_lds_sync();
+  unsigned long long b1 = 0, b2 = ~0ull;
+  int bq = -1;
+  unsigned bc = 0;
+  for (int q = lane; q < K; q += WAVE) {
+    const unsigned c = cnt[q], f = first[q];
+    cnt[q] = 0;
+    first[q] = NOFIRST;
+    const int adj = (int)c - (q == own ? 1 : 0);
+    const double sc = (double)adj * s_pen[q];
+    const bool ok = c != 0 && sc > 0;
+    const unsigned long long k1 = ok ? (unsigned long long)__double_as_longlong(sc) : 0ull;
+    const unsigned long long k2 = ((unsigned long long)s_skey[q] << 32) | f;
+    const bool better = ok && (k1 > b1 || (k1 == b1 && k2 < b2));
+    b1 = better ? k1 : b1;
+    b2 = better ? k2 : b2;
+    bq = better ? q : bq;
+    bc = better ? c : bc;
+  }
+  const unsigned long long m1 = wave_max_u64(b1);
+  const unsigned long long m2 = wave_min_u64(b1 == m1 ? b2 : ~0ull);
+  const int wl = __ffsll((long long)__ballot(b1 == m1 && b2 == m2)) - 1;
+  d = __builtin_amdgcn_readlane(bq, wl);
+  craw = (unsigned)__builtin_amdgcn_readlane((int)bc, wl);
+  wave_lds_sync();
 }
 
 // The evaluation's modes: a refinement round (TV/partition.cpp:732-805), or a
@@ -236,7 +288,6 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
           const int q = (int)(x & ~EV_SINGLE);
           const int qs = ok ? q : spare;
           atomicAdd(&cnt[qs], 1u);
-          atomicMin(&first[qs], ((unsigned)(c0 + j) << a.pb) | (unsigned)t);
           // leave gain: nets where v is the only pin of its part (TV/partition.cpp:365-368)
           lgl += ok && q == own && (x & EV_SINGLE) ? 1 : 0;
         }
@@ -279,7 +330,6 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
             const int q = (int)(x & ~EV_SINGLE);
             const int qs = ok ? q : spare;
             atomicAdd(&cnt[qs], 1u);
-            atomicMin(&first[qs], ((unsigned)(c0 + tj[u]) << a.pb) | (unsigned)t);
             lgl += ok && q == own && (x & EV_SINGLE) ? 1 : 0;
           }
         }
@@ -294,33 +344,40 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
     // patterns order like the values: key1 = score bits (max), key2 =
     // size_key(size) << 32 | first rank (min among key1 ties).
     const unsigned cfb = cnt[fb];
-    unsigned long long b1 = 0, b2 = ~0ull;
+    // candidates: lexicographic max of (score, -size, -first rank) over score >
+    // 0 (TV/partition.cpp:98-116). The first rank decides only between parts
+    // of equal score and size: the sweep counts pins only, and a vertex whose
+    // best (score, size) is shared by two parts is decided again with ranks
+    // (tie_decision). Scores are positive doubles, so their bit patterns order
+    // like the values: key1 = score bits (max), key2 = size_key(size) (min
+    // among key1 ties); ltie: this lane met its best keys twice.
+    unsigned long long b1 = 0;
+    unsigned b2 = ~0u;
     int bq = -1;
     unsigned bc = 0;
+    bool ltie = false;
     // four 64-part chunks per step, without branches: their LDS reads issue
     // together; every slot is cleared for the next vertex
     for (int q0 = 0; q0 < K; q0 += 4 * WAVE) {
-      unsigned cv[4], fv[4];
+      unsigned cv[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const int q = min(q0 + u * WAVE + lane, K - 1);
         cv[u] = q0 + u * WAVE + lane < K ? cnt[q] : 0u;
-        fv[u] = first[q];
       }
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const int q = q0 + u * WAVE + lane;
         const int qc = min(q, K - 1);
-        if (q < K) {
-          cnt[q] = 0;
-          first[q] = NOFIRST;
-        }
+        if (q < K) cnt[q] = 0;
         const int adj = (int)cv[u] - (q == own ? 1 : 0);   // scores[part_id]-- (TV/partition.cpp:345)
         const double sc = (double)adj * s_pen[qc];
         const bool ok = cv[u] != 0 && sc > 0;
         const unsigned long long k1 = ok ? (unsigned long long)__double_as_longlong(sc) : 0ull;
-        const unsigned long long k2 = ((unsigned long long)s_skey[qc] << 32) | fv[u];
+        const unsigned k2 = s_skey[qc];
         const bool better = ok && (k1 > b1 || (k1 == b1 && k2 < b2));
+        const bool same = ok && k1 == b1 && k2 == b2;
+        ltie = better ? false : (ltie || same);
         b1 = better ? k1 : b1;
         b2 = better ? k2 : b2;
         bq = better ? q : bq;
@@ -335,12 +392,16 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
       // the second key is needed only when several lanes hold the best score
       unsigned long long cand = __ballot(b1 == m1);
       if (__popcll(cand) > 1) {
-        const unsigned long long m2 = wave_min_u64(b1 == m1 ? b2 : ~0ull);
+        const unsigned m2 = wave_min_u32(b1 == m1 ? b2 : ~0u);
         cand = __ballot(b1 == m1 && b2 == m2);
       }
       const int wl = __ffsll((long long)cand) - 1;
-      d = __builtin_amdgcn_readlane(bq, wl);
-      craw = (unsigned)__builtin_amdgcn_readlane((int)bc, wl);
+      if (__popcll(cand) > 1 || __builtin_amdgcn_readlane((int)ltie, wl)) {
+        tie_decision<MODE>(a, vb, deg, own, cnt, first, s_pen, s_skey, d, craw);
+      } else {
+        d = __builtin_amdgcn_readlane(bq, wl);
+        craw = (unsigned)__builtin_amdgcn_readlane((int)bc, wl);
+      }
     }
     // the bands, compared as size keys (the same order)
     bool accept;
