@@ -18,9 +18,10 @@
 //   k_cn_alloc / k_cn_scatter   per touched net, the op list (push indices)
 //   k_cn_bigsort  nets with more than CN_MAXOPS ops: their push indices in
 //                 order through a bitmap over the pass's pushes (one block)
-//   k_cn_apply    one wavefront per touched net: sorts its ops by push index
-//                 in LDS (or reads the big nets' ordered ops) and replays
-//                 remove/add on the list
+//   k_cn_apply_small / k_cn_apply   one wavefront per touched net: sorts
+//                 its ops by push index (up to 64 in registers, up to
+//                 CN_MAXOPS in LDS; a big net's ordered ops are read) and
+//                 replays remove/add on the list
 //   k_cn_commit   part vector and part sizes (CN/partition.cpp:786-789)
 #include "fp_internal.h"
 
@@ -170,8 +171,6 @@ struct CnPrepArgs {
   const int* cn_new;
   int* mv_from;
   int* mv_last;
-  unsigned tag;
-  unsigned* net_stamp;
   int* touched;
   int* tcount;
   int* opcnt;
@@ -209,7 +208,9 @@ __global__ void __launch_bounds__(256) k_cn_prep(CnPrepArgs a) {
   a.mv_last[m] = last ? 1 : 0;
   for (unsigned j = vb; j < ve; j++) {
     const unsigned t = a.vtx_nets[j];
-    const bool isnew = atomicExch(&a.net_stamp[t], a.tag) != a.tag;
+    // the net's first op of the pass puts it on the touched list (op counts
+    // are 0 between passes: the apply resets them)
+    const bool isnew = atomicAdd(&a.opcnt[t], 1) == 0;
     // the lanes still in this loop append their new nets with one atomic
     const unsigned long long b = __ballot(isnew);
     if (b) {
@@ -218,7 +219,6 @@ __global__ void __launch_bounds__(256) k_cn_prep(CnPrepArgs a) {
       base = __builtin_amdgcn_readfirstlane(base);
       if (isnew) a.touched[base + (int)__popcll(b & ((1ull << lane) - 1ull))] = (int)t;
     }
-    atomicAdd(&a.opcnt[t], 1);
   }
   ops += ve - vb;
   }
@@ -368,6 +368,69 @@ __device__ void wave_bitonic_sort(unsigned* keys, int P) {
   }
 }
 
+// ascending sort of one key per lane across the wavefront (bitonic network
+// over lane exchanges)
+__device__ __forceinline__ unsigned wave_sort_lanes(unsigned x) {
+  const int lane = lane_id();
+  for (int k = 2; k <= WAVE; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const unsigned y = (unsigned)__shfl_xor((int)x, j);
+      // the lower lane of a pair keeps the minimum in an ascending block
+      x = (((lane & j) == 0) == ((lane & k) == 0)) ? min(x, y) : max(x, y);
+    }
+  return x;
+}
+
+// The touched nets with at most 64 ops (most of them), one wavefront per net,
+// four per block: the ops are sorted in registers, so the LDS holds only the
+// list (and its index), and more waves fit a CU than in k_cn_apply, whose
+// sort buffer they do not need. In an overflowed pass (flags) every touched
+// net's op count is reset and nothing else is done.
+template <bool INDEXED>
+__global__ void __launch_bounds__(256) k_cn_apply_small(CnApplyArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
+  unsigned* list = reinterpret_cast<unsigned*>(lds) + (size_t)wave * (size_t)a.K;
+  unsigned short* at = reinterpret_cast<unsigned short*>(reinterpret_cast<unsigned*>(lds) + (size_t)nw * (size_t)a.K) +
+                       (size_t)wave * (size_t)a.K;
+  if (INDEXED) wave_index_init(at, a.K);
+  const int nt = *a.tcount;
+  const bool skip = (a.flags[0] | a.flags[1]) != 0;
+  for (int i = (int)blockIdx.x * nw + wave; i < nt; i += (int)gridDim.x * nw) {
+    const int t = a.touched[i];
+    if (skip) {
+      if (lane == 0) a.opcnt_reset[t] = 0;
+      continue;
+    }
+    const int c = a.opcnt[t];
+    if (c > WAVE) continue;  // k_cn_apply's
+    const long long off = a.opoff[t];
+    const unsigned m = wave_sort_lanes(lane < c ? (unsigned)a.ops[off + lane] : 0xFFFFFFFFu);
+    int from_l = 0, to_l = 0;
+    if (lane < c) {
+      from_l = a.mv_from[m];
+      to_l = a.cn_new[a.mv_net[m]];
+    }
+    const unsigned s0 = a.net_ptr[t], seg = s0 + (unsigned)t;
+    int L = wave_list_load(a.ent, seg, a.net_ptr[t + 1] - s0, list);
+    if (INDEXED) wave_index_set(list, L, at, a.pmask);
+    for (int l = 0; l < c; l++) {
+      const int p = __builtin_amdgcn_readlane(from_l, l), q = __builtin_amdgcn_readlane(to_l, l);
+      if (INDEXED) {
+        unsigned co;
+        bool found;
+        wave_ilist_move(list, at, L, p, q, a.pb, a.pmask, co, found);
+      } else {
+        wave_list_move(list, L, p, q, a.pb, a.pmask);
+      }
+    }
+    wave_list_store(a.ent, a.ev, seg, list, L, a.pb, a.pmask);
+    if (INDEXED) wave_index_clear(list, L, at, a.pmask);
+    if (lane == 0) a.opcnt_reset[t] = 0;
+    wave_lds_sync();
+  }
+}
+
 // INDEXED: a part -> position index of the list in LDS (2 bytes per part
 // after the keys), for K where it fits.
 template <bool INDEXED>
@@ -379,14 +442,11 @@ __global__ void __launch_bounds__(WAVE) k_cn_apply(CnApplyArgs a) {
   const int lane = lane_id();
   if (INDEXED) wave_index_init(at, a.K);
   const int nt = *a.tcount;
-  const bool skip = (a.flags[0] | a.flags[1]) != 0;
+  if ((a.flags[0] | a.flags[1]) != 0) return;  // k_cn_apply_small reset the op counts
   for (int i = (int)blockIdx.x; i < nt; i += (int)gridDim.x) {
     const int t = a.touched[i];
-    if (skip) {
-      if (lane == 0) a.opcnt_reset[t] = 0;
-      continue;
-    }
     const int c = a.opcnt[t];
+    if (c <= WAVE) continue;  // k_cn_apply_small's (reset to 0 there)
     const long long off = a.opoff[t];
     // up to CN_MAXOPS ops are sorted here; a big net's ops come in order
     // from k_cn_bigsort and are read straight from the op list
@@ -426,6 +486,20 @@ __global__ void __launch_bounds__(WAVE) k_cn_apply(CnApplyArgs a) {
     if (lane == 0) a.opcnt_reset[t] = 0;
     wave_lds_sync();
   }
+}
+
+// LDS of k_cn_apply_small: per wave a K-entry list and, where four waves'
+// fit, the part -> position index; fewer waves per block for large K
+bool cn_small_indexed(int K) {
+  return (size_t)4 * (size_t)K * (sizeof(unsigned) + sizeof(unsigned short)) <= (size_t)LDS_BUDGET;
+}
+int cn_small_waves(int K) {
+  if (cn_small_indexed(K)) return 4;
+  return (int)std::max<size_t>(1, std::min<size_t>(4, (size_t)LDS_BUDGET / ((size_t)K * sizeof(unsigned))));
+}
+size_t cn_small_lds(int K) {
+  return (size_t)cn_small_waves(K) * (size_t)K *
+         (sizeof(unsigned) + (cn_small_indexed(K) ? sizeof(unsigned short) : 0));
 }
 
 // the CN apply's LDS: list, sort keys, and the part -> position index where
@@ -509,7 +583,7 @@ fp_status pass_cn(fp_ctx* ctx, const RefinePlan&) {
   const size_t lds_apply = cn_apply_lds(ctx->K);
   auto cn_apply = cn_apply_indexed(ctx->K) ? k_cn_apply<true> : k_cn_apply<false>;
   const unsigned long long* nm = s.totals + 3;
-  // one round per pass: tag 1 after clearing the stamps
+  // one round per pass: the touched-net counters start at 0
   pass_begin(ctx, nullptr, 0, nullptr);
   fill_async<unsigned long long>(ctx, s.totals, 5, 0ull);
   CnEvalArgs ea{ctx->net_ptr, ctx->net_pins, ctx->vtx_ptr, ctx->vtx_seg, ctx->ent, ctx->ev, ctx->part,
@@ -523,7 +597,7 @@ fp_status pass_cn(fp_ctx* ctx, const RefinePlan&) {
       ctx->net_ptr, ctx->net_pins, ctx->part, s.old_part, s.pbase, noe, s.mv_v, s.mv_net);
   const int par = 1;
   CnPrepArgs pa{s.mv_v, s.mv_net, nm, ctx->vtx_ptr, ctx->vtx_nets, ctx->part, s.old_part, s.new_part, s.mv_from,
-                s.mv_last, 1u, ctx->net_stamp, ctx->touched, ctx->counters + par, s.opcnt, s.totals};
+                s.mv_last, ctx->touched, ctx->counters + par, s.opcnt, s.totals};
   const int gm = (int)std::min<long long>((std::max(ctx->pins, 1ll) + 255) / 256, 8192);
   k_cn_prep<<<gm, 256, 0, ctx->stream>>>(pa);
   k_cn_alloc<<<(noe + 255) / 256, 256, 0, ctx->stream>>>(ctx->touched, ctx->counters + par, s.opcnt, s.opoff,
@@ -535,6 +609,8 @@ fp_status pass_cn(fp_ctx* ctx, const RefinePlan&) {
                                                      s.bigbits, s.ops);
   CnApplyArgs aa{ctx->net_ptr, ctx->touched, ctx->counters + par, s.opcnt, s.opoff, s.ops, s.mv_from, s.mv_net,
                  s.new_part, ctx->ent, ctx->ev, ctx->K, ctx->pb, ctx->pmask, s.opcnt, ctx->flags};
+  auto cn_small = cn_small_indexed(ctx->K) ? k_cn_apply_small<true> : k_cn_apply_small<false>;
+  cn_small<<<4096, cn_small_waves(ctx->K) * WAVE, cn_small_lds(ctx->K), ctx->stream>>>(aa);
   cn_apply<<<8192, WAVE, lds_apply, ctx->stream>>>(aa);
   k_cn_commit<<<gm, 256, 0, ctx->stream>>>(s.mv_v, s.mv_net, s.mv_from, s.mv_last, nm, ctx->flags, s.new_part,
                                             ctx->vwgt, ctx->part, ctx->sizes, ctx->moves);
