@@ -11,7 +11,9 @@
 // the device, with the same result:
 //   k_bfs_claim   one wavefront per frontier vertex (position p): every net j
 //                 of it not expanded yet keeps the smallest (p, j) claim; the
-//                 first claim of a net puts it on the level's net list
+//                 first claim of a net puts it on the level's net list (for a
+//                 large frontier, k_bfs_claim_nets finds the same claims from
+//                 the nets' side, without atomics)
 //   k_bfs_expand  one wavefront per claimed net: every unvisited pin k keeps
 //                 the smallest key (p, j, k) -- its first discovery in the
 //                 sequential order; the first key of a pin puts it on the
@@ -63,6 +65,68 @@ __global__ void __launch_bounds__(BFS_BLOCK) k_bfs_claim(const int* __restrict__
       }
       wave_append(first, e, level_nets, nlevel);
     }
+  }
+}
+
+// The same claims from the nets' side, for a large frontier (claims from the
+// frontier would put thousands of atomics on each big net's owner word):
+// one wavefront per net not expanded yet reads its pins, and the pins in the
+// frontier (plev[v] == level, position pos[v]) give the net's smallest
+// (position, index of the net in the pin's net list) claim.
+__global__ void __launch_bounds__(BFS_BLOCK) k_bfs_claim_nets(const unsigned* __restrict__ net_ptr,
+                                                              const unsigned* __restrict__ pins, int noe,
+                                                              const unsigned* __restrict__ vtx_ptr,
+                                                              const unsigned* __restrict__ vtx_nets,
+                                                              const unsigned char* __restrict__ net_done,
+                                                              const int* __restrict__ plev,
+                                                              const int* __restrict__ pos, int level,
+                                                              unsigned long long* __restrict__ net_owner,
+                                                              int* __restrict__ level_nets, int* __restrict__ nlevel) {
+  const int lane = lane_id(), nw = (int)(blockDim.x / WAVE);
+  for (int e0 = ((int)blockIdx.x * nw + wave_id()) * WAVE; e0 < noe; e0 += (int)gridDim.x * nw * WAVE) {
+    // 64 nets per wave: their done flags together, then the open ones in turn
+    const int ee = e0 + lane;
+    unsigned long long open = __ballot(ee < noe && !net_done[ee]);
+    bool any = false;
+    while (open) {
+      const int e = e0 + __ffsll((long long)open) - 1;
+      open &= open - 1;
+      const unsigned s = net_ptr[e], len = net_ptr[e + 1] - s;
+      unsigned long long best = NO_KEY;
+      for (unsigned k0 = 0; k0 < len; k0 += WAVE) {
+        const unsigned k = k0 + (unsigned)lane;
+        if (k < len) {
+          const int v = (int)pins[s + k];
+          if (plev[v] == level) {
+            // the net's index in v's (ascending) net list
+            unsigned lo = vtx_ptr[v], hi = vtx_ptr[v + 1];
+            const unsigned base = lo;
+            while (lo < hi) {
+              const unsigned mid = (lo + hi) >> 1;
+              if (vtx_nets[mid] < (unsigned)e) lo = mid + 1;
+              else hi = mid;
+            }
+            best = min(best, ((unsigned long long)pos[v] << 32) | (lo - base));
+          }
+        }
+      }
+      best = wave_min_u64(best);
+      if (best != NO_KEY && lane == e - e0) {
+        net_owner[e] = best;
+        any = true;
+      }
+    }
+    wave_append(any, ee, level_nets, nlevel);
+  }
+}
+
+// positions of the frontier's vertices for k_bfs_claim_nets
+__global__ void __launch_bounds__(BFS_BLOCK) k_bfs_positions(const int* __restrict__ frontier, int nf, int level,
+                                                             int* __restrict__ plev, int* __restrict__ pos) {
+  for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < nf; i += (int)(gridDim.x * blockDim.x)) {
+    const int v = frontier[i];
+    plev[v] = level;
+    pos[v] = i;
   }
 }
 
@@ -132,6 +196,8 @@ int bits_of(unsigned long long x) {
 
 struct BfsScratch {
   int* dist = nullptr;
+  int* plev = nullptr;
+  int* pos = nullptr;
   unsigned long long* vkey = nullptr;
   unsigned long long* net_owner = nullptr;
   unsigned char* net_done = nullptr;
@@ -164,6 +230,7 @@ fp_status bfs_run(fp_ctx* ctx, BfsScratch& s, const unsigned* pins, int kb, int 
                   int* levels) {
   const int nov = ctx->nov, noe = ctx->noe;
   fill_async<int>(ctx, s.dist, (size_t)nov, -1);
+  fill_async<int>(ctx, s.plev, (size_t)nov, -1);
   fill_async<unsigned long long>(ctx, s.vkey, (size_t)nov, NO_KEY);
   fill_async<unsigned long long>(ctx, s.net_owner, (size_t)std::max(noe, 1), NO_KEY);
   fill_async<unsigned char>(ctx, s.net_done, (size_t)std::max(noe, 1), (unsigned char)0);
@@ -173,9 +240,19 @@ fp_status bfs_run(fp_ctx* ctx, BfsScratch& s, const unsigned* pins, int kb, int 
   int* frontier = s.order;
   for (;;) {
     fill_async<int>(ctx, s.counts, 2, 0);
-    const int cgrid = std::max(1, std::min((nf + 3) / 4, 8192));
-    k_bfs_claim<<<cgrid, BFS_BLOCK, 0, ctx->stream>>>(frontier, nf, ctx->vtx_ptr, ctx->vtx_nets, s.net_done,
-                                                      s.net_owner, s.level_nets, s.counts);
+    // a frontier whose incidences reach an eighth of the nets claims from
+    // the nets' side
+    if ((double)nf * (double)ctx->pins / (double)std::max(nov, 1) * 8.0 >= (double)noe) {
+      k_bfs_positions<<<std::max(1, std::min((nf + BFS_BLOCK - 1) / BFS_BLOCK, 4096)), BFS_BLOCK, 0, ctx->stream>>>(
+          frontier, nf, level, s.plev, s.pos);
+      k_bfs_claim_nets<<<std::max(1, std::min((noe + 255) / 256, 8192)), BFS_BLOCK, 0, ctx->stream>>>(
+          ctx->net_ptr, pins, noe, ctx->vtx_ptr, ctx->vtx_nets, s.net_done, s.plev, s.pos, level, s.net_owner,
+          s.level_nets, s.counts);
+    } else {
+      const int cgrid = std::max(1, std::min((nf + 3) / 4, 8192));
+      k_bfs_claim<<<cgrid, BFS_BLOCK, 0, ctx->stream>>>(frontier, nf, ctx->vtx_ptr, ctx->vtx_nets, s.net_done,
+                                                        s.net_owner, s.level_nets, s.counts);
+    }
     k_bfs_expand<<<8192, BFS_BLOCK, 0, ctx->stream>>>(s.level_nets, s.counts, ctx->net_ptr, pins, s.net_owner,
                                                       s.net_done, s.dist, kb, s.vkey, s.cand, s.counts + 1);
     FP_HIP_OK(hipGetLastError());
@@ -232,7 +309,7 @@ fp_status fp_bfs_ordering(fp_ctx* ctx, int start, int view, int* ordering, int* 
   FP_HIP_OK(hipSetDevice(ctx->device));
   BfsScratch s;
   const size_t nv = (size_t)nov, ne = (size_t)std::max(noe, 1);
-  if (!s.get(&s.dist, nv) || !s.get(&s.vkey, nv) || !s.get(&s.net_owner, ne) || !s.get(&s.net_done, ne) ||
+  if (!s.get(&s.dist, nv) || !s.get(&s.plev, nv) || !s.get(&s.pos, nv) || !s.get(&s.vkey, nv) || !s.get(&s.net_owner, ne) || !s.get(&s.net_done, ne) ||
       !s.get(&s.level_nets, ne) || !s.get(&s.cand, nv) || !s.get(&s.keys, nv) || !s.get(&s.keys_out, nv) ||
       !s.get(&s.vals, nv) || !s.get(&s.order, nv) || !s.get(&s.counts, 4))
     return fail(ctx, FP_ERR_OOM, "BFS scratch allocation failed");
