@@ -169,6 +169,8 @@ struct CnPrepArgs {
   const int* part;
   const int* cn_old;
   const int* cn_new;
+  const long long* pbase;  // first push of each accepted net
+  const int* npush;        // pushes of each accepted net
   int* mv_from;
   int* mv_last;
   int* touched;
@@ -209,8 +211,13 @@ __global__ void __launch_bounds__(256) k_cn_prep(CnPrepArgs a) {
   for (unsigned j = vb; j < ve; j++) {
     const unsigned t = a.vtx_nets[j];
     // the net's first op of the pass puts it on the touched list (op counts
-    // are 0 between passes: the apply resets them)
-    const bool isnew = atomicAdd(&a.opcnt[t], 1) == 0;
+    // are 0 between passes: the apply resets them). The pushing net's own ops
+    // are its pushes (its pins are distinct: refinement rejects repeated
+    // pins), added once by its first push instead of one atomic per push on
+    // one word
+    bool isnew = false;
+    if ((int)t != n) isnew = atomicAdd(&a.opcnt[t], 1) == 0;
+    else if (m == a.pbase[n]) isnew = atomicAdd(&a.opcnt[t], a.npush[n]) == 0;
     // the lanes still in this loop append their new nets with one atomic
     const unsigned long long b = __ballot(isnew);
     if (b) {
@@ -244,7 +251,8 @@ __global__ void k_cn_check(const unsigned long long* __restrict__ totals, long l
 // Each touched net's op range: the block's nets take consecutive ranges
 // (a block scan), the block's start comes from one atomic on the running total.
 __global__ void __launch_bounds__(256) k_cn_alloc(const int* __restrict__ touched, const int* __restrict__ tcount,
-                                                  const int* __restrict__ opcnt, long long* __restrict__ opoff,
+                                                  const int* __restrict__ opcnt, const int* __restrict__ cn_old,
+                                                  const int* __restrict__ npush, long long* __restrict__ opoff,
                                                   int* __restrict__ opfill, unsigned long long* __restrict__ totals,
                                                   int* __restrict__ big, long long big_cap) {
   __shared__ long long s_w[256 / WAVE];
@@ -262,7 +270,8 @@ __global__ void __launch_bounds__(256) k_cn_alloc(const int* __restrict__ touche
   __syncthreads();
   if (valid) {
     opoff[t] = s_base + ex;
-    opfill[t] = 0;
+    // an accepted net's own pushes take the first slots of its range, by push index
+    opfill[t] = cn_old[t] >= 0 ? npush[t] : 0;
     // too many ops to sort in LDS: the net goes on the big list (totals[4])
     if (c > CN_MAXOPS) {
       const unsigned long long k = atomicAdd(&totals[4], 1ull);
@@ -272,17 +281,19 @@ __global__ void __launch_bounds__(256) k_cn_alloc(const int* __restrict__ touche
   if (threadIdx.x == 0 && s_max) atomicMax(reinterpret_cast<unsigned long long*>(&totals[2]), (unsigned long long)s_max);
 }
 
-__global__ void k_cn_scatter(const int* __restrict__ mv_v, const unsigned long long* __restrict__ nmoves,
+__global__ void k_cn_scatter(const int* __restrict__ mv_v, const int* __restrict__ mv_net,
+                             const long long* __restrict__ pbase, const unsigned long long* __restrict__ nmoves,
                              const int* __restrict__ flags, const unsigned* __restrict__ vtx_ptr,
                              const unsigned* __restrict__ vtx_nets, const long long* __restrict__ opoff,
                              int* __restrict__ opfill, int* __restrict__ ops) {
   if (flags[0] | flags[1]) return;
   const long long nm = (long long)*nmoves;
   for (long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x; m < nm; m += (long long)gridDim.x * blockDim.x) {
-    const int v = mv_v[m];
+    const int v = mv_v[m], n = mv_net[m];
     for (unsigned j = vtx_ptr[v]; j < vtx_ptr[v + 1]; j++) {
       const unsigned t = vtx_nets[j];
-      ops[opoff[t] + atomicAdd(&opfill[t], 1)] = (int)m;
+      const long long slot = (int)t == n ? m - pbase[n] : (long long)atomicAdd(&opfill[t], 1);
+      ops[opoff[t] + slot] = (int)m;
     }
   }
 }
@@ -514,21 +525,45 @@ size_t cn_apply_lds(int K) {
          (cn_apply_indexed(K) ? (size_t)K * sizeof(unsigned short) : 0);
 }
 
-__global__ void k_cn_commit(const int* __restrict__ mv_v, const int* __restrict__ mv_net,
-                            const int* __restrict__ mv_from, const int* __restrict__ mv_last,
-                            const unsigned long long* __restrict__ nmoves, const int* __restrict__ flags,
-                            const int* __restrict__ cn_new, const int* __restrict__ vwgt, int* __restrict__ part,
-                            long long* __restrict__ sizes, unsigned long long* __restrict__ moves) {
+// AGG: the block sums its size changes per part in LDS and adds them to the
+// sizes with one atomic per part (a pass's pushes otherwise queue on the K
+// size words at a few L2 slices); launched with few blocks, each covering
+// many pushes.
+template <bool AGG>
+__global__ void __launch_bounds__(256) k_cn_commit(const int* __restrict__ mv_v, const int* __restrict__ mv_net,
+                                                   const int* __restrict__ mv_from, const int* __restrict__ mv_last,
+                                                   const unsigned long long* __restrict__ nmoves,
+                                                   const int* __restrict__ flags, const int* __restrict__ cn_new,
+                                                   const int* __restrict__ vwgt, int K, int* __restrict__ part,
+                                                   long long* __restrict__ sizes, unsigned long long* __restrict__ moves) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  unsigned long long* s_dsz = reinterpret_cast<unsigned long long*>(lds);
   if (flags[0] | flags[1]) return;
+  if (AGG) {
+    for (int q = (int)threadIdx.x; q < K; q += (int)blockDim.x) s_dsz[q] = 0;
+    __syncthreads();
+  }
   const long long nm = (long long)*nmoves;
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(moves, (unsigned long long)nm);
   for (long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x; m < nm; m += (long long)gridDim.x * blockDim.x) {
     const int v = mv_v[m];
     const int to = cn_new[mv_net[m]];
     const long long w = vwgt ? vwgt[v] : 1;
-    atomic_add_i64(&sizes[mv_from[m]], -w);
-    atomic_add_i64(&sizes[to], w);
+    if (AGG) {
+      atomicAdd(&s_dsz[mv_from[m]], (unsigned long long)-w);
+      atomicAdd(&s_dsz[to], (unsigned long long)w);
+    } else {
+      atomic_add_i64(&sizes[mv_from[m]], -w);
+      atomic_add_i64(&sizes[to], w);
+    }
     if (mv_last[m]) part[v] = to;
+  }
+  if (AGG) {
+    __syncthreads();
+    for (int q = (int)threadIdx.x; q < K; q += (int)blockDim.x) {
+      const long long x = (long long)s_dsz[q];
+      if (x) atomic_add_i64(&sizes[q], x);
+    }
   }
 }
 
@@ -596,15 +631,15 @@ fp_status pass_cn(fp_ctx* ctx, const RefinePlan&) {
   k_cn_pushes<<<std::max(1, std::min((waves_needed + 3) / 4, 8192)), 256, 0, ctx->stream>>>(
       ctx->net_ptr, ctx->net_pins, ctx->part, s.old_part, s.pbase, noe, s.mv_v, s.mv_net);
   const int par = 1;
-  CnPrepArgs pa{s.mv_v, s.mv_net, nm, ctx->vtx_ptr, ctx->vtx_nets, ctx->part, s.old_part, s.new_part, s.mv_from,
-                s.mv_last, ctx->touched, ctx->counters + par, s.opcnt, s.totals};
+  CnPrepArgs pa{s.mv_v, s.mv_net, nm, ctx->vtx_ptr, ctx->vtx_nets, ctx->part, s.old_part, s.new_part, s.pbase,
+                s.npush, s.mv_from, s.mv_last, ctx->touched, ctx->counters + par, s.opcnt, s.totals};
   const int gm = (int)std::min<long long>((std::max(ctx->pins, 1ll) + 255) / 256, 8192);
   k_cn_prep<<<gm, 256, 0, ctx->stream>>>(pa);
-  k_cn_alloc<<<(noe + 255) / 256, 256, 0, ctx->stream>>>(ctx->touched, ctx->counters + par, s.opcnt, s.opoff,
-                                                         s.opfill, s.totals, s.big, s.big_cap);
+  k_cn_alloc<<<(noe + 255) / 256, 256, 0, ctx->stream>>>(ctx->touched, ctx->counters + par, s.opcnt, s.old_part,
+                                                         s.npush, s.opoff, s.opfill, s.totals, s.big, s.big_cap);
   k_cn_check<<<1, 1, 0, ctx->stream>>>(s.totals, s.ops_cap, ctx->dup_pins ? 1 : 0, ctx->flags);
-  k_cn_scatter<<<gm, 256, 0, ctx->stream>>>(s.mv_v, nm, ctx->flags, ctx->vtx_ptr, ctx->vtx_nets, s.opoff, s.opfill,
-                                            s.ops);
+  k_cn_scatter<<<gm, 256, 0, ctx->stream>>>(s.mv_v, s.mv_net, s.pbase, nm, ctx->flags, ctx->vtx_ptr, ctx->vtx_nets,
+                                            s.opoff, s.opfill, s.ops);
   k_cn_bigsort<<<1, BIGSORT_BLOCK, 0, ctx->stream>>>(s.big, s.big_cap, s.totals, nm, ctx->flags, s.opcnt, s.opoff,
                                                      s.bigbits, s.ops);
   CnApplyArgs aa{ctx->net_ptr, ctx->touched, ctx->counters + par, s.opcnt, s.opoff, s.ops, s.mv_from, s.mv_net,
@@ -612,8 +647,13 @@ fp_status pass_cn(fp_ctx* ctx, const RefinePlan&) {
   auto cn_small = cn_small_indexed(ctx->K) ? k_cn_apply_small<true> : k_cn_apply_small<false>;
   cn_small<<<4096, cn_small_waves(ctx->K) * WAVE, cn_small_lds(ctx->K), ctx->stream>>>(aa);
   cn_apply<<<8192, WAVE, lds_apply, ctx->stream>>>(aa);
-  k_cn_commit<<<gm, 256, 0, ctx->stream>>>(s.mv_v, s.mv_net, s.mv_from, s.mv_last, nm, ctx->flags, s.new_part,
-                                            ctx->vwgt, ctx->part, ctx->sizes, ctx->moves);
+  if ((size_t)ctx->K * sizeof(long long) <= (size_t)LDS_BUDGET)
+    k_cn_commit<true><<<std::min(gm, 256), 256, (size_t)ctx->K * sizeof(long long), ctx->stream>>>(
+        s.mv_v, s.mv_net, s.mv_from, s.mv_last, nm, ctx->flags, s.new_part, ctx->vwgt, ctx->K, ctx->part, ctx->sizes,
+        ctx->moves);
+  else
+    k_cn_commit<false><<<gm, 256, 0, ctx->stream>>>(s.mv_v, s.mv_net, s.mv_from, s.mv_last, nm, ctx->flags,
+                                                    s.new_part, ctx->vwgt, ctx->K, ctx->part, ctx->sizes, ctx->moves);
   return FP_OK;
 }
 
