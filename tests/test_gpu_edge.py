@@ -124,3 +124,56 @@ def test_k_beyond_the_packing_is_rejected():
         eng.upload(hg)
         with pytest.raises(FastPartError):
             eng.set_partition(16385, np.arange(hg.nov, dtype=np.int32))
+
+
+@pytest.mark.parametrize("mode", ["ldg_bands", "ldg", "cn_ldg"])
+@pytest.mark.parametrize("K", [1, 2, 16])
+@pytest.mark.parametrize("R", [1, 64, 5000])
+def test_device_stream_ragged(mode, K, R):
+    # the device initial partition on the ragged hypergraph: isolated
+    # vertices (fallback turns), single-pin nets, one net over almost every
+    # vertex (its list grows to K entries), a round longer than the graph
+    hg = _ragged(seed=6, nov=1500, isolated=60)
+    W = hg.total_weight()
+    C, max_pw, _ = H.ref_constants(W, K)
+    cap = H.balance_cap(W, K, 0.05)
+    print(f"device stream ragged: mode={mode} K={K} R={R}", flush=True)
+    with Engine(0) as eng:
+        eng.upload(hg)
+        eng.initial_partition({"ldg_bands": 0, "ldg": 1, "cn_ldg": 2}[mode], K, R, C, max_pw, cap)
+        got = eng.get_state()
+    want, _ = O.round_ldg(hg, K, mode, C, max_pw, R, cap)
+    assert np.array_equal(got.part, want.part)
+    assert np.array_equal(got.ep_ends, want.ep_ends)
+    assert np.array_equal(got.ep_parts, want.ep_parts)
+    assert np.array_equal(got.ep_cnt, want.ep_cnt)
+    assert np.array_equal(got.sizes, want.sizes)
+
+
+@pytest.mark.parametrize("view", ["ascending", "file"])
+def test_bfs_ragged(view):
+    # isolated vertices are never reached and follow in ascending order; the
+    # big net is expanded once, by the first frontier vertex that holds it
+    from fastpartitioner_amd.engine import BFS_PINS_ASCENDING, BFS_PINS_UPLOADED
+    hg = _ragged(seed=7, nov=2000, isolated=80)
+    print(f"bfs ragged: view={view}", flush=True)
+    with Engine(0) as eng:
+        eng.upload(hg)
+        got, f1, f2, st = eng.bfs_ordering(3, BFS_PINS_ASCENDING if view == "ascending" else BFS_PINS_UPLOADED)
+    want, g1, g2 = O.bfs_ordering(hg, 3, view)
+    assert (f1, f2) == (g1, g2)
+    assert np.array_equal(got, want)
+    assert st.reached == hg.nov - 80
+    assert np.array_equal(got[-80:], np.arange(hg.nov - 80, hg.nov))
+
+
+def test_device_stream_rejects_bad_arguments():
+    hg = _ragged(seed=8, nov=500, isolated=10)
+    with Engine(0) as eng:
+        eng.upload(hg)
+        for mode, K, R, C in ((3, 4, 16, 100.0), (0, 0, 16, 100.0), (0, 4, 0, 100.0), (0, 4, 16, 0.0)):
+            with pytest.raises(FastPartError):
+                eng.initial_partition(mode, K, R, C, 1000, 1000)
+        # the context stays usable
+        eng.initial_partition(0, 4, 16, 130.0, 1000, 140)
+        assert eng.get_state().part.min() >= 0
