@@ -88,8 +88,8 @@ __global__ void __launch_bounds__(VOL_BLOCK) k_vol_eval(VolEvalArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int K = a.K;
   double* s_pen = reinterpret_cast<double*>(lds);
-  long long* s_size = reinterpret_cast<long long*>(s_pen + K);
-  unsigned* wbase = reinterpret_cast<unsigned*>(s_size + K);
+  unsigned* s_skey = reinterpret_cast<unsigned*>(s_pen + K);  // size_key of each frozen part size
+  unsigned* wbase = s_skey + K;
   __shared__ long long red_sz[VOL_BLOCK / WAVE];
   __shared__ int red_ix[VOL_BLOCK / WAVE];
   const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(VOL_BLOCK) k_vol_eval(VolEvalArgs a) {
   int bi = INT_MAX;
   for (int q = (int)threadIdx.x; q < K; q += (int)blockDim.x) {
     const long long sz = a.sizes[q];
-    s_size[q] = sz;
+    s_skey[q] = size_key(sz);
     s_pen[q] = 1.0 - ((double)sz / a.C);
     if (sz < bs) {
       bs = sz;
@@ -170,26 +170,36 @@ __global__ void __launch_bounds__(VOL_BLOCK) k_vol_eval(VolEvalArgs a) {
       });
     }
     wave_lds_sync();
-    // LDG: lexicographic max of (score, -size, -first rank) over score > 0
+    // LDG: lexicographic max of (score, -size, -first rank) over score > 0;
+    // four 64-part chunks per step, without branches: their LDS reads issue
+    // together; every slot is cleared for the next vertex
     unsigned long long b1 = 0, b2 = ~0ull;
     int bq = -1;
-    for (int q = lane; q < K; q += WAVE) {
-      const unsigned c = cnt[q];
-      if (c) {
-        const unsigned f = first[q];
-        cnt[q] = 0;
-        first[q] = NOFIRST;
-        const int adj = (int)c - (q == own ? 1 : 0);  // scores[part_id]-- (MGV/partition.cpp:792-796)
-        const double sc = (double)adj * s_pen[q];
-        if (sc > 0) {
-          const unsigned long long k1 = (unsigned long long)__double_as_longlong(sc);
-          const unsigned long long k2 = ((unsigned long long)size_key(s_size[q]) << 32) | f;
-          if (k1 > b1 || (k1 == b1 && k2 < b2)) {
-            b1 = k1;
-            b2 = k2;
-            bq = q;
-          }
+    for (int q0 = 0; q0 < K; q0 += 4 * WAVE) {
+      unsigned cv[4], fv[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int q = min(q0 + u * WAVE + lane, K - 1);
+        cv[u] = q0 + u * WAVE + lane < K ? cnt[q] : 0u;
+        fv[u] = first[q];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int q = q0 + u * WAVE + lane;
+        const int qc = min(q, K - 1);
+        if (q < K) {
+          cnt[q] = 0;
+          first[q] = NOFIRST;
         }
+        const int adj = (int)cv[u] - (q == own ? 1 : 0);  // scores[part_id]-- (MGV/partition.cpp:792-796)
+        const double sc = (double)adj * s_pen[qc];
+        const bool ok = cv[u] != 0 && sc > 0;
+        const unsigned long long k1 = ok ? (unsigned long long)__double_as_longlong(sc) : 0ull;
+        const unsigned long long k2 = ((unsigned long long)s_skey[qc] << 32) | fv[u];
+        const bool better = ok && (k1 > b1 || (k1 == b1 && k2 < b2));
+        b1 = better ? k1 : b1;
+        b2 = better ? k2 : b2;
+        bq = better ? q : bq;
       }
     }
     const unsigned long long m1 = wave_max_u64(b1);
@@ -746,7 +756,7 @@ __global__ void __launch_bounds__(WAVE) k_vol_scan(int* __restrict__ vol_g, int 
 
 int vol_eval_waves(int K) {
   for (int w = VOL_BLOCK / WAVE; w >= 1; w--)
-    if ((size_t)K * 16 + (size_t)w * 8 * (size_t)K <= (size_t)LDS_BUDGET) return w;
+    if ((size_t)K * 12 + (size_t)w * 8 * (size_t)K <= (size_t)LDS_BUDGET) return w;
   return 0;
 }
 
@@ -809,7 +819,7 @@ fp_status plan_vol(fp_ctx* ctx, RefinePlan& p) {
 fp_status pass_vol(fp_ctx* ctx, const RefinePlan& p) {
   const bool msrv = p.metric == FP_METRIC_MSRV;
   const int nov = ctx->nov, K = ctx->K, R = p.R, waves = p.eval_waves;
-  const size_t lds_e = (size_t)K * 16 + (size_t)waves * 8 * (size_t)K;
+  const size_t lds_e = (size_t)K * 12 + (size_t)waves * 8 * (size_t)K;
   // the replay's part -> position index where it fits the LDS budget
   const bool indexed = (size_t)(REPLAY_BLOCK / WAVE) * (size_t)K * (sizeof(unsigned) + sizeof(unsigned short)) <=
                        (size_t)LDS_BUDGET;
