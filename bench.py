@@ -92,11 +92,18 @@ def _pmc_values(csv_dir, counter, kernel):
 COUNTER_FACTOR = {"FETCH_SIZE": 2.0, "WRITE_SIZE": 1.0}
 
 
+# the LDS and atomic counters reported beside the traffic, one rocprofv3 pass:
+# LDS instructions, LDS atomic instructions, LDS bank-conflict cycles, and the
+# atomic requests that reach the L2 (global atomics)
+LDS_ATOMIC_COUNTERS = ("SQ_INSTS_LDS", "SQ_INSTS_LDS_ATOMIC", "SQ_LDS_BANK_CONFLICT", "TCC_ATOMIC_sum")
+
+
 def measure_traffic(args):
     """HBM bytes per k_tv_eval launch, measured in this run: rocprofv3 --pmc
     FETCH_SIZE, then WRITE_SIZE (KiB per dispatch) over a 2-pass probe of the
-    same workload (--traffic-probe), each scaled by COUNTER_FACTOR. Returns
-    (bytes, detail)."""
+    same workload (--traffic-probe), each scaled by COUNTER_FACTOR; then the
+    LDS and atomic counters per dispatch in a third pass. Returns (bytes,
+    detail)."""
     import glob
     import shutil
     import subprocess
@@ -122,9 +129,20 @@ def measure_traffic(args):
                     return None, {"error": f"no {counter} values for k_tv_eval"}
                 out[counter] = {"kib_per_dispatch": round(sum(kv) / len(kv), 1), "dispatches": len(kv),
                                 "factor": COUNTER_FACTOR[counter]}
+            odir = os.path.join(d, "lds_atomic")
+            cmd = [rocprof, "--pmc", *LDS_ATOMIC_COUNTERS, "-f", "csv", "-d", odir, "-o", "run", "--",
+                   sys.executable, "-u", os.path.abspath(__file__), "--traffic-probe", "--round", str(args.round),
+                   "--sched", args.sched]
+            p = subprocess.run(cmd, capture_output=True, text=True, timeout=240, cwd="/tmp", env=env)
+            la = {}
+            for counter in LDS_ATOMIC_COUNTERS:
+                kv = _pmc_values(odir, counter, "k_tv_eval")
+                if kv:
+                    la[counter] = round(sum(kv) / len(kv), 1)
+            traffic = sum(v["kib_per_dispatch"] * 1024.0 * v["factor"] for v in out.values())
+            out["lds_atomic_per_dispatch"] = la if la else {"error": f"rocprofv3 rc={p.returncode}"}
     except Exception as e:  # reported, never fatal for the GPU number
         return None, {"error": str(e)[:300]}
-    traffic = sum(v["kib_per_dispatch"] * 1024.0 * v["factor"] for v in out.values())
     return int(traffic), out
 
 
