@@ -47,6 +47,7 @@ constexpr int SCAN_RUN = 16;     // consecutive tiles per thread in the BAL scan
 constexpr int APPLY_BLOCK = 256;
 constexpr int SMALL_NET = 8;     // nets of at most this many pins are replayed one per lane
 constexpr int APPLY_NET_BLOCKS = 2048;
+constexpr int APPLY_GROUP = 4;   // 64-rank windows of a net whose loads the apply issues together
 
 struct EvalArgs {
   const unsigned* vtx_ptr;
@@ -758,26 +759,41 @@ __global__ void __launch_bounds__(APPLY_BLOCK) k_apply(ApplyArgs a) {
     int L = wave_list_stage(a.ent, seg, h.x, list);
     if (INDEXED) wave_index_set(list, L, at, a.pmask);
     bool dirty = false;
-    for (unsigned k0 = lo; k0 < hi; k0 += WAVE) {
-      const unsigned k = k0 + (unsigned)lane;
-      int dd = -1, oo = -1;
-      const int r = small ? r_small : (k < hi ? a.apply_pins[k] : INT_MAX);
-      if (r >= a.r0 && r < a.r1) {
-        dd = a.slot_dec[r - a.r0];
-        oo = a.slot_own[r - a.r0];
+    // the round's pins of the net, APPLY_GROUP windows of 64 ranks at a time:
+    // their ranks load together, then their slots' decisions, so that a big
+    // net (a Zipf net of thousands of pins, a quarter of them in the round)
+    // takes a few dependent load rounds instead of one per window
+    for (unsigned k0 = lo; k0 < hi; k0 += APPLY_GROUP * WAVE) {
+      int rr[APPLY_GROUP], dd[APPLY_GROUP], oo[APPLY_GROUP];
+#pragma unroll
+      for (int u = 0; u < APPLY_GROUP; u++) {
+        const unsigned k = k0 + (unsigned)(u * WAVE + lane);
+        rr[u] = small ? (u == 0 ? r_small : INT_MAX) : (k < hi ? a.apply_pins[k] : INT_MAX);
       }
-      unsigned long long m = __ballot(dd >= 0);
-      while (m) {
-        dirty = true;
-        const int l = __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1);
-        m &= m - 1;
-        const int o = __builtin_amdgcn_readlane(oo, l), d = __builtin_amdgcn_readlane(dd, l);
-        if (INDEXED) {
-          unsigned co;
-          bool found;
-          wave_ilist_move(list, at, L, o, d, a.pb, a.pmask, co, found);
-        } else {
-          wave_list_move(list, L, o, d, a.pb, a.pmask);
+#pragma unroll
+      for (int u = 0; u < APPLY_GROUP; u++) {
+        dd[u] = -1;
+        oo[u] = -1;
+        if (rr[u] >= a.r0 && rr[u] < a.r1) {
+          dd[u] = a.slot_dec[rr[u] - a.r0];
+          oo[u] = a.slot_own[rr[u] - a.r0];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < APPLY_GROUP; u++) {
+        unsigned long long m = __ballot(dd[u] >= 0);
+        while (m) {
+          dirty = true;
+          const int l = __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1);
+          m &= m - 1;
+          const int o = __builtin_amdgcn_readlane(oo[u], l), d = __builtin_amdgcn_readlane(dd[u], l);
+          if (INDEXED) {
+            unsigned co;
+            bool found;
+            wave_ilist_move(list, at, L, o, d, a.pb, a.pmask, co, found);
+          } else {
+            wave_list_move(list, L, o, d, a.pb, a.pmask);
+          }
         }
       }
     }
