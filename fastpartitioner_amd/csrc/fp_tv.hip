@@ -46,7 +46,7 @@ constexpr int LONG_TASKS = FP_LONG_TASKS;  // windows of long lists (any of the 
 constexpr int SCAN_RUN = 16;     // consecutive tiles per thread in the BAL scan
 constexpr int APPLY_BLOCK = 256;
 constexpr int SMALL_NET = 8;     // nets of at most this many pins are replayed one per lane
-constexpr int APPLY_NET_BLOCKS = 2048;
+constexpr int APPLY_NET_BLOCKS = 8192;
 constexpr int APPLY_GROUP = 4;   // 64-rank windows of a net whose loads the apply issues together
 
 struct EvalArgs {
