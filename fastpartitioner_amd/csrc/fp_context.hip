@@ -135,6 +135,17 @@ namespace {
 
 constexpr int BUILD_KMAX = 4096;  // k_build_lists keeps four K-int arrays per wavefront in LDS
 
+// Per incidence of the vertex view: the segment of its net in the packed
+// connectivity layout (the net's list starts at net_ptr[n] + n).
+__global__ void __launch_bounds__(256) k_vtx_seg(const unsigned* __restrict__ net_ptr,
+                                                 const unsigned* __restrict__ vtx_nets, size_t pins,
+                                                 unsigned* __restrict__ seg) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < pins; k += (size_t)gridDim.x * blockDim.x) {
+    const unsigned n = vtx_nets[k];
+    seg[k] = net_ptr[n] + n;
+  }
+}
+
 // One wavefront per net: the net's connectivity list from the part vector,
 // in first-appearance order along the ordering (the state the reference's
 // streaming initial partition leaves, TV/partition.cpp:240-263). The pins are
@@ -330,8 +341,14 @@ fp_status fp_upload(fp_ctx* ctx, int nov, int noe, const unsigned* net_ptr, cons
     if (net_ptr[n + 1] < net_ptr[n]) return fail(ctx, FP_ERR_ARG, "fp_upload: net_ptr not monotone at %d", n);
     max_net = std::max(max_net, net_ptr[n + 1] - net_ptr[n]);
   }
-  for (long long k = 0; k < pins; k++)
-    if (net_pins[k] >= (unsigned)nov) return fail(ctx, FP_ERR_ARG, "fp_upload: pin %lld out of range", k);
+  {
+    // the first pin out of range, on every host thread
+    long long bad = LLONG_MAX;
+#pragma omp parallel for schedule(static) reduction(min : bad)
+    for (long long k = 0; k < pins; k++)
+      if (net_pins[k] >= (unsigned)nov) bad = std::min(bad, k);
+    if (bad != LLONG_MAX) return fail(ctx, FP_ERR_ARG, "fp_upload: pin %lld out of range", bad);
+  }
   if (src)
     for (int n = 0; n < noe; n++)
       if (src[n] < -1 || src[n] >= nov) return fail(ctx, FP_ERR_ARG, "fp_upload: src[%d] out of range", n);
@@ -369,34 +386,40 @@ fp_status fp_upload(fp_ctx* ctx, int nov, int noe, const unsigned* net_ptr, cons
   ctx->total_weight = tw;
   ctx->max_net = max_net;
   ctx->h_net_ptr.assign(net_ptr, net_ptr + noe + 1);
-  // vertex->nets view
-  std::vector<unsigned> h_vptr, h_vnets;
+  // vertex->nets view: the caller's, or a transposition on the host
+  std::vector<unsigned> t_vptr, t_vnets;
+  const unsigned* h_vptr = vtx_ptr;
+  const unsigned* h_vnets = vtx_nets;
   if (vtx_ptr && vtx_nets) {
     if (vtx_ptr[0] != 0 || (long long)vtx_ptr[nov] != pins) return fail(ctx, FP_ERR_ARG, "fp_upload: vtx_ptr inconsistent");
     for (int v = 0; v < nov; v++)
       if (vtx_ptr[v + 1] < vtx_ptr[v]) return fail(ctx, FP_ERR_ARG, "fp_upload: vtx_ptr not monotone");
-    for (long long k = 0; k < pins; k++)
-      if (vtx_nets[k] >= (unsigned)noe) return fail(ctx, FP_ERR_ARG, "fp_upload: vtx_nets out of range");
-    h_vptr.assign(vtx_ptr, vtx_ptr + nov + 1);
-    h_vnets.assign(vtx_nets, vtx_nets + pins);
+    bool bad = false;
+#pragma omp parallel for schedule(static) reduction(|| : bad)
+    for (long long k = 0; k < pins; k++) bad = bad || vtx_nets[k] >= (unsigned)noe;
+    if (bad) return fail(ctx, FP_ERR_ARG, "fp_upload: vtx_nets out of range");
   } else {
     // counting-sort transpose: ascending nets per vertex (TV/tv.cpp:79-100)
-    h_vptr.assign((size_t)nov + 1, 0);
-    h_vnets.assign((size_t)pins, 0);
-    for (long long k = 0; k < pins; k++) h_vptr[net_pins[k] + 1]++;
-    for (int v = 0; v < nov; v++) h_vptr[(size_t)v + 1] += h_vptr[(size_t)v];
-    std::vector<unsigned> fill(h_vptr.begin(), h_vptr.end() - 1);
+    t_vptr.assign((size_t)nov + 1, 0);
+    t_vnets.assign((size_t)pins, 0);
+    for (long long k = 0; k < pins; k++) t_vptr[net_pins[k] + 1]++;
+    for (int v = 0; v < nov; v++) t_vptr[(size_t)v + 1] += t_vptr[(size_t)v];
+    std::vector<unsigned> fill(t_vptr.begin(), t_vptr.end() - 1);
     for (int n = 0; n < noe; n++)
-      for (unsigned k = net_ptr[n]; k < net_ptr[n + 1]; k++) h_vnets[fill[net_pins[k]]++] = (unsigned)n;
+      for (unsigned k = net_ptr[n]; k < net_ptr[n + 1]; k++) t_vnets[fill[net_pins[k]]++] = (unsigned)n;
+    h_vptr = t_vptr.data();
+    h_vnets = t_vnets.data();
   }
   unsigned max_vdeg = 0;
-  for (int v = 0; v < nov; v++) max_vdeg = std::max(max_vdeg, h_vptr[(size_t)v + 1] - h_vptr[(size_t)v]);
-  ctx->max_vdeg = max_vdeg;
-  ctx->sum_deg2 = 0;
+  unsigned long long sum_deg2 = 0;
+#pragma omp parallel for schedule(static) reduction(max : max_vdeg) reduction(+ : sum_deg2)
   for (int v = 0; v < nov; v++) {
-    const unsigned long long d = h_vptr[(size_t)v + 1] - h_vptr[(size_t)v];
-    ctx->sum_deg2 += d * d;
+    const unsigned d = h_vptr[v + 1] - h_vptr[v];
+    max_vdeg = std::max(max_vdeg, d);
+    sum_deg2 += (unsigned long long)d * d;
   }
+  ctx->max_vdeg = max_vdeg;
+  ctx->sum_deg2 = sum_deg2;
   // per net: ordering ranks of its pins, ascending (the apply replays moves in this order)
   ctx->h_rank_pins.resize((size_t)pins);
   bool dup = false;
@@ -438,8 +461,8 @@ fp_status fp_upload(fp_ctx* ctx, int nov, int noe, const unsigned* net_ptr, cons
   FP_TRY(dalloc(ctx, &ctx->src, (size_t)std::max(noe, 1)));
   FP_TRY(h2d(ctx, ctx->net_ptr, net_ptr, (size_t)noe + 1));
   FP_TRY(h2d(ctx, ctx->net_pins, net_pins, (size_t)pins));
-  FP_TRY(h2d(ctx, ctx->vtx_ptr, h_vptr.data(), (size_t)nov + 1));
-  FP_TRY(h2d(ctx, ctx->vtx_nets, h_vnets.data(), (size_t)pins));
+  FP_TRY(h2d(ctx, ctx->vtx_ptr, h_vptr, (size_t)nov + 1));
+  FP_TRY(h2d(ctx, ctx->vtx_nets, h_vnets, (size_t)pins));
   FP_TRY(h2d(ctx, ctx->apply_pins, ctx->h_rank_pins.data(), (size_t)pins));
   if (src) {
     FP_TRY(h2d(ctx, ctx->src, src, (size_t)noe));
@@ -462,14 +485,10 @@ fp_status fp_upload(fp_ctx* ctx, int nov, int noe, const unsigned* net_ptr, cons
   FP_TRY(dalloc(ctx, &ctx->ev, (size_t)pins + (size_t)noe + ENT_PAD));
   fill_async<unsigned>(ctx, ctx->ent, (size_t)pins + (size_t)noe + ENT_PAD, 0u);
   fill_async<unsigned short>(ctx, ctx->ev, (size_t)pins + (size_t)noe + ENT_PAD, (unsigned short)0);
-  {
-    // per incidence: the segment of its net
-    std::vector<unsigned> seg((size_t)pins);
-    for (long long k = 0; k < pins; k++) seg[(size_t)k] = net_ptr[h_vnets[(size_t)k]] + h_vnets[(size_t)k];
-    FP_TRY(dalloc(ctx, &ctx->vtx_seg, (size_t)pins));
-    FP_TRY(h2d(ctx, ctx->vtx_seg, seg.data(), (size_t)pins));
-    FP_HIP_OK(hipStreamSynchronize(ctx->stream));
-  }
+  FP_TRY(dalloc(ctx, &ctx->vtx_seg, (size_t)pins));
+  if (pins > 0)
+    k_vtx_seg<<<(int)std::min<long long>((pins + 255) / 256, 8192), 256, 0, ctx->stream>>>(
+        ctx->net_ptr, ctx->vtx_nets, (size_t)pins, ctx->vtx_seg);
   FP_TRY(dalloc(ctx, &ctx->net_stamp, (size_t)std::max(noe, 1)));
   FP_TRY(dalloc(ctx, &ctx->touched, (size_t)std::max(noe, 1)));
   FP_TRY(dalloc(ctx, &ctx->counters, 4));
