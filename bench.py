@@ -485,7 +485,7 @@ def cn_pass_bytes(hg, part, lam):
 
 
 def run_tv1m(args):
-    from fastpartitioner_amd.engine import Engine
+    from fastpartitioner_amd.engine import Engine, pinned_copy
 
     t = time.time()
     hg, K, eps, cap, part0, sched = tv1m_instance(args)
@@ -551,6 +551,18 @@ def run_tv1m(args):
         ist = eng.initial_partition(0, K, 16384, C, max_pw, cap)
         t_init = time.perf_counter() - t
         tv_dev_init = eng.metrics()["TV"]
+    # the upload again into fresh contexts (the process's first-use costs
+    # paid): from pageable arrays through the staging buffers, and from a
+    # reader that filled both CSR views in pinned memory (direct DMA)
+    hp = pinned_copy(hg)
+    t_up2 = {}
+    for name, h in (("pageable", hg), ("pinned", hp)):
+        with Engine(0) as eng2:
+            t = time.perf_counter()
+            eng2.upload(h)
+            eng2.set_partition(K, part0)
+            t_up2[name] = time.perf_counter() - t
+    del hp
     secs = t1 - t0
     value = args.steps * hg.pins / secs
 
@@ -622,7 +634,11 @@ def run_tv1m(args):
         "metrics_evaluation": {"ms": round((te1 - te0) * 1e3 / 10, 4),
                                "bytes": int(8 * hg.pins + 12 * hg.noe)},
         "pcie_inclusive": {"upload_s": round(t_up, 4), "state_download_s": round(t_down, 4),
-                           "pin_visits_per_s": round(args.steps * hg.pins / (secs + t_up + t_down), 1)},
+                           "pin_visits_per_s": round(args.steps * hg.pins / (secs + t_up + t_down), 1),
+                           "upload_again_pageable_s": round(t_up2["pageable"], 4),
+                           "upload_again_pinned_s": round(t_up2["pinned"], 4),
+                           "pin_visits_per_s_pinned": round(
+                               args.steps * hg.pins / (secs + t_up2["pinned"] + t_down), 1)},
         "host_path_on_device": {
             "bfs_ordering": {"device_ms": round(bst.device_ms, 3), "wall_ms": round(t_bfs * 1e3, 3),
                              "levels": int(bst.levels), "reached": int(bst.reached)},

@@ -102,6 +102,80 @@ int bits_for(int K) {
 
 }  // namespace
 
+namespace {
+
+constexpr size_t STAGE_BYTES = size_t{16} << 20;  // per staging buffer
+constexpr size_t STAGE_MIN = size_t{4} << 20;     // smaller copies go straight from pageable memory
+constexpr size_t STAGE_PIECE = size_t{1} << 20;   // host threads fill a staging buffer in pieces of this
+
+bool host_pinned(const void* h) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, h) != hipSuccess) {
+    (void)hipGetLastError();  // pageable memory the runtime does not know
+    return false;
+  }
+  return a.type == hipMemoryTypeHost;
+}
+
+void free_stage(fp_ctx* ctx) {
+  for (int b = 0; b < 2; b++) {
+    if (ctx->stage_ev[b]) (void)hipEventDestroy(ctx->stage_ev[b]);
+    if (ctx->stage[b]) (void)hipHostFree(ctx->stage[b]);
+    ctx->stage_ev[b] = nullptr;
+    ctx->stage[b] = nullptr;
+  }
+}
+
+// the staging buffers, or false (then copies go straight from pageable memory)
+bool ensure_stage(fp_ctx* ctx) {
+  if (ctx->stage_ev[1]) return true;
+  for (int b = 0; b < 2; b++) {
+    void* p = nullptr;
+    hipEvent_t e = nullptr;
+    const bool ok = hipHostMalloc(&p, STAGE_BYTES, hipHostMallocDefault) == hipSuccess;
+    if (ok) ctx->stage[b] = p;
+    if (!ok || hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      free_stage(ctx);
+      return false;
+    }
+    ctx->stage_ev[b] = e;
+    if (hipEventRecord(e, ctx->stream) != hipSuccess) {
+      (void)hipGetLastError();
+      free_stage(ctx);
+      return false;
+    }
+  }
+  return true;
+}
+
+}  // namespace
+
+fp_status h2d_bulk(fp_ctx* ctx, void* d, const void* h, size_t bytes) {
+  if (bytes == 0) return FP_OK;
+  if (bytes < STAGE_MIN || host_pinned(h) || !ensure_stage(ctx)) {
+    FP_HIP_OK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return FP_OK;
+  }
+  for (size_t off = 0, i = 0; off < bytes; off += STAGE_BYTES, i++) {
+    const size_t n = std::min(STAGE_BYTES, bytes - off);
+    const int b = (int)(i & 1);
+    // the copy out of this buffer two pieces ago has finished
+    FP_HIP_OK(hipEventSynchronize(ctx->stage_ev[b]));
+    char* dst = static_cast<char*>(ctx->stage[b]);
+    const char* src = static_cast<const char*>(h) + off;
+    const long long pieces = (long long)((n + STAGE_PIECE - 1) / STAGE_PIECE);
+#pragma omp parallel for schedule(static)
+    for (long long q = 0; q < pieces; q++) {
+      const size_t o = (size_t)q * STAGE_PIECE;
+      std::memcpy(dst + o, src + o, std::min(STAGE_PIECE, n - o));
+    }
+    FP_HIP_OK(hipMemcpyAsync(static_cast<char*>(d) + off, dst, n, hipMemcpyHostToDevice, ctx->stream));
+    FP_HIP_OK(hipEventRecord(ctx->stage_ev[b], ctx->stream));
+  }
+  return FP_OK;
+}
+
 // K-sized arrays and the packed-entry layout for K parts
 fp_status ensure_k(fp_ctx* ctx, int K) {
   if (K < 1) return fail(ctx, FP_ERR_ARG, "K must be >= 1");
@@ -296,12 +370,26 @@ void fp_ctx_destroy(fp_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   dfree_all(ctx);
+  free_stage(ctx);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
 
 const char* fp_ctx_last_error(const fp_ctx* ctx) { return ctx ? ctx->err : "null context"; }
+
+void* fp_host_alloc_pinned(size_t bytes) {
+  void* p = nullptr;
+  if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return p;
+}
+
+void fp_host_free_pinned(void* p) {
+  if (p) (void)hipHostFree(p);
+}
 
 long long fp_ctx_device_bytes(const fp_ctx* ctx) { return ctx ? ctx->dev_bytes : 0; }
 
@@ -593,6 +681,7 @@ fp_status fp_get_state(fp_ctx* ctx, int* part, int* ep_ends, int* ep_parts, int*
   if (part) FP_TRY(d2h(ctx, part, ctx->part, (size_t)ctx->nov));
   if (sizes) FP_TRY(d2h(ctx, sizes, ctx->sizes, (size_t)ctx->K));
   FP_HIP_OK(hipStreamSynchronize(ctx->stream));
+#pragma omp parallel for schedule(dynamic, 4096)
   for (int n = 0; n < ctx->noe; n++) {
     const unsigned s = ctx->h_net_ptr[(size_t)n], e = ctx->h_net_ptr[(size_t)n + 1];
     const unsigned* seg = ent.data() + s + (unsigned)n;

@@ -639,6 +639,11 @@ struct fp_ctx {
   std::vector<int> h_rank_pins;  // per net, ordering ranks of its pins ascending
   std::vector<int> h_order;      // rank -> vertex
   std::vector<int> h_vwgt;
+  // pinned host staging of large uploads from pageable memory: two buffers
+  // of STAGE_BYTES (allocated on the first such upload, freed with the
+  // context), each with the event of the last copy out of it
+  void* stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
 
   // state
   int K = 0, pb = 0;
@@ -778,11 +783,15 @@ fp_status dalloc(fp_ctx* ctx, T** p, size_t count) {
     if (s_ != FP_OK) return s_; \
   } while (0)
 
+// Host to device on ctx->stream, returning when the host array may be
+// reused. Pinned host arrays are copied directly; large pageable ones go
+// through the context's pinned staging buffers, filled by all host threads
+// while the previous piece is in flight.
+fp_status h2d_bulk(fp_ctx* ctx, void* d, const void* h, size_t bytes);
+
 template <typename T>
 fp_status h2d(fp_ctx* ctx, T* d, const T* h, size_t count) {
-  if (count == 0) return FP_OK;
-  FP_HIP_OK(hipMemcpyAsync(d, h, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
-  return FP_OK;
+  return h2d_bulk(ctx, d, h, count * sizeof(T));
 }
 template <typename T>
 fp_status d2h(fp_ctx* ctx, T* h, const T* d, size_t count) {

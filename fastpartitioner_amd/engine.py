@@ -68,6 +68,10 @@ def declare(lib) -> None:
     lib.fp_ctx_destroy.restype = None
     lib.fp_device_count.argtypes = [c.POINTER(c.c_int)]
     lib.fp_upload.argtypes = [P, c.c_int, c.c_int, _U32P, _U32P, _U32P, _U32P, _I32P, _I32P, _I32P]
+    lib.fp_host_alloc_pinned.restype = P
+    lib.fp_host_alloc_pinned.argtypes = [c.c_size_t]
+    lib.fp_host_free_pinned.restype = None
+    lib.fp_host_free_pinned.argtypes = [P]
     lib.fp_set_partition.argtypes = [P, c.c_int, _I32P]
     lib.fp_bfs_ordering.argtypes = [P, c.c_int, c.c_int, _I32P, c.POINTER(c.c_int), c.POINTER(c.c_int),
                                      c.POINTER(BfsStats)]
@@ -120,6 +124,43 @@ def device_count() -> int:
 
 def schedule(kind: int, round_size: int, C: float, max_pw: int, min_pw: int, cap: int = 0) -> Schedule:
     return Schedule(kind, round_size, C, max_pw, min_pw, cap)
+
+
+class _PinnedBuffer:
+    """Page-locked host memory (fp_host_alloc_pinned), freed when the last
+    array over it goes."""
+
+    def __init__(self, n: int, dtype):
+        self.lib = _native.engine()
+        dt = np.dtype(dtype)
+        self.ptr = self.lib.fp_host_alloc_pinned(max(n * dt.itemsize, 1))
+        if not self.ptr:
+            raise MemoryError(f"fp_host_alloc_pinned({n * dt.itemsize}) failed")
+        self.__array_interface__ = {"shape": (n,), "typestr": dt.str, "data": (self.ptr, False), "version": 3}
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self.lib.fp_host_free_pinned(self.ptr)
+            self.ptr = None
+
+
+def pinned_empty(n: int, dtype) -> np.ndarray:
+    """An uninitialised 1-D array in pinned host memory: fp_upload copies it
+    by direct DMA instead of through its staging buffers."""
+    return np.asarray(_PinnedBuffer(n, dtype))
+
+
+def pinned_copy(hg: Hypergraph) -> Hypergraph:
+    """The hypergraph with every array in pinned host memory."""
+    def pin(a):
+        if a is None:
+            return None
+        out = pinned_empty(a.shape[0], a.dtype)
+        out[:] = a
+        return out
+
+    return Hypergraph(hg.nov, hg.noe, pin(hg.net_ptr), pin(hg.net_pins), pin(hg.vtx_ptr), pin(hg.vtx_nets),
+                      pin(hg.src), pin(hg.vwgt))
 
 
 @dataclasses.dataclass

@@ -26,6 +26,7 @@
  */
 #ifndef FASTPART_H
 #define FASTPART_H
+#include <stddef.h>
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -117,6 +118,14 @@ long long fp_ctx_device_bytes(const fp_ctx* ctx);
 fp_status fp_upload(fp_ctx* ctx, int nov, int noe, const unsigned* net_ptr, const unsigned* net_pins,
                     const unsigned* vtx_ptr, const unsigned* vtx_nets, const int* src, const int* vwgt,
                     const int* ordering);
+/* Host arrays travel by DMA: arrays in pinned memory (fp_host_alloc_pinned,
+ * or any other page-locked allocation) are copied directly; pageable arrays
+ * of 4 MiB and more pass through two 16 MiB pinned staging buffers of the
+ * context (allocated on the first such copy, freed by fp_ctx_destroy), filled
+ * by all host threads while the previous piece is in flight. A reader that
+ * fills both CSR views in pinned memory uploads at the PCIe rate. */
+void* fp_host_alloc_pinned(size_t bytes); /* NULL on failure */
+void fp_host_free_pinned(void* p);
 
 /* ---- connectivity state ------------------------------------------------- */
 /* State from a part vector alone. Each net's list is ordered by first
