@@ -108,6 +108,7 @@ struct ConnEvalArgs {
   const int* msg;
   const int* mstat;
   int K, W, msm;
+  int rcap, scap;  // LDS entries per wavefront for the receiver and sender lists
   int r0, nslots;
   double C;
   int* slot_own;
@@ -127,19 +128,20 @@ __device__ __forceinline__ bool bit_of(const unsigned* m, int p) { return (m[p >
 // other nets; gain = source-net targets t with pc[own][t] == 1 plus, for the
 // FIRST net of each sender part s, 1 if v is own's only pin there and
 // pc[s][own] == 1. TM: best k by gain - loss; MSM: the max-set rule.
-__global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
+__global__ void __launch_bounds__(CONN_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) k_conn_eval(ConnEvalArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int K = a.K, KW = (K + 31) / 32;
   const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
   // block-shared: bit k of vmask = part k is lighter than C (TM's candidates)
   unsigned long long* vmask = reinterpret_cast<unsigned long long*>(lds);
-  int* recv = reinterpret_cast<int*>(vmask + a.W) + (size_t)wave * (size_t)(2 * K + 3 * KW);
-  int* send = recv + K;
-  unsigned* rmark = reinterpret_cast<unsigned*>(send + K);
+  const size_t per = (size_t)(a.rcap + a.scap + 3 * KW);
+  int* recv = reinterpret_cast<int*>(vmask + a.W) + (size_t)wave * per;
+  int* send = recv + a.rcap;
+  unsigned* rmark = reinterpret_cast<unsigned*>(send + a.scap);
   unsigned* smark = rmark + KW;
   unsigned* sgain = smark + KW;
   // per wavefront: the nets its moving vertices touch, before they go to the touched list
-  WaveTouch tb{reinterpret_cast<unsigned*>(vmask + a.W) + (size_t)nw * (size_t)(2 * K + 3 * KW) +
+  WaveTouch tb{reinterpret_cast<unsigned*>(vmask + a.W) + (size_t)nw * per +
                    (size_t)wave * TOUCH_BUF,
                0};
   for (int q = lane; q < KW; q += WAVE) {
@@ -192,7 +194,8 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
             isnew = (atomicOr(&rmark[q >> 5], bit) & bit) == 0u;
           }
           const unsigned long long bm = __ballot(isnew);
-          if (isnew) recv[nrecv + __popcll(bm & lt)] = q;
+          const int at = nrecv + __popcll(bm & lt);
+          if (isnew && at < a.rcap) recv[at] = q;
           nrecv += __popcll(bm);
           if (ok && q != own && a.pc[(size_t)own * (size_t)K + (size_t)q] == 1) gain_own++;
         } else if (__ballot(ok && q == own && (x & EV_SINGLE))) {
@@ -208,7 +211,8 @@ __global__ void __launch_bounds__(CONN_BLOCK) k_conn_eval(ConnEvalArgs a) {
       wave_lds_sync();
       const unsigned long long fm = __ballot(first);
       if (first) {
-        send[nsend + __popcll(fm & lt)] = ps;
+        const int at = nsend + __popcll(fm & lt);
+        if (at < a.scap) send[at] = ps;
         atomicOr(&smark[ps >> 5], 1u << (ps & 31));
         if (((smask >> lane) & 1ull) && a.pc[(size_t)ps * (size_t)K + (size_t)own] == 1) {
           gain_s++;
@@ -738,14 +742,23 @@ __global__ void __launch_bounds__(WAVE) k_msm_scan(int* __restrict__ pc, int* __
 // LDS of k_conn_eval: the block's candidate mask, then per wavefront the
 // receiver and sender lists and three K-bit marks, then per wavefront the
 // touched-net buffer
-size_t conn_eval_lds(int K, int waves) {
-  const size_t per = ((size_t)2 * (size_t)K + 3 * (size_t)((K + 31) / 32) + TOUCH_BUF) * sizeof(int);
+// Receiver and sender list capacities: a vertex's receivers are distinct
+// parts listed in the nets whose source it is (at most their pins), its
+// senders distinct source parts of its other nets (at most its degree).
+// Sized by the instance, not by K, the lists leave the LDS to more waves.
+int conn_rcap(const fp_ctx* ctx) {
+  return (int)std::max<unsigned long long>(1, std::min<unsigned long long>(ctx->max_src_pins, (unsigned long long)ctx->K));
+}
+int conn_scap(const fp_ctx* ctx) { return std::max(1, std::min((int)ctx->max_vdeg, ctx->K)); }
+
+size_t conn_eval_lds(int K, int waves, int rcap, int scap) {
+  const size_t per = ((size_t)rcap + (size_t)scap + 3 * (size_t)((K + 31) / 32) + TOUCH_BUF) * sizeof(int);
   return (size_t)((K + 63) / 64) * sizeof(unsigned long long) + (size_t)waves * per;
 }
 
-int conn_eval_waves(int K) {
+int conn_eval_waves(int K, int rcap, int scap) {
   for (int w = CONN_BLOCK / WAVE; w >= 1; w--)
-    if (conn_eval_lds(K, w) <= (size_t)LDS_BUDGET) return w;
+    if (conn_eval_lds(K, w, rcap, scap) <= (size_t)LDS_BUDGET) return w;
   return 0;
 }
 
@@ -828,7 +841,8 @@ fp_status plan_conn(fp_ctx* ctx, RefinePlan& p) {
                 "TM/MSM refinement visits vertex ids in order (TM/partition.cpp:976); upload with the identity ordering");
   if (p.sched.kind != FP_SCHED_REF)
     return fail(ctx, FP_ERR_UNSUPPORTED, "message refinement runs the reference round schedule (FP_SCHED_REF) only");
-  const int waves = p.eval_waves > 0 ? std::min(p.eval_waves, conn_eval_waves(K)) : conn_eval_waves(K);
+  const int cw = conn_eval_waves(K, conn_rcap(ctx), conn_scap(ctx));
+  const int waves = p.eval_waves > 0 ? std::min(p.eval_waves, cw) : cw;
   if (waves == 0) return fail(ctx, FP_ERR_UNSUPPORTED, "K = %d needs more LDS than a workgroup has", K);
   p.eval_waves = waves;
   p.R = std::min(p.sched.round, nov);
@@ -848,7 +862,8 @@ fp_status plan_conn(fp_ctx* ctx, RefinePlan& p) {
 fp_status pass_conn(fp_ctx* ctx, const RefinePlan& p) {
   const bool msm = p.metric == FP_METRIC_MSM;
   const int nov = ctx->nov, K = ctx->K, W = (K + 63) / 64, R = p.R, waves = p.eval_waves;
-  const size_t lds_e = conn_eval_lds(K, waves);
+  const int rcap = conn_rcap(ctx), scap = conn_scap(ctx);
+  const size_t lds_e = conn_eval_lds(K, waves, rcap, scap);
   const size_t lds_r = (size_t)(REPLAY_BLOCK / WAVE) * (size_t)K * sizeof(unsigned);
   auto& c = ctx->conn;
   const double thr = p.sched.ldg_C * (1.02);
@@ -874,6 +889,8 @@ fp_status pass_conn(fp_ctx* ctx, const RefinePlan& p) {
     a.K = K;
     a.W = W;
     a.msm = msm ? 1 : 0;
+    a.rcap = rcap;
+    a.scap = scap;
     a.r0 = r0;
     a.nslots = nslots;
     a.C = p.sched.ldg_C;

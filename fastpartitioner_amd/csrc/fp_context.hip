@@ -540,6 +540,20 @@ fp_status fp_upload(fp_ctx* ctx, int nov, int noe, const unsigned* net_ptr, cons
     }
   }
   ctx->src_on_pins = src_ok;
+  ctx->max_src_pins = 0;
+  if (src) {
+    std::vector<unsigned long long> sp((size_t)nov, 0);
+#pragma omp parallel for schedule(static)
+    for (int n = 0; n < noe; n++)
+      if (src[n] >= 0) {
+#pragma omp atomic
+        sp[(size_t)src[n]] += net_ptr[n + 1] - net_ptr[n];
+      }
+    unsigned long long m = 0;
+#pragma omp parallel for schedule(static) reduction(max : m)
+    for (int v = 0; v < nov; v++) m = std::max(m, sp[(size_t)v]);
+    ctx->max_src_pins = m;
+  }
 
   FP_TRY(dalloc(ctx, &ctx->net_ptr, (size_t)noe + 1));
   FP_TRY(dalloc(ctx, &ctx->net_pins, (size_t)pins));

@@ -35,6 +35,8 @@ namespace fpe {
 
 constexpr int WAVE = 64;
 constexpr int LDS_BUDGET = 64 * 1024;
+constexpr int LDS_PER_CU = 160 * 1024;  // gfx950: LDS of a compute unit, shared by its workgroups
+constexpr int MAX_WAVES_PER_CU = 32;    // 8 wavefronts per SIMD
 constexpr unsigned NOFIRST = 0xFFFFFFFFu;
 constexpr int ENT_PAD = 64;  // tail padding of the segment arrays
 constexpr unsigned EV_SINGLE = 0x8000u;  // evaluation view: the part has one pin in the net
@@ -634,6 +636,7 @@ struct fp_ctx {
   bool src_on_pins = false;  // every non-empty net's source is one of its pins
   long long total_weight = 0;
   unsigned max_net = 0, max_vdeg = 0;
+  unsigned long long max_src_pins = 0;  // max over vertices of the pins of the nets whose source it is
   unsigned long long sum_deg2 = 0;  // sum over vertices of degree^2 (bounds CN's per-pass edit list)
   std::vector<unsigned> h_net_ptr;
   std::vector<int> h_rank_pins;  // per net, ordering ranks of its pins ascending

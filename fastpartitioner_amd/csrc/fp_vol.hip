@@ -27,7 +27,7 @@
 namespace fpe {
 namespace {
 
-constexpr int VOL_BLOCK = 256;
+constexpr int VOL_BLOCK = 512;  // at most 8 wavefronts; vol_eval_waves picks the count
 constexpr int VOL_KMAX = 4096;  // k_vol_scan: one lane of the max-set tracking holds at most 64 parts
 constexpr int REPLAY_BLOCK = 256;
 constexpr int REPLAY_NET_BLOCKS = 2048;
@@ -754,10 +754,22 @@ __global__ void __launch_bounds__(WAVE) k_vol_scan(int* __restrict__ vol_g, int 
   }
 }
 
+// Wavefronts per evaluation workgroup: 12 bytes per part shared by the
+// workgroup and 8 per part for each wavefront; the count that keeps the most
+// wavefronts resident on a CU (K = 1024: 5 per workgroup, 3 workgroups, 15
+// wavefronts; 4 per workgroup would leave 12), the larger on ties.
 int vol_eval_waves(int K) {
-  for (int w = VOL_BLOCK / WAVE; w >= 1; w--)
-    if ((size_t)K * 12 + (size_t)w * 8 * (size_t)K <= (size_t)LDS_BUDGET) return w;
-  return 0;
+  int best = 0, bw = 0;
+  for (int w = 1; w <= VOL_BLOCK / WAVE; w++) {
+    const size_t lds = (size_t)K * 12 + (size_t)w * 8 * (size_t)K;
+    if (lds > (size_t)LDS_BUDGET) break;
+    const int per_cu = w * std::min((int)((size_t)LDS_PER_CU / lds), MAX_WAVES_PER_CU / w);
+    if (per_cu >= best) {
+      best = per_cu;
+      bw = w;
+    }
+  }
+  return bw;
 }
 
 fp_status ensure_vol_scratch(fp_ctx* ctx, int R) {

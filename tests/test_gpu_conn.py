@@ -71,3 +71,18 @@ def test_msm_moves(R):
 def test_msm_k64():
     hg = H.colnet(2000, H.DEG_UNIFORM, 1, 4, seed=3)
     _run("MSM", hg, 64, 4, 128)
+
+
+@pytest.mark.parametrize("metric", ["TM", "MSM"])
+def test_conn_lists_beyond_a_wavefront(metric):
+    # Zipf nets of up to 5000 pins spread over K = 256 parts: vertices with
+    # more than 64 receivers (their rows read in several wavefront loads)
+    hg = H.workload("c2", scale=0.02)
+    _run(metric, hg, 256, 3, 1 << 30 if metric == "TM" else 512)
+
+
+@pytest.mark.parametrize("metric", ["TM", "MSM"])
+def test_conn_k2048(metric):
+    # 32 bitmap words per row; receiver lists sized by the instance, far below K
+    hg = H.colnet(6000, H.DEG_UNIFORM, 1, 3, seed=4)
+    _run(metric, hg, 2048, 3, 1 << 30 if metric == "TM" else 1024)
