@@ -84,6 +84,57 @@ struct VolEvalArgs {
   int* tcount_next;
 };
 
+// The first-appearance rank of the LDG tie-break (MGV/partition.cpp:765-790)
+// for the parts tied at the best (score, size): the count array is rebuilt
+// by a second sweep, the tied parts are marked in it, and a third sweep
+// keeps each marked part's lowest rank + 1 (ranks (net index, list position)
+// are distinct, so the minimum picks one part). Leaves cnt all zero.
+__device__ __forceinline__ int vol_tie_decision(const VolEvalArgs& a, unsigned vb, int deg, int own, unsigned* cnt,
+                                                const double* s_pen, const unsigned* s_skey,
+                                                unsigned long long m1, unsigned m2) {
+  const int lane = lane_id(), K = a.K;
+  for (int c0 = 0; c0 < deg; c0 += WAVE) {
+    const int nn = min(WAVE, deg - c0);
+    const unsigned sg = lane < nn ? a.vtx_seg[vb + (unsigned)(c0 + lane)] : 0u;
+    wave_sweep_lists<1, 1>(a.ev, sg, nn, [&](int, int, int, bool ok, unsigned x) {
+      if (ok) atomicAdd(&cnt[(int)(x & ~EV_SINGLE)], 1u);
+    });
+  }
+  wave_lds_sync();
+  for (int q = lane; q < K; q += WAVE) {
+    const unsigned c = cnt[q];
+    const int adj = (int)c - (q == own ? 1 : 0);
+    const double sc = (double)adj * s_pen[q];
+    const bool ok = c != 0 && sc > 0;
+    const unsigned long long k1 = ok ? (unsigned long long)__double_as_longlong(sc) : 0ull;
+    cnt[q] = ok && k1 == m1 && s_skey[q] == m2 ? NOFIRST : 0u;
+  }
+  wave_lds_sync();
+  for (int c0 = 0; c0 < deg; c0 += WAVE) {
+    const int nn = min(WAVE, deg - c0);
+    const unsigned sg = lane < nn ? a.vtx_seg[vb + (unsigned)(c0 + lane)] : 0u;
+    wave_sweep_lists<1, 1>(a.ev, sg, nn, [&](int j, int, int t, bool ok, unsigned x) {
+      const int q = (int)(x & ~EV_SINGLE);
+      if (ok && cnt[q] != 0u) atomicMin(&cnt[q], (((unsigned)(c0 + j) << a.pb) | (unsigned)t) + 1u);
+    });
+  }
+  wave_lds_sync();
+  unsigned br = NOFIRST;
+  int bq = -1;
+  for (int q = lane; q < K; q += WAVE) {
+    const unsigned r = cnt[q];
+    cnt[q] = 0u;
+    if (r != 0u && r < br) {
+      br = r;
+      bq = q;
+    }
+  }
+  const unsigned mr = wave_min_u32(br);
+  const int wl = __ffsll((long long)__ballot(br == mr && bq >= 0)) - 1;
+  wave_lds_sync();
+  return __builtin_amdgcn_readlane(bq, wl);
+}
+
 __global__ void __launch_bounds__(VOL_BLOCK) k_vol_eval(VolEvalArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int K = a.K;
@@ -93,8 +144,9 @@ __global__ void __launch_bounds__(VOL_BLOCK) k_vol_eval(VolEvalArgs a) {
   __shared__ long long red_sz[VOL_BLOCK / WAVE];
   __shared__ int red_ix[VOL_BLOCK / WAVE];
   const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
-  unsigned* cnt = wbase + (size_t)wave * 2 * (size_t)K;
-  unsigned* first = cnt + K;
+  const int KW = (K + 31) / 32;
+  unsigned* cnt = wbase + (size_t)wave * (size_t)(K + KW);
+  unsigned* hitb = cnt + K;                  // max parts the move affects (MSRV), one bit per part
   int* delta = reinterpret_cast<int*>(cnt);  // reused once the decision is made (cnt is all 0 again)
 
   // penalty = 1 - (1.0 * size / C) (MGV/partition.cpp:36); fallback = lowest-index lightest part (:48-60)
@@ -109,10 +161,8 @@ __global__ void __launch_bounds__(VOL_BLOCK) k_vol_eval(VolEvalArgs a) {
       bi = q;
     }
   }
-  for (int q = lane; q < K; q += WAVE) {
-    cnt[q] = 0;
-    first[q] = NOFIRST;
-  }
+  for (int q = lane; q < K; q += WAVE) cnt[q] = 0;
+  for (int q = lane; q < KW; q += WAVE) hitb[q] = 0;
   for (int off = WAVE / 2; off > 0; off >>= 1) {
     const long long os = __shfl_xor(bs, off);
     const int oi = __shfl_xor(bi, off);
@@ -155,60 +205,62 @@ __global__ void __launch_bounds__(VOL_BLOCK) k_vol_eval(VolEvalArgs a) {
       ip0 = isrc0 ? own : a.part[s];
     }
 
-    // scores: pin counts per part over v's nets, first appearance ranked by
-    // (net index, list position) as the reference's scan meets them
-    // (MGV/partition.cpp:765-790)
+    // scores: pin counts per part over v's nets (MGV/partition.cpp:765-790)
     for (int c0 = 0; c0 < deg; c0 += WAVE) {
       const int nn = min(WAVE, deg - c0);
       const unsigned sg = c0 == 0 ? sg0 : (lane < nn ? a.vtx_seg[vb + (unsigned)(c0 + lane)] : 0u);
-      wave_sweep_lists<8, SWEEP_LONG_TASKS>(a.ev, sg, nn, [&](int j, int, int t, bool ok, unsigned x) {
-        if (ok) {
-          const int q = (int)(x & ~EV_SINGLE);
-          atomicAdd(&cnt[q], 1u);
-          atomicMin(&first[q], ((unsigned)(c0 + j) << a.pb) | (unsigned)t);
-        }
+      wave_sweep_lists<8, SWEEP_LONG_TASKS>(a.ev, sg, nn, [&](int, int, int, bool ok, unsigned x) {
+        if (ok) atomicAdd(&cnt[(int)(x & ~EV_SINGLE)], 1u);
       });
     }
     wave_lds_sync();
-    // LDG: lexicographic max of (score, -size, -first rank) over score > 0;
-    // four 64-part chunks per step, without branches: their LDS reads issue
+    // LDG: lexicographic max of (score, -size, -first rank) over score > 0.
+    // Scores are positive doubles, so their bit patterns order like the
+    // values: key1 = score bits (max), key2 = size_key(size) (min among key1
+    // ties); ltie: this lane met its best keys twice. The first-appearance
+    // rank only matters between parts tied on both keys (vol_tie_decision).
+    // Four 64-part chunks per step, without branches: their LDS reads issue
     // together; every slot is cleared for the next vertex
-    unsigned long long b1 = 0, b2 = ~0ull;
+    unsigned long long b1 = 0;
+    unsigned b2 = ~0u;
     int bq = -1;
+    bool ltie = false;
     for (int q0 = 0; q0 < K; q0 += 4 * WAVE) {
-      unsigned cv[4], fv[4];
+      unsigned cv[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const int q = min(q0 + u * WAVE + lane, K - 1);
         cv[u] = q0 + u * WAVE + lane < K ? cnt[q] : 0u;
-        fv[u] = first[q];
       }
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const int q = q0 + u * WAVE + lane;
         const int qc = min(q, K - 1);
-        if (q < K) {
-          cnt[q] = 0;
-          first[q] = NOFIRST;
-        }
+        if (q < K) cnt[q] = 0;
         const int adj = (int)cv[u] - (q == own ? 1 : 0);  // scores[part_id]-- (MGV/partition.cpp:792-796)
         const double sc = (double)adj * s_pen[qc];
         const bool ok = cv[u] != 0 && sc > 0;
         const unsigned long long k1 = ok ? (unsigned long long)__double_as_longlong(sc) : 0ull;
-        const unsigned long long k2 = ((unsigned long long)s_skey[qc] << 32) | fv[u];
+        const unsigned k2 = s_skey[qc];
         const bool better = ok && (k1 > b1 || (k1 == b1 && k2 < b2));
+        const bool same = ok && k1 == b1 && k2 == b2;
+        ltie = better ? false : (ltie || same);
         b1 = better ? k1 : b1;
         b2 = better ? k2 : b2;
         bq = better ? q : bq;
       }
     }
     const unsigned long long m1 = wave_max_u64(b1);
-    const unsigned long long m2 = wave_min_u64(b1 == m1 ? b2 : ~0ull);
+    const unsigned m2 = wave_min_u32(b1 == m1 ? b2 : ~0u);
     wave_lds_sync();
     int d = fb;
     if (m1 != 0) {
-      const int wl = __ffsll((long long)__ballot(b1 == m1 && b2 == m2)) - 1;
-      d = __builtin_amdgcn_readlane(bq, wl);
+      const unsigned long long cand = __ballot(b1 == m1 && b2 == m2);
+      if (__popcll(cand) > 1 || __ballot(b1 == m1 && b2 == m2 && ltie) != 0ull) {
+        d = vol_tie_decision(a, vb, deg, own, cnt, s_pen, s_skey, m1, m2);
+      } else {
+        d = __builtin_amdgcn_readlane(bq, __ffsll((long long)cand) - 1);
+      }
     }
 
     int dec = -1;
@@ -282,28 +334,28 @@ __global__ void __launch_bounds__(VOL_BLOCK) k_vol_eval(VolEvalArgs a) {
           }
         }
       };
+      // MSRV: max parts untouched by the move have gains = losses = 0 and fail
+      // the strict test, so every max part must be affected: maxhit counts
+      // the distinct max parts among the items
       bool bad = false;
-      for_items([&](int it, int p) {
+      int maxhit = 0;
+      for_items([&](int, int p) {
         const int dl = delta[p];
         const int vv = a.vol[p];
         if (vv == M) {
           if (a.msrv ? dl >= 0 : dl > 0) bad = true;  // a max part must gain (MSRV: strictly)
-          atomicMin(&first[p], (unsigned)it);
+          const unsigned bit = 1u << (p & 31);
+          if (!(atomicOr(&hitb[p >> 5], bit) & bit)) maxhit++;
         } else if (vv + dl >= M) {
           bad = true;  // no other part may reach the max
         }
       });
       bad = __ballot(bad) != 0;
-      wave_lds_sync();
-      // MSRV: max parts untouched by the move have gains = losses = 0 and fail
-      // the strict test, so every max part must be affected
-      int maxhit = 0;
-      for_items([&](int it, int p) { maxhit += first[p] == (unsigned)it ? 1 : 0; });
       maxhit = wave_sum(maxhit);
       wave_lds_sync();
       for_items([&](int, int p) {
         delta[p] = 0;
-        first[p] = NOFIRST;
+        hitb[p >> 5] = 0u;
       });
       wave_lds_sync();
       if (!bad && (!a.msrv || maxhit == cntM)) dec = d;
@@ -754,14 +806,19 @@ __global__ void __launch_bounds__(WAVE) k_vol_scan(int* __restrict__ vol_g, int 
   }
 }
 
-// Wavefronts per evaluation workgroup: 12 bytes per part shared by the
-// workgroup and 8 per part for each wavefront; the count that keeps the most
-// wavefronts resident on a CU (K = 1024: 5 per workgroup, 3 workgroups, 15
-// wavefronts; 4 per workgroup would leave 12), the larger on ties.
+// LDS of an evaluation workgroup: 12 bytes per part shared by the workgroup,
+// per wavefront a count per part and a bit per part
+size_t vol_eval_lds(int K, int waves) {
+  return (size_t)K * 12 + (size_t)waves * 4 * ((size_t)K + (size_t)((K + 31) / 32));
+}
+
+// Wavefronts per evaluation workgroup: the count that keeps the most
+// wavefronts resident on a CU (K = 1024: 8 per workgroup, 3 workgroups, 24
+// wavefronts), the larger on ties.
 int vol_eval_waves(int K) {
   int best = 0, bw = 0;
   for (int w = 1; w <= VOL_BLOCK / WAVE; w++) {
-    const size_t lds = (size_t)K * 12 + (size_t)w * 8 * (size_t)K;
+    const size_t lds = vol_eval_lds(K, w);
     if (lds > (size_t)LDS_BUDGET) break;
     const int per_cu = w * std::min((int)((size_t)LDS_PER_CU / lds), MAX_WAVES_PER_CU / w);
     if (per_cu >= best) {
@@ -831,7 +888,7 @@ fp_status plan_vol(fp_ctx* ctx, RefinePlan& p) {
 fp_status pass_vol(fp_ctx* ctx, const RefinePlan& p) {
   const bool msrv = p.metric == FP_METRIC_MSRV;
   const int nov = ctx->nov, K = ctx->K, R = p.R, waves = p.eval_waves;
-  const size_t lds_e = (size_t)K * 12 + (size_t)waves * 8 * (size_t)K;
+  const size_t lds_e = vol_eval_lds(K, waves);
   // the replay's part -> position index where it fits the LDS budget
   const bool indexed = (size_t)(REPLAY_BLOCK / WAVE) * (size_t)K * (sizeof(unsigned) + sizeof(unsigned short)) <=
                        (size_t)LDS_BUDGET;
