@@ -130,6 +130,7 @@ __device__ __forceinline__ bool bit_of(const unsigned* m, int p) { return (m[p >
 // pc[s][own] == 1. TM: best k by gain - loss; MSM: the max-set rule.
 __global__ void __launch_bounds__(CONN_BLOCK) __attribute__((amdgpu_waves_per_eu(8))) k_conn_eval(ConnEvalArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
+  __shared__ int s_tn[CONN_BLOCK / WAVE + 1];
   const int K = a.K, KW = (K + 31) / 32;
   const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
   // block-shared: bit k of vmask = part k is lighter than C (TM's candidates)
@@ -388,7 +389,7 @@ __global__ void __launch_bounds__(CONN_BLOCK) __attribute__((amdgpu_waves_per_eu
     }
     wave_lds_sync();
   }
-  wave_touch_flush(tb, a.touched, a.tcount);
+  block_touch_flush(tb, a.touched, a.tcount, s_tn);
 }
 
 // TM apply order (TM/partition.cpp:1111-1125): the round's candidates (the

@@ -495,6 +495,29 @@ __device__ __forceinline__ void wave_touch_flush(WaveTouch& t, int* __restrict__
   t.n = 0;
 }
 
+// The last flush of a kernel, for every wave of the workgroup together: one
+// atomic on the list's counter per workgroup. Every thread of the workgroup
+// must call it (it synchronizes the workgroup); s_tn holds waves + 1 ints.
+__device__ __forceinline__ void block_touch_flush(WaveTouch& t, int* __restrict__ touched, int* __restrict__ tcount,
+                                                  int* s_tn) {
+  const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
+  if (lane == 0) s_tn[wave] = t.n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int w = 0; w < nw; w++) {
+      const int c = s_tn[w];
+      s_tn[w] = tot;
+      tot += c;
+    }
+    s_tn[nw] = tot ? atomicAdd(tcount, tot) : 0;
+  }
+  __syncthreads();
+  const int base = s_tn[nw] + s_tn[wave];
+  for (int i = lane; i < t.n; i += WAVE) touched[base + i] = (int)t.buf[i];
+  t.n = 0;
+}
+
 // wave_touch_nets through the wave's buffer
 __device__ __forceinline__ void wave_touch_nets_staged(const unsigned* __restrict__ vn, int deg, unsigned tag,
                                                        unsigned* __restrict__ net_stamp, WaveTouch& t,

@@ -163,6 +163,7 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
   unsigned* wbase = s_skey + K;
   __shared__ long long red_sz[EVAL_BLOCK / WAVE];
   __shared__ int red_ix[EVAL_BLOCK / WAVE];
+  __shared__ int s_tn[EVAL_BLOCK / WAVE + 1];
   const int lane = lane_id(), wave = wave_id(), nw = (int)(blockDim.x / WAVE);
   // per wavefront: cnt[K] and first[K], each followed by 64 spare slots that
   // take the atomics of lanes past a list's end (no branch around them)
@@ -437,7 +438,7 @@ __global__ void __launch_bounds__(EVAL_BLOCK) __attribute__((amdgpu_waves_per_eu
     hv[1] = hv[2]; hown[1] = hown[2]; hvb[1] = hvb[2]; hdeg[1] = hdeg[2];
     sg0 = sg1;
   }
-  if (a.touched) wave_touch_flush(tb, a.touched, a.tcount);
+  if (a.touched) block_touch_flush(tb, a.touched, a.tcount, s_tn);
 }
 
 // BAL step 1: over-subscribed parts (candidate weight > cap - frozen size) get
