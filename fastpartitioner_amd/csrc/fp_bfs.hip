@@ -31,17 +31,6 @@ namespace {
 constexpr int BFS_BLOCK = 256;
 constexpr unsigned long long NO_KEY = ~0ull;
 
-// appends the lanes' items (where take) to list[*count ...] with one atomic per wave
-__device__ __forceinline__ void wave_append(bool take, int item, int* __restrict__ list, int* __restrict__ count) {
-  const unsigned long long b = __ballot(take);
-  if (!b) return;
-  const int lane = lane_id();
-  int base = 0;
-  if (lane == 0) base = atomicAdd(count, (int)__popcll(b));
-  base = __builtin_amdgcn_readfirstlane(base);
-  if (take) list[base + (int)__popcll(b & ((1ull << lane) - 1ull))] = item;
-}
-
 __global__ void __launch_bounds__(BFS_BLOCK) k_bfs_claim(const int* __restrict__ frontier, int nf,
                                                          const unsigned* __restrict__ vtx_ptr,
                                                          const unsigned* __restrict__ vtx_nets,
@@ -49,6 +38,9 @@ __global__ void __launch_bounds__(BFS_BLOCK) k_bfs_claim(const int* __restrict__
                                                          unsigned long long* __restrict__ net_owner,
                                                          int* __restrict__ level_nets, int* __restrict__ nlevel) {
   const int lane = lane_id(), nw = (int)(blockDim.x / WAVE);
+  __shared__ unsigned s_buf[BFS_BLOCK / WAVE][TOUCH_BUF];
+  __shared__ int s_tn[BFS_BLOCK / WAVE + 1];
+  WaveTouch st{s_buf[wave_id()], 0};
   for (int p = (int)blockIdx.x * nw + wave_id(); p < nf; p += (int)gridDim.x * nw) {
     const int v = frontier[p];
     const unsigned vb = vtx_ptr[v], deg = vtx_ptr[v + 1] - vb;
@@ -63,9 +55,10 @@ __global__ void __launch_bounds__(BFS_BLOCK) k_bfs_claim(const int* __restrict__
         const unsigned long long key = ((unsigned long long)p << 32) | j;
         if (!net_done[e] && key < net_owner[e]) first = atomicMin(&net_owner[e], key) == NO_KEY;
       }
-      wave_append(first, e, level_nets, nlevel);
+      wave_stage_append(first, (unsigned)e, st, level_nets, nlevel);
     }
   }
+  block_touch_flush(st, level_nets, nlevel, s_tn);
 }
 
 // The same claims from the nets' side, for a large frontier (claims from the
@@ -83,6 +76,9 @@ __global__ void __launch_bounds__(BFS_BLOCK) k_bfs_claim_nets(const unsigned* __
                                                               unsigned long long* __restrict__ net_owner,
                                                               int* __restrict__ level_nets, int* __restrict__ nlevel) {
   const int lane = lane_id(), nw = (int)(blockDim.x / WAVE);
+  __shared__ unsigned s_buf[BFS_BLOCK / WAVE][TOUCH_BUF];
+  __shared__ int s_tn[BFS_BLOCK / WAVE + 1];
+  WaveTouch st{s_buf[wave_id()], 0};
   for (int e0 = ((int)blockIdx.x * nw + wave_id()) * WAVE; e0 < noe; e0 += (int)gridDim.x * nw * WAVE) {
     // 64 nets per wave: their done flags together, then the open ones in turn
     const int ee = e0 + lane;
@@ -116,8 +112,9 @@ __global__ void __launch_bounds__(BFS_BLOCK) k_bfs_claim_nets(const unsigned* __
         any = true;
       }
     }
-    wave_append(any, ee, level_nets, nlevel);
+    wave_stage_append(any, (unsigned)ee, st, level_nets, nlevel);
   }
+  block_touch_flush(st, level_nets, nlevel, s_tn);
 }
 
 // positions of the frontier's vertices for k_bfs_claim_nets
@@ -140,6 +137,9 @@ __global__ void __launch_bounds__(BFS_BLOCK) k_bfs_expand(const int* __restrict_
                                                           unsigned long long* __restrict__ vkey,
                                                           int* __restrict__ cand, int* __restrict__ ncand) {
   const int lane = lane_id(), nw = (int)(blockDim.x / WAVE);
+  __shared__ unsigned s_buf[BFS_BLOCK / WAVE][TOUCH_BUF];
+  __shared__ int s_tn[BFS_BLOCK / WAVE + 1];
+  WaveTouch st{s_buf[wave_id()], 0};
   const int n = *nlevel;
   for (int i = (int)blockIdx.x * nw + wave_id(); i < n; i += (int)gridDim.x * nw) {
     const int e = level_nets[i];
@@ -157,9 +157,10 @@ __global__ void __launch_bounds__(BFS_BLOCK) k_bfs_expand(const int* __restrict_
         u = (int)pins[s + k];
         if (dist[u] < 0) first = atomicMin(&vkey[u], hi | jk | k) == NO_KEY;
       }
-      wave_append(first, u, cand, ncand);
+      wave_stage_append(first, (unsigned)u, st, cand, ncand);
     }
   }
+  block_touch_flush(st, cand, ncand, s_tn);
 }
 
 __global__ void __launch_bounds__(BFS_BLOCK) k_bfs_mark(const int* __restrict__ cand, int n, int level,

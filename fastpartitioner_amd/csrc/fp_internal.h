@@ -495,6 +495,19 @@ __device__ __forceinline__ void wave_touch_flush(WaveTouch& t, int* __restrict__
   t.n = 0;
 }
 
+// Appends the lanes' items (where take) to a list through the wave's LDS
+// buffer (a WaveTouch of TOUCH_BUF entries): one atomic on the list's counter
+// per full buffer instead of one per call. The order of the list is not kept.
+__device__ __forceinline__ void wave_stage_append(bool take, unsigned item, WaveTouch& t, int* __restrict__ list,
+                                                  int* __restrict__ count) {
+  const unsigned long long b = __ballot(take);
+  const int c = (int)__popcll(b);
+  if (c == 0) return;
+  if (t.n + c > TOUCH_BUF) wave_touch_flush(t, list, count);
+  if (take) t.buf[t.n + (int)__popcll(b & ((1ull << lane_id()) - 1ull))] = item;
+  t.n += c;
+}
+
 // The last flush of a kernel, for every wave of the workgroup together: one
 // atomic on the list's counter per workgroup. Every thread of the workgroup
 // must call it (it synchronizes the workgroup); s_tn holds waves + 1 ints.
