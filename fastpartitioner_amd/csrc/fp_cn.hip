@@ -183,52 +183,64 @@ struct CnPrepArgs {
 // pass (every accepted net whose old part is v's part at pass start); pushes
 // apply in net order, so the source of the push by net n is the new part of
 // the largest such net below n, else v's part at pass start.
-// The touched-net appends are counted per wave and the op total per block:
-// one atomic each on those counters instead of one per push (same-address
-// atomics queue at one L2 slice).
+// The touched-net appends go through a per-wave LDS buffer (one atomic on
+// the list's counter per full buffer, one per workgroup at the end) and the
+// op total is summed per block: same-address atomics whose returns the waves
+// wait for queue at one L2 slice. A wave takes 64 consecutive pushes and
+// walks their net lists in step, so that its appends stay wave-uniform.
 __global__ void __launch_bounds__(256) k_cn_prep(CnPrepArgs a) {
   __shared__ unsigned long long s_ops;
+  __shared__ unsigned s_buf[256 / WAVE][TOUCH_BUF];
+  __shared__ int s_tn[256 / WAVE + 1];
   if (threadIdx.x == 0) s_ops = 0;
   __syncthreads();
-  const int lane = lane_id();
+  const int lane = lane_id(), wave = wave_id();
+  WaveTouch st{s_buf[wave], 0};
   unsigned long long ops = 0;
   const long long nm = (long long)*a.nmoves;
-  for (long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x; m < nm; m += (long long)gridDim.x * blockDim.x) {
-  const int v = a.mv_v[m], n = a.mv_net[m];
-  const int p0 = a.part[v];
-  int prev = -1;
-  bool last = true;
-  const unsigned vb = a.vtx_ptr[v], ve = a.vtx_ptr[v + 1];
-  for (unsigned j = vb; j < ve; j++) {
-    const int k = (int)a.vtx_nets[j];
-    if (k != n && a.cn_old[k] == p0) {
-      if (k < n) prev = max(prev, k);
-      else last = false;
+  for (long long m0 = (long long)blockIdx.x * blockDim.x + (long long)wave * WAVE; m0 < nm;
+       m0 += (long long)gridDim.x * blockDim.x) {
+    const long long m = m0 + lane;
+    const bool act = m < nm;
+    int v = 0, n = -1;
+    unsigned vb = 0, deg = 0;
+    if (act) {
+      v = a.mv_v[m];
+      n = a.mv_net[m];
+      const int p0 = a.part[v];
+      int prev = -1;
+      bool last = true;
+      vb = a.vtx_ptr[v];
+      deg = a.vtx_ptr[v + 1] - vb;
+      for (unsigned j = vb; j < vb + deg; j++) {
+        const int k = (int)a.vtx_nets[j];
+        if (k != n && a.cn_old[k] == p0) {
+          if (k < n) prev = max(prev, k);
+          else last = false;
+        }
+      }
+      a.mv_from[m] = prev >= 0 ? a.cn_new[prev] : p0;
+      a.mv_last[m] = last ? 1 : 0;
+      ops += deg;
+    }
+    const int dmax = wave_max((int)deg);
+    for (int jj = 0; jj < dmax; jj++) {
+      // the net's first op of the pass puts it on the touched list (op counts
+      // are 0 between passes: the apply resets them). The pushing net's own
+      // ops are its pushes (its pins are distinct: refinement rejects repeated
+      // pins), added once by its first push instead of one atomic per push on
+      // one word
+      bool isnew = false;
+      unsigned t = 0;
+      if (act && (unsigned)jj < deg) {
+        t = a.vtx_nets[vb + (unsigned)jj];
+        if ((int)t != n) isnew = atomicAdd(&a.opcnt[t], 1) == 0;
+        else if (m == a.pbase[n]) isnew = atomicAdd(&a.opcnt[t], a.npush[n]) == 0;
+      }
+      wave_stage_append(isnew, t, st, a.touched, a.tcount);
     }
   }
-  a.mv_from[m] = prev >= 0 ? a.cn_new[prev] : p0;
-  a.mv_last[m] = last ? 1 : 0;
-  for (unsigned j = vb; j < ve; j++) {
-    const unsigned t = a.vtx_nets[j];
-    // the net's first op of the pass puts it on the touched list (op counts
-    // are 0 between passes: the apply resets them). The pushing net's own ops
-    // are its pushes (its pins are distinct: refinement rejects repeated
-    // pins), added once by its first push instead of one atomic per push on
-    // one word
-    bool isnew = false;
-    if ((int)t != n) isnew = atomicAdd(&a.opcnt[t], 1) == 0;
-    else if (m == a.pbase[n]) isnew = atomicAdd(&a.opcnt[t], a.npush[n]) == 0;
-    // the lanes still in this loop append their new nets with one atomic
-    const unsigned long long b = __ballot(isnew);
-    if (b) {
-      int base = 0;
-      if (lane == (int)__builtin_ctzll(__ballot(true))) base = atomicAdd(a.tcount, (int)__popcll(b));
-      base = __builtin_amdgcn_readfirstlane(base);
-      if (isnew) a.touched[base + (int)__popcll(b & ((1ull << lane) - 1ull))] = (int)t;
-    }
-  }
-  ops += ve - vb;
-  }
+  block_touch_flush(st, a.touched, a.tcount, s_tn);
   if (ops) atomicAdd(&s_ops, ops);
   __syncthreads();
   if (threadIdx.x == 0 && s_ops) atomicAdd(&a.totals[0], s_ops);
