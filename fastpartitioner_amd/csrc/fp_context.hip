@@ -105,7 +105,7 @@ int bits_for(int K) {
 namespace {
 
 constexpr size_t STAGE_BYTES = size_t{16} << 20;  // per staging buffer
-constexpr size_t STAGE_MIN = size_t{4} << 20;     // smaller copies go straight from pageable memory
+constexpr size_t STAGE_MIN = size_t{16} << 20;    // smaller copies go straight from pageable memory
 constexpr size_t STAGE_PIECE = size_t{1} << 20;   // host threads fill a staging buffer in pieces of this
 
 bool host_pinned(const void* h) {
@@ -690,11 +690,15 @@ fp_status fp_get_state(fp_ctx* ctx, int* part, int* ep_ends, int* ep_parts, int*
   if (!ctx) return FP_ERR_ARG;
   if (!ctx->has_state) return fail(ctx, FP_ERR_STATE, "no state");
   FP_HIP_OK(hipSetDevice(ctx->device));
-  std::vector<unsigned> ent((size_t)ctx->pins + (size_t)ctx->noe);
-  FP_TRY(d2h(ctx, ent.data(), ctx->ent, ent.size()));
+  // the lists travel only when asked for (a caller after the part vector
+  // alone, as the server's requests, skips their download and unpacking)
+  const bool lists = ep_ends || ep_parts || ep_cnt;
+  std::vector<unsigned> ent(lists ? (size_t)ctx->pins + (size_t)ctx->noe : 0);
+  if (lists) FP_TRY(d2h(ctx, ent.data(), ctx->ent, ent.size()));
   if (part) FP_TRY(d2h(ctx, part, ctx->part, (size_t)ctx->nov));
   if (sizes) FP_TRY(d2h(ctx, sizes, ctx->sizes, (size_t)ctx->K));
   FP_HIP_OK(hipStreamSynchronize(ctx->stream));
+  if (!lists) return FP_OK;
 #pragma omp parallel for schedule(dynamic, 4096)
   for (int n = 0; n < ctx->noe; n++) {
     const unsigned s = ctx->h_net_ptr[(size_t)n], e = ctx->h_net_ptr[(size_t)n + 1];

@@ -120,7 +120,7 @@ fp_status fp_upload(fp_ctx* ctx, int nov, int noe, const unsigned* net_ptr, cons
                     const int* ordering);
 /* Host arrays travel by DMA: arrays in pinned memory (fp_host_alloc_pinned,
  * or any other page-locked allocation) are copied directly; pageable arrays
- * of 4 MiB and more pass through two 16 MiB pinned staging buffers of the
+ * of 16 MiB and more pass through two 16 MiB pinned staging buffers of the
  * context (allocated on the first such copy, freed by fp_ctx_destroy), filled
  * by all host threads while the previous piece is in flight. A reader that
  * fills both CSR views in pinned memory uploads at the PCIe rate. */
@@ -195,7 +195,8 @@ fp_status fp_set_state(fp_ctx* ctx, int K, const int* part, const int* ep_ends, 
                        const int* ep_cnt, const long long* sizes);
 
 /* Downloads the state in the reference's layout; slots past a net's lambda
- * read part -1 / count 0, as the reference leaves them. Any pointer may be NULL. */
+ * read part -1 / count 0, as the reference leaves them. Any pointer may be
+ * NULL; with ep_ends, ep_parts and ep_cnt all NULL the lists are not read. */
 fp_status fp_get_state(fp_ctx* ctx, int* part, int* ep_ends, int* ep_parts, int* ep_cnt, long long* sizes);
 
 /* ---- refinement --------------------------------------------------------- */

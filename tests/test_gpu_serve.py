@@ -93,7 +93,9 @@ def test_serve_budget_bounds_the_bytes_in_flight():
 
 def test_replicas_run_one_hypergraphs_requests_side_by_side():
     graphs, reqs = _workload()
-    one = [(0, metric, K, part, sched, passes) for g, metric, K, part, sched, passes in reqs if g == 0]
+    # every request on hypergraph 0, four times over and with more passes, so
+    # that requests are still in flight when the next ones arrive
+    one = [(0, metric, K, part, sched, 12) for g, metric, K, part, sched, passes in reqs if g == 0] * 4
     with Server(0, hbm_budget=1 << 36, workers=6) as srv:
         srv.set_replicas(3)
         srv.add_graph(graphs[0])
