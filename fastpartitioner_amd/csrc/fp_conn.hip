@@ -904,10 +904,7 @@ fp_status pass_conn(fp_ctx* ctx, const RefinePlan& p) {
     a.tcount_next = ctx->counters + (par ^ 1);
     int egrid = std::max(1, std::min((nslots + waves - 1) / waves, 8192));
     if (p.eval_grid > 0) egrid = std::min(egrid, p.eval_grid);
-    hipEvent_t tb = nullptr;
-    FP_TRY(timing_begin(ctx, &tb));
-    k_conn_eval<<<egrid, waves * WAVE, lds_e, ctx->stream>>>(a);
-    FP_TRY(timing_end(ctx, tb));
+    FP_TRY(launch_timed(ctx, k_conn_eval, egrid, waves * WAVE, lds_e, a));
     ConnApplyArgs ap{};
     ap.net_ptr = ctx->net_ptr;
     ap.apply_pins = ctx->apply_pins;

@@ -18,6 +18,7 @@
 //                                    read brings a net's lambda and first 63 entries
 //   sizes[K] (int64)                 part weights
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -845,21 +846,23 @@ void fill_async(fp_ctx* ctx, T* p, size_t n, T v) {
   k_fill<T><<<blocks, 256, 0, ctx->stream>>>(p, n, v);
 }
 
-// With ctx->kernel_timing, events around the next launch of a pass's
-// dominant kernel (fp_last_kernel_timing adds them up after fp_refine):
-// timing_begin before the launch, timing_end after it.
-inline fp_status timing_begin(fp_ctx* ctx, hipEvent_t* tb) {
-  *tb = nullptr;
-  if (!ctx->kernel_timing) return FP_OK;
-  hipEvent_t ta = pooled_event(ctx, ctx->timed_next);
-  *tb = pooled_event(ctx, ctx->timed_next + 1);
-  if (!ta || !*tb) return fail(ctx, FP_ERR_HIP, "hipEventCreate failed");
+// Launches a pass's dominant kernel, kern<<<grid, block, lds>>>(arg) on the
+// context's stream. With ctx->kernel_timing the launch's own start and end
+// go into the next two pooled events (fp_last_kernel_timing adds them up after
+// fp_refine): the runtime stamps the dispatch itself, so no event packets sit
+// between the pass's kernels (each such record had left the GPU idle ~5 us).
+template <class Arg>
+fp_status launch_timed(fp_ctx* ctx, void (*kern)(Arg), int grid, int block, size_t lds, Arg a) {
+  if (!ctx->kernel_timing) {
+    kern<<<grid, block, lds, ctx->stream>>>(a);
+    return FP_OK;
+  }
+  hipEvent_t ta = pooled_event(ctx, ctx->timed_next), tb = pooled_event(ctx, ctx->timed_next + 1);
+  if (!ta || !tb) return fail(ctx, FP_ERR_HIP, "hipEventCreate failed");
   ctx->timed_next += 2;
-  if (hipEventRecord(ta, ctx->stream) != hipSuccess) return fail(ctx, FP_ERR_HIP, "hipEventRecord failed");
-  return FP_OK;
-}
-inline fp_status timing_end(fp_ctx* ctx, hipEvent_t tb) {
-  if (tb && hipEventRecord(tb, ctx->stream) != hipSuccess) return fail(ctx, FP_ERR_HIP, "hipEventRecord failed");
+  void* args[] = {&a};
+  FP_HIP_OK(hipExtLaunchKernel(reinterpret_cast<const void*>(kern), dim3((unsigned)grid), dim3((unsigned)block), args,
+                               lds, ctx->stream, ta, tb, 0));
   return FP_OK;
 }
 

@@ -986,18 +986,9 @@ fp_status pass_tv(fp_ctx* ctx, const RefinePlan& p) {
       a.hist = ctx->hist;
       a.sizes_frozen = ctx->sizes_frozen;
     }
-    hipEvent_t ta = nullptr, tb = nullptr;
-    if (ctx->kernel_timing) {
-      ta = pooled_event(ctx, ctx->timed_next);
-      tb = pooled_event(ctx, ctx->timed_next + 1);
-      if (!ta || !tb) return fail(ctx, FP_ERR_HIP, "hipEventCreate failed");
-      ctx->timed_next += 2;
-      FP_HIP_OK(hipEventRecord(ta, ctx->stream));
-    }
     int grid = eval_grid(nslots, waves);
     if (p.eval_grid > 0) grid = std::min(grid, p.eval_grid);
-    k_tv_eval<EVAL_REFINE><<<grid, waves * WAVE, lds_e, ctx->stream>>>(a);
-    if (ctx->kernel_timing) FP_HIP_OK(hipEventRecord(tb, ctx->stream));
+    FP_TRY(launch_timed(ctx, k_tv_eval<EVAL_REFINE>, grid, waves * WAVE, lds_e, a));
     if (bal) {
       k_bal_scan<<<std::max(1, std::min(K, 512)), 256, lds_s, ctx->stream>>>(
           ctx->hist, ctx->pref, ntiles, K, ctx->sizes_frozen, ctx->cand_tot, sched->cap, ctx->over_idx,

@@ -177,3 +177,26 @@ def test_device_stream_rejects_bad_arguments():
         # the context stays usable
         eng.initial_partition(0, 4, 16, 130.0, 1000, 140)
         assert eng.get_state().part.min() >= 0
+
+
+@pytest.mark.parametrize("metric", ["TV", "MSV", "TM"])
+def test_kernel_timing_times_every_evaluation_and_changes_nothing(metric):
+    # the evaluation launches timed by their own dispatch (fp_set_kernel_timing):
+    # one measurement per round, and the same refinement as without timing
+    hg = H.workload("c2", seed=2, scale=0.02)
+    K, passes, R = 64, 3, 4096
+    C, max_pw, min_pw = H.ref_constants(hg.nov, K)
+    part = H.initial_partition_for(metric, hg, K)
+    s = schedule(SCHED_REF, hg.nov if metric == "TM" else R, C, max_pw, min_pw)
+    out = []
+    for timed in (False, True):
+        with Engine(0) as eng:
+            eng.upload(hg)
+            eng.set_partition(K, part)
+            eng.set_kernel_timing(timed)
+            st = eng.refine(metric, passes, s)
+            launches, ms = eng.kernel_timing()
+            out.append((st.moves, eng.get_state().part, eng.metrics()))
+            if timed:
+                assert launches == st.rounds and ms > 0
+    assert out[0][0] == out[1][0] and np.array_equal(out[0][1], out[1][1]) and out[0][2] == out[1][2]
