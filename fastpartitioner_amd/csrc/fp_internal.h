@@ -850,7 +850,7 @@ void fill_async(fp_ctx* ctx, T* p, size_t n, T v) {
 // context's stream. With ctx->kernel_timing the launch's own start and end
 // go into the next two pooled events (fp_last_kernel_timing adds them up after
 // fp_refine): the runtime stamps the dispatch itself, so no event packets sit
-// between the pass's kernels (each such record had left the GPU idle ~5 us).
+// between the pass's kernels.
 template <class Arg>
 fp_status launch_timed(fp_ctx* ctx, void (*kern)(Arg), int grid, int block, size_t lds, Arg a) {
   if (!ctx->kernel_timing) {
