@@ -545,7 +545,7 @@ template <bool AGG>
 __global__ void __launch_bounds__(256) k_cn_commit(const int* __restrict__ mv_v, const int* __restrict__ mv_net,
                                                    const int* __restrict__ mv_from, const int* __restrict__ mv_last,
                                                    const unsigned long long* __restrict__ nmoves,
-                                                   const int* __restrict__ flags, const int* __restrict__ cn_new,
+                                                   int* __restrict__ flags, const int* __restrict__ cn_new,
                                                    const int* __restrict__ vwgt, int K, int* __restrict__ part,
                                                    long long* __restrict__ sizes, unsigned long long* __restrict__ moves) {
   extern __shared__ __align__(16) unsigned char lds[];
@@ -556,7 +556,10 @@ __global__ void __launch_bounds__(256) k_cn_commit(const int* __restrict__ mv_v,
     __syncthreads();
   }
   const long long nm = (long long)*nmoves;
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(moves, (unsigned long long)nm);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    atomicAdd(moves, (unsigned long long)nm);
+    atomicAdd(&flags[2], 1);  // passes applied since the host last cleared it
+  }
   for (long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x; m < nm; m += (long long)gridDim.x * blockDim.x) {
     const int v = mv_v[m];
     const int to = cn_new[mv_net[m]];
@@ -666,6 +669,33 @@ fp_status pass_cn(fp_ctx* ctx, const RefinePlan&) {
   else
     k_cn_commit<false><<<gm, 256, 0, ctx->stream>>>(s.mv_v, s.mv_net, s.mv_from, s.mv_last, nm, ctx->flags,
                                                     s.new_part, ctx->vwgt, ctx->K, ctx->part, ctx->sizes, ctx->moves);
+  return FP_OK;
+}
+
+fp_status cn_passes(fp_ctx* ctx, const RefinePlan& p, int passes) {
+  int done = 0;
+  while (done < passes) {
+    fill_async<int>(ctx, ctx->flags + 2, 1, 0);
+    const int run = passes - done;
+    for (int it = 0; it < run; it++) FP_TRY(pass_cn(ctx, p));
+    int f[3] = {0, 0, 0};
+    FP_TRY(d2h(ctx, f, ctx->flags, 3));
+    FP_HIP_OK(hipStreamSynchronize(ctx->stream));
+    if (f[1])
+      return fail(ctx, FP_ERR_UNSUPPORTED,
+                  "a net with a repeated pin receives more than %d moves in one CN pass; that pass was not applied",
+                  CN_MAXOPS);
+    if (f[2] < 0 || f[2] > run) return fail(ctx, FP_ERR_HIP, "CN pass count %d out of [0, %d]", f[2], run);
+    done += f[2];
+    if (f[0]) {
+      auto& s = ctx->cn;
+      s.ops_cap = std::max<long long>((long long)f[0], 2 * s.ops_cap);
+      FP_TRY(dalloc(ctx, &s.ops, (size_t)s.ops_cap));
+      fill_async<int>(ctx, ctx->flags, 4, 0);
+    } else if (f[2] != run) {
+      return fail(ctx, FP_ERR_HIP, "CN passes: %d of %d applied without an overflow", f[2], run);
+    }
+  }
   return FP_OK;
 }
 

@@ -699,7 +699,8 @@ struct fp_ctx {
   // round scratch
   int rmax = 0;
   int *slot_own = nullptr, *slot_dec = nullptr, *touched = nullptr, *counters = nullptr;
-  int* flags = nullptr;  // [0] CN edit list overflow: the size it needed (0 = none)
+  int* flags = nullptr;  // [0] CN edit list overflow: the size it needed (0 = none), [1] a repeated
+                         // pin's net over CN_MAXOPS ops, [2] CN passes applied (counted by the commit)
   unsigned* net_stamp = nullptr;  // round tag of the last round that touched the net (cleared every pass)
   long long *cand_tot = nullptr, *sizes_frozen = nullptr, *pref = nullptr;
   int* hist = nullptr;
@@ -799,6 +800,12 @@ fp_status record_pass(fp_ctx* ctx, const RefinePlan& p);
 // CN only: after a pass, grows the edit list if the pass overflowed it (the
 // pass then left the state unchanged); *redo = 1 when it must run again
 fp_status cn_after_pass(fp_ctx* ctx, int* redo);
+// CN only: `passes` passes enqueued back to back with one host check at the
+// end. A pass that overflows the edit list leaves the state unchanged, and so
+// does every later pass of the burst (the same state needs the same list);
+// the commit counts the applied passes, so the rest run again after the list
+// has grown.
+fp_status cn_passes(fp_ctx* ctx, const RefinePlan& p, int passes);
 fp_status dalloc_bytes(fp_ctx* ctx, void** p, size_t bytes);
 void dfree_all(fp_ctx* ctx);
 hipEvent_t pooled_event(fp_ctx* ctx, size_t i);

@@ -1083,13 +1083,10 @@ fp_status fp_refine(fp_ctx* ctx, fp_metric metric, int passes, const fp_schedule
   hipEvent_t e0 = pooled_event(ctx, 0), e1 = pooled_event(ctx, 1);
   if (!e0 || !e1) return fail(ctx, FP_ERR_HIP, "hipEventCreate failed");
   FP_HIP_OK(hipEventRecord(e0, ctx->stream));
-  for (int it = 0; it < passes; it++) {
-    FP_TRY(record_pass(ctx, p));
-    if (metric == FP_METRIC_CN) {
-      int redo = 0;
-      FP_TRY(cn_after_pass(ctx, &redo));
-      if (redo) it--;
-    }
+  if (metric == FP_METRIC_CN) {
+    FP_TRY(cn_passes(ctx, p, passes));
+  } else {
+    for (int it = 0; it < passes; it++) FP_TRY(record_pass(ctx, p));
   }
   FP_HIP_OK(hipGetLastError());
   FP_HIP_OK(hipEventRecord(e1, ctx->stream));

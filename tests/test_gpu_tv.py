@@ -147,6 +147,32 @@ def test_cn_refinement_matches_oracle(name, scale, K):
     assert stats.moves > 0
 
 
+def test_cn_edit_list_growth_reruns_the_overflowed_passes():
+    # vertices in about 20 nets each, pushed in bulk from a random 2-way
+    # partition: a pass edits more lists than four times the pins, the edit
+    # list's first size. The passes are enqueued back to back; the one that
+    # overflows and the ones after it leave the state unchanged and run again
+    # once the list has grown, which must give the oracle's refinement
+    hg = H.colnet(3000, H.DEG_FIXED, 19, seed=9)
+    K = 2
+    part = np.random.default_rng(9).integers(0, K, hg.nov).astype(np.int32)
+    st = O.build_state(hg, K, part)
+    st.sizes = st.sizes.astype(np.int32)
+    with Engine(0) as eng:
+        eng.upload(hg)
+        eng.set_partition(K, part)
+        eng.refine("CN", 0)
+        b0 = eng.device_bytes()
+        stats = eng.refine("CN", 4)
+        b1 = eng.device_bytes()
+        got = eng.get_state()
+    O.cn_refine(hg, st, 4, serial=False)
+    assert b1 > b0  # the edit list grew
+    assert stats.moves > 0
+    _same_state(got, st)
+    assert np.array_equal(got.sizes, st.sizes.astype(np.int64))
+
+
 def _hubs(seed=11, nov=3000, degrees=(62, 63, 64, 125, 126, 127, 188, 189, 190, 200)):
     """Random nets of 2..12 pins, plus hub vertices 0, 1, ... that belong to
     the given numbers of nets: degrees at and around the ends of the
