@@ -30,6 +30,7 @@ namespace {
 
 constexpr int CN_MAXOPS = 4096;  // ops per net and pass sorted in LDS; more go through k_cn_bigsort
 constexpr int SCAN_BLOCK = 1024;
+constexpr int CN_EVAL_GROUP = 4;  // a pin's nets whose lambdas k_cn_eval loads together
 
 struct CnEvalArgs {
   const unsigned* net_ptr;
@@ -73,8 +74,19 @@ __global__ void __launch_bounds__(256) k_cn_eval(CnEvalArgs a) {
         if (k < e) {
           const int v = (int)a.net_pins[k];
           isold = a.part[v] == oldp;
-          if (isold)
-            for (unsigned j = a.vtx_ptr[v]; j < a.vtx_ptr[v + 1] && !b; j++) b = a.ev[a.vtx_seg[j]] == 1;
+          if (isold) {
+            // v's nets CN_EVAL_GROUP at a time: their segment offsets load
+            // together, then their lambdas, instead of one dependent pair of
+            // loads per net
+            const unsigned jb = a.vtx_ptr[v], je = a.vtx_ptr[v + 1];
+            for (unsigned j = jb; j < je && !b; j += CN_EVAL_GROUP) {
+              unsigned sg[CN_EVAL_GROUP];
+#pragma unroll
+              for (int u = 0; u < CN_EVAL_GROUP; u++) sg[u] = a.vtx_seg[min(j + (unsigned)u, je - 1)];
+#pragma unroll
+              for (int u = 0; u < CN_EVAL_GROUP; u++) b |= a.ev[sg[u]] == 1;
+            }
+          }
         }
         cnt += __popcll(__ballot(isold));
         bad = __ballot(b) != 0;
