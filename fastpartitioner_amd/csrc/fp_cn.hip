@@ -224,11 +224,20 @@ __global__ void __launch_bounds__(256) k_cn_prep(CnPrepArgs a) {
       bool last = true;
       vb = a.vtx_ptr[v];
       deg = a.vtx_ptr[v + 1] - vb;
-      for (unsigned j = vb; j < vb + deg; j++) {
-        const int k = (int)a.vtx_nets[j];
-        if (k != n && a.cn_old[k] == p0) {
-          if (k < n) prev = max(prev, k);
-          else last = false;
+      // v's nets CN_EVAL_GROUP at a time: ids, then their accepted old parts
+      for (unsigned j = vb; j < vb + deg; j += CN_EVAL_GROUP) {
+        int kk[CN_EVAL_GROUP], co[CN_EVAL_GROUP];
+#pragma unroll
+        for (int u = 0; u < CN_EVAL_GROUP; u++) kk[u] = (int)a.vtx_nets[min(j + (unsigned)u, vb + deg - 1)];
+#pragma unroll
+        for (int u = 0; u < CN_EVAL_GROUP; u++) co[u] = a.cn_old[kk[u]];
+#pragma unroll
+        for (int u = 0; u < CN_EVAL_GROUP; u++) {
+          const int k = kk[u];
+          if (j + (unsigned)u < vb + deg && k != n && co[u] == p0) {
+            if (k < n) prev = max(prev, k);
+            else last = false;
+          }
         }
       }
       a.mv_from[m] = prev >= 0 ? a.cn_new[prev] : p0;
@@ -314,10 +323,23 @@ __global__ void k_cn_scatter(const int* __restrict__ mv_v, const int* __restrict
   const long long nm = (long long)*nmoves;
   for (long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x; m < nm; m += (long long)gridDim.x * blockDim.x) {
     const int v = mv_v[m], n = mv_net[m];
-    for (unsigned j = vtx_ptr[v]; j < vtx_ptr[v + 1]; j++) {
-      const unsigned t = vtx_nets[j];
-      const long long slot = (int)t == n ? m - pbase[n] : (long long)atomicAdd(&opfill[t], 1);
-      ops[opoff[t] + slot] = (int)m;
+    const unsigned jb = vtx_ptr[v], je = vtx_ptr[v + 1];
+    // CN_EVAL_GROUP nets at a time: their ids, then their list offsets and
+    // slots (independent loads and atomics in flight together)
+    for (unsigned j = jb; j < je; j += CN_EVAL_GROUP) {
+      unsigned t[CN_EVAL_GROUP];
+      long long off[CN_EVAL_GROUP], slot[CN_EVAL_GROUP];
+#pragma unroll
+      for (int u = 0; u < CN_EVAL_GROUP; u++) t[u] = vtx_nets[min(j + (unsigned)u, je - 1)];
+#pragma unroll
+      for (int u = 0; u < CN_EVAL_GROUP; u++) {
+        off[u] = opoff[t[u]];
+        slot[u] = 0;
+        if (j + (unsigned)u < je) slot[u] = (int)t[u] == n ? m - pbase[n] : (long long)atomicAdd(&opfill[t[u]], 1);
+      }
+#pragma unroll
+      for (int u = 0; u < CN_EVAL_GROUP; u++)
+        if (j + (unsigned)u < je) ops[off[u] + slot[u]] = (int)m;
     }
   }
 }
