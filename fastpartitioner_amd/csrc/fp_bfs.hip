@@ -227,8 +227,10 @@ struct BfsScratch {
 
 // One BFS from start: the visit order into s.order[0 .. *reached), the last
 // level's lowest vertex into *farthest.
+// ordered: each level in the sequential visit order (radix sort by key);
+// otherwise in any order, which gives the same levels and farthest vertex
 fp_status bfs_run(fp_ctx* ctx, BfsScratch& s, const unsigned* pins, int kb, int start, int* reached, int* farthest,
-                  int* levels) {
+                  int* levels, bool ordered) {
   const int nov = ctx->nov, noe = ctx->noe;
   fill_async<int>(ctx, s.dist, (size_t)nov, -1);
   fill_async<int>(ctx, s.plev, (size_t)nov, -1);
@@ -263,13 +265,15 @@ fp_status bfs_run(fp_ctx* ctx, BfsScratch& s, const unsigned* pins, int kb, int 
     if (nc == 0) break;
     level++;
     k_bfs_mark<<<std::max(1, std::min((nc + BFS_BLOCK - 1) / BFS_BLOCK, 4096)), BFS_BLOCK, 0, ctx->stream>>>(
-        s.cand, nc, level, s.dist, s.vkey, s.keys, s.vals);
-    // keys: position of the parent in its level (< 2^32) above (net index, pin index)
-    const int end_bit = std::min(64, 32 + bits_of((unsigned long long)nf));
-    size_t tb = s.temp_bytes;
-    if (hipcub::DeviceRadixSort::SortPairs(s.temp, tb, s.keys, s.keys_out, s.vals, s.order + count, nc, 0, end_bit,
-                                           ctx->stream) != hipSuccess)
-      return fail(ctx, FP_ERR_HIP, "BFS level sort failed");
+        s.cand, nc, level, s.dist, s.vkey, s.keys, ordered ? s.vals : s.order + count);
+    if (ordered) {
+      // keys: position of the parent in its level (< 2^32) above (net index, pin index)
+      const int end_bit = std::min(64, 32 + bits_of((unsigned long long)nf));
+      size_t tb = s.temp_bytes;
+      if (hipcub::DeviceRadixSort::SortPairs(s.temp, tb, s.keys, s.keys_out, s.vals, s.order + count, nc, 0, end_bit,
+                                             ctx->stream) != hipSuccess)
+        return fail(ctx, FP_ERR_HIP, "BFS level sort failed");
+    }
     frontier = s.order + count;
     nf = nc;
     count += nc;
@@ -325,9 +329,10 @@ fp_status fp_bfs_ordering(fp_ctx* ctx, int start, int view, int* ordering, int* 
   if (!e0 || !e1) return fail(ctx, FP_ERR_HIP, "hipEventCreate failed");
   FP_HIP_OK(hipEventRecord(e0, ctx->stream));
   int reached = 0, f1 = 0, f2 = 0, far = 0, levels = 0;
-  FP_TRY(bfs_run(ctx, s, pins, kb, start, &reached, &f1, nullptr));
-  FP_TRY(bfs_run(ctx, s, pins, kb, f1, &reached, &f2, nullptr));
-  FP_TRY(bfs_run(ctx, s, pins, kb, f2, &reached, &far, &levels));
+  // the first two BFS only find the farthest vertex: their levels need no order
+  FP_TRY(bfs_run(ctx, s, pins, kb, start, &reached, &f1, nullptr, false));
+  FP_TRY(bfs_run(ctx, s, pins, kb, f1, &reached, &f2, nullptr, false));
+  FP_TRY(bfs_run(ctx, s, pins, kb, f2, &reached, &far, &levels, true));
   FP_HIP_OK(hipEventRecord(e1, ctx->stream));
   FP_TRY(d2h(ctx, ordering, s.order, (size_t)reached));
   FP_HIP_OK(hipStreamSynchronize(ctx->stream));
