@@ -543,13 +543,20 @@ def run_tv1m(args):
         # both sides of the refinement on the device: the BFS ordering (from
         # glibc's first rand() % nov, as TV/tv.cpp:120) and the streaming
         # initial partition (LDG2, rounds of 16384) from which TV refines
-        t = time.perf_counter()
-        _, _, _, bst = eng.bfs_ordering(1804289383 % hg.nov)
-        t_bfs = time.perf_counter() - t
+        # each twice: the first call in the process also pays the first
+        # launch of each of its kernels (reported as cold), the second is a
+        # repeated use
+        bst, t_bfs = [], []
+        for _ in range(2):
+            t = time.perf_counter()
+            bst.append(eng.bfs_ordering(1804289383 % hg.nov)[3])
+            t_bfs.append(time.perf_counter() - t)
         C, max_pw = args.ldg
-        t = time.perf_counter()
-        ist = eng.initial_partition(0, K, 16384, C, max_pw, cap)
-        t_init = time.perf_counter() - t
+        ist, t_init = [], []
+        for _ in range(2):
+            t = time.perf_counter()
+            ist.append(eng.initial_partition(0, K, 16384, C, max_pw, cap))
+            t_init.append(time.perf_counter() - t)
         tv_dev_init = eng.metrics()["TV"]
     # the upload again into fresh contexts (the process's first-use costs
     # paid): from pageable arrays through the staging buffers, and from a
@@ -640,11 +647,15 @@ def run_tv1m(args):
                            "pin_visits_per_s_pinned": round(
                                args.steps * hg.pins / (secs + t_up2["pinned"] + t_down), 1)},
         "host_path_on_device": {
-            "bfs_ordering": {"device_ms": round(bst.device_ms, 3), "wall_ms": round(t_bfs * 1e3, 3),
-                             "levels": int(bst.levels), "reached": int(bst.reached)},
+            "bfs_ordering": {"device_ms": round(bst[1].device_ms, 3), "wall_ms": round(t_bfs[1] * 1e3, 3),
+                             "cold_device_ms": round(bst[0].device_ms, 3),
+                             "cold_wall_ms": round(t_bfs[0] * 1e3, 3),
+                             "levels": int(bst[1].levels), "reached": int(bst[1].reached)},
             "initial_partition": {"stream": "LDG2 with bands", "round": 16384,
-                                  "device_ms": round(ist.device_ms, 3), "wall_ms": round(t_init * 1e3, 3),
-                                  "subrounds": int(ist.subrounds), "tv": int(tv_dev_init),
+                                  "device_ms": round(ist[1].device_ms, 3), "wall_ms": round(t_init[1] * 1e3, 3),
+                                  "cold_device_ms": round(ist[0].device_ms, 3),
+                                  "cold_wall_ms": round(t_init[0] * 1e3, 3),
+                                  "subrounds": int(ist[1].subrounds), "tv": int(tv_dev_init),
                                   "tv_sequential_stream": int(m0["TV"]),
                                   "host_sequential_stream_s": round(args.host_stream_s, 3)}},
     }

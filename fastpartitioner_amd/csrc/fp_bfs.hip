@@ -14,7 +14,8 @@
 //                 first claim of a net puts it on the level's net list (for a
 //                 large frontier, k_bfs_claim_nets finds the same claims from
 //                 the nets' side, without atomics)
-//   k_bfs_expand  one wavefront per claimed net: every unvisited pin k keeps
+//   k_bfs_expand  each claimed net (a lane for a small net, the wavefront
+//                 for a large one): every unvisited pin k keeps
 //                 the smallest key (p, j, k) -- its first discovery in the
 //                 sequential order; the first key of a pin puts it on the
 //                 level's candidate list
@@ -30,6 +31,11 @@ namespace {
 
 constexpr int BFS_BLOCK = 256;
 constexpr unsigned long long NO_KEY = ~0ull;
+// nets of at most this many pins are taken by one lane each (four pins per
+// load round trip) in k_bfs_claim_nets and k_bfs_expand; larger ones by the
+// whole wavefront
+constexpr unsigned BFS_CLAIM_LANE_PINS = 32;
+constexpr unsigned BFS_EXPAND_LANE_PINS = 8;
 
 __global__ void __launch_bounds__(BFS_BLOCK) k_bfs_claim(const int* __restrict__ frontier, int nf,
                                                          const unsigned* __restrict__ vtx_ptr,
@@ -61,6 +67,20 @@ __global__ void __launch_bounds__(BFS_BLOCK) k_bfs_claim(const int* __restrict__
   block_touch_flush(st, level_nets, nlevel, s_tn);
 }
 
+// (position of v in the frontier, index of net e in v's ascending net list)
+__device__ __forceinline__ unsigned long long bfs_claim_key(int v, int e, const unsigned* __restrict__ vtx_ptr,
+                                                            const unsigned* __restrict__ vtx_nets,
+                                                            const int* __restrict__ pos) {
+  unsigned lo = vtx_ptr[v], hi = vtx_ptr[v + 1];
+  const unsigned base = lo;
+  while (lo < hi) {
+    const unsigned mid = (lo + hi) >> 1;
+    if (vtx_nets[mid] < (unsigned)e) lo = mid + 1;
+    else hi = mid;
+  }
+  return ((unsigned long long)pos[v] << 32) | (lo - base);
+}
+
 // The same claims from the nets' side, for a large frontier (claims from the
 // frontier would put thousands of atomics on each big net's owner word):
 // one wavefront per net not expanded yet reads its pins, and the pins in the
@@ -80,30 +100,45 @@ __global__ void __launch_bounds__(BFS_BLOCK) k_bfs_claim_nets(const unsigned* __
   __shared__ int s_tn[BFS_BLOCK / WAVE + 1];
   WaveTouch st{s_buf[wave_id()], 0};
   for (int e0 = ((int)blockIdx.x * nw + wave_id()) * WAVE; e0 < noe; e0 += (int)gridDim.x * nw * WAVE) {
-    // 64 nets per wave: their done flags together, then the open ones in turn
+    // 64 nets per wave, one per lane: a small open net is scanned by its
+    // lane, four pins per load round trip; the large ones by the wave in turn
     const int ee = e0 + lane;
-    unsigned long long open = __ballot(ee < noe && !net_done[ee]);
+    const bool open = ee < noe && !net_done[ee];
+    unsigned ns = 0, nlen = 0;
+    if (open) {
+      ns = net_ptr[ee];
+      nlen = net_ptr[ee + 1] - ns;
+    }
+    const bool mine = open && nlen <= BFS_CLAIM_LANE_PINS;
     bool any = false;
-    while (open) {
-      const int e = e0 + __ffsll((long long)open) - 1;
-      open &= open - 1;
+    if (mine) {
+      unsigned long long best = NO_KEY;
+      for (unsigned k0 = 0; k0 < nlen; k0 += 4) {
+        int v[4], lv[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = k0 + q < nlen ? (int)pins[ns + k0 + q] : -1;
+#pragma unroll
+        for (int q = 0; q < 4; q++) lv[q] = v[q] >= 0 ? plev[v[q]] : -2;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (lv[q] == level) best = min(best, bfs_claim_key(v[q], ee, vtx_ptr, vtx_nets, pos));
+      }
+      if (best != NO_KEY) {
+        net_owner[ee] = best;
+        any = true;
+      }
+    }
+    unsigned long long big = __ballot(open && !mine);
+    while (big) {
+      const int e = e0 + __ffsll((long long)big) - 1;
+      big &= big - 1;
       const unsigned s = net_ptr[e], len = net_ptr[e + 1] - s;
       unsigned long long best = NO_KEY;
       for (unsigned k0 = 0; k0 < len; k0 += WAVE) {
         const unsigned k = k0 + (unsigned)lane;
         if (k < len) {
           const int v = (int)pins[s + k];
-          if (plev[v] == level) {
-            // the net's index in v's (ascending) net list
-            unsigned lo = vtx_ptr[v], hi = vtx_ptr[v + 1];
-            const unsigned base = lo;
-            while (lo < hi) {
-              const unsigned mid = (lo + hi) >> 1;
-              if (vtx_nets[mid] < (unsigned)e) lo = mid + 1;
-              else hi = mid;
-            }
-            best = min(best, ((unsigned long long)pos[v] << 32) | (lo - base));
-          }
+          if (plev[v] == level) best = min(best, bfs_claim_key(v, e, vtx_ptr, vtx_nets, pos));
         }
       }
       best = wave_min_u64(best);
@@ -141,23 +176,60 @@ __global__ void __launch_bounds__(BFS_BLOCK) k_bfs_expand(const int* __restrict_
   __shared__ int s_tn[BFS_BLOCK / WAVE + 1];
   WaveTouch st{s_buf[wave_id()], 0};
   const int n = *nlevel;
-  for (int i = (int)blockIdx.x * nw + wave_id(); i < n; i += (int)gridDim.x * nw) {
-    const int e = level_nets[i];
-    const unsigned long long own = net_owner[e];
-    // (p, j) of the owner, then the pin's index in the net
-    const unsigned long long hi = own & 0xFFFFFFFF00000000ull;
-    const unsigned long long jk = (own & 0xFFFFFFFFull) << kb;
-    if (lane == 0) net_done[e] = 1;
-    const unsigned s = net_ptr[e], len = net_ptr[e + 1] - s;
-    for (unsigned k0 = 0; k0 < len; k0 += WAVE) {
-      const unsigned k = k0 + (unsigned)lane;
-      bool first = false;
-      int u = 0;
-      if (k < len) {
-        u = (int)pins[s + k];
-        if (dist[u] < 0) first = atomicMin(&vkey[u], hi | jk | k) == NO_KEY;
+  // c of the level's nets per wave (c = the level's share per wave of the
+  // grid, 1 to 64), one per lane: a small net is expanded by its lane, four
+  // pins per load round trip; the large ones by the wave in turn (so a level
+  // of few nets still spreads them over the grid)
+  const int tw = (int)gridDim.x * nw;
+  const int c = (int)min((long long)WAVE, max(1LL, ((long long)n + tw - 1) / tw));
+  for (int i0 = ((int)blockIdx.x * nw + wave_id()) * c; i0 < n; i0 += tw * c) {
+    const int i = i0 + lane;
+    int e = -1;
+    unsigned s = 0, len = 0;
+    unsigned long long hi = 0, jk = 0;
+    if (lane < c && i < n) {
+      e = level_nets[i];
+      const unsigned long long own = net_owner[e];
+      // (p, j) of the owner, then the pin's index in the net
+      hi = own & 0xFFFFFFFF00000000ull;
+      jk = (own & 0xFFFFFFFFull) << kb;
+      net_done[e] = 1;
+      s = net_ptr[e];
+      len = net_ptr[e + 1] - s;
+    }
+    const bool mine = e >= 0 && len <= BFS_EXPAND_LANE_PINS;
+    const unsigned mlen = (unsigned)wave_max(mine ? (int)len : 0);
+    for (unsigned k0 = 0; k0 < mlen; k0 += 4) {
+      int u[4];
+      bool first[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) u[q] = mine && k0 + q < len ? (int)pins[s + k0 + q] : -1;
+#pragma unroll
+      for (int q = 0; q < 4; q++) first[q] = u[q] >= 0 && dist[u[q]] < 0;
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (first[q]) first[q] = atomicMin(&vkey[u[q]], hi | jk | (k0 + q)) == NO_KEY;
+#pragma unroll
+      for (int q = 0; q < 4; q++) wave_stage_append(first[q], (unsigned)u[q], st, cand, ncand);
+    }
+    unsigned long long big = __ballot(e >= 0 && !mine);
+    while (big) {
+      const int b = __ffsll((long long)big) - 1;
+      big &= big - 1;
+      const unsigned bs = (unsigned)__shfl((int)s, b), blen = (unsigned)__shfl((int)len, b);
+      const unsigned long long bhi = (unsigned long long)(unsigned)__shfl((int)(hi >> 32), b) << 32;
+      const unsigned long long bjk = ((unsigned long long)(unsigned)__shfl((int)(jk >> 32), b) << 32) |
+                                     (unsigned)__shfl((int)(unsigned)jk, b);
+      for (unsigned k0 = 0; k0 < blen; k0 += WAVE) {
+        const unsigned k = k0 + (unsigned)lane;
+        bool first = false;
+        int u = 0;
+        if (k < blen) {
+          u = (int)pins[bs + k];
+          if (dist[u] < 0) first = atomicMin(&vkey[u], bhi | bjk | k) == NO_KEY;
+        }
+        wave_stage_append(first, (unsigned)u, st, cand, ncand);
       }
-      wave_stage_append(first, (unsigned)u, st, cand, ncand);
     }
   }
   block_touch_flush(st, cand, ncand, s_tn);
