@@ -504,6 +504,8 @@ __global__ void __launch_bounds__(256) k_bal_scan(int* __restrict__ hist, long l
 // the j-th fallback slot of the sub-round, in slot order, takes part
 // (fb + j) mod K, fb the lowest-index part of minimum frozen size, and its
 // weight joins the candidate totals and tile histogram.
+constexpr int FALLBACK_RUN = 16;  // consecutive slots per thread in k_init_fallback
+
 __global__ void __launch_bounds__(1024) k_init_fallback(int* __restrict__ slot_dec, const int* __restrict__ order,
                                                         const int* __restrict__ vwgt, int r0, int nslots, int ntiles,
                                                         int K, const int* __restrict__ fb_in,
@@ -511,19 +513,31 @@ __global__ void __launch_bounds__(1024) k_init_fallback(int* __restrict__ slot_d
   __shared__ int s_wi[16];
   const int fb = *fb_in;
   int carry = 0;
-  for (int b0 = 0; b0 < nslots; b0 += (int)blockDim.x) {
-    const int slot = b0 + (int)threadIdx.x;
-    const bool f = slot < nslots && slot_dec[slot] == SLOT_FALLBACK;
+  // each thread takes FALLBACK_RUN consecutive slots, their decisions loaded
+  // together: a sub-round of 16,384 slots is one load round trip and one
+  // block scan
+  for (int b0 = 0; b0 < nslots; b0 += (int)blockDim.x * FALLBACK_RUN) {
+    const int base = b0 + (int)threadIdx.x * FALLBACK_RUN;
+    int dec[FALLBACK_RUN];
+#pragma unroll
+    for (int q = 0; q < FALLBACK_RUN; q++) dec[q] = base + q < nslots ? slot_dec[base + q] : 0;
+    int cnt = 0;
+#pragma unroll
+    for (int q = 0; q < FALLBACK_RUN; q++) cnt += dec[q] == SLOT_FALLBACK ? 1 : 0;
     int tot;
-    const int j = carry + block_excl_scan(f ? 1 : 0, s_wi, &tot);
-    if (f) {
-      const int d = (int)(((long long)fb + j) % K);
-      slot_dec[slot] = d;
-      const int z = r0 + slot;
-      const int w = vwgt ? vwgt[order ? order[z] : z] : 1;
-      atomic_add_i64(&cand_tot[d], w);
-      atomicAdd(&hist[(size_t)d * (size_t)ntiles + (size_t)(slot / TILE)], w);
-    }
+    int j = carry + block_excl_scan(cnt, s_wi, &tot);
+    if (cnt)
+      for (int q = 0; q < FALLBACK_RUN; q++) {
+        if (dec[q] != SLOT_FALLBACK) continue;
+        const int slot = base + q;
+        const int d = (int)(((long long)fb + j) % K);
+        j++;
+        slot_dec[slot] = d;
+        const int z = r0 + slot;
+        const int w = vwgt ? vwgt[order ? order[z] : z] : 1;
+        atomic_add_i64(&cand_tot[d], w);
+        atomicAdd(&hist[(size_t)d * (size_t)ntiles + (size_t)(slot / TILE)], w);
+      }
     carry += tot;
   }
 }
